@@ -1,0 +1,203 @@
+"""Benchmark of the wave stepper on one MI355X.
+
+Workload: the precision form at 4096 x 4096 -- [-2,2]^2, 64 seeded disc
+obstacles, three Gaussian pulses, courant 0.1, absorbing edges, 5-point
+Laplacian.  A "step" is one leapfrog time step over the whole grid (one
+evolve_wave_half of the reference, wave_billiard.c:302), so one step is
+4096^2 cell-updates.  The headline value is the FP32 double-float (split)
+form; the FP64 form (bit-identical to the reference) runs the same steps
+beside it, and the relative L2 distance and energy drift between the two are
+reported.  Inputs are resident in HBM before the timed region starts.
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line.  The CPU baseline is the oracle (this repository's C +
+OpenMP restatement of the reference evolve_wave_half, pinned bit-exact to the
+reference's own code) timed on a bounded number of steps of the same FP64
+workload on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "youtube-simulations_amd"))
+
+HBM_PEAK = 8.0e12              # MI355X HBM3E, bytes/s
+B_ALG = {"f32split": 24.125,   # read u_hi, u_lo, v + write them (3 x 4 B each way) + 1-bit mask
+         "fp64": 24.125}       # read u^n, u^{n-1} + write u^{n+1} (3 x 8 B) + 1-bit mask
+METRIC = "Gcell-updates/s at 4096^2 (FP64 plain vs FP32 split) + % of HBM roofline"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_form(wh, w, form, steps, warmup, tb_depth=0, rows=0, tune=False):
+    W = wh.Wave(w["params"], form=form, tb_depth=tb_depth, rows_per_tile=rows)
+    W.load(w)
+    if tune:
+        W.autotune(200)
+    e0 = W.energy()
+    if warmup:
+        W.step(warmup)
+        W.sync()
+        W.set_state(w["phi"], w["psi"])   # the timed run starts from the initial state
+    W.sync()
+    t0 = time.perf_counter()
+    ms_dev, launches = W.step_timed(steps)   # HIP events on the stepping stream, waits
+    W.sync()
+    wall = time.perf_counter() - t0
+    e1 = W.energy()
+    u, um = W.get_state()
+    k, L, cols = W.tiling()
+    W.close()
+    return dict(wall=wall, ms_dev=ms_dev, launches=launches, e0=e0, e1=e1, u=u, k=k, L=L, cols=cols)
+
+
+def cpu_baseline(w, target_s=10.0):
+    """the oracle on the host cores, a bounded sample of the FP64 workload"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    pw = w["params"]
+    p = pyoracle.make_params(pw["nx"], pw["ny"], bcond=pw["bcond"])
+    args = (w["xy_in"], w["tcc"], w["tc"], w["tgamma"])
+    t0 = time.perf_counter()
+    phi, psi, tmp = pyoracle.run(p, w["phi"], w["psi"], 4, *args)
+    probe = (time.perf_counter() - t0) / 4
+    n = int(max(8, min(2000, target_s / max(probe, 1e-6))))
+    t0 = time.perf_counter()
+    pyoracle.run(p, phi, psi, n, *args, tmp=tmp)
+    dt = time.perf_counter() - t0
+    cells = pw["nx"] * pw["ny"]
+    return dict(value=cells * n / dt / 1e9, unit="Gcell-updates/s", cores=pyoracle.num_threads(),
+                kind="port",
+                sample="%d steps of the 4096^2 FP64 5-point workload (oracle restatement of "
+                       "evolve_wave_half, row pointers, OpenMP parallel for over i), %.1f s" % (n, dt))
+
+
+def traffic_from_profile(k, L):
+    """HBM bytes per launch of the split kernel from the committed rocprofv3
+    --pmc summary, when it was measured with the same tiling"""
+    f = os.path.join(ROOT, "profiles", "traffic_split_c3_4096.json")
+    if not os.path.exists(f):
+        return None
+    d = json.load(open(f))
+    if d.get("tb_depth") == k and d.get("rows_per_tile") == L:
+        return d.get("bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--tb-depth", type=int, default=0)
+    ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--tune", action="store_true", help="autotune tiling before timing")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--only-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (every run here uses a "
+              "single MI355X); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import wavehip as wh
+    from wavehip import workloads
+
+    log("building workload %d^2 ..." % args.size)
+    w = workloads.c3(n=args.size, nsteps=args.steps)
+    cells = w["params"]["nx"] * w["params"]["ny"]
+    res = {}
+    forms = []
+    if not args.only_fp64:
+        forms.append(("f32split", wh.FORM_F32SPLIT))
+    if not args.no_fp64 or args.only_fp64:
+        forms.append(("fp64", wh.FORM_FP64))
+    for name, form in forms:
+        log("running %s: %d warmup + %d timed steps" % (name, args.warmup, args.steps))
+        r = run_form(wh, w, form, args.steps, args.warmup, args.tb_depth, args.rows, args.tune)
+        r["gcells"] = cells * args.steps / r["wall"] / 1e9
+        r["gcells_dev"] = cells * args.steps / (r["ms_dev"] * 1e-3) / 1e9
+        r["avg_launch_ms"] = r["ms_dev"] / r["launches"]
+        # algorithmic bytes per launch: B_alg x cells x fused steps per launch
+        r["alg_bytes_per_launch"] = B_ALG[name] * cells * r["k"]
+        r["achieved_Bps"] = r["alg_bytes_per_launch"] / (r["avg_launch_ms"] * 1e-3)
+        log("%s: %.1f Gcell/s (wall), %.1f (events), k=%d rows=%d, %d launches, %.3f ms/launch"
+            % (name, r["gcells"], r["gcells_dev"], r["k"], r["L"], r["launches"], r["avg_launch_ms"]))
+        res[name] = r
+
+    head = res.get("f32split") or res["fp64"]
+    hname = "f32split" if "f32split" in res else "fp64"
+    out = {
+        "metric": METRIC,
+        "value": round(head["gcells"], 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["wall"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if hname == "f32split" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "precision form %dx%d: 64 seeded disc obstacles, 3 Gaussian pulses, courant 0.1, "
+                        "absorbing edges, 5-point, %s state, %d leapfrog steps per run"
+                        % (args.size, args.size, "FP32 double-float (u_hi, u_lo, v)" if hname == "f32split"
+                           else "FP64", args.steps),
+            "grid": [args.size, args.size],
+            "form": hname,
+            "tb_depth": head["k"],
+            "rows_per_tile": head["L"],
+            "tile_cols": head["cols"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": head["achieved_Bps"] / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": head["achieved_Bps"] / HBM_PEAK,
+            "traffic": traffic_from_profile(head["k"], head["L"]) if hname == "f32split" else None,
+            "kernel": "k_step_split<%d,5>" % head["k"] if hname == "f32split" else "k_step_fp64<%d,5>" % head["k"],
+            "alg_bytes_per_launch": head["alg_bytes_per_launch"],
+            "avg_launch_ms": head["avg_launch_ms"],
+            "b_alg_per_cell_update": B_ALG[hname],
+        },
+    }
+    if "fp64" in res:
+        f = res["fp64"]
+        out["fp64"] = {
+            "gcells": round(f["gcells"], 3),
+            "gcells_events": round(f["gcells_dev"], 3),
+            "tb_depth": f["k"], "rows_per_tile": f["L"],
+            "roofline_frac": f["achieved_Bps"] / HBM_PEAK,
+            "energy_drift": (f["e1"] - f["e0"]) / f["e0"],
+        }
+    if "fp64" in res and "f32split" in res:
+        s, f = res["f32split"], res["fp64"]
+        out["split_vs_fp64"] = {
+            "speedup": s["gcells"] / f["gcells"],
+            "rel_l2_phi": float(np.linalg.norm(s["u"] - f["u"]) / np.linalg.norm(f["u"])),
+            "energy_drift_split": (s["e1"] - s["e0"]) / s["e0"],
+            "energy_drift_fp64": (f["e1"] - f["e0"]) / f["e0"],
+            "energy_rel_diff": (s["e1"] - f["e1"]) / f["e0"],
+            "steps": args.steps,
+        }
+    if not args.no_cpu:
+        log("timing the CPU baseline ...")
+        out["cpu_baseline"] = cpu_baseline(w)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,162 @@
+/*
+ * wavehip.h -- C ABI of libwavehip.so, the MI355X (gfx950) stepper for the
+ * masked leapfrog update of the 2D linear wave equation.
+ *
+ * It replaces the time-stepping core of nilsberglund-orleans/YouTube-simulations:
+ *   evolve_wave_half  wave_billiard.c:302-610   (row pointers, double *phi[NX])
+ *   evolve_wave       wave_billiard.c:613-631   (3 steps, (phi,psi,tmp) rotation)
+ *   evolve_wave_half  wave_3d.c:600-1007        (flat k = i*NY + j)
+ *   evolve_wave       wave_3d.c:1010-1031
+ * The reference fixes NX, NY, B_COND, TWOSPEEDS, KAPPA ... as compile-time
+ * macros and keeps the step counter in function statics; here they live in a
+ * context.  include/wavehip_compat.h maps the reference signatures onto this
+ * ABI for a host program that keeps its macros.
+ *
+ * Conventions: every array is flat, k = i*NY + j (i = x, slow; j = y,
+ * contiguous), host memory, caller-owned.  Every int-returning call returns
+ * WV_OK (0) or a negative WV_E* code and then wv_last_error() says why.
+ * A context belongs to one host thread.  Device state stays resident between
+ * calls; wv_step() is asynchronous on the context's HIP stream, wv_get_*,
+ * wv_energy() and wv_sync() wait for it.
+ */
+#ifndef WAVEHIP_H
+#define WAVEHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WV_OK            0
+#define WV_EINVAL       -1   /* bad argument or parameter combination       */
+#define WV_EHIP         -2   /* HIP runtime error (message in wv_last_error) */
+#define WV_ESTATE       -3   /* call out of order (no tables / no state)     */
+#define WV_EUNSUPPORTED -4   /* reference feature not provided (see text)    */
+#define WV_ENOMEM       -5
+
+/* B_COND codes, global_pdes.c:270-274 */
+#define WV_BC_DIRICHLET   0  /* zero-flux outer edges (3 neighbours - 3x)  */
+#define WV_BC_PERIODIC    1
+#define WV_BC_ABSORBING   2
+#define WV_BC_VPER_HABS   3
+#define WV_BC_ABS_REFLECT 4  /* wave_3d.c; equals ABSORBING without COMPARISON */
+
+/* storage / arithmetic form */
+#define WV_FORM_FP64      0  /* binary64 leapfrog, bit-identical to the reference */
+#define WV_FORM_F32SPLIT  1  /* binary32 double-float u = hi + lo plus velocity v */
+
+typedef struct wv_ctx wv_ctx;
+
+typedef struct wv_params {
+    int nx, ny;            /* NX, NY (>= 3 each)                                   */
+    int bcond;             /* B_COND, WV_BC_*                                      */
+    int twospeeds;         /* TWOSPEEDS                                            */
+    int stencil;           /* 5 = reference Laplacian; 9 = isotropic 9-point bulk   */
+    int form;              /* WV_FORM_*                                            */
+    int oscillate_topbot;  /* OSCILLATE_TOPBOT                                     */
+    int tb_shift;          /* tb_shift of the reference (wave_billiard.c:319)      */
+    double kappa;          /* KAPPA                                                */
+    double kappa_sides;    /* KAPPA_SIDES                                          */
+    double gamma_sides;    /* GAMMA_SIDES                                          */
+    double kappa_topbot;   /* KAPPA_TOPBOT                                         */
+    double gamma_topbot;   /* GAMMA_TOPBOT                                         */
+    double e_scale;        /* E_SCALE of compute_energy (wave_common.c:469)        */
+    int tb_depth;          /* steps fused per launch: 0 = tuned default, 1..8 fixed */
+    int rows_per_tile;     /* rows per tile of a fused launch: 0 = tuned default   */
+    int device;            /* HIP device ordinal                                   */
+    int flags;             /* WV_FLAG_*                                            */
+} wv_params;
+
+/* drop-in entry points upload the tables on the first call only */
+#define WV_FLAG_TABLES_FIXED 1
+
+/* Fill *p with the reference's defaults for an nx x ny grid (B_COND 0, 5-point,
+ * FP64, all constants 0, e_scale 1, tb_shift nx, tuned tiling). */
+void wv_params_default(wv_params *p, int nx, int ny);
+
+int  wv_create(wv_ctx **out, const wv_params *p);
+void wv_destroy(wv_ctx *c);
+const char *wv_last_error(const wv_ctx *c);  /* never NULL; c may be NULL */
+
+/* Tables of init_ior_2d (sub_wave.c:12164) and the domain mask of
+ * init_wave_flat (wave_common.c:294).  Cells are updated as the reference
+ * does: interior cells when TWOSPEEDS or xy_in > 0 (wave_billiard.c:365),
+ * edge cells when TWOSPEEDS or xy_in != 0; other cells are never written. */
+int wv_set_tables(wv_ctx *c, const short *xy_in, const double *tcc, const double *tc,
+                  const double *tgamma);
+
+/* u^n (phi) and u^{n-1} (psi); the third buffer of the rotation starts at 0. */
+int wv_set_state(wv_ctx *c, const double *phi, const double *psi);
+/* all three buffers of evolve_wave's rotation (phi, psi, tmp) */
+int wv_set_state3(wv_ctx *c, const double *phi, const double *psi, const double *tmp);
+int wv_get_state(wv_ctx *c, double *phi, double *psi);
+int wv_get_state3(wv_ctx *c, double *phi, double *psi, double *tmp);
+
+/* Soft point sources (no reference counterpart).  After step g (g = 1, 2, ...
+ * counted from this call), source k adds amp[k*amp_len + g-1] to u at
+ * (i[k], j[k]) while g <= amp_len.  Source cells must be active cells.
+ * n = 0 removes all sources. */
+int wv_set_sources(wv_ctx *c, int n, const int *i, const int *j, const double *amp,
+                   long amp_len);
+
+int wv_step(wv_ctx *c, long nsteps);
+/* wv_step bracketed by HIP events on the context stream; waits; *ms is the
+ * device time of the whole sequence and *launches the kernel launches in it */
+int wv_step_timed(wv_ctx *c, long nsteps, double *ms, long *launches);
+int wv_sync(wv_ctx *c);
+long wv_steps_done(const wv_ctx *c);
+
+/* Total energy: compute_energy (wave_common.c:453-472) per cell with c^2 from
+ * tcc, summed in FP64 in a fixed order (64 lane-strided partials per row, a
+ * halving tree, then the same over rows): deterministic run to run. */
+int wv_energy(wv_ctx *c, double *e);
+
+/* What wv_step launches: fused steps per launch, rows per tile, tile columns */
+int wv_get_tiling(const wv_ctx *c, int *tb_depth, int *rows_per_tile, int *tile_cols);
+int wv_set_tiling(wv_ctx *c, int tb_depth, int rows_per_tile);
+/* Time candidate tilings on the device for the current state (state is
+ * restored afterwards) and keep the fastest; *best_gcells is its throughput. */
+int wv_autotune(wv_ctx *c, long steps_per_trial, double *best_gcells);
+
+/* ---- drop-in forms of the reference functions (one context per program) ---
+ * Each call uploads phi_in, psi_in and phi_out (so cells the step does not
+ * write keep the caller's values), runs one step (or three) and writes the
+ * result back into the caller's arrays, exactly as the reference does. */
+
+/* evolve_wave_half, wave_billiard.c:302-304 */
+int wv_evolve_wave_half_rows(wv_ctx *c, double *const *phi_in, double *const *psi_in,
+                             double *const *phi_out, short *const *xy_in,
+                             double *const *tcc, double *const *tc, double *const *tgamma);
+/* evolve_wave, wave_billiard.c:613 */
+int wv_evolve_wave_rows(wv_ctx *c, double *const *phi, double *const *psi, double *const *tmp,
+                        short *const *xy_in, double *const *tcc_table,
+                        double *const *tc_table, double *const *tgamma_table);
+/* evolve_wave_half, wave_3d.c:600 (table order tc, tcc, tgamma) */
+int wv_evolve_wave_half_flat(wv_ctx *c, double *phi_in, double *psi_in, double *phi_out,
+                             short *xy_in, double *tc, double *tcc, double *tgamma);
+/* evolve_wave, wave_3d.c:1010 (without the t_laplacian tables: PRECOMPUTE_BC 0) */
+int wv_evolve_wave_flat(wv_ctx *c, double *phi, double *psi, double *tmp, short *xy_in,
+                        double *tc, double *tcc, double *tgamma);
+
+/* ---- batches of independent grids (no reference counterpart) ------------
+ * n instances of any sizes, BCs and tables advance together: one launch per
+ * fused block of steps covers the tiles of every instance, and the launch
+ * sequence of a wv_batch_step is captured once into a HIP graph and replayed.
+ * Every instance must use the same form and stencil. */
+typedef struct wv_batch wv_batch;
+int  wv_batch_create(wv_batch **out, int n, const wv_params *ps);
+void wv_batch_destroy(wv_batch *b);
+const char *wv_batch_last_error(const wv_batch *b);
+int wv_batch_set_tables(wv_batch *b, int k, const short *xy_in, const double *tcc,
+                        const double *tc, const double *tgamma);
+int wv_batch_set_state(wv_batch *b, int k, const double *phi, const double *psi);
+int wv_batch_get_state(wv_batch *b, int k, double *phi, double *psi);
+int wv_batch_step(wv_batch *b, long nsteps);
+int wv_batch_step_timed(wv_batch *b, long nsteps, double *ms, long *launches);
+int wv_batch_energy(wv_batch *b, double *e /* n values */);
+int wv_batch_set_tiling(wv_batch *b, int tb_depth, int rows_per_tile);
+int wv_batch_get_tiling(const wv_batch *b, int *tb_depth, int *rows_per_tile, int *tile_cols);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,64 @@
+#!/usr/bin/env bash
+# Compile the reference's own stepper into oracle/_ref/ for the golden-fixture
+# generator and the oracle cross-check.  The translation unit is assembled on
+# stdin from the reference files where they lie (nothing is copied into the
+# repository): the parameter block oracle/ref_params.h, global_pdes.c whole,
+# then xy_to_ij (sub_wave.c:412-422), oscillating_bc (sub_wave.c:11965-12162),
+# compute_energy (wave_common.c:453-472) and the stepper (wave_billiard.c:302-631
+# or wave_3d.c:600-1007), then oracle/ref_harness.c.  Flags follow makefile:1-4
+# (gcc -O3 -fopenmp) plus -ffp-contract=off.  Needs /root/reference; the GPU
+# box never runs this.
+set -euo pipefail
+REF=${REF:-/root/reference}
+HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
+OUT="$HERE/_ref"
+[ -f "$REF/wave_billiard.c" ] || { echo "reference not found at $REF" >&2; exit 1; }
+mkdir -p "$OUT"
+
+build_one() {  # build_one <tag> <flat:0|1> <gcc -D flags...>
+    local tag=$1 flat=$2; shift 2
+    local so="$OUT/libref_${tag}.so"
+    {
+        cat "$HERE/ref_params.h"
+        echo "#include \"$REF/global_pdes.c\""
+        sed -n '412,422p' "$REF/sub_wave.c"
+        sed -n '11965,12162p' "$REF/sub_wave.c"
+        if [ "$flat" = 1 ]; then
+            echo "#define FLAT_TWIN 1"
+            sed -n '600,1007p' "$REF/wave_3d.c"
+        else
+            sed -n '453,472p' "$REF/wave_common.c"
+            sed -n '302,631p' "$REF/wave_billiard.c"
+        fi
+        cat "$HERE/ref_harness.c"
+    } | gcc -x c - -O3 -fopenmp -fPIC -shared -ffp-contract=off -w "$@" -o "$so" -lm
+}
+
+KS="-DKAPPA=1.0e-3 -DKAPPA_SIDES=5.0e-4 -DGAMMA_SIDES=1.0e-4 -DKAPPA_TOPBOT=2.0e-4 -DGAMMA_TOPBOT=1.0e-7"
+pids=()
+for g in "24 17" "16 29"; do
+    set -- $g; nx=$1; ny=$2
+    for bc in 0 1 2 3; do
+        for ts in 0 1; do
+            build_one "row_${nx}x${ny}_bc${bc}_ts${ts}" 0 -DNX=$nx -DNY=$ny -DB_COND=$bc -DTWOSPEEDS=$ts $KS &
+            pids+=($!)
+        done
+    done
+    for bc in 0 1 2 3 4; do
+        for ts in 0 1; do
+            build_one "flat_${nx}x${ny}_bc${bc}_ts${ts}" 1 -DNX=$nx -DNY=$ny -DB_COND=$bc -DTWOSPEEDS=$ts $KS &
+            pids+=($!)
+        done
+    done
+    build_one "row_${nx}x${ny}_bc0_ts0_tb" 0 -DNX=$nx -DNY=$ny -DB_COND=0 -DTWOSPEEDS=0 -DOSCILLATE_TOPBOT=1 $KS &
+    pids+=($!)
+    build_one "flat_${nx}x${ny}_bc2_ts1_tb" 1 -DNX=$nx -DNY=$ny -DB_COND=2 -DTWOSPEEDS=1 -DOSCILLATE_TOPBOT=1 $KS &
+    pids+=($!)
+    # wait in batches of 8 to keep the container within its 8 CPUs
+    for p in "${pids[@]}"; do wait "$p"; done; pids=()
+done
+# the CPU-runnable plain workload: 1280x720, 5-point, B_COND 0, courant 0.1
+build_one "row_1280x720_bc0_ts0" 0 -DNX=1280 -DNY=720 -DB_COND=0 -DTWOSPEEDS=0
+# a lens medium with damping and absorbing edges (energy fixture, two speeds)
+build_one "row_64x48_bc2_ts1" 0 -DNX=64 -DNY=48 -DB_COND=2 -DTWOSPEEDS=1 -DCOURANTB=0.0666666666666666667 $KS
+ls "$OUT" | wc -l

@@ -1,0 +1,214 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) and of the reference
+builds in oracle/_ref/.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+CPU-baseline leg of bench.py, never by the product library.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(HERE, "_ref")
+
+_D = ctypes.POINTER(ctypes.c_double)
+_F = ctypes.POINTER(ctypes.c_float)
+_S = ctypes.POINTER(ctypes.c_short)
+_I = ctypes.POINTER(ctypes.c_int)
+
+
+class OrcParams(ctypes.Structure):
+    """orc_params of wave_oracle.h."""
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("bcond", ctypes.c_int),
+        ("twospeeds", ctypes.c_int), ("stencil", ctypes.c_int),
+        ("oscillate_topbot", ctypes.c_int), ("tb_shift", ctypes.c_int),
+        ("kappa", ctypes.c_double), ("kappa_sides", ctypes.c_double),
+        ("gamma_sides", ctypes.c_double), ("kappa_topbot", ctypes.c_double),
+        ("gamma_topbot", ctypes.c_double),
+    ]
+
+
+def make_params(nx, ny, bcond=0, twospeeds=0, stencil=5, oscillate_topbot=0, tb_shift=None,
+                kappa=0.0, kappa_sides=0.0, gamma_sides=0.0, kappa_topbot=0.0, gamma_topbot=0.0):
+    if tb_shift is None:
+        tb_shift = nx
+    return OrcParams(nx, ny, bcond, twospeeds, stencil, oscillate_topbot, tb_shift,
+                     kappa, kappa_sides, gamma_sides, kappa_topbot, gamma_topbot)
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _c64(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        L = ctypes.CDLL(path)
+        P = ctypes.POINTER(OrcParams)
+        L.orc_evolve_wave_half_flat.argtypes = [P, _D, _D, _D, _S, _D, _D, _D]
+        L.orc_evolve_wave_flat.argtypes = [P, _D, _D, _D, _S, _D, _D, _D]
+        L.orc_run.argtypes = [P, _D, _D, _D, _I, ctypes.c_long, _S, _D, _D, _D, ctypes.c_int,
+                              _I, _I, _D, ctypes.c_long, ctypes.c_long]
+        L.orc_energy_cell.restype = ctypes.c_double
+        L.orc_energy_cell.argtypes = [P, _D, _D, _S, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.orc_energy_total.restype = ctypes.c_double
+        L.orc_energy_total.argtypes = [P, _D, _D, _S, _D, ctypes.c_double]
+        L.orc_fixed_order_sum.restype = ctypes.c_double
+        L.orc_fixed_order_sum.argtypes = [ctypes.c_int, ctypes.c_int, _D]
+        L.orc_split_from_fp64.argtypes = [ctypes.c_long, _D, _D, _F, _F, _F]
+        L.orc_split_to_fp64.argtypes = [ctypes.c_long, _F, _F, _F, _D, _D]
+        L.orc_split_run.argtypes = [P, _F, _F, _F, ctypes.c_long, _S, _D, _D, _D, ctypes.c_int,
+                                    _I, _I, _D, ctypes.c_long, ctypes.c_long]
+        L.orc_num_threads.restype = ctypes.c_int
+        L.orc_split_energy_total.restype = ctypes.c_double
+        L.orc_split_energy_total.argtypes = [P, _F, _F, _F, _S, _D, ctypes.c_double]
+        _lib = L
+    return _lib
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+def evolve_wave_half_flat(p, phi_in, psi_in, phi_out, xy_in, tc, tcc, tgamma):
+    """One step in place on phi_out (flat twin signature, wave_3d.c:600)."""
+    lib().orc_evolve_wave_half_flat(ctypes.byref(p), _p(phi_in, _D), _p(psi_in, _D),
+                                    _p(phi_out, _D), _p(xy_in, _S), _p(tc, _D), _p(tcc, _D),
+                                    _p(tgamma, _D))
+
+
+def evolve_wave_flat(p, phi, psi, tmp, xy_in, tcc, tc, tgamma):
+    lib().orc_evolve_wave_flat(ctypes.byref(p), _p(phi, _D), _p(psi, _D), _p(tmp, _D),
+                               _p(xy_in, _S), _p(tcc, _D), _p(tc, _D), _p(tgamma, _D))
+
+
+def run(p, phi, psi, nsteps, xy_in, tcc, tc, tgamma, tmp=None, sources=None, step0=0):
+    """Advance nsteps with the evolve_wave rotation. Returns (phi, psi, tmp)
+    views of the three buffers after the run (new arrays; inputs untouched)."""
+    b = [np.array(phi, dtype=np.float64, copy=True), np.array(psi, dtype=np.float64, copy=True),
+         np.zeros_like(phi, dtype=np.float64) if tmp is None else np.array(tmp, dtype=np.float64)]
+    st = ctypes.c_int(0)
+    nsrc, si, sj, amp, alen = _sources(sources)
+    lib().orc_run(ctypes.byref(p), _p(b[0], _D), _p(b[1], _D), _p(b[2], _D), ctypes.byref(st),
+                  nsteps, _p(xy_in, _S), _p(tcc, _D), _p(tc, _D), _p(tgamma, _D), nsrc,
+                  _p(si, _I), _p(sj, _I), _p(amp, _D), alen, step0)
+    s = st.value
+    return b[s], b[(s + 1) % 3], b[(s + 2) % 3]
+
+
+def _sources(sources):
+    if not sources:
+        z = np.zeros(1, np.int32)
+        return 0, z, z, np.zeros(1), 0
+    si = np.ascontiguousarray(sources["i"], np.int32)
+    sj = np.ascontiguousarray(sources["j"], np.int32)
+    amp = np.ascontiguousarray(sources["amp"], np.float64)
+    return len(si), si, sj, amp.reshape(-1), amp.shape[1]
+
+
+def energy_cell(p, phi, psi, xy_in, courant, courantb, e_scale, i, j):
+    return lib().orc_energy_cell(ctypes.byref(p), _p(phi, _D), _p(psi, _D), _p(xy_in, _S),
+                                 courant, courantb, e_scale, i, j)
+
+
+def energy_total(p, phi, psi, xy_in, tcc, e_scale=1.0):
+    return lib().orc_energy_total(ctypes.byref(p), _p(phi, _D), _p(psi, _D), _p(xy_in, _S),
+                                  _p(tcc, _D), e_scale)
+
+
+def fixed_order_sum(e):
+    e = np.ascontiguousarray(e, np.float64)
+    return lib().orc_fixed_order_sum(e.shape[0], e.shape[1], _p(e, _D))
+
+
+def split_from_fp64(u_n, u_nm1):
+    n = u_n.size
+    hi = np.empty(u_n.shape, np.float32); lo = np.empty_like(hi); v = np.empty_like(hi)
+    lib().orc_split_from_fp64(n, _p(_c64(u_n), _D), _p(_c64(u_nm1), _D), _p(hi, _F), _p(lo, _F),
+                              _p(v, _F))
+    return hi, lo, v
+
+
+def split_to_fp64(hi, lo, v):
+    u = np.empty(hi.shape, np.float64); um = np.empty_like(u)
+    lib().orc_split_to_fp64(hi.size, _p(hi, _F), _p(lo, _F), _p(v, _F), _p(u, _D), _p(um, _D))
+    return u, um
+
+
+def split_run(p, hi, lo, v, nsteps, xy_in, tcc, tc, tgamma, sources=None, step0=0):
+    hi = np.array(hi, np.float32, copy=True); lo = np.array(lo, np.float32, copy=True)
+    v = np.array(v, np.float32, copy=True)
+    nsrc, si, sj, amp, alen = _sources(sources)
+    lib().orc_split_run(ctypes.byref(p), _p(hi, _F), _p(lo, _F), _p(v, _F), nsteps, _p(xy_in, _S),
+                        _p(tcc, _D), _p(tc, _D), _p(tgamma, _D), nsrc, _p(si, _I), _p(sj, _I),
+                        _p(amp, _D), alen, step0)
+    return hi, lo, v
+
+
+def split_energy_total(p, hi, lo, v, xy_in, tcc, e_scale=1.0):
+    return lib().orc_split_energy_total(ctypes.byref(p), _p(hi, _F), _p(lo, _F), _p(v, _F),
+                                        _p(xy_in, _S), _p(tcc, _D), e_scale)
+
+
+# ---------------------------------------------------------------- reference --
+
+def ref_available():
+    return os.path.isdir(REF_DIR) and any(f.endswith(".so") for f in os.listdir(REF_DIR))
+
+
+class RefBuild:
+    """One reference build from oracle/build_ref.sh (compile-time NX, NY, B_COND...)."""
+
+    def __init__(self, tag):
+        path = os.path.join(REF_DIR, "libref_%s.so" % tag)
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.L = ctypes.CDLL(path)
+        self.nx = self.L.ref_nx()
+        self.ny = self.L.ref_ny()
+        self.flat = hasattr(self.L, "ref_evolve_wave_half_flat")
+        if self.flat:
+            self.L.ref_evolve_wave_half_flat.argtypes = [_D, _D, _D, _S, _D, _D, _D]
+        else:
+            self.L.ref_evolve_wave_half.argtypes = [_D, _D, _D, _S, _D, _D, _D]
+            self.L.ref_evolve_wave.argtypes = [_D, _D, _D, _S, _D, _D, _D]
+            self.L.ref_compute_energy.restype = ctypes.c_double
+            self.L.ref_compute_energy.argtypes = [_D, _D, _S, ctypes.c_int, ctypes.c_int]
+
+    def half(self, phi_in, psi_in, phi_out, xy_in, tcc, tc, tgamma):
+        if self.flat:
+            self.L.ref_evolve_wave_half_flat(_p(phi_in, _D), _p(psi_in, _D), _p(phi_out, _D),
+                                             _p(xy_in, _S), _p(tc, _D), _p(tcc, _D), _p(tgamma, _D))
+        else:
+            self.L.ref_evolve_wave_half(_p(phi_in, _D), _p(psi_in, _D), _p(phi_out, _D),
+                                        _p(xy_in, _S), _p(tcc, _D), _p(tc, _D), _p(tgamma, _D))
+
+    def evolve_wave(self, phi, psi, tmp, xy_in, tcc, tc, tgamma):
+        self.L.ref_evolve_wave(_p(phi, _D), _p(psi, _D), _p(tmp, _D), _p(xy_in, _S), _p(tcc, _D),
+                               _p(tc, _D), _p(tgamma, _D))
+
+    def run(self, phi, psi, tmp, nsteps, xy_in, tcc, tc, tgamma):
+        """nsteps half-steps with the evolve_wave rotation; returns (phi, psi, tmp)."""
+        b = [phi, psi, tmp]
+        s = 0
+        for _ in range(nsteps):
+            self.half(b[s], b[(s + 1) % 3], b[(s + 2) % 3], xy_in, tcc, tc, tgamma)
+            s = (s + 2) % 3
+        return b[s], b[(s + 1) % 3], b[(s + 2) % 3]
+
+    def energy(self, phi, psi, xy_in, i, j):
+        return self.L.ref_compute_energy(_p(phi, _D), _p(psi, _D), _p(xy_in, _S), i, j)

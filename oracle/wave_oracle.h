@@ -1,0 +1,126 @@
+/*
+ * wave_oracle.h -- CPU oracle for the 2D wave leapfrog stepper.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+ * calls this code: it is the checker used by tests/, by the smoke check and
+ * by the CPU-baseline leg of the benchmark.
+ *
+ * It restates, in plain C + OpenMP, the reference's masked leapfrog update
+ *   evolve_wave_half   wave_billiard.c:302-610  (row-pointer layout)
+ *   evolve_wave        wave_billiard.c:613-631  (3-buffer rotation)
+ *   evolve_wave_half   wave_3d.c:600-1007       (flat twin, adds BC_ABS_REFLECT)
+ * with the reference's compile-time macros (NX, NY, B_COND, TWOSPEEDS, KAPPA,
+ * KAPPA_SIDES, GAMMA_SIDES, KAPPA_TOPBOT, GAMMA_TOPBOT, OSCILLATE_TOPBOT) turned
+ * into the runtime struct below.  Same loop order, same left-to-right
+ * expression order, same edge / corner quirks, so results are bit-identical
+ * to the reference compiled with its own flags (gcc -O3 -fopenmp, no FMA).
+ *
+ * Forms with no reference counterpart (9-point bulk Laplacian, soft point
+ * sources, the FP64 energy sum order, the FP32 double-float "split" state)
+ * are defined here once; the HIP kernels implement the same arithmetic.
+ */
+#ifndef WAVE_ORACLE_H
+#define WAVE_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* B_COND codes, global_pdes.c:270-275 */
+#define ORC_BC_DIRICHLET  0
+#define ORC_BC_PERIODIC   1
+#define ORC_BC_ABSORBING  2
+#define ORC_BC_VPER_HABS  3
+#define ORC_BC_ABS_REFLECT 4
+
+typedef struct {
+    int nx, ny;               /* NX, NY: i (x) is the slow index, j (y) contiguous   */
+    int bcond;                /* B_COND                                             */
+    int twospeeds;            /* TWOSPEEDS                                          */
+    int stencil;              /* 5 (reference) or 9 (build-defined bulk Laplacian)  */
+    int oscillate_topbot;     /* OSCILLATE_TOPBOT                                   */
+    int tb_shift;             /* tb_shift, wave_billiard.c:319                      */
+    double kappa;             /* KAPPA                                              */
+    double kappa_sides;       /* KAPPA_SIDES                                        */
+    double gamma_sides;       /* GAMMA_SIDES                                        */
+    double kappa_topbot;      /* KAPPA_TOPBOT                                       */
+    double gamma_topbot;      /* GAMMA_TOPBOT                                       */
+} orc_params;
+
+/* 1.0/6.0: weight of the build-defined 9-point Laplacian
+ * lap9 = ((4*(E+W+N+S) + (NE+NW+SE+SW)) - 20*x) * ORC_W9 */
+#define ORC_W9 (1.0 / 6.0)
+
+/* OpenMP threads the parallel loops use */
+int orc_num_threads(void);
+
+/* One leapfrog step on row pointers (wave_billiard.c:302-610). */
+void orc_evolve_wave_half(const orc_params *p,
+                          double *const *phi_in, double *const *psi_in, double *const *phi_out,
+                          short *const *xy_in, double *const *tcc, double *const *tc,
+                          double *const *tgamma);
+
+/* One leapfrog step on flat k=i*NY+j arrays; table order (tc, tcc, tgamma)
+ * as in wave_3d.c:600. */
+void orc_evolve_wave_half_flat(const orc_params *p,
+                               double *phi_in, double *psi_in, double *phi_out, short *xy_in,
+                               double *tc, double *tcc, double *tgamma);
+
+/* Three steps with the (phi,psi,tmp) rotation of wave_billiard.c:613-631,
+ * flat arrays. */
+void orc_evolve_wave_flat(const orc_params *p, double *phi, double *psi, double *tmp,
+                          short *xy_in, double *tcc, double *tc, double *tgamma);
+
+/* nsteps steps on three flat buffers b[0..2].  *state is the rotation index s:
+ * phi = b[s], psi = b[(s+1)%3], tmp = b[(s+2)%3]; each step writes tmp and
+ * sets s = (s+2)%3, exactly the rotation of evolve_wave.  After step number
+ * g (g = step0 + m + 1, m = 0..nsteps-1) and when g-1 < amp_len, every source
+ * k adds amp[k*amp_len + g-1] to the new phi at (si[k], sj[k]). */
+void orc_run(const orc_params *p, double *b0, double *b1, double *b2, int *state,
+             long nsteps, const short *xy_in, const double *tcc, const double *tc,
+             const double *tgamma, int nsrc, const int *si, const int *sj,
+             const double *amp, long amp_len, long step0);
+
+/* Per-cell energy, compute_energy wave_common.c:453-472, with the Courant
+ * constants given explicitly (c2_in = COURANT^2 for xy_in != 0, c2_out =
+ * COURANTB^2 used when TWOSPEEDS). */
+double orc_energy_cell(const orc_params *p, const double *phi, const double *psi,
+                       const short *xy_in, double courant, double courantb,
+                       double e_scale, int i, int j);
+
+/* Build-defined global energy: per-cell energy with c^2 taken from tcc
+ * (cells with xy_in != 0, or all cells if TWOSPEEDS; 0 elsewhere), summed
+ * in a fixed order: for each row, 64 lane-strided partials then a halving
+ * tree (32,16,...,1); the NX row sums are reduced the same way. */
+double orc_energy_total(const orc_params *p, const double *phi, const double *psi,
+                        const short *xy_in, const double *tcc, double e_scale);
+/* Same order over a precomputed per-cell energy array e[NX*NY]. */
+double orc_fixed_order_sum(int nx, int ny, const double *e);
+
+/* ---- FP32 double-float ("split") form, build-defined ----------------------
+ * State per cell: u = hi + lo (two floats, |lo| <= ulp(hi)/2) and the
+ * velocity v = u^n - u^{n-1} as one float.  Interior active cells:
+ *   dh   = ((hE-hc) + (hW-hc)) + ((hN-hc) + (hS-hc))   (9-pt: see .c)
+ *   dl   = same on lo;  lap = dh + dl
+ *   vn   = ((vc + fmaf(tcc_hi, lap, tcc_lo*lap)) - kappa*hc) - gamma*vc
+ *   (hi', lo') = hc + lc + vn   (two-sum then quick-two-sum)
+ * Edge cells use the FP64 reference formula on u = (double)hi + (double)lo,
+ * u^{n-1} = u - (double)v.  Inactive cells are untouched. */
+void orc_split_from_fp64(long n, const double *u_n, const double *u_nm1,
+                         float *hi, float *lo, float *v);
+void orc_split_to_fp64(long n, const float *hi, const float *lo, const float *v,
+                       double *u_n, double *u_nm1);
+void orc_split_run(const orc_params *p, float *hi, float *lo, float *v, long nsteps,
+                   const short *xy_in, const double *tcc, const double *tc,
+                   const double *tgamma, int nsrc, const int *si, const int *sj,
+                   const double *amp, long amp_len, long step0);
+double orc_split_energy_total(const orc_params *p, const float *hi, const float *lo,
+                              const float *v, const short *xy_in, const double *tcc,
+                              double e_scale);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,64 @@
+"""Deterministic inputs for the golden fixtures (SplitMix64 counter form, so
+the same arrays are rebuilt on any machine without storing them)."""
+import zlib
+
+import numpy as np
+
+GOLD = np.uint64(0x9E3779B97F4A7C15)
+
+
+def splitmix64_stream(seed, n):
+    """n successive SplitMix64 outputs from state `seed` (Steele et al. 2014)."""
+    with np.errstate(over="ignore"):
+        k = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + k * GOLD
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed, shape, a, b):
+    n = int(np.prod(shape))
+    u = (splitmix64_stream(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return (a + (b - a) * u).reshape(shape)
+
+
+def case_seed(tag):
+    return zlib.crc32(tag.encode())
+
+
+def case_inputs(tag, nx, ny):
+    """xy_in in {-1,0,1,2} (10/20/50/20 %), tcc U[0,0.2], tc U[0,1],
+    tgamma U[0,1e-3], phi/psi/tmp U[-1,1]."""
+    s = case_seed(tag)
+    r = uniform(s, (nx, ny), 0.0, 1.0)
+    xy = np.where(r < 0.1, -1, np.where(r < 0.3, 0, np.where(r < 0.8, 1, 2))).astype(np.int16)
+    return dict(
+        xy_in=xy,
+        tcc=uniform(s + 1, (nx, ny), 0.0, 0.2),
+        tc=uniform(s + 2, (nx, ny), 0.0, 1.0),
+        tgamma=uniform(s + 3, (nx, ny), 0.0, 1e-3),
+        phi=uniform(s + 4, (nx, ny), -1.0, 1.0),
+        psi=uniform(s + 5, (nx, ny), -1.0, 1.0),
+        tmp=uniform(s + 6, (nx, ny), -1.0, 1.0),
+    )
+
+
+KAPPAS = dict(kappa=1.0e-3, kappa_sides=5.0e-4, gamma_sides=1.0e-4, kappa_topbot=2.0e-4,
+              gamma_topbot=1.0e-7)
+
+
+def small_cases():
+    """(tag, nx, ny, bcond, twospeeds, oscillate_topbot, tb_shift) for every
+    small reference build used as a fixture."""
+    out = []
+    for nx, ny in ((24, 17), (16, 29)):
+        for bc in range(4):
+            for ts in range(2):
+                out.append(("row_%dx%d_bc%d_ts%d" % (nx, ny, bc, ts), nx, ny, bc, ts, 0, nx))
+        for ts in range(2):
+            out.append(("flat_%dx%d_bc4_ts%d" % (nx, ny, ts), nx, ny, 4, ts, 0, ny - 2))
+        out.append(("row_%dx%d_bc0_ts0_tb" % (nx, ny), nx, ny, 0, 0, 1, nx))
+        # wave_3d.c:615-616: tb_shift = NX, then NY-2 once it reaches NX-1
+        out.append(("flat_%dx%d_bc2_ts1_tb" % (nx, ny), nx, ny, 2, 1, 1, ny - 2))
+    return out

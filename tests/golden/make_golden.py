@@ -1,0 +1,81 @@
+"""Write the golden fixtures from the reference's own stepper.
+
+Run in the development container only (needs oracle/_ref/, built from
+/root/reference by oracle/build_ref.sh):
+
+    bash oracle/build_ref.sh && python tests/golden/make_golden.py
+
+Outputs (committed):
+  small_cases.npz   phi, psi, tmp after 7 evolve_wave_half steps with the
+                    evolve_wave rotation, for every case of gen_inputs.small_cases()
+  c1_digests.json   SHA-256 of phi after every 100 steps of the 1280x720
+                    plain workload (1,000 steps), plus probe values
+  energy_64x48.npz  per-cell compute_energy of a two-speed state
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "youtube-simulations_amd"))
+sys.path.insert(0, HERE)
+
+import pyoracle  # noqa: E402
+from gen_inputs import case_inputs, small_cases  # noqa: E402
+from wavehip import workloads  # noqa: E402
+
+NSTEPS_SMALL = 7
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def main():
+    out = {}
+    for (tag, nx, ny, bc, ts, tb, tbs) in small_cases():
+        R = pyoracle.RefBuild(tag)
+        assert (R.nx, R.ny) == (nx, ny)
+        d = case_inputs(tag, nx, ny)
+        b = [d["phi"].copy(), d["psi"].copy(), d["tmp"].copy()]
+        phi, psi, tmp = R.run(b[0], b[1], b[2], NSTEPS_SMALL, d["xy_in"], d["tcc"], d["tc"], d["tgamma"])
+        out[tag + "/phi"] = phi
+        out[tag + "/psi"] = psi
+        out[tag + "/tmp"] = tmp
+    np.savez_compressed(os.path.join(HERE, "small_cases.npz"), **out)
+
+    # 1280x720 plain workload, 1,000 steps, digest every 100
+    w = workloads.c1()
+    R = pyoracle.RefBuild("row_1280x720_bc0_ts0")
+    b = [w["phi"].copy(), w["psi"].copy(), np.zeros_like(w["phi"])]
+    dig = dict(inputs=dict(phi=sha(w["phi"]), psi=sha(w["psi"]), xy_in=sha(w["xy_in"]),
+                           tcc=sha(w["tcc"]), tc=sha(w["tc"]), tgamma=sha(w["tgamma"])),
+               phi={}, probes={})
+    s = 0
+    probes = [(320, 360), (400, 360), (640, 500), (1000, 100), (0, 0), (1279, 719)]
+    for n in range(1, 1001):
+        R.half(b[s], b[(s + 1) % 3], b[(s + 2) % 3], w["xy_in"], w["tcc"], w["tc"], w["tgamma"])
+        s = (s + 2) % 3
+        if n % 100 == 0:
+            dig["phi"][str(n)] = sha(b[s])
+            dig["probes"][str(n)] = [float(b[s][i, j]).hex() for (i, j) in probes]
+    dig["probe_cells"] = probes
+    with open(os.path.join(HERE, "c1_digests.json"), "w") as f:
+        json.dump(dig, f, indent=1)
+
+    # per-cell energy (wave_common.c:453-472) of a two-speed 64x48 state
+    R = pyoracle.RefBuild("row_64x48_bc2_ts1")
+    d = case_inputs("energy_64x48", 64, 48)
+    e = np.array([[R.energy(d["phi"], d["psi"], d["xy_in"], i, j) for j in range(48)] for i in range(64)])
+    np.savez_compressed(os.path.join(HERE, "energy_64x48.npz"), e=e, courant=0.1,
+                        courantb=0.0666666666666666667, e_scale=300.0)
+    print("fixtures written")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,213 @@
+"""Checks of the oracle by mathematics, for what the reference's own code
+does not fix: the build-defined 9-point Laplacian, soft sources, the split
+(double-float) form and the energy sum -- and as a second, independent pin
+of the reference branches (exact plane-wave dispersion on the torus, the
+conserved discrete energy of leapfrog, exact transmission through the
+absorbing edge at Courant 1)."""
+import math
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from wavehip import workloads
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def uniform_tables(nx, ny, cc, tc=0.1, g=0.0):
+    xy = np.ones((nx, ny), np.int16)
+    return xy, np.full((nx, ny), cc), np.full((nx, ny), tc), np.full((nx, ny), g)
+
+
+@pytest.mark.parametrize("m,l", [(1, 0), (2, 3), (5, 1)])
+def test_periodic_plane_wave_dispersion(oracle, m, l):
+    """u^n = cos(n w) cos(kx i + ky j) with cos w = 1 - 2C^2 (sin^2(kx/2) + sin^2(ky/2))
+    is the exact leapfrog solution on the periodic (B_COND 1) torus."""
+    nx, ny, C2, n = 48, 40, 0.3, 400
+    kx, ky = 2 * math.pi * m / nx, 2 * math.pi * l / ny
+    I, J = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    f = np.cos(kx * I + ky * J)
+    w = math.acos(1 - 2 * C2 * (math.sin(kx / 2) ** 2 + math.sin(ky / 2) ** 2))
+    xy, tcc, tc, tg = uniform_tables(nx, ny, C2)
+    p = oracle.make_params(nx, ny, bcond=1)
+    phi, psi, _ = oracle.run(p, f, math.cos(w) * f, n, xy, tcc, tc, tg)
+    assert np.abs(phi - math.cos(n * w) * f).max() < 1e-11
+    assert np.abs(psi - math.cos((n - 1) * w) * f).max() < 1e-11
+
+
+def test_nine_point_weights_from_symbol(oracle):
+    """one 9-point step from psi = phi = cos(k.x): interior cells get
+    1 + C^2 * s(k) with s(k) = (8 cos kx + 8 cos ky + 4 cos kx cos ky - 20)/6"""
+    nx, ny, C2 = 32, 30, 0.2
+    kx, ky = 0.7, 0.45
+    I, J = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    f = np.cos(kx * I + ky * J)
+    s = (8 * math.cos(kx) + 8 * math.cos(ky) + 4 * math.cos(kx) * math.cos(ky) - 20) / 6
+    xy, tcc, tc, tg = uniform_tables(nx, ny, C2)
+    p = oracle.make_params(nx, ny, bcond=0, stencil=9)
+    phi, _, _ = oracle.run(p, f, f, 1, xy, tcc, tc, tg)
+    assert np.abs(phi[1:-1, 1:-1] - (1 + C2 * s) * f[1:-1, 1:-1]).max() < 1e-14
+
+
+@pytest.mark.parametrize("stencil", [5, 9])
+def test_laplacian_exact_on_quadratics(oracle, stencil):
+    """Both Laplacians return 2a + 2b on a i^2 + b j^2 + c i j + d i + e j."""
+    nx, ny = 20, 17
+    a, b, c, d, e = 0.01, -0.003, 0.002, 0.1, -0.2
+    I, J = np.meshgrid(np.arange(nx, dtype=float), np.arange(ny, dtype=float), indexing="ij")
+    f = a * I * I + b * J * J + c * I * J + d * I + e * J
+    xy, tcc, tc, tg = uniform_tables(nx, ny, 1.0)
+    p = oracle.make_params(nx, ny, bcond=0, stencil=stencil)
+    phi, _, _ = oracle.run(p, f, f, 1, xy, tcc, tc, tg)
+    assert np.abs(phi[1:-1, 1:-1] - f[1:-1, 1:-1] - (2 * a + 2 * b)).max() < 1e-12
+
+
+def test_leapfrog_conserved_energy_torus(oracle):
+    """E = |u^{n+1}-u^n|^2 + C^2 sum_edges (grad u^{n+1})(grad u^n) is exactly
+    conserved by the 5-point leapfrog on the torus with gamma = kappa = 0."""
+    nx, ny, C2 = 40, 36, 0.25
+    rng = np.random.default_rng(3)
+    u0 = rng.standard_normal((nx, ny))
+    u1 = u0 + 0.01 * rng.standard_normal((nx, ny))
+    xy, tcc, tc, tg = uniform_tables(nx, ny, C2)
+    p = oracle.make_params(nx, ny, bcond=1)
+
+    def E(a, b):  # a = u^{n+1}, b = u^n
+        ex = (np.roll(a, -1, 0) - a) * (np.roll(b, -1, 0) - b)
+        ey = (np.roll(a, -1, 1) - a) * (np.roll(b, -1, 1) - b)
+        return ((a - b) ** 2).sum() + C2 * (ex.sum() + ey.sum())
+
+    e0 = E(u1, u0)
+    phi, psi, _ = oracle.run(p, u1, u0, 500, xy, tcc, tc, tg)
+    assert abs(E(phi, psi) - e0) < 1e-10 * abs(e0)
+    # a wrong sign of the damping term breaks it
+    xy, tcc, tc, tg = uniform_tables(nx, ny, C2, g=1e-3)
+    phi, psi, _ = oracle.run(p, u1, u0, 500, xy, tcc, tc, tg)
+    assert E(phi, psi) < 0.9 * e0
+
+
+def test_absorbing_edge_courant_one_is_transparent(oracle):
+    """At Courant 1 the 5-point leapfrog is exact for waves along x and the
+    absorbing edge with tc = 1 becomes u_edge <- u_inward: a right-moving
+    pulse leaves the box without reflection (rows far from the corners)."""
+    nx, ny, T = 64, 200, 70
+    i = np.arange(nx, dtype=float)
+    f = lambda x: np.exp(-((x - 32.0) / 4.0) ** 2)
+    u0 = np.repeat(f(i)[:, None], ny, 1)
+    um1 = np.repeat(f(i + 1)[:, None], ny, 1)
+    xy, tcc, tc, tg = uniform_tables(nx, ny, 1.0, tc=1.0)
+    p = oracle.make_params(nx, ny, bcond=2)
+    phi, _, _ = oracle.run(p, u0, um1, T, xy, tcc, tc, tg)
+    mid = phi[:, T + 2: ny - T - 2]
+    assert np.abs(mid).max() < 1e-12
+    # the same run with reflecting (zero-flux) edges keeps the pulse
+    p = oracle.make_params(nx, ny, bcond=0)
+    phi, _, _ = oracle.run(p, u0, um1, T, xy, tcc, tc, tg)
+    assert np.abs(phi[:, T + 2: ny - T - 2]).max() > 0.5
+
+
+def test_absorbing_single_step_reduces_to_inward(oracle):
+    nx, ny = 9, 7
+    rng = np.random.default_rng(1)
+    u0, um1 = rng.standard_normal((nx, ny)), rng.standard_normal((nx, ny))
+    xy, tcc, tc, tg = uniform_tables(nx, ny, 0.1, tc=1.0)
+    p = oracle.make_params(nx, ny, bcond=2)
+    phi, _, _ = oracle.run(p, u0, um1, 1, xy, tcc, tc, tg)
+    tol = 1e-15 * 8  # x - (x - inward) rounds once
+    assert np.abs(phi[0, 1:-1] - u0[1, 1:-1]).max() < tol
+    assert np.abs(phi[-1, 1:-1] - u0[-2, 1:-1]).max() < tol
+    assert np.abs(phi[:, -1] - u0[:, -2]).max() < tol
+    assert np.abs(phi[:, 0] - u0[:, 1]).max() < tol
+
+
+def test_masked_cells_stay_constant(oracle):
+    w = workloads.random_case(30, 25, 5, bcond=1, neg_cells=False)
+    w["phi"][w["xy_in"] == 0] = 0
+    w["psi"][w["xy_in"] == 0] = 0
+    p = oracle.make_params(30, 25, bcond=1)
+    phi, psi, tmp = oracle.run(p, w["phi"], w["psi"], 11, w["xy_in"], w["tcc"], w["tc"], w["tgamma"])
+    m = w["xy_in"] == 0
+    m[0, :] = m[-1, :] = m[:, 0] = m[:, -1] = False
+    for a in (phi, psi, tmp):
+        assert np.all(a[m] == 0)
+
+
+def test_sources_are_linear_and_reciprocal(oracle):
+    nx, ny, n = 40, 40, 120
+    xy, tcc, tc, tg = uniform_tables(nx, ny, 0.2)
+    p = oracle.make_params(nx, ny, bcond=1)
+    z = np.zeros((nx, ny))
+    t = np.arange(1, n + 1)
+    a1 = np.sin(2 * math.pi * t / 17.0)[None, :]
+    a2 = np.cos(2 * math.pi * t / 23.0)[None, :]
+    S = lambda i, j, a: dict(i=[i], j=[j], amp=a)
+    u1 = oracle.run(p, z, z, n, xy, tcc, tc, tg, sources=S(10, 12, a1))[0]
+    u2 = oracle.run(p, z, z, n, xy, tcc, tc, tg, sources=S(25, 30, a2))[0]
+    u12 = oracle.run(p, z, z, n, xy, tcc, tc, tg,
+                     sources=dict(i=[10, 25], j=[12, 30], amp=np.vstack([a1, a2])))[0]
+    assert np.abs(u12 - (u1 + u2)).max() < 1e-12 * max(1, np.abs(u12).max())
+    # impulse reciprocity G(a->b) == G(b->a) for the symmetric operator
+    imp = np.zeros((1, n)); imp[0, 0] = 1.0
+    ga = oracle.run(p, z, z, n, xy, tcc, tc, tg, sources=S(10, 12, imp))[0]
+    gb = oracle.run(p, z, z, n, xy, tcc, tc, tg, sources=S(25, 30, imp))[0]
+    assert abs(ga[25, 30] - gb[10, 12]) < 1e-14 and abs(ga[25, 30]) > 1e-8
+
+
+def test_split_form_tracks_fp64(oracle):
+    """The FP32 double-float form stays within 1e-6 of FP64 (L2, relative
+    to the field's norm; with absorbing edges, relative to the initial norm,
+    since the field itself drains out of the box)."""
+    nx = ny = 96
+    x, y = workloads.grid_xy(nx, ny, -1, 1, -1, 1)
+    f = workloads.gaussian_bumps(x, y, [(-0.2, 0.1)], 0.15)
+    xy = np.ones((nx, ny), np.int16)
+    tcc, tc, tg = workloads.constant_media_tables(xy, 0.1, 0.1, 0.0, 0.0, 0)
+    for bc, st in ((2, 5), (1, 9), (0, 5), (3, 9)):
+        p = oracle.make_params(nx, ny, bcond=bc, stencil=st)
+        u, um, _ = oracle.run(p, f, f, 3000, xy, tcc, tc, tg)
+        hi, lo, v = oracle.split_from_fp64(f, f)
+        hi, lo, v = oracle.split_run(p, hi, lo, v, 3000, xy, tcc, tc, tg)
+        us, ums = oracle.split_to_fp64(hi, lo, v)
+        ref = np.linalg.norm(f) if bc in (2, 3) else np.linalg.norm(u)
+        rel = np.linalg.norm(us - u) / ref
+        assert rel < 1e-6, (bc, st, rel)
+        assert np.linalg.norm((us - ums) - (u - um)) / np.linalg.norm(f) < 1e-7
+
+
+def test_split_conversion_round_trip(oracle):
+    rng = np.random.default_rng(0)
+    u = rng.standard_normal((20, 20))
+    um = u - 1e-3 * rng.standard_normal((20, 20))
+    hi, lo, v = oracle.split_from_fp64(u, um)
+    a, b = oracle.split_to_fp64(hi, lo, v)
+    assert np.abs(a - u).max() <= 2.0 ** -46 * np.abs(u).max()
+    # v = u - um is rounded once to binary32
+    assert np.all(np.abs(b - um) <= 2.0 ** -24 * np.abs(u - um) + 2.0 ** -46 * np.abs(u))
+
+
+def test_fixed_order_sum_is_accurate_and_deterministic(oracle):
+    rng = np.random.default_rng(2)
+    e = rng.uniform(0, 1, (300, 257))
+    s = oracle.fixed_order_sum(e)
+    assert s == oracle.fixed_order_sum(e.copy())
+    assert abs(s - math.fsum(e.ravel())) < 1e-12 * s
+
+
+def test_thread_count_does_not_change_bits(oracle, tmp_path):
+    code = (
+        "import sys, numpy as np; sys.path[:0] = %r\n"
+        "import pyoracle as O; from wavehip import workloads\n"
+        "w = workloads.random_case(70, 53, 9, bcond=3, kappas=True)\n"
+        "p = O.make_params(70, 53, bcond=3, kappa=1e-3, kappa_sides=5e-4, gamma_sides=1e-4)\n"
+        "phi = O.run(p, w['phi'], w['psi'], 25, w['xy_in'], w['tcc'], w['tc'], w['tgamma'])[0]\n"
+        "np.save(sys.argv[1], phi)\n" % ([os.path.join(ROOT, "oracle"), os.path.join(ROOT, "youtube-simulations_amd")],))
+    out = []
+    for th in ("1", "5"):
+        f = tmp_path / ("phi%s.npy" % th)
+        env = dict(os.environ, OMP_NUM_THREADS=th)
+        subprocess.run([sys.executable, "-c", code, str(f)], check=True, env=env)
+        out.append(np.load(f))
+    assert np.array_equal(out[0].view(np.uint64), out[1].view(np.uint64))

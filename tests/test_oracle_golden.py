@@ -1,0 +1,135 @@
+"""The CPU oracle against the reference's own stepper.
+
+Golden vectors (tests/golden/) were produced by tests/golden/make_golden.py
+from the reference compiled from its sources (wave_billiard.c:302-631,
+wave_3d.c:600-1007, wave_common.c:453-472).  Comparisons are bit-exact.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from gen_inputs import KAPPAS, case_inputs, small_cases
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def bits_equal(a, b):
+    return np.array_equal(np.asarray(a).view(np.uint64), np.asarray(b).view(np.uint64))
+
+
+@pytest.fixture(scope="module")
+def small():
+    return np.load(os.path.join(GOLD, "small_cases.npz"))
+
+
+@pytest.mark.parametrize("case", small_cases(), ids=lambda c: c[0])
+def test_oracle_matches_reference_fixture(oracle, small, case):
+    tag, nx, ny, bc, ts, tb, tbs = case
+    d = case_inputs(tag, nx, ny)
+    p = oracle.make_params(nx, ny, bcond=bc, twospeeds=ts, oscillate_topbot=tb, tb_shift=tbs, **KAPPAS)
+    phi, psi, tmp = oracle.run(p, d["phi"], d["psi"], 7, d["xy_in"], d["tcc"], d["tc"], d["tgamma"],
+                               tmp=d["tmp"])
+    assert bits_equal(phi, small[tag + "/phi"])
+    assert bits_equal(psi, small[tag + "/psi"])
+    assert bits_equal(tmp, small[tag + "/tmp"])
+
+
+def test_flat_signature_matches_rowptr(oracle, small):
+    """evolve_wave_flat (wave_3d.c:1010 order) gives the same three buffers."""
+    tag, nx, ny, bc, ts, tb, tbs = small_cases()[2]
+    d = case_inputs(tag, nx, ny)
+    p = oracle.make_params(nx, ny, bcond=bc, twospeeds=ts, **KAPPAS)
+    a, b, c = d["phi"].copy(), d["psi"].copy(), d["tmp"].copy()
+    for _ in range(2):
+        oracle.evolve_wave_flat(p, a, b, c, d["xy_in"], d["tcc"], d["tc"], d["tgamma"])
+    # 6 steps with the rotation = run(6)
+    phi, psi, tmp = oracle.run(p, d["phi"], d["psi"], 6, d["xy_in"], d["tcc"], d["tc"], d["tgamma"],
+                               tmp=d["tmp"])
+    assert bits_equal(a, phi) and bits_equal(b, psi) and bits_equal(c, tmp)
+
+
+def test_fixture_detects_planted_mistakes(oracle, small):
+    """A plausible mistake in the oracle (sign of KAPPA, a transposed
+    B_COND, a dropped damping term) must fail the golden comparison."""
+    tag, nx, ny, bc, ts, tb, tbs = ("row_24x17_bc3_ts1", 24, 17, 3, 1, 0, 24)
+    d = case_inputs(tag, nx, ny)
+    mutants = [
+        dict(KAPPAS, kappa=-KAPPAS["kappa"]),
+        dict(KAPPAS, gamma_sides=0.0),
+        dict(KAPPAS, kappa_topbot=KAPPAS["kappa_sides"], kappa_sides=KAPPAS["kappa_topbot"]),
+    ]
+    for kw in mutants:
+        p = oracle.make_params(nx, ny, bcond=bc, twospeeds=ts, **kw)
+        phi, _, _ = oracle.run(p, d["phi"], d["psi"], 7, d["xy_in"], d["tcc"], d["tc"], d["tgamma"],
+                               tmp=d["tmp"])
+        assert not bits_equal(phi, small[tag + "/phi"])
+    for wrong_bc in (0, 1, 2):
+        p = oracle.make_params(nx, ny, bcond=wrong_bc, twospeeds=ts, **KAPPAS)
+        phi, _, _ = oracle.run(p, d["phi"], d["psi"], 7, d["xy_in"], d["tcc"], d["tc"], d["tgamma"],
+                               tmp=d["tmp"])
+        assert not bits_equal(phi, small[tag + "/phi"])
+
+
+def test_c1_plain_workload_digests(oracle):
+    """1280x720 plain workload: phi after every 100 of 1,000 steps equals
+    the reference's bit for bit (SHA-256)."""
+    from wavehip import workloads
+    w = workloads.c1()
+    dig = json.load(open(os.path.join(GOLD, "c1_digests.json")))
+    for k in ("phi", "psi", "xy_in", "tcc", "tc", "tgamma"):
+        assert sha(w[k]) == dig["inputs"][k], "workload builder changed: " + k
+    p = oracle.make_params(1280, 720, bcond=0)
+    phi, psi, tmp = w["phi"], w["psi"], np.zeros_like(w["phi"])
+    for n in range(100, 1001, 100):
+        phi, psi, tmp = oracle.run(p, phi, psi, 100, w["xy_in"], w["tcc"], w["tc"], w["tgamma"], tmp=tmp)
+        assert sha(phi) == dig["phi"][str(n)], "step %d" % n
+    probes = [float.fromhex(h) for h in dig["probes"]["1000"]]
+    assert max(abs(v) for v in probes) > 0  # the pulse is still there
+
+
+def test_energy_cell_matches_reference(oracle):
+    z = np.load(os.path.join(GOLD, "energy_64x48.npz"))
+    d = case_inputs("energy_64x48", 64, 48)
+    p = oracle.make_params(64, 48, bcond=2, twospeeds=1)
+    e = np.array([[oracle.energy_cell(p, d["phi"], d["psi"], d["xy_in"], float(z["courant"]),
+                                      float(z["courantb"]), float(z["e_scale"]), i, j)
+                   for j in range(48)] for i in range(64)])
+    assert bits_equal(e, z["e"])
+
+
+def test_energy_total_uses_table_courant(oracle):
+    """Build-defined total: per-cell compute_energy with c^2 from tcc equals
+    the reference's COURANT^2 form on constant media, summed in fixed order."""
+    from wavehip import workloads
+    z = np.load(os.path.join(GOLD, "energy_64x48.npz"))
+    d = case_inputs("energy_64x48", 64, 48)
+    tcc, tc, tg = workloads.constant_media_tables(d["xy_in"], float(z["courant"]), float(z["courantb"]),
+                                                  0.0, 0.0, 1)
+    p = oracle.make_params(64, 48, bcond=2, twospeeds=1)
+    tot = oracle.energy_total(p, d["phi"], d["psi"], d["xy_in"], tcc, float(z["e_scale"]))
+    assert tot == oracle.fixed_order_sum(z["e"])
+    assert abs(tot - z["e"].sum()) <= 1e-12 * z["e"].sum()
+
+
+@pytest.mark.skipif(not __import__("pyoracle").ref_available(), reason="oracle/_ref not built (needs /root/reference)")
+def test_oracle_matches_reference_build_random(oracle):
+    """Direct cross-check against oracle/_ref on fresh random inputs."""
+    import pyoracle
+    rng = np.random.default_rng(7)
+    for (tag, nx, ny, bc, ts, tb, tbs) in small_cases():
+        R = pyoracle.RefBuild(tag)
+        xy = rng.choice(np.array([-1, 0, 1, 2], np.int16), size=(nx, ny))
+        t = [rng.uniform(0, 0.25, (nx, ny)) for _ in range(3)]
+        s = [rng.standard_normal((nx, ny)) for _ in range(3)]
+        ref = R.run(s[0].copy(), s[1].copy(), s[2].copy(), 5, xy, t[0], t[1], t[2])
+        p = oracle.make_params(nx, ny, bcond=bc, twospeeds=ts, oscillate_topbot=tb, tb_shift=tbs, **KAPPAS)
+        got = oracle.run(p, s[0], s[1], 5, xy, t[0], t[1], t[2], tmp=s[2])
+        for a, b in zip(ref, got):
+            assert bits_equal(a, b), tag

@@ -1,0 +1,1116 @@
+// wv_api.cpp -- host side of libwavehip.so: the C ABI of include/wavehip.h.
+//
+// One Engine drives n independent grids ("instances") that share a form
+// (FP64 / FP32 split), a stencil and a tiling; a wv_ctx is an Engine with one
+// instance plus the drop-in entry points, a wv_batch is an Engine with many.
+// Each stepping launch covers the tiles of every instance (one tile = one
+// wave64 streaming a strip of rows), so small grids of a batch fill the chip
+// together.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/wavehip.h"
+#include "wv_internal.h"
+
+namespace {
+
+struct Grid {
+    wv_params p{};
+    int ldy = 0, ldw = 0;
+    double *u[4] = {nullptr, nullptr, nullptr, nullptr};
+    float *hi[2] = {nullptr, nullptr}, *lo[2] = {nullptr, nullptr}, *v[2] = {nullptr, nullptr};
+    uint64_t *act = nullptr, *eact = nullptr, *med = nullptr;
+    double *tccf = nullptr, *tgf = nullptr, *tc = nullptr;
+    float *tccf_hi = nullptr, *tccf_lo = nullptr, *tgf32 = nullptr;
+    int *src_ptr = nullptr, *src_col = nullptr, *src_id = nullptr;
+    double *amp = nullptr;
+    long amp_len = 0;
+    int coef = WV_COEF_UNIFORM;
+    double tcc0 = 0, tcc1 = 0, tg0 = 0, tg1 = 0;
+    bool tables = false, state = false;
+    bool exact = false;                 // masked cells not all +0: one step per launch
+    std::vector<uint64_t> h_act;        // host copy of the active plane
+};
+
+struct Engine {
+    int n = 0, device = 0, form = WV_FORM_FP64, stencil = 5;
+    hipStream_t stream = nullptr;
+    std::vector<Grid> g;
+    int role[4] = {0, 1, 2, 3};  // FP64: phi, psi, free1 (tmp), free2
+    int scur = 0;                // split: current set
+    WvInst *d_inst = nullptr;
+    std::map<int, std::pair<WvTile *, int>> tiles;  // depth -> device tile table
+    int K = 0, L = 0;            // tiling (0 = untuned default)
+    long steps_done = 0, last_launches = 0;
+    long *d_gstep = nullptr;
+    double *d_partial = nullptr, *d_eout = nullptr;
+    int2 *d_erows = nullptr;
+    int *d_row0 = nullptr;
+    int nrows_total = 0;
+    std::string err;
+    // batch graph cache: (nsteps, rotation state) -> executable graph
+    std::map<std::pair<long, long>, hipGraphExec_t> graphs;
+    bool use_graph = false;
+    // staging for the drop-in entry points
+    std::vector<double> st[3], stt[3];
+    std::vector<short> stm;
+    bool dropin_tables = false;
+};
+
+int fail(Engine *e, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (e) e->err = buf;
+    return code;
+}
+
+#define HIPCHK(e, call)                                                                          \
+    do {                                                                                         \
+        hipError_t err_ = (call);                                                                \
+        if (err_ != hipSuccess)                                                                  \
+            return fail(e, WV_EHIP, "%s failed: %s", #call, hipGetErrorString(err_));           \
+    } while (0)
+
+int round_up(int a, int b) { return (a + b - 1) / b * b; }
+
+int default_depth(const Engine *e) { return e->form == WV_FORM_F32SPLIT ? 4 : 4; }
+int default_rows(const Engine *) { return 64; }
+int depth(const Engine *e) { return e->K > 0 ? e->K : default_depth(e); }
+int rows(const Engine *e) { return e->L > 0 ? e->L : default_rows(e); }
+
+int check_params(Engine *e, const wv_params *p)
+{
+    if (!p) return fail(e, WV_EINVAL, "params is NULL");
+    if (p->nx < 3 || p->ny < 3) return fail(e, WV_EINVAL, "nx, ny must be >= 3 (got %d, %d)", p->nx, p->ny);
+    if ((long)p->nx * p->ny > (1L << 31)) return fail(e, WV_EINVAL, "grid too large");
+    if (p->bcond < 0 || p->bcond > 4) return fail(e, WV_EINVAL, "bcond %d is not a B_COND code 0..4", p->bcond);
+    if (p->stencil != 5 && p->stencil != 9) return fail(e, WV_EINVAL, "stencil must be 5 or 9");
+    if (p->form != WV_FORM_FP64 && p->form != WV_FORM_F32SPLIT) return fail(e, WV_EINVAL, "unknown form %d", p->form);
+    if (p->tb_depth < 0 || p->tb_depth > 8 || (p->tb_depth && !wv_have_depth(p->tb_depth)))
+        return fail(e, WV_EINVAL, "tb_depth must be 0 or one of 1,2,3,4,6,8");
+    if (p->rows_per_tile < 0) return fail(e, WV_EINVAL, "rows_per_tile must be >= 0");
+    return WV_OK;
+}
+
+int alloc_grid(Engine *e, Grid &G)
+{
+    const size_t cells = (size_t)G.p.nx * G.ldy;
+    const size_t words = (size_t)G.p.nx * G.ldw;
+    if (e->form == WV_FORM_FP64) {
+        for (int k = 0; k < 4; k++) {
+            HIPCHK(e, hipMalloc(&G.u[k], cells * sizeof(double)));
+            HIPCHK(e, hipMemset(G.u[k], 0, cells * sizeof(double)));
+        }
+    } else {
+        for (int k = 0; k < 2; k++) {
+            HIPCHK(e, hipMalloc(&G.hi[k], cells * sizeof(float)));
+            HIPCHK(e, hipMalloc(&G.lo[k], cells * sizeof(float)));
+            HIPCHK(e, hipMalloc(&G.v[k], cells * sizeof(float)));
+            HIPCHK(e, hipMemset(G.hi[k], 0, cells * sizeof(float)));
+            HIPCHK(e, hipMemset(G.lo[k], 0, cells * sizeof(float)));
+            HIPCHK(e, hipMemset(G.v[k], 0, cells * sizeof(float)));
+        }
+    }
+    HIPCHK(e, hipMalloc(&G.act, words * 8));
+    HIPCHK(e, hipMalloc(&G.eact, words * 8));
+    HIPCHK(e, hipMalloc(&G.med, words * 8));
+    HIPCHK(e, hipMalloc(&G.tc, cells * sizeof(double)));
+    HIPCHK(e, hipMemset(G.act, 0, words * 8));
+    HIPCHK(e, hipMemset(G.eact, 0, words * 8));
+    HIPCHK(e, hipMemset(G.med, 0, words * 8));
+    HIPCHK(e, hipMemset(G.tc, 0, cells * sizeof(double)));
+    return WV_OK;
+}
+
+void free_grid(Grid &G)
+{
+    void *ptrs[] = {G.u[0], G.u[1], G.u[2], G.u[3], G.hi[0], G.hi[1], G.lo[0], G.lo[1], G.v[0], G.v[1],
+                    G.act, G.eact, G.med, G.tccf, G.tgf, G.tc, G.tccf_hi, G.tccf_lo, G.tgf32,
+                    G.src_ptr, G.src_col, G.src_id, G.amp};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+}
+
+WvInst make_inst(const Engine *e, const Grid &G)
+{
+    WvInst d{};
+    for (int k = 0; k < 4; k++) d.u[k] = G.u[k];
+    for (int k = 0; k < 2; k++) { d.hi[k] = G.hi[k]; d.lo[k] = G.lo[k]; d.v[k] = G.v[k]; }
+    d.act = G.act; d.eact = G.eact; d.med = G.med;
+    d.tccf = G.tccf; d.tgf = G.tgf; d.tc = G.tc;
+    d.tccf_hi = G.tccf_hi; d.tccf_lo = G.tccf_lo; d.tgf32 = G.tgf32;
+    d.src_ptr = G.src_ptr; d.src_col = G.src_col; d.src_id = G.src_id;
+    d.amp = G.amp; d.amp_len = G.amp_len;
+    d.nx = G.p.nx; d.ny = G.p.ny; d.ldy = G.ldy; d.ldw = G.ldw;
+    d.bc = G.p.bcond; d.osc_tb = G.p.oscillate_topbot; d.tb_shift = G.p.tb_shift; d.coef = G.coef;
+    d.rows_per_tile = rows(e);
+    d.tcc0 = G.tcc0; d.tcc1 = G.tcc1; d.tg0 = G.tg0; d.tg1 = G.tg1;
+    d.tcch0 = (float)G.tcc0; d.tccl0 = (float)(G.tcc0 - (double)d.tcch0);
+    d.tcch1 = (float)G.tcc1; d.tccl1 = (float)(G.tcc1 - (double)d.tcch1);
+    d.tgf0 = (float)G.tg0; d.tgf1 = (float)G.tg1;
+    d.kappa = G.p.kappa; d.ks = G.p.kappa_sides; d.gs = G.p.gamma_sides;
+    d.kt = G.p.kappa_topbot; d.gt = G.p.gamma_topbot;
+    d.es2 = G.p.e_scale * G.p.e_scale;
+    return d;
+}
+
+int sync_inst(Engine *e)
+{
+    std::vector<WvInst> h(e->n);
+    for (int k = 0; k < e->n; k++) h[k] = make_inst(e, e->g[k]);
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipMemcpy(e->d_inst, h.data(), sizeof(WvInst) * e->n, hipMemcpyHostToDevice));
+    return WV_OK;
+}
+
+void drop_graphs(Engine *e)
+{
+    for (auto &kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
+    e->graphs.clear();
+}
+
+void drop_tiles(Engine *e)
+{
+    for (auto &kv : e->tiles) (void)hipFree(kv.second.first);
+    e->tiles.clear();
+    drop_graphs(e);
+}
+
+// tile table of a launch of depth k: every instance, row strips of
+// rows(e) rows, column blocks of wv_tile_cols(k) columns
+int get_tiles(Engine *e, int k, WvTile **out, int *count)
+{
+    auto it = e->tiles.find(k);
+    if (it != e->tiles.end()) {
+        *out = it->second.first;
+        *count = it->second.second;
+        return WV_OK;
+    }
+    const int wout = wv_tile_cols(k), L = rows(e);
+    std::vector<WvTile> h;
+    for (int q = 0; q < e->n; q++) {
+        const Grid &G = e->g[q];
+        const int ncb = (G.p.ny + wout - 1) / wout, nsb = (G.p.nx + L - 1) / L;
+        for (int sb = 0; sb < nsb; sb++)
+            for (int cb = 0; cb < ncb; cb++) h.push_back(WvTile{q, cb, sb, 0});
+    }
+    WvTile *d = nullptr;
+    HIPCHK(e, hipMalloc(&d, sizeof(WvTile) * h.size()));
+    HIPCHK(e, hipMemcpy(d, h.data(), sizeof(WvTile) * h.size(), hipMemcpyHostToDevice));
+    e->tiles[k] = {d, (int)h.size()};
+    *out = d;
+    *count = (int)h.size();
+    return WV_OK;
+}
+
+int engine_create(Engine **out, int n, const wv_params *ps)
+{
+    Engine *e = new Engine();
+    *out = e;
+    if (n < 1) return fail(e, WV_EINVAL, "need at least one grid");
+    for (int k = 0; k < n; k++) {
+        int rc = check_params(e, &ps[k]);
+        if (rc) return rc;
+        if (ps[k].form != ps[0].form || ps[k].stencil != ps[0].stencil || ps[k].device != ps[0].device)
+            return fail(e, WV_EINVAL, "all grids of a batch need the same form, stencil and device");
+    }
+    e->n = n;
+    e->device = ps[0].device;
+    e->form = ps[0].form;
+    e->stencil = ps[0].stencil;
+    e->K = ps[0].tb_depth;
+    e->L = ps[0].rows_per_tile;
+    int ndev = 0;
+    HIPCHK(e, hipGetDeviceCount(&ndev));
+    if (e->device < 0 || e->device >= ndev) return fail(e, WV_EINVAL, "device %d not present (%d devices)", e->device, ndev);
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    e->g.resize(n);
+    int totrows = 0;
+    for (int k = 0; k < n; k++) {
+        Grid &G = e->g[k];
+        G.p = ps[k];
+        G.ldy = round_up(G.p.ny, 64);
+        G.ldw = G.ldy / 64;
+        int rc = alloc_grid(e, G);
+        if (rc) return rc;
+        totrows += G.p.nx;
+    }
+    HIPCHK(e, hipMalloc(&e->d_inst, sizeof(WvInst) * n));
+    HIPCHK(e, hipMalloc(&e->d_gstep, sizeof(long)));
+    HIPCHK(e, hipMemset(e->d_gstep, 0, sizeof(long)));
+    // energy row table
+    std::vector<int2> rowsv;
+    std::vector<int> row0(n + 1, 0);
+    for (int k = 0; k < n; k++) {
+        row0[k] = (int)rowsv.size();
+        for (int i = 0; i < e->g[k].p.nx; i++) rowsv.push_back(make_int2(k, i));
+    }
+    row0[n] = (int)rowsv.size();
+    e->nrows_total = totrows;
+    HIPCHK(e, hipMalloc(&e->d_erows, sizeof(int2) * rowsv.size()));
+    HIPCHK(e, hipMemcpy(e->d_erows, rowsv.data(), sizeof(int2) * rowsv.size(), hipMemcpyHostToDevice));
+    HIPCHK(e, hipMalloc(&e->d_row0, sizeof(int) * (n + 1)));
+    HIPCHK(e, hipMemcpy(e->d_row0, row0.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    HIPCHK(e, hipMalloc(&e->d_partial, sizeof(double) * rowsv.size()));
+    HIPCHK(e, hipMalloc(&e->d_eout, sizeof(double) * n));
+    return sync_inst(e);
+}
+
+void engine_destroy(Engine *e)
+{
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    drop_tiles(e);
+    for (auto &G : e->g) free_grid(G);
+    void *ptrs[] = {e->d_inst, e->d_gstep, e->d_partial, e->d_eout, e->d_erows, e->d_row0};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+inline bool bit(const std::vector<uint64_t> &pl, int ldw, int i, int j)
+{
+    return (pl[(size_t)i * ldw + (j >> 6)] >> (j & 63)) & 1u;
+}
+
+inline bool bits_zero(double x)
+{
+    uint64_t b;
+    memcpy(&b, &x, 8);
+    return b == 0;
+}
+
+// do the masked (never written) cells of these arrays all hold +0?
+bool masked_zero(const Grid &G, const double *a)
+{
+    if (!a) return true;
+    const int nx = G.p.nx, ny = G.p.ny;
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++)
+            if (!bit(G.h_act, G.ldw, i, j) && !bits_zero(a[(size_t)i * ny + j])) return false;
+    return true;
+}
+
+int set_tables(Engine *e, int k, const short *xy, const double *tcc, const double *tc, const double *tg)
+{
+    if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
+    if (!xy || !tcc || !tc || !tg) return fail(e, WV_EINVAL, "table pointer is NULL");
+    Grid &G = e->g[k];
+    const int nx = G.p.nx, ny = G.p.ny, ldw = G.ldw, ldy = G.ldy;
+    const bool ts = G.p.twospeeds != 0;
+    std::vector<uint64_t> act((size_t)nx * ldw, 0), eact((size_t)nx * ldw, 0), med((size_t)nx * ldw, 0);
+    // distinct (tcc, tgamma) pairs over the cells a step or the energy reads
+    std::vector<std::pair<uint64_t, uint64_t>> cls;
+    bool field = false;
+    auto key = [](double a) { uint64_t b; memcpy(&b, &a, 8); return b; };
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++) {
+            const size_t c = (size_t)i * ny + j;
+            const bool edge = i == 0 || i == nx - 1 || j == 0 || j == ny - 1;
+            const bool a = ts || (edge ? xy[c] != 0 : xy[c] > 0);
+            const bool en = ts || xy[c] != 0;
+            const size_t w = (size_t)i * ldw + (j >> 6);
+            const uint64_t b = 1ull << (j & 63);
+            if (a) act[w] |= b;
+            if (en) eact[w] |= b;
+            if (!(a || en) || field) continue;
+            const auto pr = std::make_pair(key(tcc[c]), key(tg[c]));
+            int idx = -1;
+            for (size_t q = 0; q < cls.size(); q++)
+                if (cls[q] == pr) idx = (int)q;
+            if (idx < 0) {
+                if (cls.size() == 2) { field = true; continue; }
+                cls.push_back(pr);
+                idx = (int)cls.size() - 1;
+            }
+            if (idx == 1) med[w] |= b;
+        }
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    const size_t wb = (size_t)nx * ldw * 8;
+    HIPCHK(e, hipMemcpy(G.act, act.data(), wb, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(G.eact, eact.data(), wb, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(G.med, med.data(), wb, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy2D(G.tc, ldy * sizeof(double), tc, ny * sizeof(double), ny * sizeof(double), nx,
+                          hipMemcpyHostToDevice));
+    for (double **q : {&G.tccf, &G.tgf})
+        if (*q) { (void)hipFree(*q); *q = nullptr; }
+    for (float **q : {&G.tccf_hi, &G.tccf_lo, &G.tgf32})
+        if (*q) { (void)hipFree(*q); *q = nullptr; }
+    auto dbl = [](uint64_t b) { double a; memcpy(&a, &b, 8); return a; };
+    if (!field) {
+        G.coef = cls.size() == 2 ? WV_COEF_TWO : WV_COEF_UNIFORM;
+        G.tcc0 = cls.empty() ? 0.0 : dbl(cls[0].first);
+        G.tg0 = cls.empty() ? 0.0 : dbl(cls[0].second);
+        G.tcc1 = cls.size() == 2 ? dbl(cls[1].first) : G.tcc0;
+        G.tg1 = cls.size() == 2 ? dbl(cls[1].second) : G.tg0;
+    } else {
+        G.coef = WV_COEF_FIELD;
+        const size_t cells = (size_t)nx * ldy;
+        // tgamma: one value over the read cells, or a field
+        bool tg_uniform = true;
+        double tgu = 0;
+        bool first = true;
+        for (int i = 0; i < nx && tg_uniform; i++)
+            for (int j = 0; j < ny; j++) {
+                const size_t c = (size_t)i * ny + j;
+                if (!(bit(act, ldw, i, j) || bit(eact, ldw, i, j))) continue;
+                if (first) { tgu = tg[c]; first = false; }
+                else if (key(tg[c]) != key(tgu)) { tg_uniform = false; break; }
+            }
+        G.tg0 = G.tg1 = tgu;
+        HIPCHK(e, hipMalloc(&G.tccf, cells * sizeof(double)));
+        HIPCHK(e, hipMemcpy2D(G.tccf, ldy * sizeof(double), tcc, ny * sizeof(double), ny * sizeof(double),
+                              nx, hipMemcpyHostToDevice));
+        if (!tg_uniform) {
+            HIPCHK(e, hipMalloc(&G.tgf, cells * sizeof(double)));
+            HIPCHK(e, hipMemcpy2D(G.tgf, ldy * sizeof(double), tg, ny * sizeof(double), ny * sizeof(double),
+                                  nx, hipMemcpyHostToDevice));
+        }
+        if (e->form == WV_FORM_F32SPLIT) {
+            std::vector<float> h((size_t)nx * ldy, 0.f), l((size_t)nx * ldy, 0.f);
+            bool anylo = false;
+            for (int i = 0; i < nx; i++)
+                for (int j = 0; j < ny; j++) {
+                    const double t = tcc[(size_t)i * ny + j];
+                    const float th = (float)t;
+                    const float tl = (float)(t - (double)th);
+                    h[(size_t)i * ldy + j] = th;
+                    l[(size_t)i * ldy + j] = tl;
+                    if (tl != 0.0f) anylo = true;
+                }
+            HIPCHK(e, hipMalloc(&G.tccf_hi, cells * sizeof(float)));
+            HIPCHK(e, hipMemcpy(G.tccf_hi, h.data(), cells * sizeof(float), hipMemcpyHostToDevice));
+            if (anylo) {
+                HIPCHK(e, hipMalloc(&G.tccf_lo, cells * sizeof(float)));
+                HIPCHK(e, hipMemcpy(G.tccf_lo, l.data(), cells * sizeof(float), hipMemcpyHostToDevice));
+            }
+            if (!tg_uniform) {
+                for (int i = 0; i < nx; i++)
+                    for (int j = 0; j < ny; j++) h[(size_t)i * ldy + j] = (float)tg[(size_t)i * ny + j];
+                HIPCHK(e, hipMalloc(&G.tgf32, cells * sizeof(float)));
+                HIPCHK(e, hipMemcpy(G.tgf32, h.data(), cells * sizeof(float), hipMemcpyHostToDevice));
+            }
+        }
+    }
+    // a state set under another mask may hold non-zero values on cells that
+    // are now never written: keep it, but step it one step per launch
+    if (G.state && G.h_act != act) G.exact = true;
+    G.h_act.swap(act);
+    G.tables = true;
+    drop_graphs(e);
+    return sync_inst(e);
+}
+
+int upload2d(Engine *e, void *dst, int ldy, const void *src, int nx, int ny, size_t es)
+{
+    HIPCHK(e, hipMemcpy2DAsync(dst, ldy * es, src, ny * es, ny * es, nx, hipMemcpyHostToDevice, e->stream));
+    return WV_OK;
+}
+
+int download2d(Engine *e, void *dst, const void *src, int ldy, int nx, int ny, size_t es)
+{
+    HIPCHK(e, hipMemcpy2DAsync(dst, ny * es, src, ldy * es, ny * es, nx, hipMemcpyDeviceToHost, e->stream));
+    return WV_OK;
+}
+
+int set_state(Engine *e, int k, const double *phi, const double *psi, const double *tmp)
+{
+    if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
+    if (!phi || !psi) return fail(e, WV_EINVAL, "state pointer is NULL");
+    Grid &G = e->g[k];
+    if (!G.tables) return fail(e, WV_ESTATE, "set the tables before the state");
+    const int nx = G.p.nx, ny = G.p.ny;
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    int rc;
+    if (e->form == WV_FORM_FP64) {
+        const size_t bytes = (size_t)nx * G.ldy * sizeof(double);
+        if ((rc = upload2d(e, G.u[e->role[0]], G.ldy, phi, nx, ny, 8))) return rc;
+        if ((rc = upload2d(e, G.u[e->role[1]], G.ldy, psi, nx, ny, 8))) return rc;
+        if (tmp) {
+            if ((rc = upload2d(e, G.u[e->role[2]], G.ldy, tmp, nx, ny, 8))) return rc;
+        } else {
+            HIPCHK(e, hipMemsetAsync(G.u[e->role[2]], 0, bytes, e->stream));
+        }
+        HIPCHK(e, hipMemsetAsync(G.u[e->role[3]], 0, bytes, e->stream));
+        G.exact = !(masked_zero(G, phi) && masked_zero(G, psi) && masked_zero(G, tmp));
+    } else {
+        const size_t n = (size_t)nx * ny;
+        std::vector<float> h(n), l(n), v(n);
+        for (size_t c = 0; c < n; c++) {
+            const float hh = (float)phi[c];
+            h[c] = hh;
+            l[c] = (float)(phi[c] - (double)hh);
+            v[c] = (float)(phi[c] - psi[c]);
+        }
+        for (int s = 0; s < 2; s++) {
+            if ((rc = upload2d(e, G.hi[s], G.ldy, h.data(), nx, ny, 4))) return rc;
+            if ((rc = upload2d(e, G.lo[s], G.ldy, l.data(), nx, ny, 4))) return rc;
+            if ((rc = upload2d(e, G.v[s], G.ldy, v.data(), nx, ny, 4))) return rc;
+        }
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+        G.exact = false;
+    }
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    G.state = true;
+    return WV_OK;
+}
+
+int get_state(Engine *e, int k, double *phi, double *psi, double *tmp)
+{
+    if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
+    Grid &G = e->g[k];
+    if (!G.state) return fail(e, WV_ESTATE, "no state set");
+    const int nx = G.p.nx, ny = G.p.ny;
+    HIPCHK(e, hipSetDevice(e->device));
+    int rc;
+    if (e->form == WV_FORM_FP64) {
+        if (phi && (rc = download2d(e, phi, G.u[e->role[0]], G.ldy, nx, ny, 8))) return rc;
+        if (psi && (rc = download2d(e, psi, G.u[e->role[1]], G.ldy, nx, ny, 8))) return rc;
+        if (tmp && (rc = download2d(e, tmp, G.u[e->role[2]], G.ldy, nx, ny, 8))) return rc;
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+    } else {
+        if (tmp) return fail(e, WV_EUNSUPPORTED, "the split form keeps one time level and a velocity; no third buffer");
+        const size_t n = (size_t)nx * ny;
+        std::vector<float> h(n), l(n), v(n);
+        const int s = e->scur;
+        if ((rc = download2d(e, h.data(), G.hi[s], G.ldy, nx, ny, 4))) return rc;
+        if ((rc = download2d(e, l.data(), G.lo[s], G.ldy, nx, ny, 4))) return rc;
+        if ((rc = download2d(e, v.data(), G.v[s], G.ldy, nx, ny, 4))) return rc;
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+        for (size_t c = 0; c < n; c++) {
+            const double u = (double)h[c] + (double)l[c];
+            if (phi) phi[c] = u;
+            if (psi) psi[c] = u - (double)v[c];
+        }
+    }
+    return WV_OK;
+}
+
+int set_sources(Engine *e, int k, int n, const int *si, const int *sj, const double *amp, long amp_len)
+{
+    if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
+    Grid &G = e->g[k];
+    if (!G.tables) return fail(e, WV_ESTATE, "set the tables before the sources");
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    for (int **q : {&G.src_ptr, &G.src_col, &G.src_id})
+        if (*q) { (void)hipFree(*q); *q = nullptr; }
+    if (G.amp) { (void)hipFree(G.amp); G.amp = nullptr; }
+    G.amp_len = 0;
+    if (n < 0) return fail(e, WV_EINVAL, "negative source count");
+    if (n > 0) {
+        if (!si || !sj || !amp || amp_len <= 0) return fail(e, WV_EINVAL, "source arrays missing");
+        for (int q = 0; q < n; q++) {
+            if (si[q] < 0 || si[q] >= G.p.nx || sj[q] < 0 || sj[q] >= G.p.ny)
+                return fail(e, WV_EINVAL, "source %d at (%d,%d) is outside the grid", q, si[q], sj[q]);
+            if (!bit(G.h_act, G.ldw, si[q], sj[q]))
+                return fail(e, WV_EINVAL, "source %d at (%d,%d) is on a cell the step never writes", q, si[q], sj[q]);
+        }
+        std::vector<int> order(n);
+        for (int q = 0; q < n; q++) order[q] = q;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return si[a] < si[b]; });
+        std::vector<int> ptr(G.p.nx + 1, 0), col(n), id(n);
+        for (int q = 0; q < n; q++) ptr[si[q] + 1]++;
+        for (int i = 0; i < G.p.nx; i++) ptr[i + 1] += ptr[i];
+        for (int q = 0; q < n; q++) { col[q] = sj[order[q]]; id[q] = order[q]; }
+        HIPCHK(e, hipMalloc(&G.src_ptr, sizeof(int) * ptr.size()));
+        HIPCHK(e, hipMalloc(&G.src_col, sizeof(int) * n));
+        HIPCHK(e, hipMalloc(&G.src_id, sizeof(int) * n));
+        HIPCHK(e, hipMalloc(&G.amp, sizeof(double) * (size_t)n * amp_len));
+        HIPCHK(e, hipMemcpy(G.src_ptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice));
+        HIPCHK(e, hipMemcpy(G.src_col, col.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+        HIPCHK(e, hipMemcpy(G.src_id, id.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+        HIPCHK(e, hipMemcpy(G.amp, amp, sizeof(double) * (size_t)n * amp_len, hipMemcpyHostToDevice));
+        G.amp_len = amp_len;
+    }
+    HIPCHK(e, hipMemset(e->d_gstep, 0, sizeof(long)));
+    drop_graphs(e);
+    return sync_inst(e);
+}
+
+bool any_exact(const Engine *e)
+{
+    for (const auto &G : e->g)
+        if (G.exact) return true;
+    return false;
+}
+
+int ready(Engine *e)
+{
+    for (int k = 0; k < e->n; k++) {
+        if (!e->g[k].tables) return fail(e, WV_ESTATE, "grid %d has no tables", k);
+        if (!e->g[k].state) return fail(e, WV_ESTATE, "grid %d has no state", k);
+    }
+    return WV_OK;
+}
+
+int max_depth(const Engine *e, bool force_k1)
+{
+    return (force_k1 || (e->form == WV_FORM_FP64 && any_exact(e))) ? 1 : depth(e);
+}
+
+// depth of the next launch when `left` steps remain
+int next_depth(int kmax, long left)
+{
+    int k = (int)std::min<long>(kmax, left);
+    while (!wv_have_depth(k)) k--;
+    return k;
+}
+
+// host-side rotation after one launch of depth k
+void advance_roles(Engine *e, int k)
+{
+    if (e->form == WV_FORM_FP64) {
+        const int r0 = e->role[0], r1 = e->role[1], r2 = e->role[2], r3 = e->role[3];
+        if (k == 1) { e->role[0] = r2; e->role[1] = r0; e->role[2] = r1; e->role[3] = r3; }
+        else { e->role[0] = r2; e->role[1] = r3; e->role[2] = r0; e->role[3] = r1; }
+    } else {
+        e->scur = 1 - e->scur;
+    }
+}
+
+// make every tile table a step sequence needs (no allocation during capture)
+int prepare_tiles(Engine *e, long nsteps, int kmax)
+{
+    long done = 0;
+    while (done < nsteps) {
+        const int k = next_depth(kmax, nsteps - done);
+        WvTile *t;
+        int nt, rc;
+        if ((rc = get_tiles(e, k, &t, &nt))) return rc;
+        done += k;
+        if (k == kmax) done += (nsteps - done) / kmax * kmax;  // the rest of the full blocks
+    }
+    return WV_OK;
+}
+
+// enqueue nsteps steps; force_k1 keeps one step per launch (drop-in forms)
+int enqueue_steps(Engine *e, long nsteps, bool force_k1, long *launches)
+{
+    long done = 0, nl = 0;
+    const int kmax = max_depth(e, force_k1);
+    while (done < nsteps) {
+        const int k = next_depth(kmax, nsteps - done);
+        WvTile *t;
+        int nt, rc;
+        if ((rc = get_tiles(e, k, &t, &nt))) return rc;
+        int lr;
+        if (e->form == WV_FORM_FP64) {
+            lr = wv_launch_fp64(k, e->stencil, e->d_inst, t, nt, e->d_gstep, done, e->role[0], e->role[1],
+                                e->role[2], e->role[3], e->stream);
+        } else {
+            lr = wv_launch_split(k, e->stencil, e->d_inst, t, nt, e->d_gstep, done, e->scur, 1 - e->scur,
+                                 e->stream);
+        }
+        advance_roles(e, k);
+        if (lr) return fail(e, WV_EHIP, "kernel launch failed: %s", hipGetErrorString((hipError_t)lr));
+        done += k;
+        nl++;
+    }
+    int lr = wv_launch_bump(e->d_gstep, nsteps, e->stream);
+    if (lr) return fail(e, WV_EHIP, "kernel launch failed: %s", hipGetErrorString((hipError_t)lr));
+    e->steps_done += nsteps;
+    if (launches) *launches = nl;
+    return WV_OK;
+}
+
+long rot_key(const Engine *e)
+{
+    return e->form == WV_FORM_FP64 ? e->role[0] + 4 * e->role[1] + 16 * e->role[2] + 64 * e->role[3] : e->scur;
+}
+
+int engine_step(Engine *e, long nsteps, bool force_k1)
+{
+    if (nsteps < 0) return fail(e, WV_EINVAL, "negative step count");
+    int rc = ready(e);
+    if (rc) return rc;
+    HIPCHK(e, hipSetDevice(e->device));
+    if (nsteps == 0) return WV_OK;
+    if (!e->use_graph || force_k1) {
+        rc = enqueue_steps(e, nsteps, force_k1, &e->last_launches);
+        return rc;
+    }
+    // batch: capture the launch sequence once per (nsteps, rotation) and replay
+    const int kmax = max_depth(e, false);
+    if ((rc = prepare_tiles(e, nsteps, kmax))) return rc;
+    const auto key = std::make_pair(nsteps, rot_key(e));
+    auto it = e->graphs.find(key);
+    int role0[4];
+    memcpy(role0, e->role, sizeof role0);
+    const int s0 = e->scur;
+    const long sd0 = e->steps_done;
+    if (it == e->graphs.end()) {
+        hipGraph_t graph;
+        HIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        long nl = 0;
+        rc = enqueue_steps(e, nsteps, false, &nl);
+        hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+        // capture only recorded the work: rewind the bookkeeping
+        memcpy(e->role, role0, sizeof role0);
+        e->scur = s0;
+        e->steps_done = sd0;
+        if (rc) return rc;
+        if (ce != hipSuccess) return fail(e, WV_EHIP, "graph capture failed: %s", hipGetErrorString(ce));
+        hipGraphExec_t ex;
+        hipError_t ie = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) return fail(e, WV_EHIP, "graph instantiate failed: %s", hipGetErrorString(ie));
+        it = e->graphs.emplace(key, ex).first;
+    }
+    HIPCHK(e, hipGraphLaunch(it->second, e->stream));
+    // advance the host-side rotation exactly as the captured sequence does
+    long done = 0, nl = 0;
+    while (done < nsteps) {
+        const int k = next_depth(kmax, nsteps - done);
+        advance_roles(e, k);
+        done += k;
+        nl++;
+    }
+    e->steps_done += nsteps;
+    e->last_launches = nl;
+    return WV_OK;
+}
+
+int engine_energy(Engine *e, double *out)
+{
+    int rc = ready(e);
+    if (rc) return rc;
+    if (!out) return fail(e, WV_EINVAL, "output pointer is NULL");
+    HIPCHK(e, hipSetDevice(e->device));
+    const bool split = e->form == WV_FORM_F32SPLIT;
+    int lr = wv_launch_energy(split, e->d_inst, e->d_erows, e->nrows_total, e->d_partial, e->d_row0, e->n,
+                              e->d_eout, split ? e->scur : e->role[0], split ? 0 : e->role[1], e->stream);
+    if (lr) return fail(e, WV_EHIP, "energy launch failed: %s", hipGetErrorString((hipError_t)lr));
+    HIPCHK(e, hipMemcpyAsync(out, e->d_eout, sizeof(double) * e->n, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return WV_OK;
+}
+
+int engine_step_timed(Engine *e, long nsteps, double *ms, long *launches)
+{
+    int rc = ready(e);
+    if (rc) return rc;
+    HIPCHK(e, hipSetDevice(e->device));
+    hipEvent_t a, b;
+    HIPCHK(e, hipEventCreate(&a));
+    HIPCHK(e, hipEventCreate(&b));
+    HIPCHK(e, hipEventRecord(a, e->stream));
+    rc = engine_step(e, nsteps, false);
+    if (rc) { (void)hipEventDestroy(a); (void)hipEventDestroy(b); return rc; }
+    HIPCHK(e, hipEventRecord(b, e->stream));
+    HIPCHK(e, hipEventSynchronize(b));
+    float t = 0;
+    HIPCHK(e, hipEventElapsedTime(&t, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (ms) *ms = t;
+    if (launches) *launches = e->last_launches;
+    return WV_OK;
+}
+
+int engine_set_tiling(Engine *e, int k, int L)
+{
+    if (k < 0 || k > 8 || (k && !wv_have_depth(k))) return fail(e, WV_EINVAL, "tb_depth must be 0 or one of 1,2,3,4,6,8");
+    if (L < 0) return fail(e, WV_EINVAL, "rows_per_tile must be >= 0");
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    e->K = k;
+    e->L = L;
+    drop_tiles(e);
+    return sync_inst(e);
+}
+
+}  // namespace
+
+struct wv_ctx {
+    Engine *e;
+};
+struct wv_batch {
+    Engine *e;
+};
+
+static const char *no_ctx = "no context";
+
+extern "C" {
+
+void wv_params_default(wv_params *p, int nx, int ny)
+{
+    if (!p) return;
+    memset(p, 0, sizeof *p);
+    p->nx = nx;
+    p->ny = ny;
+    p->stencil = 5;
+    p->form = WV_FORM_FP64;
+    p->tb_shift = nx;
+    p->e_scale = 1.0;
+}
+
+int wv_create(wv_ctx **out, const wv_params *p)
+{
+    if (!out) return WV_EINVAL;
+    *out = new wv_ctx{nullptr};
+    return engine_create(&(*out)->e, 1, p);
+}
+
+void wv_destroy(wv_ctx *c)
+{
+    if (!c) return;
+    engine_destroy(c->e);
+    delete c;
+}
+
+const char *wv_last_error(const wv_ctx *c) { return c && c->e ? c->e->err.c_str() : no_ctx; }
+
+int wv_set_tables(wv_ctx *c, const short *xy_in, const double *tcc, const double *tc, const double *tgamma)
+{
+    if (!c) return WV_EINVAL;
+    return set_tables(c->e, 0, xy_in, tcc, tc, tgamma);
+}
+
+int wv_set_state(wv_ctx *c, const double *phi, const double *psi)
+{
+    if (!c) return WV_EINVAL;
+    return set_state(c->e, 0, phi, psi, nullptr);
+}
+
+int wv_set_state3(wv_ctx *c, const double *phi, const double *psi, const double *tmp)
+{
+    if (!c) return WV_EINVAL;
+    if (!tmp) return fail(c->e, WV_EINVAL, "tmp is NULL");
+    if (c->e->form != WV_FORM_FP64) return fail(c->e, WV_EUNSUPPORTED, "three-buffer state needs the FP64 form");
+    return set_state(c->e, 0, phi, psi, tmp);
+}
+
+int wv_get_state(wv_ctx *c, double *phi, double *psi)
+{
+    if (!c) return WV_EINVAL;
+    return get_state(c->e, 0, phi, psi, nullptr);
+}
+
+int wv_get_state3(wv_ctx *c, double *phi, double *psi, double *tmp)
+{
+    if (!c) return WV_EINVAL;
+    if (c->e->form != WV_FORM_FP64) return fail(c->e, WV_EUNSUPPORTED, "three-buffer state needs the FP64 form");
+    return get_state(c->e, 0, phi, psi, tmp);
+}
+
+int wv_set_sources(wv_ctx *c, int n, const int *i, const int *j, const double *amp, long amp_len)
+{
+    if (!c) return WV_EINVAL;
+    return set_sources(c->e, 0, n, i, j, amp, amp_len);
+}
+
+int wv_step(wv_ctx *c, long nsteps)
+{
+    if (!c) return WV_EINVAL;
+    return engine_step(c->e, nsteps, false);
+}
+
+int wv_step_timed(wv_ctx *c, long nsteps, double *ms, long *launches)
+{
+    if (!c) return WV_EINVAL;
+    return engine_step_timed(c->e, nsteps, ms, launches);
+}
+
+int wv_sync(wv_ctx *c)
+{
+    if (!c) return WV_EINVAL;
+    Engine *e = c->e;
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return WV_OK;
+}
+
+long wv_steps_done(const wv_ctx *c) { return c ? c->e->steps_done : -1; }
+
+int wv_energy(wv_ctx *c, double *e)
+{
+    if (!c) return WV_EINVAL;
+    return engine_energy(c->e, e);
+}
+
+int wv_get_tiling(const wv_ctx *c, int *tb_depth, int *rows_per_tile, int *tile_cols)
+{
+    if (!c) return WV_EINVAL;
+    const Engine *e = c->e;
+    if (tb_depth) *tb_depth = depth(e);
+    if (rows_per_tile) *rows_per_tile = rows(e);
+    if (tile_cols) *tile_cols = wv_tile_cols(depth(e));
+    return WV_OK;
+}
+
+int wv_set_tiling(wv_ctx *c, int tb_depth, int rows_per_tile)
+{
+    if (!c) return WV_EINVAL;
+    return engine_set_tiling(c->e, tb_depth, rows_per_tile);
+}
+
+int wv_autotune(wv_ctx *c, long steps_per_trial, double *best_gcells);
+
+// ---- drop-in forms ----------------------------------------------------------
+
+static int dropin_common(wv_ctx *c, int nsteps, double *const *a, double *const *b, double *const *t,
+                         short *const *m, double *const *tcc, double *const *tc, double *const *tg,
+                         bool rowptr, bool three)
+{
+    if (!c) return WV_EINVAL;
+    Engine *e = c->e;
+    if (e->form != WV_FORM_FP64) return fail(e, WV_EUNSUPPORTED, "the drop-in entry points use the FP64 form");
+    const Grid &G = e->g[0];
+    const int nx = G.p.nx, ny = G.p.ny;
+    const size_t n = (size_t)nx * ny;
+    if (!a || !b || !t || !m || !tcc || !tc || !tg) return fail(e, WV_EINVAL, "NULL argument");
+    const double *pa, *pb, *pt;
+    if (rowptr) {
+        for (int k = 0; k < 3; k++) e->st[k].resize(n);
+        for (int i = 0; i < nx; i++) {
+            memcpy(&e->st[0][(size_t)i * ny], a[i], ny * sizeof(double));
+            memcpy(&e->st[1][(size_t)i * ny], b[i], ny * sizeof(double));
+            memcpy(&e->st[2][(size_t)i * ny], t[i], ny * sizeof(double));
+        }
+        pa = e->st[0].data(); pb = e->st[1].data(); pt = e->st[2].data();
+    } else {
+        pa = (const double *)a; pb = (const double *)b; pt = (const double *)t;
+    }
+    int rc;
+    if (!e->dropin_tables || !(G.p.flags & WV_FLAG_TABLES_FIXED)) {
+        if (rowptr) {
+            e->stm.resize(n);
+            for (int k = 0; k < 3; k++) e->stt[k].resize(n);
+            for (int i = 0; i < nx; i++) {
+                memcpy(&e->stm[(size_t)i * ny], m[i], ny * sizeof(short));
+                memcpy(&e->stt[0][(size_t)i * ny], tcc[i], ny * sizeof(double));
+                memcpy(&e->stt[1][(size_t)i * ny], tc[i], ny * sizeof(double));
+                memcpy(&e->stt[2][(size_t)i * ny], tg[i], ny * sizeof(double));
+            }
+            rc = set_tables(e, 0, e->stm.data(), e->stt[0].data(), e->stt[1].data(), e->stt[2].data());
+        } else {
+            rc = set_tables(e, 0, (const short *)m, (const double *)tcc, (const double *)tc, (const double *)tg);
+        }
+        if (rc) return rc;
+        e->dropin_tables = true;
+    }
+    // evolve_wave_half(phi_in, psi_in, phi_out): the three rotation buffers
+    // are (phi_in, psi_in, phi_out); evolve_wave(phi, psi, tmp) likewise
+    if ((rc = set_state(e, 0, pa, pb, pt))) return rc;
+    if ((rc = engine_step(e, nsteps, true))) return rc;
+    if (rowptr) {
+        double *o[3] = {e->st[0].data(), e->st[1].data(), e->st[2].data()};
+        if (three) {
+            // after 3 steps the roles are back: phi, psi, tmp = u^{n+3}, u^{n+2}, u^{n+1}
+            if ((rc = get_state(e, 0, o[0], o[1], o[2]))) return rc;
+            for (int i = 0; i < nx; i++) {
+                memcpy(a[i], &e->st[0][(size_t)i * ny], ny * sizeof(double));
+                memcpy(b[i], &e->st[1][(size_t)i * ny], ny * sizeof(double));
+                memcpy(t[i], &e->st[2][(size_t)i * ny], ny * sizeof(double));
+            }
+        } else {
+            // one step: the new phi is the caller's phi_out
+            if ((rc = get_state(e, 0, o[2], nullptr, nullptr))) return rc;
+            for (int i = 0; i < nx; i++) memcpy(t[i], &e->st[2][(size_t)i * ny], ny * sizeof(double));
+        }
+    } else {
+        if (three) rc = get_state(e, 0, (double *)a, (double *)b, (double *)t);
+        else rc = get_state(e, 0, (double *)t, nullptr, nullptr);
+        if (rc) return rc;
+    }
+    return WV_OK;
+}
+
+int wv_evolve_wave_half_rows(wv_ctx *c, double *const *phi_in, double *const *psi_in, double *const *phi_out,
+                             short *const *xy_in, double *const *tcc, double *const *tc,
+                             double *const *tgamma)
+{
+    return dropin_common(c, 1, phi_in, psi_in, phi_out, xy_in, tcc, tc, tgamma, true, false);
+}
+
+int wv_evolve_wave_rows(wv_ctx *c, double *const *phi, double *const *psi, double *const *tmp,
+                        short *const *xy_in, double *const *tcc_table, double *const *tc_table,
+                        double *const *tgamma_table)
+{
+    return dropin_common(c, 3, phi, psi, tmp, xy_in, tcc_table, tc_table, tgamma_table, true, true);
+}
+
+int wv_evolve_wave_half_flat(wv_ctx *c, double *phi_in, double *psi_in, double *phi_out, short *xy_in,
+                             double *tc, double *tcc, double *tgamma)
+{
+    return dropin_common(c, 1, (double *const *)phi_in, (double *const *)psi_in, (double *const *)phi_out,
+                         (short *const *)xy_in, (double *const *)tcc, (double *const *)tc,
+                         (double *const *)tgamma, false, false);
+}
+
+int wv_evolve_wave_flat(wv_ctx *c, double *phi, double *psi, double *tmp, short *xy_in, double *tc,
+                        double *tcc, double *tgamma)
+{
+    return dropin_common(c, 3, (double *const *)phi, (double *const *)psi, (double *const *)tmp,
+                         (short *const *)xy_in, (double *const *)tcc, (double *const *)tc,
+                         (double *const *)tgamma, false, true);
+}
+
+// ---- batches ----------------------------------------------------------------
+
+int wv_batch_create(wv_batch **out, int n, const wv_params *ps)
+{
+    if (!out) return WV_EINVAL;
+    *out = new wv_batch{nullptr};
+    int rc = engine_create(&(*out)->e, n, ps);
+    if (!rc) (*out)->e->use_graph = true;
+    return rc;
+}
+
+void wv_batch_destroy(wv_batch *b)
+{
+    if (!b) return;
+    engine_destroy(b->e);
+    delete b;
+}
+
+const char *wv_batch_last_error(const wv_batch *b) { return b && b->e ? b->e->err.c_str() : no_ctx; }
+
+int wv_batch_set_tables(wv_batch *b, int k, const short *xy_in, const double *tcc, const double *tc,
+                        const double *tgamma)
+{
+    if (!b) return WV_EINVAL;
+    return set_tables(b->e, k, xy_in, tcc, tc, tgamma);
+}
+
+int wv_batch_set_state(wv_batch *b, int k, const double *phi, const double *psi)
+{
+    if (!b) return WV_EINVAL;
+    return set_state(b->e, k, phi, psi, nullptr);
+}
+
+int wv_batch_get_state(wv_batch *b, int k, double *phi, double *psi)
+{
+    if (!b) return WV_EINVAL;
+    return get_state(b->e, k, phi, psi, nullptr);
+}
+
+int wv_batch_step(wv_batch *b, long nsteps)
+{
+    if (!b) return WV_EINVAL;
+    return engine_step(b->e, nsteps, false);
+}
+
+int wv_batch_step_timed(wv_batch *b, long nsteps, double *ms, long *launches)
+{
+    if (!b) return WV_EINVAL;
+    return engine_step_timed(b->e, nsteps, ms, launches);
+}
+
+int wv_batch_energy(wv_batch *b, double *e)
+{
+    if (!b) return WV_EINVAL;
+    return engine_energy(b->e, e);
+}
+
+int wv_batch_set_tiling(wv_batch *b, int tb_depth, int rows_per_tile)
+{
+    if (!b) return WV_EINVAL;
+    return engine_set_tiling(b->e, tb_depth, rows_per_tile);
+}
+
+int wv_batch_get_tiling(const wv_batch *b, int *tb_depth, int *rows_per_tile, int *tile_cols)
+{
+    if (!b) return WV_EINVAL;
+    const Engine *e = b->e;
+    if (tb_depth) *tb_depth = depth(e);
+    if (rows_per_tile) *rows_per_tile = rows(e);
+    if (tile_cols) *tile_cols = wv_tile_cols(depth(e));
+    return WV_OK;
+}
+
+}  // extern "C"
+
+// ---- autotuner ----------------------------------------------------------------
+// Times every (depth, rows per tile) candidate on the current state and keeps
+// the fastest.  The state is saved to spare device buffers first and restored
+// afterwards, so tuning does not advance the simulation.
+extern "C" int wv_autotune(wv_ctx *c, long steps_per_trial, double *best_gcells)
+{
+    if (!c) return WV_EINVAL;
+    Engine *e = c->e;
+    int rc = ready(e);
+    if (rc) return rc;
+    if (steps_per_trial < 1) return fail(e, WV_EINVAL, "steps_per_trial must be >= 1");
+    Grid &G = e->g[0];
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    const size_t cells = (size_t)G.p.nx * G.ldy;
+    // save
+    std::vector<void *> save;
+    std::vector<std::pair<void *, size_t>> what;
+    if (e->form == WV_FORM_FP64) {
+        for (int k = 0; k < 4; k++) what.push_back({G.u[k], cells * 8});
+    } else {
+        for (int k = 0; k < 2; k++) {
+            what.push_back({G.hi[k], cells * 4});
+            what.push_back({G.lo[k], cells * 4});
+            what.push_back({G.v[k], cells * 4});
+        }
+    }
+    for (auto &w : what) {
+        void *p = nullptr;
+        HIPCHK(e, hipMalloc(&p, w.second));
+        HIPCHK(e, hipMemcpy(p, w.first, w.second, hipMemcpyDeviceToDevice));
+        save.push_back(p);
+    }
+    int role0[4];
+    memcpy(role0, e->role, sizeof role0);
+    const int s0 = e->scur;
+    const long sd0 = e->steps_done;
+    const int K0 = e->K, L0 = e->L;
+    long gstep0 = 0;
+    HIPCHK(e, hipMemcpy(&gstep0, e->d_gstep, sizeof(long), hipMemcpyDeviceToHost));
+    const int depths[] = {1, 2, 3, 4, 6, 8};
+    const int rowsc[] = {16, 32, 64, 128, 256};
+    double best = -1;
+    int bk = depth(e), bl = rows(e);
+    const bool exact = e->form == WV_FORM_FP64 && any_exact(e);
+    for (int k : depths) {
+        if (exact && k > 1) continue;
+        for (int L : rowsc) {
+            if ((rc = engine_set_tiling(e, k, L))) break;
+            double ms = 0;
+            long nl = 0;
+            if ((rc = engine_step_timed(e, std::max<long>(1, steps_per_trial / 4), &ms, &nl))) break;  // warm
+            if ((rc = engine_step_timed(e, steps_per_trial, &ms, &nl))) break;
+            const double gc = (double)G.p.nx * G.p.ny * steps_per_trial / (ms * 1e6);
+            if (gc > best) { best = gc; bk = k; bl = L; }
+        }
+        if (rc) break;
+    }
+    // restore
+    for (size_t q = 0; q < what.size(); q++) {
+        (void)hipMemcpy(what[q].first, save[q], what[q].second, hipMemcpyDeviceToDevice);
+        (void)hipFree(save[q]);
+    }
+    (void)hipMemcpy(e->d_gstep, &gstep0, sizeof(long), hipMemcpyHostToDevice);
+    memcpy(e->role, role0, sizeof role0);
+    e->scur = s0;
+    e->steps_done = sd0;
+    if (rc) {
+        engine_set_tiling(e, K0, L0);
+        return rc;
+    }
+    if ((rc = engine_set_tiling(e, bk, bl))) return rc;
+    if (best_gcells) *best_gcells = best;
+    return WV_OK;
+}

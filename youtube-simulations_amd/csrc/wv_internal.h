@@ -1,0 +1,72 @@
+// wv_internal.h -- device-side data layout shared by the kernels and the host
+// code of libwavehip.so.
+//
+// Layout in HBM, per grid ("instance"):
+//   state      FP64 form: four NX x ldy arrays of double (three rotate like
+//              evolve_wave's phi/psi/tmp, the fourth lets a fused launch write
+//              two time levels out of place).  Split form: two ping-pong sets
+//              of three NX x ldy float arrays (hi, lo, v).  ldy = NY rounded
+//              up to 64 elements, so every row starts 512-byte aligned.
+//   act        1 bit per cell, uint64 words, ldw = ldy/64 words per row:
+//              cell is written by a step (interior: TWOSPEEDS || xy_in > 0,
+//              edge: TWOSPEEDS || xy_in != 0).
+//   eact       1 bit per cell: cell counts in the energy (xy_in != 0 || TWOSPEEDS).
+//   med        1 bit per cell: which of two (tcc, tgamma) pairs applies
+//              (coefficient mode WV_COEF_TWO only).
+//   tccf, tgf  per-cell coefficient tables (mode WV_COEF_FIELD only).
+//   tc         per-cell tc (read at edge cells only).
+//   sources    CSR by row: src_ptr[NX+1], src_col[], src_id[]; amplitudes
+//              amp[id*amp_len + g].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define WV_COEF_UNIFORM 0
+#define WV_COEF_TWO 1
+#define WV_COEF_FIELD 2
+
+// cells per lane along j in the stepping kernels (a wave covers 64*WV_C columns)
+#define WV_C 2
+
+struct WvInst {
+    // FP64 form: u[0..3]; split form: hi[2], lo[2], v[2]
+    double *u[4];
+    float *hi[2], *lo[2], *v[2];
+    const uint64_t *act, *eact, *med;
+    const double *tccf, *tgf, *tc;       // per-cell tables (FP64)
+    const float *tccf_hi, *tccf_lo, *tgf32;  // per-cell tables (split form)
+    const int *src_ptr, *src_col, *src_id;
+    const double *amp;
+    long amp_len;
+    int nx, ny, ldy, ldw;
+    int bc, osc_tb, tb_shift, coef;
+    int rows_per_tile, ncb;              // tiling of this instance
+    int tile0;                           // first tile index of this instance
+    double tcc0, tcc1, tg0, tg1;         // coefficient pairs (modes UNIFORM / TWO)
+    float tcch0, tccl0, tcch1, tccl1, tgf0, tgf1;
+    double kappa, ks, gs, kt, gt, es2;
+};
+
+// one tile of a stepping launch: instance, column block, row strip
+struct WvTile {
+    int inst, cb, sb, pad;
+};
+
+// number of output columns of one wave for a fused depth k
+static inline int wv_halo(int k) { return (k + WV_C - 1) / WV_C * WV_C; }
+static inline int wv_tile_cols(int k) { return 64 * WV_C - 2 * wv_halo(k); }
+
+// Launchers (wv_kernels.hip).  ia/ib: buffers holding u^n, u^{n-1}; oc/od:
+// buffers receiving u^{n+k}, u^{n+k-1} (od unused when k == 1).  Split form:
+// ia = input set, oc = output set.  g0 / gstep: amplitude index of the first
+// step = *gstep + g0 when gstep is non-null.
+int wv_launch_fp64(int k, int stencil, const WvInst *d_inst, const WvTile *d_tiles, int ntiles,
+                   const long *gstep, long g0, int ia, int ib, int oc, int od, hipStream_t s);
+int wv_launch_split(int k, int stencil, const WvInst *d_inst, const WvTile *d_tiles, int ntiles,
+                    const long *gstep, long g0, int ia, int oc, hipStream_t s);
+bool wv_have_depth(int k);
+// energy: rows table (inst,row) pairs; partial[nrows_total]; out[ninst]
+int wv_launch_energy(int split, const WvInst *d_inst, const int2 *d_rows, int nrows,
+                     double *d_partial, const int *d_row0, int ninst, double *d_out, int ia, int ib,
+                     hipStream_t s);
+int wv_launch_bump(long *gstep, long n, hipStream_t s);

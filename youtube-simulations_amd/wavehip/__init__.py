@@ -1,0 +1,367 @@
+"""wavehip -- Python mirror of the C ABI in include/wavehip.h (libwavehip.so).
+
+The stepper itself is the HIP library; this module only moves numpy arrays
+across the ABI with ctypes.  There is no CPU fallback: if libwavehip.so is
+missing, or no GPU is visible, calls raise.
+
+Names follow the reference (nilsberglund-orleans/YouTube-simulations):
+evolve_wave_half / evolve_wave (wave_billiard.c:302-631, wave_3d.c:600-1031),
+phi = u^n, psi = u^{n-1}, tmp the third rotation buffer, xy_in the domain
+mask, tcc / tc / tgamma the tables of init_ior_2d (sub_wave.c:12164).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwavehip.so")
+
+WV_OK, WV_EINVAL, WV_EHIP, WV_ESTATE, WV_EUNSUPPORTED, WV_ENOMEM = 0, -1, -2, -3, -4, -5
+BC_DIRICHLET, BC_PERIODIC, BC_ABSORBING, BC_VPER_HABS, BC_ABS_REFLECT = 0, 1, 2, 3, 4
+FORM_FP64, FORM_F32SPLIT = 0, 1
+FLAG_TABLES_FIXED = 1
+
+# every symbol include/wavehip.h declares
+EXPORTS = [
+    "wv_params_default", "wv_create", "wv_destroy", "wv_last_error", "wv_set_tables",
+    "wv_set_state", "wv_set_state3", "wv_get_state", "wv_get_state3", "wv_set_sources",
+    "wv_step", "wv_step_timed", "wv_sync", "wv_steps_done", "wv_energy", "wv_get_tiling",
+    "wv_set_tiling", "wv_autotune", "wv_evolve_wave_half_rows", "wv_evolve_wave_rows",
+    "wv_evolve_wave_half_flat", "wv_evolve_wave_flat", "wv_batch_create", "wv_batch_destroy",
+    "wv_batch_last_error", "wv_batch_set_tables", "wv_batch_set_state", "wv_batch_get_state",
+    "wv_batch_step", "wv_batch_step_timed", "wv_batch_energy", "wv_batch_set_tiling",
+    "wv_batch_get_tiling",
+]
+
+
+class WvParams(ctypes.Structure):
+    """struct wv_params of include/wavehip.h"""
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("bcond", ctypes.c_int),
+        ("twospeeds", ctypes.c_int), ("stencil", ctypes.c_int), ("form", ctypes.c_int),
+        ("oscillate_topbot", ctypes.c_int), ("tb_shift", ctypes.c_int),
+        ("kappa", ctypes.c_double), ("kappa_sides", ctypes.c_double),
+        ("gamma_sides", ctypes.c_double), ("kappa_topbot", ctypes.c_double),
+        ("gamma_topbot", ctypes.c_double), ("e_scale", ctypes.c_double),
+        ("tb_depth", ctypes.c_int), ("rows_per_tile", ctypes.c_int), ("device", ctypes.c_int),
+        ("flags", ctypes.c_int),
+    ]
+
+
+class WaveError(RuntimeError):
+    pass
+
+
+_D = ctypes.POINTER(ctypes.c_double)
+_S = ctypes.POINTER(ctypes.c_short)
+_I = ctypes.POINTER(ctypes.c_int)
+_DD = ctypes.POINTER(_D)
+_SS = ctypes.POINTER(_S)
+_lib = None
+
+
+def load():
+    """Load libwavehip.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise WaveError("libwavehip.so is not built at %s (run __graft_entry__.build())" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER(WvParams)
+    vp = ctypes.c_void_p
+    sig = {
+        "wv_params_default": (None, [P, ctypes.c_int, ctypes.c_int]),
+        "wv_create": (ctypes.c_int, [ctypes.POINTER(vp), P]),
+        "wv_destroy": (None, [vp]),
+        "wv_last_error": (ctypes.c_char_p, [vp]),
+        "wv_set_tables": (ctypes.c_int, [vp, _S, _D, _D, _D]),
+        "wv_set_state": (ctypes.c_int, [vp, _D, _D]),
+        "wv_set_state3": (ctypes.c_int, [vp, _D, _D, _D]),
+        "wv_get_state": (ctypes.c_int, [vp, _D, _D]),
+        "wv_get_state3": (ctypes.c_int, [vp, _D, _D, _D]),
+        "wv_set_sources": (ctypes.c_int, [vp, ctypes.c_int, _I, _I, _D, ctypes.c_long]),
+        "wv_step": (ctypes.c_int, [vp, ctypes.c_long]),
+        "wv_step_timed": (ctypes.c_int, [vp, ctypes.c_long, _D, ctypes.POINTER(ctypes.c_long)]),
+        "wv_sync": (ctypes.c_int, [vp]),
+        "wv_steps_done": (ctypes.c_long, [vp]),
+        "wv_energy": (ctypes.c_int, [vp, _D]),
+        "wv_get_tiling": (ctypes.c_int, [vp, _I, _I, _I]),
+        "wv_set_tiling": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
+        "wv_autotune": (ctypes.c_int, [vp, ctypes.c_long, _D]),
+        "wv_evolve_wave_half_rows": (ctypes.c_int, [vp, _DD, _DD, _DD, _SS, _DD, _DD, _DD]),
+        "wv_evolve_wave_rows": (ctypes.c_int, [vp, _DD, _DD, _DD, _SS, _DD, _DD, _DD]),
+        "wv_evolve_wave_half_flat": (ctypes.c_int, [vp, _D, _D, _D, _S, _D, _D, _D]),
+        "wv_evolve_wave_flat": (ctypes.c_int, [vp, _D, _D, _D, _S, _D, _D, _D]),
+        "wv_batch_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int, P]),
+        "wv_batch_destroy": (None, [vp]),
+        "wv_batch_last_error": (ctypes.c_char_p, [vp]),
+        "wv_batch_set_tables": (ctypes.c_int, [vp, ctypes.c_int, _S, _D, _D, _D]),
+        "wv_batch_set_state": (ctypes.c_int, [vp, ctypes.c_int, _D, _D]),
+        "wv_batch_get_state": (ctypes.c_int, [vp, ctypes.c_int, _D, _D]),
+        "wv_batch_step": (ctypes.c_int, [vp, ctypes.c_long]),
+        "wv_batch_step_timed": (ctypes.c_int, [vp, ctypes.c_long, _D, ctypes.POINTER(ctypes.c_long)]),
+        "wv_batch_energy": (ctypes.c_int, [vp, _D]),
+        "wv_batch_set_tiling": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
+        "wv_batch_get_tiling": (ctypes.c_int, [vp, _I, _I, _I]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def make_params(p):
+    """wv_params from a dict (the 'params' of a workload) with defaults."""
+    s = WvParams()
+    load().wv_params_default(ctypes.byref(s), int(p["nx"]), int(p["ny"]))
+    for k, v in p.items():
+        if hasattr(s, k) and not k.startswith("_"):
+            setattr(s, k, type(getattr(s, k))(v))
+    return s
+
+
+def _d(a):
+    return a.ctypes.data_as(_D)
+
+
+def _f64(a, shape):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.shape != shape:
+        raise ValueError("expected shape %s, got %s" % (shape, a.shape))
+    return a
+
+
+class Wave:
+    """One grid on the GPU (a wv_ctx)."""
+
+    def __init__(self, params, **overrides):
+        p = dict(params)
+        p.update(overrides)
+        self.p = make_params(p)
+        self.shape = (self.p.nx, self.p.ny)
+        self._L = load()
+        self._h = ctypes.c_void_p()
+        rc = self._L.wv_create(ctypes.byref(self._h), ctypes.byref(self.p))
+        if rc:
+            msg = self._L.wv_last_error(self._h).decode()
+            self._L.wv_destroy(self._h)
+            self._h = None
+            raise WaveError("wv_create: %d %s" % (rc, msg))
+        self._keep = []
+
+    def close(self):
+        if self._h:
+            self._L.wv_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc:
+            raise WaveError("%s: %d %s" % (what, rc, self._L.wv_last_error(self._h).decode()))
+
+    def set_tables(self, xy_in, tcc, tc, tgamma):
+        xy = np.ascontiguousarray(xy_in, dtype=np.int16)
+        a, b, c = (_f64(t, self.shape) for t in (tcc, tc, tgamma))
+        self._chk(self._L.wv_set_tables(self._h, xy.ctypes.data_as(_S), _d(a), _d(b), _d(c)),
+                  "wv_set_tables")
+
+    def set_state(self, phi, psi, tmp=None):
+        a, b = _f64(phi, self.shape), _f64(psi, self.shape)
+        if tmp is None:
+            self._chk(self._L.wv_set_state(self._h, _d(a), _d(b)), "wv_set_state")
+        else:
+            t = _f64(tmp, self.shape)
+            self._chk(self._L.wv_set_state3(self._h, _d(a), _d(b), _d(t)), "wv_set_state3")
+
+    def get_state(self, three=False):
+        a = np.empty(self.shape)
+        b = np.empty(self.shape)
+        if three:
+            t = np.empty(self.shape)
+            self._chk(self._L.wv_get_state3(self._h, _d(a), _d(b), _d(t)), "wv_get_state3")
+            return a, b, t
+        self._chk(self._L.wv_get_state(self._h, _d(a), _d(b)), "wv_get_state")
+        return a, b
+
+    def set_sources(self, sources):
+        if not sources:
+            self._chk(self._L.wv_set_sources(self._h, 0, None, None, None, 0), "wv_set_sources")
+            return
+        si = np.ascontiguousarray(sources["i"], np.int32)
+        sj = np.ascontiguousarray(sources["j"], np.int32)
+        amp = np.ascontiguousarray(sources["amp"], np.float64)
+        self._chk(self._L.wv_set_sources(self._h, len(si), si.ctypes.data_as(_I), sj.ctypes.data_as(_I),
+                                         _d(amp), amp.shape[1]), "wv_set_sources")
+
+    def load(self, w):
+        """tables, state and sources of a workload dict"""
+        self.set_tables(w["xy_in"], w["tcc"], w["tc"], w["tgamma"])
+        self.set_state(w["phi"], w["psi"])
+        if w.get("sources"):
+            self.set_sources(w["sources"])
+
+    def step(self, n):
+        self._chk(self._L.wv_step(self._h, int(n)), "wv_step")
+
+    def step_timed(self, n):
+        ms = ctypes.c_double()
+        nl = ctypes.c_long()
+        self._chk(self._L.wv_step_timed(self._h, int(n), ctypes.byref(ms), ctypes.byref(nl)),
+                  "wv_step_timed")
+        return ms.value, nl.value
+
+    def sync(self):
+        self._chk(self._L.wv_sync(self._h), "wv_sync")
+
+    def steps_done(self):
+        return self._L.wv_steps_done(self._h)
+
+    def energy(self):
+        e = ctypes.c_double()
+        self._chk(self._L.wv_energy(self._h, ctypes.byref(e)), "wv_energy")
+        return e.value
+
+    def tiling(self):
+        k, L, w = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._chk(self._L.wv_get_tiling(self._h, ctypes.byref(k), ctypes.byref(L), ctypes.byref(w)),
+                  "wv_get_tiling")
+        return k.value, L.value, w.value
+
+    def set_tiling(self, tb_depth, rows_per_tile):
+        self._chk(self._L.wv_set_tiling(self._h, int(tb_depth), int(rows_per_tile)), "wv_set_tiling")
+
+    def autotune(self, steps_per_trial=64):
+        g = ctypes.c_double()
+        self._chk(self._L.wv_autotune(self._h, int(steps_per_trial), ctypes.byref(g)), "wv_autotune")
+        return g.value
+
+    # ---- drop-in forms (reference signatures) --------------------------------
+    def evolve_wave_half_flat(self, phi_in, psi_in, phi_out, xy_in, tc, tcc, tgamma):
+        """evolve_wave_half of wave_3d.c:600 -- in place on phi_out."""
+        arrs = [phi_in, psi_in, phi_out, tc, tcc, tgamma]
+        for a in arrs:
+            assert a.dtype == np.float64 and a.flags.c_contiguous
+        self._chk(self._L.wv_evolve_wave_half_flat(self._h, _d(phi_in), _d(psi_in), _d(phi_out),
+                                                   xy_in.ctypes.data_as(_S), _d(tc), _d(tcc),
+                                                   _d(tgamma)), "wv_evolve_wave_half_flat")
+
+    def evolve_wave_flat(self, phi, psi, tmp, xy_in, tc, tcc, tgamma):
+        self._chk(self._L.wv_evolve_wave_flat(self._h, _d(phi), _d(psi), _d(tmp),
+                                              xy_in.ctypes.data_as(_S), _d(tc), _d(tcc), _d(tgamma)),
+                  "wv_evolve_wave_flat")
+
+    @staticmethod
+    def _rows(a, t):
+        """array of row pointers into a C-contiguous 2D array (double *a[NX])"""
+        n = a.shape[0]
+        base = a.ctypes.data
+        stride = a.strides[0]
+        arr = (ctypes.POINTER(t) * n)(*[ctypes.cast(base + i * stride, ctypes.POINTER(t)) for i in range(n)])
+        return arr
+
+    def evolve_wave_half_rows(self, phi_in, psi_in, phi_out, xy_in, tcc, tc, tgamma):
+        """evolve_wave_half of wave_billiard.c:302 on row pointers -- in place on phi_out."""
+        R = [self._rows(a, ctypes.c_double) for a in (phi_in, psi_in, phi_out)]
+        M = self._rows(xy_in, ctypes.c_short)
+        T = [self._rows(a, ctypes.c_double) for a in (tcc, tc, tgamma)]
+        self._chk(self._L.wv_evolve_wave_half_rows(self._h, R[0], R[1], R[2], M, T[0], T[1], T[2]),
+                  "wv_evolve_wave_half_rows")
+
+    def evolve_wave_rows(self, phi, psi, tmp, xy_in, tcc, tc, tgamma):
+        """evolve_wave of wave_billiard.c:613 on row pointers (three steps, in place)."""
+        R = [self._rows(a, ctypes.c_double) for a in (phi, psi, tmp)]
+        M = self._rows(xy_in, ctypes.c_short)
+        T = [self._rows(a, ctypes.c_double) for a in (tcc, tc, tgamma)]
+        self._chk(self._L.wv_evolve_wave_rows(self._h, R[0], R[1], R[2], M, T[0], T[1], T[2]),
+                  "wv_evolve_wave_rows")
+
+
+class Batch:
+    """Many independent grids stepped together (a wv_batch)."""
+
+    def __init__(self, params_list, **overrides):
+        ps = []
+        for p in params_list:
+            q = dict(p)
+            q.update(overrides)
+            ps.append(make_params(q))
+        self.n = len(ps)
+        self.shapes = [(p.nx, p.ny) for p in ps]
+        arr = (WvParams * self.n)(*ps)
+        self._L = load()
+        self._h = ctypes.c_void_p()
+        rc = self._L.wv_batch_create(ctypes.byref(self._h), self.n, arr)
+        if rc:
+            msg = self._L.wv_batch_last_error(self._h).decode()
+            self._L.wv_batch_destroy(self._h)
+            self._h = None
+            raise WaveError("wv_batch_create: %d %s" % (rc, msg))
+
+    def close(self):
+        if self._h:
+            self._L.wv_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc:
+            raise WaveError("%s: %d %s" % (what, rc, self._L.wv_batch_last_error(self._h).decode()))
+
+    def set_tables(self, k, xy_in, tcc, tc, tgamma):
+        xy = np.ascontiguousarray(xy_in, dtype=np.int16)
+        a, b, c = (_f64(t, self.shapes[k]) for t in (tcc, tc, tgamma))
+        self._chk(self._L.wv_batch_set_tables(self._h, k, xy.ctypes.data_as(_S), _d(a), _d(b), _d(c)),
+                  "wv_batch_set_tables")
+
+    def set_state(self, k, phi, psi):
+        a, b = _f64(phi, self.shapes[k]), _f64(psi, self.shapes[k])
+        self._chk(self._L.wv_batch_set_state(self._h, k, _d(a), _d(b)), "wv_batch_set_state")
+
+    def get_state(self, k):
+        a = np.empty(self.shapes[k])
+        b = np.empty(self.shapes[k])
+        self._chk(self._L.wv_batch_get_state(self._h, k, _d(a), _d(b)), "wv_batch_get_state")
+        return a, b
+
+    def load(self, k, w):
+        self.set_tables(k, w["xy_in"], w["tcc"], w["tc"], w["tgamma"])
+        self.set_state(k, w["phi"], w["psi"])
+
+    def step(self, n):
+        self._chk(self._L.wv_batch_step(self._h, int(n)), "wv_batch_step")
+
+    def step_timed(self, n):
+        ms = ctypes.c_double()
+        nl = ctypes.c_long()
+        self._chk(self._L.wv_batch_step_timed(self._h, int(n), ctypes.byref(ms), ctypes.byref(nl)),
+                  "wv_batch_step_timed")
+        return ms.value, nl.value
+
+    def energy(self):
+        e = np.empty(self.n)
+        self._chk(self._L.wv_batch_energy(self._h, _d(e)), "wv_batch_energy")
+        return e
+
+    def set_tiling(self, tb_depth, rows_per_tile):
+        self._chk(self._L.wv_batch_set_tiling(self._h, int(tb_depth), int(rows_per_tile)),
+                  "wv_batch_set_tiling")
+
+    def tiling(self):
+        k, L, w = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._chk(self._L.wv_batch_get_tiling(self._h, ctypes.byref(k), ctypes.byref(L), ctypes.byref(w)),
+                  "wv_batch_get_tiling")
+        return k.value, L.value, w.value
