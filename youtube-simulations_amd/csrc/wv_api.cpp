@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -63,6 +64,7 @@ struct Engine {
     std::vector<double> st[3], stt[3];
     std::vector<short> stm;
     bool dropin_tables = false;
+    bool xcd_order = true;  // WAVEHIP_TILE_ORDER=linear turns the XCD interleave off
 };
 
 int fail(Engine *e, int code, const char *fmt, ...)
@@ -188,8 +190,13 @@ void drop_tiles(Engine *e)
     drop_graphs(e);
 }
 
-// tile table of a launch of depth k: every instance, row strips of
-// rows(e) rows, column blocks of wv_tile_cols(k) columns
+// Tile table of a launch of depth k: every instance, row strips of rows(e)
+// rows, column blocks of wv_tile_cols(k) columns.  Workgroups are dealt to
+// the 8 XCDs round-robin by workgroup id, so the table interleaves strips:
+// ids 8m+x of one group of 8 strips all belong to strip x of the group, and
+// every column tile of a strip runs on the same XCD -- the halo columns two
+// neighbouring tiles both read, and the cache lines their outputs share,
+// stay in one L2.  Tiles with inst = -1 only pad the interleave.
 int get_tiles(Engine *e, int k, WvTile **out, int *count)
 {
     auto it = e->tiles.find(k);
@@ -199,12 +206,30 @@ int get_tiles(Engine *e, int k, WvTile **out, int *count)
         return WV_OK;
     }
     const int wout = wv_tile_cols(k), L = rows(e);
-    std::vector<WvTile> h;
+    struct Strip { int inst, sb, ncb; };
+    std::vector<Strip> strips;
     for (int q = 0; q < e->n; q++) {
         const Grid &G = e->g[q];
         const int ncb = (G.p.ny + wout - 1) / wout, nsb = (G.p.nx + L - 1) / L;
-        for (int sb = 0; sb < nsb; sb++)
-            for (int cb = 0; cb < ncb; cb++) h.push_back(WvTile{q, cb, sb, 0});
+        for (int sb = 0; sb < nsb; sb++) strips.push_back(Strip{q, sb, ncb});
+    }
+    std::vector<WvTile> h;
+    if (e->xcd_order) {
+        const int X = 8;
+        for (size_t g0 = 0; g0 < strips.size(); g0 += X) {
+            const size_t g1 = std::min(strips.size(), g0 + X);
+            int maxcb = 0;
+            for (size_t q = g0; q < g1; q++) maxcb = std::max(maxcb, strips[q].ncb);
+            for (int cb = 0; cb < maxcb; cb++)
+                for (size_t q = g0; q < g0 + X; q++) {
+                    if (q < g1 && cb < strips[q].ncb) h.push_back(WvTile{strips[q].inst, cb, strips[q].sb, 0});
+                    else if (q < g1 || g1 == g0 + X) h.push_back(WvTile{-1, 0, 0, 0});
+                }
+        }
+        while (!h.empty() && h.back().inst < 0) h.pop_back();
+    } else {
+        for (const Strip &st : strips)
+            for (int cb = 0; cb < st.ncb; cb++) h.push_back(WvTile{st.inst, cb, st.sb, 0});
     }
     WvTile *d = nullptr;
     HIPCHK(e, hipMalloc(&d, sizeof(WvTile) * h.size()));
@@ -232,6 +257,7 @@ int engine_create(Engine **out, int n, const wv_params *ps)
     e->stencil = ps[0].stencil;
     e->K = ps[0].tb_depth;
     e->L = ps[0].rows_per_tile;
+    if (const char *o = getenv("WAVEHIP_TILE_ORDER")) e->xcd_order = strcmp(o, "linear") != 0;
     int ndev = 0;
     HIPCHK(e, hipGetDeviceCount(&ndev));
     if (e->device < 0 || e->device >= ndev) return fail(e, WV_EINVAL, "device %d not present (%d devices)", e->device, ndev);

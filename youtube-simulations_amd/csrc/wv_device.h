@@ -1,0 +1,281 @@
+// wv_device.h -- device helpers shared by the stepping and energy kernels.
+//
+// Arithmetic follows the reference expression by expression (the files are
+// compiled with -ffp-contract=off, so nothing is fused into an FMA that the
+// reference does not have).
+#pragma once
+#include <type_traits>
+
+#include "wv_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace wv {
+
+// global-address-space views of pointers read from the instance table (a
+// plain pointer loaded from memory would be accessed with flat instructions)
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ gptr<const T> gc(const T *p) { return (gptr<const T>)p; }
+template <typename T>
+__device__ __forceinline__ gptr<T> gm(T *p) { return (gptr<T>)p; }
+
+// native two-element vectors (usable in the global address space)
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec2 __attribute__((ext_vector_type(2)));
+
+constexpr double W9 = 1.0 / 6.0;  // 9-point weight: ((4*s4 + sd) - 20x) * W9
+
+__device__ __forceinline__ int map_idx(int r, int n, bool wrap)
+{
+    if (wrap) {
+        r %= n;
+        return r < 0 ? r + n : r;
+    }
+    return r < 0 ? 0 : (r >= n ? n - 1 : r);
+}
+
+// value held by lane+1 / lane-1 of the wave: DPP wave_shl:1 / wave_shr:1
+// (lanes 63 / 0, which have no source lane, keep their own value)
+__device__ __forceinline__ int dpp_next(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_prev(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ float from_next(float v) { return __int_as_float(dpp_next(__float_as_int(v))); }
+__device__ __forceinline__ float from_prev(float v) { return __int_as_float(dpp_prev(__float_as_int(v))); }
+__device__ __forceinline__ double from_next(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_next((int)b), hi = dpp_next((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double from_prev(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_prev((int)b), hi = dpp_prev((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// wave_billiard.c:373  -y + 2*x + tcc*delta - KAPPA*x - tgamma*(x-y)
+__device__ __forceinline__ double leap(double x, double y, double delta, double cc, double kappa,
+                                       double g)
+{
+    double a = -y + 2.0 * x;
+    a = a + cc * delta;
+    a = a - kappa * x;
+    return a - g * (x - y);
+}
+
+// wave_billiard.c:418  x - tc*(x - inward) - K*x - G*(x-y)
+__device__ __forceinline__ double absorb(double x, double y, double inward, double c, double k,
+                                         double g)
+{
+    double a = x - c * (x - inward);
+    a = a - k * x;
+    return a - g * (x - y);
+}
+
+// Edge cell (rm, jm) of the outer box, every B_COND branch of
+// wave_billiard.c:393-583 and wave_3d.c:649-871.  E/Wv: rows rm+1 / rm-1,
+// N/S: columns jm+1 / jm-1 (already wrapped where the BC wraps).
+// Scalar parameters of one grid, read once into registers before a kernel's
+// main loop (reading them through the instance table inside the loop would
+// re-load them after every store, since a store could alias the table).
+struct Par {
+    int nx, ny, ldy, ldw, bc, osc_tb, tb_shift, coef;
+    double kappa, ks, gs, kt, gt;
+    double tcc0, tcc1, tg0, tg1;
+    float tcch0, tccl0, tcch1, tccl1, tgf0, tgf1;
+    __device__ __forceinline__ explicit Par(const WvInst &d)
+        : nx(d.nx), ny(d.ny), ldy(d.ldy), ldw(d.ldw), bc(d.bc), osc_tb(d.osc_tb),
+          tb_shift(d.tb_shift), coef(d.coef), kappa(d.kappa), ks(d.ks), gs(d.gs), kt(d.kt),
+          gt(d.gt), tcc0(d.tcc0), tcc1(d.tcc1), tg0(d.tg0), tg1(d.tg1), tcch0(d.tcch0),
+          tccl0(d.tccl0), tcch1(d.tcch1), tccl1(d.tccl1), tgf0(d.tgf0), tgf1(d.tgf1)
+    {
+    }
+};
+
+__device__ inline double edge_fp64(const Par &d, int rm, int jm, double x, double y, double E,
+                                   double Wv, double N, double S, double cc, double g, double tcv)
+{
+    const int nx = d.nx, ny = d.ny, bc = d.bc;
+    double delta;
+    if (jm == ny - 1 || jm == 0) {
+        // top (j = NY-1) and bottom (j = 0) rows, corners included
+        const bool top = (jm == ny - 1);
+        const double V = top ? S : N;  // inward neighbour along j
+        const double O = top ? N : S;  // neighbour across the periodic seam
+        if (d.osc_tb && rm < d.tb_shift && rm < nx - 1 && rm > 0) {
+            delta = (E + Wv) + -2.0 * x;
+            return leap(x, y, delta, cc, d.kappa, g);
+        }
+        switch (bc) {
+        case 0: {
+            const double Ec = (rm == nx - 1) ? x : E;
+            const double Wc = (rm == 0) ? x : Wv;
+            delta = ((Ec + Wc) + V) - 3.0 * x;
+            return leap(x, y, delta, cc, d.kappa, g);
+        }
+        case 1:
+            delta = (((E + Wv) + V) + O) - 4.0 * x;
+            return leap(x, y, delta, cc, d.kappa, g);
+        case 3: {
+            if (rm == 0) return absorb(x, y, E, tcv, d.ks, d.gs);
+            const double Ec = (rm == nx - 1) ? x : E;
+            delta = (((Ec + Wv) + V) + O) - 4.0 * x;
+            return leap(x, y, delta, cc, d.kappa, g);
+        }
+        default:
+            return absorb(x, y, V, tcv, d.kt, d.gt);
+        }
+    }
+    if (rm == 0) {  // left column, 1 <= j <= NY-2
+        switch (bc) {
+        case 0: delta = ((E + N) + S) - 3.0 * x; return leap(x, y, delta, cc, d.kappa, g);
+        case 1: delta = (((E + Wv) + N) + S) - 4.0 * x; return leap(x, y, delta, cc, d.kappa, g);
+        default: return absorb(x, y, E, tcv, d.ks, d.gs);
+        }
+    }
+    // right column i = NX-1
+    switch (bc) {
+    case 0: delta = ((Wv + N) + S) - 3.0 * x; return leap(x, y, delta, cc, d.kappa, g);
+    case 1: delta = (((Wv + E) + N) + S) - 4.0 * x; return leap(x, y, delta, cc, d.kappa, g);
+    default: return absorb(x, y, Wv, tcv, d.ks, d.gs);
+    }
+}
+
+// Column geometry of one lane: FAST lanes lie in 1..ny-2 and read/write
+// contiguously; other lanes map every column (wrap or clamp) one by one.
+template <bool FAST>
+struct Cols {
+    int j0;
+    int jm[WV_C];
+    bool colint[WV_C], wcol[WV_C];
+    bool inside;
+    __device__ __forceinline__ Cols(int j0_, int jb, int wout, int ny, bool wrapj) : j0(j0_)
+    {
+#pragma unroll
+        for (int c = 0; c < WV_C; c++) {
+            const int jj = j0 + c;
+            jm[c] = FAST ? jj : map_idx(jj, ny, wrapj);
+            colint[c] = FAST ? true : (jm[c] >= 1 && jm[c] <= ny - 2);
+            wcol[c] = jj >= jb && jj < jb + wout && jj < ny;
+        }
+        inside = FAST ? true : (j0 >= 0 && j0 + WV_C <= ny);
+    }
+};
+
+// The 32-bit word of a bit plane that holds the lane's WV_C bits, returned
+// raw (bits at position j0 & 31) so that a prefetched word is not touched
+// until it is consumed; bits_of() extracts them.  j0 is even, so the WV_C = 2
+// bits never straddle a word.  Lanes with mapped columns assemble their bits
+// at the same position.
+template <bool FAST>
+__device__ __forceinline__ unsigned lane_word(const uint64_t *plane, int ldw, int rm, const Cols<FAST> &L)
+{
+    if (FAST || L.inside) {
+        gptr<const uint32_t> row = (gptr<const uint32_t>)(gc(plane) + (size_t)rm * ldw);
+        return row[L.j0 >> 5];
+    }
+    gptr<const uint64_t> row = gc(plane) + (size_t)rm * ldw;
+    unsigned b = 0;
+#pragma unroll
+    for (int c = 0; c < WV_C; c++) b |= (unsigned)((row[L.jm[c] >> 6] >> (L.jm[c] & 63)) & 1u) << c;
+    return b << (L.j0 & 31);
+}
+
+template <bool FAST>
+__device__ __forceinline__ unsigned bits_of(unsigned w, const Cols<FAST> &L)
+{
+    return (w >> (L.j0 & 31)) & ((1u << WV_C) - 1u);
+}
+
+template <bool FAST>
+__device__ __forceinline__ void load_row(const double *base, int ldy, int rm, const Cols<FAST> &L,
+                                         double (&out)[WV_C])
+{
+    gptr<const double> row = gc(base) + (size_t)rm * ldy;
+    static_assert(WV_C == 2, "vector row loads assume 2 cells per lane");
+    if (FAST || L.inside) {
+        const dvec2 v = *(gptr<const dvec2>)(row + L.j0);
+        out[0] = v.x;
+        out[1] = v.y;
+    } else {
+#pragma unroll
+        for (int c = 0; c < WV_C; c++) out[c] = row[L.jm[c]];
+    }
+}
+
+template <bool FAST>
+__device__ __forceinline__ void load_row(const float *base, int ldy, int rm, const Cols<FAST> &L,
+                                         float (&out)[WV_C])
+{
+    gptr<const float> row = gc(base) + (size_t)rm * ldy;
+    if (FAST || L.inside) {
+        const fvec2 v = *(gptr<const fvec2>)(row + L.j0);
+        out[0] = v.x;
+        out[1] = v.y;
+    } else {
+#pragma unroll
+        for (int c = 0; c < WV_C; c++) out[c] = row[L.jm[c]];
+    }
+}
+
+// (tcc, tgamma) of a cell by coefficient mode (values, not loads, for the
+// two constant-media modes)
+template <bool FIELD>
+__device__ __forceinline__ void coef_fp64(const Par &d, const double *tccf, const double *tgf, int rm,
+                                          int jm, unsigned medbit, double &cc, double &g)
+{
+    if (FIELD) {
+        const size_t k = (size_t)rm * d.ldy + jm;
+        cc = gc(tccf)[k];
+        g = tgf ? gc(tgf)[k] : d.tg0;
+    } else {
+        cc = medbit ? d.tcc1 : d.tcc0;
+        g = medbit ? d.tg1 : d.tg0;
+    }
+}
+
+// Sources of one grid (CSR by row) and their amplitudes
+struct Src {
+    const int *ptr, *col, *id;
+    const double *amp;
+    long len;
+    __device__ __forceinline__ explicit Src(const WvInst &d)
+        : ptr(d.src_ptr), col(d.src_col), id(d.src_id), amp(d.amp), len(d.amp_len)
+    {
+    }
+    __device__ __forceinline__ bool any() const { return ptr != nullptr && len > 0; }
+};
+
+// source amplitudes at (rm, jm) for amplitude index g, added one by one in
+// source order to the updated value
+template <typename T>
+__device__ __forceinline__ T add_sources(const Src &q, int rm, int jm, long g, T v)
+{
+    const int k0 = gc(q.ptr)[rm], k1 = gc(q.ptr)[rm + 1];
+    for (int k = k0; k < k1; k++)
+        if (gc(q.col)[k] == jm) v = v + (T)gc(q.amp)[(size_t)gc(q.id)[k] * q.len + g];
+    return v;
+}
+
+__device__ __forceinline__ void dd_add_f(float h, float l, float w, float &ho, float &lo)
+{
+    const float s = h + w;
+    const float bb = s - h;
+    const float e = (h - (s - bb)) + (w - bb);
+    const float t = l + e;
+    const float hn = s + t;
+    ho = hn;
+    lo = t - (hn - s);
+}
+
+// Tile geometry shared by the stepping kernels
+template <int K>
+struct Tile {
+    static constexpr int C = WV_C;
+    static constexpr int H = (K + C - 1) / C * C;  // halo columns per side (>= K)
+    static constexpr int WOUT = 64 * C - 2 * H;    // output columns per tile
+};
+
+}  // namespace wv

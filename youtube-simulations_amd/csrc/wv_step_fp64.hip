@@ -1,0 +1,240 @@
+// wv_step_fp64.hip -- FP64 stepping kernel, one fused depth K per build
+// (compiled once per K with -DWV_K=k).
+//
+// A workgroup is one wave64 that owns a tile of 64*WV_C columns (halo
+// included) and a strip of rows, and streams down the strip one row per
+// iteration.  Skewed pipeline: at iteration s, level t (1..K) computes row
+// s-2t from the level t-1 rows produced in iterations s-1 (row below: E),
+// s-2 (centre) and s-3 (row above: W), so the K levels of one iteration are
+// independent of each other and their arithmetic interleaves; new rows are
+// committed after all levels have read.  Level-0 rows (u^n, u^{n-1} and the
+// active / medium bits) arrive through a 3-deep prefetch ring: row s+3 is
+// requested at the end of iteration s.  Window slots rotate with the 3-way
+// unrolled loop: at phase P the rows of iterations s-1, s-2, s-3 sit in
+// slots (P+2)%3, (P+1)%3 and P.  Level K at row s-2K is final and is written
+// out with level K-1 of the same row (the next launch's u^{n-1}).
+//
+// The outermost columns and rows of a tile are recomputed by the neighbouring
+// tiles (overlapped tiling): a tile reads K halo rows above and below its
+// strip and H >= K halo columns on each side, and writes only its interior.
+// With periodic edges the halo indices wrap, so a halo cell is an exact copy
+// of the cell it aliases.  Every cell evaluates the reference expression in
+// the reference order whatever K, strip height or tile width, so results are
+// bit-identical to one step at a time and to the reference stepper.
+#include "wv_device.h"
+
+#ifndef WV_K
+#error "compile with -DWV_K=<fused depth>"
+#endif
+
+using namespace wv;
+
+namespace {
+
+// INTERIOR: the tile's rows and columns never reach the outer box (bulk
+// formula only, no index wrapping); otherwise every cell is classified.
+template <int K, int ST, bool INTERIOR, bool FIELD, bool SRC>
+__device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long gbase, int ia, int ib,
+                                         int oc, int od)
+{
+    constexpr int C = WV_C;
+    constexpr int H = Tile<K>::H, WOUT = Tile<K>::WOUT;
+    constexpr unsigned CMASK = (1u << C) - 1u;
+    static_assert(2 * K * C <= 32, "bit shift register holds 2K rows of C bits");
+    const Par p(d);
+    const Src src(d);
+    const int nx = p.nx, ny = p.ny, ldy = p.ldy, ldw = p.ldw;
+    const bool wrapi = p.bc == 1, wrapj = (p.bc == 1 || p.bc == 3);
+    const int jb = t.cb * WOUT;
+    constexpr bool FAST = INTERIOR;
+    const Cols<FAST> L(jb - H + (int)threadIdx.x * C, jb, WOUT, ny, wrapj);
+    const int ibeg = t.sb * d.rows_per_tile;
+    const int iend = min(ibeg + d.rows_per_tile, nx);
+    const bool has_src = SRC && src.any();
+    const double *u0 = d.u[ia], *um1 = d.u[ib];
+    const uint64_t *act = d.act, *med = d.med;
+    const double *tcp = d.tc, *tccf = d.tccf, *tgf = d.tgf;
+    gptr<double> out0 = gm(d.u[oc]), out1 = gm(d.u[od]);
+
+    double U[K][3][C], Up[K][3][C];  // windows: u^t and u^{t-1} per row
+    double QU[3][C], QP[3][C];       // prefetch ring: u^n, u^{n-1} rows
+    unsigned QB[3], QM[3];           // prefetched active / medium bit words
+    unsigned mreg = 0, medreg = 0;   // bits of rows s-1, s-2, ... (C bits each, newest lowest)
+#pragma unroll
+    for (int a = 0; a < K; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int c = 0; c < C; c++) U[a][b][c] = Up[a][b][c] = 0.0;
+
+    const int s0 = ibeg - K;         // first level-0 row
+    const int slast = iend + K - 1;  // last level-0 row the strip needs
+    const int send = iend + 2 * K;   // iterations s0 .. send-1
+
+    auto fetch = [&](int row, double(&qu)[C], double(&qp)[C], unsigned &qb, unsigned &qm) {
+        const int rm = INTERIOR ? min(max(row, 0), nx - 1) : map_idx(row, nx, wrapi);
+        load_row(u0, ldy, rm, L, qu);
+        load_row(um1, ldy, rm, L, qp);
+        qb = lane_word(act, ldw, rm, L);
+        // unconditional: a load under a branch makes the compiler drain every
+        // outstanding load (the prefetch ring) where the branch joins
+        qm = lane_word(med, ldw, rm, L);
+    };
+    fetch(s0, QU[0], QP[0], QB[0], QM[0]);
+    fetch(s0 + 1, QU[1], QP[1], QB[1], QM[1]);
+    fetch(s0 + 2, QU[2], QP[2], QB[2], QM[2]);
+
+    auto iter = [&](int s, auto phase) {
+        constexpr int P = decltype(phase)::value;
+        constexpr int PE = (P + 2) % 3, PC = (P + 1) % 3, PW = P;
+        double NU[K + 1][C], NP[K + 1][C];
+        unsigned abK = 0;
+#pragma unroll
+        for (int lv = 1; lv <= K; lv++) {
+            const int r = s - 2 * lv;
+            // rows outside the strip's dependence cone are computed but never used:
+            // INTERIOR tiles only clamp them so that table reads stay in bounds
+            const int rm = INTERIOR ? min(max(r, 0), nx - 1) : map_idx(r, nx, wrapi);
+            const bool rowint = INTERIOR || (rm >= 1 && rm <= nx - 2);
+            const double(&Xe)[C] = U[lv - 1][PE];
+            const double(&Xc)[C] = U[lv - 1][PC];
+            const double(&Xw)[C] = U[lv - 1][PW];
+            const double(&Yc)[C] = Up[lv - 1][PC];
+            const int sh = (2 * lv - 1) * C;
+            const unsigned ab = (mreg >> sh) & CMASK;
+            const unsigned mb = (medreg >> sh) & CMASK;
+            if (lv == K) abK = ab;
+            const double cn = from_next(Xc[0]), cp = from_prev(Xc[C - 1]);
+            double en = 0, ep = 0, wn = 0, wp = 0;
+            if (ST == 9) {
+                en = from_next(Xe[0]); ep = from_prev(Xe[C - 1]);
+                wn = from_next(Xw[0]); wp = from_prev(Xw[C - 1]);
+            }
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                const int cn1 = c < C - 1 ? c + 1 : 0, cp1 = c > 0 ? c - 1 : 0;
+                const double x = Xc[c], y = Yc[c], E = Xe[c], Wv = Xw[c];
+                const double N = (c < C - 1) ? Xc[cn1] : cn;
+                const double S = (c > 0) ? Xc[cp1] : cp;
+                double cc, g;
+                coef_fp64<FIELD>(p, tccf, tgf, rm, L.jm[c], (mb >> c) & 1u, cc, g);
+                double v;
+                if (rowint && L.colint[c]) {
+                    double delta;
+                    if (ST == 9) {
+                        const double NE = (c < C - 1) ? Xe[cn1] : en;
+                        const double SE = (c > 0) ? Xe[cp1] : ep;
+                        const double NW = (c < C - 1) ? Xw[cn1] : wn;
+                        const double SW = (c > 0) ? Xw[cp1] : wp;
+                        const double s4 = ((E + Wv) + N) + S;
+                        const double sd = ((NE + NW) + SE) + SW;
+                        delta = ((4.0 * s4 + sd) - 20.0 * x) * W9;
+                    } else {
+                        delta = (((E + Wv) + N) + S) - 4.0 * x;
+                    }
+                    v = leap(x, y, delta, cc, p.kappa, g);
+                } else {
+                    const double tcv = gc(tcp)[(size_t)rm * ldy + L.jm[c]];
+                    v = edge_fp64(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv);
+                }
+                if (has_src) {
+                    const long g1 = gbase + lv - 1;
+                    if (g1 < src.len) v = add_sources(src, rm, L.jm[c], g1, v);
+                }
+                NU[lv][c] = ((ab >> c) & 1u) ? v : x;  // cells the step never writes keep u
+                NP[lv][c] = x;
+            }
+        }
+        // level K at row s-2K is final
+        const int r = s - 2 * K;
+        if (r >= ibeg && r < iend) {
+            gptr<double> o0 = out0 + (size_t)r * ldy + L.j0;
+            gptr<double> o1 = out1 + (size_t)r * ldy + L.j0;
+            if (FAST && abK == CMASK && L.wcol[0] && L.wcol[C - 1]) {
+                const dvec2 a = {NU[K][0], NU[K][1]};
+                *(gptr<dvec2>)o0 = a;
+                if (K >= 2) {
+                    const dvec2 b = {NP[K][0], NP[K][1]};
+                    *(gptr<dvec2>)o1 = b;
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < C; c++)
+                    if (L.wcol[c] && ((abK >> c) & 1u)) {
+                        o0[c] = NU[K][c];
+                        if (K >= 2) o1[c] = NP[K][c];
+                    }
+            }
+        }
+        // commit: level 0 from the prefetch ring, levels 1..K-1 from this iteration
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            U[0][P][c] = QU[P][c];
+            Up[0][P][c] = QP[P][c];
+        }
+        mreg = (mreg << C) | bits_of(QB[P], L);
+        medreg = (medreg << C) | bits_of(QM[P], L);
+#pragma unroll
+        for (int lv = 1; lv < K; lv++)
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                U[lv][P][c] = NU[lv][c];
+                Up[lv][P][c] = NP[lv][c];
+            }
+        // unconditional (past the last needed row it re-reads that cached row):
+        // a conditional load would make the compiler drain the ring each iteration
+        fetch(min(s + 3, slast), QU[P], QP[P], QB[P], QM[P]);
+    };
+
+    for (int s = s0; s < send; s += 3) {
+        iter(s, std::integral_constant<int, 0>{});
+        if (s + 1 >= send) break;
+        iter(s + 1, std::integral_constant<int, 1>{});
+        if (s + 2 >= send) break;
+        iter(s + 2, std::integral_constant<int, 2>{});
+    }
+}
+
+template <int K, int ST>
+__global__ void __launch_bounds__(64)
+k_step_fp64(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, const long *gstep,
+            long g0, int ia, int ib, int oc, int od)
+{
+    const WvTile t = tiles[blockIdx.x];
+    if (t.inst < 0) return;  // padding of the XCD interleave
+    const WvInst &d = I[t.inst];
+    const int js = t.cb * Tile<K>::WOUT - Tile<K>::H;  // first column of the tile
+    const long gbase = (gstep ? *gstep : 0) + g0;
+    // wave-uniform: every column of the tile is an interior column
+    const int L = d.rows_per_tile, ibeg = t.sb * L, iend = min(ibeg + L, d.nx);
+    const bool interior = js >= 1 && js + 64 * WV_C <= d.ny - 1 && ibeg >= K && iend <= d.nx - K;
+    const bool field = d.coef == WV_COEF_FIELD;
+    const bool src = d.src_ptr != nullptr && d.amp_len > 0;
+    if (interior) {
+        if (field) {
+            if (src) run_fp64<K, ST, true, true, true>(d, t, gbase, ia, ib, oc, od);
+            else run_fp64<K, ST, true, true, false>(d, t, gbase, ia, ib, oc, od);
+        } else {
+            if (src) run_fp64<K, ST, true, false, true>(d, t, gbase, ia, ib, oc, od);
+            else run_fp64<K, ST, true, false, false>(d, t, gbase, ia, ib, oc, od);
+        }
+    } else {
+        if (field) run_fp64<K, ST, false, true, true>(d, t, gbase, ia, ib, oc, od);
+        else run_fp64<K, ST, false, false, true>(d, t, gbase, ia, ib, oc, od);
+    }
+}
+
+}  // namespace
+
+#define WV_CAT2(a, b) a##b
+#define WV_CAT(a, b) WV_CAT2(a, b)
+
+int WV_CAT(wv_launch_fp64_k, WV_K)(int stencil, const WvInst *I, const WvTile *T, int n, const long *gs,
+                                    long g0, int ia, int ib, int oc, int od, hipStream_t s)
+{
+    if (stencil == 9)
+        hipLaunchKernelGGL((k_step_fp64<WV_K, 9>), dim3(n), dim3(64), 0, s, I, T, gs, g0, ia, ib, oc, od);
+    else
+        hipLaunchKernelGGL((k_step_fp64<WV_K, 5>), dim3(n), dim3(64), 0, s, I, T, gs, g0, ia, ib, oc, od);
+    return (int)hipGetLastError();
+}
