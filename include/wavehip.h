@@ -104,6 +104,10 @@ int wv_step(wv_ctx *c, long nsteps);
 int wv_step_timed(wv_ctx *c, long nsteps, double *ms, long *launches);
 int wv_sync(wv_ctx *c);
 long wv_steps_done(const wv_ctx *c);
+/* Kernel time of the last wv_step_timed: the summed durations of its stepping
+ * launches, each bracketed by a HIP event pair on the context stream (one
+ * launch per fused block of steps covers every tile; boundary_ms is 0). */
+int wv_last_kernel_ms(const wv_ctx *c, double *interior_ms, double *boundary_ms);
 
 /* Total energy: compute_energy (wave_common.c:453-472) per cell with c^2 from
  * tcc, summed in FP64 in a fixed order (64 lane-strided partials per row, a
@@ -152,6 +156,7 @@ int wv_batch_set_state(wv_batch *b, int k, const double *phi, const double *psi)
 int wv_batch_get_state(wv_batch *b, int k, double *phi, double *psi);
 int wv_batch_step(wv_batch *b, long nsteps);
 int wv_batch_step_timed(wv_batch *b, long nsteps, double *ms, long *launches);
+int wv_batch_last_kernel_ms(const wv_batch *b, double *interior_ms, double *boundary_ms);
 int wv_batch_energy(wv_batch *b, double *e /* n values */);
 int wv_batch_set_tiling(wv_batch *b, int tb_depth, int rows_per_tile);
 int wv_batch_get_tiling(const wv_batch *b, int *tb_depth, int *rows_per_tile, int *tile_cols);

@@ -408,7 +408,10 @@ void orc_split_run(const orc_params *p, float *hi, float *lo, float *v, long nst
                 const float ch = (float)tcc[k];
                 const float cl = (float)(tcc[k] - (double)ch);
                 const float gf = (float)tgamma[k];
-                float vn = ((vc + fmaf(ch, lap, cl * lap)) - kf * hc) - gf * vc;
+                /* a term whose coefficient is zero is not evaluated */
+                float vn = vc + fmaf(ch, lap, cl * lap);
+                if (kf != 0.0f) vn = vn - kf * hc;
+                if (gf != 0.0f) vn = vn - gf * vc;
                 for (int s = 0; s < nsrc; s++)
                     if (g < amp_len && si[s] == i && sj[s] == j) vn = vn + (float)amp[(size_t)s * amp_len + g];
                 dd_add_f(hc, lc, vn, &h2[k], &l2[k]);

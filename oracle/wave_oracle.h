@@ -105,6 +105,7 @@ double orc_fixed_order_sum(int nx, int ny, const double *e);
  *   dh   = ((hE-hc) + (hW-hc)) + ((hN-hc) + (hS-hc))   (9-pt: see .c)
  *   dl   = same on lo;  lap = dh + dl
  *   vn   = ((vc + fmaf(tcc_hi, lap, tcc_lo*lap)) - kappa*hc) - gamma*vc
+ *          (a term whose coefficient is zero is not evaluated)
  *   (hi', lo') = hc + lc + vn   (two-sum then quick-two-sum)
  * Edge cells use the FP64 reference formula on u = (double)hi + (double)lo,
  * u^{n-1} = u - (double)v.  Inactive cells are untouched. */

@@ -48,7 +48,8 @@ struct Engine {
     int role[4] = {0, 1, 2, 3};  // FP64: phi, psi, free1 (tmp), free2
     int scur = 0;                // split: current set
     WvInst *d_inst = nullptr;
-    std::map<int, std::pair<WvTile *, int>> tiles;  // depth -> device tile table
+    struct Tiles { WvTile *d = nullptr; int ni = 0, nb = 0; };  // interior tiles, then boundary tiles
+    std::map<int, Tiles> tiles;  // depth -> device tile tables
     int K = 0, L = 0;            // tiling (0 = untuned default)
     long steps_done = 0, last_launches = 0;
     long *d_gstep = nullptr;
@@ -65,6 +66,12 @@ struct Engine {
     std::vector<short> stm;
     bool dropin_tables = false;
     bool xcd_order = true;  // WAVEHIP_TILE_ORDER=linear turns the XCD interleave off
+    // per-kernel timing of the last wv_step_timed: an event pair around each
+    // stepping launch
+    bool time_kernels = false;
+    std::vector<hipEvent_t> kev;
+    size_t kev_used = 0;
+    double last_int_ms = 0, last_bnd_ms = 0;
 };
 
 int fail(Engine *e, int code, const char *fmt, ...)
@@ -87,8 +94,9 @@ int fail(Engine *e, int code, const char *fmt, ...)
 
 int round_up(int a, int b) { return (a + b - 1) / b * b; }
 
-int default_depth(const Engine *e) { return e->form == WV_FORM_F32SPLIT ? 4 : 4; }
-int default_rows(const Engine *) { return 64; }
+// defaults measured on MI355X at 4096^2 (both forms): 4 fused steps, 12-row strips
+int default_depth(const Engine *) { return 4; }
+int default_rows(const Engine *) { return 12; }
 int depth(const Engine *e) { return e->K > 0 ? e->K : default_depth(e); }
 int rows(const Engine *e) { return e->L > 0 ? e->L : default_rows(e); }
 
@@ -165,6 +173,7 @@ WvInst make_inst(const Engine *e, const Grid &G)
     d.kappa = G.p.kappa; d.ks = G.p.kappa_sides; d.gs = G.p.gamma_sides;
     d.kt = G.p.kappa_topbot; d.gt = G.p.gamma_topbot;
     d.es2 = G.p.e_scale * G.p.e_scale;
+    d.damped = G.p.kappa != 0.0 || G.tg0 != 0.0 || G.tg1 != 0.0 || G.tgf != nullptr;
     return d;
 }
 
@@ -185,7 +194,7 @@ void drop_graphs(Engine *e)
 
 void drop_tiles(Engine *e)
 {
-    for (auto &kv : e->tiles) (void)hipFree(kv.second.first);
+    for (auto &kv : e->tiles) (void)hipFree(kv.second.d);
     e->tiles.clear();
     drop_graphs(e);
 }
@@ -197,12 +206,11 @@ void drop_tiles(Engine *e)
 // every column tile of a strip runs on the same XCD -- the halo columns two
 // neighbouring tiles both read, and the cache lines their outputs share,
 // stay in one L2.  Tiles with inst = -1 only pad the interleave.
-int get_tiles(Engine *e, int k, WvTile **out, int *count)
+int get_tiles(Engine *e, int k, Engine::Tiles *out)
 {
     auto it = e->tiles.find(k);
     if (it != e->tiles.end()) {
-        *out = it->second.first;
-        *count = it->second.second;
+        *out = it->second;
         return WV_OK;
     }
     const int wout = wv_tile_cols(k), L = rows(e);
@@ -213,7 +221,25 @@ int get_tiles(Engine *e, int k, WvTile **out, int *count)
         const int ncb = (G.p.ny + wout - 1) / wout, nsb = (G.p.nx + L - 1) / L;
         for (int sb = 0; sb < nsb; sb++) strips.push_back(Strip{q, sb, ncb});
     }
-    std::vector<WvTile> h;
+    // flags: INTERIOR when every row and column the tile reads lies inside the
+    // outer box (no edge formula, no wrapping); ALLACT when, in addition,
+    // every one of those cells is written by a step
+    auto flags_of = [&](int q, int cb, int sb) {
+        const Grid &G = e->g[q];
+        const int nx = G.p.nx, ny = G.p.ny;
+        const int js = cb * wout - wv_halo(k);
+        const int ibeg = sb * L, iend = std::min(ibeg + L, nx);
+        const bool interior = js >= 1 && js + 64 * WV_C <= ny - 1 && ibeg >= k && iend <= nx - k;
+        if (!interior) return 0;
+        bool all = !G.h_act.empty();
+        for (int i = ibeg - k; all && i < iend + k; i++)
+            for (int j = js; j < js + 64 * WV_C; j++)
+                if (!((G.h_act[(size_t)i * G.ldw + (j >> 6)] >> (j & 63)) & 1u)) { all = false; break; }
+        return WV_TILE_INTERIOR | (all ? WV_TILE_ALLACT : 0);
+    };
+    // one list in XCD-interleave order; boundary tiles keep their place so
+    // that they run among the interior tiles instead of as a tail
+    std::vector<WvTile> h, hb;
     if (e->xcd_order) {
         const int X = 8;
         for (size_t g0 = 0; g0 < strips.size(); g0 += X) {
@@ -222,21 +248,27 @@ int get_tiles(Engine *e, int k, WvTile **out, int *count)
             for (size_t q = g0; q < g1; q++) maxcb = std::max(maxcb, strips[q].ncb);
             for (int cb = 0; cb < maxcb; cb++)
                 for (size_t q = g0; q < g0 + X; q++) {
-                    if (q < g1 && cb < strips[q].ncb) h.push_back(WvTile{strips[q].inst, cb, strips[q].sb, 0});
-                    else if (q < g1 || g1 == g0 + X) h.push_back(WvTile{-1, 0, 0, 0});
+                    if (q < g1 && cb < strips[q].ncb)
+                        h.push_back(WvTile{strips[q].inst, cb, strips[q].sb, flags_of(strips[q].inst, cb, strips[q].sb)});
+                    else if (q < g1 || g1 == g0 + X)
+                        h.push_back(WvTile{-1, 0, 0, 0});
                 }
         }
         while (!h.empty() && h.back().inst < 0) h.pop_back();
     } else {
         for (const Strip &st : strips)
-            for (int cb = 0; cb < st.ncb; cb++) h.push_back(WvTile{st.inst, cb, st.sb, 0});
+            for (int cb = 0; cb < st.ncb; cb++) h.push_back(WvTile{st.inst, cb, st.sb, flags_of(st.inst, cb, st.sb)});
     }
-    WvTile *d = nullptr;
-    HIPCHK(e, hipMalloc(&d, sizeof(WvTile) * h.size()));
-    HIPCHK(e, hipMemcpy(d, h.data(), sizeof(WvTile) * h.size(), hipMemcpyHostToDevice));
-    e->tiles[k] = {d, (int)h.size()};
-    *out = d;
-    *count = (int)h.size();
+    Engine::Tiles T;
+    T.ni = (int)h.size();
+    T.nb = (int)hb.size();
+    h.insert(h.end(), hb.begin(), hb.end());
+    if (!h.empty()) {
+        HIPCHK(e, hipMalloc(&T.d, sizeof(WvTile) * h.size()));
+        HIPCHK(e, hipMemcpy(T.d, h.data(), sizeof(WvTile) * h.size(), hipMemcpyHostToDevice));
+    }
+    e->tiles[k] = T;
+    *out = T;
     return WV_OK;
 }
 
@@ -301,6 +333,7 @@ void engine_destroy(Engine *e)
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     drop_tiles(e);
+    for (hipEvent_t ev : e->kev) (void)hipEventDestroy(ev);
     for (auto &G : e->g) free_grid(G);
     void *ptrs[] = {e->d_inst, e->d_gstep, e->d_partial, e->d_eout, e->d_erows, e->d_row0};
     for (void *q : ptrs)
@@ -439,7 +472,7 @@ int set_tables(Engine *e, int k, const short *xy, const double *tcc, const doubl
     if (G.state && G.h_act != act) G.exact = true;
     G.h_act.swap(act);
     G.tables = true;
-    drop_graphs(e);
+    drop_tiles(e);  // tile flags depend on the mask
     return sync_inst(e);
 }
 
@@ -618,9 +651,9 @@ int prepare_tiles(Engine *e, long nsteps, int kmax)
     long done = 0;
     while (done < nsteps) {
         const int k = next_depth(kmax, nsteps - done);
-        WvTile *t;
-        int nt, rc;
-        if ((rc = get_tiles(e, k, &t, &nt))) return rc;
+        Engine::Tiles t;
+        int rc;
+        if ((rc = get_tiles(e, k, &t))) return rc;
         done += k;
         if (k == kmax) done += (nsteps - done) / kmax * kmax;  // the rest of the full blocks
     }
@@ -634,16 +667,30 @@ int enqueue_steps(Engine *e, long nsteps, bool force_k1, long *launches)
     const int kmax = max_depth(e, force_k1);
     while (done < nsteps) {
         const int k = next_depth(kmax, nsteps - done);
-        WvTile *t;
-        int nt, rc;
-        if ((rc = get_tiles(e, k, &t, &nt))) return rc;
+        Engine::Tiles t;
+        int rc;
+        if ((rc = get_tiles(e, k, &t))) return rc;
+        auto launch = [&](int ni, int nb) {
+            if (e->form == WV_FORM_FP64)
+                return wv_launch_fp64(k, e->stencil, e->d_inst, t.d, ni, t.d + t.ni, nb, e->d_gstep, done,
+                                      e->role[0], e->role[1], e->role[2], e->role[3], e->stream);
+            return wv_launch_split(k, e->stencil, e->d_inst, t.d, ni, t.d + t.ni, nb, e->d_gstep, done,
+                                   e->scur, 1 - e->scur, e->stream);
+        };
         int lr;
-        if (e->form == WV_FORM_FP64) {
-            lr = wv_launch_fp64(k, e->stencil, e->d_inst, t, nt, e->d_gstep, done, e->role[0], e->role[1],
-                                e->role[2], e->role[3], e->stream);
+        if (e->time_kernels) {
+            while (e->kev.size() < e->kev_used + 2) {
+                hipEvent_t ev;
+                HIPCHK(e, hipEventCreate(&ev));
+                e->kev.push_back(ev);
+            }
+            hipEvent_t *ev = &e->kev[e->kev_used];
+            e->kev_used += 2;
+            HIPCHK(e, hipEventRecord(ev[0], e->stream));
+            lr = launch(t.ni, t.nb);
+            HIPCHK(e, hipEventRecord(ev[1], e->stream));
         } else {
-            lr = wv_launch_split(k, e->stencil, e->d_inst, t, nt, e->d_gstep, done, e->scur, 1 - e->scur,
-                                 e->stream);
+            lr = launch(t.ni, t.nb);
         }
         advance_roles(e, k);
         if (lr) return fail(e, WV_EHIP, "kernel launch failed: %s", hipGetErrorString((hipError_t)lr));
@@ -738,12 +785,27 @@ int engine_step_timed(Engine *e, long nsteps, double *ms, long *launches)
     HIPCHK(e, hipEventCreate(&a));
     HIPCHK(e, hipEventCreate(&b));
     HIPCHK(e, hipEventRecord(a, e->stream));
+    // per-kernel events need the launches themselves, not a replayed graph
+    const bool graph = e->use_graph;
+    e->use_graph = false;
+    e->time_kernels = true;
+    e->kev_used = 0;
     rc = engine_step(e, nsteps, false);
+    e->time_kernels = false;
+    e->use_graph = graph;
     if (rc) { (void)hipEventDestroy(a); (void)hipEventDestroy(b); return rc; }
     HIPCHK(e, hipEventRecord(b, e->stream));
     HIPCHK(e, hipEventSynchronize(b));
     float t = 0;
     HIPCHK(e, hipEventElapsedTime(&t, a, b));
+    double ti = 0, tb = 0;
+    for (size_t q = 0; q + 2 <= e->kev_used; q += 2) {
+        float x = 0;
+        HIPCHK(e, hipEventElapsedTime(&x, e->kev[q], e->kev[q + 1]));
+        ti += x;
+    }
+    e->last_int_ms = ti;
+    e->last_bnd_ms = tb;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (ms) *ms = t;
@@ -865,6 +927,14 @@ int wv_sync(wv_ctx *c)
 }
 
 long wv_steps_done(const wv_ctx *c) { return c ? c->e->steps_done : -1; }
+
+int wv_last_kernel_ms(const wv_ctx *c, double *interior_ms, double *boundary_ms)
+{
+    if (!c) return WV_EINVAL;
+    if (interior_ms) *interior_ms = c->e->last_int_ms;
+    if (boundary_ms) *boundary_ms = c->e->last_bnd_ms;
+    return WV_OK;
+}
 
 int wv_energy(wv_ctx *c, double *e)
 {
@@ -1039,6 +1109,14 @@ int wv_batch_step_timed(wv_batch *b, long nsteps, double *ms, long *launches)
 {
     if (!b) return WV_EINVAL;
     return engine_step_timed(b->e, nsteps, ms, launches);
+}
+
+int wv_batch_last_kernel_ms(const wv_batch *b, double *interior_ms, double *boundary_ms)
+{
+    if (!b) return WV_EINVAL;
+    if (interior_ms) *interior_ms = b->e->last_int_ms;
+    if (boundary_ms) *boundary_ms = b->e->last_bnd_ms;
+    return WV_OK;
 }
 
 int wv_batch_energy(wv_batch *b, double *e)

@@ -222,11 +222,14 @@ __device__ __forceinline__ void load_row(const float *base, int ldy, int rm, con
 
 // (tcc, tgamma) of a cell by coefficient mode (values, not loads, for the
 // two constant-media modes)
-template <bool FIELD>
+template <int COEF>
 __device__ __forceinline__ void coef_fp64(const Par &d, const double *tccf, const double *tgf, int rm,
                                           int jm, unsigned medbit, double &cc, double &g)
 {
-    if (FIELD) {
+    if (COEF == WV_COEF_UNIFORM) {
+        cc = d.tcc0;
+        g = d.tg0;
+    } else if (COEF == WV_COEF_FIELD) {
         const size_t k = (size_t)rm * d.ldy + jm;
         cc = gc(tccf)[k];
         g = tgf ? gc(tgf)[k] : d.tg0;

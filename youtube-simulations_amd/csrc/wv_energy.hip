@@ -45,8 +45,9 @@ k_energy_rows(const WvInst *__restrict__ I, const int2 *__restrict__ rows, doubl
             const unsigned mb = d.coef == WV_COEF_TWO
                                     ? (unsigned)((gc(d.med)[(size_t)i * d.ldw + (j >> 6)] >> (j & 63)) & 1u)
                                     : 0u;
-            if (d.coef == WV_COEF_FIELD) coef_fp64<true>(Par(d), d.tccf, d.tgf, i, j, mb, cc, g);
-            else coef_fp64<false>(Par(d), d.tccf, d.tgf, i, j, mb, cc, g);
+            if (d.coef == WV_COEF_FIELD) coef_fp64<WV_COEF_FIELD>(Par(d), d.tccf, d.tgf, i, j, mb, cc, g);
+            else if (d.coef == WV_COEF_TWO) coef_fp64<WV_COEF_TWO>(Par(d), d.tccf, d.tgf, i, j, mb, cc, g);
+            else coef_fp64<WV_COEF_UNIFORM>(Par(d), d.tccf, d.tgf, i, j, mb, cc, g);
             const double a1 = uip - c, a2 = c - uim, b1 = ujp - c, b2 = c - ujm;
             const double gx2 = a1 * a1 + a2 * a2, gy2 = b1 * b1 + b2 * b2;
             e = d.es2 * (v * v + 0.5 * cc * (gx2 + gy2));
@@ -79,33 +80,33 @@ __global__ void k_bump(long *g, long n)
 }  // namespace
 
 #define WV_DEPTHS(X) X(1) X(2) X(3) X(4) X(6) X(8)
-#define WV_DECL(k)                                                                                  \
-    int wv_launch_fp64_k##k(int, const WvInst *, const WvTile *, int, const long *, long, int, int, \
-                            int, int, hipStream_t);                                                \
-    int wv_launch_split_k##k(int, const WvInst *, const WvTile *, int, const long *, long, int, int, \
-                             hipStream_t);
+#define WV_DECL(k)                                                                              \
+    int wv_launch_fp64_k##k(int, const WvInst *, const WvTile *, int, const WvTile *, int,          \
+                            const long *, long, int, int, int, int, hipStream_t);                 \
+    int wv_launch_split_k##k(int, const WvInst *, const WvTile *, int, const WvTile *, int,         \
+                             const long *, long, int, int, hipStream_t);
 WV_DEPTHS(WV_DECL)
 
 bool wv_have_depth(int k) { return k == 1 || k == 2 || k == 3 || k == 4 || k == 6 || k == 8; }
 
-int wv_launch_fp64(int k, int stencil, const WvInst *I, const WvTile *T, int n, const long *gs,
-                   long g0, int ia, int ib, int oc, int od, hipStream_t s)
+int wv_launch_fp64(int k, int stencil, const WvInst *I, const WvTile *Ti, int ni, const WvTile *Tb,
+                   int nb, const long *gs, long g0, int ia, int ib, int oc, int od, hipStream_t s)
 {
     switch (k) {
 #define WV_CASE(kk) \
-    case kk: return wv_launch_fp64_k##kk(stencil, I, T, n, gs, g0, ia, ib, oc, od, s);
+    case kk: return wv_launch_fp64_k##kk(stencil, I, Ti, ni, Tb, nb, gs, g0, ia, ib, oc, od, s);
         WV_DEPTHS(WV_CASE)
 #undef WV_CASE
     }
     return -1;
 }
 
-int wv_launch_split(int k, int stencil, const WvInst *I, const WvTile *T, int n, const long *gs,
-                    long g0, int ia, int oc, hipStream_t s)
+int wv_launch_split(int k, int stencil, const WvInst *I, const WvTile *Ti, int ni, const WvTile *Tb,
+                    int nb, const long *gs, long g0, int ia, int oc, hipStream_t s)
 {
     switch (k) {
 #define WV_CASE(kk) \
-    case kk: return wv_launch_split_k##kk(stencil, I, T, n, gs, g0, ia, oc, s);
+    case kk: return wv_launch_split_k##kk(stencil, I, Ti, ni, Tb, nb, gs, g0, ia, oc, s);
         WV_DEPTHS(WV_CASE)
 #undef WV_CASE
     }

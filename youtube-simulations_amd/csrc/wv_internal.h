@@ -41,29 +41,33 @@ struct WvInst {
     int nx, ny, ldy, ldw;
     int bc, osc_tb, tb_shift, coef;
     int rows_per_tile, ncb;              // tiling of this instance
+    int damped;                          // KAPPA or some tgamma is non-zero
     int tile0;                           // first tile index of this instance
     double tcc0, tcc1, tg0, tg1;         // coefficient pairs (modes UNIFORM / TWO)
     float tcch0, tccl0, tcch1, tccl1, tgf0, tgf1;
     double kappa, ks, gs, kt, gt, es2;
 };
 
-// one tile of a stepping launch: instance, column block, row strip
+// one tile of a stepping launch: instance, column block, row strip, flags
+#define WV_TILE_INTERIOR 1  // rows and columns the tile reads never reach the outer box
+#define WV_TILE_ALLACT 2    // ... and every one of those cells is written by a step
 struct WvTile {
-    int inst, cb, sb, pad;
+    int inst, cb, sb, flags;
 };
 
 // number of output columns of one wave for a fused depth k
 static inline int wv_halo(int k) { return (k + WV_C - 1) / WV_C * WV_C; }
 static inline int wv_tile_cols(int k) { return 64 * WV_C - 2 * wv_halo(k); }
 
-// Launchers (wv_kernels.hip).  ia/ib: buffers holding u^n, u^{n-1}; oc/od:
-// buffers receiving u^{n+k}, u^{n+k-1} (od unused when k == 1).  Split form:
-// ia = input set, oc = output set.  g0 / gstep: amplitude index of the first
-// step = *gstep + g0 when gstep is non-null.
-int wv_launch_fp64(int k, int stencil, const WvInst *d_inst, const WvTile *d_tiles, int ntiles,
-                   const long *gstep, long g0, int ia, int ib, int oc, int od, hipStream_t s);
-int wv_launch_split(int k, int stencil, const WvInst *d_inst, const WvTile *d_tiles, int ntiles,
-                    const long *gstep, long g0, int ia, int oc, hipStream_t s);
+// Launchers (wv_step_*.hip, dispatched in wv_energy.hip).  Ti/ni: interior
+// tiles, Tb/nb: boundary tiles (two kernels, one after the other).  ia/ib:
+// buffers holding u^n, u^{n-1}; oc/od: buffers receiving u^{n+k}, u^{n+k-1}
+// (od unused when k == 1).  Split form: ia = input set, oc = output set.
+// The amplitude index of the first step is *gstep + g0.
+int wv_launch_fp64(int k, int stencil, const WvInst *d_inst, const WvTile *Ti, int ni, const WvTile *Tb,
+                   int nb, const long *gstep, long g0, int ia, int ib, int oc, int od, hipStream_t s);
+int wv_launch_split(int k, int stencil, const WvInst *d_inst, const WvTile *Ti, int ni, const WvTile *Tb,
+                    int nb, const long *gstep, long g0, int ia, int oc, hipStream_t s);
 bool wv_have_depth(int k);
 // energy: rows table (inst,row) pairs; partial[nrows_total]; out[ninst]
 int wv_launch_energy(int split, const WvInst *d_inst, const int2 *d_rows, int nrows,

@@ -33,24 +33,27 @@ namespace {
 
 // INTERIOR: the tile's rows and columns never reach the outer box (bulk
 // formula only, no index wrapping); otherwise every cell is classified.
-template <int K, int ST, bool INTERIOR, bool FIELD, bool SRC>
+// COEF: WV_COEF_UNIFORM / TWO / FIELD.  ALLACT (INTERIOR only): every cell the
+// tile reads is written by a step, so no active-bit plane is read.
+template <int K, int ST, bool INTERIOR, int COEF, bool ALLACT>
 __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long gbase, int ia, int ib,
                                          int oc, int od)
 {
     constexpr int C = WV_C;
     constexpr int H = Tile<K>::H, WOUT = Tile<K>::WOUT;
     constexpr unsigned CMASK = (1u << C) - 1u;
+    constexpr bool FAST = INTERIOR;
     static_assert(2 * K * C <= 32, "bit shift register holds 2K rows of C bits");
+    static_assert(INTERIOR || !ALLACT, "ALLACT tiles are interior tiles");
     const Par p(d);
     const Src src(d);
     const int nx = p.nx, ny = p.ny, ldy = p.ldy, ldw = p.ldw;
     const bool wrapi = p.bc == 1, wrapj = (p.bc == 1 || p.bc == 3);
     const int jb = t.cb * WOUT;
-    constexpr bool FAST = INTERIOR;
     const Cols<FAST> L(jb - H + (int)threadIdx.x * C, jb, WOUT, ny, wrapj);
     const int ibeg = t.sb * d.rows_per_tile;
     const int iend = min(ibeg + d.rows_per_tile, nx);
-    const bool has_src = SRC && src.any();
+    const bool has_src = src.any();
     const double *u0 = d.u[ia], *um1 = d.u[ib];
     const uint64_t *act = d.act, *med = d.med;
     const double *tcp = d.tc, *tccf = d.tccf, *tgf = d.tgf;
@@ -75,10 +78,10 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
         const int rm = INTERIOR ? min(max(row, 0), nx - 1) : map_idx(row, nx, wrapi);
         load_row(u0, ldy, rm, L, qu);
         load_row(um1, ldy, rm, L, qp);
-        qb = lane_word(act, ldw, rm, L);
-        // unconditional: a load under a branch makes the compiler drain every
-        // outstanding load (the prefetch ring) where the branch joins
-        qm = lane_word(med, ldw, rm, L);
+        // unconditional loads: a load under a runtime branch makes the compiler
+        // drain every outstanding load (the prefetch ring) where the branch joins
+        qb = ALLACT ? ~0u : lane_word(act, ldw, rm, L);
+        qm = COEF == WV_COEF_TWO ? lane_word(med, ldw, rm, L) : 0u;
     };
     fetch(s0, QU[0], QP[0], QB[0], QM[0]);
     fetch(s0 + 1, QU[1], QP[1], QB[1], QM[1]);
@@ -87,13 +90,16 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
     auto iter = [&](int s, auto phase) {
         constexpr int P = decltype(phase)::value;
         constexpr int PE = (P + 2) % 3, PC = (P + 1) % 3, PW = P;
-        double NU[K + 1][C], NP[K + 1][C];
-        unsigned abK = 0;
+        // levels in decreasing order: level lv overwrites its own slot P (row
+        // s-3 of level lv) after level lv+1 has read it, and reads level lv-1
+        // before level lv-1 is advanced -- no temporaries, no copies
 #pragma unroll
-        for (int lv = 1; lv <= K; lv++) {
+        for (int lv = K; lv >= 1; lv--) {
+            // level lv is needed at rows [ibeg-K+lv, iend+K-lv) only (wave-uniform)
+            if (s < ibeg - K + 3 * lv || s >= iend + K + lv) continue;
             const int r = s - 2 * lv;
-            // rows outside the strip's dependence cone are computed but never used:
-            // INTERIOR tiles only clamp them so that table reads stay in bounds
+            // rows outside the dependence cone of the strip's output are never
+            // used: INTERIOR tiles only clamp them so table reads stay in bounds
             const int rm = INTERIOR ? min(max(r, 0), nx - 1) : map_idx(r, nx, wrapi);
             const bool rowint = INTERIOR || (rm >= 1 && rm <= nx - 2);
             const double(&Xe)[C] = U[lv - 1][PE];
@@ -101,15 +107,15 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
             const double(&Xw)[C] = U[lv - 1][PW];
             const double(&Yc)[C] = Up[lv - 1][PC];
             const int sh = (2 * lv - 1) * C;
-            const unsigned ab = (mreg >> sh) & CMASK;
+            const unsigned ab = ALLACT ? CMASK : (mreg >> sh) & CMASK;
             const unsigned mb = (medreg >> sh) & CMASK;
-            if (lv == K) abK = ab;
             const double cn = from_next(Xc[0]), cp = from_prev(Xc[C - 1]);
             double en = 0, ep = 0, wn = 0, wp = 0;
             if (ST == 9) {
                 en = from_next(Xe[0]); ep = from_prev(Xe[C - 1]);
                 wn = from_next(Xw[0]); wp = from_prev(Xw[C - 1]);
             }
+            double nu[C], np[C];
 #pragma unroll
             for (int c = 0; c < C; c++) {
                 const int cn1 = c < C - 1 ? c + 1 : 0, cp1 = c > 0 ? c - 1 : 0;
@@ -117,7 +123,7 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
                 const double N = (c < C - 1) ? Xc[cn1] : cn;
                 const double S = (c > 0) ? Xc[cp1] : cp;
                 double cc, g;
-                coef_fp64<FIELD>(p, tccf, tgf, rm, L.jm[c], (mb >> c) & 1u, cc, g);
+                coef_fp64<COEF>(p, tccf, tgf, rm, L.jm[c], (mb >> c) & 1u, cc, g);
                 double v;
                 if (rowint && L.colint[c]) {
                     double delta;
@@ -141,32 +147,37 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
                     const long g1 = gbase + lv - 1;
                     if (g1 < src.len) v = add_sources(src, rm, L.jm[c], g1, v);
                 }
-                NU[lv][c] = ((ab >> c) & 1u) ? v : x;  // cells the step never writes keep u
-                NP[lv][c] = x;
+                nu[c] = (ALLACT || ((ab >> c) & 1u)) ? v : x;  // cells the step never writes keep u
+                np[c] = x;
             }
-        }
-        // level K at row s-2K is final
-        const int r = s - 2 * K;
-        if (r >= ibeg && r < iend) {
-            gptr<double> o0 = out0 + (size_t)r * ldy + L.j0;
-            gptr<double> o1 = out1 + (size_t)r * ldy + L.j0;
-            if (FAST && abK == CMASK && L.wcol[0] && L.wcol[C - 1]) {
-                const dvec2 a = {NU[K][0], NU[K][1]};
-                *(gptr<dvec2>)o0 = a;
-                if (K >= 2) {
-                    const dvec2 b = {NP[K][0], NP[K][1]};
-                    *(gptr<dvec2>)o1 = b;
+            if (lv == K) {
+                // level K at row s-2K is final (the live test puts it inside the strip)
+                gptr<double> o0 = out0 + (size_t)r * ldy + L.j0;
+                gptr<double> o1 = out1 + (size_t)r * ldy + L.j0;
+                if (FAST && (ALLACT || ab == CMASK) && L.wcol[0] && L.wcol[C - 1]) {
+                    const dvec2 a = {nu[0], nu[1]};
+                    *(gptr<dvec2>)o0 = a;
+                    if (K >= 2) {
+                        const dvec2 b = {np[0], np[1]};
+                        *(gptr<dvec2>)o1 = b;
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < C; c++)
+                        if (L.wcol[c] && ((ab >> c) & 1u)) {
+                            o0[c] = nu[c];
+                            if (K >= 2) o1[c] = np[c];
+                        }
                 }
             } else {
 #pragma unroll
-                for (int c = 0; c < C; c++)
-                    if (L.wcol[c] && ((abK >> c) & 1u)) {
-                        o0[c] = NU[K][c];
-                        if (K >= 2) o1[c] = NP[K][c];
-                    }
+                for (int c = 0; c < C; c++) {
+                    U[lv < K ? lv : 0][P][c] = nu[c];
+                    Up[lv < K ? lv : 0][P][c] = np[c];
+                }
             }
         }
-        // commit: level 0 from the prefetch ring, levels 1..K-1 from this iteration
+        // level 0 from the prefetch ring, then request row s+3 into the freed slot
 #pragma unroll
         for (int c = 0; c < C; c++) {
             U[0][P][c] = QU[P][c];
@@ -174,15 +185,7 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
         }
         mreg = (mreg << C) | bits_of(QB[P], L);
         medreg = (medreg << C) | bits_of(QM[P], L);
-#pragma unroll
-        for (int lv = 1; lv < K; lv++)
-#pragma unroll
-            for (int c = 0; c < C; c++) {
-                U[lv][P][c] = NU[lv][c];
-                Up[lv][P][c] = NP[lv][c];
-            }
-        // unconditional (past the last needed row it re-reads that cached row):
-        // a conditional load would make the compiler drain the ring each iteration
+        // unconditional (past the last needed row it re-reads that cached row)
         fetch(min(s + 3, slast), QU[P], QP[P], QB[P], QM[P]);
     };
 
@@ -195,32 +198,36 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
     }
 }
 
+// Waves per SIMD the register allocation must allow: the rarely taken
+// boundary path (per-lane column mapping, edge formulas) would otherwise set
+// the register count of the whole kernel.
+template <int K>
+struct Occ {
+    static constexpr int waves = K <= 4 ? 3 : 2;
+};
+
 template <int K, int ST>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64, Occ<K>::waves)
 k_step_fp64(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, const long *gstep,
             long g0, int ia, int ib, int oc, int od)
 {
     const WvTile t = tiles[blockIdx.x];
     if (t.inst < 0) return;  // padding of the XCD interleave
     const WvInst &d = I[t.inst];
-    const int js = t.cb * Tile<K>::WOUT - Tile<K>::H;  // first column of the tile
     const long gbase = (gstep ? *gstep : 0) + g0;
-    // wave-uniform: every column of the tile is an interior column
-    const int L = d.rows_per_tile, ibeg = t.sb * L, iend = min(ibeg + L, d.nx);
-    const bool interior = js >= 1 && js + 64 * WV_C <= d.ny - 1 && ibeg >= K && iend <= d.nx - K;
-    const bool field = d.coef == WV_COEF_FIELD;
-    const bool src = d.src_ptr != nullptr && d.amp_len > 0;
-    if (interior) {
-        if (field) {
-            if (src) run_fp64<K, ST, true, true, true>(d, t, gbase, ia, ib, oc, od);
-            else run_fp64<K, ST, true, true, false>(d, t, gbase, ia, ib, oc, od);
-        } else {
-            if (src) run_fp64<K, ST, true, false, true>(d, t, gbase, ia, ib, oc, od);
-            else run_fp64<K, ST, true, false, false>(d, t, gbase, ia, ib, oc, od);
-        }
-    } else {
-        if (field) run_fp64<K, ST, false, true, true>(d, t, gbase, ia, ib, oc, od);
-        else run_fp64<K, ST, false, false, true>(d, t, gbase, ia, ib, oc, od);
+    const bool interior = t.flags & WV_TILE_INTERIOR;
+    const bool allact = t.flags & WV_TILE_ALLACT;
+    switch (d.coef) {
+#define WV_RUN(COEF)                                                                   \
+    case COEF:                                                                         \
+        if (!interior) run_fp64<K, ST, false, COEF, false>(d, t, gbase, ia, ib, oc, od);              \
+        else if (allact) run_fp64<K, ST, true, COEF, true>(d, t, gbase, ia, ib, oc, od);              \
+        else run_fp64<K, ST, true, COEF, false>(d, t, gbase, ia, ib, oc, od);                         \
+        break;
+        WV_RUN(WV_COEF_UNIFORM)
+        WV_RUN(WV_COEF_TWO)
+        WV_RUN(WV_COEF_FIELD)
+#undef WV_RUN
     }
 }
 
@@ -229,9 +236,14 @@ k_step_fp64(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, cons
 #define WV_CAT2(a, b) a##b
 #define WV_CAT(a, b) WV_CAT2(a, b)
 
-int WV_CAT(wv_launch_fp64_k, WV_K)(int stencil, const WvInst *I, const WvTile *T, int n, const long *gs,
-                                    long g0, int ia, int ib, int oc, int od, hipStream_t s)
+int WV_CAT(wv_launch_fp64_k, WV_K)(int stencil, const WvInst *I, const WvTile *Ti, int ni, const WvTile *Tb,
+                                    int nb, const long *gs, long g0, int ia, int ib, int oc, int od,
+                                    hipStream_t s)
 {
+    // the two tile lists are contiguous in memory: one launch covers both
+    const WvTile *T = ni ? Ti : Tb;
+    const int n = ni + nb;
+    if (!n) return 0;
     if (stencil == 9)
         hipLaunchKernelGGL((k_step_fp64<WV_K, 9>), dim3(n), dim3(64), 0, s, I, T, gs, g0, ia, ib, oc, od);
     else

@@ -25,23 +25,27 @@ namespace {
 
 // INTERIOR: the tile's rows and columns never reach the outer box (bulk
 // formula only, no index wrapping); otherwise every cell is classified.
-template <int K, int ST, bool INTERIOR, bool FIELD, bool SRC>
+// COEF: WV_COEF_UNIFORM / TWO / FIELD.  ALLACT (INTERIOR only): every cell the
+// tile reads is written by a step.  DAMP: the grid has a non-zero KAPPA or
+// tgamma (terms with a zero coefficient are not evaluated).
+template <int K, int ST, bool INTERIOR, int COEF, bool ALLACT, bool DAMP>
 __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long gbase, int ia, int oc)
 {
     constexpr int C = WV_C;
     constexpr int H = Tile<K>::H, WOUT = Tile<K>::WOUT;
     constexpr unsigned CMASK = (1u << C) - 1u;
+    constexpr bool FAST = INTERIOR;
     static_assert(2 * K * C <= 32, "bit shift register holds 2K rows of C bits");
+    static_assert(INTERIOR || !ALLACT, "ALLACT tiles are interior tiles");
     const Par p(d);
     const Src src(d);
     const int nx = p.nx, ny = p.ny, ldy = p.ldy, ldw = p.ldw;
     const bool wrapi = p.bc == 1, wrapj = (p.bc == 1 || p.bc == 3);
     const int jb = t.cb * WOUT;
-    constexpr bool FAST = INTERIOR;
     const Cols<FAST> L(jb - H + (int)threadIdx.x * C, jb, WOUT, ny, wrapj);
     const int ibeg = t.sb * d.rows_per_tile;
     const int iend = min(ibeg + d.rows_per_tile, nx);
-    const bool has_src = SRC && src.any();
+    const bool has_src = src.any();
     const uint64_t *act = d.act, *med = d.med;
     const double *tcp = d.tc, *tccf = d.tccf, *tgf = d.tgf;
     const float *tccf_hi = d.tccf_hi, *tccf_lo = d.tccf_lo, *tgf32 = d.tgf32;
@@ -70,10 +74,10 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
         load_row(hin, ldy, rm, L, qh);
         load_row(lin, ldy, rm, L, ql);
         load_row(vin, ldy, rm, L, qv);
-        qb = lane_word(act, ldw, rm, L);
-        // unconditional: a load under a branch makes the compiler drain every
-        // outstanding load (the prefetch ring) where the branch joins
-        qm = lane_word(med, ldw, rm, L);
+        // unconditional loads: a load under a runtime branch makes the compiler
+        // drain every outstanding load (the prefetch ring) where the branch joins
+        qb = ALLACT ? ~0u : lane_word(act, ldw, rm, L);
+        qm = COEF == WV_COEF_TWO ? lane_word(med, ldw, rm, L) : 0u;
     };
     fetch(s0, QH[0], QL[0], QV[0], QB[0], QM[0]);
     fetch(s0 + 1, QH[1], QL[1], QV[1], QB[1], QM[1]);
@@ -82,13 +86,12 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
     auto iter = [&](int s, auto phase) {
         constexpr int P = decltype(phase)::value;
         constexpr int PE = (P + 2) % 3, PC = (P + 1) % 3, PW = P;
-        float NH[K + 1][C], NL[K + 1][C], NV[K + 1][C];
-        unsigned abK = 0;
+        // levels in decreasing order: each level overwrites its own slot P in
+        // place (see the FP64 kernel)
 #pragma unroll
-        for (int lv = 1; lv <= K; lv++) {
+        for (int lv = K; lv >= 1; lv--) {
+            if (s < ibeg - K + 3 * lv || s >= iend + K + lv) continue;
             const int r = s - 2 * lv;
-            // rows outside the strip's dependence cone are computed but never used:
-            // INTERIOR tiles only clamp them so that table reads stay in bounds
             const int rm = INTERIOR ? min(max(r, 0), nx - 1) : map_idx(r, nx, wrapi);
             const bool rowint = INTERIOR || (rm >= 1 && rm <= nx - 2);
             const float(&he)[C] = Hw[lv - 1][PE];
@@ -99,9 +102,8 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
             const float(&lw)[C] = Lw[lv - 1][PW];
             const float(&vc)[C] = Vw[lv - 1][PC];
             const int sh = (2 * lv - 1) * C;
-            const unsigned ab = (mreg >> sh) & CMASK;
+            const unsigned ab = ALLACT ? CMASK : (mreg >> sh) & CMASK;
             const unsigned mb = (medreg >> sh) & CMASK;
-            if (lv == K) abK = ab;
             const float hcn = from_next(hc[0]), hcp = from_prev(hc[C - 1]);
             const float lcn = from_next(lc[0]), lcp = from_prev(lc[C - 1]);
             float hen = 0, hep = 0, hwn = 0, hwp = 0, len_ = 0, lep = 0, lwn = 0, lwp = 0;
@@ -111,6 +113,7 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
                 len_ = from_next(le[0]); lep = from_prev(le[C - 1]);
                 lwn = from_next(lw[0]); lwp = from_prev(lw[C - 1]);
             }
+            float nh[C], nl[C], nv[C];
 #pragma unroll
             for (int c = 0; c < C; c++) {
                 const int cn = c < C - 1 ? c + 1 : 0, cp = c > 0 ? c - 1 : 0;
@@ -135,18 +138,26 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
                         lap = dh + dl;
                     }
                     float ch, cl, gf;
-                    if (FIELD) {
+                    if (COEF == WV_COEF_FIELD) {
                         const size_t k = (size_t)rm * ldy + L.jm[c];
                         ch = gc(tccf_hi)[k];
                         cl = tccf_lo ? gc(tccf_lo)[k] : 0.0f;
                         gf = tgf32 ? gc(tgf32)[k] : p.tgf0;
+                    } else if (COEF == WV_COEF_UNIFORM) {
+                        ch = p.tcch0;
+                        cl = p.tccl0;
+                        gf = p.tgf0;
                     } else {
                         ch = mbit ? p.tcch1 : p.tcch0;
                         cl = mbit ? p.tccl1 : p.tccl0;
                         gf = mbit ? p.tgf1 : p.tgf0;
                     }
                     const float v0 = vc[c];
-                    float vn = ((v0 + __fmaf_rn(ch, lap, cl * lap)) - kf * h0) - gf * v0;
+                    float vn = v0 + __fmaf_rn(ch, lap, cl * lap);
+                    if (DAMP) {  // a term whose coefficient is zero is not evaluated
+                        if (kf != 0.0f) vn = vn - kf * h0;
+                        vn = (gf != 0.0f) ? vn - gf * v0 : vn;
+                    }
                     if (has_src) {
                         const long g1 = gbase + lv - 1;
                         if (g1 < src.len) vn = add_sources(src, rm, L.jm[c], g1, vn);
@@ -162,7 +173,7 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
                     const double N = (double)hN + (double)lN;
                     const double S = (double)hS + (double)lS;
                     double cc, g;
-                    coef_fp64<FIELD>(p, tccf, tgf, rm, L.jm[c], mbit, cc, g);
+                    coef_fp64<COEF>(p, tccf, tgf, rm, L.jm[c], mbit, cc, g);
                     const double tcv = gc(tcp)[(size_t)rm * ldy + L.jm[c]];
                     double out = edge_fp64(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv);
                     if (has_src) {
@@ -173,28 +184,34 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
                     ol = (float)(out - (double)oh);
                     ov = (float)(out - x);
                 }
-                const bool act = (ab >> c) & 1u;  // cells the step never writes keep their state
-                NH[lv][c] = act ? oh : hc[c];
-                NL[lv][c] = act ? ol : lc[c];
-                NV[lv][c] = act ? ov : vc[c];
+                const bool live = ALLACT || ((ab >> c) & 1u);  // cells the step never writes keep their state
+                nh[c] = live ? oh : hc[c];
+                nl[c] = live ? ol : lc[c];
+                nv[c] = live ? ov : vc[c];
             }
-        }
-        const int r = s - 2 * K;
-        if (r >= ibeg && r < iend) {
-            const size_t ro = (size_t)r * ldy + L.j0;
-            if (FAST && abK == CMASK && L.wcol[0] && L.wcol[C - 1]) {
-                const fvec2 a = {NH[K][0], NH[K][1]}, b = {NL[K][0], NL[K][1]}, e = {NV[K][0], NV[K][1]};
-                *(gptr<fvec2>)(hout + ro) = a;
-                *(gptr<fvec2>)(lout + ro) = b;
-                *(gptr<fvec2>)(vout + ro) = e;
+            if (lv == K) {
+                const size_t ro = (size_t)r * ldy + L.j0;
+                if (FAST && (ALLACT || ab == CMASK) && L.wcol[0] && L.wcol[C - 1]) {
+                    const fvec2 a = {nh[0], nh[1]}, b = {nl[0], nl[1]}, e = {nv[0], nv[1]};
+                    *(gptr<fvec2>)(hout + ro) = a;
+                    *(gptr<fvec2>)(lout + ro) = b;
+                    *(gptr<fvec2>)(vout + ro) = e;
+                } else {
+#pragma unroll
+                    for (int c = 0; c < C; c++)
+                        if (L.wcol[c] && ((ab >> c) & 1u)) {
+                            hout[ro + c] = nh[c];
+                            lout[ro + c] = nl[c];
+                            vout[ro + c] = nv[c];
+                        }
+                }
             } else {
 #pragma unroll
-                for (int c = 0; c < C; c++)
-                    if (L.wcol[c] && ((abK >> c) & 1u)) {
-                        hout[ro + c] = NH[K][c];
-                        lout[ro + c] = NL[K][c];
-                        vout[ro + c] = NV[K][c];
-                    }
+                for (int c = 0; c < C; c++) {
+                    Hw[lv < K ? lv : 0][P][c] = nh[c];
+                    Lw[lv < K ? lv : 0][P][c] = nl[c];
+                    Vw[lv < K ? lv : 0][P][c] = nv[c];
+                }
             }
         }
 #pragma unroll
@@ -205,16 +222,7 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
         }
         mreg = (mreg << C) | bits_of(QB[P], L);
         medreg = (medreg << C) | bits_of(QM[P], L);
-#pragma unroll
-        for (int lv = 1; lv < K; lv++)
-#pragma unroll
-            for (int c = 0; c < C; c++) {
-                Hw[lv][P][c] = NH[lv][c];
-                Lw[lv][P][c] = NL[lv][c];
-                Vw[lv][P][c] = NV[lv][c];
-            }
-        // unconditional (past the last needed row it re-reads that cached row):
-        // a conditional load would make the compiler drain the ring each iteration
+        // unconditional (past the last needed row it re-reads that cached row)
         fetch(min(s + 3, slast), QH[P], QL[P], QV[P], QB[P], QM[P]);
     };
 
@@ -227,31 +235,39 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
     }
 }
 
+// Waves per SIMD the register allocation must allow: the rarely taken
+// boundary path (per-lane column mapping, edge formulas) would otherwise set
+// the register count of the whole kernel.
+template <int K>
+struct Occ {
+    static constexpr int waves = K <= 4 ? 3 : 2;
+};
+
 template <int K, int ST>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64, Occ<K>::waves)
 k_step_split(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, const long *gstep,
-             long g0, int ia, int oc)
+            long g0, int ia, int oc)
 {
     const WvTile t = tiles[blockIdx.x];
     if (t.inst < 0) return;  // padding of the XCD interleave
     const WvInst &d = I[t.inst];
-    const int js = t.cb * Tile<K>::WOUT - Tile<K>::H;
     const long gbase = (gstep ? *gstep : 0) + g0;
-    const int L = d.rows_per_tile, ibeg = t.sb * L, iend = min(ibeg + L, d.nx);
-    const bool interior = js >= 1 && js + 64 * WV_C <= d.ny - 1 && ibeg >= K && iend <= d.nx - K;
-    const bool field = d.coef == WV_COEF_FIELD;
-    const bool src = d.src_ptr != nullptr && d.amp_len > 0;
-    if (interior) {
-        if (field) {
-            if (src) run_split<K, ST, true, true, true>(d, t, gbase, ia, oc);
-            else run_split<K, ST, true, true, false>(d, t, gbase, ia, oc);
-        } else {
-            if (src) run_split<K, ST, true, false, true>(d, t, gbase, ia, oc);
-            else run_split<K, ST, true, false, false>(d, t, gbase, ia, oc);
-        }
-    } else {
-        if (field) run_split<K, ST, false, true, true>(d, t, gbase, ia, oc);
-        else run_split<K, ST, false, false, true>(d, t, gbase, ia, oc);
+    const bool interior = t.flags & WV_TILE_INTERIOR;
+    const bool allact = t.flags & WV_TILE_ALLACT;
+    const bool damp = d.damped;
+    switch (d.coef) {
+#define WV_RUN(COEF)                                                                          \
+    case COEF:                                                                                \
+        if (!interior) run_split<K, ST, false, COEF, false, true>(d, t, gbase, ia, oc);               \
+        else if (allact && !damp) run_split<K, ST, true, COEF, true, false>(d, t, gbase, ia, oc);     \
+        else if (allact) run_split<K, ST, true, COEF, true, true>(d, t, gbase, ia, oc);               \
+        else if (!damp) run_split<K, ST, true, COEF, false, false>(d, t, gbase, ia, oc);              \
+        else run_split<K, ST, true, COEF, false, true>(d, t, gbase, ia, oc);                          \
+        break;
+        WV_RUN(WV_COEF_UNIFORM)
+        WV_RUN(WV_COEF_TWO)
+        WV_RUN(WV_COEF_FIELD)
+#undef WV_RUN
     }
 }
 
@@ -260,9 +276,13 @@ k_step_split(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, con
 #define WV_CAT2(a, b) a##b
 #define WV_CAT(a, b) WV_CAT2(a, b)
 
-int WV_CAT(wv_launch_split_k, WV_K)(int stencil, const WvInst *I, const WvTile *T, int n, const long *gs,
-                                     long g0, int ia, int oc, hipStream_t s)
+int WV_CAT(wv_launch_split_k, WV_K)(int stencil, const WvInst *I, const WvTile *Ti, int ni, const WvTile *Tb,
+                                     int nb, const long *gs, long g0, int ia, int oc, hipStream_t s)
 {
+    // the two tile lists are contiguous in memory: one launch covers both
+    const WvTile *T = ni ? Ti : Tb;
+    const int n = ni + nb;
+    if (!n) return 0;
     if (stencil == 9)
         hipLaunchKernelGGL((k_step_split<WV_K, 9>), dim3(n), dim3(64), 0, s, I, T, gs, g0, ia, oc);
     else

@@ -26,11 +26,13 @@ FLAG_TABLES_FIXED = 1
 EXPORTS = [
     "wv_params_default", "wv_create", "wv_destroy", "wv_last_error", "wv_set_tables",
     "wv_set_state", "wv_set_state3", "wv_get_state", "wv_get_state3", "wv_set_sources",
-    "wv_step", "wv_step_timed", "wv_sync", "wv_steps_done", "wv_energy", "wv_get_tiling",
+    "wv_step", "wv_step_timed", "wv_sync", "wv_steps_done", "wv_last_kernel_ms", "wv_energy",
+    "wv_get_tiling",
     "wv_set_tiling", "wv_autotune", "wv_evolve_wave_half_rows", "wv_evolve_wave_rows",
     "wv_evolve_wave_half_flat", "wv_evolve_wave_flat", "wv_batch_create", "wv_batch_destroy",
     "wv_batch_last_error", "wv_batch_set_tables", "wv_batch_set_state", "wv_batch_get_state",
-    "wv_batch_step", "wv_batch_step_timed", "wv_batch_energy", "wv_batch_set_tiling",
+    "wv_batch_step", "wv_batch_step_timed", "wv_batch_last_kernel_ms", "wv_batch_energy",
+    "wv_batch_set_tiling",
     "wv_batch_get_tiling",
 ]
 
@@ -86,6 +88,8 @@ def load():
         "wv_step_timed": (ctypes.c_int, [vp, ctypes.c_long, _D, ctypes.POINTER(ctypes.c_long)]),
         "wv_sync": (ctypes.c_int, [vp]),
         "wv_steps_done": (ctypes.c_long, [vp]),
+        "wv_last_kernel_ms": (ctypes.c_int, [vp, _D, _D]),
+        "wv_batch_last_kernel_ms": (ctypes.c_int, [vp, _D, _D]),
         "wv_energy": (ctypes.c_int, [vp, _D]),
         "wv_get_tiling": (ctypes.c_int, [vp, _I, _I, _I]),
         "wv_set_tiling": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
@@ -219,6 +223,12 @@ class Wave:
                   "wv_step_timed")
         return ms.value, nl.value
 
+    def kernel_ms(self):
+        """(interior, boundary) kernel time of the last step_timed, ms"""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._chk(self._L.wv_last_kernel_ms(self._h, ctypes.byref(a), ctypes.byref(b)), "wv_last_kernel_ms")
+        return a.value, b.value
+
     def sync(self):
         self._chk(self._L.wv_sync(self._h), "wv_sync")
 
@@ -350,6 +360,12 @@ class Batch:
         self._chk(self._L.wv_batch_step_timed(self._h, int(n), ctypes.byref(ms), ctypes.byref(nl)),
                   "wv_batch_step_timed")
         return ms.value, nl.value
+
+    def kernel_ms(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._chk(self._L.wv_batch_last_kernel_ms(self._h, ctypes.byref(a), ctypes.byref(b)),
+                  "wv_batch_last_kernel_ms")
+        return a.value, b.value
 
     def energy(self):
         e = np.empty(self.n)
