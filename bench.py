@@ -61,21 +61,21 @@ def run_form(wh, w, form, steps, warmup, tb_depth=0, rows=0, tune=False):
                 u=u, k=k, L=L, cols=cols)
 
 
-def cpu_baseline(w, target_s=12.0):
-    """the oracle on the host cores, a bounded sample of the FP64 workload"""
+def cpu_baseline(w, target_s=10.0, chunk=40):
+    """the oracle on the host cores, a bounded sample of the FP64 workload:
+    chunks of steps until target_s seconds of CPU work have been timed"""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     pw = w["params"]
     p = pyoracle.make_params(pw["nx"], pw["ny"], bcond=pw["bcond"])
     args = (w["xy_in"], w["tcc"], w["tc"], w["tgamma"])
     phi, psi, tmp = pyoracle.run(p, w["phi"], w["psi"], 2, *args)   # touch the buffers
-    t0 = time.perf_counter()
-    phi, psi, tmp = pyoracle.run(p, phi, psi, 8, *args, tmp=tmp)
-    probe = (time.perf_counter() - t0) / 8
-    n = int(max(8, min(4000, target_s / max(probe, 1e-6))))
-    t0 = time.perf_counter()
-    pyoracle.run(p, phi, psi, n, *args, tmp=tmp)
-    dt = time.perf_counter() - t0
+    n, dt = 0, 0.0
+    while dt < target_s:
+        t0 = time.perf_counter()
+        phi, psi, tmp = pyoracle.run(p, phi, psi, chunk, *args, tmp=tmp)
+        dt += time.perf_counter() - t0
+        n += chunk
     cells = pw["nx"] * pw["ny"]
     return dict(value=cells * n / dt / 1e9, unit="Gcell-updates/s", cores=pyoracle.num_threads(),
                 kind="port",
@@ -84,11 +84,12 @@ def cpu_baseline(w, target_s=12.0):
                        % (n, pw["nx"], pw["ny"], dt))
 
 
-def traffic_from_profile(k, L):
-    """HBM bytes per launch of the split kernel from the committed rocprofv3
-    --pmc summary, when it was measured with the same tiling"""
-    f = os.path.join(ROOT, "profiles", "traffic_split_c3_4096.json")
-    if not os.path.exists(f):
+def traffic_from_profile(form, k, L, size):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when it was measured on this
+    workload with the same tiling; otherwise None"""
+    f = os.path.join(ROOT, "profiles", "traffic_%s_c3_4096.json" % ("split" if form == "f32split" else "fp64"))
+    if size != 4096 or not os.path.exists(f):
         return None
     d = json.load(open(f))
     if d.get("tb_depth") == k and d.get("rows_per_tile") == L:
@@ -131,17 +132,15 @@ def main():
         r = run_form(wh, w, form, args.steps, args.warmup, args.tb_depth, args.rows, args.tune)
         r["gcells"] = cells * args.steps / r["wall"] / 1e9
         r["gcells_dev"] = cells * args.steps / (r["ms_dev"] * 1e-3) / 1e9
-        # the dominant kernel is the interior-tile kernel; each fused block of k
-        # steps is one launch of it (plus one of the boundary-tile kernel)
+        # the stepping kernel: one launch per fused block of k steps, timed by
+        # a HIP event pair around every launch on the stepping stream
         r["avg_launch_ms"] = r["ms_int"] / r["launches"]
-        r["avg_bnd_ms"] = r["ms_bnd"] / r["launches"]
-        # algorithmic bytes per launch: B_alg x cells x fused steps per launch,
-        # scaled to the interior kernel's share of the cells
+        # algorithmic bytes per launch: B_alg x cells x fused steps per launch
         r["alg_bytes_per_launch"] = B_ALG[name] * cells * r["k"]
-        r["achieved_Bps"] = r["alg_bytes_per_launch"] / ((r["avg_launch_ms"] + r["avg_bnd_ms"]) * 1e-3)
-        log("%s: %.1f Gcell/s (wall), %.1f (events), k=%d rows=%d, %d launches, %.4f + %.4f ms/launch"
-            % (name, r["gcells"], r["gcells_dev"], r["k"], r["L"], r["launches"], r["avg_launch_ms"],
-               r["avg_bnd_ms"]))
+        r["achieved_Bps"] = r["alg_bytes_per_launch"] / (r["avg_launch_ms"] * 1e-3)
+        r["traffic"] = traffic_from_profile(name, r["k"], r["L"], args.size)
+        log("%s: %.1f Gcell/s (wall), %.1f (events), k=%d rows=%d, %d launches, %.4f ms/launch"
+            % (name, r["gcells"], r["gcells_dev"], r["k"], r["L"], r["launches"], r["avg_launch_ms"]))
         res[name] = r
 
     head = res.get("f32split") or res["fp64"]
@@ -176,13 +175,12 @@ def main():
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": head["achieved_Bps"] / HBM_PEAK,
-            "traffic": traffic_from_profile(head["k"], head["L"]) if hname == "f32split" else None,
-            "kernel": ("k_step_split_int<%d,5> + k_step_split_bnd<%d,5>" if hname == "f32split"
-                       else "k_step_fp64_int<%d,5> + k_step_fp64_bnd<%d,5>") % (head["k"], head["k"]),
+            "traffic": head["traffic"],
+            "traffic_frac": (head["traffic"] / (head["avg_launch_ms"] * 1e-3) / HBM_PEAK
+                             if head["traffic"] else None),
+            "kernel": ("k_step_split<%d,5>" if hname == "f32split" else "k_step_fp64<%d,5>") % head["k"],
             "alg_bytes_per_launch": head["alg_bytes_per_launch"],
-            "avg_launch_ms": head["avg_launch_ms"] + head["avg_bnd_ms"],
-            "avg_launch_ms_interior": head["avg_launch_ms"],
-            "avg_launch_ms_boundary": head["avg_bnd_ms"],
+            "avg_launch_ms": head["avg_launch_ms"],
             "b_alg_per_cell_update": B_ALG[hname],
         },
     }
@@ -193,6 +191,8 @@ def main():
             "gcells_events": round(f["gcells_dev"], 3),
             "tb_depth": f["k"], "rows_per_tile": f["L"],
             "roofline_frac": f["achieved_Bps"] / HBM_PEAK,
+            "traffic_bytes_per_launch": f["traffic"],
+            "avg_launch_ms": f["avg_launch_ms"],
             "energy_drift": (f["e1"] - f["e0"]) / f["e0"],
         }
     if "fp64" in res and "f32split" in res:

@@ -94,9 +94,10 @@ int fail(Engine *e, int code, const char *fmt, ...)
 
 int round_up(int a, int b) { return (a + b - 1) / b * b; }
 
-// defaults measured on MI355X at 4096^2 (both forms): 4 fused steps, 12-row strips
-int default_depth(const Engine *) { return 4; }
-int default_rows(const Engine *) { return 12; }
+// defaults measured on MI355X at 4096^2: split form 4 fused steps in 12-row
+// strips, FP64 form 6 fused steps in 16-row strips
+int default_depth(const Engine *e) { return e->form == WV_FORM_F32SPLIT ? 4 : 6; }
+int default_rows(const Engine *e) { return e->form == WV_FORM_F32SPLIT ? 12 : 16; }
 int depth(const Engine *e) { return e->K > 0 ? e->K : default_depth(e); }
 int rows(const Engine *e) { return e->L > 0 ? e->L : default_rows(e); }
 

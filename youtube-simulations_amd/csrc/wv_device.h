@@ -166,8 +166,8 @@ struct Cols {
 
 // The 32-bit word of a bit plane that holds the lane's WV_C bits, returned
 // raw (bits at position j0 & 31) so that a prefetched word is not touched
-// until it is consumed; bits_of() extracts them.  j0 is even, so the WV_C = 2
-// bits never straddle a word.  Lanes with mapped columns assemble their bits
+// until it is consumed; bits_of() extracts them.  j0 is a multiple of WV_C
+// (1 or 2), so the lane's bits never straddle a word.  Lanes with mapped columns assemble their bits
 // at the same position.
 template <bool FAST>
 __device__ __forceinline__ unsigned lane_word(const uint64_t *plane, int ldw, int rm, const Cols<FAST> &L)
@@ -194,11 +194,15 @@ __device__ __forceinline__ void load_row(const double *base, int ldy, int rm, co
                                          double (&out)[WV_C])
 {
     gptr<const double> row = gc(base) + (size_t)rm * ldy;
-    static_assert(WV_C == 2, "vector row loads assume 2 cells per lane");
     if (FAST || L.inside) {
-        const dvec2 v = *(gptr<const dvec2>)(row + L.j0);
-        out[0] = v.x;
-        out[1] = v.y;
+        if (WV_C == 2) {
+            const dvec2 v = *(gptr<const dvec2>)(row + L.j0);
+            out[0] = v.x;
+            out[WV_C - 1] = v.y;
+        } else {
+#pragma unroll
+            for (int c = 0; c < WV_C; c++) out[c] = row[L.j0 + c];
+        }
     } else {
 #pragma unroll
         for (int c = 0; c < WV_C; c++) out[c] = row[L.jm[c]];
@@ -211,9 +215,14 @@ __device__ __forceinline__ void load_row(const float *base, int ldy, int rm, con
 {
     gptr<const float> row = gc(base) + (size_t)rm * ldy;
     if (FAST || L.inside) {
-        const fvec2 v = *(gptr<const fvec2>)(row + L.j0);
-        out[0] = v.x;
-        out[1] = v.y;
+        if (WV_C == 2) {
+            const fvec2 v = *(gptr<const fvec2>)(row + L.j0);
+            out[0] = v.x;
+            out[WV_C - 1] = v.y;
+        } else {
+#pragma unroll
+            for (int c = 0; c < WV_C; c++) out[c] = row[L.j0 + c];
+        }
     } else {
 #pragma unroll
         for (int c = 0; c < WV_C; c++) out[c] = row[L.jm[c]];

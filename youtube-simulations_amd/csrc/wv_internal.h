@@ -26,7 +26,9 @@
 #define WV_COEF_FIELD 2
 
 // cells per lane along j in the stepping kernels (a wave covers 64*WV_C columns)
-#define WV_C 2
+#ifndef WV_C
+#define WV_C 1
+#endif
 
 struct WvInst {
     // FP64 form: u[0..3]; split form: hi[2], lo[2], v[2]

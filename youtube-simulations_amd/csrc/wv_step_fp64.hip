@@ -154,11 +154,11 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
                 // level K at row s-2K is final (the live test puts it inside the strip)
                 gptr<double> o0 = out0 + (size_t)r * ldy + L.j0;
                 gptr<double> o1 = out1 + (size_t)r * ldy + L.j0;
-                if (FAST && (ALLACT || ab == CMASK) && L.wcol[0] && L.wcol[C - 1]) {
-                    const dvec2 a = {nu[0], nu[1]};
+                if (C == 2 && FAST && (ALLACT || ab == CMASK) && L.wcol[0] && L.wcol[C - 1]) {
+                    const dvec2 a = {nu[0], nu[C - 1]};
                     *(gptr<dvec2>)o0 = a;
                     if (K >= 2) {
-                        const dvec2 b = {np[0], np[1]};
+                        const dvec2 b = {np[0], np[C - 1]};
                         *(gptr<dvec2>)o1 = b;
                     }
                 } else {

@@ -191,8 +191,8 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
             }
             if (lv == K) {
                 const size_t ro = (size_t)r * ldy + L.j0;
-                if (FAST && (ALLACT || ab == CMASK) && L.wcol[0] && L.wcol[C - 1]) {
-                    const fvec2 a = {nh[0], nh[1]}, b = {nl[0], nl[1]}, e = {nv[0], nv[1]};
+                if (C == 2 && FAST && (ALLACT || ab == CMASK) && L.wcol[0] && L.wcol[C - 1]) {
+                    const fvec2 a = {nh[0], nh[C - 1]}, b = {nl[0], nl[C - 1]}, e = {nv[0], nv[C - 1]};
                     *(gptr<fvec2>)(hout + ro) = a;
                     *(gptr<fvec2>)(lout + ro) = b;
                     *(gptr<fvec2>)(vout + ro) = e;
