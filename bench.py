@@ -97,6 +97,129 @@ def traffic_from_profile(form, k, L, size):
     return None
 
 
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def run_single(wh, w, form, nsteps, energy_every=0, tb_depth=0, rows=0):
+    """one grid through nsteps; energy sampled every energy_every steps (the
+    samples are part of the timed work); returns state, energies, time"""
+    W = wh.Wave(w["params"], form=form, tb_depth=tb_depth, rows_per_tile=rows)
+    W.load(w)
+    W.sync()
+    es = [W.energy()]
+    t0 = time.perf_counter()
+    if energy_every:
+        for _ in range(nsteps // energy_every):
+            W.step(energy_every)
+            es.append(W.energy())
+    else:
+        W.step(nsteps)
+        W.sync()
+    dt = time.perf_counter() - t0
+    u, um = W.get_state()
+    k, L, _ = W.tiling()
+    W.close()
+    return dict(u=u, um=um, e=np.array(es), dt=dt, k=k, L=L)
+
+
+def configs_report(wh, workloads, which):
+    """every configuration in both forms: throughput, rel-L2 of the split form
+    against the FP64 form, energy drift (the FP64 form is pinned bit-exact to
+    the reference's evolve_wave_half by the test suite)"""
+    import hashlib
+    rep = {}
+    if "c1" in which:
+        w = workloads.c1()
+        dig = json.load(open(os.path.join(ROOT, "tests", "golden", "c1_digests.json")))
+        W = wh.Wave(w["params"])
+        W.load(w)
+        ok = True
+        for n in range(100, 1001, 100):
+            W.step(100)
+            phi, _ = W.get_state()
+            ok &= hashlib.sha256(np.ascontiguousarray(phi).tobytes()).hexdigest() == dig["phi"][str(n)]
+        W.close()
+        f = run_single(wh, w, wh.FORM_FP64, 1000)
+        s = run_single(wh, w, wh.FORM_F32SPLIT, 1000)
+        cells = w["params"]["nx"] * w["params"]["ny"]
+        cpu = cpu_baseline(w, target_s=4.0, chunk=100)
+        rep["c1"] = dict(grid=[1280, 720], steps=1000, fp64_bitexact_vs_reference=bool(ok),
+                         gcells_fp64=cells * 1000 / f["dt"] / 1e9, gcells_split=cells * 1000 / s["dt"] / 1e9,
+                         rel_l2_split=rel_l2(s["u"], f["u"]),
+                         energy_drift_fp64=float((f["e"][-1] - f["e"][0]) / f["e"][0]),
+                         energy_drift_split=float((s["e"][-1] - s["e"][0]) / s["e"][0]),
+                         cpu_gcells=cpu["value"], cpu_cores=cpu["cores"])
+        log("c1", rep["c1"])
+    if "c2" in which:
+        w = workloads.c2()
+        cells = w["params"]["nx"] * w["params"]["ny"]
+        f = run_single(wh, w, wh.FORM_FP64, 5000, 50)
+        s = run_single(wh, w, wh.FORM_F32SPLIT, 5000, 50)
+        rep["c2"] = dict(grid=[2560, 1440], steps=5000, stencil=9, energy_every=50,
+                         gcells_fp64=cells * 5000 / f["dt"] / 1e9, gcells_split=cells * 5000 / s["dt"] / 1e9,
+                         rel_l2_split=rel_l2(s["u"], f["u"]),
+                         energy_series_max_rel_diff=float(np.abs(s["e"] - f["e"]).max() / np.abs(f["e"]).max()),
+                         energy_final_fp64=float(f["e"][-1]), energy_final_split=float(s["e"][-1]))
+        log("c2", rep["c2"])
+    if "c3" in which:
+        w = workloads.c3()
+        cells = w["params"]["nx"] * w["params"]["ny"]
+        f = run_single(wh, w, wh.FORM_FP64, 10000)
+        s = run_single(wh, w, wh.FORM_F32SPLIT, 10000)
+        rep["c3"] = dict(grid=[4096, 4096], steps=10000,
+                         gcells_fp64=cells * 10000 / f["dt"] / 1e9, gcells_split=cells * 10000 / s["dt"] / 1e9,
+                         rel_l2_split=rel_l2(s["u"], f["u"]),
+                         energy_drift_fp64=float((f["e"][-1] - f["e"][0]) / f["e"][0]),
+                         energy_drift_split=float((s["e"][-1] - s["e"][0]) / s["e"][0]))
+        log("c3", rep["c3"])
+    if "c4" in which:
+        ws = workloads.c4_instances()
+        cells = sum(w["params"]["nx"] * w["params"]["ny"] for w in ws)
+        out = {}
+        for name, form in (("fp64", wh.FORM_FP64), ("split", wh.FORM_F32SPLIT)):
+            B = wh.Batch([w["params"] for w in ws], form=form)
+            best = None
+            # each pass restarts from the initial state; the rotation of the
+            # buffers returns to a captured (nsteps, rotation) graph within a
+            # few passes, so the fastest pass is a pure graph replay
+            for k, w in enumerate(ws):
+                B.load(k, w)
+            for _ in range(5):
+                for k, w in enumerate(ws):
+                    B.set_state(k, w["phi"], w["psi"])   # keeps the tables and the graphs
+                e0 = B.energy()
+                t0 = time.perf_counter()
+                B.step(2000)
+                e1 = B.energy()   # waits for the stream
+                dt = time.perf_counter() - t0
+                best = dt if best is None else min(best, dt)
+            states = [B.get_state(k)[0] for k in range(len(ws))]
+            B.close()
+            out[name] = dict(e0=e0, e1=e1, u=states, dt=best)
+        rels = [rel_l2(a, b) for a, b in zip(out["split"]["u"], out["fp64"]["u"])]
+        dr = lambda o: (o["e1"] - o["e0"]) / o["e0"]
+        rep["c4"] = dict(instances=len(ws), total_cells=cells, steps=2000,
+                         gcells_fp64=cells * 2000 / out["fp64"]["dt"] / 1e9,
+                         gcells_split=cells * 2000 / out["split"]["dt"] / 1e9,
+                         rel_l2_split_max=float(max(rels)), rel_l2_split_median=float(np.median(rels)),
+                         energy_drift_fp64_median=float(np.median(dr(out["fp64"]))),
+                         energy_drift_diff_max=float(np.abs(dr(out["split"]) - dr(out["fp64"])).max()))
+        log("c4", rep["c4"])
+    if "c5" in which:
+        w = workloads.c5()
+        cells = w["params"]["nx"] * w["params"]["ny"]
+        f = run_single(wh, w, wh.FORM_FP64, 20000, 100)
+        s = run_single(wh, w, wh.FORM_F32SPLIT, 20000, 100)
+        rep["c5"] = dict(grid=[8192, 8192], steps=20000, stencil=9, energy_every=100, sources=20,
+                         gcells_fp64=cells * 20000 / f["dt"] / 1e9, gcells_split=cells * 20000 / s["dt"] / 1e9,
+                         rel_l2_split=rel_l2(s["u"], f["u"]),
+                         energy_series_max_rel_diff=float(np.abs(s["e"] - f["e"]).max() / np.abs(f["e"]).max()),
+                         tiling_fp64=[f["k"], f["L"]], tiling_split=[s["k"], s["L"]])
+        log("c5", rep["c5"])
+    return rep
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,14 +232,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--only-fp64", action="store_true")
+    ap.add_argument("--configs", default="", help="comma list of c1..c5: run those workloads in both "
+                    "forms and print their report instead of the benchmark line")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (every run here uses a "
-              "single MI355X); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the stepper runs each grid on a single MI355X and has "
+              "no multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import wavehip as wh
     from wavehip import workloads
+
+    if args.configs:
+        rep = configs_report(wh, workloads, args.configs.split(","))
+        print(json.dumps(dict(configs=rep)), flush=True)
+        return
 
     log("building workload %d^2 ..." % args.size)
     w = workloads.c3(n=args.size, nsteps=args.steps)

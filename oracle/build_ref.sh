@@ -6,8 +6,8 @@
 # then xy_to_ij (sub_wave.c:412-422), oscillating_bc (sub_wave.c:11965-12162),
 # compute_energy (wave_common.c:453-472) and the stepper (wave_billiard.c:302-631
 # or wave_3d.c:600-1007), then oracle/ref_harness.c.  Flags follow makefile:1-4
-# (gcc -O3 -fopenmp) plus -ffp-contract=off.  Needs /root/reference; the GPU
-# box never runs this.
+# (gcc -O3 -fopenmp) plus -ffp-contract=off.  Needs the reference sources at
+# /root/reference; machines without them use the oracle alone.
 set -euo pipefail
 REF=${REF:-/root/reference}
 HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"

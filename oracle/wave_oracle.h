@@ -2,8 +2,8 @@
  * wave_oracle.h -- CPU oracle for the 2D wave leapfrog stepper.
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
- * calls this code: it is the checker used by tests/, by the smoke check and
- * by the CPU-baseline leg of the benchmark.
+ * calls this code: tests/, the smoke check and the CPU-baseline leg of the
+ * benchmark use it to check and time against.
  *
  * It restates, in plain C + OpenMP, the reference's masked leapfrog update
  *   evolve_wave_half   wave_billiard.c:302-610  (row-pointer layout)

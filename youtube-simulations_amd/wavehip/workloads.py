@@ -230,14 +230,13 @@ def c5(n=8192, nsteps=20000, seed=5, nsrc=20):
     box = (-2.0, 2.0, -2.0, 2.0)
     rng = SplitMix64(seed)
     x, y = grid_xy(n, n, *box)
-    X, Y = np.meshgrid(x, y, indexing="ij")
     s = np.zeros((n, n), np.float64)
     for _ in range(32):
         xc, yc = rng.uniform(-2, 2), rng.uniform(-2, 2)
         w = rng.uniform(0.2, 0.8)
         a = rng.uniform(-1.0, 1.0)
-        s += a * np.exp(-((X - xc) ** 2 + (Y - yc) ** 2) / (w * w))
-    del X, Y
+        # separable: exp(-(dx^2 + dy^2)/w^2) = exp(-dx^2/w^2) * exp(-dy^2/w^2)
+        s += np.outer(a * np.exp(-((x - xc) ** 2) / (w * w)), np.exp(-((y - yc) ** 2) / (w * w)))
     s = (s - s.min()) / (s.max() - s.min())
     c = 0.5 + s
     del s
