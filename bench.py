@@ -117,6 +117,8 @@ def run_single(wh, w, form, nsteps, energy_every=0, tb_depth=0, rows=0):
         W.step(nsteps)
         W.sync()
     dt = time.perf_counter() - t0
+    if not energy_every:
+        es.append(W.energy())
     u, um = W.get_state()
     k, L, _ = W.tiling()
     W.close()

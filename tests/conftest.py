@@ -10,6 +10,12 @@ for sub in ("oracle", "youtube-simulations_amd", os.path.join("tests", "golden")
         sys.path.insert(0, p)
 
 
+def say(*a):
+    """progress line naming the case, flushed before the GPU work starts"""
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a GPU (MI355X) and libwavehip.so")
 

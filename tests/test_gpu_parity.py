@@ -14,15 +14,11 @@ import sys
 import numpy as np
 import pytest
 
+from conftest import say
 from gen_inputs import KAPPAS, case_inputs, small_cases
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-
-
-def say(*a):
-    print(*a, flush=True)
-    sys.stdout.flush()
 
 
 def bits_equal(a, b):
