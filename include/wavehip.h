@@ -68,6 +68,12 @@ typedef struct wv_params {
 
 /* drop-in entry points upload the tables on the first call only */
 #define WV_FLAG_TABLES_FIXED 1
+/* FP64 form: evaluate -KAPPA*x - tgamma*(x-y) at bulk cells even when KAPPA and
+ * every tgamma are 0.  Without the flag those zero terms are left out for states
+ * with no -0, infinity or NaN (checked at wv_set_state*), which gives the same
+ * bits for every finite field; the flag also reproduces the reference's NaN
+ * where a field that overflows meets a zero coefficient. */
+#define WV_FLAG_FULL_FORMULA 2
 
 /* Fill *p with the reference's defaults for an nx x ny grid (B_COND 0, 5-point,
  * FP64, all constants 0, e_scale 1, tb_shift nx, tuned tiling). */
@@ -104,10 +110,12 @@ int wv_step(wv_ctx *c, long nsteps);
 int wv_step_timed(wv_ctx *c, long nsteps, double *ms, long *launches);
 int wv_sync(wv_ctx *c);
 long wv_steps_done(const wv_ctx *c);
-/* Kernel time of the last wv_step_timed: the summed durations of its stepping
- * launches, each bracketed by a HIP event pair on the context stream (one
- * launch per fused block of steps covers every tile; boundary_ms is 0). */
-int wv_last_kernel_ms(const wv_ctx *c, double *interior_ms, double *boundary_ms);
+/* Kernel time of the last wv_step_timed: one stepping launch in eight (the
+ * first included) is bracketed by a HIP event pair on the context stream;
+ * *avg_ms is the average duration of those at the full fused depth and
+ * *timed_launches their number.  A launch covers every tile for one fused
+ * block of steps. */
+int wv_last_kernel_ms(const wv_ctx *c, double *avg_ms, double *timed_launches);
 
 /* Total energy: compute_energy (wave_common.c:453-472) per cell with c^2 from
  * tcc, summed in FP64 in a fixed order (64 lane-strided partials per row, a
@@ -156,7 +164,7 @@ int wv_batch_set_state(wv_batch *b, int k, const double *phi, const double *psi)
 int wv_batch_get_state(wv_batch *b, int k, double *phi, double *psi);
 int wv_batch_step(wv_batch *b, long nsteps);
 int wv_batch_step_timed(wv_batch *b, long nsteps, double *ms, long *launches);
-int wv_batch_last_kernel_ms(const wv_batch *b, double *interior_ms, double *boundary_ms);
+int wv_batch_last_kernel_ms(const wv_batch *b, double *avg_ms, double *timed_launches);
 int wv_batch_energy(wv_batch *b, double *e /* n values */);
 int wv_batch_set_tiling(wv_batch *b, int tb_depth, int rows_per_tile);
 int wv_batch_get_tiling(const wv_batch *b, int *tb_depth, int *rows_per_tile, int *tile_cols);

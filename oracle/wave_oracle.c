@@ -393,14 +393,17 @@ void orc_split_run(const orc_params *p, float *hi, float *lo, float *v, long nst
                 size_t k = (size_t)i * NY + j;
                 if (!ACTIVE_BULK(p, xy_in[k])) continue;
                 const float hc = hi[k], lc = lo[k], vc = v[k];
-                float dh = ((hi[k + NY] - hc) + (hi[k - NY] - hc)) + ((hi[k + 1] - hc) + (hi[k - 1] - hc));
-                float dl = ((lo[k + NY] - lc) + (lo[k - NY] - lc)) + ((lo[k + 1] - lc) + (lo[k - 1] - lc));
+                /* every neighbour difference is (east/north/diagonal neighbour)
+                 * minus (cell), evaluated once per pair of cells: the west and
+                 * south ones enter as minus the difference their neighbour takes */
+                float dh = ((hi[k + NY] - hc) - (hc - hi[k - NY])) + ((hi[k + 1] - hc) - (hc - hi[k - 1]));
+                float dl = ((lo[k + NY] - lc) - (lc - lo[k - NY])) + ((lo[k + 1] - lc) - (lc - lo[k - 1]));
                 float lap;
                 if (p->stencil == 9) {
-                    float dhd = ((hi[k + NY + 1] - hc) + (hi[k - NY + 1] - hc))
-                                + ((hi[k + NY - 1] - hc) + (hi[k - NY - 1] - hc));
-                    float dld = ((lo[k + NY + 1] - lc) + (lo[k - NY + 1] - lc))
-                                + ((lo[k + NY - 1] - lc) + (lo[k - NY - 1] - lc));
+                    float dhd = ((hi[k + NY + 1] - hc) - (hc - hi[k - NY + 1]))
+                                + ((hi[k + NY - 1] - hc) - (hc - hi[k - NY - 1]));
+                    float dld = ((lo[k + NY + 1] - lc) - (lc - lo[k - NY + 1]))
+                                + ((lo[k + NY - 1] - lc) - (lc - lo[k - NY - 1]));
                     lap = ((4.0f * dh + dhd) + (4.0f * dl + dld)) * w9;
                 } else {
                     lap = dh + dl;

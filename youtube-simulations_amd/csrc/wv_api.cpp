@@ -38,7 +38,11 @@ struct Grid {
     double tcc0 = 0, tcc1 = 0, tg0 = 0, tg1 = 0;
     bool tables = false, state = false;
     bool exact = false;                 // masked cells not all +0: one step per launch
+    // the state holds no -0, infinity or NaN, and the source amplitudes are
+    // finite: undamped bulk cells may skip their zero damping terms
+    bool clean_state = true, clean_amp = true;
     std::vector<uint64_t> h_act;        // host copy of the active plane
+    std::vector<int> h_si, h_sj;        // source cells
 };
 
 struct Engine {
@@ -70,8 +74,9 @@ struct Engine {
     // stepping launch
     bool time_kernels = false;
     std::vector<hipEvent_t> kev;
+    std::vector<int> kev_depth;  // fused depth of each timed launch
     size_t kev_used = 0;
-    double last_int_ms = 0, last_bnd_ms = 0;
+    double last_avg_ms = 0, last_timed = 0;
 };
 
 int fail(Engine *e, int code, const char *fmt, ...)
@@ -96,7 +101,9 @@ int round_up(int a, int b) { return (a + b - 1) / b * b; }
 
 // defaults measured on MI355X at 4096^2: split form 4 fused steps in 12-row
 // strips, FP64 form 6 fused steps in 16-row strips
-int default_depth(const Engine *e) { return e->form == WV_FORM_F32SPLIT ? 4 : 6; }
+// measured on MI355X at 4096^2 (5-point, uniform medium): split form 6 steps
+// per launch in 12-row strips, FP64 8 steps in 16-row strips
+int default_depth(const Engine *e) { return e->form == WV_FORM_F32SPLIT ? 6 : 8; }
 int default_rows(const Engine *e) { return e->form == WV_FORM_F32SPLIT ? 12 : 16; }
 int depth(const Engine *e) { return e->K > 0 ? e->K : default_depth(e); }
 int rows(const Engine *e) { return e->L > 0 ? e->L : default_rows(e); }
@@ -175,6 +182,9 @@ WvInst make_inst(const Engine *e, const Grid &G)
     d.kt = G.p.kappa_topbot; d.gt = G.p.gamma_topbot;
     d.es2 = G.p.e_scale * G.p.e_scale;
     d.damped = G.p.kappa != 0.0 || G.tg0 != 0.0 || G.tg1 != 0.0 || G.tgf != nullptr;
+    // a - KAPPA*x - tgamma*(x-y) with zero coefficients equals a for every
+    // value a step can produce from a clean state (leap_undamped, wv_device.h)
+    d.fp64_full = d.damped || !G.clean_state || !G.clean_amp || (G.p.flags & WV_FLAG_FULL_FORMULA);
     return d;
 }
 
@@ -231,12 +241,15 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
         const int js = cb * wout - wv_halo(k);
         const int ibeg = sb * L, iend = std::min(ibeg + L, nx);
         const bool interior = js >= 1 && js + 64 * WV_C <= ny - 1 && ibeg >= k && iend <= nx - k;
-        if (!interior) return 0;
+        if (!interior) return G.h_si.empty() ? 0 : WV_TILE_SRC;  // (wrapped halos: any source)
         bool all = !G.h_act.empty();
         for (int i = ibeg - k; all && i < iend + k; i++)
             for (int j = js; j < js + 64 * WV_C; j++)
                 if (!((G.h_act[(size_t)i * G.ldw + (j >> 6)] >> (j & 63)) & 1u)) { all = false; break; }
-        return WV_TILE_INTERIOR | (all ? WV_TILE_ALLACT : 0);
+        bool src = false;
+        for (size_t q = 0; q < G.h_si.size() && !src; q++)
+            src = G.h_si[q] >= ibeg - k && G.h_si[q] < iend + k && G.h_sj[q] >= js && G.h_sj[q] < js + 64 * WV_C;
+        return WV_TILE_INTERIOR | (all ? WV_TILE_ALLACT : 0) | (src ? WV_TILE_SRC : 0);
     };
     // one list in XCD-interleave order; boundary tiles keep their place so
     // that they run among the interior tiles instead of as a tail
@@ -356,6 +369,17 @@ inline bool bits_zero(double x)
 }
 
 // do the masked (never written) cells of these arrays all hold +0?
+// no -0, infinity or NaN among n values (NULL counts as clean)
+bool clean_values(const double *a, size_t n)
+{
+    if (!a) return true;
+    for (size_t c = 0; c < n; c++) {
+        const double v = a[c];
+        if (!std::isfinite(v) || (v == 0.0 && std::signbit(v))) return false;
+    }
+    return true;
+}
+
 bool masked_zero(const Grid &G, const double *a)
 {
     if (!a) return true;
@@ -510,6 +534,12 @@ int set_state(Engine *e, int k, const double *phi, const double *psi, const doub
         }
         HIPCHK(e, hipMemsetAsync(G.u[e->role[3]], 0, bytes, e->stream));
         G.exact = !(masked_zero(G, phi) && masked_zero(G, psi) && masked_zero(G, tmp));
+        const size_t n = (size_t)nx * ny;
+        const bool clean = clean_values(phi, n) && clean_values(psi, n) && clean_values(tmp, n);
+        if (clean != G.clean_state) {
+            G.clean_state = clean;
+            if ((rc = sync_inst(e))) return rc;
+        }
     } else {
         const size_t n = (size_t)nx * ny;
         std::vector<float> h(n), l(n), v(n);
@@ -574,6 +604,10 @@ int set_sources(Engine *e, int k, int n, const int *si, const int *sj, const dou
         if (*q) { (void)hipFree(*q); *q = nullptr; }
     if (G.amp) { (void)hipFree(G.amp); G.amp = nullptr; }
     G.amp_len = 0;
+    G.h_si.clear();
+    G.h_sj.clear();
+    G.clean_amp = true;
+    drop_tiles(e);  // tile flags mark the tiles that hold a source
     if (n < 0) return fail(e, WV_EINVAL, "negative source count");
     if (n > 0) {
         if (!si || !sj || !amp || amp_len <= 0) return fail(e, WV_EINVAL, "source arrays missing");
@@ -599,6 +633,9 @@ int set_sources(Engine *e, int k, int n, const int *si, const int *sj, const dou
         HIPCHK(e, hipMemcpy(G.src_id, id.data(), sizeof(int) * n, hipMemcpyHostToDevice));
         HIPCHK(e, hipMemcpy(G.amp, amp, sizeof(double) * (size_t)n * amp_len, hipMemcpyHostToDevice));
         G.amp_len = amp_len;
+        G.h_si.assign(si, si + n);
+        G.h_sj.assign(sj, sj + n);
+        for (size_t q = 0; q < (size_t)n * amp_len && G.clean_amp; q++) G.clean_amp = std::isfinite(amp[q]);
     }
     HIPCHK(e, hipMemset(e->d_gstep, 0, sizeof(long)));
     drop_graphs(e);
@@ -666,6 +703,10 @@ int enqueue_steps(Engine *e, long nsteps, bool force_k1, long *launches)
 {
     long done = 0, nl = 0;
     const int kmax = max_depth(e, force_k1);
+    // coefficient modes present: the kernel built for just those
+    int cm = e->g[0].coef;
+    for (const Grid &G : e->g)
+        if (G.coef != cm) cm = WV_CM_ALL;
     while (done < nsteps) {
         const int k = next_depth(kmax, nsteps - done);
         Engine::Tiles t;
@@ -673,19 +714,23 @@ int enqueue_steps(Engine *e, long nsteps, bool force_k1, long *launches)
         if ((rc = get_tiles(e, k, &t))) return rc;
         auto launch = [&](int ni, int nb) {
             if (e->form == WV_FORM_FP64)
-                return wv_launch_fp64(k, e->stencil, e->d_inst, t.d, ni, t.d + t.ni, nb, e->d_gstep, done,
+                return wv_launch_fp64(k, e->stencil, cm, e->d_inst, t.d, ni, t.d + t.ni, nb, e->d_gstep, done,
                                       e->role[0], e->role[1], e->role[2], e->role[3], e->stream);
-            return wv_launch_split(k, e->stencil, e->d_inst, t.d, ni, t.d + t.ni, nb, e->d_gstep, done,
+            return wv_launch_split(k, e->stencil, cm, e->d_inst, t.d, ni, t.d + t.ni, nb, e->d_gstep, done,
                                    e->scur, 1 - e->scur, e->stream);
         };
         int lr;
-        if (e->time_kernels) {
+        // one launch in eight is bracketed by an event pair: enough samples of
+        // identical launches, without the gaps a pair around every launch adds
+        if (e->time_kernels && nl % 8 == 0) {
             while (e->kev.size() < e->kev_used + 2) {
                 hipEvent_t ev;
                 HIPCHK(e, hipEventCreate(&ev));
                 e->kev.push_back(ev);
+                e->kev_depth.push_back(0);
             }
             hipEvent_t *ev = &e->kev[e->kev_used];
+            e->kev_depth[e->kev_used] = k;
             e->kev_used += 2;
             HIPCHK(e, hipEventRecord(ev[0], e->stream));
             lr = launch(t.ni, t.nb);
@@ -799,14 +844,19 @@ int engine_step_timed(Engine *e, long nsteps, double *ms, long *launches)
     HIPCHK(e, hipEventSynchronize(b));
     float t = 0;
     HIPCHK(e, hipEventElapsedTime(&t, a, b));
-    double ti = 0, tb = 0;
+    // average over the timed launches of full depth (a shorter last one differs)
+    const int kmax = max_depth(e, false);
+    double ti = 0;
+    int nt = 0;
     for (size_t q = 0; q + 2 <= e->kev_used; q += 2) {
+        if (e->kev_depth[q] != kmax) continue;
         float x = 0;
         HIPCHK(e, hipEventElapsedTime(&x, e->kev[q], e->kev[q + 1]));
         ti += x;
+        nt++;
     }
-    e->last_int_ms = ti;
-    e->last_bnd_ms = tb;
+    e->last_avg_ms = nt ? ti / nt : 0.0;
+    e->last_timed = nt;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (ms) *ms = t;
@@ -929,11 +979,11 @@ int wv_sync(wv_ctx *c)
 
 long wv_steps_done(const wv_ctx *c) { return c ? c->e->steps_done : -1; }
 
-int wv_last_kernel_ms(const wv_ctx *c, double *interior_ms, double *boundary_ms)
+int wv_last_kernel_ms(const wv_ctx *c, double *avg_ms, double *timed_launches)
 {
     if (!c) return WV_EINVAL;
-    if (interior_ms) *interior_ms = c->e->last_int_ms;
-    if (boundary_ms) *boundary_ms = c->e->last_bnd_ms;
+    if (avg_ms) *avg_ms = c->e->last_avg_ms;
+    if (timed_launches) *timed_launches = c->e->last_timed;
     return WV_OK;
 }
 
@@ -1112,11 +1162,11 @@ int wv_batch_step_timed(wv_batch *b, long nsteps, double *ms, long *launches)
     return engine_step_timed(b->e, nsteps, ms, launches);
 }
 
-int wv_batch_last_kernel_ms(const wv_batch *b, double *interior_ms, double *boundary_ms)
+int wv_batch_last_kernel_ms(const wv_batch *b, double *avg_ms, double *timed_launches)
 {
     if (!b) return WV_EINVAL;
-    if (interior_ms) *interior_ms = b->e->last_int_ms;
-    if (boundary_ms) *boundary_ms = b->e->last_bnd_ms;
+    if (avg_ms) *avg_ms = b->e->last_avg_ms;
+    if (timed_launches) *timed_launches = b->e->last_timed;
     return WV_OK;
 }
 
@@ -1185,7 +1235,7 @@ extern "C" int wv_autotune(wv_ctx *c, long steps_per_trial, double *best_gcells)
     long gstep0 = 0;
     HIPCHK(e, hipMemcpy(&gstep0, e->d_gstep, sizeof(long), hipMemcpyDeviceToHost));
     const int depths[] = {1, 2, 3, 4, 6, 8};
-    const int rowsc[] = {16, 32, 64, 128, 256};
+    const int rowsc[] = {8, 12, 16, 24, 32, 64};
     double best = -1;
     int bk = depth(e), bl = rows(e);
     const bool exact = e->form == WV_FORM_FP64 && any_exact(e);

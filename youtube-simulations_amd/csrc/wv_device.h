@@ -36,10 +36,14 @@ __device__ __forceinline__ int map_idx(int r, int n, bool wrap)
     return r < 0 ? 0 : (r >= n ? n - 1 : r);
 }
 
-// value held by lane+1 / lane-1 of the wave: DPP wave_shl:1 / wave_shr:1
-// (lanes 63 / 0, which have no source lane, keep their own value)
-__device__ __forceinline__ int dpp_next(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
-__device__ __forceinline__ int dpp_prev(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+// value held by lane+1 / lane-1 of the wave: DPP wave_shl:1 / wave_shr:1.
+// Lanes 63 / 0 have no source lane and read 0 (bound_ctrl): they are halo
+// columns of every tile, and a halo of H >= K columns keeps whatever they
+// compute out of the K-step dependence cone of the written columns.  Zero
+// fill lets the move write its destination directly (no copy of the old
+// value first).
+__device__ __forceinline__ int dpp_next(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, true); }
+__device__ __forceinline__ int dpp_prev(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, true); }
 __device__ __forceinline__ float from_next(float v) { return __int_as_float(dpp_next(__float_as_int(v))); }
 __device__ __forceinline__ float from_prev(float v) { return __int_as_float(dpp_prev(__float_as_int(v))); }
 __device__ __forceinline__ double from_next(double v)
@@ -63,6 +67,18 @@ __device__ __forceinline__ double leap(double x, double y, double delta, double 
     a = a + cc * delta;
     a = a - kappa * x;
     return a - g * (x - y);
+}
+
+// leap() without its last two terms.  With KAPPA = 0 and tgamma = 0,
+// a - KAPPA*x - tgamma*(x-y) equals a unless a is -0 or x, y are not finite,
+// and a = (-y + 2x) + tcc*delta is -0 only when x is -0 (a sum is -0 only if
+// both addends are), so for a state without -0, infinities or NaNs the steps
+// never create one and the two forms agree bit for bit.  The host selects
+// this form only for such states.
+__device__ __forceinline__ double leap_undamped(double x, double y, double delta, double cc)
+{
+    double a = -y + 2.0 * x;
+    return a + cc * delta;
 }
 
 // wave_billiard.c:418  x - tc*(x - inward) - K*x - G*(x-y)

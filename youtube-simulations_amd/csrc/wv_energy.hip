@@ -81,32 +81,32 @@ __global__ void k_bump(long *g, long n)
 
 #define WV_DEPTHS(X) X(1) X(2) X(3) X(4) X(6) X(8)
 #define WV_DECL(k)                                                                              \
-    int wv_launch_fp64_k##k(int, const WvInst *, const WvTile *, int, const WvTile *, int,          \
+    int wv_launch_fp64_k##k(int, int, const WvInst *, const WvTile *, int, const WvTile *, int,          \
                             const long *, long, int, int, int, int, hipStream_t);                 \
-    int wv_launch_split_k##k(int, const WvInst *, const WvTile *, int, const WvTile *, int,         \
+    int wv_launch_split_k##k(int, int, const WvInst *, const WvTile *, int, const WvTile *, int,         \
                              const long *, long, int, int, hipStream_t);
 WV_DEPTHS(WV_DECL)
 
 bool wv_have_depth(int k) { return k == 1 || k == 2 || k == 3 || k == 4 || k == 6 || k == 8; }
 
-int wv_launch_fp64(int k, int stencil, const WvInst *I, const WvTile *Ti, int ni, const WvTile *Tb,
+int wv_launch_fp64(int k, int stencil, int cm, const WvInst *I, const WvTile *Ti, int ni, const WvTile *Tb,
                    int nb, const long *gs, long g0, int ia, int ib, int oc, int od, hipStream_t s)
 {
     switch (k) {
 #define WV_CASE(kk) \
-    case kk: return wv_launch_fp64_k##kk(stencil, I, Ti, ni, Tb, nb, gs, g0, ia, ib, oc, od, s);
+    case kk: return wv_launch_fp64_k##kk(stencil, cm, I, Ti, ni, Tb, nb, gs, g0, ia, ib, oc, od, s);
         WV_DEPTHS(WV_CASE)
 #undef WV_CASE
     }
     return -1;
 }
 
-int wv_launch_split(int k, int stencil, const WvInst *I, const WvTile *Ti, int ni, const WvTile *Tb,
+int wv_launch_split(int k, int stencil, int cm, const WvInst *I, const WvTile *Ti, int ni, const WvTile *Tb,
                     int nb, const long *gs, long g0, int ia, int oc, hipStream_t s)
 {
     switch (k) {
 #define WV_CASE(kk) \
-    case kk: return wv_launch_split_k##kk(stencil, I, Ti, ni, Tb, nb, gs, g0, ia, oc, s);
+    case kk: return wv_launch_split_k##kk(stencil, cm, I, Ti, ni, Tb, nb, gs, g0, ia, oc, s);
         WV_DEPTHS(WV_CASE)
 #undef WV_CASE
     }

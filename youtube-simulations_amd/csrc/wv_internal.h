@@ -25,6 +25,11 @@
 #define WV_COEF_TWO 1
 #define WV_COEF_FIELD 2
 
+// coefficient modes a stepping kernel is built for: one mode (the value of
+// WV_COEF_*) or any mixture (batches); a kernel built for fewer modes holds
+// fewer code paths and needs no more registers than those paths do
+#define WV_CM_ALL 3
+
 // cells per lane along j in the stepping kernels (a wave covers 64*WV_C columns)
 #ifndef WV_C
 #define WV_C 1
@@ -44,6 +49,8 @@ struct WvInst {
     int bc, osc_tb, tb_shift, coef;
     int rows_per_tile, ncb;              // tiling of this instance
     int damped;                          // KAPPA or some tgamma is non-zero
+    int fp64_full;                       // FP64 bulk cells evaluate -KAPPA*x - tgamma*(x-y)
+                                         // even with zero coefficients (see wv_api.cpp)
     int tile0;                           // first tile index of this instance
     double tcc0, tcc1, tg0, tg1;         // coefficient pairs (modes UNIFORM / TWO)
     float tcch0, tccl0, tcch1, tccl1, tgf0, tgf1;
@@ -53,6 +60,7 @@ struct WvInst {
 // one tile of a stepping launch: instance, column block, row strip, flags
 #define WV_TILE_INTERIOR 1  // rows and columns the tile reads never reach the outer box
 #define WV_TILE_ALLACT 2    // ... and every one of those cells is written by a step
+#define WV_TILE_SRC 4       // a point source lies in the cells the tile computes
 struct WvTile {
     int inst, cb, sb, flags;
 };
@@ -61,14 +69,15 @@ struct WvTile {
 static inline int wv_halo(int k) { return (k + WV_C - 1) / WV_C * WV_C; }
 static inline int wv_tile_cols(int k) { return 64 * WV_C - 2 * wv_halo(k); }
 
-// Launchers (wv_step_*.hip, dispatched in wv_energy.hip).  Ti/ni: interior
+// Launchers (wv_step_*.hip, dispatched in wv_energy.hip).  cm: WV_COEF_* of
+// every grid of the launch, or WV_CM_ALL.  Ti/ni: interior
 // tiles, Tb/nb: boundary tiles (two kernels, one after the other).  ia/ib:
 // buffers holding u^n, u^{n-1}; oc/od: buffers receiving u^{n+k}, u^{n+k-1}
 // (od unused when k == 1).  Split form: ia = input set, oc = output set.
 // The amplitude index of the first step is *gstep + g0.
-int wv_launch_fp64(int k, int stencil, const WvInst *d_inst, const WvTile *Ti, int ni, const WvTile *Tb,
+int wv_launch_fp64(int k, int stencil, int cm, const WvInst *d_inst, const WvTile *Ti, int ni, const WvTile *Tb,
                    int nb, const long *gstep, long g0, int ia, int ib, int oc, int od, hipStream_t s);
-int wv_launch_split(int k, int stencil, const WvInst *d_inst, const WvTile *Ti, int ni, const WvTile *Tb,
+int wv_launch_split(int k, int stencil, int cm, const WvInst *d_inst, const WvTile *Ti, int ni, const WvTile *Tb,
                     int nb, const long *gstep, long g0, int ia, int oc, hipStream_t s);
 bool wv_have_depth(int k);
 // energy: rows table (inst,row) pairs; partial[nrows_total]; out[ninst]

@@ -31,14 +31,27 @@ using namespace wv;
 
 namespace {
 
+// Coefficient access of a kernel variant
+constexpr int CF_UNIFORM = 0;  // one (tcc, tgamma) pair: values in registers
+constexpr int CF_TWO = 1;      // two pairs, selected by the medium bit
+constexpr int CF_FIELD = 2;    // per-cell tcc, one tgamma: loaded one iteration ahead
+constexpr int CF_FIELDG = 3;   // per-cell tcc and tgamma: loaded one iteration ahead
+constexpr int CF_ANY = 4;      // per-cell tables read in place (boundary tiles)
+
 // INTERIOR: the tile's rows and columns never reach the outer box (bulk
 // formula only, no index wrapping); otherwise every cell is classified.
-// COEF: WV_COEF_UNIFORM / TWO / FIELD.  ALLACT (INTERIOR only): every cell the
-// tile reads is written by a step, so no active-bit plane is read.
-template <int K, int ST, bool INTERIOR, int COEF, bool ALLACT>
+// CF: coefficient access (above).  ALLACT (INTERIOR only): every cell the
+// tile reads is written by a step, so no active-bit plane is read.  DAMP:
+// bulk cells evaluate -KAPPA*x - tgamma*(x-y); without it they use
+// leap_undamped(), which the host selects only where that is bit-identical.
+template <int K, int ST, bool INTERIOR, int CF, bool ALLACT, bool DAMP>
 __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long gbase, int ia, int ib,
                                          int oc, int od)
 {
+    constexpr bool PREF = CF == CF_FIELD || CF == CF_FIELDG;
+    constexpr int COEF = CF == CF_UNIFORM ? WV_COEF_UNIFORM : (CF == CF_TWO ? WV_COEF_TWO : WV_COEF_FIELD);
+    static_assert(INTERIOR || !PREF, "prefetched coefficients are for interior tiles");
+    static_assert(INTERIOR || DAMP, "boundary tiles evaluate the full formula");
     constexpr int C = WV_C;
     constexpr int H = Tile<K>::H, WOUT = Tile<K>::WOUT;
     constexpr unsigned CMASK = (1u << C) - 1u;
@@ -53,7 +66,7 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
     const Cols<FAST> L(jb - H + (int)threadIdx.x * C, jb, WOUT, ny, wrapj);
     const int ibeg = t.sb * d.rows_per_tile;
     const int iend = min(ibeg + d.rows_per_tile, nx);
-    const bool has_src = src.any();
+    const bool has_src = (t.flags & WV_TILE_SRC) && src.any();
     const double *u0 = d.u[ia], *um1 = d.u[ib];
     const uint64_t *act = d.act, *med = d.med;
     const double *tcp = d.tc, *tccf = d.tccf, *tgf = d.tgf;
@@ -87,6 +100,24 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
     fetch(s0 + 1, QU[1], QP[1], QB[1], QM[1]);
     fetch(s0 + 2, QU[2], QP[2], QB[2], QM[2]);
 
+    // per-cell coefficients: level lv computes row s-2lv at iteration s and
+    // row s+1-2lv at the next one, so each level loads its next row's values
+    // one iteration ahead into registers of its own
+    double TC[K][C], TG[K][C];
+    auto fetch_coef = [&](int lv, int row) {
+        const int rm = min(max(row, 0), nx - 1);
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            const size_t k = (size_t)rm * ldy + L.jm[c];
+            TC[lv - 1][c] = gc(tccf)[k];
+            TG[lv - 1][c] = CF == CF_FIELDG ? gc(tgf)[k] : p.tg0;
+        }
+    };
+    if (PREF) {
+#pragma unroll
+        for (int lv = 1; lv <= K; lv++) fetch_coef(lv, s0 - 2 * lv);
+    }
+
     auto iter = [&](int s, auto phase) {
         constexpr int P = decltype(phase)::value;
         constexpr int PE = (P + 2) % 3, PC = (P + 1) % 3, PW = P;
@@ -96,86 +127,96 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
 #pragma unroll
         for (int lv = K; lv >= 1; lv--) {
             // level lv is needed at rows [ibeg-K+lv, iend+K-lv) only (wave-uniform)
-            if (s < ibeg - K + 3 * lv || s >= iend + K + lv) continue;
-            const int r = s - 2 * lv;
-            // rows outside the dependence cone of the strip's output are never
-            // used: INTERIOR tiles only clamp them so table reads stay in bounds
-            const int rm = INTERIOR ? min(max(r, 0), nx - 1) : map_idx(r, nx, wrapi);
-            const bool rowint = INTERIOR || (rm >= 1 && rm <= nx - 2);
-            const double(&Xe)[C] = U[lv - 1][PE];
-            const double(&Xc)[C] = U[lv - 1][PC];
-            const double(&Xw)[C] = U[lv - 1][PW];
-            const double(&Yc)[C] = Up[lv - 1][PC];
-            const int sh = (2 * lv - 1) * C;
-            const unsigned ab = ALLACT ? CMASK : (mreg >> sh) & CMASK;
-            const unsigned mb = (medreg >> sh) & CMASK;
-            const double cn = from_next(Xc[0]), cp = from_prev(Xc[C - 1]);
-            double en = 0, ep = 0, wn = 0, wp = 0;
-            if (ST == 9) {
-                en = from_next(Xe[0]); ep = from_prev(Xe[C - 1]);
-                wn = from_next(Xw[0]); wp = from_prev(Xw[C - 1]);
-            }
-            double nu[C], np[C];
-#pragma unroll
-            for (int c = 0; c < C; c++) {
-                const int cn1 = c < C - 1 ? c + 1 : 0, cp1 = c > 0 ? c - 1 : 0;
-                const double x = Xc[c], y = Yc[c], E = Xe[c], Wv = Xw[c];
-                const double N = (c < C - 1) ? Xc[cn1] : cn;
-                const double S = (c > 0) ? Xc[cp1] : cp;
-                double cc, g;
-                coef_fp64<COEF>(p, tccf, tgf, rm, L.jm[c], (mb >> c) & 1u, cc, g);
-                double v;
-                if (rowint && L.colint[c]) {
-                    double delta;
-                    if (ST == 9) {
-                        const double NE = (c < C - 1) ? Xe[cn1] : en;
-                        const double SE = (c > 0) ? Xe[cp1] : ep;
-                        const double NW = (c < C - 1) ? Xw[cn1] : wn;
-                        const double SW = (c > 0) ? Xw[cp1] : wp;
-                        const double s4 = ((E + Wv) + N) + S;
-                        const double sd = ((NE + NW) + SE) + SW;
-                        delta = ((4.0 * s4 + sd) - 20.0 * x) * W9;
-                    } else {
-                        delta = (((E + Wv) + N) + S) - 4.0 * x;
-                    }
-                    v = leap(x, y, delta, cc, p.kappa, g);
-                } else {
-                    const double tcv = gc(tcp)[(size_t)rm * ldy + L.jm[c]];
-                    v = edge_fp64(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv);
+            const bool live = s >= ibeg - K + 3 * lv && s < iend + K + lv;
+            if (live) {
+                const int r = s - 2 * lv;
+                // rows outside the dependence cone of the strip's output are never
+                // used: INTERIOR tiles only clamp them so table reads stay in bounds
+                const int rm = INTERIOR ? min(max(r, 0), nx - 1) : map_idx(r, nx, wrapi);
+                const bool rowint = INTERIOR || (rm >= 1 && rm <= nx - 2);
+                const double(&Xe)[C] = U[lv - 1][PE];
+                const double(&Xc)[C] = U[lv - 1][PC];
+                const double(&Xw)[C] = U[lv - 1][PW];
+                const double(&Yc)[C] = Up[lv - 1][PC];
+                const int sh = (2 * lv - 1) * C;
+                const unsigned ab = ALLACT ? CMASK : (mreg >> sh) & CMASK;
+                const unsigned mb = (medreg >> sh) & CMASK;
+                const double cn = from_next(Xc[0]), cp = from_prev(Xc[C - 1]);
+                double en = 0, ep = 0, wn = 0, wp = 0;
+                if (ST == 9) {
+                    en = from_next(Xe[0]); ep = from_prev(Xe[C - 1]);
+                    wn = from_next(Xw[0]); wp = from_prev(Xw[C - 1]);
                 }
-                if (has_src) {
-                    const long g1 = gbase + lv - 1;
-                    if (g1 < src.len) v = add_sources(src, rm, L.jm[c], g1, v);
-                }
-                nu[c] = (ALLACT || ((ab >> c) & 1u)) ? v : x;  // cells the step never writes keep u
-                np[c] = x;
-            }
-            if (lv == K) {
-                // level K at row s-2K is final (the live test puts it inside the strip)
-                gptr<double> o0 = out0 + (size_t)r * ldy + L.j0;
-                gptr<double> o1 = out1 + (size_t)r * ldy + L.j0;
-                if (C == 2 && FAST && (ALLACT || ab == CMASK) && L.wcol[0] && L.wcol[C - 1]) {
-                    const dvec2 a = {nu[0], nu[C - 1]};
-                    *(gptr<dvec2>)o0 = a;
-                    if (K >= 2) {
-                        const dvec2 b = {np[0], np[C - 1]};
-                        *(gptr<dvec2>)o1 = b;
-                    }
-                } else {
-#pragma unroll
-                    for (int c = 0; c < C; c++)
-                        if (L.wcol[c] && ((ab >> c) & 1u)) {
-                            o0[c] = nu[c];
-                            if (K >= 2) o1[c] = np[c];
-                        }
-                }
-            } else {
+                double nu[C], np[C];
 #pragma unroll
                 for (int c = 0; c < C; c++) {
-                    U[lv < K ? lv : 0][P][c] = nu[c];
-                    Up[lv < K ? lv : 0][P][c] = np[c];
+                    const int cn1 = c < C - 1 ? c + 1 : 0, cp1 = c > 0 ? c - 1 : 0;
+                    const double x = Xc[c], y = Yc[c], E = Xe[c], Wv = Xw[c];
+                    const double N = (c < C - 1) ? Xc[cn1] : cn;
+                    const double S = (c > 0) ? Xc[cp1] : cp;
+                    double cc, g;
+                    if (PREF) {
+                        cc = TC[lv - 1][c];
+                        g = TG[lv - 1][c];
+                    } else {
+                        coef_fp64<COEF>(p, tccf, tgf, rm, L.jm[c], (mb >> c) & 1u, cc, g);
+                    }
+                    double v;
+                    if (rowint && L.colint[c]) {
+                        double delta;
+                        if (ST == 9) {
+                            const double NE = (c < C - 1) ? Xe[cn1] : en;
+                            const double SE = (c > 0) ? Xe[cp1] : ep;
+                            const double NW = (c < C - 1) ? Xw[cn1] : wn;
+                            const double SW = (c > 0) ? Xw[cp1] : wp;
+                            const double s4 = ((E + Wv) + N) + S;
+                            const double sd = ((NE + NW) + SE) + SW;
+                            delta = ((4.0 * s4 + sd) - 20.0 * x) * W9;
+                        } else {
+                            delta = (((E + Wv) + N) + S) - 4.0 * x;
+                        }
+                        v = DAMP ? leap(x, y, delta, cc, p.kappa, g) : leap_undamped(x, y, delta, cc);
+                    } else {
+                        const double tcv = gc(tcp)[(size_t)rm * ldy + L.jm[c]];
+                        v = edge_fp64(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv);
+                    }
+                    if (has_src) {
+                        const long g1 = gbase + lv - 1;
+                        if (g1 < src.len) v = add_sources(src, rm, L.jm[c], g1, v);
+                    }
+                    nu[c] = (ALLACT || ((ab >> c) & 1u)) ? v : x;  // cells the step never writes keep u
+                    np[c] = x;
+                }
+                if (lv == K) {
+                    // level K at row s-2K is final (the live test puts it inside the strip)
+                    gptr<double> o0 = out0 + (size_t)r * ldy + L.j0;
+                    gptr<double> o1 = out1 + (size_t)r * ldy + L.j0;
+                    if (C == 2 && FAST && (ALLACT || ab == CMASK) && L.wcol[0] && L.wcol[C - 1]) {
+                        const dvec2 a = {nu[0], nu[C - 1]};
+                        *(gptr<dvec2>)o0 = a;
+                        if (K >= 2) {
+                            const dvec2 b = {np[0], np[C - 1]};
+                            *(gptr<dvec2>)o1 = b;
+                        }
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < C; c++)
+                            if (L.wcol[c] && ((ab >> c) & 1u)) {
+                                o0[c] = nu[c];
+                                if (K >= 2) o1[c] = np[c];
+                            }
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < C; c++) {
+                        U[lv < K ? lv : 0][P][c] = nu[c];
+                        Up[lv < K ? lv : 0][P][c] = np[c];
+                    }
                 }
             }
+            // unconditional, like the ring (a load under the branch would
+            // make the compiler wait for every load in flight at the join)
+            if (PREF) fetch_coef(lv, s + 1 - 2 * lv);
         }
         // level 0 from the prefetch ring, then request row s+3 into the freed slot
 #pragma unroll
@@ -203,10 +244,14 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
 // the register count of the whole kernel.
 template <int K>
 struct Occ {
+#ifdef WV_OCC_FP64
+    static constexpr int waves = WV_OCC_FP64;
+#else
     static constexpr int waves = K <= 4 ? 3 : 2;
+#endif
 };
 
-template <int K, int ST>
+template <int K, int ST, int CM>
 __global__ void __launch_bounds__(64, Occ<K>::waves)
 k_step_fp64(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, const long *gstep,
             long g0, int ia, int ib, int oc, int od)
@@ -217,18 +262,38 @@ k_step_fp64(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, cons
     const long gbase = (gstep ? *gstep : 0) + g0;
     const bool interior = t.flags & WV_TILE_INTERIOR;
     const bool allact = t.flags & WV_TILE_ALLACT;
-    switch (d.coef) {
-#define WV_RUN(COEF)                                                                   \
-    case COEF:                                                                         \
-        if (!interior) run_fp64<K, ST, false, COEF, false>(d, t, gbase, ia, ib, oc, od);              \
-        else if (allact) run_fp64<K, ST, true, COEF, true>(d, t, gbase, ia, ib, oc, od);              \
-        else run_fp64<K, ST, true, COEF, false>(d, t, gbase, ia, ib, oc, od);                         \
-        break;
-        WV_RUN(WV_COEF_UNIFORM)
-        WV_RUN(WV_COEF_TWO)
-        WV_RUN(WV_COEF_FIELD)
-#undef WV_RUN
+    const bool damp = d.fp64_full;
+#define WV_RUN(CF, A, D) run_fp64<K, ST, true, CF, A, D>(d, t, gbase, ia, ib, oc, od)
+#define WV_INTERIOR(CF)                            \
+    if (allact) {                                  \
+        if (damp) WV_RUN(CF, true, true);          \
+        else WV_RUN(CF, true, false);              \
+    } else {                                       \
+        if (damp) WV_RUN(CF, false, true);         \
+        else WV_RUN(CF, false, false);             \
     }
+    // the modes this kernel is not built for are folded away
+    const int coef = CM == WV_CM_ALL ? d.coef : CM;
+    switch (coef) {
+    case WV_COEF_UNIFORM:
+        if (!interior) run_fp64<K, ST, false, CF_UNIFORM, false, true>(d, t, gbase, ia, ib, oc, od);
+        else WV_INTERIOR(CF_UNIFORM)
+        break;
+    case WV_COEF_TWO:
+        if (!interior) run_fp64<K, ST, false, CF_TWO, false, true>(d, t, gbase, ia, ib, oc, od);
+        else WV_INTERIOR(CF_TWO)
+        break;
+    default:
+        if (!interior) run_fp64<K, ST, false, CF_ANY, false, true>(d, t, gbase, ia, ib, oc, od);
+        else if (d.tgf) {
+            // a tgamma field implies damping
+            if (allact) WV_RUN(CF_FIELDG, true, true);
+            else WV_RUN(CF_FIELDG, false, true);
+        } else WV_INTERIOR(CF_FIELD)
+        break;
+    }
+#undef WV_INTERIOR
+#undef WV_RUN
 }
 
 }  // namespace
@@ -236,7 +301,7 @@ k_step_fp64(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, cons
 #define WV_CAT2(a, b) a##b
 #define WV_CAT(a, b) WV_CAT2(a, b)
 
-int WV_CAT(wv_launch_fp64_k, WV_K)(int stencil, const WvInst *I, const WvTile *Ti, int ni, const WvTile *Tb,
+int WV_CAT(wv_launch_fp64_k, WV_K)(int stencil, int cm, const WvInst *I, const WvTile *Ti, int ni, const WvTile *Tb,
                                     int nb, const long *gs, long g0, int ia, int ib, int oc, int od,
                                     hipStream_t s)
 {
@@ -244,9 +309,21 @@ int WV_CAT(wv_launch_fp64_k, WV_K)(int stencil, const WvInst *I, const WvTile *T
     const WvTile *T = ni ? Ti : Tb;
     const int n = ni + nb;
     if (!n) return 0;
-    if (stencil == 9)
-        hipLaunchKernelGGL((k_step_fp64<WV_K, 9>), dim3(n), dim3(64), 0, s, I, T, gs, g0, ia, ib, oc, od);
-    else
-        hipLaunchKernelGGL((k_step_fp64<WV_K, 5>), dim3(n), dim3(64), 0, s, I, T, gs, g0, ia, ib, oc, od);
+    const dim3 grid(n), block(64);
+#define WV_GO(ST, CM) hipLaunchKernelGGL((k_step_fp64<WV_K, ST, CM>), grid, block, 0, s, I, T, gs, g0, ia, ib, oc, od)
+#define WV_ST(ST)                                 \
+    switch (cm) {                                 \
+    case WV_COEF_UNIFORM: WV_GO(ST, WV_COEF_UNIFORM); break; \
+    case WV_COEF_TWO: WV_GO(ST, WV_COEF_TWO); break;         \
+    case WV_COEF_FIELD: WV_GO(ST, WV_COEF_FIELD); break;     \
+    default: WV_GO(ST, WV_CM_ALL); break;                     \
+    }
+    if (stencil == 9) {
+        WV_ST(9)
+    } else {
+        WV_ST(5)
+    }
+#undef WV_ST
+#undef WV_GO
     return (int)hipGetLastError();
 }

@@ -23,14 +23,35 @@ using namespace wv;
 
 namespace {
 
+// Coefficient access of a kernel variant
+constexpr int CF_UNIFORM = 0;  // one (tcc, tgamma) pair: values in registers
+constexpr int CF_TWO = 1;      // two pairs, selected by the medium bit
+constexpr int CF_FIELD = 2;    // per-cell tcc_hi, one tgamma: loaded one iteration ahead
+constexpr int CF_FIELDG = 3;   // per-cell tcc_hi and tgamma
+constexpr int CF_FIELDL = 4;   // per-cell tcc_hi and tcc_lo, one tgamma
+constexpr int CF_FIELDLG = 5;  // per-cell tcc_hi, tcc_lo and tgamma
+constexpr int CF_ANY = 6;      // per-cell tables read in place (boundary tiles)
+
 // INTERIOR: the tile's rows and columns never reach the outer box (bulk
 // formula only, no index wrapping); otherwise every cell is classified.
-// COEF: WV_COEF_UNIFORM / TWO / FIELD.  ALLACT (INTERIOR only): every cell the
+// CF: coefficient access (above).  ALLACT (INTERIOR only): every cell the
 // tile reads is written by a step.  DAMP: the grid has a non-zero KAPPA or
 // tgamma (terms with a zero coefficient are not evaluated).
-template <int K, int ST, bool INTERIOR, int COEF, bool ALLACT, bool DAMP>
+//
+// Every neighbour difference is (east / north / diagonal neighbour) minus
+// (cell); a cell's west, south and two western diagonal terms are minus the
+// difference that neighbour takes towards it.  Interior tiles evaluate each
+// difference once: along i it is kept for the next iteration of the same
+// level, along j and the diagonals it comes from the adjacent lane.  Boundary
+// tiles evaluate the same subtractions directly; the results are the same.
+template <int K, int ST, bool INTERIOR, int CF, bool ALLACT, bool DAMP>
 __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long gbase, int ia, int oc)
 {
+    constexpr bool PREF = CF >= CF_FIELD && CF <= CF_FIELDLG;
+    constexpr bool PG = CF == CF_FIELDG || CF == CF_FIELDLG, PL = CF == CF_FIELDL || CF == CF_FIELDLG;
+    constexpr int COEF = CF == CF_UNIFORM ? WV_COEF_UNIFORM : (CF == CF_TWO ? WV_COEF_TWO : WV_COEF_FIELD);
+    constexpr bool SHARE = INTERIOR && WV_C == 1;
+    static_assert(INTERIOR || !PREF, "prefetched coefficients are for interior tiles");
     constexpr int C = WV_C;
     constexpr int H = Tile<K>::H, WOUT = Tile<K>::WOUT;
     constexpr unsigned CMASK = (1u << C) - 1u;
@@ -45,7 +66,7 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
     const Cols<FAST> L(jb - H + (int)threadIdx.x * C, jb, WOUT, ny, wrapj);
     const int ibeg = t.sb * d.rows_per_tile;
     const int iend = min(ibeg + d.rows_per_tile, nx);
-    const bool has_src = src.any();
+    const bool has_src = (t.flags & WV_TILE_SRC) && src.any();
     const uint64_t *act = d.act, *med = d.med;
     const double *tcp = d.tc, *tccf = d.tccf, *tgf = d.tgf;
     const float *tccf_hi = d.tccf_hi, *tccf_lo = d.tccf_lo, *tgf32 = d.tgf32;
@@ -83,6 +104,28 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
     fetch(s0 + 1, QH[1], QL[1], QV[1], QB[1], QM[1]);
     fetch(s0 + 2, QH[2], QL[2], QV[2], QB[2], QM[2]);
 
+    // per-cell coefficients, one iteration ahead per level (see the FP64 kernel)
+    float TH[K][C], TL[K][C], TG[K][C];
+    auto fetch_coef = [&](int lv, int row) {
+        const int rm = min(max(row, 0), nx - 1);
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            const size_t k = (size_t)rm * ldy + L.jm[c];
+            TH[lv - 1][c] = gc(tccf_hi)[k];
+            TL[lv - 1][c] = PL ? gc(tccf_lo)[k] : 0.0f;
+            TG[lv - 1][c] = PG ? gc(tgf32)[k] : p.tgf0;
+        }
+    };
+    if (PREF) {
+#pragma unroll
+        for (int lv = 1; lv <= K; lv++) fetch_coef(lv, s0 - 2 * lv);
+    }
+    // differences kept from the previous iteration of each level (SHARE):
+    // row r+1 minus row r of hi and lo, and the two diagonal ones (9-point)
+    float DEh[K], DEl[K], DNEh[K], DNEl[K], DSEh[K], DSEl[K];
+#pragma unroll
+    for (int a = 0; a < K; a++) DEh[a] = DEl[a] = DNEh[a] = DNEl[a] = DSEh[a] = DSEl[a] = 0.0f;
+
     auto iter = [&](int s, auto phase) {
         constexpr int P = decltype(phase)::value;
         constexpr int PE = (P + 2) % 3, PC = (P + 1) % 3, PW = P;
@@ -90,129 +133,206 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
         // place (see the FP64 kernel)
 #pragma unroll
         for (int lv = K; lv >= 1; lv--) {
-            if (s < ibeg - K + 3 * lv || s >= iend + K + lv) continue;
-            const int r = s - 2 * lv;
-            const int rm = INTERIOR ? min(max(r, 0), nx - 1) : map_idx(r, nx, wrapi);
-            const bool rowint = INTERIOR || (rm >= 1 && rm <= nx - 2);
-            const float(&he)[C] = Hw[lv - 1][PE];
-            const float(&hc)[C] = Hw[lv - 1][PC];
-            const float(&hw)[C] = Hw[lv - 1][PW];
-            const float(&le)[C] = Lw[lv - 1][PE];
-            const float(&lc)[C] = Lw[lv - 1][PC];
-            const float(&lw)[C] = Lw[lv - 1][PW];
-            const float(&vc)[C] = Vw[lv - 1][PC];
-            const int sh = (2 * lv - 1) * C;
-            const unsigned ab = ALLACT ? CMASK : (mreg >> sh) & CMASK;
-            const unsigned mb = (medreg >> sh) & CMASK;
-            const float hcn = from_next(hc[0]), hcp = from_prev(hc[C - 1]);
-            const float lcn = from_next(lc[0]), lcp = from_prev(lc[C - 1]);
-            float hen = 0, hep = 0, hwn = 0, hwp = 0, len_ = 0, lep = 0, lwn = 0, lwp = 0;
-            if (ST == 9) {
-                hen = from_next(he[0]); hep = from_prev(he[C - 1]);
-                hwn = from_next(hw[0]); hwp = from_prev(hw[C - 1]);
-                len_ = from_next(le[0]); lep = from_prev(le[C - 1]);
-                lwn = from_next(lw[0]); lwp = from_prev(lw[C - 1]);
-            }
-            float nh[C], nl[C], nv[C];
-#pragma unroll
-            for (int c = 0; c < C; c++) {
-                const int cn = c < C - 1 ? c + 1 : 0, cp = c > 0 ? c - 1 : 0;
-                const float hN = (c < C - 1) ? hc[cn] : hcn, hS = (c > 0) ? hc[cp] : hcp;
-                const float lN = (c < C - 1) ? lc[cn] : lcn, lS = (c > 0) ? lc[cp] : lcp;
-                const unsigned mbit = (mb >> c) & 1u;
-                float oh, ol, ov;
-                if (rowint && L.colint[c]) {
-                    const float h0 = hc[c], l0 = lc[c];
-                    const float dh = ((he[c] - h0) + (hw[c] - h0)) + ((hN - h0) + (hS - h0));
-                    const float dl = ((le[c] - l0) + (lw[c] - l0)) + ((lN - l0) + (lS - l0));
-                    float lap;
-                    if (ST == 9) {
-                        const float hNE = (c < C - 1) ? he[cn] : hen, hSE = (c > 0) ? he[cp] : hep;
-                        const float hNW = (c < C - 1) ? hw[cn] : hwn, hSW = (c > 0) ? hw[cp] : hwp;
-                        const float lNE = (c < C - 1) ? le[cn] : len_, lSE = (c > 0) ? le[cp] : lep;
-                        const float lNW = (c < C - 1) ? lw[cn] : lwn, lSW = (c > 0) ? lw[cp] : lwp;
-                        const float dhd = ((hNE - h0) + (hNW - h0)) + ((hSE - h0) + (hSW - h0));
-                        const float dld = ((lNE - l0) + (lNW - l0)) + ((lSE - l0) + (lSW - l0));
-                        lap = ((4.0f * dh + dhd) + (4.0f * dl + dld)) * w9;
-                    } else {
-                        lap = dh + dl;
-                    }
-                    float ch, cl, gf;
-                    if (COEF == WV_COEF_FIELD) {
-                        const size_t k = (size_t)rm * ldy + L.jm[c];
-                        ch = gc(tccf_hi)[k];
-                        cl = tccf_lo ? gc(tccf_lo)[k] : 0.0f;
-                        gf = tgf32 ? gc(tgf32)[k] : p.tgf0;
-                    } else if (COEF == WV_COEF_UNIFORM) {
-                        ch = p.tcch0;
-                        cl = p.tccl0;
-                        gf = p.tgf0;
-                    } else {
-                        ch = mbit ? p.tcch1 : p.tcch0;
-                        cl = mbit ? p.tccl1 : p.tccl0;
-                        gf = mbit ? p.tgf1 : p.tgf0;
-                    }
-                    const float v0 = vc[c];
-                    float vn = v0 + __fmaf_rn(ch, lap, cl * lap);
-                    if (DAMP) {  // a term whose coefficient is zero is not evaluated
-                        if (kf != 0.0f) vn = vn - kf * h0;
-                        vn = (gf != 0.0f) ? vn - gf * v0 : vn;
-                    }
-                    if (has_src) {
-                        const long g1 = gbase + lv - 1;
-                        if (g1 < src.len) vn = add_sources(src, rm, L.jm[c], g1, vn);
-                    }
-                    dd_add_f(h0, l0, vn, oh, ol);
-                    ov = vn;
-                } else {
-                    // edge cells in FP64 with the reference formulas
-                    const double x = (double)hc[c] + (double)lc[c];
-                    const double y = x - (double)vc[c];
-                    const double E = (double)he[c] + (double)le[c];
-                    const double Wv = (double)hw[c] + (double)lw[c];
-                    const double N = (double)hN + (double)lN;
-                    const double S = (double)hS + (double)lS;
-                    double cc, g;
-                    coef_fp64<COEF>(p, tccf, tgf, rm, L.jm[c], mbit, cc, g);
-                    const double tcv = gc(tcp)[(size_t)rm * ldy + L.jm[c]];
-                    double out = edge_fp64(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv);
-                    if (has_src) {
-                        const long g1 = gbase + lv - 1;
-                        if (g1 < src.len) out = add_sources(src, rm, L.jm[c], g1, out);
-                    }
-                    oh = (float)out;
-                    ol = (float)(out - (double)oh);
-                    ov = (float)(out - x);
+            // differences of this iteration's rows (east minus centre), also in
+            // iterations where the level is not needed: the first needed one
+            // takes its west difference from here
+            float De_h = 0, De_l = 0, Dne_h = 0, Dne_l = 0, Dse_h = 0, Dse_l = 0;
+            if (SHARE) {
+                const float h0 = Hw[lv - 1][PC][0], l0 = Lw[lv - 1][PC][0];
+                const float he0 = Hw[lv - 1][PE][0], le0 = Lw[lv - 1][PE][0];
+                De_h = he0 - h0;
+                De_l = le0 - l0;
+                if (ST == 9) {
+                    Dne_h = from_next(he0) - h0;
+                    Dne_l = from_next(le0) - l0;
+                    Dse_h = from_prev(he0) - h0;
+                    Dse_l = from_prev(le0) - l0;
                 }
-                const bool live = ALLACT || ((ab >> c) & 1u);  // cells the step never writes keep their state
-                nh[c] = live ? oh : hc[c];
-                nl[c] = live ? ol : lc[c];
-                nv[c] = live ? ov : vc[c];
             }
-            if (lv == K) {
-                const size_t ro = (size_t)r * ldy + L.j0;
-                if (C == 2 && FAST && (ALLACT || ab == CMASK) && L.wcol[0] && L.wcol[C - 1]) {
-                    const fvec2 a = {nh[0], nh[C - 1]}, b = {nl[0], nl[C - 1]}, e = {nv[0], nv[C - 1]};
-                    *(gptr<fvec2>)(hout + ro) = a;
-                    *(gptr<fvec2>)(lout + ro) = b;
-                    *(gptr<fvec2>)(vout + ro) = e;
-                } else {
-#pragma unroll
-                    for (int c = 0; c < C; c++)
-                        if (L.wcol[c] && ((ab >> c) & 1u)) {
-                            hout[ro + c] = nh[c];
-                            lout[ro + c] = nl[c];
-                            vout[ro + c] = nv[c];
-                        }
+            const bool live = s >= ibeg - K + 3 * lv && s < iend + K + lv;
+            if (live) {
+                const int r = s - 2 * lv;
+                const int rm = INTERIOR ? min(max(r, 0), nx - 1) : map_idx(r, nx, wrapi);
+                const bool rowint = INTERIOR || (rm >= 1 && rm <= nx - 2);
+                const float(&he)[C] = Hw[lv - 1][PE];
+                const float(&hc)[C] = Hw[lv - 1][PC];
+                const float(&hw)[C] = Hw[lv - 1][PW];
+                const float(&le)[C] = Lw[lv - 1][PE];
+                const float(&lc)[C] = Lw[lv - 1][PC];
+                const float(&lw)[C] = Lw[lv - 1][PW];
+                const float(&vc)[C] = Vw[lv - 1][PC];
+                const int sh = (2 * lv - 1) * C;
+                const unsigned ab = ALLACT ? CMASK : (mreg >> sh) & CMASK;
+                const unsigned mb = (medreg >> sh) & CMASK;
+                const float hcn = from_next(hc[0]), hcp = from_prev(hc[C - 1]);
+                const float lcn = from_next(lc[0]), lcp = from_prev(lc[C - 1]);
+                float hen = 0, hep = 0, hwn = 0, hwp = 0, len_ = 0, lep = 0, lwn = 0, lwp = 0;
+                if (ST == 9) {
+                    hen = from_next(he[0]); hep = from_prev(he[C - 1]);
+                    hwn = from_next(hw[0]); hwp = from_prev(hw[C - 1]);
+                    len_ = from_next(le[0]); lep = from_prev(le[C - 1]);
+                    lwn = from_next(lw[0]); lwp = from_prev(lw[C - 1]);
                 }
-            } else {
+                float nh[C], nl[C], nv[C];
 #pragma unroll
                 for (int c = 0; c < C; c++) {
-                    Hw[lv < K ? lv : 0][P][c] = nh[c];
-                    Lw[lv < K ? lv : 0][P][c] = nl[c];
-                    Vw[lv < K ? lv : 0][P][c] = nv[c];
+                    const int cn = c < C - 1 ? c + 1 : 0, cp = c > 0 ? c - 1 : 0;
+                    const float hN = (c < C - 1) ? hc[cn] : hcn, hS = (c > 0) ? hc[cp] : hcp;
+                    const float lN = (c < C - 1) ? lc[cn] : lcn, lS = (c > 0) ? lc[cp] : lcp;
+                    const unsigned mbit = (mb >> c) & 1u;
+                    float oh, ol, ov;
+                    if (SHARE) {
+                        // h0 - hw is the east difference of row r-1 (previous iteration),
+                        // h0 - hS the north difference of lane j-1
+                        const float h0 = hc[c], l0 = lc[c];
+                        const float Dn_h = hN - h0, Dn_l = lN - l0;
+                        const float dh = (De_h - DEh[lv - 1]) + (Dn_h - from_prev(Dn_h));
+                        const float dl = (De_l - DEl[lv - 1]) + (Dn_l - from_prev(Dn_l));
+                        float lap;
+                        if (ST == 9) {
+                            // h0 - hNW is the SE difference of lane j+1 in row r-1,
+                            // h0 - hSW the NE difference of lane j-1 in row r-1
+                            const float dhd = (Dne_h - from_next(DSEh[lv - 1])) + (Dse_h - from_prev(DNEh[lv - 1]));
+                            const float dld = (Dne_l - from_next(DSEl[lv - 1])) + (Dse_l - from_prev(DNEl[lv - 1]));
+                            lap = ((4.0f * dh + dhd) + (4.0f * dl + dld)) * w9;
+                        } else {
+                            lap = dh + dl;
+                        }
+                        float ch, cl, gf;
+                        if (PREF) {
+                            ch = TH[lv - 1][c];
+                            cl = TL[lv - 1][c];
+                            gf = TG[lv - 1][c];
+                        } else if (CF == CF_UNIFORM) {
+                            ch = p.tcch0;
+                            cl = p.tccl0;
+                            gf = p.tgf0;
+                        } else {
+                            ch = mbit ? p.tcch1 : p.tcch0;
+                            cl = mbit ? p.tccl1 : p.tccl0;
+                            gf = mbit ? p.tgf1 : p.tgf0;
+                        }
+                        const float v0 = vc[c];
+                        float vn = v0 + __fmaf_rn(ch, lap, cl * lap);
+                        if (DAMP) {
+                            if (kf != 0.0f) vn = vn - kf * h0;
+                            vn = (gf != 0.0f) ? vn - gf * v0 : vn;
+                        }
+                        if (has_src) {
+                            const long g1 = gbase + lv - 1;
+                            if (g1 < src.len) vn = add_sources(src, rm, L.jm[c], g1, vn);
+                        }
+                        dd_add_f(h0, l0, vn, oh, ol);
+                        ov = vn;
+                    } else if (rowint && L.colint[c]) {
+                        const float h0 = hc[c], l0 = lc[c];
+                        const float dh = ((he[c] - h0) - (h0 - hw[c])) + ((hN - h0) - (h0 - hS));
+                        const float dl = ((le[c] - l0) - (l0 - lw[c])) + ((lN - l0) - (l0 - lS));
+                        float lap;
+                        if (ST == 9) {
+                            const float hNE = (c < C - 1) ? he[cn] : hen, hSE = (c > 0) ? he[cp] : hep;
+                            const float hNW = (c < C - 1) ? hw[cn] : hwn, hSW = (c > 0) ? hw[cp] : hwp;
+                            const float lNE = (c < C - 1) ? le[cn] : len_, lSE = (c > 0) ? le[cp] : lep;
+                            const float lNW = (c < C - 1) ? lw[cn] : lwn, lSW = (c > 0) ? lw[cp] : lwp;
+                            const float dhd = ((hNE - h0) - (h0 - hNW)) + ((hSE - h0) - (h0 - hSW));
+                            const float dld = ((lNE - l0) - (l0 - lNW)) + ((lSE - l0) - (l0 - lSW));
+                            lap = ((4.0f * dh + dhd) + (4.0f * dl + dld)) * w9;
+                        } else {
+                            lap = dh + dl;
+                        }
+                        float ch, cl, gf;
+                        if (PREF) {
+                            ch = TH[lv - 1][c];
+                            cl = TL[lv - 1][c];
+                            gf = TG[lv - 1][c];
+                        } else if (COEF == WV_COEF_FIELD) {
+                            const size_t k = (size_t)rm * ldy + L.jm[c];
+                            ch = gc(tccf_hi)[k];
+                            cl = tccf_lo ? gc(tccf_lo)[k] : 0.0f;
+                            gf = tgf32 ? gc(tgf32)[k] : p.tgf0;
+                        } else if (COEF == WV_COEF_UNIFORM) {
+                            ch = p.tcch0;
+                            cl = p.tccl0;
+                            gf = p.tgf0;
+                        } else {
+                            ch = mbit ? p.tcch1 : p.tcch0;
+                            cl = mbit ? p.tccl1 : p.tccl0;
+                            gf = mbit ? p.tgf1 : p.tgf0;
+                        }
+                        const float v0 = vc[c];
+                        float vn = v0 + __fmaf_rn(ch, lap, cl * lap);
+                        if (DAMP) {  // a term whose coefficient is zero is not evaluated
+                            if (kf != 0.0f) vn = vn - kf * h0;
+                            vn = (gf != 0.0f) ? vn - gf * v0 : vn;
+                        }
+                        if (has_src) {
+                            const long g1 = gbase + lv - 1;
+                            if (g1 < src.len) vn = add_sources(src, rm, L.jm[c], g1, vn);
+                        }
+                        dd_add_f(h0, l0, vn, oh, ol);
+                        ov = vn;
+                    } else {
+                        // edge cells in FP64 with the reference formulas
+                        const double x = (double)hc[c] + (double)lc[c];
+                        const double y = x - (double)vc[c];
+                        const double E = (double)he[c] + (double)le[c];
+                        const double Wv = (double)hw[c] + (double)lw[c];
+                        const double N = (double)hN + (double)lN;
+                        const double S = (double)hS + (double)lS;
+                        double cc, g;
+                        coef_fp64<COEF>(p, tccf, tgf, rm, L.jm[c], mbit, cc, g);
+                        const double tcv = gc(tcp)[(size_t)rm * ldy + L.jm[c]];
+                        double out = edge_fp64(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv);
+                        if (has_src) {
+                            const long g1 = gbase + lv - 1;
+                            if (g1 < src.len) out = add_sources(src, rm, L.jm[c], g1, out);
+                        }
+                        oh = (float)out;
+                        ol = (float)(out - (double)oh);
+                        ov = (float)(out - x);
+                    }
+                    const bool live = ALLACT || ((ab >> c) & 1u);  // cells the step never writes keep their state
+                    nh[c] = live ? oh : hc[c];
+                    nl[c] = live ? ol : lc[c];
+                    nv[c] = live ? ov : vc[c];
+                }
+                if (lv == K) {
+                    const size_t ro = (size_t)r * ldy + L.j0;
+                    if (C == 2 && FAST && (ALLACT || ab == CMASK) && L.wcol[0] && L.wcol[C - 1]) {
+                        const fvec2 a = {nh[0], nh[C - 1]}, b = {nl[0], nl[C - 1]}, e = {nv[0], nv[C - 1]};
+                        *(gptr<fvec2>)(hout + ro) = a;
+                        *(gptr<fvec2>)(lout + ro) = b;
+                        *(gptr<fvec2>)(vout + ro) = e;
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < C; c++)
+                            if (L.wcol[c] && ((ab >> c) & 1u)) {
+                                hout[ro + c] = nh[c];
+                                lout[ro + c] = nl[c];
+                                vout[ro + c] = nv[c];
+                            }
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < C; c++) {
+                        Hw[lv < K ? lv : 0][P][c] = nh[c];
+                        Lw[lv < K ? lv : 0][P][c] = nl[c];
+                        Vw[lv < K ? lv : 0][P][c] = nv[c];
+                    }
                 }
             }
+            if (SHARE) {
+                DEh[lv - 1] = De_h;
+                DEl[lv - 1] = De_l;
+                if (ST == 9) {
+                    DNEh[lv - 1] = Dne_h;
+                    DNEl[lv - 1] = Dne_l;
+                    DSEh[lv - 1] = Dse_h;
+                    DSEl[lv - 1] = Dse_l;
+                }
+            }
+            // unconditional, like the ring
+            if (PREF) fetch_coef(lv, s + 1 - 2 * lv);
         }
 #pragma unroll
         for (int c = 0; c < C; c++) {
@@ -240,10 +360,14 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
 // the register count of the whole kernel.
 template <int K>
 struct Occ {
+#ifdef WV_OCC_SPLIT
+    static constexpr int waves = WV_OCC_SPLIT;
+#else
     static constexpr int waves = K <= 4 ? 3 : 2;
+#endif
 };
 
-template <int K, int ST>
+template <int K, int ST, int CM>
 __global__ void __launch_bounds__(64, Occ<K>::waves)
 k_step_split(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, const long *gstep,
             long g0, int ia, int oc)
@@ -255,20 +379,43 @@ k_step_split(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, con
     const bool interior = t.flags & WV_TILE_INTERIOR;
     const bool allact = t.flags & WV_TILE_ALLACT;
     const bool damp = d.damped;
-    switch (d.coef) {
-#define WV_RUN(COEF)                                                                          \
-    case COEF:                                                                                \
-        if (!interior) run_split<K, ST, false, COEF, false, true>(d, t, gbase, ia, oc);               \
-        else if (allact && !damp) run_split<K, ST, true, COEF, true, false>(d, t, gbase, ia, oc);     \
-        else if (allact) run_split<K, ST, true, COEF, true, true>(d, t, gbase, ia, oc);               \
-        else if (!damp) run_split<K, ST, true, COEF, false, false>(d, t, gbase, ia, oc);              \
-        else run_split<K, ST, true, COEF, false, true>(d, t, gbase, ia, oc);                          \
-        break;
-        WV_RUN(WV_COEF_UNIFORM)
-        WV_RUN(WV_COEF_TWO)
-        WV_RUN(WV_COEF_FIELD)
-#undef WV_RUN
+#define WV_RUN(CF, A, D) run_split<K, ST, true, CF, A, D>(d, t, gbase, ia, oc)
+#define WV_INTERIOR(CF)                            \
+    if (allact) {                                  \
+        if (damp) WV_RUN(CF, true, true);          \
+        else WV_RUN(CF, true, false);              \
+    } else {                                       \
+        if (damp) WV_RUN(CF, false, true);         \
+        else WV_RUN(CF, false, false);             \
     }
+#define WV_DAMPED(CF)                              \
+    if (allact) WV_RUN(CF, true, true);            \
+    else WV_RUN(CF, false, true);
+    // the modes this kernel is not built for are folded away
+    const int coef = CM == WV_CM_ALL ? d.coef : CM;
+    switch (coef) {
+    case WV_COEF_UNIFORM:
+        if (!interior) run_split<K, ST, false, CF_UNIFORM, false, true>(d, t, gbase, ia, oc);
+        else WV_INTERIOR(CF_UNIFORM)
+        break;
+    case WV_COEF_TWO:
+        if (!interior) run_split<K, ST, false, CF_TWO, false, true>(d, t, gbase, ia, oc);
+        else WV_INTERIOR(CF_TWO)
+        break;
+    default:
+        if (!interior) run_split<K, ST, false, CF_ANY, false, true>(d, t, gbase, ia, oc);
+        else if (d.tccf_lo) {
+            if (d.tgf32) { WV_DAMPED(CF_FIELDLG) }  // a tgamma field implies damping
+            else WV_INTERIOR(CF_FIELDL)
+        } else {
+            if (d.tgf32) { WV_DAMPED(CF_FIELDG) }
+            else WV_INTERIOR(CF_FIELD)
+        }
+        break;
+    }
+#undef WV_DAMPED
+#undef WV_INTERIOR
+#undef WV_RUN
 }
 
 }  // namespace
@@ -276,16 +423,28 @@ k_step_split(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, con
 #define WV_CAT2(a, b) a##b
 #define WV_CAT(a, b) WV_CAT2(a, b)
 
-int WV_CAT(wv_launch_split_k, WV_K)(int stencil, const WvInst *I, const WvTile *Ti, int ni, const WvTile *Tb,
+int WV_CAT(wv_launch_split_k, WV_K)(int stencil, int cm, const WvInst *I, const WvTile *Ti, int ni, const WvTile *Tb,
                                      int nb, const long *gs, long g0, int ia, int oc, hipStream_t s)
 {
     // the two tile lists are contiguous in memory: one launch covers both
     const WvTile *T = ni ? Ti : Tb;
     const int n = ni + nb;
     if (!n) return 0;
-    if (stencil == 9)
-        hipLaunchKernelGGL((k_step_split<WV_K, 9>), dim3(n), dim3(64), 0, s, I, T, gs, g0, ia, oc);
-    else
-        hipLaunchKernelGGL((k_step_split<WV_K, 5>), dim3(n), dim3(64), 0, s, I, T, gs, g0, ia, oc);
+    const dim3 grid(n), block(64);
+#define WV_GO(ST, CM) hipLaunchKernelGGL((k_step_split<WV_K, ST, CM>), grid, block, 0, s, I, T, gs, g0, ia, oc)
+#define WV_ST(ST)                                 \
+    switch (cm) {                                 \
+    case WV_COEF_UNIFORM: WV_GO(ST, WV_COEF_UNIFORM); break; \
+    case WV_COEF_TWO: WV_GO(ST, WV_COEF_TWO); break;         \
+    case WV_COEF_FIELD: WV_GO(ST, WV_COEF_FIELD); break;     \
+    default: WV_GO(ST, WV_CM_ALL); break;                     \
+    }
+    if (stencil == 9) {
+        WV_ST(9)
+    } else {
+        WV_ST(5)
+    }
+#undef WV_ST
+#undef WV_GO
     return (int)hipGetLastError();
 }

@@ -21,6 +21,7 @@ WV_OK, WV_EINVAL, WV_EHIP, WV_ESTATE, WV_EUNSUPPORTED, WV_ENOMEM = 0, -1, -2, -3
 BC_DIRICHLET, BC_PERIODIC, BC_ABSORBING, BC_VPER_HABS, BC_ABS_REFLECT = 0, 1, 2, 3, 4
 FORM_FP64, FORM_F32SPLIT = 0, 1
 FLAG_TABLES_FIXED = 1
+FLAG_FULL_FORMULA = 2
 
 # every symbol include/wavehip.h declares
 EXPORTS = [
@@ -224,10 +225,11 @@ class Wave:
         return ms.value, nl.value
 
     def kernel_ms(self):
-        """(interior, boundary) kernel time of the last step_timed, ms"""
+        """(average duration in ms, number) of the event-timed launches of the
+        last step_timed (one launch in eight, full fused depth)"""
         a, b = ctypes.c_double(), ctypes.c_double()
         self._chk(self._L.wv_last_kernel_ms(self._h, ctypes.byref(a), ctypes.byref(b)), "wv_last_kernel_ms")
-        return a.value, b.value
+        return a.value, int(b.value)
 
     def sync(self):
         self._chk(self._L.wv_sync(self._h), "wv_sync")
@@ -362,10 +364,12 @@ class Batch:
         return ms.value, nl.value
 
     def kernel_ms(self):
+        """(average duration in ms, number) of the event-timed launches of the
+        last step_timed"""
         a, b = ctypes.c_double(), ctypes.c_double()
         self._chk(self._L.wv_batch_last_kernel_ms(self._h, ctypes.byref(a), ctypes.byref(b)),
                   "wv_batch_last_kernel_ms")
-        return a.value, b.value
+        return a.value, int(b.value)
 
     def energy(self):
         e = np.empty(self.n)
