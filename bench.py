@@ -311,7 +311,7 @@ def main():
             "traffic": head["traffic"],
             "traffic_frac": (head["traffic"] / (head["avg_launch_ms"] * 1e-3) / HBM_PEAK
                              if head["traffic"] else None),
-            "kernel": ("k_step_split<%d,5>" if hname == "f32split" else "k_step_fp64<%d,5>") % head["k"],
+            "kernel": ("k_step_split<%d,5,0>" if hname == "f32split" else "k_step_fp64<%d,5,0>") % head["k"],
             "alg_bytes_per_launch": head["alg_bytes_per_launch"],
             "avg_launch_ms": head["avg_launch_ms"],
             "timed_launches": head["n_timed"],

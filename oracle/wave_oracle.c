@@ -353,16 +353,14 @@ void orc_split_to_fp64(long n, const float *hi, const float *lo, const float *v,
     }
 }
 
-/* u + w in double-float: two-sum of hi and w, then quick-two-sum renormalise */
+/* u + w for u = h + l: w takes the low word first (t = l + w, rounded at the
+ * increment's own precision), then one Fast2Sum with h */
 static inline void dd_add_f(float h, float l, float w, float *ho, float *lo)
 {
-    float s = h + w;
-    float bb = s - h;
-    float e = (h - (s - bb)) + (w - bb);
-    float t = l + e;
-    float hn = s + t;
+    float t = l + w;
+    float hn = h + t;
     *ho = hn;
-    *lo = t - (hn - s);
+    *lo = t - (hn - h);
 }
 
 void orc_split_run(const orc_params *p, float *hi, float *lo, float *v, long nsteps,

@@ -102,11 +102,11 @@ double orc_fixed_order_sum(int nx, int ny, const double *e);
 /* ---- FP32 double-float ("split") form, build-defined ----------------------
  * State per cell: u = hi + lo (two floats, |lo| <= ulp(hi)/2) and the
  * velocity v = u^n - u^{n-1} as one float.  Interior active cells:
- *   dh   = ((hE-hc) + (hW-hc)) + ((hN-hc) + (hS-hc))   (9-pt: see .c)
+ *   dh   = ((hE-hc) - (hc-hW)) + ((hN-hc) - (hc-hS))   (9-pt: see .c)
  *   dl   = same on lo;  lap = dh + dl
  *   vn   = ((vc + fmaf(tcc_hi, lap, tcc_lo*lap)) - kappa*hc) - gamma*vc
  *          (a term whose coefficient is zero is not evaluated)
- *   (hi', lo') = hc + lc + vn   (two-sum then quick-two-sum)
+ *   t = lc + vn;  hi' = hc + t;  lo' = t - (hi' - hc)   (Fast2Sum)
  * Edge cells use the FP64 reference formula on u = (double)hi + (double)lo,
  * u^{n-1} = u - (double)v.  Inactive cells are untouched. */
 void orc_split_from_fp64(long n, const double *u_n, const double *u_nm1,

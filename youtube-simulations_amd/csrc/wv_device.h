@@ -59,6 +59,24 @@ __device__ __forceinline__ double from_prev(double v)
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// Move a prefetched value out of its ring register with a copy the compiler
+// keeps.  Without it the copy is folded away, the ring register goes on living
+// as a window row, the next load into that ring slot gets another register,
+// and the loop back-edge copies it back -- which waits for that load and so
+// drains the prefetch ring once per loop trip.
+__device__ __forceinline__ double ring_take(double x)
+{
+    double y;
+    asm volatile("v_mov_b64 %0, %1" : "=v"(y) : "v"(x));
+    return y;
+}
+__device__ __forceinline__ float ring_take(float x)
+{
+    float y;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(y) : "v"(x));
+    return y;
+}
+
 // wave_billiard.c:373  -y + 2*x + tcc*delta - KAPPA*x - tgamma*(x-y)
 __device__ __forceinline__ double leap(double x, double y, double delta, double cc, double kappa,
                                        double g)
@@ -287,15 +305,17 @@ __device__ __forceinline__ T add_sources(const Src &q, int rm, int jm, long g, T
     return v;
 }
 
+// u + w for u = h + l (double-float): the increment w takes the low word first
+// (t = l + w, rounded at the increment's own binary32 precision), then one
+// Fast2Sum with h, exact whenever |h| >= |t| -- the normal case for a per-step
+// increment; near a zero crossing of u its error stays at the increment's
+// rounding level
 __device__ __forceinline__ void dd_add_f(float h, float l, float w, float &ho, float &lo)
 {
-    const float s = h + w;
-    const float bb = s - h;
-    const float e = (h - (s - bb)) + (w - bb);
-    const float t = l + e;
-    const float hn = s + t;
+    const float t = l + w;
+    const float hn = h + t;
     ho = hn;
-    lo = t - (hn - s);
+    lo = t - (hn - h);
 }
 
 // Tile geometry shared by the stepping kernels

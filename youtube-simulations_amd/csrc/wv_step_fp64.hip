@@ -221,8 +221,8 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
         // level 0 from the prefetch ring, then request row s+3 into the freed slot
 #pragma unroll
         for (int c = 0; c < C; c++) {
-            U[0][P][c] = QU[P][c];
-            Up[0][P][c] = QP[P][c];
+            U[0][P][c] = ring_take(QU[P][c]);
+            Up[0][P][c] = ring_take(QP[P][c]);
         }
         mreg = (mreg << C) | bits_of(QB[P], L);
         medreg = (medreg << C) | bits_of(QM[P], L);
@@ -230,11 +230,12 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
         fetch(min(s + 3, slast), QU[P], QP[P], QB[P], QM[P]);
     };
 
+    // one exit per trip (up to two trailing iterations past send-1 find every
+    // level dead and re-read the last row): an exit between the phases would
+    // make the compiler copy ring registers at the loop edges
     for (int s = s0; s < send; s += 3) {
         iter(s, std::integral_constant<int, 0>{});
-        if (s + 1 >= send) break;
         iter(s + 1, std::integral_constant<int, 1>{});
-        if (s + 2 >= send) break;
         iter(s + 2, std::integral_constant<int, 2>{});
     }
 }

@@ -8,8 +8,9 @@
 // rearranged so that no 2x - y cancellation happens in binary32: lap(u) is
 // summed from neighbour differences of hi and of lo separately (differences
 // of nearby values are exact), tcc is applied as a hi + lo pair, and u' is
-// formed with an exact two-sum.  Edge cells (outer rows and columns) are
-// evaluated in FP64 with the reference's own formulas.
+// formed by adding v' to lo and that sum to hi with a Fast2Sum.  Edge cells
+// (outer rows and columns) are evaluated in FP64 with the reference's own
+// formulas.
 //
 // Same tile, skewed pipeline and prefetch ring as the FP64 kernel
 // (wv_step_fp64.hip); a window entry is (hi, lo, v) of one row.
@@ -336,9 +337,9 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
         }
 #pragma unroll
         for (int c = 0; c < C; c++) {
-            Hw[0][P][c] = QH[P][c];
-            Lw[0][P][c] = QL[P][c];
-            Vw[0][P][c] = QV[P][c];
+            Hw[0][P][c] = ring_take(QH[P][c]);
+            Lw[0][P][c] = ring_take(QL[P][c]);
+            Vw[0][P][c] = ring_take(QV[P][c]);
         }
         mreg = (mreg << C) | bits_of(QB[P], L);
         medreg = (medreg << C) | bits_of(QM[P], L);
@@ -346,11 +347,12 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
         fetch(min(s + 3, slast), QH[P], QL[P], QV[P], QB[P], QM[P]);
     };
 
+    // one exit per trip (up to two trailing iterations past send-1 find every
+    // level dead and re-read the last row): an exit between the phases would
+    // make the compiler copy ring registers at the loop edges
     for (int s = s0; s < send; s += 3) {
         iter(s, std::integral_constant<int, 0>{});
-        if (s + 1 >= send) break;
         iter(s + 1, std::integral_constant<int, 1>{});
-        if (s + 2 >= send) break;
         iter(s + 2, std::integral_constant<int, 2>{});
     }
 }
