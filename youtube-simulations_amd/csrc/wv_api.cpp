@@ -212,11 +212,13 @@ void drop_tiles(Engine *e)
 
 // Tile table of a launch of depth k: every instance, row strips of rows(e)
 // rows, column blocks of wv_tile_cols(k) columns.  Workgroups are dealt to
-// the 8 XCDs round-robin by workgroup id, so the table interleaves strips:
-// ids 8m+x of one group of 8 strips all belong to strip x of the group, and
-// every column tile of a strip runs on the same XCD -- the halo columns two
-// neighbouring tiles both read, and the cache lines their outputs share,
-// stay in one L2.  Tiles with inst = -1 only pad the interleave.
+// the 8 XCDs round-robin by workgroup id, so position 8j + x of the table is
+// the j-th tile XCD x runs.  Each XCD gets one contiguous block of strips and
+// runs it strip after strip: the K halo rows a tile reads above and below
+// its strip were read moments before by the neighbouring strip on the same
+// XCD, so they come from that XCD's L2 instead of HBM, and so do the halo
+// columns two neighbouring tiles of a strip share.  Tiles with inst = -1 only
+// pad the shorter blocks.
 int get_tiles(Engine *e, int k, Engine::Tiles *out)
 {
     auto it = e->tiles.find(k);
@@ -256,18 +258,17 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
     std::vector<WvTile> h, hb;
     if (e->xcd_order) {
         const int X = 8;
-        for (size_t g0 = 0; g0 < strips.size(); g0 += X) {
-            const size_t g1 = std::min(strips.size(), g0 + X);
-            int maxcb = 0;
-            for (size_t q = g0; q < g1; q++) maxcb = std::max(maxcb, strips[q].ncb);
-            for (int cb = 0; cb < maxcb; cb++)
-                for (size_t q = g0; q < g0 + X; q++) {
-                    if (q < g1 && cb < strips[q].ncb)
-                        h.push_back(WvTile{strips[q].inst, cb, strips[q].sb, flags_of(strips[q].inst, cb, strips[q].sb)});
-                    else if (q < g1 || g1 == g0 + X)
-                        h.push_back(WvTile{-1, 0, 0, 0});
-                }
-        }
+        // XCD x: strips [x*S/X, (x+1)*S/X), strip-major
+        std::vector<std::vector<WvTile>> per(X);
+        const size_t S = strips.size();
+        for (int x = 0; x < X; x++)
+            for (size_t q = S * x / X; q < S * (x + 1) / X; q++)
+                for (int cb = 0; cb < strips[q].ncb; cb++)
+                    per[x].push_back(WvTile{strips[q].inst, cb, strips[q].sb, flags_of(strips[q].inst, cb, strips[q].sb)});
+        size_t len = 0;
+        for (const auto &v : per) len = std::max(len, v.size());
+        for (size_t j = 0; j < len; j++)
+            for (int x = 0; x < X; x++) h.push_back(j < per[x].size() ? per[x][j] : WvTile{-1, 0, 0, 0});
         while (!h.empty() && h.back().inst < 0) h.pop_back();
     } else {
         for (const Strip &st : strips)

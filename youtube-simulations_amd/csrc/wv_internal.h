@@ -30,6 +30,11 @@
 // fewer code paths and needs no more registers than those paths do
 #define WV_CM_ALL 3
 
+// rows in flight in a stepping kernel's prefetch ring (3 or 6)
+#ifndef WV_RING
+#define WV_RING 3
+#endif
+
 // cells per lane along j in the stepping kernels (a wave covers 64*WV_C columns)
 #ifndef WV_C
 #define WV_C 1

@@ -73,8 +73,8 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
     gptr<double> out0 = gm(d.u[oc]), out1 = gm(d.u[od]);
 
     double U[K][3][C], Up[K][3][C];  // windows: u^t and u^{t-1} per row
-    double QU[3][C], QP[3][C];       // prefetch ring: u^n, u^{n-1} rows
-    unsigned QB[3], QM[3];           // prefetched active / medium bit words
+    double QU[WV_RING][C], QP[WV_RING][C];  // prefetch ring: u^n, u^{n-1} rows
+    unsigned QB[WV_RING], QM[WV_RING];      // prefetched active / medium bit words
     unsigned mreg = 0, medreg = 0;   // bits of rows s-1, s-2, ... (C bits each, newest lowest)
 #pragma unroll
     for (int a = 0; a < K; a++)
@@ -96,9 +96,8 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
         qb = ALLACT ? ~0u : lane_word(act, ldw, rm, L);
         qm = COEF == WV_COEF_TWO ? lane_word(med, ldw, rm, L) : 0u;
     };
-    fetch(s0, QU[0], QP[0], QB[0], QM[0]);
-    fetch(s0 + 1, QU[1], QP[1], QB[1], QM[1]);
-    fetch(s0 + 2, QU[2], QP[2], QB[2], QM[2]);
+#pragma unroll
+    for (int q = 0; q < WV_RING; q++) fetch(min(s0 + q, slast), QU[q], QP[q], QB[q], QM[q]);
 
     // per-cell coefficients: level lv computes row s-2lv at iteration s and
     // row s+1-2lv at the next one, so each level loads its next row's values
@@ -119,7 +118,8 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
     }
 
     auto iter = [&](int s, auto phase) {
-        constexpr int P = decltype(phase)::value;
+        constexpr int PR = decltype(phase)::value;  // ring slot of row s
+        constexpr int P = PR % 3;
         constexpr int PE = (P + 2) % 3, PC = (P + 1) % 3, PW = P;
         // levels in decreasing order: level lv overwrites its own slot P (row
         // s-3 of level lv) after level lv+1 has read it, and reads level lv-1
@@ -221,22 +221,28 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
         // level 0 from the prefetch ring, then request row s+3 into the freed slot
 #pragma unroll
         for (int c = 0; c < C; c++) {
-            U[0][P][c] = ring_take(QU[P][c]);
-            Up[0][P][c] = ring_take(QP[P][c]);
+            U[0][P][c] = ring_take(QU[PR][c]);
+            Up[0][P][c] = ring_take(QP[PR][c]);
         }
-        mreg = (mreg << C) | bits_of(QB[P], L);
-        medreg = (medreg << C) | bits_of(QM[P], L);
+        mreg = (mreg << C) | bits_of(QB[PR], L);
+        medreg = (medreg << C) | bits_of(QM[PR], L);
         // unconditional (past the last needed row it re-reads that cached row)
-        fetch(min(s + 3, slast), QU[P], QP[P], QB[P], QM[P]);
+        fetch(min(s + WV_RING, slast), QU[PR], QP[PR], QB[PR], QM[PR]);
     };
 
     // one exit per trip (up to two trailing iterations past send-1 find every
     // level dead and re-read the last row): an exit between the phases would
     // make the compiler copy ring registers at the loop edges
-    for (int s = s0; s < send; s += 3) {
+    static_assert(WV_RING == 3 || WV_RING == 6, "ring of 3 or 6 rows");
+    for (int s = s0; s < send; s += WV_RING) {
         iter(s, std::integral_constant<int, 0>{});
         iter(s + 1, std::integral_constant<int, 1>{});
         iter(s + 2, std::integral_constant<int, 2>{});
+        if (WV_RING == 6) {
+            iter(s + 3, std::integral_constant<int, 3 % WV_RING>{});
+            iter(s + 4, std::integral_constant<int, 4 % WV_RING>{});
+            iter(s + 5, std::integral_constant<int, 5 % WV_RING>{});
+        }
     }
 }
 

@@ -77,8 +77,8 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
     gptr<float> hout = gm(d.hi[oc]), lout = gm(d.lo[oc]), vout = gm(d.v[oc]);
 
     float Hw[K][3][C], Lw[K][3][C], Vw[K][3][C];
-    float QH[3][C], QL[3][C], QV[3][C];
-    unsigned QB[3], QM[3];
+    float QH[WV_RING][C], QL[WV_RING][C], QV[WV_RING][C];
+    unsigned QB[WV_RING], QM[WV_RING];
     unsigned mreg = 0, medreg = 0;
 #pragma unroll
     for (int a = 0; a < K; a++)
@@ -101,9 +101,8 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
         qb = ALLACT ? ~0u : lane_word(act, ldw, rm, L);
         qm = COEF == WV_COEF_TWO ? lane_word(med, ldw, rm, L) : 0u;
     };
-    fetch(s0, QH[0], QL[0], QV[0], QB[0], QM[0]);
-    fetch(s0 + 1, QH[1], QL[1], QV[1], QB[1], QM[1]);
-    fetch(s0 + 2, QH[2], QL[2], QV[2], QB[2], QM[2]);
+#pragma unroll
+    for (int q = 0; q < WV_RING; q++) fetch(min(s0 + q, slast), QH[q], QL[q], QV[q], QB[q], QM[q]);
 
     // per-cell coefficients, one iteration ahead per level (see the FP64 kernel)
     float TH[K][C], TL[K][C], TG[K][C];
@@ -128,7 +127,8 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
     for (int a = 0; a < K; a++) DEh[a] = DEl[a] = DNEh[a] = DNEl[a] = DSEh[a] = DSEl[a] = 0.0f;
 
     auto iter = [&](int s, auto phase) {
-        constexpr int P = decltype(phase)::value;
+        constexpr int PR = decltype(phase)::value;  // ring slot of row s
+        constexpr int P = PR % 3;
         constexpr int PE = (P + 2) % 3, PC = (P + 1) % 3, PW = P;
         // levels in decreasing order: each level overwrites its own slot P in
         // place (see the FP64 kernel)
@@ -337,23 +337,29 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
         }
 #pragma unroll
         for (int c = 0; c < C; c++) {
-            Hw[0][P][c] = ring_take(QH[P][c]);
-            Lw[0][P][c] = ring_take(QL[P][c]);
-            Vw[0][P][c] = ring_take(QV[P][c]);
+            Hw[0][P][c] = ring_take(QH[PR][c]);
+            Lw[0][P][c] = ring_take(QL[PR][c]);
+            Vw[0][P][c] = ring_take(QV[PR][c]);
         }
-        mreg = (mreg << C) | bits_of(QB[P], L);
-        medreg = (medreg << C) | bits_of(QM[P], L);
+        mreg = (mreg << C) | bits_of(QB[PR], L);
+        medreg = (medreg << C) | bits_of(QM[PR], L);
         // unconditional (past the last needed row it re-reads that cached row)
-        fetch(min(s + 3, slast), QH[P], QL[P], QV[P], QB[P], QM[P]);
+        fetch(min(s + WV_RING, slast), QH[PR], QL[PR], QV[PR], QB[PR], QM[PR]);
     };
 
     // one exit per trip (up to two trailing iterations past send-1 find every
     // level dead and re-read the last row): an exit between the phases would
     // make the compiler copy ring registers at the loop edges
-    for (int s = s0; s < send; s += 3) {
+    static_assert(WV_RING == 3 || WV_RING == 6, "ring of 3 or 6 rows");
+    for (int s = s0; s < send; s += WV_RING) {
         iter(s, std::integral_constant<int, 0>{});
         iter(s + 1, std::integral_constant<int, 1>{});
         iter(s + 2, std::integral_constant<int, 2>{});
+        if (WV_RING == 6) {
+            iter(s + 3, std::integral_constant<int, 3 % WV_RING>{});
+            iter(s + 4, std::integral_constant<int, 4 % WV_RING>{});
+            iter(s + 5, std::integral_constant<int, 5 % WV_RING>{});
+        }
     }
 }
 
