@@ -101,11 +101,15 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def run_single(wh, w, form, nsteps, energy_every=0, tb_depth=0, rows=0):
+def run_single(wh, w, form, nsteps, energy_every=0, tb_depth=0, rows=0, tune=0):
     """one grid through nsteps; energy sampled every energy_every steps (the
-    samples are part of the timed work); returns state, energies, time"""
+    samples are part of the timed work); returns state, energies, time.
+    tune > 0: autotune the tiling first (tune steps per candidate; the tuner
+    restores the state, and its time is not counted)"""
     W = wh.Wave(w["params"], form=form, tb_depth=tb_depth, rows_per_tile=rows)
     W.load(w)
+    if tune:
+        W.autotune(tune)
     W.sync()
     es = [W.energy()]
     t0 = time.perf_counter()
@@ -156,9 +160,10 @@ def configs_report(wh, workloads, which):
     if "c2" in which:
         w = workloads.c2()
         cells = w["params"]["nx"] * w["params"]["ny"]
-        f = run_single(wh, w, wh.FORM_FP64, 5000, 50)
-        s = run_single(wh, w, wh.FORM_F32SPLIT, 5000, 50)
+        f = run_single(wh, w, wh.FORM_FP64, 5000, 50, tune=96)
+        s = run_single(wh, w, wh.FORM_F32SPLIT, 5000, 50, tune=96)
         rep["c2"] = dict(grid=[2560, 1440], steps=5000, stencil=9, energy_every=50,
+                         tiling_fp64=[f["k"], f["L"]], tiling_split=[s["k"], s["L"]],
                          gcells_fp64=cells * 5000 / f["dt"] / 1e9, gcells_split=cells * 5000 / s["dt"] / 1e9,
                          rel_l2_split=rel_l2(s["u"], f["u"]),
                          energy_series_max_rel_diff=float(np.abs(s["e"] - f["e"]).max() / np.abs(f["e"]).max()),
@@ -181,12 +186,14 @@ def configs_report(wh, workloads, which):
         out = {}
         for name, form in (("fp64", wh.FORM_FP64), ("split", wh.FORM_F32SPLIT)):
             B = wh.Batch([w["params"] for w in ws], form=form)
+            for k, w in enumerate(ws):
+                B.load(k, w)
+            B.autotune(48)           # restores every grid's state
+            tiling = B.tiling()[:2]
             best = None
             # each pass restarts from the initial state; the rotation of the
             # buffers returns to a captured (nsteps, rotation) graph within a
             # few passes, so the fastest pass is a pure graph replay
-            for k, w in enumerate(ws):
-                B.load(k, w)
             for _ in range(5):
                 for k, w in enumerate(ws):
                     B.set_state(k, w["phi"], w["psi"])   # keeps the tables and the graphs
@@ -198,10 +205,11 @@ def configs_report(wh, workloads, which):
                 best = dt if best is None else min(best, dt)
             states = [B.get_state(k)[0] for k in range(len(ws))]
             B.close()
-            out[name] = dict(e0=e0, e1=e1, u=states, dt=best)
+            out[name] = dict(e0=e0, e1=e1, u=states, dt=best, tiling=list(tiling))
         rels = [rel_l2(a, b) for a, b in zip(out["split"]["u"], out["fp64"]["u"])]
         dr = lambda o: (o["e1"] - o["e0"]) / o["e0"]
         rep["c4"] = dict(instances=len(ws), total_cells=cells, steps=2000,
+                         tiling_fp64=out["fp64"]["tiling"], tiling_split=out["split"]["tiling"],
                          gcells_fp64=cells * 2000 / out["fp64"]["dt"] / 1e9,
                          gcells_split=cells * 2000 / out["split"]["dt"] / 1e9,
                          rel_l2_split_max=float(max(rels)), rel_l2_split_median=float(np.median(rels)),
@@ -211,8 +219,8 @@ def configs_report(wh, workloads, which):
     if "c5" in which:
         w = workloads.c5()
         cells = w["params"]["nx"] * w["params"]["ny"]
-        f = run_single(wh, w, wh.FORM_FP64, 20000, 100)
-        s = run_single(wh, w, wh.FORM_F32SPLIT, 20000, 100)
+        f = run_single(wh, w, wh.FORM_FP64, 20000, 100, tune=48)
+        s = run_single(wh, w, wh.FORM_F32SPLIT, 20000, 100, tune=48)
         rep["c5"] = dict(grid=[8192, 8192], steps=20000, stencil=9, energy_every=100, sources=20,
                          gcells_fp64=cells * 20000 / f["dt"] / 1e9, gcells_split=cells * 20000 / s["dt"] / 1e9,
                          rel_l2_split=rel_l2(s["u"], f["u"]),

@@ -168,6 +168,8 @@ int wv_batch_last_kernel_ms(const wv_batch *b, double *avg_ms, double *timed_lau
 int wv_batch_energy(wv_batch *b, double *e /* n values */);
 int wv_batch_set_tiling(wv_batch *b, int tb_depth, int rows_per_tile);
 int wv_batch_get_tiling(const wv_batch *b, int *tb_depth, int *rows_per_tile, int *tile_cols);
+/* wv_autotune over all grids of the batch (one tiling for every launch) */
+int wv_batch_autotune(wv_batch *b, long steps_per_trial, double *best_gcells);
 
 #ifdef __cplusplus
 }

@@ -352,6 +352,35 @@ def test_autotune_keeps_state(wh):
     W.close()
 
 
+def test_batch_autotune_keeps_state_and_results(wh, oracle):
+    """Tuning a batch leaves every grid's state untouched; the tuned tiling
+    then steps every grid bit-identically to the oracle."""
+    from wavehip import workloads
+    ws = []
+    for k, (nx, ny, bc) in enumerate(((90, 130, 2), (201, 77, 1), (64, 300, 0))):
+        w = workloads.random_case(nx, ny, 40 + k, bcond=bc, twospeeds=1, neg_cells=False)
+        w["phi"][w["xy_in"] == 0] = 0
+        w["psi"][w["xy_in"] == 0] = 0
+        ws.append(w)
+    say("batch autotune")
+    B = wh.Batch([w["params"] for w in ws])
+    for k, w in enumerate(ws):
+        B.load(k, w)
+    B.step(3)
+    before = [B.get_state(k) for k in range(len(ws))]
+    assert B.autotune(8) > 0
+    for k in range(len(ws)):
+        a, b = B.get_state(k)
+        assert bits_equal(a, before[k][0]) and bits_equal(b, before[k][1])
+    B.step(20)
+    for k, w in enumerate(ws):
+        p = oracle.make_params(w["params"]["nx"], w["params"]["ny"], bcond=w["params"]["bcond"], twospeeds=1)
+        ref = oracle.run(p, w["phi"], w["psi"], 23, w["xy_in"], w["tcc"], w["tc"], w["tgamma"])
+        got = B.get_state(k)
+        assert bits_equal(got[0], ref[0]) and bits_equal(got[1], ref[1]), k
+    B.close()
+
+
 def test_error_paths(wh):
     from wavehip import workloads
     w = workloads.random_case(20, 20, 1, neg_cells=False)

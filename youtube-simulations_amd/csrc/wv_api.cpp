@@ -1196,37 +1196,49 @@ int wv_batch_get_tiling(const wv_batch *b, int *tb_depth, int *rows_per_tile, in
 }  // extern "C"
 
 // ---- autotuner ----------------------------------------------------------------
-// Times every (depth, rows per tile) candidate on the current state and keeps
-// the fastest.  The state is saved to spare device buffers first and restored
-// afterwards, so tuning does not advance the simulation.
-extern "C" int wv_autotune(wv_ctx *c, long steps_per_trial, double *best_gcells)
+// Times every (depth, rows per tile) candidate on the current state of every
+// grid of the engine and keeps the fastest.  The states are saved to spare
+// device buffers first and restored afterwards, so tuning does not advance
+// the simulation.
+static int engine_autotune(Engine *e, long steps_per_trial, double *best_gcells)
 {
-    if (!c) return WV_EINVAL;
-    Engine *e = c->e;
     int rc = ready(e);
     if (rc) return rc;
     if (steps_per_trial < 1) return fail(e, WV_EINVAL, "steps_per_trial must be >= 1");
-    Grid &G = e->g[0];
     HIPCHK(e, hipSetDevice(e->device));
     HIPCHK(e, hipStreamSynchronize(e->stream));
-    const size_t cells = (size_t)G.p.nx * G.ldy;
     // save
-    std::vector<void *> save;
     std::vector<std::pair<void *, size_t>> what;
-    if (e->form == WV_FORM_FP64) {
-        for (int k = 0; k < 4; k++) what.push_back({G.u[k], cells * 8});
-    } else {
-        for (int k = 0; k < 2; k++) {
-            what.push_back({G.hi[k], cells * 4});
-            what.push_back({G.lo[k], cells * 4});
-            what.push_back({G.v[k], cells * 4});
+    double cells_total = 0;
+    for (const Grid &G : e->g) {
+        const size_t cells = (size_t)G.p.nx * G.ldy;
+        cells_total += (double)G.p.nx * G.p.ny;
+        if (e->form == WV_FORM_FP64) {
+            for (int k = 0; k < 4; k++) what.push_back({G.u[k], cells * 8});
+        } else {
+            for (int k = 0; k < 2; k++) {
+                what.push_back({G.hi[k], cells * 4});
+                what.push_back({G.lo[k], cells * 4});
+                what.push_back({G.v[k], cells * 4});
+            }
         }
     }
+    std::vector<void *> save;
+    auto release = [&]() {
+        for (void *q : save) (void)hipFree(q);
+        save.clear();
+    };
     for (auto &w : what) {
-        void *p = nullptr;
-        HIPCHK(e, hipMalloc(&p, w.second));
-        HIPCHK(e, hipMemcpy(p, w.first, w.second, hipMemcpyDeviceToDevice));
-        save.push_back(p);
+        void *q = nullptr;
+        if (hipMalloc(&q, w.second) != hipSuccess) {
+            release();
+            return fail(e, WV_ENOMEM, "no device memory to save the state while tuning");
+        }
+        save.push_back(q);
+        if (hipMemcpy(q, w.first, w.second, hipMemcpyDeviceToDevice) != hipSuccess) {
+            release();
+            return fail(e, WV_EHIP, "saving the state failed");
+        }
     }
     int role0[4];
     memcpy(role0, e->role, sizeof role0);
@@ -1248,16 +1260,15 @@ extern "C" int wv_autotune(wv_ctx *c, long steps_per_trial, double *best_gcells)
             long nl = 0;
             if ((rc = engine_step_timed(e, std::max<long>(1, steps_per_trial / 4), &ms, &nl))) break;  // warm
             if ((rc = engine_step_timed(e, steps_per_trial, &ms, &nl))) break;
-            const double gc = (double)G.p.nx * G.p.ny * steps_per_trial / (ms * 1e6);
+            const double gc = cells_total * steps_per_trial / (ms * 1e6);
             if (gc > best) { best = gc; bk = k; bl = L; }
         }
         if (rc) break;
     }
     // restore
-    for (size_t q = 0; q < what.size(); q++) {
+    for (size_t q = 0; q < what.size(); q++)
         (void)hipMemcpy(what[q].first, save[q], what[q].second, hipMemcpyDeviceToDevice);
-        (void)hipFree(save[q]);
-    }
+    release();
     (void)hipMemcpy(e->d_gstep, &gstep0, sizeof(long), hipMemcpyHostToDevice);
     memcpy(e->role, role0, sizeof role0);
     e->scur = s0;
@@ -1269,4 +1280,16 @@ extern "C" int wv_autotune(wv_ctx *c, long steps_per_trial, double *best_gcells)
     if ((rc = engine_set_tiling(e, bk, bl))) return rc;
     if (best_gcells) *best_gcells = best;
     return WV_OK;
+}
+
+extern "C" int wv_autotune(wv_ctx *c, long steps_per_trial, double *best_gcells)
+{
+    if (!c) return WV_EINVAL;
+    return engine_autotune(c->e, steps_per_trial, best_gcells);
+}
+
+extern "C" int wv_batch_autotune(wv_batch *b, long steps_per_trial, double *best_gcells)
+{
+    if (!b) return WV_EINVAL;
+    return engine_autotune(b->e, steps_per_trial, best_gcells);
 }

@@ -34,7 +34,7 @@ EXPORTS = [
     "wv_batch_last_error", "wv_batch_set_tables", "wv_batch_set_state", "wv_batch_get_state",
     "wv_batch_step", "wv_batch_step_timed", "wv_batch_last_kernel_ms", "wv_batch_energy",
     "wv_batch_set_tiling",
-    "wv_batch_get_tiling",
+    "wv_batch_get_tiling", "wv_batch_autotune",
 ]
 
 
@@ -110,6 +110,7 @@ def load():
         "wv_batch_energy": (ctypes.c_int, [vp, _D]),
         "wv_batch_set_tiling": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
         "wv_batch_get_tiling": (ctypes.c_int, [vp, _I, _I, _I]),
+        "wv_batch_autotune": (ctypes.c_int, [vp, ctypes.c_long, _D]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -385,3 +386,8 @@ class Batch:
         self._chk(self._L.wv_batch_get_tiling(self._h, ctypes.byref(k), ctypes.byref(L), ctypes.byref(w)),
                   "wv_batch_get_tiling")
         return k.value, L.value, w.value
+
+    def autotune(self, steps_per_trial=64):
+        g = ctypes.c_double()
+        self._chk(self._L.wv_batch_autotune(self._h, int(steps_per_trial), ctypes.byref(g)), "wv_batch_autotune")
+        return g.value
