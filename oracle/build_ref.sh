@@ -59,6 +59,19 @@ for g in "24 17" "16 29"; do
 done
 # the CPU-runnable plain workload: 1280x720, 5-point, B_COND 0, courant 0.1
 build_one "row_1280x720_bc0_ts0" 0 -DNX=1280 -DNY=720 -DB_COND=0 -DTWOSPEEDS=0
+# the features of the checked-in wave_billiard.c the plain builds leave off:
+# Neumann ghost points, the oscillating left edge (periodic schedule) and the
+# FLOOR clamp; 16x29 so that the corner write at phi_out[0][NX-1]
+# (wave_billiard.c:589) stays inside the row
+for bc in 0 1 2; do
+    build_one "row_16x29_bc${bc}_ts0_ext" 0 -DNX=16 -DNY=29 -DB_COND=$bc -DTWOSPEEDS=0 -DBC_NEUMANN=1 \
+        -DOSCILLATE_LEFT=1 -DOSCILLATION_SCHEDULE=0 -DFLOOR=1 -DVMAX=0.9 -DOMEGA=0.3 $KS &
+    pids+=($!)
+done
+build_one "row_16x29_bc2_ts1_ext" 0 -DNX=16 -DNY=29 -DB_COND=2 -DTWOSPEEDS=1 -DBC_NEUMANN=1 \
+    -DOSCILLATE_LEFT=1 -DOSCILLATION_SCHEDULE=0 -DFLOOR=1 -DVMAX=0.9 -DOMEGA=0.3 $KS &
+pids+=($!)
+for p in "${pids[@]}"; do wait "$p"; done; pids=()
 # a lens medium with damping and absorbing edges (energy fixture, two speeds)
 build_one "row_64x48_bc2_ts1" 0 -DNX=64 -DNY=48 -DB_COND=2 -DTWOSPEEDS=1 -DCOURANTB=0.0666666666666666667 $KS
 ls "$OUT" | wc -l

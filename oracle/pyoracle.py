@@ -38,6 +38,17 @@ def make_params(nx, ny, bcond=0, twospeeds=0, stencil=5, oscillate_topbot=0, tb_
                      kappa, kappa_sides, gamma_sides, kappa_topbot, gamma_topbot)
 
 
+class OrcExtras(ctypes.Structure):
+    """orc_extras of wave_oracle.h (Neumann ghost points, driven cells, FLOOR)."""
+    _fields_ = [
+        ("n_neu", ctypes.c_int),
+        ("ni", _I), ("nj", _I), ("nbi1", _I), ("nbj1", _I), ("nbi2", _I), ("nbj2", _I),
+        ("nnx", _D), ("nny", _D),
+        ("n_drv", ctypes.c_int), ("di", _I), ("dj", _I), ("dval", _D), ("dlen", ctypes.c_long),
+        ("vmax", ctypes.c_double),
+    ]
+
+
 def _p(a, t):
     return a.ctypes.data_as(t)
 
@@ -73,6 +84,8 @@ def lib():
         L.orc_split_to_fp64.argtypes = [ctypes.c_long, _F, _F, _F, _D, _D]
         L.orc_split_run.argtypes = [P, _F, _F, _F, ctypes.c_long, _S, _D, _D, _D, ctypes.c_int,
                                     _I, _I, _D, ctypes.c_long, ctypes.c_long]
+        L.orc_run_x.argtypes = [P, _D, _D, _D, _I, ctypes.c_long, _S, _D, _D, _D, ctypes.c_int,
+                                _I, _I, _D, ctypes.c_long, ctypes.c_long, ctypes.POINTER(OrcExtras)]
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_split_energy_total.restype = ctypes.c_double
         L.orc_split_energy_total.argtypes = [P, _F, _F, _F, _S, _D, ctypes.c_double]
@@ -106,6 +119,55 @@ def run(p, phi, psi, nsteps, xy_in, tcc, tc, tgamma, tmp=None, sources=None, ste
     lib().orc_run(ctypes.byref(p), _p(b[0], _D), _p(b[1], _D), _p(b[2], _D), ctypes.byref(st),
                   nsteps, _p(xy_in, _S), _p(tcc, _D), _p(tc, _D), _p(tgamma, _D), nsrc,
                   _p(si, _I), _p(sj, _I), _p(amp, _D), alen, step0)
+    s = st.value
+    return b[s], b[(s + 1) % 3], b[(s + 2) % 3]
+
+
+def run_x(p, phi, psi, nsteps, xy_in, tcc, tc, tgamma, tmp=None, sources=None, step0=0,
+          neumann=None, drive=None, vmax=0.0):
+    """orc_run with the reference's Neumann ghost points, driven cells and
+    FLOOR clamp.  neumann: dict of int arrays i, j, nbi1, nbj1, nbi2, nbj2 and
+    float arrays nx, ny; drive: dict i, j, val (n x len)."""
+    b = [np.array(phi, dtype=np.float64, copy=True), np.array(psi, dtype=np.float64, copy=True),
+         np.zeros_like(phi, dtype=np.float64) if tmp is None else np.array(tmp, dtype=np.float64)]
+    st = ctypes.c_int(0)
+    nsrc, si, sj, amp, alen = _sources(sources)
+    keep = []
+
+    def ia(a):
+        a = np.ascontiguousarray(a, np.int32)
+        keep.append(a)
+        return _p(a, _I)
+
+    def da(a):
+        a = np.ascontiguousarray(a, np.float64)
+        keep.append(a)
+        return _p(a, _D)
+
+    x = OrcExtras()
+    z = np.zeros(1, np.int32)
+    if neumann:
+        x.n_neu = len(neumann["i"])
+        x.ni, x.nj = ia(neumann["i"]), ia(neumann["j"])
+        x.nbi1, x.nbj1 = ia(neumann["nbi1"]), ia(neumann["nbj1"])
+        x.nbi2, x.nbj2 = ia(neumann["nbi2"]), ia(neumann["nbj2"])
+        x.nnx, x.nny = da(neumann["nx"]), da(neumann["ny"])
+    else:
+        x.n_neu = 0
+        x.ni = x.nj = x.nbi1 = x.nbj1 = x.nbi2 = x.nbj2 = ia(z)
+        x.nnx = x.nny = da(np.zeros(1))
+    if drive:
+        val = np.ascontiguousarray(drive["val"], np.float64)
+        x.n_drv, x.dlen = val.shape[0], val.shape[1]
+        x.di, x.dj, x.dval = ia(drive["i"]), ia(drive["j"]), da(val.reshape(-1))
+    else:
+        x.n_drv, x.dlen = 0, 0
+        x.di = x.dj = ia(z)
+        x.dval = da(np.zeros(1))
+    x.vmax = vmax
+    lib().orc_run_x(ctypes.byref(p), _p(b[0], _D), _p(b[1], _D), _p(b[2], _D), ctypes.byref(st),
+                    nsteps, _p(xy_in, _S), _p(tcc, _D), _p(tc, _D), _p(tgamma, _D), nsrc,
+                    _p(si, _I), _p(sj, _I), _p(amp, _D), alen, step0, ctypes.byref(x))
     s = st.value
     return b[s], b[(s + 1) % 3], b[(s + 2) % 3]
 
@@ -188,6 +250,20 @@ class RefBuild:
             self.L.ref_evolve_wave.argtypes = [_D, _D, _D, _S, _D, _D, _D]
             self.L.ref_compute_energy.restype = ctypes.c_double
             self.L.ref_compute_energy.argtypes = [_D, _D, _S, ctypes.c_int, ctypes.c_int]
+        if hasattr(self.L, "ref_set_neumann"):
+            self.L.ref_set_neumann.argtypes = [ctypes.c_int] + [_I] * 6 + [_D, _D]
+            self.L.ref_oscillating_bc.restype = ctypes.c_double
+            self.L.ref_oscillating_bc.argtypes = [ctypes.c_int, ctypes.c_int]
+
+    def set_neumann(self, nb):
+        """hand Neumann ghost points to the reference stepper (dict as run_x)"""
+        arrs = [np.ascontiguousarray(nb[k], np.int32) for k in ("i", "j", "nbi1", "nbj1", "nbi2", "nbj2")]
+        fl = [np.ascontiguousarray(nb[k], np.float64) for k in ("nx", "ny")]
+        rc = self.L.ref_set_neumann(len(arrs[0]), *[_p(a, _I) for a in arrs], *[_p(a, _D) for a in fl])
+        assert rc == 0
+
+    def oscillating_bc(self, time, j):
+        return self.L.ref_oscillating_bc(time, j)
 
     def half(self, phi_in, psi_in, phi_out, xy_in, tcc, tc, tgamma):
         if self.flat:

@@ -4,10 +4,34 @@
  * tests can call them through ctypes.  FLAT_TWIN selects which reference
  * stepper was compiled in (wave_billiard.c row-pointer or wave_3d.c flat).
  */
-static t_neumann_bc ref_no_neumann[1];
+/* Neumann ghost points handed to the reference stepper (n_neumann_points of
+ * global_pdes.c says how many are used; none until ref_set_neumann) */
+#define REF_NEU_MAX 4096
+static t_neumann_bc ref_neumann[REF_NEU_MAX];
+__attribute__((constructor)) static void ref_init(void) { n_neumann_points = 0; }
 
 int ref_nx(void) { return NX; }
 int ref_ny(void) { return NY; }
+
+int ref_set_neumann(int n, const int *pi, const int *pj, const int *bi1, const int *bj1,
+                    const int *bi2, const int *bj2, const double *nx, const double *ny)
+{
+    if (n < 0 || n > REF_NEU_MAX) return -1;
+    memset(ref_neumann, 0, sizeof ref_neumann);
+    for (int k = 0; k < n; k++) {
+        ref_neumann[k].i = pi[k]; ref_neumann[k].j = pj[k];
+        ref_neumann[k].nbi1 = bi1[k]; ref_neumann[k].nbj1 = bj1[k];
+        ref_neumann[k].nbi2 = bi2[k]; ref_neumann[k].nbj2 = bj2[k];
+        ref_neumann[k].nx = nx[k]; ref_neumann[k].ny = ny[k];
+        ref_neumann[k].active = 1;
+    }
+    n_neumann_points = n;
+    return 0;
+}
+
+/* the left-edge drive, sub_wave.c:11965 */
+double ref_oscillating_bc(int time, int j) { return oscillating_bc(time, j); }
+int ref_beam_bc(void) { return BEAM_BC; }
 
 #ifndef FLAT_TWIN
 /* evolve_wave_half, wave_billiard.c:302 */
@@ -21,7 +45,7 @@ void ref_evolve_wave_half(double *phi_in, double *psi_in, double *phi_out, short
         m[i] = xy_in + (size_t)i * NY;
         t1[i] = tcc + (size_t)i * NY; t2[i] = tc + (size_t)i * NY; t3[i] = tgamma + (size_t)i * NY;
     }
-    evolve_wave_half(a, b, c, m, t1, t2, t3, ref_no_neumann);
+    evolve_wave_half(a, b, c, m, t1, t2, t3, ref_neumann);
 }
 
 /* evolve_wave, wave_billiard.c:613 */
@@ -35,7 +59,7 @@ void ref_evolve_wave(double *phi, double *psi, double *tmp, short *xy_in, double
         m[i] = xy_in + (size_t)i * NY;
         t1[i] = tcc + (size_t)i * NY; t2[i] = tc + (size_t)i * NY; t3[i] = tgamma + (size_t)i * NY;
     }
-    evolve_wave(a, b, c, m, t1, t2, t3, ref_no_neumann);
+    evolve_wave(a, b, c, m, t1, t2, t3, ref_neumann);
 }
 
 /* compute_energy, wave_common.c:453 */
@@ -53,6 +77,6 @@ double ref_compute_energy(double *phi, double *psi, short *xy_in, int i, int j)
 void ref_evolve_wave_half_flat(double *phi_in, double *psi_in, double *phi_out, short *xy_in,
                                double *tc, double *tcc, double *tgamma)
 {
-    evolve_wave_half(phi_in, psi_in, phi_out, xy_in, tc, tcc, tgamma, ref_no_neumann);
+    evolve_wave_half(phi_in, psi_in, phi_out, xy_in, tc, tcc, tgamma, ref_neumann);
 }
 #endif

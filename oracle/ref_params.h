@@ -54,8 +54,12 @@
 #define XMAX 2.0
 #define YMIN -1.125
 #define YMAX 1.125
+#ifndef OSCILLATE_LEFT
 #define OSCILLATE_LEFT 0
+#endif
+#ifndef OSCILLATION_SCHEDULE
 #define OSCILLATION_SCHEDULE 0
+#endif
 #define OSCIL_YMAX 1.2
 #define OSCIL_YMID 0.0
 #define OSCIL_LEFT_YSHIFT 0.0
@@ -67,9 +71,15 @@
 #define ACHIRP 0.25
 #define DAMPING 0.0
 #define INITIAL_VARIANCE 0.0003
+#ifndef BC_NEUMANN
 #define BC_NEUMANN 0
+#endif
 #define COMPARISON 0
+#ifndef FLOOR
 #define FLOOR 0
+#endif
+#ifndef VMAX
 #define VMAX 2.5
+#endif
 #define NSTEPS 10
 #define BEAM_BC ((OSCILLATION_SCHEDULE == OSC_BEAM_SINE)||(OSCILLATION_SCHEDULE == OSC_BEAM_SINE_DECREASING)||(OSCILLATION_SCHEDULE == OSC_BEAM_SINE_TWOPERIODS)||(OSCILLATION_SCHEDULE == OSC_BEAM_SINE_CHIRP))

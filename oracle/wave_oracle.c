@@ -330,6 +330,50 @@ double orc_energy_total(const orc_params *p, const double *phi, const double *ps
     return t;
 }
 
+void orc_run_x(const orc_params *p, double *b0, double *b1, double *b2, int *state,
+               long nsteps, const short *xy_in, const double *tcc, const double *tc,
+               const double *tgamma, int nsrc, const int *si, const int *sj,
+               const double *amp, long amp_len, long step0, const orc_extras *x)
+{
+    double **R[3];
+    double *base[3] = {b0, b1, b2};
+    for (int k = 0; k < 3; k++) {
+        R[k] = (double **)malloc(sizeof(double *) * (size_t)p->nx);
+        for (int r = 0; r < p->nx; r++) R[k][r] = base[k] + (size_t)r * p->ny;
+    }
+    ROWS(m, xy_in, short); ROWS(t1, tcc, double); ROWS(t2, tc, double); ROWS(t3, tgamma, double);
+    int s = *state;
+    for (long n = 0; n < nsteps; n++) {
+        double **phi = R[s], **psi = R[(s + 1) % 3], **tmp = R[(s + 2) % 3];
+        const long g = step0 + n; /* amplitude index of this step */
+        bulk_step(p, phi, psi, tmp, m, t1, t3);
+        /* wave_billiard.c:379-389 */
+        for (int k = 0; k < x->n_neu; k++) {
+            double v = x->nnx[k] * phi[x->nbi1[k]][x->nbj1[k]];
+            v += x->nny[k] * phi[x->nbi2[k]][x->nbj2[k]];
+            tmp[x->ni[k]][x->nj[k]] = v;
+        }
+        edge_step(p, phi, psi, tmp, m, t1, t2, t3);
+        /* wave_billiard.c:397, 585-590 */
+        if (g < x->dlen)
+            for (int k = 0; k < x->n_drv; k++) tmp[x->di[k]][x->dj[k]] = x->dval[(size_t)k * x->dlen + g];
+        /* wave_billiard.c:592-609 */
+        if (x->vmax > 0.0)
+            for (int i = 0; i < p->nx; i++)
+                for (int j = 0; j < p->ny; j++)
+                    if (m[i][j] != 0) {
+                        if (tmp[i][j] > x->vmax) tmp[i][j] = x->vmax;
+                        else if (tmp[i][j] < -x->vmax) tmp[i][j] = -x->vmax;
+                    }
+        if (g < amp_len)
+            for (int k = 0; k < nsrc; k++) tmp[si[k]][sj[k]] += amp[(size_t)k * amp_len + g];
+        s = (s + 2) % 3;
+    }
+    *state = s;
+    for (int k = 0; k < 3; k++) free(R[k]);
+    free(m); free(t1); free(t2); free(t3);
+}
+
 /* ------------------------------------------------------------ split form -- */
 
 void orc_split_from_fp64(long n, const double *u_n, const double *u_nm1, float *hi, float *lo,

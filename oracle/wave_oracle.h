@@ -83,6 +83,33 @@ void orc_run(const orc_params *p, double *b0, double *b1, double *b2, int *state
              const double *tgamma, int nsrc, const int *si, const int *sj,
              const double *amp, long amp_len, long step0);
 
+/* Reference features outside the plain stepper, applied by orc_run_x in the
+ * order of evolve_wave_half (wave_billiard.c:302-610):
+ *   bulk (361-377) -> Neumann ghost points (379-389) -> edges (393-583,
+ *   where a driven left-edge cell takes its drive value instead of the edge
+ *   formula, 397) -> driven corners (585-590) -> FLOOR (592-609)
+ * and then the build-defined point sources.
+ * Neumann point k: phi_out[ni][nj] = nnx*phi_in[nbi1][nbj1] + nny*phi_in[nbi2][nbj2]
+ * (two roundings, as phi_out = a; phi_out += b).  Driven cell k: after step
+ * g (g = step0 + m + 1), while g-1 < dlen, phi_out[di][dj] = dval[k*dlen+g-1]
+ * (the reference computes oscillating_bc(time, j) there, sub_wave.c:11965).
+ * FLOOR (vmax > 0): every cell with xy_in != 0 is clamped to [-vmax, vmax]. */
+typedef struct orc_extras {
+    int n_neu;
+    const int *ni, *nj, *nbi1, *nbj1, *nbi2, *nbj2;
+    const double *nnx, *nny;
+    int n_drv;
+    const int *di, *dj;
+    const double *dval;
+    long dlen;
+    double vmax;
+} orc_extras;
+
+void orc_run_x(const orc_params *p, double *b0, double *b1, double *b2, int *state,
+               long nsteps, const short *xy_in, const double *tcc, const double *tc,
+               const double *tgamma, int nsrc, const int *si, const int *sj,
+               const double *amp, long amp_len, long step0, const orc_extras *x);
+
 /* Per-cell energy, compute_energy wave_common.c:453-472, with the Courant
  * constants given explicitly (c2_in = COURANT^2 for xy_in != 0, c2_out =
  * COURANTB^2 used when TWOSPEEDS). */

@@ -62,3 +62,45 @@ def small_cases():
         # wave_3d.c:615-616: tb_shift = NX, then NY-2 once it reaches NX-1
         out.append(("flat_%dx%d_bc2_ts1_tb" % (nx, ny), nx, ny, 2, 1, 1, ny - 2))
     return out
+
+
+def ext_cases():
+    """(tag, nx, ny, bcond, twospeeds) of the reference builds with Neumann
+    ghost points, the oscillating left edge and FLOOR (VMAX 0.9)."""
+    return [("row_16x29_bc%d_ts%d_ext" % (bc, ts), 16, 29, bc, ts) for bc, ts in ((0, 0), (1, 0), (2, 0), (2, 1))]
+
+
+EXT_VMAX = 0.9
+EXT_STEPS = 9
+
+
+def ext_inputs(tag, nx, ny):
+    """case_inputs with the fields halved (inside +-VMAX) and 12 Neumann ghost
+    points at distinct random cells, each with two neighbours among its 3x3
+    block inside the grid and weights (w, 1-w)."""
+    d = case_inputs(tag, nx, ny)
+    for k in ("phi", "psi", "tmp"):
+        d[k] = d[k] * 0.5
+    u = uniform(case_seed(tag) + 7, (4096,), 0.0, 1.0)
+    pos = 0
+
+    def nxt():
+        nonlocal pos
+        pos += 1
+        return u[pos - 1]
+
+    taken, nb = set(), {k: [] for k in ("i", "j", "nbi1", "nbj1", "nbi2", "nbj2", "nx", "ny")}
+    while len(nb["i"]) < 12:
+        i, j = int(nxt() * nx), int(nxt() * ny)
+        if (i, j) in taken:
+            continue
+        taken.add((i, j))
+        near = [(i + a, j + b) for a in (-1, 0, 1) for b in (-1, 0, 1) if 0 <= i + a < nx and 0 <= j + b < ny]
+        p1, p2 = near[int(nxt() * len(near))], near[int(nxt() * len(near))]
+        w = nxt()
+        for k, v in zip(("i", "j", "nbi1", "nbj1", "nbi2", "nbj2", "nx", "ny"), (i, j, p1[0], p1[1], p2[0], p2[1], w, 1.0 - w)):
+            nb[k].append(v)
+    d["neumann"] = {k: np.array(v, np.float64 if k in ("nx", "ny") else np.int32) for k, v in nb.items()}
+    # the cells the reference drives: (0, j) for 0 < j < NY-1, then the corners (0, 0) and (0, NX-1)
+    d["drive_cells"] = [(0, j) for j in range(1, ny - 1)] + [(0, 0), (0, nx - 1)]
+    return d

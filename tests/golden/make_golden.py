@@ -11,6 +11,9 @@ Outputs (committed):
   c1_digests.json   SHA-256 of phi after every 100 steps of the 1280x720
                     plain workload (1,000 steps), plus probe values
   energy_64x48.npz  per-cell compute_energy of a two-speed state
+  ext_cases.npz     Neumann ghost points + oscillating left edge + FLOOR:
+                    the drive values oscillating_bc(time, j) and phi, psi,
+                    tmp after EXT_STEPS steps, for every gen_inputs.ext_cases()
 """
 import hashlib
 import json
@@ -26,7 +29,7 @@ sys.path.insert(0, os.path.join(ROOT, "youtube-simulations_amd"))
 sys.path.insert(0, HERE)
 
 import pyoracle  # noqa: E402
-from gen_inputs import case_inputs, small_cases  # noqa: E402
+from gen_inputs import EXT_STEPS, case_inputs, ext_cases, ext_inputs, small_cases  # noqa: E402
 from wavehip import workloads  # noqa: E402
 
 NSTEPS_SMALL = 7
@@ -74,6 +77,19 @@ def main():
     e = np.array([[R.energy(d["phi"], d["psi"], d["xy_in"], i, j) for j in range(48)] for i in range(64)])
     np.savez_compressed(os.path.join(HERE, "energy_64x48.npz"), e=e, courant=0.1,
                         courantb=0.0666666666666666667, e_scale=300.0)
+    # Neumann ghost points, oscillating left edge (OSC_PERIODIC) and FLOOR
+    out = {}
+    for (tag, nx, ny, bc, ts) in ext_cases():
+        R = pyoracle.RefBuild(tag)
+        assert (R.nx, R.ny) == (nx, ny)
+        d = ext_inputs(tag, nx, ny)
+        R.set_neumann(d["neumann"])
+        out[tag + "/drive_val"] = np.array([[R.oscillating_bc(t, j) for t in range(1, EXT_STEPS + 1)]
+                                            for (_, j) in d["drive_cells"]])
+        b = [d["phi"].copy(), d["psi"].copy(), d["tmp"].copy()]
+        phi, psi, tmp = R.run(b[0], b[1], b[2], EXT_STEPS, d["xy_in"], d["tcc"], d["tc"], d["tgamma"])
+        out[tag + "/phi"], out[tag + "/psi"], out[tag + "/tmp"] = phi, psi, tmp
+    np.savez_compressed(os.path.join(HERE, "ext_cases.npz"), **out)
     print("fixtures written")
 
 

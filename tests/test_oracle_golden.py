@@ -11,7 +11,7 @@ import os
 import numpy as np
 import pytest
 
-from gen_inputs import KAPPAS, case_inputs, small_cases
+from gen_inputs import EXT_STEPS, EXT_VMAX, KAPPAS, case_inputs, ext_cases, ext_inputs, small_cases
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -75,6 +75,46 @@ def test_fixture_detects_planted_mistakes(oracle, small):
         phi, _, _ = oracle.run(p, d["phi"], d["psi"], 7, d["xy_in"], d["tcc"], d["tc"], d["tgamma"],
                                tmp=d["tmp"])
         assert not bits_equal(phi, small[tag + "/phi"])
+
+
+@pytest.fixture(scope="module")
+def ext():
+    return np.load(os.path.join(GOLD, "ext_cases.npz"))
+
+
+def ext_run(oracle, case, ext, **mut):
+    tag, nx, ny, bc, ts = case
+    d = ext_inputs(tag, nx, ny)
+    p = oracle.make_params(nx, ny, bcond=bc, twospeeds=ts, **KAPPAS)
+    cells = d["drive_cells"]
+    drive = dict(i=[c[0] for c in cells], j=[c[1] for c in cells], val=ext[tag + "/drive_val"])
+    kw = dict(neumann=d["neumann"], drive=drive, vmax=EXT_VMAX)
+    kw.update(mut)
+    return oracle.run_x(p, d["phi"], d["psi"], EXT_STEPS, d["xy_in"], d["tcc"], d["tc"], d["tgamma"],
+                        tmp=d["tmp"], **kw)
+
+
+@pytest.mark.parametrize("case", ext_cases(), ids=lambda c: c[0])
+def test_oracle_extras_match_reference_fixture(oracle, ext, case):
+    """Neumann ghost points (wave_billiard.c:379-389), the oscillating left
+    edge and its corners (397, 585-590) and FLOOR (592-609), bit for bit."""
+    tag = case[0]
+    phi, psi, tmp = ext_run(oracle, case, ext)
+    assert bits_equal(phi, ext[tag + "/phi"])
+    assert bits_equal(psi, ext[tag + "/psi"])
+    assert bits_equal(tmp, ext[tag + "/tmp"])
+
+
+def test_extras_fixture_detects_planted_mistakes(oracle, ext):
+    """Dropping any one of the three features, or swapping the Neumann
+    weights, fails the comparison."""
+    case = ext_cases()[2]
+    tag, nx, ny = case[:3]
+    d = ext_inputs(tag, nx, ny)
+    swapped = dict(d["neumann"], nx=d["neumann"]["ny"], ny=d["neumann"]["nx"])
+    for mut in (dict(neumann=None), dict(drive=None), dict(vmax=0.0), dict(neumann=swapped)):
+        phi, _, _ = ext_run(oracle, case, ext, **mut)
+        assert not bits_equal(phi, ext[tag + "/phi"]), mut
 
 
 def test_c1_plain_workload_digests(oracle):
