@@ -104,6 +104,31 @@ int wv_get_state3(wv_ctx *c, double *phi, double *psi, double *tmp);
 int wv_set_sources(wv_ctx *c, int n, const int *i, const int *j, const double *amp,
                    long amp_len);
 
+/* Reference features outside the plain stepper (FP64 form; set them before
+ * wv_set_state -- changed later, the context steps one step per launch until
+ * the next wv_set_state).  In one step the order is that of
+ * evolve_wave_half: bulk cells, Neumann points, edge cells, driven cells,
+ * FLOOR, then the point sources of wv_set_sources.
+ *
+ * Neumann ghost points, wave_billiard.c:379-389 (t_neumann_bc of
+ * global_pdes.c:518-528 as plain arrays, active points only): point q at
+ * (i[q], j[q]) takes nx[q]*u(nbi1,nbj1) + ny[q]*u(nbi2,nbj2) of the previous
+ * time level; an edge cell that the edge formulas write keeps the edge value.
+ * Both neighbours must lie in the grid within one cell of the point (what
+ * init_neumann_bc, sub_wave.c:13720-13760, builds).  n = 0 removes them. */
+int wv_set_neumann(wv_ctx *c, int n, const int *i, const int *j, const int *nbi1, const int *nbj1,
+                   const int *nbi2, const int *nbj2, const double *nx, const double *ny);
+/* Driven cells -- the OSCILLATE_LEFT edge and corners, wave_billiard.c:397 and
+ * 585-590, with the schedule (oscillating_bc, sub_wave.c:11965) evaluated by
+ * the caller: after step g (g = 1, 2, ... counted from this call) cell k is
+ * set to val[k*len + g-1] while g <= len.  A later call with the same cells
+ * only replaces the values.  n = 0 removes them. */
+int wv_set_drive(wv_ctx *c, int n, const int *i, const int *j, const double *val, long len);
+/* FLOOR, wave_billiard.c:592-609: after each step every cell with xy_in != 0
+ * is clamped to [-vmax, vmax] (with fused steps the cells no step writes hold
+ * +0); vmax = 0 turns it off. */
+int wv_set_floor(wv_ctx *c, double vmax);
+
 int wv_step(wv_ctx *c, long nsteps);
 /* wv_step bracketed by HIP events on the context stream; waits; *ms is the
  * device time of the whole sequence and *launches the kernel launches in it */

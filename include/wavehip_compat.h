@@ -19,10 +19,19 @@
  * Define WAVEHIP_TABLES_FIXED when tcc / tc / tgamma / xy_in do not change
  * between calls (no REFRESH_IOR): they are then uploaded once.
  *
+ * BC_NEUMANN (wave_billiard.c:379-389): the program's n_neumann_points
+ * active points of neumann_bc are handed to the library on the first call
+ * (with WAVEHIP_TABLES_FIXED) or on every call.  OSCILLATE_LEFT
+ * (wave_billiard.c:320-341, 397, 585-590): the left-edge cells and corners
+ * the reference drives are set, step by step, to the program's own
+ * oscillating_bc(time, j), with the step counter of the reference (the
+ * flat program drives the whole left edge and its corners, wave_3d.c:651-654,
+ * 989-993); the corner (0, NX-1) of the row-pointer reference lies outside
+ * row 0 when NX > NY and is then left out.  FLOOR (wave_billiard.c:592-609)
+ * clamps to VMAX.
+ *
  * Not provided (the program aborts with a message on the first call):
- * OSCILLATE_LEFT (wave_billiard.c:320-341,397,585-590), BC_NEUMANN
- * (wave_billiard.c:379-389), FLOOR (wave_billiard.c:592-609), COMPARISON
- * (wave_3d.c:805-976).
+ * COMPARISON (wave_3d.c:805-976).
  *
  * Faster path for the frame loop (wave_billiard.c:897-899): keep the state on
  * the GPU and call wv_step(wavehip_ctx(), 3*NVID) plus one wv_get_state per
@@ -55,18 +64,6 @@ static wv_ctx *wavehip_ctx(void)
     if (!ctx) {
         wv_params p;
         int rc;
-#if OSCILLATE_LEFT
-        fprintf(stderr, "wavehip: OSCILLATE_LEFT is not supported by libwavehip\n");
-        abort();
-#endif
-#if BC_NEUMANN
-        fprintf(stderr, "wavehip: BC_NEUMANN is not supported by libwavehip\n");
-        abort();
-#endif
-#if FLOOR
-        fprintf(stderr, "wavehip: FLOOR is not supported by libwavehip\n");
-        abort();
-#endif
 #if defined(COMPARISON) && COMPARISON
         fprintf(stderr, "wavehip: COMPARISON is not supported by libwavehip\n");
         abort();
@@ -90,9 +87,106 @@ static wv_ctx *wavehip_ctx(void)
 #endif
         rc = wv_create(&ctx, &p);
         if (rc) wavehip_die(ctx, "wv_create", rc);
+#if FLOOR
+        rc = wv_set_floor(ctx, VMAX);
+        if (rc) wavehip_die(ctx, "wv_set_floor", rc);
+#endif
     }
     return ctx;
 }
+
+#if BC_NEUMANN
+/* the program's active Neumann points (init_neumann_bc, sub_wave.c:13910) */
+static void wavehip_neumann(const t_neumann_bc *nb)
+{
+    static int done = 0;
+    int n = 0, q, rc;
+    int *a;
+    double *w;
+#ifdef WAVEHIP_TABLES_FIXED
+    if (done) return;
+#endif
+    a = (int *)malloc(sizeof(int) * 6 * (n_neumann_points > 0 ? n_neumann_points : 1));
+    w = (double *)malloc(sizeof(double) * 2 * (n_neumann_points > 0 ? n_neumann_points : 1));
+    if (!a || !w) wavehip_die(wavehip_ctx(), "malloc", -1);
+    for (q = 0; q < n_neumann_points; q++)
+        if (nb[q].active) {
+#ifndef WAVEHIP_FLAT
+            a[n] = nb[q].i;
+            a[n_neumann_points + n] = nb[q].j;
+            a[2 * n_neumann_points + n] = nb[q].nbi1;
+            a[3 * n_neumann_points + n] = nb[q].nbj1;
+            a[4 * n_neumann_points + n] = nb[q].nbi2;
+            a[5 * n_neumann_points + n] = nb[q].nbj2;
+#else
+            /* wave_3d.c:640-644 indexes the flat arrays with k, nbk1, nbk2 */
+            a[n] = nb[q].k / NY;
+            a[n_neumann_points + n] = nb[q].k % NY;
+            a[2 * n_neumann_points + n] = nb[q].nbk1 / NY;
+            a[3 * n_neumann_points + n] = nb[q].nbk1 % NY;
+            a[4 * n_neumann_points + n] = nb[q].nbk2 / NY;
+            a[5 * n_neumann_points + n] = nb[q].nbk2 % NY;
+#endif
+            w[n] = nb[q].nx;
+            w[n_neumann_points + n] = nb[q].ny;
+            n++;
+        }
+    rc = wv_set_neumann(wavehip_ctx(), n, a, a + n_neumann_points, a + 2 * n_neumann_points,
+                        a + 3 * n_neumann_points, a + 4 * n_neumann_points, a + 5 * n_neumann_points, w,
+                        w + n_neumann_points);
+    free(a);
+    free(w);
+    if (rc) wavehip_die(wavehip_ctx(), "wv_set_neumann", rc);
+    done = 1;
+    (void)done;
+}
+#endif
+
+#if OSCILLATE_LEFT
+/* the driven cells of the next nsteps steps, values oscillating_bc(time, j):
+ * row-pointer program: (0, j) for bc_jmin < j < bc_jmax, 0 < j < NY-1
+ * (wave_billiard.c:320-341, 397), then the corners (0, 0) and (0, NX-1)
+ * unless BEAM_BC (585-590); flat program: (0, j) for 0 < j < NY-1 and the
+ * corners (0, 0), (0, NY-1) (wave_3d.c:651-654, 989-993) */
+static void wavehip_drive(int nsteps)
+{
+    static long time = 0;  /* the reference's step counter, wave_billiard.c:310,315 */
+    static int cells[NY + 2], ncell = -1;
+    static double vals[(NY + 2) * 3];
+    int q, s, rc, bc_jmin = 0, bc_jmax = NY - 1, ij[2];
+    int ii[NY + 2];
+    if (ncell < 0) {
+#ifdef WAVEHIP_FLAT
+        ncell = 0;
+        for (q = 1; q < NY - 1; q++) cells[ncell++] = q;
+        cells[ncell++] = 0;
+        cells[ncell++] = NY - 1;
+        (void)bc_jmin; (void)bc_jmax; (void)ij;
+#else
+        if (BEAM_BC) {
+            xy_to_ij(0.0, OSCIL_YMID + OSCIL_YMAX, ij);
+            bc_jmax = ij[1];
+            xy_to_ij(0.0, OSCIL_YMID - OSCIL_YMAX, ij);
+            bc_jmin = ij[1];
+        }
+        ncell = 0;
+        for (q = 1; q < NY - 1; q++)
+            if (q > bc_jmin && q < bc_jmax) cells[ncell++] = q;
+        if (!BEAM_BC) {
+            cells[ncell++] = 0;
+            if (NX - 1 < NY) cells[ncell++] = NX - 1;
+        }
+#endif
+    }
+    for (q = 0; q < ncell; q++) {
+        ii[q] = 0;
+        for (s = 0; s < nsteps; s++) vals[q * nsteps + s] = oscillating_bc((int)(time + s + 1), cells[q]);
+    }
+    time += nsteps;
+    rc = wv_set_drive(wavehip_ctx(), ncell, ii, cells, vals, nsteps);
+    if (rc) wavehip_die(wavehip_ctx(), "wv_set_drive", rc);
+}
+#endif
 
 #ifndef WAVEHIP_FLAT
 /* wave_billiard.c:302-304 */
@@ -100,7 +194,14 @@ static void evolve_wave_half(double *phi_in[NX], double *psi_in[NX], double *phi
                              short int *xy_in[NX], double *tcc[NX], double *tc[NX],
                              double *tgamma[NX], t_neumann_bc neumann_bc[N_NEUMANN_POINTS])
 {
-    int rc = wv_evolve_wave_half_rows(wavehip_ctx(), phi_in, psi_in, phi_out, xy_in, tcc, tc, tgamma);
+    int rc;
+#if BC_NEUMANN
+    wavehip_neumann(neumann_bc);
+#endif
+#if OSCILLATE_LEFT
+    wavehip_drive(1);
+#endif
+    rc = wv_evolve_wave_half_rows(wavehip_ctx(), phi_in, psi_in, phi_out, xy_in, tcc, tc, tgamma);
     (void)neumann_bc;
     if (rc) wavehip_die(wavehip_ctx(), "evolve_wave_half", rc);
 }
@@ -110,7 +211,14 @@ static void evolve_wave(double *phi[NX], double *psi[NX], double *tmp[NX], short
                         double *tcc_table[NX], double *tc_table[NX], double *tgamma_table[NX],
                         t_neumann_bc neumann_bc[N_NEUMANN_POINTS])
 {
-    int rc = wv_evolve_wave_rows(wavehip_ctx(), phi, psi, tmp, xy_in, tcc_table, tc_table, tgamma_table);
+    int rc;
+#if BC_NEUMANN
+    wavehip_neumann(neumann_bc);
+#endif
+#if OSCILLATE_LEFT
+    wavehip_drive(3);
+#endif
+    rc = wv_evolve_wave_rows(wavehip_ctx(), phi, psi, tmp, xy_in, tcc_table, tc_table, tgamma_table);
     (void)neumann_bc;
     if (rc) wavehip_die(wavehip_ctx(), "evolve_wave", rc);
 }
@@ -120,7 +228,14 @@ static void evolve_wave_half(double phi_in[NX * NY], double psi_in[NX * NY], dou
                              short int xy_in[NX * NY], double tc[NX * NY], double tcc[NX * NY],
                              double tgamma[NX * NY], t_neumann_bc neumann_bc[N_NEUMANN_POINTS])
 {
-    int rc = wv_evolve_wave_half_flat(wavehip_ctx(), phi_in, psi_in, phi_out, xy_in, tc, tcc, tgamma);
+    int rc;
+#if BC_NEUMANN
+    wavehip_neumann(neumann_bc);
+#endif
+#if OSCILLATE_LEFT
+    wavehip_drive(1);
+#endif
+    rc = wv_evolve_wave_half_flat(wavehip_ctx(), phi_in, psi_in, phi_out, xy_in, tc, tcc, tgamma);
     (void)neumann_bc;
     if (rc) wavehip_die(wavehip_ctx(), "evolve_wave_half", rc);
 }
@@ -132,7 +247,14 @@ static void evolve_wave(double phi[NX * NY], double psi[NX * NY], double tmp[NX 
                         t_laplacian laplace1[NX * NY], t_laplacian laplace2[NX * NY],
                         t_neumann_bc neumann_bc[N_NEUMANN_POINTS])
 {
-    int rc = wv_evolve_wave_flat(wavehip_ctx(), phi, psi, tmp, xy_in, tc, tcc, tgamma);
+    int rc;
+#if BC_NEUMANN
+    wavehip_neumann(neumann_bc);
+#endif
+#if OSCILLATE_LEFT
+    wavehip_drive(3);
+#endif
+    rc = wv_evolve_wave_flat(wavehip_ctx(), phi, psi, tmp, xy_in, tc, tcc, tgamma);
     (void)laplace; (void)laplace1; (void)laplace2; (void)neumann_bc;
     if (rc) wavehip_die(wavehip_ctx(), "evolve_wave", rc);
 }

@@ -11,7 +11,9 @@
  *
  *   compat_host IN OUT NCALLS MODE
  *   IN    phi, psi, tmp, tcc, tc, tgamma (NX*NY doubles each, k = i*NY + j)
- *         followed by xy_in (NX*NY shorts)
+ *         followed by xy_in (NX*NY shorts), then int32 n and n Neumann points
+ *         (int32 i, j, nbi1, nbj1, nbi2, nbj2; doubles nx, ny), then int32 T
+ *         and T*NY doubles: oscillating_bc(time, j) for time 1..T
  *   MODE  "wave": NCALLS calls of evolve_wave (3 steps each);
  *         "half": NCALLS calls of evolve_wave_half(phi, psi, tmp), the host
  *         rotating (phi, psi, tmp) <- (tmp, phi, psi) after each
@@ -25,17 +27,49 @@
 
 #define XMIN -2.0
 #define XMAX 2.0
+#ifndef OSCILLATE_LEFT
 #define OSCILLATE_LEFT 0
+#endif
+#ifndef BC_NEUMANN
 #define BC_NEUMANN 0
+#endif
+#ifndef FLOOR
 #define FLOOR 0
-#define N_NEUMANN_POINTS 1
+#endif
+#ifndef VMAX
+#define VMAX 2.5
+#endif
+#define BEAM_BC 0
+#define OSCIL_YMID 0.0
+#define OSCIL_YMAX 1.0
+#define N_NEUMANN_POINTS 4096
 
+/* the fields of global_pdes.c:518-528 the stepper reads */
 typedef struct {
-    double x, y, nx, ny;
+    double nx, ny;
+    int nbi1, nbj1, nbk1;
+    int nbi2, nbj2, nbk2;
+    int i, j, k;
+    short int active;
 } t_neumann_bc;
 typedef struct {
     int nneighb;
 } t_laplacian;
+
+int n_neumann_points = 0;
+
+/* the left-edge schedule: a table read from the input (time 1, 2, ...) */
+static double *drive_table;
+static int drive_steps;
+double oscillating_bc(int time, int j)
+{
+    return (time >= 1 && time <= drive_steps) ? drive_table[(size_t)(time - 1) * NY + j] : 0.0;
+}
+void xy_to_ij(double x, double y, int ij[2])
+{
+    ij[0] = (int)((x - XMIN) / (XMAX - XMIN) * NX);
+    ij[1] = (int)((y + 1.0) / 2.0 * NY);
+}
 
 #include "wavehip_compat.h"
 
@@ -71,10 +105,11 @@ static void write_all(FILE *f, const void *p, size_t sz, size_t n)
 int main(int argc, char **argv)
 {
     if (argc == 2 && strcmp(argv[1], "--params") == 0) {
-        printf("nx=%d ny=%d bcond=%d twospeeds=%d oscillate_topbot=%d flat=%d fixed=%d "
+        printf("nx=%d ny=%d bcond=%d twospeeds=%d oscillate_topbot=%d flat=%d fixed=%d neumann=%d "
+               "oscillate_left=%d floor=%d vmax=%.17g "
                "kappa=%.17g kappa_sides=%.17g gamma_sides=%.17g kappa_topbot=%.17g gamma_topbot=%.17g\n",
-               NX, NY, B_COND, TWOSPEEDS, OSCILLATE_TOPBOT, HOST_FLAT, HOST_FIXED, KAPPA, KAPPA_SIDES,
-               GAMMA_SIDES, KAPPA_TOPBOT, GAMMA_TOPBOT);
+               NX, NY, B_COND, TWOSPEEDS, OSCILLATE_TOPBOT, HOST_FLAT, HOST_FIXED, BC_NEUMANN, OSCILLATE_LEFT,
+               FLOOR, (double)VMAX, KAPPA, KAPPA_SIDES, GAMMA_SIDES, KAPPA_TOPBOT, GAMMA_TOPBOT);
         return 0;
     }
     if (argc != 5) {
@@ -92,9 +127,26 @@ int main(int argc, char **argv)
         read_all(f, flat[a], sizeof(double), NC);
     }
     read_all(f, xyf, sizeof(short), NC);
+    static t_neumann_bc nbc[N_NEUMANN_POINTS];
+    int nn = 0;
+    read_all(f, &nn, sizeof(int), 1);
+    if (nn < 0 || nn > N_NEUMANN_POINTS) return 2;
+    for (int q = 0; q < nn; q++) {
+        int v[6];
+        read_all(f, v, sizeof(int), 6);
+        nbc[q].i = v[0]; nbc[q].j = v[1]; nbc[q].k = v[0] * NY + v[1];
+        nbc[q].nbi1 = v[2]; nbc[q].nbj1 = v[3]; nbc[q].nbk1 = v[2] * NY + v[3];
+        nbc[q].nbi2 = v[4]; nbc[q].nbj2 = v[5]; nbc[q].nbk2 = v[4] * NY + v[5];
+        nbc[q].active = 1;
+    }
+    for (int q = 0; q < nn; q++) read_all(f, &nbc[q].nx, sizeof(double), 1);
+    for (int q = 0; q < nn; q++) read_all(f, &nbc[q].ny, sizeof(double), 1);
+    n_neumann_points = nn;
+    read_all(f, &drive_steps, sizeof(int), 1);
+    if (drive_steps < 0) return 2;
+    drive_table = malloc(sizeof(double) * (size_t)(drive_steps > 0 ? drive_steps : 1) * NY);
+    read_all(f, drive_table, sizeof(double), (size_t)drive_steps * NY);
     fclose(f);
-    t_neumann_bc nbc[N_NEUMANN_POINTS];
-    memset(nbc, 0, sizeof nbc);
 
 #ifndef WAVEHIP_FLAT
     /* row-pointer tables, one malloc per row */

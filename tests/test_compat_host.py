@@ -17,7 +17,8 @@ from conftest import say
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 BIN = os.path.join(HERE, "compat", "bin")
-VARIANTS = ["rows_abs", "rows_per", "rows_dir_tb", "flat_vper", "flat_abs_fixed"]
+VARIANTS = ["rows_abs", "rows_per", "rows_dir_tb", "flat_vper", "flat_abs_fixed", "rows_ext", "rows_ext_wide",
+            "flat_ext"]
 
 
 def host_params(name):
@@ -49,10 +50,33 @@ def oracle_params(oracle, d):
                               gamma_topbot=d["gamma_topbot"])
 
 
+def drive_cells(d):
+    """the cells the reference drives: wave_billiard.c:336-338, 397, 585-590
+    (row pointers; the corner (0, NX-1) only when it lies in row 0) or
+    wave_3d.c:651-654, 989-993 (flat)"""
+    nx, ny = d["nx"], d["ny"]
+    cells = [(0, j) for j in range(1, ny - 1)]
+    if d["flat"]:
+        return cells + [(0, 0), (0, ny - 1)]
+    return cells + [(0, 0)] + ([(0, nx - 1)] if nx - 1 < ny else [])
+
+
 def oracle_expect(oracle, d, w, ncalls, mode):
     p = oracle_params(oracle, d)
     phi, psi, tmp = (np.array(w[k], dtype=np.float64, copy=True) for k in ("phi", "psi", "tmp"))
     xy, tcc, tc, tg = w["xy_in"], w["tcc"], w["tc"], w["tgamma"]
+    if d["neumann"] or d["oscillate_left"] or d["floor"]:
+        n = 3 * ncalls if mode == "wave" else ncalls
+        kw = {}
+        if d["neumann"]:
+            kw["neumann"] = w["nb"]
+        if d["oscillate_left"]:
+            cells = drive_cells(d)
+            kw["drive"] = dict(i=[c[0] for c in cells], j=[c[1] for c in cells],
+                               val=np.array([w["table"][:n, j] for (_, j) in cells]))
+        if d["floor"]:
+            kw["vmax"] = d["vmax"]
+        return oracle.run_x(p, phi, psi, n, xy, tcc, tc, tg, tmp=tmp, **kw)
     if not d["flat"]:
         n = 3 * ncalls if mode == "wave" else ncalls
         return oracle.run(p, phi, psi, n, xy, tcc, tc, tg, tmp=tmp)
@@ -63,6 +87,22 @@ def oracle_expect(oracle, d, w, ncalls, mode):
             oracle.evolve_wave_half_flat(p, phi, psi, tmp, xy, tc, tcc, tg)
             phi, psi, tmp = tmp, phi, psi
     return phi, psi, tmp
+
+
+def neumann_points(xy, rng, n):
+    """ghost points at distinct cells, each averaging two in-grid neighbours
+    within one cell (weights w, 1-w)"""
+    nx, ny = xy.shape
+    cells = rng.choice(nx * ny, size=n, replace=False)
+    out = {k: [] for k in ("i", "j", "nbi1", "nbj1", "nbi2", "nbj2", "nx", "ny")}
+    for c in cells:
+        i, j = divmod(int(c), ny)
+        near = [(i + a, j + b) for a in (-1, 0, 1) for b in (-1, 0, 1) if 0 <= i + a < nx and 0 <= j + b < ny]
+        p1, p2 = near[rng.integers(len(near))], near[rng.integers(len(near))]
+        wgt = rng.uniform(0, 1)
+        for k, v in zip(out, (i, j, p1[0], p1[1], p2[0], p2[1], wgt, 1 - wgt)):
+            out[k].append(v)
+    return {k: np.array(v) for k, v in out.items()}
 
 
 @pytest.mark.gpu
@@ -76,11 +116,27 @@ def test_host_program_matches_oracle(oracle, tmp_path, name, mode, ncalls):
                               twospeeds=d["twospeeds"])
     # the third buffer holds values too: cells the step never writes keep them
     w["tmp"] = np.random.default_rng(7).standard_normal((d["nx"], d["ny"]))
+    if d["floor"]:
+        for k in ("phi", "psi", "tmp"):
+            w[k] = w[k] * 0.5
+    rng = np.random.default_rng(11)
+    w["nb"] = neumann_points(w["xy_in"], rng, 40) if d["neumann"] else None
+    T = 3 * ncalls
+    w["table"] = np.cos(0.4 * np.arange(1, T + 1)[:, None] + 0.05 * np.arange(d["ny"])[None, :])
     inp, outp = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(inp, "wb") as f:
         for k in ("phi", "psi", "tmp", "tcc", "tc", "tgamma"):
             f.write(np.ascontiguousarray(w[k], dtype=np.float64).tobytes())
         f.write(np.ascontiguousarray(w["xy_in"], dtype=np.int16).tobytes())
+        nb = w["nb"]
+        f.write(np.int32(0 if nb is None else len(nb["i"])).tobytes())
+        if nb is not None:
+            f.write(np.ascontiguousarray(np.stack([nb[k] for k in ("i", "j", "nbi1", "nbj1", "nbi2", "nbj2")], 1),
+                                         np.int32).tobytes())
+            f.write(np.ascontiguousarray(nb["nx"], np.float64).tobytes())
+            f.write(np.ascontiguousarray(nb["ny"], np.float64).tobytes())
+        f.write(np.int32(T).tobytes())
+        f.write(np.ascontiguousarray(w["table"], np.float64).tobytes())
     r = subprocess.run([os.path.join(BIN, name), str(inp), str(outp), str(ncalls), mode],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr

@@ -43,6 +43,18 @@ struct Grid {
     bool clean_state = true, clean_amp = true;
     std::vector<uint64_t> h_act;        // host copy of the active plane
     std::vector<int> h_si, h_sj;        // source cells
+    // FP64 form: Neumann ghost points, driven cells, FLOOR
+    uint64_t *nz = nullptr;             // xy_in != 0, 1 bit per cell
+    std::vector<uint64_t> h_nz;
+    int *neu_ptr = nullptr, *neu_col = nullptr, *neu_code = nullptr;
+    double *neu_w = nullptr;
+    int *drv_ptr = nullptr, *drv_col = nullptr, *drv_id = nullptr;
+    double *drv_val = nullptr;
+    long drv_len = 0, drv_base = 0;
+    double vmax = 0;
+    std::vector<int> h_ni, h_nj, h_di, h_dj;
+    // cells a step writes: h_act plus the Neumann points and driven cells
+    std::vector<uint64_t> h_wr;
 };
 
 struct Engine {
@@ -144,6 +156,7 @@ int alloc_grid(Engine *e, Grid &G)
     HIPCHK(e, hipMalloc(&G.act, words * 8));
     HIPCHK(e, hipMalloc(&G.eact, words * 8));
     HIPCHK(e, hipMalloc(&G.med, words * 8));
+    HIPCHK(e, hipMalloc(&G.nz, words * 8));
     HIPCHK(e, hipMalloc(&G.tc, cells * sizeof(double)));
     HIPCHK(e, hipMemset(G.act, 0, words * 8));
     HIPCHK(e, hipMemset(G.eact, 0, words * 8));
@@ -156,7 +169,8 @@ void free_grid(Grid &G)
 {
     void *ptrs[] = {G.u[0], G.u[1], G.u[2], G.u[3], G.hi[0], G.hi[1], G.lo[0], G.lo[1], G.v[0], G.v[1],
                     G.act, G.eact, G.med, G.tccf, G.tgf, G.tc, G.tccf_hi, G.tccf_lo, G.tgf32,
-                    G.src_ptr, G.src_col, G.src_id, G.amp};
+                    G.src_ptr, G.src_col, G.src_id, G.amp, G.nz, G.neu_ptr, G.neu_col, G.neu_code,
+                    G.neu_w, G.drv_ptr, G.drv_col, G.drv_id, G.drv_val};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
 }
@@ -171,6 +185,11 @@ WvInst make_inst(const Engine *e, const Grid &G)
     d.tccf_hi = G.tccf_hi; d.tccf_lo = G.tccf_lo; d.tgf32 = G.tgf32;
     d.src_ptr = G.src_ptr; d.src_col = G.src_col; d.src_id = G.src_id;
     d.amp = G.amp; d.amp_len = G.amp_len;
+    d.neu_ptr = G.neu_ptr; d.neu_col = G.neu_col; d.neu_code = G.neu_code; d.neu_w = G.neu_w;
+    d.drv_ptr = G.drv_ptr; d.drv_col = G.drv_col; d.drv_id = G.drv_id; d.drv_val = G.drv_val;
+    d.drv_len = G.drv_len; d.drv_base = G.drv_base;
+    d.nz = G.nz;
+    d.vmax = G.vmax;
     d.nx = G.p.nx; d.ny = G.p.ny; d.ldy = G.ldy; d.ldw = G.ldw;
     d.bc = G.p.bcond; d.osc_tb = G.p.oscillate_topbot; d.tb_shift = G.p.tb_shift; d.coef = G.coef;
     d.rows_per_tile = rows(e);
@@ -243,11 +262,23 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
         const int js = cb * wout - wv_halo(k);
         const int ibeg = sb * L, iend = std::min(ibeg + L, nx);
         const bool interior = js >= 1 && js + 64 * WV_C <= ny - 1 && ibeg >= k && iend <= nx - k;
-        if (!interior) return G.h_si.empty() ? 0 : WV_TILE_SRC;  // (wrapped halos: any source)
-        bool all = !G.h_act.empty();
+        // (wrapped halos: any source, Neumann point or driven cell of the grid)
+        const int bflags = (G.h_si.empty() ? 0 : WV_TILE_SRC) | (G.h_ni.empty() ? 0 : WV_TILE_NEU) |
+                           (G.h_di.empty() ? 0 : WV_TILE_DRV);
+        if (!interior) return bflags;
+        int xflags = 0;
+        for (size_t q = 0; q < G.h_ni.size() && !(xflags & WV_TILE_NEU); q++)
+            if (G.h_ni[q] >= ibeg - k && G.h_ni[q] < iend + k && G.h_nj[q] >= js && G.h_nj[q] < js + 64 * WV_C)
+                xflags |= WV_TILE_NEU;
+        for (size_t q = 0; q < G.h_di.size() && !(xflags & WV_TILE_DRV); q++)
+            if (G.h_di[q] >= ibeg - k && G.h_di[q] < iend + k && G.h_dj[q] >= js && G.h_dj[q] < js + 64 * WV_C)
+                xflags |= WV_TILE_DRV;
+        // FLOOR and the two lists run on the boundary path of the kernels
+        if (xflags || G.vmax > 0) return xflags | (G.h_si.empty() ? 0 : WV_TILE_SRC);
+        bool all = !G.h_wr.empty();
         for (int i = ibeg - k; all && i < iend + k; i++)
             for (int j = js; j < js + 64 * WV_C; j++)
-                if (!((G.h_act[(size_t)i * G.ldw + (j >> 6)] >> (j & 63)) & 1u)) { all = false; break; }
+                if (!((G.h_wr[(size_t)i * G.ldw + (j >> 6)] >> (j & 63)) & 1u)) { all = false; break; }
         bool src = false;
         for (size_t q = 0; q < G.h_si.size() && !src; q++)
             src = G.h_si[q] >= ibeg - k && G.h_si[q] < iend + k && G.h_sj[q] >= js && G.h_sj[q] < js + 64 * WV_C;
@@ -362,6 +393,17 @@ inline bool bit(const std::vector<uint64_t> &pl, int ldw, int i, int j)
     return (pl[(size_t)i * ldw + (j >> 6)] >> (j & 63)) & 1u;
 }
 
+// the cells a step writes: the active plane plus Neumann points and driven cells
+void rebuild_written(Grid &G)
+{
+    G.h_wr = G.h_act;
+    if (G.h_wr.empty()) return;
+    for (size_t q = 0; q < G.h_ni.size(); q++)
+        G.h_wr[(size_t)G.h_ni[q] * G.ldw + (G.h_nj[q] >> 6)] |= 1ull << (G.h_nj[q] & 63);
+    for (size_t q = 0; q < G.h_di.size(); q++)
+        G.h_wr[(size_t)G.h_di[q] * G.ldw + (G.h_dj[q] >> 6)] |= 1ull << (G.h_dj[q] & 63);
+}
+
 inline bool bits_zero(double x)
 {
     uint64_t b;
@@ -387,7 +429,7 @@ bool masked_zero(const Grid &G, const double *a)
     const int nx = G.p.nx, ny = G.p.ny;
     for (int i = 0; i < nx; i++)
         for (int j = 0; j < ny; j++)
-            if (!bit(G.h_act, G.ldw, i, j) && !bits_zero(a[(size_t)i * ny + j])) return false;
+            if (!bit(G.h_wr, G.ldw, i, j) && !bits_zero(a[(size_t)i * ny + j])) return false;
     return true;
 }
 
@@ -431,6 +473,12 @@ int set_tables(Engine *e, int k, const short *xy, const double *tcc, const doubl
     HIPCHK(e, hipMemcpy(G.act, act.data(), wb, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(G.eact, eact.data(), wb, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(G.med, med.data(), wb, hipMemcpyHostToDevice));
+    std::vector<uint64_t> nzp((size_t)nx * ldw, 0);
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++)
+            if (xy[(size_t)i * ny + j] != 0) nzp[(size_t)i * ldw + (j >> 6)] |= 1ull << (j & 63);
+    HIPCHK(e, hipMemcpy(G.nz, nzp.data(), wb, hipMemcpyHostToDevice));
+    G.h_nz.swap(nzp);
     HIPCHK(e, hipMemcpy2D(G.tc, ldy * sizeof(double), tc, ny * sizeof(double), ny * sizeof(double), nx,
                           hipMemcpyHostToDevice));
     for (double **q : {&G.tccf, &G.tgf})
@@ -497,6 +545,7 @@ int set_tables(Engine *e, int k, const short *xy, const double *tcc, const doubl
     // are now never written: keep it, but step it one step per launch
     if (G.state && G.h_act != act) G.exact = true;
     G.h_act.swap(act);
+    rebuild_written(G);
     G.tables = true;
     drop_tiles(e);  // tile flags depend on the mask
     return sync_inst(e);
@@ -535,6 +584,7 @@ int set_state(Engine *e, int k, const double *phi, const double *psi, const doub
         }
         HIPCHK(e, hipMemsetAsync(G.u[e->role[3]], 0, bytes, e->stream));
         G.exact = !(masked_zero(G, phi) && masked_zero(G, psi) && masked_zero(G, tmp));
+
         const size_t n = (size_t)nx * ny;
         const bool clean = clean_values(phi, n) && clean_values(psi, n) && clean_values(tmp, n);
         if (clean != G.clean_state) {
@@ -615,7 +665,7 @@ int set_sources(Engine *e, int k, int n, const int *si, const int *sj, const dou
         for (int q = 0; q < n; q++) {
             if (si[q] < 0 || si[q] >= G.p.nx || sj[q] < 0 || sj[q] >= G.p.ny)
                 return fail(e, WV_EINVAL, "source %d at (%d,%d) is outside the grid", q, si[q], sj[q]);
-            if (!bit(G.h_act, G.ldw, si[q], sj[q]))
+            if (!bit(G.h_wr, G.ldw, si[q], sj[q]))
                 return fail(e, WV_EINVAL, "source %d at (%d,%d) is on a cell the step never writes", q, si[q], sj[q]);
         }
         std::vector<int> order(n);
@@ -638,8 +688,141 @@ int set_sources(Engine *e, int k, int n, const int *si, const int *sj, const dou
         G.h_sj.assign(sj, sj + n);
         for (size_t q = 0; q < (size_t)n * amp_len && G.clean_amp; q++) G.clean_amp = std::isfinite(amp[q]);
     }
+    // the source counter restarts at 0; driven cells keep counting from their own start
+    long gnow = 0;
+    HIPCHK(e, hipMemcpy(&gnow, e->d_gstep, sizeof(long), hipMemcpyDeviceToHost));
+    for (auto &H : e->g) H.drv_base -= gnow;
     HIPCHK(e, hipMemset(e->d_gstep, 0, sizeof(long)));
     drop_graphs(e);
+    return sync_inst(e);
+}
+
+// rows of a CSR list by row: ptr[nx+1], and the order of the entries
+void csr_by_row(int nx, int n, const int *row, std::vector<int> &ptr, std::vector<int> &order)
+{
+    order.resize(n);
+    for (int q = 0; q < n; q++) order[q] = q;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return row[a] < row[b]; });
+    ptr.assign(nx + 1, 0);
+    for (int q = 0; q < n; q++) ptr[row[q] + 1]++;
+    for (int i = 0; i < nx; i++) ptr[i + 1] += ptr[i];
+}
+
+template <typename T>
+int upload_vec(Engine *e, T **dst, const std::vector<T> &v)
+{
+    if (*dst) { (void)hipFree(*dst); *dst = nullptr; }
+    if (v.empty()) return WV_OK;
+    HIPCHK(e, hipMalloc(dst, sizeof(T) * v.size()));
+    HIPCHK(e, hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return WV_OK;
+}
+
+int set_neumann(Engine *e, int k, int n, const int *pi, const int *pj, const int *bi1, const int *bj1,
+                const int *bi2, const int *bj2, const double *wx, const double *wy)
+{
+    if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
+    Grid &G = e->g[k];
+    if (e->form != WV_FORM_FP64 || WV_C != 1)
+        return fail(e, WV_EUNSUPPORTED, "Neumann ghost points are provided for the FP64 form");
+    if (n < 0) return fail(e, WV_EINVAL, "negative point count");
+    if (n > 0 && (!pi || !pj || !bi1 || !bj1 || !bi2 || !bj2 || !wx || !wy))
+        return fail(e, WV_EINVAL, "Neumann arrays missing");
+    const int nx = G.p.nx, ny = G.p.ny;
+    std::vector<uint64_t> seen((size_t)nx * G.ldw, 0);
+    for (int q = 0; q < n; q++) {
+        const int i = pi[q], j = pj[q];
+        if (i < 0 || i >= nx || j < 0 || j >= ny) return fail(e, WV_EINVAL, "Neumann point %d at (%d,%d) is outside the grid", q, i, j);
+        const int b[4] = {bi1[q], bj1[q], bi2[q], bj2[q]};
+        for (int m = 0; m < 4; m += 2)
+            if (b[m] < 0 || b[m] >= nx || b[m + 1] < 0 || b[m + 1] >= ny || std::abs(b[m] - i) > 1 || std::abs(b[m + 1] - j) > 1)
+                return fail(e, WV_EINVAL,
+                            "Neumann point %d at (%d,%d): neighbour (%d,%d) must lie in the grid within one cell", q, i,
+                            j, b[m], b[m + 1]);
+        uint64_t &w = seen[(size_t)i * G.ldw + (j >> 6)];
+        if ((w >> (j & 63)) & 1u) return fail(e, WV_EINVAL, "two Neumann points at (%d,%d)", i, j);
+        w |= 1ull << (j & 63);
+    }
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    std::vector<int> ptr, order;
+    csr_by_row(nx, n, pi, ptr, order);
+    std::vector<int> col(n), code(n);
+    std::vector<double> wv(2 * (size_t)n);
+    for (int q = 0; q < n; q++) {
+        const int o = order[q], i = pi[o], j = pj[o];
+        col[q] = j;
+        // (row offset + 1) | (column offset + 1) << 2, first neighbour low
+        code[q] = ((bi1[o] - i + 1) | ((bj1[o] - j + 1) << 2)) | (((bi2[o] - i + 1) | ((bj2[o] - j + 1) << 2)) << 4);
+        wv[2 * (size_t)q] = wx[o];
+        wv[2 * (size_t)q + 1] = wy[o];
+    }
+    int rc;
+    if (n == 0) ptr.clear();
+    if ((rc = upload_vec(e, &G.neu_ptr, ptr)) || (rc = upload_vec(e, &G.neu_col, col)) ||
+        (rc = upload_vec(e, &G.neu_code, code)) || (rc = upload_vec(e, &G.neu_w, wv)))
+        return rc;
+    G.h_ni.assign(pi, pi + n);
+    G.h_nj.assign(pj, pj + n);
+    rebuild_written(G);
+    if (G.state) G.exact = true;  // the set of written cells changed under the state
+    drop_tiles(e);
+    return sync_inst(e);
+}
+
+int set_drive(Engine *e, int k, int n, const int *pi, const int *pj, const double *val, long len)
+{
+    if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
+    Grid &G = e->g[k];
+    if (e->form != WV_FORM_FP64) return fail(e, WV_EUNSUPPORTED, "driven cells are provided for the FP64 form");
+    if (n < 0 || len < 0) return fail(e, WV_EINVAL, "negative count");
+    if (n > 0 && (!pi || !pj || !val || len < 1)) return fail(e, WV_EINVAL, "drive arrays missing");
+    const int nx = G.p.nx, ny = G.p.ny;
+    for (int q = 0; q < n; q++)
+        if (pi[q] < 0 || pi[q] >= nx || pj[q] < 0 || pj[q] >= ny)
+            return fail(e, WV_EINVAL, "driven cell %d at (%d,%d) is outside the grid", q, pi[q], pj[q]);
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    // the drive counts steps from this call (device counter minus its value now)
+    long gnow = 0;
+    HIPCHK(e, hipMemcpy(&gnow, e->d_gstep, sizeof(long), hipMemcpyDeviceToHost));
+    const bool same = G.h_di.size() == (size_t)n && std::equal(pi, pi + n, G.h_di.begin()) &&
+                      std::equal(pj, pj + n, G.h_dj.begin());
+    int rc;
+    if (!same) {
+        std::vector<int> ptr, order;
+        csr_by_row(nx, n, pi, ptr, order);
+        std::vector<int> col(n), id(n);
+        for (int q = 0; q < n; q++) { col[q] = pj[order[q]]; id[q] = order[q]; }
+        if (n == 0) ptr.clear();
+        if ((rc = upload_vec(e, &G.drv_ptr, ptr)) || (rc = upload_vec(e, &G.drv_col, col)) ||
+            (rc = upload_vec(e, &G.drv_id, id)))
+            return rc;
+        G.h_di.assign(pi, pi + n);
+        G.h_dj.assign(pj, pj + n);
+        rebuild_written(G);
+        if (G.state) G.exact = true;
+        drop_tiles(e);
+    }
+    std::vector<double> v(val, val + (size_t)n * len);
+    if ((rc = upload_vec(e, &G.drv_val, v))) return rc;
+    G.drv_len = n ? len : 0;
+    G.drv_base = gnow;
+    drop_graphs(e);
+    return sync_inst(e);
+}
+
+int set_floor(Engine *e, int k, double vmax)
+{
+    if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
+    Grid &G = e->g[k];
+    if (e->form != WV_FORM_FP64) return fail(e, WV_EUNSUPPORTED, "FLOOR is provided for the FP64 form");
+    if (!(vmax >= 0)) return fail(e, WV_EINVAL, "vmax must be >= 0 (0 turns FLOOR off)");
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    const double old = G.vmax;
+    G.vmax = vmax;
+    if ((old > 0) != (vmax > 0)) drop_tiles(e);
     return sync_inst(e);
 }
 
@@ -706,8 +889,12 @@ int enqueue_steps(Engine *e, long nsteps, bool force_k1, long *launches)
     const int kmax = max_depth(e, force_k1);
     // coefficient modes present: the kernel built for just those
     int cm = e->g[0].coef;
-    for (const Grid &G : e->g)
+    bool extras = false;
+    for (const Grid &G : e->g) {
         if (G.coef != cm) cm = WV_CM_ALL;
+        extras = extras || !G.h_ni.empty() || !G.h_di.empty() || G.vmax > 0;
+    }
+    if (extras && e->form == WV_FORM_FP64) cm = WV_CM_ALLX;  // the variant with those paths
     while (done < nsteps) {
         const int k = next_depth(kmax, nsteps - done);
         Engine::Tiles t;
@@ -1280,6 +1467,25 @@ static int engine_autotune(Engine *e, long steps_per_trial, double *best_gcells)
     if ((rc = engine_set_tiling(e, bk, bl))) return rc;
     if (best_gcells) *best_gcells = best;
     return WV_OK;
+}
+
+extern "C" int wv_set_neumann(wv_ctx *c, int n, const int *i, const int *j, const int *nbi1, const int *nbj1,
+                              const int *nbi2, const int *nbj2, const double *nx, const double *ny)
+{
+    if (!c) return WV_EINVAL;
+    return set_neumann(c->e, 0, n, i, j, nbi1, nbj1, nbi2, nbj2, nx, ny);
+}
+
+extern "C" int wv_set_drive(wv_ctx *c, int n, const int *i, const int *j, const double *val, long len)
+{
+    if (!c) return WV_EINVAL;
+    return set_drive(c->e, 0, n, i, j, val, len);
+}
+
+extern "C" int wv_set_floor(wv_ctx *c, double vmax)
+{
+    if (!c) return WV_EINVAL;
+    return set_floor(c->e, 0, vmax);
 }
 
 extern "C" int wv_autotune(wv_ctx *c, long steps_per_trial, double *best_gcells)

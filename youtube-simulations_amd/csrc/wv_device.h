@@ -310,6 +310,65 @@ __device__ __forceinline__ T add_sources(const Src &q, int rm, int jm, long g, T
 // Fast2Sum with h, exact whenever |h| >= |t| -- the normal case for a per-step
 // increment; near a zero crossing of u its error stays at the increment's
 // rounding level
+// Neumann ghost points, driven cells and the FLOOR clamp of one grid
+struct Xtra {
+    const int *nptr, *ncol, *ncode;
+    const double *nw;
+    const int *dptr, *dcol, *did;
+    const double *dval;
+    long dlen, dbase;
+    const uint64_t *nz;
+    int ldw;
+    double vmax;
+    __device__ __forceinline__ explicit Xtra(const WvInst &d)
+        : nptr(d.neu_ptr), ncol(d.neu_col), ncode(d.neu_code), nw(d.neu_w), dptr(d.drv_ptr),
+          dcol(d.drv_col), did(d.drv_id), dval(d.drv_val), dlen(d.drv_len), dbase(d.drv_base), nz(d.nz),
+          ldw(d.ldw), vmax(d.vmax)
+    {
+    }
+};
+
+// Neumann point at (rm, jm): its packed neighbour offsets and weights
+__device__ inline bool neu_find(const Xtra &q, int rm, int jm, int &code, double &w1, double &w2)
+{
+    const int k0 = gc(q.nptr)[rm], k1 = gc(q.nptr)[rm + 1];
+    bool found = false;
+    for (int k = k0; k < k1; k++)
+        if (gc(q.ncol)[k] == jm) {
+            found = true;
+            code = gc(q.ncode)[k];
+            w1 = gc(q.nw)[2 * k];
+            w2 = gc(q.nw)[2 * k + 1];
+        }
+    return found;
+}
+
+// value of a driven cell (rm, jm) at amplitude index g, if it is driven then
+__device__ inline bool drv_find(const Xtra &q, int rm, int jm, long g, double &val)
+{
+    if (g < 0 || g >= q.dlen) return false;
+    const int k0 = gc(q.dptr)[rm], k1 = gc(q.dptr)[rm + 1];
+    bool found = false;
+    for (int k = k0; k < k1; k++)
+        if (gc(q.dcol)[k] == jm) {
+            found = true;
+            val = gc(q.dval)[(size_t)gc(q.did)[k] * q.dlen + g];
+        }
+    return found;
+}
+
+// one of the 3x3 values around a cell: code = (row offset + 1) | (column offset + 1) << 2;
+// wS, w, wN: row i-1 at columns j-1, j, j+1; cS, c, cN: row i; eS, e, eN: row i+1
+__device__ __forceinline__ double pick9(int code, double wS, double w, double wN, double cS, double c,
+                                        double cN, double eS, double e, double eN)
+{
+    const int a = code & 3, b = (code >> 2) & 3;
+    const double r0 = b == 0 ? wS : (b == 1 ? w : wN);
+    const double r1 = b == 0 ? cS : (b == 1 ? c : cN);
+    const double r2 = b == 0 ? eS : (b == 1 ? e : eN);
+    return a == 0 ? r0 : (a == 1 ? r1 : r2);
+}
+
 __device__ __forceinline__ void dd_add_f(float h, float l, float w, float &ho, float &lo)
 {
     const float t = l + w;

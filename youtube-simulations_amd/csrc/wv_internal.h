@@ -29,6 +29,8 @@
 // WV_COEF_*) or any mixture (batches); a kernel built for fewer modes holds
 // fewer code paths and needs no more registers than those paths do
 #define WV_CM_ALL 3
+// any mixture, plus Neumann ghost points, driven cells and FLOOR (FP64 form)
+#define WV_CM_ALLX 4
 
 // rows in flight in a stepping kernel's prefetch ring (3 or 6)
 #ifndef WV_RING
@@ -57,6 +59,17 @@ struct WvInst {
     int fp64_full;                       // FP64 bulk cells evaluate -KAPPA*x - tgamma*(x-y)
                                          // even with zero coefficients (see wv_api.cpp)
     int tile0;                           // first tile index of this instance
+    // FP64 form only: Neumann ghost points (CSR by row: column, packed
+    // neighbour offsets, weights nx, ny), driven cells (CSR by row: column,
+    // id; values drv_val[id*drv_len + g - drv_base]) and the FLOOR clamp
+    // (vmax > 0; nz = xy_in != 0, 1 bit per cell)
+    const int *neu_ptr, *neu_col, *neu_code;
+    const double *neu_w;
+    const int *drv_ptr, *drv_col, *drv_id;
+    const double *drv_val;
+    long drv_len, drv_base;
+    const uint64_t *nz;
+    double vmax;
     double tcc0, tcc1, tg0, tg1;         // coefficient pairs (modes UNIFORM / TWO)
     float tcch0, tccl0, tcch1, tccl1, tgf0, tgf1;
     double kappa, ks, gs, kt, gt, es2;
@@ -66,6 +79,8 @@ struct WvInst {
 #define WV_TILE_INTERIOR 1  // rows and columns the tile reads never reach the outer box
 #define WV_TILE_ALLACT 2    // ... and every one of those cells is written by a step
 #define WV_TILE_SRC 4       // a point source lies in the cells the tile computes
+#define WV_TILE_NEU 8       // ... a Neumann ghost point (never an INTERIOR tile)
+#define WV_TILE_DRV 16      // ... a driven cell (never an INTERIOR tile)
 struct WvTile {
     int inst, cb, sb, flags;
 };

@@ -44,7 +44,7 @@ constexpr int CF_ANY = 4;      // per-cell tables read in place (boundary tiles)
 // tile reads is written by a step, so no active-bit plane is read.  DAMP:
 // bulk cells evaluate -KAPPA*x - tgamma*(x-y); without it they use
 // leap_undamped(), which the host selects only where that is bit-identical.
-template <int K, int ST, bool INTERIOR, int CF, bool ALLACT, bool DAMP>
+template <int K, int ST, bool INTERIOR, int CF, bool ALLACT, bool DAMP, bool XT = false>
 __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long gbase, int ia, int ib,
                                          int oc, int od)
 {
@@ -67,6 +67,12 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
     const int ibeg = t.sb * d.rows_per_tile;
     const int iend = min(ibeg + d.rows_per_tile, nx);
     const bool has_src = (t.flags & WV_TILE_SRC) && src.any();
+    // Neumann ghost points, driven cells and FLOOR (XT: built into the kernel
+    // variant for grids that use them; boundary-path tiles only)
+    const Xtra xq(d);
+    const bool xneu = XT && !INTERIOR && WV_C == 1 && (t.flags & WV_TILE_NEU);
+    const bool xdrv = XT && !INTERIOR && (t.flags & WV_TILE_DRV);
+    const bool xfloor = XT && !INTERIOR && xq.vmax > 0.0;
     const double *u0 = d.u[ia], *um1 = d.u[ib];
     const uint64_t *act = d.act, *med = d.med;
     const double *tcp = d.tc, *tccf = d.tccf, *tgf = d.tgf;
@@ -143,11 +149,12 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
                 const unsigned mb = (medreg >> sh) & CMASK;
                 const double cn = from_next(Xc[0]), cp = from_prev(Xc[C - 1]);
                 double en = 0, ep = 0, wn = 0, wp = 0;
-                if (ST == 9) {
+                if (ST == 9 || xneu) {
                     en = from_next(Xe[0]); ep = from_prev(Xe[C - 1]);
                     wn = from_next(Xw[0]); wp = from_prev(Xw[C - 1]);
                 }
                 double nu[C], np[C];
+                bool wr[C], nzc[C] = {};
 #pragma unroll
                 for (int c = 0; c < C; c++) {
                     const int cn1 = c < C - 1 ? c + 1 : 0, cp1 = c > 0 ? c - 1 : 0;
@@ -180,11 +187,37 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
                         const double tcv = gc(tcp)[(size_t)rm * ldy + L.jm[c]];
                         v = edge_fp64(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv);
                     }
+                    bool w = ALLACT || ((ab >> c) & 1u);
+                    if (XT && !INTERIOR) {
+                        // wave_billiard.c:379-389: a Neumann point takes the weighted sum of
+                        // two neighbours unless it is an edge cell the edge loops write
+                        int code = 0;
+                        double w1 = 0, w2 = 0;
+                        if (xneu && neu_find(xq, rm, L.jm[c], code, w1, w2) && !(w && !(rowint && L.colint[c]))) {
+                            double t = w1 * pick9(code & 15, wp, Wv, wn, S, x, N, ep, E, en);
+                            t = t + w2 * pick9(code >> 4, wp, Wv, wn, S, x, N, ep, E, en);
+                            v = t;
+                            w = true;
+                        }
+                        // wave_billiard.c:397, 585-590: a driven cell takes its prescribed value
+                        double dv = 0;
+                        if (xdrv && drv_find(xq, rm, L.jm[c], gbase + lv - 1 - xq.dbase, dv)) {
+                            v = dv;
+                            w = true;
+                        }
+                        // wave_billiard.c:592-609: FLOOR clamps cells with xy_in != 0
+                        nzc[c] = xfloor && ((gc(xq.nz)[(size_t)rm * xq.ldw + (L.jm[c] >> 6)] >> (L.jm[c] & 63)) & 1u);
+                        if (nzc[c] && w) {
+                            if (v > xq.vmax) v = xq.vmax;
+                            else if (v < -xq.vmax) v = -xq.vmax;
+                        }
+                    }
                     if (has_src) {
                         const long g1 = gbase + lv - 1;
                         if (g1 < src.len) v = add_sources(src, rm, L.jm[c], g1, v);
                     }
-                    nu[c] = (ALLACT || ((ab >> c) & 1u)) ? v : x;  // cells the step never writes keep u
+                    wr[c] = w;
+                    nu[c] = w ? v : x;  // cells the step never writes keep u
                     np[c] = x;
                 }
                 if (lv == K) {
@@ -201,9 +234,17 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
                     } else {
 #pragma unroll
                         for (int c = 0; c < C; c++)
-                            if (L.wcol[c] && ((ab >> c) & 1u)) {
+                            if (L.wcol[c] && wr[c]) {
                                 o0[c] = nu[c];
                                 if (K >= 2) o1[c] = np[c];
+                            } else if (K == 1 && L.wcol[c] && nzc[c]) {
+                                // FLOOR also clamps what a cell no step writes holds in
+                                // the output buffer (one step per launch, three buffers;
+                                // with fused launches such cells are +0)
+                                double q = o0[c];
+                                if (q > xq.vmax) q = xq.vmax;
+                                else if (q < -xq.vmax) q = -xq.vmax;
+                                o0[c] = q;
                             }
                     }
                 } else {
@@ -280,18 +321,19 @@ k_step_fp64(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, cons
         else WV_RUN(CF, false, false);             \
     }
     // the modes this kernel is not built for are folded away
-    const int coef = CM == WV_CM_ALL ? d.coef : CM;
+    constexpr bool XT = CM == WV_CM_ALLX;
+    const int coef = CM >= WV_CM_ALL ? d.coef : CM;
     switch (coef) {
     case WV_COEF_UNIFORM:
-        if (!interior) run_fp64<K, ST, false, CF_UNIFORM, false, true>(d, t, gbase, ia, ib, oc, od);
+        if (!interior) run_fp64<K, ST, false, CF_UNIFORM, false, true, XT>(d, t, gbase, ia, ib, oc, od);
         else WV_INTERIOR(CF_UNIFORM)
         break;
     case WV_COEF_TWO:
-        if (!interior) run_fp64<K, ST, false, CF_TWO, false, true>(d, t, gbase, ia, ib, oc, od);
+        if (!interior) run_fp64<K, ST, false, CF_TWO, false, true, XT>(d, t, gbase, ia, ib, oc, od);
         else WV_INTERIOR(CF_TWO)
         break;
     default:
-        if (!interior) run_fp64<K, ST, false, CF_ANY, false, true>(d, t, gbase, ia, ib, oc, od);
+        if (!interior) run_fp64<K, ST, false, CF_ANY, false, true, XT>(d, t, gbase, ia, ib, oc, od);
         else if (d.tgf) {
             // a tgamma field implies damping
             if (allact) WV_RUN(CF_FIELDG, true, true);
@@ -323,6 +365,7 @@ int WV_CAT(wv_launch_fp64_k, WV_K)(int stencil, int cm, const WvInst *I, const W
     case WV_COEF_UNIFORM: WV_GO(ST, WV_COEF_UNIFORM); break; \
     case WV_COEF_TWO: WV_GO(ST, WV_COEF_TWO); break;         \
     case WV_COEF_FIELD: WV_GO(ST, WV_COEF_FIELD); break;     \
+    case WV_CM_ALLX: WV_GO(ST, WV_CM_ALLX); break;           \
     default: WV_GO(ST, WV_CM_ALL); break;                     \
     }
     if (stencil == 9) {

@@ -27,6 +27,7 @@ FLAG_FULL_FORMULA = 2
 EXPORTS = [
     "wv_params_default", "wv_create", "wv_destroy", "wv_last_error", "wv_set_tables",
     "wv_set_state", "wv_set_state3", "wv_get_state", "wv_get_state3", "wv_set_sources",
+    "wv_set_neumann", "wv_set_drive", "wv_set_floor",
     "wv_step", "wv_step_timed", "wv_sync", "wv_steps_done", "wv_last_kernel_ms", "wv_energy",
     "wv_get_tiling",
     "wv_set_tiling", "wv_autotune", "wv_evolve_wave_half_rows", "wv_evolve_wave_rows",
@@ -85,6 +86,9 @@ def load():
         "wv_get_state": (ctypes.c_int, [vp, _D, _D]),
         "wv_get_state3": (ctypes.c_int, [vp, _D, _D, _D]),
         "wv_set_sources": (ctypes.c_int, [vp, ctypes.c_int, _I, _I, _D, ctypes.c_long]),
+        "wv_set_neumann": (ctypes.c_int, [vp, ctypes.c_int, _I, _I, _I, _I, _I, _I, _D, _D]),
+        "wv_set_drive": (ctypes.c_int, [vp, ctypes.c_int, _I, _I, _D, ctypes.c_long]),
+        "wv_set_floor": (ctypes.c_int, [vp, ctypes.c_double]),
         "wv_step": (ctypes.c_int, [vp, ctypes.c_long]),
         "wv_step_timed": (ctypes.c_int, [vp, ctypes.c_long, _D, ctypes.POINTER(ctypes.c_long)]),
         "wv_sync": (ctypes.c_int, [vp]),
@@ -207,6 +211,32 @@ class Wave:
         amp = np.ascontiguousarray(sources["amp"], np.float64)
         self._chk(self._L.wv_set_sources(self._h, len(si), si.ctypes.data_as(_I), sj.ctypes.data_as(_I),
                                          _d(amp), amp.shape[1]), "wv_set_sources")
+
+    def set_neumann(self, nb):
+        """Neumann ghost points: dict of int arrays i, j, nbi1, nbj1, nbi2, nbj2 and
+        float arrays nx, ny (or None to remove them)"""
+        if not nb:
+            self._chk(self._L.wv_set_neumann(self._h, 0, None, None, None, None, None, None, None, None),
+                      "wv_set_neumann")
+            return
+        ia = [np.ascontiguousarray(nb[k], np.int32) for k in ("i", "j", "nbi1", "nbj1", "nbi2", "nbj2")]
+        fa = [np.ascontiguousarray(nb[k], np.float64) for k in ("nx", "ny")]
+        self._chk(self._L.wv_set_neumann(self._h, len(ia[0]), *[a.ctypes.data_as(_I) for a in ia],
+                                         *[_d(a) for a in fa]), "wv_set_neumann")
+
+    def set_drive(self, drive):
+        """driven cells: dict i, j, val (n x len), or None"""
+        if not drive:
+            self._chk(self._L.wv_set_drive(self._h, 0, None, None, None, 0), "wv_set_drive")
+            return
+        i = np.ascontiguousarray(drive["i"], np.int32)
+        j = np.ascontiguousarray(drive["j"], np.int32)
+        val = np.ascontiguousarray(drive["val"], np.float64)
+        self._chk(self._L.wv_set_drive(self._h, len(i), i.ctypes.data_as(_I), j.ctypes.data_as(_I), _d(val),
+                                       val.shape[1]), "wv_set_drive")
+
+    def set_floor(self, vmax):
+        self._chk(self._L.wv_set_floor(self._h, float(vmax)), "wv_set_floor")
 
     def load(self, w):
         """tables, state and sources of a workload dict"""
