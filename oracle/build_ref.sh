@@ -5,7 +5,8 @@
 # repository): the parameter block oracle/ref_params.h, global_pdes.c whole,
 # then xy_to_ij (sub_wave.c:412-422), oscillating_bc (sub_wave.c:11965-12162),
 # compute_energy (wave_common.c:453-472) and the stepper (wave_billiard.c:302-631
-# or wave_3d.c:600-1007), then oracle/ref_harness.c.  Flags follow makefile:1-4
+# or wave_3d.c:600-1007; for the neighbour-table stepper sub_wave.c:11555-11963
+# and wave_3d.c:345-597), then oracle/ref_harness.c.  Flags follow makefile:1-4
 # (gcc -O3 -fopenmp) plus -ffp-contract=off.  Needs the reference sources at
 # /root/reference; machines without them use the oracle alone.
 set -euo pipefail
@@ -15,7 +16,7 @@ OUT="$HERE/_ref"
 [ -f "$REF/wave_billiard.c" ] || { echo "reference not found at $REF" >&2; exit 1; }
 mkdir -p "$OUT"
 
-build_one() {  # build_one <tag> <flat:0|1> <gcc -D flags...>
+build_one() {  # build_one <tag> <0 row pointers | 1 flat | 2 neighbour table> <gcc -D flags...>
     local tag=$1 flat=$2; shift 2
     local so="$OUT/libref_${tag}.so"
     {
@@ -23,7 +24,11 @@ build_one() {  # build_one <tag> <flat:0|1> <gcc -D flags...>
         echo "#include \"$REF/global_pdes.c\""
         sed -n '412,422p' "$REF/sub_wave.c"
         sed -n '11965,12162p' "$REF/sub_wave.c"
-        if [ "$flat" = 1 ]; then
+        if [ "$flat" = 2 ]; then
+            echo "#define TABLE_TWIN 1"
+            sed -n '11555,11963p' "$REF/sub_wave.c"
+            sed -n '345,597p' "$REF/wave_3d.c"
+        elif [ "$flat" = 1 ]; then
             echo "#define FLAT_TWIN 1"
             sed -n '600,1007p' "$REF/wave_3d.c"
         else
@@ -65,11 +70,32 @@ build_one "row_1280x720_bc0_ts0" 0 -DNX=1280 -DNY=720 -DB_COND=0 -DTWOSPEEDS=0
 # (wave_billiard.c:589) stays inside the row
 for bc in 0 1 2; do
     build_one "row_16x29_bc${bc}_ts0_ext" 0 -DNX=16 -DNY=29 -DB_COND=$bc -DTWOSPEEDS=0 -DBC_NEUMANN=1 \
-        -DOSCILLATE_LEFT=1 -DOSCILLATION_SCHEDULE=0 -DFLOOR=1 -DVMAX=0.9 -DOMEGA=0.3 $KS &
+        -DOSCILLATE_LEFT=1 -DOSCILLATION_SCHEDULE=0 -DFLOOR=1 -DVMAX=0.9 $KS &
     pids+=($!)
 done
 build_one "row_16x29_bc2_ts1_ext" 0 -DNX=16 -DNY=29 -DB_COND=2 -DTWOSPEEDS=1 -DBC_NEUMANN=1 \
-    -DOSCILLATE_LEFT=1 -DOSCILLATION_SCHEDULE=0 -DFLOOR=1 -DVMAX=0.9 -DOMEGA=0.3 $KS &
+    -DOSCILLATE_LEFT=1 -DOSCILLATION_SCHEDULE=0 -DFLOOR=1 -DVMAX=0.9 $KS &
+pids+=($!)
+for p in "${pids[@]}"; do wait "$p"; done; pids=()
+# the neighbour-table stepper (PRECOMPUTE_BC): evolve_wave_half_new
+# (wave_3d.c:345-597) with init_laplacian_coords / compute_laplacian
+# (sub_wave.c:11555-11963), B_COND 0-4 and BC_LSHAPE (10)
+for g in "24 17" "16 29"; do
+    set -- $g; nx=$1; ny=$2
+    for bc in 0 1 2 3 4 10; do
+        build_one "tbl_${nx}x${ny}_bc${bc}_ts0" 2 -DNX=$nx -DNY=$ny -DB_COND=$bc -DTWOSPEEDS=0 $KS &
+        pids+=($!)
+    done
+    for p in "${pids[@]}"; do wait "$p"; done; pids=()
+done
+build_one "tbl_24x17_bc1_ts1" 2 -DNX=24 -DNY=17 -DB_COND=1 -DTWOSPEEDS=1 $KS &
+pids+=($!)
+build_one "tbl_24x17_bc0_ts0_tb" 2 -DNX=24 -DNY=17 -DB_COND=0 -DTWOSPEEDS=0 -DOSCILLATE_TOPBOT=1 $KS &
+pids+=($!)
+build_one "tbl_16x29_bc3_ts0_tb" 2 -DNX=16 -DNY=29 -DB_COND=3 -DTWOSPEEDS=0 -DOSCILLATE_TOPBOT=1 $KS &
+pids+=($!)
+build_one "tbl_16x29_bc2_ts0_ext" 2 -DNX=16 -DNY=29 -DB_COND=2 -DTWOSPEEDS=0 -DOSCILLATE_LEFT=1 \
+    -DOSCILLATION_SCHEDULE=0 -DFLOOR=1 -DVMAX=0.9 $KS &
 pids+=($!)
 for p in "${pids[@]}"; do wait "$p"; done; pids=()
 # a lens medium with damping and absorbing edges (energy fixture, two speeds)

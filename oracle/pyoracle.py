@@ -86,6 +86,10 @@ def lib():
                                     _I, _I, _D, ctypes.c_long, ctypes.c_long]
         L.orc_run_x.argtypes = [P, _D, _D, _D, _I, ctypes.c_long, _S, _D, _D, _D, ctypes.c_int,
                                 _I, _I, _D, ctypes.c_long, ctypes.c_long, ctypes.POINTER(OrcExtras)]
+        L.orc_laplacian_table.argtypes = [P, _I, _I, _I]
+        L.orc_run_table.argtypes = [P, _D, _D, _D, _I, ctypes.c_long, _S, _D, _D, _D, _I, _I,
+                                    ctypes.c_int, _I, _I, _D, ctypes.c_long, ctypes.c_long,
+                                    ctypes.POINTER(OrcExtras)]
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_split_energy_total.restype = ctypes.c_double
         L.orc_split_energy_total.argtypes = [P, _F, _F, _F, _S, _D, ctypes.c_double]
@@ -172,6 +176,64 @@ def run_x(p, phi, psi, nsteps, xy_in, tcc, tc, tgamma, tmp=None, sources=None, s
     return b[s], b[(s + 1) % 3], b[(s + 2) % 3]
 
 
+def laplacian_table(p, lsh=(0, 0, 0, 0, 0, 0)):
+    """init_laplacian_coords as (nneighb[NX,NY], nghb[NX,NY,4]) flat indices;
+    lsh = (i1, j1, i2, j2, i3, j3) for BC_LSHAPE."""
+    nn = np.zeros((p.nx, p.ny), np.int32)
+    nb = np.zeros((p.nx, p.ny, 4), np.int32)
+    ls = np.ascontiguousarray(lsh, np.int32)
+    lib().orc_laplacian_table(ctypes.byref(p), _p(ls, _I), _p(nn, _I), _p(nb, _I))
+    return nn, nb
+
+
+def _extras(drive, vmax, keep):
+    def ia(a):
+        a = np.ascontiguousarray(a, np.int32)
+        keep.append(a)
+        return _p(a, _I)
+
+    def da(a):
+        a = np.ascontiguousarray(a, np.float64)
+        keep.append(a)
+        return _p(a, _D)
+
+    x = OrcExtras()
+    z = np.zeros(1, np.int32)
+    x.n_neu = 0
+    x.ni = x.nj = x.nbi1 = x.nbj1 = x.nbi2 = x.nbj2 = ia(z)
+    x.nnx = x.nny = da(np.zeros(1))
+    if drive:
+        val = np.ascontiguousarray(drive["val"], np.float64)
+        x.n_drv, x.dlen = val.shape[0], val.shape[1]
+        x.di, x.dj, x.dval = ia(drive["i"]), ia(drive["j"]), da(val.reshape(-1))
+    else:
+        x.n_drv, x.dlen = 0, 0
+        x.di = x.dj = ia(z)
+        x.dval = da(np.zeros(1))
+    x.vmax = vmax
+    return x
+
+
+def run_table(p, phi, psi, nsteps, xy_in, tcc, tc, tgamma, nneighb, nghb, tmp=None, sources=None,
+              step0=0, drive=None, vmax=0.0):
+    """nsteps of the neighbour-table stepper (evolve_wave_half_new) with the
+    evolve_wave rotation, driven cells and FLOOR as run_x."""
+    b = [np.array(phi, dtype=np.float64, copy=True), np.array(psi, dtype=np.float64, copy=True),
+         np.zeros_like(phi, dtype=np.float64) if tmp is None else np.array(tmp, dtype=np.float64)]
+    st = ctypes.c_int(0)
+    nsrc, si, sj, amp, alen = _sources(sources)
+    keep = []
+    x = _extras(drive, vmax, keep)
+    nn = np.ascontiguousarray(nneighb, np.int32)
+    nb = np.ascontiguousarray(nghb, np.int32)
+    lib().orc_run_table(ctypes.byref(p), _p(b[0], _D), _p(b[1], _D), _p(b[2], _D), ctypes.byref(st),
+                        nsteps, _p(xy_in, _S), _p(tcc, _D), _p(tc, _D), _p(tgamma, _D), _p(nn, _I),
+                        _p(nb, _I), nsrc, _p(si, _I), _p(sj, _I), _p(amp, _D), alen, step0,
+                        ctypes.byref(x))
+    s = st.value
+    return b[s], b[(s + 1) % 3], b[(s + 2) % 3]
+
+
 def _sources(sources):
     if not sources:
         z = np.zeros(1, np.int32)
@@ -243,7 +305,14 @@ class RefBuild:
         self.nx = self.L.ref_nx()
         self.ny = self.L.ref_ny()
         self.flat = hasattr(self.L, "ref_evolve_wave_half_flat")
-        if self.flat:
+        self.table = hasattr(self.L, "ref_evolve_wave_half_new")
+        if self.table:
+            self.L.ref_evolve_wave_half_new.argtypes = [_D, _D, _D, _S, _D, _D, _D]
+            self.L.ref_laplacian_table.argtypes = [_I, _I]
+            self.L.ref_lshape_lines.argtypes = [_I]
+            self.L.ref_corner_drive.restype = ctypes.c_double
+            self.L.ref_corner_drive.argtypes = [ctypes.c_int]
+        elif self.flat:
             self.L.ref_evolve_wave_half_flat.argtypes = [_D, _D, _D, _S, _D, _D, _D]
         else:
             self.L.ref_evolve_wave_half.argtypes = [_D, _D, _D, _S, _D, _D, _D]
@@ -265,8 +334,26 @@ class RefBuild:
     def oscillating_bc(self, time, j):
         return self.L.ref_oscillating_bc(time, j)
 
+    def laplacian_table(self):
+        """the reference's init_laplacian_coords table as counts and flat indices"""
+        nn = np.zeros((self.nx, self.ny), np.int32)
+        nb = np.zeros((self.nx, self.ny, 4), np.int32)
+        self.L.ref_laplacian_table(_p(nn, _I), _p(nb, _I))
+        return nn, nb
+
+    def lshape_lines(self):
+        a = np.zeros(6, np.int32)
+        self.L.ref_lshape_lines(_p(a, _I))
+        return tuple(int(v) for v in a)
+
+    def corner_drive(self, time):
+        return self.L.ref_corner_drive(time)
+
     def half(self, phi_in, psi_in, phi_out, xy_in, tcc, tc, tgamma):
-        if self.flat:
+        if self.table:
+            self.L.ref_evolve_wave_half_new(_p(phi_in, _D), _p(psi_in, _D), _p(phi_out, _D),
+                                            _p(xy_in, _S), _p(tc, _D), _p(tcc, _D), _p(tgamma, _D))
+        elif self.flat:
             self.L.ref_evolve_wave_half_flat(_p(phi_in, _D), _p(psi_in, _D), _p(phi_out, _D),
                                              _p(xy_in, _S), _p(tc, _D), _p(tcc, _D), _p(tgamma, _D))
         else:

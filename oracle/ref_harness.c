@@ -1,8 +1,9 @@
 /*
  * ref_harness.c -- appended after the reference functions by
  * oracle/build_ref.sh.  Exposes them on flat k = i*NY + j arrays so the
- * tests can call them through ctypes.  FLAT_TWIN selects which reference
- * stepper was compiled in (wave_billiard.c row-pointer or wave_3d.c flat).
+ * tests can call them through ctypes.  FLAT_TWIN / TABLE_TWIN select which
+ * reference stepper was compiled in (wave_billiard.c row-pointer, wave_3d.c
+ * flat, or the wave_3d.c neighbour-table stepper).
  */
 /* Neumann ghost points handed to the reference stepper (n_neumann_points of
  * global_pdes.c says how many are used; none until ref_set_neumann) */
@@ -33,7 +34,73 @@ int ref_set_neumann(int n, const int *pi, const int *pj, const int *bi1, const i
 double ref_oscillating_bc(int time, int j) { return oscillating_bc(time, j); }
 int ref_beam_bc(void) { return BEAM_BC; }
 
-#ifndef FLAT_TWIN
+#if defined(TABLE_TWIN)
+#include <fcntl.h>
+#include <unistd.h>
+/* init_laplacian_coords (sub_wave.c:11555) on a zeroed table for the field at
+ * `base`, its progress message discarded.  A table depends only on the
+ * address it was built for, so three are kept by address (the three buffers
+ * of a rotation). */
+static double *ref_tbl_base[3];
+static t_laplacian *ref_tbl[3];
+static int ref_tbl_next;
+static t_laplacian *ref_table_for(double *base)
+{
+    for (int q = 0; q < 3; q++)
+        if (ref_tbl[q] && ref_tbl_base[q] == base) return ref_tbl[q];
+    const int q = ref_tbl_next;
+    ref_tbl_next = (ref_tbl_next + 1) % 3;
+    free(ref_tbl[q]);
+    ref_tbl[q] = (t_laplacian *)calloc((size_t)NX * NY, sizeof(t_laplacian));
+    ref_tbl_base[q] = base;
+    fflush(stdout);
+    const int keep = dup(1), null = open("/dev/null", O_WRONLY);
+    dup2(null, 1);
+    init_laplacian_coords(ref_tbl[q], base);
+    fflush(stdout);
+    dup2(keep, 1);
+    close(keep);
+    close(null);
+    return ref_tbl[q];
+}
+
+/* the table as counts and flat indices (-1 where the builder set nothing) */
+void ref_laplacian_table(int *nn, int *nb)
+{
+    double *f = (double *)calloc((size_t)NX * NY, sizeof(double));
+    t_laplacian *L = ref_table_for(f);
+    for (long k = 0; k < (long)NX * NY; k++) {
+        nn[k] = L[k].nneighb;
+        for (int q = 0; q < 4; q++) nb[4 * k + q] = L[k].nghb[q] ? (int)(L[k].nghb[q] - f) : -1;
+    }
+    for (int q = 0; q < 3; q++)
+        if (ref_tbl[q] == L) { free(L); ref_tbl[q] = NULL; ref_tbl_base[q] = NULL; }
+    free(f);
+}
+
+/* the L-shape lines i1, j1, i2, j2, i3, j3 as init_laplacian_coords computes them */
+void ref_lshape_lines(int *lsh)
+{
+    int ij[2];
+    xy_to_ij(-LAMBDA, -1.0, ij);
+    lsh[0] = ij[0] + 1; lsh[1] = ij[1] + 1;
+    xy_to_ij(0.0, 0.0, ij);
+    lsh[2] = ij[0] - 1; lsh[3] = ij[1] - 1;
+    xy_to_ij(LAMBDA, 1.0, ij);
+    lsh[4] = ij[0] - 1; lsh[5] = ij[1] - 1;
+}
+
+/* the corner drive of wave_3d.c:575-579 */
+double ref_corner_drive(int time) { return AMPLITUDE * cos((double)time * OMEGA); }
+
+/* evolve_wave_half_new, wave_3d.c:345, with the table built for phi_in (as
+ * evolve_wave passes laplace / laplace1 / laplace2, wave_3d.c:1019-1021) */
+void ref_evolve_wave_half_new(double *phi_in, double *psi_in, double *phi_out, short *xy_in,
+                              double *tc, double *tcc, double *tgamma)
+{
+    evolve_wave_half_new(phi_in, psi_in, phi_out, xy_in, tc, tcc, tgamma, ref_table_for(phi_in));
+}
+#elif !defined(FLAT_TWIN)
 /* evolve_wave_half, wave_billiard.c:302 */
 void ref_evolve_wave_half(double *phi_in, double *psi_in, double *phi_out, short *xy_in,
                           double *tcc, double *tc, double *tgamma)

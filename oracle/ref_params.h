@@ -65,9 +65,14 @@
 #define OSCIL_LEFT_YSHIFT 0.0
 #define INITIAL_SHIFT 20.0
 #define WAVE_PACKET_SHIFT 200.0
+#ifndef OMEGA
 #define OMEGA 0.005
+#endif
 #define OMEGA_B 0.01
 #define AMPLITUDE 1.0
+#ifndef LAMBDA
+#define LAMBDA 1.5      /* D_LSHAPE / BC_LSHAPE half-width, sub_wave.c:6417, 11780 */
+#endif
 #define ACHIRP 0.25
 #define DAMPING 0.0
 #define INITIAL_VARIANCE 0.0003

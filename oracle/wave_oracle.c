@@ -374,6 +374,197 @@ void orc_run_x(const orc_params *p, double *b0, double *b1, double *b2, int *sta
     free(m); free(t1); free(t2); free(t3);
 }
 
+/* ------------------------------------------------- neighbour-table path -- */
+
+/* one table entry: n neighbours (a, b, c, d), flat indices i*NY + j */
+static void tbl_set(const orc_params *p, int *nn, int *nb, int i, int j, int n, int ia, int ja, int ib,
+                    int jb, int ic, int jc, int id, int jd)
+{
+    const int NY = p->ny;
+    const long k = (long)i * NY + j;
+    nn[k] = n;
+    nb[4 * k] = ia * NY + ja;
+    nb[4 * k + 1] = ib * NY + jb;
+    nb[4 * k + 2] = ic * NY + jc;
+    nb[4 * k + 3] = n == 4 ? id * NY + jd : -1;
+}
+
+void orc_laplacian_table(const orc_params *p, const int lsh[6], int *nn, int *nb)
+{
+    const int NX = p->nx, NY = p->ny;
+    for (long k = 0; k < (long)NX * NY; k++) {
+        nn[k] = 0;
+        nb[4 * k] = nb[4 * k + 1] = nb[4 * k + 2] = nb[4 * k + 3] = -1;
+    }
+    /* bulk, sub_wave.c:11563-11572: (i+1, j), (i-1, j), (i, j+1), (i, j-1) */
+    for (int i = 1; i < NX - 1; i++)
+        for (int j = 1; j < NY - 1; j++) tbl_set(p, nn, nb, i, j, 4, i + 1, j, i - 1, j, i, j + 1, i, j - 1);
+    const int bc = p->bcond;
+    if (bc >= ORC_BC_DIRICHLET && bc <= ORC_BC_VPER_HABS) {
+        const int per = bc == ORC_BC_PERIODIC;
+        /* left and right edges, 1 <= j <= NY-2 (11577-11594, 11625-11646,
+         * 11680-11697, 11729-11746): three neighbours, periodic adds the wrap */
+        for (int j = 1; j < NY - 1; j++) {
+            if (per) {
+                tbl_set(p, nn, nb, 0, j, 4, 1, j, NX - 1, j, 0, j + 1, 0, j - 1);
+                tbl_set(p, nn, nb, NX - 1, j, 4, NX - 2, j, 0, j, NX - 1, j + 1, NX - 1, j - 1);
+            } else {
+                tbl_set(p, nn, nb, 0, j, 3, 1, j, 0, j + 1, 0, j - 1, 0, 0);
+                tbl_set(p, nn, nb, NX - 1, j, 3, NX - 2, j, NX - 1, j + 1, NX - 1, j - 1, 0, 0);
+            }
+        }
+        /* top (j = NY-1) and bottom (j = 0) rows, corners included: i+-1
+         * clamped, or wrapped for periodic; periodic and VPER_HABS add the
+         * neighbour across the y seam (11596-11620, 11648-11675, 11699-11724,
+         * 11748-11775) */
+        const int four = per || bc == ORC_BC_VPER_HABS;
+        for (int i = 0; i < NX; i++) {
+            int ip, im;
+            if (per) {
+                ip = (i + 1) % NX;
+                im = (i - 1 + NX) % NX;
+            } else {
+                ip = i + 1 == NX ? NX - 1 : i + 1;
+                im = i == 0 ? 0 : i - 1;
+            }
+            tbl_set(p, nn, nb, i, NY - 1, four ? 4 : 3, ip, NY - 1, im, NY - 1, i, NY - 2, i, 0);
+            tbl_set(p, nn, nb, i, 0, four ? 4 : 3, ip, 0, im, 0, i, 1, i, NY - 1);
+        }
+    } else if (bc == ORC_BC_LSHAPE) {
+        /* sub_wave.c:11778-11889, in the reference's order (later entries win) */
+        const int i1 = lsh[0], j1 = lsh[1], i2 = lsh[2], j2 = lsh[3], i3 = lsh[4], j3 = lsh[5];
+        for (int j = j1 + 1; j < j2; j++) tbl_set(p, nn, nb, i1, j, 4, i1 + 1, j, i3, j, i1, j + 1, i1, j - 1);
+        for (int j = j2; j < j3 - 1; j++) tbl_set(p, nn, nb, i1, j, 4, i1 + 1, j, i2 - 1, j, i1, j + 1, i1, j - 1);
+        for (int j = j1 + 1; j < j2; j++) tbl_set(p, nn, nb, i3, j, 4, i1, j, i3 - 1, j, i3, j + 1, i3, j - 1);
+        for (int j = j2; j < j3 - 1; j++) tbl_set(p, nn, nb, i2, j, 4, i1, j, i2 - 1, j, i2, j + 1, i2, j - 1);
+        for (int i = i1; i < i2; i++) tbl_set(p, nn, nb, i, j3, 4, i + 1, j3, i - 1, j3, i, j1, i, j3 - 1);
+        for (int i = i2; i < i3; i++) tbl_set(p, nn, nb, i, j2, 4, i + 1, j2, i - 1, j2, i, j1, i, j2 - 1);
+        for (int i = i1; i < i2; i++) tbl_set(p, nn, nb, i, j1, 4, i + 1, j1, i - 1, j1, i, j1 + 1, i, j3);
+        for (int i = i2; i < i3; i++) tbl_set(p, nn, nb, i, j1, 4, i + 1, j1, i - 1, j1, i, j1 + 1, i, j2);
+        tbl_set(p, nn, nb, i1, j1, 4, i1 + 1, j1, i3 - 1, j1, i1, j1 + 1, i1, j3 - 1);
+        tbl_set(p, nn, nb, i3, j1, 4, i1, j1, i3 - 1, j1, i3, j1 + 1, i3, j3);
+        tbl_set(p, nn, nb, i1, j3, 4, i1 + 1, j3, i3, j3, i1, j1, i1, j3 - 1);
+    }
+}
+
+/* -n*x plus the n table neighbours in order (sub_wave.c:11927-11962) */
+static inline double tbl_delta(const double *phi, const int *nn, const int *nb, long k, int nfix)
+{
+    const int n = nfix ? nfix : nn[k];
+    double d = -(double)n * phi[k];
+    for (int q = 0; q < n; q++) d += phi[nb[4 * k + q]];
+    return d;
+}
+
+/* one step of evolve_wave_half_new, wave_3d.c:345-597 (Laplacian of
+ * compute_laplacian, sub_wave.c:11895-11963) */
+static void table_step(const orc_params *p, const double *phi_in, const double *psi_in, double *phi_out,
+                       const short *xy, const double *tc, const double *tcc, const double *tg,
+                       const int *nn, const int *nb)
+{
+    const int NX = p->nx, NY = p->ny, bc = p->bcond;
+    const double KAPPA = p->kappa, KS = p->kappa_sides, GS = p->gamma_sides;
+    const double KT = p->kappa_topbot, GT = p->gamma_topbot;
+    const int lshape = bc == ORC_BC_LSHAPE;
+    double *delta = (double *)malloc(sizeof(double) * (size_t)NX * NY);
+    int i, j;
+#pragma omp parallel for private(i, j)
+    for (i = 1; i < NX - 1; i++)
+        for (j = 1; j < NY - 1; j++) {
+            const long k = (long)i * NY + j;
+            delta[k] = lshape ? tbl_delta(phi_in, nn, nb, k, 4)
+                              : phi_in[k + NY] + phi_in[k - NY] + phi_in[k + 1] + phi_in[k - 1] - 4.0 * phi_in[k];
+        }
+    for (i = 0; i < NX; i++) {
+        delta[(long)i * NY] = tbl_delta(phi_in, nn, nb, (long)i * NY, 0);
+        delta[(long)i * NY + NY - 1] = tbl_delta(phi_in, nn, nb, (long)i * NY + NY - 1, 0);
+    }
+    for (j = 1; j < NY - 1; j++) {
+        delta[j] = tbl_delta(phi_in, nn, nb, j, 0);
+        delta[(long)(NX - 1) * NY + j] = tbl_delta(phi_in, nn, nb, (long)(NX - 1) * NY + j, 0);
+    }
+    /* bulk, wave_3d.c:390-402 */
+#pragma omp parallel for private(i, j)
+    for (i = 1; i < NX - 1; i++)
+        for (j = 1; j < NY - 1; j++) {
+            const long k = (long)i * NY + j;
+            if (p->twospeeds || xy[k] != 0) phi_out[k] = leap(phi_in[k], psi_in[k], delta[k], tcc[k], KAPPA, tg[k]);
+        }
+    /* left and right edges, wave_3d.c:404-482 */
+    for (int side = 0; side < 2; side++) {
+        const int ie = side ? NX - 1 : 0, in = side ? NX - 2 : 1;
+        for (j = 1; j < NY - 1; j++) {
+            const long k = (long)ie * NY + j;
+            if (!(p->twospeeds || xy[k] != 0)) continue;
+            const double x = phi_in[k], y = psi_in[k];
+            if (bc == ORC_BC_DIRICHLET || bc == ORC_BC_PERIODIC)
+                phi_out[k] = leap(x, y, delta[k], tcc[k], KAPPA, tg[k]);
+            else if (bc >= ORC_BC_ABSORBING && bc <= ORC_BC_ABS_REFLECT)
+                phi_out[k] = absorb(x, y, phi_in[(long)in * NY + j], tc[k], KS, GS);
+        }
+    }
+    /* top (j = NY-1) and bottom (j = 0) rows, wave_3d.c:484-572 */
+    for (int side = 0; side < 2; side++) {
+        const int je = side ? 0 : NY - 1, jn = side ? 1 : NY - 2;
+        for (i = 0; i < NX; i++) {
+            const long k = (long)i * NY + je;
+            if (!(p->twospeeds || xy[k] != 0)) continue;
+            const double x = phi_in[k], y = psi_in[k];
+            if (p->oscillate_topbot && i < p->tb_shift && i < NX - 1 && i > 0) {
+                phi_out[k] = leap(x, y, delta[k], tcc[k], KAPPA, tg[k]);
+                continue;
+            }
+            switch (bc) {
+            case ORC_BC_DIRICHLET:
+            case ORC_BC_PERIODIC:
+                phi_out[k] = leap(x, y, delta[k], tcc[k], KAPPA, tg[k]);
+                break;
+            case ORC_BC_ABSORBING:
+            case ORC_BC_ABS_REFLECT:
+                phi_out[k] = absorb(x, y, phi_in[(long)i * NY + jn], tc[k], KT, GT);
+                break;
+            case ORC_BC_VPER_HABS:
+                if (i == 0) phi_out[k] = absorb(x, y, phi_in[(long)NY + je], tc[k], KS, GS);
+                else phi_out[k] = leap(x, y, delta[k], tcc[k], KAPPA, tg[k]);
+                break;
+            default: /* BC_LSHAPE: no edge formula */
+                break;
+            }
+        }
+    }
+    free(delta);
+}
+
+void orc_run_table(const orc_params *p, double *b0, double *b1, double *b2, int *state,
+                   long nsteps, const short *xy_in, const double *tcc, const double *tc,
+                   const double *tgamma, const int *nneighb, const int *nghb, int nsrc,
+                   const int *si, const int *sj, const double *amp, long amp_len, long step0,
+                   const orc_extras *x)
+{
+    double *R[3] = {b0, b1, b2};
+    const long NY = p->ny, n = (long)p->nx * p->ny;
+    int s = *state;
+    for (long m = 0; m < nsteps; m++) {
+        double *phi = R[s], *psi = R[(s + 1) % 3], *tmp = R[(s + 2) % 3];
+        const long g = step0 + m;
+        table_step(p, phi, psi, tmp, xy_in, tc, tcc, tgamma, nneighb, nghb);
+        /* driven cells: the left edge (wave_3d.c:413) and the corners (574-579) */
+        if (x && g < x->dlen)
+            for (int k = 0; k < x->n_drv; k++) tmp[x->di[k] * NY + x->dj[k]] = x->dval[(size_t)k * x->dlen + g];
+        /* FLOOR, wave_3d.c:581-596 */
+        if (x && x->vmax > 0.0)
+            for (long k = 0; k < n; k++)
+                if (xy_in[k] != 0) {
+                    if (tmp[k] > x->vmax) tmp[k] = x->vmax;
+                    if (tmp[k] < -x->vmax) tmp[k] = -x->vmax;
+                }
+        if (g < amp_len)
+            for (int k = 0; k < nsrc; k++) tmp[si[k] * NY + sj[k]] += amp[(size_t)k * amp_len + g];
+        s = (s + 2) % 3;
+    }
+    *state = s;
+}
+
 /* ------------------------------------------------------------ split form -- */
 
 void orc_split_from_fp64(long n, const double *u_n, const double *u_nm1, float *hi, float *lo,

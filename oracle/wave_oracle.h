@@ -110,6 +110,38 @@ void orc_run_x(const orc_params *p, double *b0, double *b1, double *b2, int *sta
                const double *tgamma, int nsrc, const int *si, const int *sj,
                const double *amp, long amp_len, long step0, const orc_extras *x);
 
+/* ---- neighbour-table path (PRECOMPUTE_BC) --------------------------------
+ * The reference's second stepper, evolve_wave_half_new (wave_3d.c:345-597),
+ * takes the Laplacian's neighbours from a table (t_laplacian,
+ * global_pdes.c:493-497: a count and up to four pointers into the field the
+ * table was built for).  Here an entry is nneighb[k] and nghb[4k+q], the flat
+ * index i*NY+j of neighbour q in whichever buffer is phi_in.
+ *
+ * orc_laplacian_table restates init_laplacian_coords (sub_wave.c:11555-11892)
+ * for B_COND 0-3 and BC_LSHAPE (genus-2 wrap of the L-shaped billiard, lines
+ * i1, i2, i3 / j1, j2, j3 given in lsh[6], computed by the reference from
+ * LAMBDA with xy_to_ij).  Entries it leaves unset (edges of B_COND 4 and
+ * BC_LSHAPE) are nneighb 0, indices -1. */
+#define ORC_BC_LSHAPE 10 /* global_pdes.c:276 */
+
+void orc_laplacian_table(const orc_params *p, const int lsh[6], int *nneighb, int *nghb);
+
+/* nsteps steps of evolve_wave_half_new with the rotation of orc_run (the
+ * rotation of evolve_wave, wave_3d.c:1015-1021) and the same drive / FLOOR /
+ * point-source conventions as orc_run_x (no Neumann points in this path).
+ * In one step: Laplacian (compute_laplacian, sub_wave.c:11895-11963: bulk
+ * (((E+W)+N)+S)-4x, or for BC_LSHAPE -4x plus the four table neighbours in
+ * order; edge cells -n*x plus their n table neighbours in order), bulk cells
+ * with TWOSPEEDS or xy_in != 0, left / right / top / bottom edge cells per
+ * B_COND (no edge formula for BC_LSHAPE), driven cells, FLOOR, sources.
+ * The reference leaves the Laplacian of cells with xy_in == 0 unset and reads
+ * it when TWOSPEEDS is on; here it is computed like any other cell. */
+void orc_run_table(const orc_params *p, double *b0, double *b1, double *b2, int *state,
+                   long nsteps, const short *xy_in, const double *tcc, const double *tc,
+                   const double *tgamma, const int *nneighb, const int *nghb, int nsrc,
+                   const int *si, const int *sj, const double *amp, long amp_len, long step0,
+                   const orc_extras *x);
+
 /* Per-cell energy, compute_energy wave_common.c:453-472, with the Courant
  * constants given explicitly (c2_in = COURANT^2 for xy_in != 0, c2_out =
  * COURANTB^2 used when TWOSPEEDS). */

@@ -104,3 +104,33 @@ def ext_inputs(tag, nx, ny):
     # the cells the reference drives: (0, j) for 0 < j < NY-1, then the corners (0, 0) and (0, NX-1)
     d["drive_cells"] = [(0, j) for j in range(1, ny - 1)] + [(0, 0), (0, nx - 1)]
     return d
+
+
+TABLE_STEPS = 7
+
+
+def table_cases():
+    """(tag, nx, ny, bcond, twospeeds, oscillate_topbot, extras) of the
+    reference builds of the neighbour-table stepper (evolve_wave_half_new);
+    extras: the oscillating left edge and corners plus FLOOR (VMAX 0.9)."""
+    out = []
+    for nx, ny in ((24, 17), (16, 29)):
+        for bc in (0, 1, 2, 3, 4, 10):
+            out.append(("tbl_%dx%d_bc%d_ts0" % (nx, ny, bc), nx, ny, bc, 0, 0, 0))
+    out += [("tbl_24x17_bc1_ts1", 24, 17, 1, 1, 0, 0), ("tbl_24x17_bc0_ts0_tb", 24, 17, 0, 0, 1, 0),
+            ("tbl_16x29_bc3_ts0_tb", 16, 29, 3, 0, 1, 0), ("tbl_16x29_bc2_ts0_ext", 16, 29, 2, 0, 0, 1)]
+    return out
+
+
+def table_inputs(tag, nx, ny, twospeeds, extras):
+    """case_inputs; with TWOSPEEDS no cell has xy_in == 0 (the reference reads
+    an unset Laplacian there); with extras the fields are halved and the
+    driven cells listed: (0, j) for 0 < j < NY-1, then (0, 0) and (0, NY-1)."""
+    d = case_inputs(tag, nx, ny)
+    if twospeeds:
+        d["xy_in"] = np.where(d["xy_in"] == 0, 1, d["xy_in"]).astype(np.int16)
+    if extras:
+        for k in ("phi", "psi", "tmp"):
+            d[k] = d[k] * 0.5
+        d["drive_cells"] = [(0, j) for j in range(1, ny - 1)] + [(0, 0), (0, ny - 1)]
+    return d

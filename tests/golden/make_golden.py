@@ -14,6 +14,10 @@ Outputs (committed):
   ext_cases.npz     Neumann ghost points + oscillating left edge + FLOOR:
                     the drive values oscillating_bc(time, j) and phi, psi,
                     tmp after EXT_STEPS steps, for every gen_inputs.ext_cases()
+  table_cases.npz   the neighbour-table stepper (evolve_wave_half_new): the
+                    reference's table (counts, flat indices), the L-shape
+                    lines, drive values where driven, and phi, psi, tmp after
+                    TABLE_STEPS steps, for every gen_inputs.table_cases()
 """
 import hashlib
 import json
@@ -29,7 +33,8 @@ sys.path.insert(0, os.path.join(ROOT, "youtube-simulations_amd"))
 sys.path.insert(0, HERE)
 
 import pyoracle  # noqa: E402
-from gen_inputs import EXT_STEPS, case_inputs, ext_cases, ext_inputs, small_cases  # noqa: E402
+from gen_inputs import (EXT_STEPS, TABLE_STEPS, case_inputs, ext_cases, ext_inputs, small_cases,  # noqa: E402
+                        table_cases, table_inputs)
 from wavehip import workloads  # noqa: E402
 
 NSTEPS_SMALL = 7
@@ -90,6 +95,22 @@ def main():
         phi, psi, tmp = R.run(b[0], b[1], b[2], EXT_STEPS, d["xy_in"], d["tcc"], d["tc"], d["tgamma"])
         out[tag + "/phi"], out[tag + "/psi"], out[tag + "/tmp"] = phi, psi, tmp
     np.savez_compressed(os.path.join(HERE, "ext_cases.npz"), **out)
+    # neighbour-table stepper
+    out = {}
+    for (tag, nx, ny, bc, ts, tb, ex) in table_cases():
+        R = pyoracle.RefBuild(tag)
+        assert (R.nx, R.ny) == (nx, ny) and R.table
+        d = table_inputs(tag, nx, ny, ts, ex)
+        out[tag + "/nneighb"], out[tag + "/nghb"] = R.laplacian_table()
+        out[tag + "/lsh"] = np.array(R.lshape_lines(), np.int32)
+        if ex:
+            out[tag + "/drive_val"] = np.array(
+                [[R.oscillating_bc(t, j) for t in range(1, TABLE_STEPS + 1)] for (_, j) in d["drive_cells"][:-2]]
+                + [[R.corner_drive(t) for t in range(1, TABLE_STEPS + 1)]] * 2)
+        b = [d["phi"].copy(), d["psi"].copy(), d["tmp"].copy()]
+        phi, psi, tmp = R.run(b[0], b[1], b[2], TABLE_STEPS, d["xy_in"], d["tcc"], d["tc"], d["tgamma"])
+        out[tag + "/phi"], out[tag + "/psi"], out[tag + "/tmp"] = phi, psi, tmp
+    np.savez_compressed(os.path.join(HERE, "table_cases.npz"), **out)
     print("fixtures written")
 
 

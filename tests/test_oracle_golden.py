@@ -2,7 +2,8 @@
 
 Golden vectors (tests/golden/) were produced by tests/golden/make_golden.py
 from the reference compiled from its sources (wave_billiard.c:302-631,
-wave_3d.c:600-1007, wave_common.c:453-472).  Comparisons are bit-exact.
+wave_3d.c:345-597 and 600-1007, sub_wave.c:11555-11963,
+wave_common.c:453-472).  Comparisons are bit-exact.
 """
 import hashlib
 import json
@@ -11,7 +12,8 @@ import os
 import numpy as np
 import pytest
 
-from gen_inputs import EXT_STEPS, EXT_VMAX, KAPPAS, case_inputs, ext_cases, ext_inputs, small_cases
+from gen_inputs import (EXT_STEPS, EXT_VMAX, KAPPAS, TABLE_STEPS, case_inputs, ext_cases, ext_inputs,
+                        small_cases, table_cases, table_inputs)
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -115,6 +117,102 @@ def test_extras_fixture_detects_planted_mistakes(oracle, ext):
     for mut in (dict(neumann=None), dict(drive=None), dict(vmax=0.0), dict(neumann=swapped)):
         phi, _, _ = ext_run(oracle, case, ext, **mut)
         assert not bits_equal(phi, ext[tag + "/phi"]), mut
+
+
+@pytest.fixture(scope="module")
+def tbl():
+    return np.load(os.path.join(GOLD, "table_cases.npz"))
+
+
+def table_run(oracle, case, tbl, table=None, **kw):
+    tag, nx, ny, bc, ts, tb, ex = case
+    d = table_inputs(tag, nx, ny, ts, ex)
+    p = oracle.make_params(nx, ny, bcond=kw.pop("bcond", bc), twospeeds=ts, oscillate_topbot=tb, tb_shift=nx,
+                           **KAPPAS)
+    if table is None:
+        table = oracle.laplacian_table(p, tuple(tbl[tag + "/lsh"]))
+    if ex:
+        cells = d["drive_cells"]
+        kw.setdefault("drive", dict(i=[c[0] for c in cells], j=[c[1] for c in cells], val=tbl[tag + "/drive_val"]))
+        kw.setdefault("vmax", EXT_VMAX)
+    return oracle.run_table(p, d["phi"], d["psi"], TABLE_STEPS, d["xy_in"], d["tcc"], d["tc"], d["tgamma"],
+                            table[0], table[1], tmp=d["tmp"], **kw)
+
+
+def used(nn, nb):
+    """table entries with the unused slots blanked"""
+    return np.where(nn[..., None] > np.arange(4), nb, -1)
+
+
+@pytest.mark.parametrize("case", table_cases(), ids=lambda c: c[0])
+def test_oracle_table_matches_reference_fixture(oracle, tbl, case):
+    """init_laplacian_coords (sub_wave.c:11555-11892) and evolve_wave_half_new
+    (wave_3d.c:345-597) with compute_laplacian: the oracle's table equals the
+    reference's, and its steps equal the reference's bit for bit."""
+    tag, nx, ny, bc = case[:4]
+    p = oracle.make_params(nx, ny, bcond=bc)
+    nn, nb = oracle.laplacian_table(p, tuple(tbl[tag + "/lsh"]))
+    assert np.array_equal(nn, tbl[tag + "/nneighb"])
+    assert np.array_equal(used(nn, nb), used(tbl[tag + "/nneighb"], tbl[tag + "/nghb"]))
+    phi, psi, tmp = table_run(oracle, case, tbl)
+    assert bits_equal(phi, tbl[tag + "/phi"])
+    assert bits_equal(psi, tbl[tag + "/psi"])
+    assert bits_equal(tmp, tbl[tag + "/tmp"])
+
+
+def test_table_fixture_detects_planted_mistakes(oracle, tbl):
+    """The fixtures fail an oracle that sums edge neighbours in another
+    order, drops the L-shape wrap, uses the plain stepper's bulk rule
+    (xy_in > 0 instead of != 0, direct-order sums) or drops the drive."""
+    cases = {c[0]: c for c in table_cases()}
+    # edge neighbours summed in reverse order
+    case = cases["tbl_24x17_bc0_ts0"]
+    tag = case[0]
+    nn, nb = oracle.laplacian_table(oracle.make_params(24, 17, bcond=0))
+    rb = nb.copy()
+    edge = np.zeros((24, 17), bool)
+    edge[0, :] = edge[-1, :] = edge[:, 0] = edge[:, -1] = True
+    for (i, j) in np.argwhere(edge):
+        rb[i, j, :nn[i, j]] = nb[i, j, :nn[i, j]][::-1]
+    assert not bits_equal(table_run(oracle, case, tbl, table=(nn, rb))[0], tbl[tag + "/phi"])
+    # L-shape without its wrap (the plain bulk table), and with the direct-order bulk
+    case = cases["tbl_24x17_bc10_ts0"]
+    tag = case[0]
+    plain = oracle.laplacian_table(oracle.make_params(24, 17, bcond=2))
+    assert not bits_equal(table_run(oracle, case, tbl, table=plain)[0], tbl[tag + "/phi"])
+    assert not bits_equal(table_run(oracle, case, tbl, bcond=4)[0], tbl[tag + "/phi"])
+    # the plain stepper on the same inputs
+    case = cases["tbl_16x29_bc2_ts0"]
+    tag = case[0]
+    d = table_inputs(tag, 16, 29, 0, 0)
+    p = oracle.make_params(16, 29, bcond=2, **KAPPAS)
+    phi, _, _ = oracle.run(p, d["phi"], d["psi"], TABLE_STEPS, d["xy_in"], d["tcc"], d["tc"], d["tgamma"],
+                           tmp=d["tmp"])
+    assert not bits_equal(phi, tbl[tag + "/phi"])
+    # no drive
+    case = cases["tbl_16x29_bc2_ts0_ext"]
+    assert not bits_equal(table_run(oracle, case, tbl, drive=None)[0], tbl[case[0] + "/phi"])
+
+
+@pytest.mark.skipif(not __import__("pyoracle").ref_available(), reason="oracle/_ref not built (needs /root/reference)")
+def test_oracle_table_matches_reference_build_random(oracle):
+    """Direct cross-check of the neighbour-table stepper against oracle/_ref
+    on fresh random inputs (TWOSPEEDS off: see table_inputs)."""
+    import pyoracle
+    rng = np.random.default_rng(8)
+    for (tag, nx, ny, bc, ts, tb, ex) in table_cases():
+        if ts or ex:
+            continue
+        R = pyoracle.RefBuild(tag)
+        xy = rng.choice(np.array([-1, 0, 1, 2], np.int16), size=(nx, ny))
+        t = [rng.uniform(0, 0.25, (nx, ny)) for _ in range(3)]
+        s = [rng.standard_normal((nx, ny)) for _ in range(3)]
+        ref = R.run(s[0].copy(), s[1].copy(), s[2].copy(), 5, xy, t[0], t[1], t[2])
+        p = oracle.make_params(nx, ny, bcond=bc, oscillate_topbot=tb, tb_shift=nx, **KAPPAS)
+        nn, nb = oracle.laplacian_table(p, R.lshape_lines())
+        got = oracle.run_table(p, s[0], s[1], 5, xy, t[0], t[1], t[2], nn, nb, tmp=s[2])
+        for a, b in zip(ref, got):
+            assert bits_equal(a, b), tag
 
 
 def test_c1_plain_workload_digests(oracle):
