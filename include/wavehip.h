@@ -39,6 +39,7 @@ extern "C" {
 #define WV_BC_ABSORBING   2
 #define WV_BC_VPER_HABS   3
 #define WV_BC_ABS_REFLECT 4  /* wave_3d.c; equals ABSORBING without COMPARISON */
+#define WV_BC_LSHAPE     10  /* global_pdes.c:276, neighbour-table stepper only */
 
 /* storage / arithmetic form */
 #define WV_FORM_FP64      0  /* binary64 leapfrog, bit-identical to the reference */
@@ -128,6 +129,23 @@ int wv_set_drive(wv_ctx *c, int n, const int *i, const int *j, const double *val
  * is clamped to [-vmax, vmax] (with fused steps the cells no step writes hold
  * +0); vmax = 0 turns it off. */
 int wv_set_floor(wv_ctx *c, double vmax);
+
+/* Neighbour-table stepping, the reference's PRECOMPUTE_BC path:
+ * evolve_wave_half_new (wave_3d.c:345-597) with the t_laplacian table built
+ * by init_laplacian_coords (sub_wave.c:11555-11892) and read by
+ * compute_laplacian (sub_wave.c:11895-11963).  nneighb[k] and nghb[4k+q]
+ * (k = i*NY + j) are a table entry's count and the flat indices of its
+ * neighbours in the field (the reference's pointers minus the field's base).
+ * Where the stepper uses them, entries are summed as -n*x + neighbour 0 + ...
+ * in table order: edge cells whose B_COND branch takes the bulk formula, and
+ * for BC_LSHAPE (genus-2 wrap of the L-shaped billiard) every bulk cell (four
+ * neighbours); other bulk cells keep (((E+W)+N)+S)-4x, and entries the
+ * stepper never reads are not looked at.  The table stepper also updates
+ * bulk cells with xy_in != 0 (not only > 0), has no edge formula for
+ * BC_LSHAPE and no Neumann points; driven cells, FLOOR and point sources
+ * apply as with the plain stepper.  FP64 form, 5-point, one step per launch.
+ * nneighb = NULL returns to the plain stepper (not for BC_LSHAPE). */
+int wv_set_laplacian(wv_ctx *c, const int *nneighb, const int *nghb);
 
 int wv_step(wv_ctx *c, long nsteps);
 /* wv_step bracketed by HIP events on the context stream; waits; *ms is the

@@ -30,6 +30,17 @@
  * row 0 when NX > NY and is then left out.  FLOOR (wave_billiard.c:592-609)
  * clamps to VMAX.
  *
+ * PRECOMPUTE_BC (with WAVEHIP_FLAT, wave_3d.c:173): evolve_wave steps with
+ * evolve_wave_half_new (wave_3d.c:345-597), which this header also defines,
+ * on the neighbour tables the program built with init_laplacian_coords
+ * (sub_wave.c:11555): laplace for phi, laplace1 for tmp, laplace2 for psi,
+ * as wave_3d.c:1187-1189 builds them.  They are handed to the library as
+ * indices (wv_set_laplacian) on the first call with WAVEHIP_TABLES_FIXED, on
+ * every call without it.  B_COND may then be BC_LSHAPE.  The left-edge drive
+ * follows that stepper: cells between bc_jmin and bc_jmax (BEAM_BC) and the
+ * corners (0, 0), (0, NY-1) at AMPLITUDE*cos(time*OMEGA), wave_3d.c:358-376,
+ * 575-579.
+ *
  * Not provided (the program aborts with a message on the first call):
  * COMPARISON (wave_3d.c:805-976).
  *
@@ -40,10 +51,18 @@
 #ifndef WAVEHIP_COMPAT_H
 #define WAVEHIP_COMPAT_H
 
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 
 #include "wavehip.h"
+
+/* the neighbour-table stepper of wave_3d.c (PRECOMPUTE_BC) */
+#if defined(WAVEHIP_FLAT) && defined(PRECOMPUTE_BC)
+#define WAVEHIP_TABLE_PATH PRECOMPUTE_BC
+#else
+#define WAVEHIP_TABLE_PATH 0
+#endif
 
 #ifndef E_SCALE
 #define WAVEHIP_E_SCALE 1.0
@@ -79,7 +98,7 @@ static wv_ctx *wavehip_ctx(void)
         p.oscillate_topbot = OSCILLATE_TOPBOT;
         p.e_scale = WAVEHIP_E_SCALE;
         p.tb_shift = (int)((XMAX - XMIN) * (double)NX / (XMAX - XMIN));
-#ifdef WAVEHIP_FLAT
+#if defined(WAVEHIP_FLAT) && !WAVEHIP_TABLE_PATH
         if (p.tb_shift >= NX - 1) p.tb_shift = NY - 2;
 #endif
 #ifdef WAVEHIP_TABLES_FIXED
@@ -142,12 +161,58 @@ static void wavehip_neumann(const t_neumann_bc *nb)
 }
 #endif
 
+#if WAVEHIP_TABLE_PATH
+/* the program's neighbour tables as counts and flat indices: laplace was
+ * built for phi, laplace1 for tmp, laplace2 for psi (wave_3d.c:1187-1189);
+ * the three must describe the same neighbours */
+static void wavehip_laplacian(const t_laplacian *lap, const double *base, const t_laplacian *lap1,
+                              const double *base1, const t_laplacian *lap2, const double *base2)
+{
+    static int done = 0;
+    static int *nn = NULL, *nb = NULL;
+    long k;
+    int q, rc;
+#ifdef WAVEHIP_TABLES_FIXED
+    if (done) return;
+#endif
+    if (!nn) {
+        nn = (int *)malloc(sizeof(int) * (size_t)NX * NY);
+        nb = (int *)malloc(sizeof(int) * 4 * (size_t)NX * NY);
+        if (!nn || !nb) wavehip_die(wavehip_ctx(), "malloc", -1);
+    }
+    for (k = 0; k < (long)NX * NY; k++) {
+        const int n = lap[k].nneighb;
+        nn[k] = n;
+        for (q = 0; q < 4; q++) {
+            /* entries the stepper reads point into the field; others may not */
+            const double *a = q < n && n <= 4 ? lap[k].nghb[q] : NULL;
+            const long o = a ? (long)(a - base) : -1;
+            nb[4 * k + q] = (o >= 0 && o < (long)NX * NY) ? (int)o : -1;
+            if (lap1 && a) {
+                const double *b = lap1[k].nghb[q], *c = lap2[k].nghb[q];
+                if (lap1[k].nneighb != n || lap2[k].nneighb != n || b - base1 != o || c - base2 != o) {
+                    fprintf(stderr, "wavehip: laplace, laplace1, laplace2 differ at cell %ld\n", k);
+                    abort();
+                }
+            }
+        }
+    }
+    rc = wv_set_laplacian(wavehip_ctx(), nn, nb);
+    if (rc) wavehip_die(wavehip_ctx(), "wv_set_laplacian", rc);
+    done = 1;
+    (void)done;
+}
+#endif
+
 #if OSCILLATE_LEFT
 /* the driven cells of the next nsteps steps, values oscillating_bc(time, j):
  * row-pointer program: (0, j) for bc_jmin < j < bc_jmax, 0 < j < NY-1
  * (wave_billiard.c:320-341, 397), then the corners (0, 0) and (0, NX-1)
  * unless BEAM_BC (585-590); flat program: (0, j) for 0 < j < NY-1 and the
- * corners (0, 0), (0, NY-1) (wave_3d.c:651-654, 989-993) */
+ * corners (0, 0), (0, NY-1) (wave_3d.c:651-654, 989-993); neighbour-table
+ * stepper: (0, j) for bc_jmin < j < bc_jmax, 0 < j < NY-1, then the corners
+ * (0, 0), (0, NY-1) at AMPLITUDE*cos(time*OMEGA) (wave_3d.c:358-376, 413,
+ * 575-579) */
 static void wavehip_drive(int nsteps)
 {
     static long time = 0;  /* the reference's step counter, wave_billiard.c:310,315 */
@@ -156,7 +221,19 @@ static void wavehip_drive(int nsteps)
     int q, s, rc, bc_jmin = 0, bc_jmax = NY - 1, ij[2];
     int ii[NY + 2];
     if (ncell < 0) {
-#ifdef WAVEHIP_FLAT
+#if WAVEHIP_TABLE_PATH
+        if (BEAM_BC) {
+            xy_to_ij(0.0, OSCIL_YMID + OSCIL_YMAX, ij);
+            bc_jmax = ij[1];
+            xy_to_ij(0.0, OSCIL_YMID - OSCIL_YMAX, ij);
+            bc_jmin = ij[1];
+        }
+        ncell = 0;
+        for (q = 1; q < NY - 1; q++)
+            if (q > bc_jmin && q < bc_jmax) cells[ncell++] = q;
+        cells[ncell++] = 0;
+        cells[ncell++] = NY - 1;
+#elif defined(WAVEHIP_FLAT)
         ncell = 0;
         for (q = 1; q < NY - 1; q++) cells[ncell++] = q;
         cells[ncell++] = 0;
@@ -180,7 +257,15 @@ static void wavehip_drive(int nsteps)
     }
     for (q = 0; q < ncell; q++) {
         ii[q] = 0;
-        for (s = 0; s < nsteps; s++) vals[q * nsteps + s] = oscillating_bc((int)(time + s + 1), cells[q]);
+        for (s = 0; s < nsteps; s++) {
+#if WAVEHIP_TABLE_PATH
+            if (q >= ncell - 2) {
+                vals[q * nsteps + s] = AMPLITUDE * cos((double)(time + s + 1) * OMEGA);
+                continue;
+            }
+#endif
+            vals[q * nsteps + s] = oscillating_bc((int)(time + s + 1), cells[q]);
+        }
     }
     time += nsteps;
     rc = wv_set_drive(wavehip_ctx(), ncell, ii, cells, vals, nsteps);
@@ -229,6 +314,10 @@ static void evolve_wave_half(double phi_in[NX * NY], double psi_in[NX * NY], dou
                              double tgamma[NX * NY], t_neumann_bc neumann_bc[N_NEUMANN_POINTS])
 {
     int rc;
+#if WAVEHIP_TABLE_PATH
+    fprintf(stderr, "wavehip: with PRECOMPUTE_BC the program steps with evolve_wave_half_new\n");
+    abort();
+#endif
 #if BC_NEUMANN
     wavehip_neumann(neumann_bc);
 #endif
@@ -240,7 +329,24 @@ static void evolve_wave_half(double phi_in[NX * NY], double psi_in[NX * NY], dou
     if (rc) wavehip_die(wavehip_ctx(), "evolve_wave_half", rc);
 }
 
-/* wave_3d.c:1010 (PRECOMPUTE_BC 0: the t_laplacian tables are not read) */
+#if WAVEHIP_TABLE_PATH
+/* wave_3d.c:345 */
+static void evolve_wave_half_new(double phi_in[NX * NY], double psi_in[NX * NY], double phi_out[NX * NY],
+                                 short int xy_in[NX * NY], double tc[NX * NY], double tcc[NX * NY],
+                                 double tgamma[NX * NY], t_laplacian laplace[NX * NY])
+{
+    int rc;
+    wavehip_laplacian(laplace, phi_in, NULL, NULL, NULL, NULL);
+#if OSCILLATE_LEFT
+    wavehip_drive(1);
+#endif
+    rc = wv_evolve_wave_half_flat(wavehip_ctx(), phi_in, psi_in, phi_out, xy_in, tc, tcc, tgamma);
+    if (rc) wavehip_die(wavehip_ctx(), "evolve_wave_half_new", rc);
+}
+#endif
+
+/* wave_3d.c:1010 (PRECOMPUTE_BC 0: the t_laplacian tables are not read;
+ * PRECOMPUTE_BC 1: the three steps of evolve_wave_half_new) */
 static void evolve_wave(double phi[NX * NY], double psi[NX * NY], double tmp[NX * NY],
                         short int xy_in[NX * NY], double tc[NX * NY], double tcc[NX * NY],
                         double tgamma[NX * NY], t_laplacian laplace[NX * NY],
@@ -248,7 +354,9 @@ static void evolve_wave(double phi[NX * NY], double psi[NX * NY], double tmp[NX 
                         t_neumann_bc neumann_bc[N_NEUMANN_POINTS])
 {
     int rc;
-#if BC_NEUMANN
+#if WAVEHIP_TABLE_PATH
+    wavehip_laplacian(laplace, phi, laplace1, tmp, laplace2, psi);
+#elif BC_NEUMANN
     wavehip_neumann(neumann_bc);
 #endif
 #if OSCILLATE_LEFT

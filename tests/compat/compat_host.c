@@ -6,21 +6,30 @@
  *
  * The parameter macros come from the compile line, as they come from a
  * reference program's #define block.  The two types a reference program takes
- * from global_pdes.c (t_neumann_bc, t_laplacian) are declared in minimal form:
- * the stepper never reads them.
+ * from global_pdes.c (t_neumann_bc, t_laplacian) are declared with the
+ * fields the steppers read.  With PRECOMPUTE_BC (flat) the program steps
+ * through evolve_wave with neighbour tables for phi, tmp and psi, built from
+ * the count/index table in the input as init_laplacian_coords would build
+ * them (sub_wave.c:11555).
  *
  *   compat_host IN OUT NCALLS MODE
  *   IN    phi, psi, tmp, tcc, tc, tgamma (NX*NY doubles each, k = i*NY + j)
  *         followed by xy_in (NX*NY shorts), then int32 n and n Neumann points
  *         (int32 i, j, nbi1, nbj1, nbi2, nbj2; doubles nx, ny), then int32 T
- *         and T*NY doubles: oscillating_bc(time, j) for time 1..T
+ *         and T*NY doubles: oscillating_bc(time, j) for time 1..T; with
+ *         PRECOMPUTE_BC then NX*NY int32 counts and NX*NY*4 int32 flat
+ *         neighbour indices
  *   MODE  "wave": NCALLS calls of evolve_wave (3 steps each);
- *         "half": NCALLS calls of evolve_wave_half(phi, psi, tmp), the host
- *         rotating (phi, psi, tmp) <- (tmp, phi, psi) after each
- *   OUT   phi, psi, tmp after the calls, same layout as IN
+ *         "half": NCALLS calls of evolve_wave_half(phi, psi, tmp) (with
+ *         PRECOMPUTE_BC evolve_wave_half_new), the host rotating
+ *         (phi, psi, tmp) <- (tmp, phi, psi) after each
+ *   OUT   phi, psi, tmp after the calls, same layout as IN; with
+ *         PRECOMPUTE_BC and OSCILLATE_LEFT then T doubles
+ *         AMPLITUDE*cos(time*OMEGA), time 1..T (the corner drive)
  *
  *   compat_host --params   prints the macros it was built with
  */
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -43,6 +52,11 @@
 #define OSCIL_YMID 0.0
 #define OSCIL_YMAX 1.0
 #define N_NEUMANN_POINTS 4096
+#define AMPLITUDE 0.8
+#define OMEGA 0.05
+#ifndef PRECOMPUTE_BC
+#define PRECOMPUTE_BC 0
+#endif
 
 /* the fields of global_pdes.c:518-528 the stepper reads */
 typedef struct {
@@ -54,6 +68,7 @@ typedef struct {
 } t_neumann_bc;
 typedef struct {
     int nneighb;
+    double *nghb[4];
 } t_laplacian;
 
 int n_neumann_points = 0;
@@ -106,10 +121,10 @@ int main(int argc, char **argv)
 {
     if (argc == 2 && strcmp(argv[1], "--params") == 0) {
         printf("nx=%d ny=%d bcond=%d twospeeds=%d oscillate_topbot=%d flat=%d fixed=%d neumann=%d "
-               "oscillate_left=%d floor=%d vmax=%.17g "
+               "oscillate_left=%d floor=%d vmax=%.17g table=%d "
                "kappa=%.17g kappa_sides=%.17g gamma_sides=%.17g kappa_topbot=%.17g gamma_topbot=%.17g\n",
                NX, NY, B_COND, TWOSPEEDS, OSCILLATE_TOPBOT, HOST_FLAT, HOST_FIXED, BC_NEUMANN, OSCILLATE_LEFT,
-               FLOOR, (double)VMAX, KAPPA, KAPPA_SIDES, GAMMA_SIDES, KAPPA_TOPBOT, GAMMA_TOPBOT);
+               FLOOR, (double)VMAX, PRECOMPUTE_BC, KAPPA, KAPPA_SIDES, GAMMA_SIDES, KAPPA_TOPBOT, GAMMA_TOPBOT);
         return 0;
     }
     if (argc != 5) {
@@ -146,6 +161,12 @@ int main(int argc, char **argv)
     if (drive_steps < 0) return 2;
     drive_table = malloc(sizeof(double) * (size_t)(drive_steps > 0 ? drive_steps : 1) * NY);
     read_all(f, drive_table, sizeof(double), (size_t)drive_steps * NY);
+#if PRECOMPUTE_BC
+    int *tnn = malloc(sizeof(int) * NC), *tnb = malloc(sizeof(int) * 4 * NC);
+    if (!tnn || !tnb) return 2;
+    read_all(f, tnn, sizeof(int), NC);
+    read_all(f, tnb, sizeof(int), 4 * NC);
+#endif
     fclose(f);
 
 #ifndef WAVEHIP_FLAT
@@ -183,23 +204,51 @@ int main(int argc, char **argv)
     fclose(g);
 #else
     /* flat tables; evolve_wave_half of wave_3d.c takes (tc, tcc, tgamma) */
-    static t_laplacian lap[1];
     double *phi = flat[0], *psi = flat[1], *tmp = flat[2];
+#if PRECOMPUTE_BC
+    /* one table per buffer, as wave_3d.c:1187-1189: laplace for phi,
+     * laplace1 for tmp, laplace2 for psi */
+    t_laplacian *lap[3];
+    double *lbase[3] = {phi, tmp, psi};
+    for (int a = 0; a < 3; a++) {
+        lap[a] = calloc(NC, sizeof(t_laplacian));
+        if (!lap[a]) return 2;
+        for (size_t k = 0; k < NC; k++) {
+            lap[a][k].nneighb = tnn[k];
+            for (int q = 0; q < 4; q++) lap[a][k].nghb[q] = tnb[4 * k + q] >= 0 ? lbase[a] + tnb[4 * k + q] : NULL;
+        }
+    }
+#else
+    static t_laplacian lap1[1];
+    t_laplacian *lap[3] = {lap1, lap1, lap1};
+#endif
     for (int n = 0; n < ncalls; n++) {
         if (half) {
+#if PRECOMPUTE_BC
+            /* the table built for the buffer that is phi_in this call */
+            int a = phi == lbase[0] ? 0 : (phi == lbase[1] ? 1 : 2);
+            evolve_wave_half_new(phi, psi, tmp, xyf, flat[4], flat[3], flat[5], lap[a]);
+#else
             evolve_wave_half(phi, psi, tmp, xyf, flat[4], flat[3], flat[5], nbc);
+#endif
             double *t = psi;
             psi = phi;
             phi = tmp;
             tmp = t;
         } else {
-            evolve_wave(phi, psi, tmp, xyf, flat[4], flat[3], flat[5], lap, lap, lap, nbc);
+            evolve_wave(phi, psi, tmp, xyf, flat[4], flat[3], flat[5], lap[0], lap[1], lap[2], nbc);
         }
     }
     double *outs[3] = {phi, psi, tmp};
     FILE *g = fopen(argv[2], "wb");
     if (!g) return 2;
     for (int a = 0; a < 3; a++) write_all(g, outs[a], sizeof(double), NC);
+#if PRECOMPUTE_BC && OSCILLATE_LEFT
+    for (int t = 1; t <= drive_steps; t++) {
+        const double c = AMPLITUDE * cos((double)t * OMEGA);
+        write_all(g, &c, sizeof(double), 1);
+    }
+#endif
     fclose(g);
 #endif
     wv_destroy(wavehip_ctx());

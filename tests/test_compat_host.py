@@ -18,7 +18,7 @@ from conftest import say
 HERE = os.path.dirname(os.path.abspath(__file__))
 BIN = os.path.join(HERE, "compat", "bin")
 VARIANTS = ["rows_abs", "rows_per", "rows_dir_tb", "flat_vper", "flat_abs_fixed", "rows_ext", "rows_ext_wide",
-            "flat_ext"]
+            "flat_ext", "flat_tbl_dir", "flat_tbl_lshape"]
 
 
 def host_params(name):
@@ -36,13 +36,13 @@ def host_params(name):
 @pytest.mark.parametrize("name", VARIANTS)
 def test_host_program_built(name):
     d = host_params(name)
-    assert d["nx"] >= 3 and d["ny"] >= 3 and 0 <= d["bcond"] <= 4
+    assert d["nx"] >= 3 and d["ny"] >= 3 and (0 <= d["bcond"] <= 4 or d["bcond"] == 10)
 
 
 def oracle_params(oracle, d):
     tbs = d["nx"]                    # wave_billiard.c:319 with XMIN..XMAX the whole box
-    if d["flat"] and tbs >= d["nx"] - 1:
-        tbs = d["ny"] - 2            # wave_3d.c:613-617
+    if d["flat"] and not d["table"] and tbs >= d["nx"] - 1:
+        tbs = d["ny"] - 2            # wave_3d.c:613-617 (the table stepper keeps NX, 387)
     return oracle.make_params(d["nx"], d["ny"], bcond=d["bcond"], twospeeds=d["twospeeds"],
                               oscillate_topbot=d["oscillate_topbot"], tb_shift=tbs,
                               kappa=d["kappa"], kappa_sides=d["kappa_sides"],
@@ -53,7 +53,7 @@ def oracle_params(oracle, d):
 def drive_cells(d):
     """the cells the reference drives: wave_billiard.c:336-338, 397, 585-590
     (row pointers; the corner (0, NX-1) only when it lies in row 0) or
-    wave_3d.c:651-654, 989-993 (flat)"""
+    wave_3d.c:651-654, 989-993 (flat) or 358-376, 413, 575-579 (table stepper)"""
     nx, ny = d["nx"], d["ny"]
     cells = [(0, j) for j in range(1, ny - 1)]
     if d["flat"]:
@@ -65,6 +65,16 @@ def oracle_expect(oracle, d, w, ncalls, mode):
     p = oracle_params(oracle, d)
     phi, psi, tmp = (np.array(w[k], dtype=np.float64, copy=True) for k in ("phi", "psi", "tmp"))
     xy, tcc, tc, tg = w["xy_in"], w["tcc"], w["tc"], w["tgamma"]
+    if d["table"]:
+        n = 3 * ncalls if mode == "wave" else ncalls
+        kw = {}
+        if d["oscillate_left"]:
+            cells = drive_cells(d)
+            val = np.array([w["table"][:n, j] for (_, j) in cells[:-2]] + [w["corner"][:n]] * 2)
+            kw["drive"] = dict(i=[c[0] for c in cells], j=[c[1] for c in cells], val=val)
+        if d["floor"]:
+            kw["vmax"] = d["vmax"]
+        return oracle.run_table(p, phi, psi, n, xy, tcc, tc, tg, w["lap"][0], w["lap"][1], tmp=tmp, **kw)
     if d["neumann"] or d["oscillate_left"] or d["floor"]:
         n = 3 * ncalls if mode == "wave" else ncalls
         kw = {}
@@ -121,6 +131,9 @@ def test_host_program_matches_oracle(oracle, tmp_path, name, mode, ncalls):
             w[k] = w[k] * 0.5
     rng = np.random.default_rng(11)
     w["nb"] = neumann_points(w["xy_in"], rng, 40) if d["neumann"] else None
+    if d["table"]:
+        lsh = workloads.lshape_lines(d["nx"], d["ny"], 1.5, -2.0, 2.0, -1.125, 1.125) if d["bcond"] == 10 else None
+        w["lap"] = workloads.laplacian_table(d["nx"], d["ny"], d["bcond"], lsh)
     T = 3 * ncalls
     w["table"] = np.cos(0.4 * np.arange(1, T + 1)[:, None] + 0.05 * np.arange(d["ny"])[None, :])
     inp, outp = tmp_path / "in.bin", tmp_path / "out.bin"
@@ -137,10 +150,15 @@ def test_host_program_matches_oracle(oracle, tmp_path, name, mode, ncalls):
             f.write(np.ascontiguousarray(nb["ny"], np.float64).tobytes())
         f.write(np.int32(T).tobytes())
         f.write(np.ascontiguousarray(w["table"], np.float64).tobytes())
+        if d["table"]:
+            f.write(np.ascontiguousarray(w["lap"][0], np.int32).tobytes())
+            f.write(np.ascontiguousarray(w["lap"][1], np.int32).tobytes())
     r = subprocess.run([os.path.join(BIN, name), str(inp), str(outp), str(ncalls), mode],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    got = np.fromfile(outp, dtype=np.float64).reshape(3, d["nx"], d["ny"])
+    raw = np.fromfile(outp, dtype=np.float64)
+    got = raw[:3 * d["nx"] * d["ny"]].reshape(3, d["nx"], d["ny"])
+    w["corner"] = raw[3 * d["nx"] * d["ny"]:]  # the table stepper's corner drive, computed by the host
     exp = oracle_expect(oracle, d, w, ncalls, mode)
     for a, b, what in zip(got, exp, ("phi", "psi", "tmp")):
         assert np.array_equal(a.view(np.uint64), np.ascontiguousarray(b).view(np.uint64)), \

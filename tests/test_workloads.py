@@ -57,3 +57,36 @@ def test_workload_shapes_small():
     assert np.all(w["phi"][w["xy_in"] == 0] == 0)
     ws = workloads.c4_instances(count=4, nmax=200)
     assert len(ws) == 4 and all(64 <= w["params"]["nx"] <= 200 for w in ws)
+
+
+def test_laplacian_table_matches_reference_tables():
+    """workloads.laplacian_table equals the table init_laplacian_coords builds
+    (fixtures exported from the reference builds), and the oracle's builder at
+    sizes the fixtures do not cover."""
+    import os
+    import pyoracle
+    from gen_inputs import table_cases
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "table_cases.npz"))
+    for (tag, nx, ny, bc) in {c[:4] for c in table_cases()}:
+        if bc > 3 and bc != 10:
+            continue  # no table edges for B_COND 4
+        nn, nb = workloads.laplacian_table(nx, ny, bc, tuple(g[tag + "/lsh"]))
+        rn, rb = g[tag + "/nneighb"], g[tag + "/nghb"]
+        used = lambda n, b: np.where(n[..., None] > np.arange(4), b, -1)  # noqa: E731
+        assert np.array_equal(nn, rn) and np.array_equal(used(nn, nb), used(rn, rb)), tag
+    for nx, ny in ((131, 67), (64, 200)):
+        lsh = workloads.lshape_lines(nx, ny, 1.5, -2.0, 2.0, -1.125, 1.125)
+        for bc in (0, 1, 2, 3, 10):
+            a = pyoracle.laplacian_table(pyoracle.make_params(nx, ny, bcond=bc), lsh)
+            b = workloads.laplacian_table(nx, ny, bc, lsh)
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (nx, ny, bc)
+
+
+def test_lshape_mask_and_lines():
+    x, y = workloads.grid_xy(24, 17, -2.0, 2.0, -1.125, 1.125)
+    m = workloads.mask_lshape(x, y, 1.5)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    inside = (np.abs(X) <= 1.5) & (np.abs(Y) <= 1.0) & ~((X > 0) & (Y > 0))
+    assert np.array_equal(m == 1, inside)
+    # the seams of the 24x17 reference build (ref_lshape_lines)
+    assert workloads.lshape_lines(24, 17, 1.5, -2.0, 2.0, -1.125, 1.125) == (4, 1, 11, 7, 20, 15)

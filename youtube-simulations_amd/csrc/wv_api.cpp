@@ -55,6 +55,13 @@ struct Grid {
     std::vector<int> h_ni, h_nj, h_di, h_dj;
     // cells a step writes: h_act plus the Neumann points and driven cells
     std::vector<uint64_t> h_wr;
+    // neighbour-table stepping (evolve_wave_half_new): 0 off, 1 on, 2 BC_LSHAPE;
+    // the cells with a table Laplacian (CSR by row) and the active plane of
+    // the plain stepper (h_act holds the one in use)
+    int tbl = 0;
+    int *tbl_ptr = nullptr, *tbl_col = nullptr, *tbl_n = nullptr;
+    long long *tbl_off = nullptr;
+    std::vector<uint64_t> h_act_plain;
 };
 
 struct Engine {
@@ -125,7 +132,10 @@ int check_params(Engine *e, const wv_params *p)
     if (!p) return fail(e, WV_EINVAL, "params is NULL");
     if (p->nx < 3 || p->ny < 3) return fail(e, WV_EINVAL, "nx, ny must be >= 3 (got %d, %d)", p->nx, p->ny);
     if ((long)p->nx * p->ny > (1L << 31)) return fail(e, WV_EINVAL, "grid too large");
-    if (p->bcond < 0 || p->bcond > 4) return fail(e, WV_EINVAL, "bcond %d is not a B_COND code 0..4", p->bcond);
+    if ((p->bcond < 0 || p->bcond > 4) && p->bcond != WV_BC_LSHAPE)
+        return fail(e, WV_EINVAL, "bcond %d is not a B_COND code 0..4 or 10", p->bcond);
+    if (p->bcond == WV_BC_LSHAPE && (p->form != WV_FORM_FP64 || p->stencil != 5))
+        return fail(e, WV_EUNSUPPORTED, "BC_LSHAPE is provided for the FP64 form and the 5-point stencil");
     if (p->stencil != 5 && p->stencil != 9) return fail(e, WV_EINVAL, "stencil must be 5 or 9");
     if (p->form != WV_FORM_FP64 && p->form != WV_FORM_F32SPLIT) return fail(e, WV_EINVAL, "unknown form %d", p->form);
     if (p->tb_depth < 0 || p->tb_depth > 8 || (p->tb_depth && !wv_have_depth(p->tb_depth)))
@@ -170,7 +180,8 @@ void free_grid(Grid &G)
     void *ptrs[] = {G.u[0], G.u[1], G.u[2], G.u[3], G.hi[0], G.hi[1], G.lo[0], G.lo[1], G.v[0], G.v[1],
                     G.act, G.eact, G.med, G.tccf, G.tgf, G.tc, G.tccf_hi, G.tccf_lo, G.tgf32,
                     G.src_ptr, G.src_col, G.src_id, G.amp, G.nz, G.neu_ptr, G.neu_col, G.neu_code,
-                    G.neu_w, G.drv_ptr, G.drv_col, G.drv_id, G.drv_val};
+                    G.neu_w, G.drv_ptr, G.drv_col, G.drv_id, G.drv_val, G.tbl_ptr, G.tbl_col, G.tbl_n,
+                    G.tbl_off};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
 }
@@ -190,6 +201,8 @@ WvInst make_inst(const Engine *e, const Grid &G)
     d.drv_len = G.drv_len; d.drv_base = G.drv_base;
     d.nz = G.nz;
     d.vmax = G.vmax;
+    d.tbl = G.tbl;
+    d.tbl_ptr = G.tbl_ptr; d.tbl_col = G.tbl_col; d.tbl_n = G.tbl_n; d.tbl_off = G.tbl_off;
     d.nx = G.p.nx; d.ny = G.p.ny; d.ldy = G.ldy; d.ldw = G.ldw;
     d.bc = G.p.bcond; d.osc_tb = G.p.oscillate_topbot; d.tb_shift = G.p.tb_shift; d.coef = G.coef;
     d.rows_per_tile = rows(e);
@@ -261,7 +274,8 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
         const int nx = G.p.nx, ny = G.p.ny;
         const int js = cb * wout - wv_halo(k);
         const int ibeg = sb * L, iend = std::min(ibeg + L, nx);
-        const bool interior = js >= 1 && js + 64 * WV_C <= ny - 1 && ibeg >= k && iend <= nx - k;
+        // (BC_LSHAPE: every bulk cell sums its Laplacian in table order)
+        const bool interior = js >= 1 && js + 64 * WV_C <= ny - 1 && ibeg >= k && iend <= nx - k && G.tbl != 2;
         // (wrapped halos: any source, Neumann point or driven cell of the grid)
         const int bflags = (G.h_si.empty() ? 0 : WV_TILE_SRC) | (G.h_ni.empty() ? 0 : WV_TILE_NEU) |
                            (G.h_di.empty() ? 0 : WV_TILE_DRV);
@@ -433,6 +447,25 @@ bool masked_zero(const Grid &G, const double *a)
     return true;
 }
 
+// Active plane of the neighbour-table stepper (wave_3d.c:390-572): bulk and
+// edge cells with TWOSPEEDS or xy_in != 0 (the plain stepper's bulk rule is
+// xy_in > 0); BC_LSHAPE has no edge formula, so of its edge cells only those
+// of the OSCILLATE_TOPBOT branch are written.
+std::vector<uint64_t> table_act(const Grid &G)
+{
+    const int nx = G.p.nx, ny = G.p.ny, ldw = G.ldw;
+    std::vector<uint64_t> a((size_t)nx * ldw, 0);
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++) {
+            bool on = G.p.twospeeds || bit(G.h_nz, ldw, i, j);
+            const bool edge = i == 0 || i == nx - 1 || j == 0 || j == ny - 1;
+            if (on && edge && G.tbl == 2)
+                on = (j == 0 || j == ny - 1) && G.p.oscillate_topbot && i < G.p.tb_shift && i < nx - 1 && i > 0;
+            if (on) a[(size_t)i * ldw + (j >> 6)] |= 1ull << (j & 63);
+        }
+    return a;
+}
+
 int set_tables(Engine *e, int k, const short *xy, const double *tcc, const double *tc, const double *tg)
 {
     if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
@@ -543,7 +576,10 @@ int set_tables(Engine *e, int k, const short *xy, const double *tcc, const doubl
     }
     // a state set under another mask may hold non-zero values on cells that
     // are now never written: keep it, but step it one step per launch
+    G.h_act_plain = act;
+    if (G.tbl) act = table_act(G);
     if (G.state && G.h_act != act) G.exact = true;
+    HIPCHK(e, hipMemcpy(G.act, act.data(), wb, hipMemcpyHostToDevice));
     G.h_act.swap(act);
     rebuild_written(G);
     G.tables = true;
@@ -697,12 +733,15 @@ int set_sources(Engine *e, int k, int n, const int *si, const int *sj, const dou
     return sync_inst(e);
 }
 
-// rows of a CSR list by row: ptr[nx+1], and the order of the entries
-void csr_by_row(int nx, int n, const int *row, std::vector<int> &ptr, std::vector<int> &order)
+// rows of a CSR list by row: ptr[nx+1], and the order of the entries (by
+// row, then column, entries of one cell in list order: the kernels search a
+// row's columns and take the last entry of a cell)
+void csr_by_row(int nx, int n, const int *row, const int *col, std::vector<int> &ptr, std::vector<int> &order)
 {
     order.resize(n);
     for (int q = 0; q < n; q++) order[q] = q;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return row[a] < row[b]; });
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return row[a] != row[b] ? row[a] < row[b] : col[a] < col[b]; });
     ptr.assign(nx + 1, 0);
     for (int q = 0; q < n; q++) ptr[row[q] + 1]++;
     for (int i = 0; i < nx; i++) ptr[i + 1] += ptr[i];
@@ -723,6 +762,8 @@ int set_neumann(Engine *e, int k, int n, const int *pi, const int *pj, const int
 {
     if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
     Grid &G = e->g[k];
+    if (G.tbl && n > 0)
+        return fail(e, WV_EUNSUPPORTED, "the neighbour-table stepper has no Neumann points (wave_3d.c:345-597)");
     if (e->form != WV_FORM_FP64 || WV_C != 1)
         return fail(e, WV_EUNSUPPORTED, "Neumann ghost points are provided for the FP64 form");
     if (n < 0) return fail(e, WV_EINVAL, "negative point count");
@@ -746,7 +787,7 @@ int set_neumann(Engine *e, int k, int n, const int *pi, const int *pj, const int
     HIPCHK(e, hipSetDevice(e->device));
     HIPCHK(e, hipStreamSynchronize(e->stream));
     std::vector<int> ptr, order;
-    csr_by_row(nx, n, pi, ptr, order);
+    csr_by_row(nx, n, pi, pj, ptr, order);
     std::vector<int> col(n), code(n);
     std::vector<double> wv(2 * (size_t)n);
     for (int q = 0; q < n; q++) {
@@ -791,7 +832,7 @@ int set_drive(Engine *e, int k, int n, const int *pi, const int *pj, const doubl
     int rc;
     if (!same) {
         std::vector<int> ptr, order;
-        csr_by_row(nx, n, pi, ptr, order);
+        csr_by_row(nx, n, pi, pj, ptr, order);
         std::vector<int> col(n), id(n);
         for (int q = 0; q < n; q++) { col[q] = pj[order[q]]; id[q] = order[q]; }
         if (n == 0) ptr.clear();
@@ -826,6 +867,96 @@ int set_floor(Engine *e, int k, double vmax)
     return sync_inst(e);
 }
 
+// Does the neighbour-table stepper use the Laplacian of edge cell (i, j)?
+// wave_3d.c:404-572: the branches that take the bulk formula.
+bool table_edge_uses_delta(const wv_params &p, int i, int j)
+{
+    const int nx = p.nx, ny = p.ny, bc = p.bcond;
+    if (j == 0 || j == ny - 1) {
+        if (p.oscillate_topbot && i < p.tb_shift && i < nx - 1 && i > 0) return true;
+        return bc == WV_BC_DIRICHLET || bc == WV_BC_PERIODIC || (bc == WV_BC_VPER_HABS && i != 0);
+    }
+    return bc == WV_BC_DIRICHLET || bc == WV_BC_PERIODIC;
+}
+
+int set_laplacian(Engine *e, int k, const int *nn, const int *nb)
+{
+    if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
+    Grid &G = e->g[k];
+    if (e->form != WV_FORM_FP64 || e->stencil != 5 || WV_C != 1)
+        return fail(e, WV_EUNSUPPORTED, "the neighbour-table stepper is provided for the FP64 form, 5-point");
+    if (nn && !nb) return fail(e, WV_EINVAL, "neighbour index table is NULL");
+    if (nn && !G.h_ni.empty())
+        return fail(e, WV_EUNSUPPORTED, "the neighbour-table stepper has no Neumann points (wave_3d.c:345-597)");
+    const int nx = G.p.nx, ny = G.p.ny;
+    const long ncell = (long)nx * ny;
+    const bool lshape = G.p.bcond == WV_BC_LSHAPE;
+    if (!nn && lshape) return fail(e, WV_EINVAL, "BC_LSHAPE steps with its neighbour table");
+    // the cells whose Laplacian is not the plain 5-point one, in row-major
+    // order (so already sorted by row, then column)
+    std::vector<int> ptr, col, cnt;
+    std::vector<long long> off;
+    if (nn) {
+        ptr.assign(nx + 1, 0);
+        for (int i = 0; i < nx; i++)
+            for (int j = 0; j < ny; j++) {
+                const long c = (long)i * ny + j;
+                const bool edge = i == 0 || i == nx - 1 || j == 0 || j == ny - 1;
+                int n;
+                if (edge) {
+                    if (!table_edge_uses_delta(G.p, i, j)) continue;
+                    n = nn[c];
+                    if (n < 0 || n > 4)
+                        return fail(e, WV_EINVAL, "cell (%d,%d): %d neighbours (0..4)", i, j, n);
+                } else {
+                    if (!lshape) continue;  // compute_laplacian's direct bulk formula
+                    const long reg[4] = {c + ny, c - ny, c + 1, c - 1};
+                    if (nb[4 * c] == reg[0] && nb[4 * c + 1] == reg[1] && nb[4 * c + 2] == reg[2] &&
+                        nb[4 * c + 3] == reg[3])
+                        continue;  // E, W, N, S: summed in the kernel from registers
+                    n = 4;
+                }
+                for (int m = 0; m < 4; m++) {
+                    long long o = 0;
+                    if (m < n) {
+                        const long q = nb[4 * c + m];
+                        if (q < 0 || q >= ncell)
+                            return fail(e, WV_EINVAL, "cell (%d,%d): neighbour %d index %ld outside the grid", i, j,
+                                        m, q);
+                        o = (long long)(q / ny) * G.ldy + q % ny;
+                    }
+                    off.push_back(o);
+                }
+                col.push_back(j);
+                cnt.push_back(n);
+                ptr[i + 1]++;
+            }
+        for (int i = 0; i < nx; i++) ptr[i + 1] += ptr[i];
+    }
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    int rc;
+    if (col.empty()) ptr.clear();
+    if ((rc = upload_vec(e, &G.tbl_ptr, ptr)) || (rc = upload_vec(e, &G.tbl_col, col)) ||
+        (rc = upload_vec(e, &G.tbl_n, cnt)) || (rc = upload_vec(e, &G.tbl_off, off)))
+        return rc;
+    // no entries at all (e.g. absorbing edges): still the table stepper's rules
+    if (nn && !G.tbl_ptr) {
+        const std::vector<int> zero(nx + 1, 0);
+        if ((rc = upload_vec(e, &G.tbl_ptr, zero))) return rc;
+    }
+    G.tbl = !nn ? 0 : (lshape ? 2 : 1);
+    if (G.tables) {
+        std::vector<uint64_t> act = G.tbl ? table_act(G) : G.h_act_plain;
+        HIPCHK(e, hipMemcpy(G.act, act.data(), act.size() * 8, hipMemcpyHostToDevice));
+        if (G.state && act != G.h_act) G.exact = true;
+        G.h_act.swap(act);
+        rebuild_written(G);
+    }
+    drop_tiles(e);
+    return sync_inst(e);
+}
+
 bool any_exact(const Engine *e)
 {
     for (const auto &G : e->g)
@@ -838,13 +969,24 @@ int ready(Engine *e)
     for (int k = 0; k < e->n; k++) {
         if (!e->g[k].tables) return fail(e, WV_ESTATE, "grid %d has no tables", k);
         if (!e->g[k].state) return fail(e, WV_ESTATE, "grid %d has no state", k);
+        if (e->g[k].p.bcond == WV_BC_LSHAPE && !e->g[k].tbl)
+            return fail(e, WV_ESTATE, "grid %d: BC_LSHAPE needs its neighbour table (wv_set_laplacian)", k);
     }
     return WV_OK;
 }
 
+bool any_table(const Engine *e)
+{
+    for (const auto &G : e->g)
+        if (G.tbl) return true;
+    return false;
+}
+
+// the neighbour-table Laplacian reads its neighbours from the step's input
+// field, so it steps one step per launch (with the exact 3-buffer rotation)
 int max_depth(const Engine *e, bool force_k1)
 {
-    return (force_k1 || (e->form == WV_FORM_FP64 && any_exact(e))) ? 1 : depth(e);
+    return (force_k1 || (e->form == WV_FORM_FP64 && (any_exact(e) || any_table(e)))) ? 1 : depth(e);
 }
 
 // depth of the next launch when `left` steps remain
@@ -895,6 +1037,7 @@ int enqueue_steps(Engine *e, long nsteps, bool force_k1, long *launches)
         extras = extras || !G.h_ni.empty() || !G.h_di.empty() || G.vmax > 0;
     }
     if (extras && e->form == WV_FORM_FP64) cm = WV_CM_ALLX;  // the variant with those paths
+    if (e->form == WV_FORM_FP64 && any_table(e)) cm = WV_CM_TBL;
     while (done < nsteps) {
         const int k = next_depth(kmax, nsteps - done);
         Engine::Tiles t;
@@ -1486,6 +1629,12 @@ extern "C" int wv_set_floor(wv_ctx *c, double vmax)
 {
     if (!c) return WV_EINVAL;
     return set_floor(c->e, 0, vmax);
+}
+
+extern "C" int wv_set_laplacian(wv_ctx *c, const int *nneighb, const int *nghb)
+{
+    if (!c) return WV_EINVAL;
+    return set_laplacian(c->e, 0, nneighb, nghb);
 }
 
 extern "C" int wv_autotune(wv_ctx *c, long steps_per_trial, double *best_gcells)

@@ -305,12 +305,8 @@ __device__ __forceinline__ T add_sources(const Src &q, int rm, int jm, long g, T
     return v;
 }
 
-// u + w for u = h + l (double-float): the increment w takes the low word first
-// (t = l + w, rounded at the increment's own binary32 precision), then one
-// Fast2Sum with h, exact whenever |h| >= |t| -- the normal case for a per-step
-// increment; near a zero crossing of u its error stays at the increment's
-// rounding level
-// Neumann ghost points, driven cells and the FLOOR clamp of one grid
+// Neumann ghost points, driven cells, the FLOOR clamp and the table
+// Laplacians of one grid
 struct Xtra {
     const int *nptr, *ncol, *ncode;
     const double *nw;
@@ -320,41 +316,99 @@ struct Xtra {
     const uint64_t *nz;
     int ldw;
     double vmax;
+    const int *tptr, *tcol, *tn;
+    const long long *toff;
     __device__ __forceinline__ explicit Xtra(const WvInst &d)
         : nptr(d.neu_ptr), ncol(d.neu_col), ncode(d.neu_code), nw(d.neu_w), dptr(d.drv_ptr),
           dcol(d.drv_col), did(d.drv_id), dval(d.drv_val), dlen(d.drv_len), dbase(d.drv_base), nz(d.nz),
-          ldw(d.ldw), vmax(d.vmax)
+          ldw(d.ldw), vmax(d.vmax), tptr(d.tbl_ptr), tcol(d.tbl_col), tn(d.tbl_n), toff(d.tbl_off)
     {
     }
 };
 
+// last entry of cell (rm, jm) in a CSR-by-row list whose columns ascend
+// within a row (entries of one cell in list order), or -1: binary search (a
+// row along an edge can hold thousands of entries)
+__device__ inline int row_find(const int *ptr, const int *col, int rm, int jm)
+{
+    const int beg = gc(ptr)[rm];
+    int lo = beg, hi = gc(ptr)[rm + 1];
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (gc(col)[mid] <= jm) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo > beg && gc(col)[lo - 1] == jm) ? lo - 1 : -1;
+}
+
 // Neumann point at (rm, jm): its packed neighbour offsets and weights
 __device__ inline bool neu_find(const Xtra &q, int rm, int jm, int &code, double &w1, double &w2)
 {
-    const int k0 = gc(q.nptr)[rm], k1 = gc(q.nptr)[rm + 1];
-    bool found = false;
-    for (int k = k0; k < k1; k++)
-        if (gc(q.ncol)[k] == jm) {
-            found = true;
-            code = gc(q.ncode)[k];
-            w1 = gc(q.nw)[2 * k];
-            w2 = gc(q.nw)[2 * k + 1];
-        }
-    return found;
+    const int k = row_find(q.nptr, q.ncol, rm, jm);
+    if (k < 0) return false;
+    code = gc(q.ncode)[k];
+    w1 = gc(q.nw)[2 * k];
+    w2 = gc(q.nw)[2 * k + 1];
+    return true;
 }
 
 // value of a driven cell (rm, jm) at amplitude index g, if it is driven then
 __device__ inline bool drv_find(const Xtra &q, int rm, int jm, long g, double &val)
 {
     if (g < 0 || g >= q.dlen) return false;
-    const int k0 = gc(q.dptr)[rm], k1 = gc(q.dptr)[rm + 1];
-    bool found = false;
-    for (int k = k0; k < k1; k++)
-        if (gc(q.dcol)[k] == jm) {
-            found = true;
-            val = gc(q.dval)[(size_t)gc(q.did)[k] * q.dlen + g];
+    const int k = row_find(q.dptr, q.dcol, rm, jm);
+    if (k < 0) return false;
+    val = gc(q.dval)[(size_t)gc(q.did)[k] * q.dlen + g];
+    return true;
+}
+
+// Table Laplacian of cell (rm, jm), sub_wave.c:11908-11962: -n*x, then the
+// n neighbours of its entry added in table order, read from the step's input
+// field u0 (one step per launch).  n = nfix if non-zero (the bulk of
+// BC_LSHAPE always adds four).  False if the cell has no entry.
+__device__ inline bool tbl_delta(const Xtra &q, const double *u0, int rm, int jm, double x, int nfix,
+                                 double &delta)
+{
+    const int k = row_find(q.tptr, q.tcol, rm, jm);
+    if (k < 0) return false;
+    const int n = nfix ? nfix : gc(q.tn)[k];
+    double d = -(double)n * x;
+    for (int m = 0; m < n; m++) d = d + gc(u0)[gc(q.toff)[4 * k + m]];
+    delta = d;
+    return true;
+}
+
+// Edge cell of the neighbour-table stepper, wave_3d.c:404-572: the formulas
+// of each B_COND branch with the table Laplacian `delta` where a branch uses
+// one (no edge formula for BC_LSHAPE: those cells are not written).
+__device__ inline double edge_table_fp64(const Par &d, int rm, int jm, double x, double y, double E,
+                                         double Wv, double N, double S, double delta, double cc,
+                                         double g, double tcv)
+{
+    const int nx = d.nx, ny = d.ny, bc = d.bc;
+    if (jm == ny - 1 || jm == 0) {
+        // top (j = NY-1) and bottom (j = 0) rows, corners included
+        if (d.osc_tb && rm < d.tb_shift && rm < nx - 1 && rm > 0) return leap(x, y, delta, cc, d.kappa, g);
+        switch (bc) {
+        case 0:
+        case 1: return leap(x, y, delta, cc, d.kappa, g);
+        case 3:
+            if (rm == 0) return absorb(x, y, E, tcv, d.ks, d.gs);
+            return leap(x, y, delta, cc, d.kappa, g);
+        case 2:
+        case 4: return absorb(x, y, jm == ny - 1 ? S : N, tcv, d.kt, d.gt);
+        default: return x;
         }
-    return found;
+    }
+    // left (i = 0) and right (i = NX-1) columns, 1 <= j <= NY-2
+    switch (bc) {
+    case 0:
+    case 1: return leap(x, y, delta, cc, d.kappa, g);
+    case 2:
+    case 3:
+    case 4: return absorb(x, y, rm == 0 ? E : Wv, tcv, d.ks, d.gs);
+    default: return x;
+    }
 }
 
 // one of the 3x3 values around a cell: code = (row offset + 1) | (column offset + 1) << 2;
@@ -369,6 +423,11 @@ __device__ __forceinline__ double pick9(int code, double wS, double w, double wN
     return a == 0 ? r0 : (a == 1 ? r1 : r2);
 }
 
+// u + w for u = h + l (double-float): the increment w takes the low word first
+// (t = l + w, rounded at the increment's own binary32 precision), then one
+// Fast2Sum with h, exact whenever |h| >= |t| -- the normal case for a per-step
+// increment; near a zero crossing of u its error stays at the increment's
+// rounding level
 __device__ __forceinline__ void dd_add_f(float h, float l, float w, float &ho, float &lo)
 {
     const float t = l + w;

@@ -17,6 +17,8 @@
 //   tc         per-cell tc (read at edge cells only).
 //   sources    CSR by row: src_ptr[NX+1], src_col[], src_id[]; amplitudes
 //              amp[id*amp_len + g].
+//   Neumann points, driven cells, table Laplacians: CSR by row, columns
+//              ascending within a row (binary search).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -31,6 +33,9 @@
 #define WV_CM_ALL 3
 // any mixture, plus Neumann ghost points, driven cells and FLOOR (FP64 form)
 #define WV_CM_ALLX 4
+// WV_CM_ALLX plus the neighbour-table Laplacian (FP64 form, one step per
+// launch, 5-point)
+#define WV_CM_TBL 5
 
 // rows in flight in a stepping kernel's prefetch ring (3 or 6)
 #ifndef WV_RING
@@ -70,6 +75,12 @@ struct WvInst {
     long drv_len, drv_base;
     const uint64_t *nz;
     double vmax;
+    // neighbour-table stepping (evolve_wave_half_new): tbl = 0 off, 1 on,
+    // 2 BC_LSHAPE.  Cells whose Laplacian is not the plain one, CSR by row:
+    // column, neighbour count, four neighbour offsets i*ldy + j.
+    int tbl;
+    const int *tbl_ptr, *tbl_col, *tbl_n;
+    const long long *tbl_off;
     double tcc0, tcc1, tg0, tg1;         // coefficient pairs (modes UNIFORM / TWO)
     float tcch0, tccl0, tcch1, tccl1, tgf0, tgf1;
     double kappa, ks, gs, kt, gt, es2;
@@ -81,6 +92,7 @@ struct WvInst {
 #define WV_TILE_SRC 4       // a point source lies in the cells the tile computes
 #define WV_TILE_NEU 8       // ... a Neumann ghost point (never an INTERIOR tile)
 #define WV_TILE_DRV 16      // ... a driven cell (never an INTERIOR tile)
+#define WV_TILE_TBL 32      // ... a cell with a table Laplacian (never an INTERIOR tile)
 struct WvTile {
     int inst, cb, sb, flags;
 };

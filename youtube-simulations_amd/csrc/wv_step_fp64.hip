@@ -44,7 +44,10 @@ constexpr int CF_ANY = 4;      // per-cell tables read in place (boundary tiles)
 // tile reads is written by a step, so no active-bit plane is read.  DAMP:
 // bulk cells evaluate -KAPPA*x - tgamma*(x-y); without it they use
 // leap_undamped(), which the host selects only where that is bit-identical.
-template <int K, int ST, bool INTERIOR, int CF, bool ALLACT, bool DAMP, bool XT = false>
+// XT: Neumann ghost points, driven cells and FLOOR on boundary tiles.  TB
+// (with XT, K = 1): the neighbour-table Laplacian of evolve_wave_half_new on
+// boundary tiles of grids that use it.
+template <int K, int ST, bool INTERIOR, int CF, bool ALLACT, bool DAMP, bool XT = false, bool TB = false>
 __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long gbase, int ia, int ib,
                                          int oc, int od)
 {
@@ -73,6 +76,8 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
     const bool xneu = XT && !INTERIOR && WV_C == 1 && (t.flags & WV_TILE_NEU);
     const bool xdrv = XT && !INTERIOR && (t.flags & WV_TILE_DRV);
     const bool xfloor = XT && !INTERIOR && xq.vmax > 0.0;
+    static_assert(!TB || (XT && K == 1 && ST == 5), "table Laplacians: 5-point, one step per launch");
+    const int tblm = TB && !INTERIOR ? d.tbl : 0;  // 1: table Laplacians, 2: BC_LSHAPE
     const double *u0 = d.u[ia], *um1 = d.u[ib];
     const uint64_t *act = d.act, *med = d.med;
     const double *tcp = d.tc, *tccf = d.tccf, *tgf = d.tgf;
@@ -179,13 +184,30 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
                             const double s4 = ((E + Wv) + N) + S;
                             const double sd = ((NE + NW) + SE) + SW;
                             delta = ((4.0 * s4 + sd) - 20.0 * x) * W9;
+                        } else if (TB && tblm == 2) {
+                            // BC_LSHAPE bulk, sub_wave.c:11901-11912: -4x, then the four
+                            // table neighbours (E, W, N, S unless the cell lies on a seam)
+                            if (!tbl_delta(xq, u0, rm, L.jm[c], x, 4, delta)) {
+                                delta = -4.0 * x;
+                                delta = delta + E;
+                                delta = delta + Wv;
+                                delta = delta + N;
+                                delta = delta + S;
+                            }
                         } else {
                             delta = (((E + Wv) + N) + S) - 4.0 * x;
                         }
                         v = DAMP ? leap(x, y, delta, cc, p.kappa, g) : leap_undamped(x, y, delta, cc);
                     } else {
                         const double tcv = gc(tcp)[(size_t)rm * ldy + L.jm[c]];
-                        v = edge_fp64(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv);
+                        if (TB && tblm) {
+                            // an edge formula that uses no Laplacian has no table entry
+                            double delta = 0.0;
+                            (void)tbl_delta(xq, u0, rm, L.jm[c], x, 0, delta);
+                            v = edge_table_fp64(p, rm, L.jm[c], x, y, E, Wv, N, S, delta, cc, g, tcv);
+                        } else {
+                            v = edge_fp64(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv);
+                        }
                     }
                     bool w = ALLACT || ((ab >> c) & 1u);
                     if (XT && !INTERIOR) {
@@ -321,19 +343,20 @@ k_step_fp64(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, cons
         else WV_RUN(CF, false, false);             \
     }
     // the modes this kernel is not built for are folded away
-    constexpr bool XT = CM == WV_CM_ALLX;
+    constexpr bool TB = CM == WV_CM_TBL;
+    constexpr bool XT = CM == WV_CM_ALLX || TB;
     const int coef = CM >= WV_CM_ALL ? d.coef : CM;
     switch (coef) {
     case WV_COEF_UNIFORM:
-        if (!interior) run_fp64<K, ST, false, CF_UNIFORM, false, true, XT>(d, t, gbase, ia, ib, oc, od);
+        if (!interior) run_fp64<K, ST, false, CF_UNIFORM, false, true, XT, TB>(d, t, gbase, ia, ib, oc, od);
         else WV_INTERIOR(CF_UNIFORM)
         break;
     case WV_COEF_TWO:
-        if (!interior) run_fp64<K, ST, false, CF_TWO, false, true, XT>(d, t, gbase, ia, ib, oc, od);
+        if (!interior) run_fp64<K, ST, false, CF_TWO, false, true, XT, TB>(d, t, gbase, ia, ib, oc, od);
         else WV_INTERIOR(CF_TWO)
         break;
     default:
-        if (!interior) run_fp64<K, ST, false, CF_ANY, false, true, XT>(d, t, gbase, ia, ib, oc, od);
+        if (!interior) run_fp64<K, ST, false, CF_ANY, false, true, XT, TB>(d, t, gbase, ia, ib, oc, od);
         else if (d.tgf) {
             // a tgamma field implies damping
             if (allact) WV_RUN(CF_FIELDG, true, true);
@@ -360,12 +383,21 @@ int WV_CAT(wv_launch_fp64_k, WV_K)(int stencil, int cm, const WvInst *I, const W
     if (!n) return 0;
     const dim3 grid(n), block(64);
 #define WV_GO(ST, CM) hipLaunchKernelGGL((k_step_fp64<WV_K, ST, CM>), grid, block, 0, s, I, T, gs, g0, ia, ib, oc, od)
+    // the table variant exists for one step per launch and the 5-point stencil only
+#if WV_K == 1
+#define WV_TBL(ST)                         \
+    if (ST == 5) WV_GO(5, WV_CM_TBL);      \
+    else return (int)hipErrorInvalidValue;
+#else
+#define WV_TBL(ST) return (int)hipErrorInvalidValue;
+#endif
 #define WV_ST(ST)                                 \
     switch (cm) {                                 \
     case WV_COEF_UNIFORM: WV_GO(ST, WV_COEF_UNIFORM); break; \
     case WV_COEF_TWO: WV_GO(ST, WV_COEF_TWO); break;         \
     case WV_COEF_FIELD: WV_GO(ST, WV_COEF_FIELD); break;     \
     case WV_CM_ALLX: WV_GO(ST, WV_CM_ALLX); break;           \
+    case WV_CM_TBL: WV_TBL(ST); break;                       \
     default: WV_GO(ST, WV_CM_ALL); break;                     \
     }
     if (stencil == 9) {
@@ -374,6 +406,7 @@ int WV_CAT(wv_launch_fp64_k, WV_K)(int stencil, int cm, const WvInst *I, const W
         WV_ST(5)
     }
 #undef WV_ST
+#undef WV_TBL
 #undef WV_GO
     return (int)hipGetLastError();
 }

@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "libwavehip.so")
 
 WV_OK, WV_EINVAL, WV_EHIP, WV_ESTATE, WV_EUNSUPPORTED, WV_ENOMEM = 0, -1, -2, -3, -4, -5
 BC_DIRICHLET, BC_PERIODIC, BC_ABSORBING, BC_VPER_HABS, BC_ABS_REFLECT = 0, 1, 2, 3, 4
+BC_LSHAPE = 10
 FORM_FP64, FORM_F32SPLIT = 0, 1
 FLAG_TABLES_FIXED = 1
 FLAG_FULL_FORMULA = 2
@@ -27,7 +28,7 @@ FLAG_FULL_FORMULA = 2
 EXPORTS = [
     "wv_params_default", "wv_create", "wv_destroy", "wv_last_error", "wv_set_tables",
     "wv_set_state", "wv_set_state3", "wv_get_state", "wv_get_state3", "wv_set_sources",
-    "wv_set_neumann", "wv_set_drive", "wv_set_floor",
+    "wv_set_neumann", "wv_set_drive", "wv_set_floor", "wv_set_laplacian",
     "wv_step", "wv_step_timed", "wv_sync", "wv_steps_done", "wv_last_kernel_ms", "wv_energy",
     "wv_get_tiling",
     "wv_set_tiling", "wv_autotune", "wv_evolve_wave_half_rows", "wv_evolve_wave_rows",
@@ -89,6 +90,7 @@ def load():
         "wv_set_neumann": (ctypes.c_int, [vp, ctypes.c_int, _I, _I, _I, _I, _I, _I, _D, _D]),
         "wv_set_drive": (ctypes.c_int, [vp, ctypes.c_int, _I, _I, _D, ctypes.c_long]),
         "wv_set_floor": (ctypes.c_int, [vp, ctypes.c_double]),
+        "wv_set_laplacian": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "wv_step": (ctypes.c_int, [vp, ctypes.c_long]),
         "wv_step_timed": (ctypes.c_int, [vp, ctypes.c_long, _D, ctypes.POINTER(ctypes.c_long)]),
         "wv_sync": (ctypes.c_int, [vp]),
@@ -237,6 +239,18 @@ class Wave:
 
     def set_floor(self, vmax):
         self._chk(self._L.wv_set_floor(self._h, float(vmax)), "wv_set_floor")
+
+    def set_laplacian(self, table):
+        """neighbour-table stepping: table = (nneighb[NX,NY], nghb[NX,NY,4]) of
+        flat indices i*NY + j, or None for the plain stepper"""
+        if table is None:
+            self._chk(self._L.wv_set_laplacian(self._h, None, None), "wv_set_laplacian")
+            return
+        I = ctypes.POINTER(ctypes.c_int)
+        nn = np.ascontiguousarray(table[0], np.int32)
+        nb = np.ascontiguousarray(table[1], np.int32)
+        self._chk(self._L.wv_set_laplacian(self._h, nn.ctypes.data_as(I), nb.ctypes.data_as(I)),
+                  "wv_set_laplacian")
 
     def load(self, w):
         """tables, state and sources of a workload dict"""

@@ -85,6 +85,77 @@ def mask_discs(x, y, discs):
     return m
 
 
+def mask_lshape(x, y, lam):
+    """D_LSHAPE, sub_wave.c:6415-6421: |x| <= LAMBDA, |y| <= 1, not x > 0 and y > 0."""
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    out = (np.abs(X) > lam) | (np.abs(Y) > 1.0) | ((X > 0.0) & (Y > 0.0))
+    return (~out).astype(np.int16)
+
+
+def lshape_lines(nx, ny, lam, xmin, xmax, ymin, ymax):
+    """(i1, j1, i2, j2, i3, j3) of the L-shape seams, sub_wave.c:11780-11785."""
+    i1, j1 = xy_to_ij(-lam, -1.0, nx, ny, xmin, xmax, ymin, ymax)
+    i2, j2 = xy_to_ij(0.0, 0.0, nx, ny, xmin, xmax, ymin, ymax)
+    i3, j3 = xy_to_ij(lam, 1.0, nx, ny, xmin, xmax, ymin, ymax)
+    return i1 + 1, j1 + 1, i2 - 1, j2 - 1, i3 - 1, j3 - 1
+
+
+def laplacian_table(nx, ny, bcond, lsh=None):
+    """The neighbour table of init_laplacian_coords (sub_wave.c:11555-11892)
+    as (nneighb[nx,ny], nghb[nx,ny,4]) flat indices i*ny + j (-1: unset), for
+    B_COND 0-3 and BC_LSHAPE (10, seams lsh from lshape_lines); entries are
+    written in the reference's order, so a cell set twice keeps the later one."""
+    nn = np.zeros((nx, ny), np.int32)
+    nb = np.full((nx, ny, 4), -1, np.int32)
+
+    def put(i, j, n, pts):
+        i = np.atleast_1d(i)
+        j = np.atleast_1d(j)
+        nn[i, j] = n
+        for q, (a, b) in enumerate(pts):
+            nb[i, j, q] = np.atleast_1d(a) * ny + np.atleast_1d(b)
+
+    I, J = np.meshgrid(np.arange(1, nx - 1), np.arange(1, ny - 1), indexing="ij")
+    put(I, J, 4, [(I + 1, J), (I - 1, J), (I, J + 1), (I, J - 1)])
+    if 0 <= bcond <= 3:
+        per = bcond == 1
+        j = np.arange(1, ny - 1)
+        z, r = np.zeros_like(j), np.full_like(j, nx - 1)
+        if per:
+            put(z, j, 4, [(z + 1, j), (r, j), (z, j + 1), (z, j - 1)])
+            put(r, j, 4, [(r - 1, j), (z, j), (r, j + 1), (r, j - 1)])
+        else:
+            put(z, j, 3, [(z + 1, j), (z, j + 1), (z, j - 1)])
+            put(r, j, 3, [(r - 1, j), (r, j + 1), (r, j - 1)])
+        i = np.arange(nx)
+        ip = (i + 1) % nx if per else np.minimum(i + 1, nx - 1)
+        im = (i - 1) % nx if per else np.maximum(i - 1, 0)
+        top, bot = np.full_like(i, ny - 1), np.zeros_like(i)
+        if per or bcond == 3:
+            put(i, top, 4, [(ip, top), (im, top), (i, top - 1), (i, bot)])
+            put(i, bot, 4, [(ip, bot), (im, bot), (i, bot + 1), (i, top)])
+        else:
+            put(i, top, 3, [(ip, top), (im, top), (i, top - 1)])
+            put(i, bot, 3, [(ip, bot), (im, bot), (i, bot + 1)])
+    elif bcond == 10:
+        i1, j1, i2, j2, i3, j3 = lsh
+        ja, jb = np.arange(j1 + 1, j2), np.arange(j2, j3 - 1)
+        ia, ib = np.arange(i1, i2), np.arange(i2, i3)
+        c = np.full_like
+        put(c(ja, i1), ja, 4, [(c(ja, i1 + 1), ja), (c(ja, i3), ja), (c(ja, i1), ja + 1), (c(ja, i1), ja - 1)])
+        put(c(jb, i1), jb, 4, [(c(jb, i1 + 1), jb), (c(jb, i2 - 1), jb), (c(jb, i1), jb + 1), (c(jb, i1), jb - 1)])
+        put(c(ja, i3), ja, 4, [(c(ja, i1), ja), (c(ja, i3 - 1), ja), (c(ja, i3), ja + 1), (c(ja, i3), ja - 1)])
+        put(c(jb, i2), jb, 4, [(c(jb, i1), jb), (c(jb, i2 - 1), jb), (c(jb, i2), jb + 1), (c(jb, i2), jb - 1)])
+        put(ia, c(ia, j3), 4, [(ia + 1, c(ia, j3)), (ia - 1, c(ia, j3)), (ia, c(ia, j1)), (ia, c(ia, j3 - 1))])
+        put(ib, c(ib, j2), 4, [(ib + 1, c(ib, j2)), (ib - 1, c(ib, j2)), (ib, c(ib, j1)), (ib, c(ib, j2 - 1))])
+        put(ia, c(ia, j1), 4, [(ia + 1, c(ia, j1)), (ia - 1, c(ia, j1)), (ia, c(ia, j1 + 1)), (ia, c(ia, j3))])
+        put(ib, c(ib, j1), 4, [(ib + 1, c(ib, j1)), (ib - 1, c(ib, j1)), (ib, c(ib, j1 + 1)), (ib, c(ib, j2))])
+        put(i1, j1, 4, [(i1 + 1, j1), (i3 - 1, j1), (i1, j1 + 1), (i1, j3 - 1)])
+        put(i3, j1, 4, [(i1, j1), (i3 - 1, j1), (i3, j1 + 1), (i3, j3)])
+        put(i1, j3, 4, [(i1 + 1, j3), (i3, j3), (i1, j1), (i1, j3 - 1)])
+    return nn, nb
+
+
 def constant_media_tables(xy_in, courant, courantb, gamma, gammab, twospeeds):
     """init_ior_2d constant-media branch, sub_wave.c:12983-13003.  Cells the
     reference leaves uninitialised (xy_in == 0 and not TWOSPEEDS) get 0."""
