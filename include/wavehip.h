@@ -165,6 +165,54 @@ int wv_last_kernel_ms(const wv_ctx *c, double *avg_ms, double *timed_launches);
  * halving tree, then the same over rows): deterministic run to run. */
 int wv_energy(wv_ctx *c, double *e);
 
+/* Frame colours: what draw_wave_highres_palette (wave_common.c:1748-1847)
+ * hands to OpenGL, computed on the GPU from the field in place.  For every
+ * block cell (i, j) = (bi*size, bj*size): the value and colour of wave_value
+ * (wave_common.c:1053-1171, plot types P_AMPLITUDE 0 .. P_LOG_AVERAGE_ENERGY 9
+ * with their running sums), colour schemes C_LUM 0, C_HUE 1, C_ONEDIM 3,
+ * C_ONEDIM_LINEAR 4, C_BASIC_LINEAR 5 (colors_waves.c:161-290), palettes
+ * COL_JET 0, COL_GRAY 2 and the table palettes COL_TURBO 10 .. COL_PARULA 16,
+ * COL_TWILIGHT 17, COL_TWILIGHT_SHIFTED 18, COL_TURBO_CYCLIC 101
+ * (colormaps.c:2077-2548; the program's own table, 256 or 511 entries, via
+ * wv_frame_set_tables), then SHADE_2D shading (gradient towards blocks
+ * (bi+1, bj) and (bi, bj+1)), FADE_IN_OBSTACLE, fade, and the draw test
+ * TWOSPEEDS || xy_in.  rgba[4*(bi*nby + bj) ..] = (r, g, b, drawn) as the
+ * floats glColor3f receives; nbx = ceil(NX/size), nby = ceil(NY/size).
+ * Computed in FP64 like the reference; tanh, exp, log, atan are the GPU's
+ * (within an ulp or two of glibc's). */
+typedef struct {
+    int plot;              /* PLOT (P_* code)                                     */
+    int scheme;            /* COLOR_SCHEME                                        */
+    int palette;           /* palette argument (COLOR_PALETTE or COLOR_PALETTE_B) */
+    int asym;              /* COLOR_PALETTE >= COL_TURBO: asymmetric energy scheme */
+    int size;              /* block size of draw_wave_highres_palette             */
+    int time;              /* frame time argument                                 */
+    int shade;             /* SHADE_2D                                            */
+    int rescale_center;    /* RESCALE_COLOR_IN_CENTER (needs color_scale)         */
+    int fade_in_obstacle;  /* FADE_IN_OBSTACLE (needs fade_table)                 */
+    int fade;              /* fade argument, with fade_value                      */
+    long nsteps;           /* NSTEPS (C_LUM hue drift)                            */
+    double scale, fade_value;
+    double slope, color_range, attenuation;        /* SLOPE, COLOR_RANGE, ATTENUATION */
+    double vshift_amplitude, vscale_amplitude;     /* VSHIFT_AMPLITUDE, VSCALE_AMPLITUDE */
+    double log_shift, log_scale, flux_scale, avrg_e_factor;
+    double colorhue, colordrift, lummean, lumamp, huemean, hueamp;
+    double shade_scale, light_x, light_y;          /* SHADE_SCALE_2D, light[0..1] */
+    double courant, courantb;                      /* compute_energy's COURANT, COURANTB */
+} wv_frame_params;
+
+/* the values of the checked-in wave_billiard.c:200-241, 287 */
+void wv_frame_params_default(wv_frame_params *f);
+/* palette table (lut_n x 3 floats) or NULL; color_scale and fade_table
+ * (NX*NY, k = i*NY + j) or NULL -- replaces all three */
+int wv_frame_set_tables(wv_ctx *c, const float *lut, int lut_n, const double *color_scale,
+                        const double *fade_table);
+/* zero the running sums of the mean / average / flux plots */
+int wv_frame_reset(wv_ctx *c);
+/* one frame (updates the running sums of its plot type; a change of size
+ * starts them from zero); rgba: 4*nbx*nby floats */
+int wv_frame(wv_ctx *c, const wv_frame_params *f, float *rgba);
+
 /* What wv_step launches: fused steps per launch, rows per tile, tile columns */
 int wv_get_tiling(const wv_ctx *c, int *tb_depth, int *rows_per_tile, int *tile_cols);
 int wv_set_tiling(wv_ctx *c, int tb_depth, int rows_per_tile);

@@ -6,7 +6,9 @@
 # then xy_to_ij (sub_wave.c:412-422), oscillating_bc (sub_wave.c:11965-12162),
 # compute_energy (wave_common.c:453-472) and the stepper (wave_billiard.c:302-631
 # or wave_3d.c:600-1007; for the neighbour-table stepper sub_wave.c:11555-11963
-# and wave_3d.c:345-597), then oracle/ref_harness.c.  Flags follow makefile:1-4
+# and wave_3d.c:345-597; for the frame colours sub_wave.c:270-304, colors_waves.c
+# with the colour tables it includes, wave_common.c:453-512 and 1053-1171),
+# then oracle/ref_harness.c.  Flags follow makefile:1-4
 # (gcc -O3 -fopenmp) plus -ffp-contract=off.  Needs the reference sources at
 # /root/reference; machines without them use the oracle alone.
 set -euo pipefail
@@ -16,7 +18,7 @@ OUT="$HERE/_ref"
 [ -f "$REF/wave_billiard.c" ] || { echo "reference not found at $REF" >&2; exit 1; }
 mkdir -p "$OUT"
 
-build_one() {  # build_one <tag> <0 row pointers | 1 flat | 2 neighbour table> <gcc -D flags...>
+build_one() {  # build_one <tag> <0 row pointers | 1 flat | 2 neighbour table | 3 colours> <gcc -D flags...>
     local tag=$1 flat=$2; shift 2
     local so="$OUT/libref_${tag}.so"
     {
@@ -24,7 +26,13 @@ build_one() {  # build_one <tag> <0 row pointers | 1 flat | 2 neighbour table> <
         echo "#include \"$REF/global_pdes.c\""
         sed -n '412,422p' "$REF/sub_wave.c"
         sed -n '11965,12162p' "$REF/sub_wave.c"
-        if [ "$flat" = 2 ]; then
+        if [ "$flat" = 3 ]; then
+            echo "#define COLOR_TWIN 1"
+            sed -n '270,304p' "$REF/sub_wave.c"
+            echo "#include \"$REF/colors_waves.c\""
+            sed -n '453,512p' "$REF/wave_common.c"
+            sed -n '1053,1171p' "$REF/wave_common.c"
+        elif [ "$flat" = 2 ]; then
             echo "#define TABLE_TWIN 1"
             sed -n '11555,11963p' "$REF/sub_wave.c"
             sed -n '345,597p' "$REF/wave_3d.c"
@@ -96,6 +104,23 @@ build_one "tbl_16x29_bc3_ts0_tb" 2 -DNX=16 -DNY=29 -DB_COND=3 -DTWOSPEEDS=0 -DOS
 pids+=($!)
 build_one "tbl_16x29_bc2_ts0_ext" 2 -DNX=16 -DNY=29 -DB_COND=2 -DTWOSPEEDS=0 -DOSCILLATE_LEFT=1 \
     -DOSCILLATION_SCHEDULE=0 -DFLOOR=1 -DVMAX=0.9 $KS &
+pids+=($!)
+for p in "${pids[@]}"; do wait "$p"; done; pids=()
+# the colour path of the drawing functions: wave_value (wave_common.c:1053)
+# through color_scheme_palette / color_scheme_asym_palette and the palettes
+# of colormaps.c, one build per COLOR_SCHEME (COLOR_PALETTE >= COL_TURBO
+# switches the energy plots to the asymmetric scheme)
+COL="-DNX=24 -DNY=17 -DE_SCALE=50.0 -DCOURANT=0.1 -DCOURANTB=0.05 -DNSTEPS=500 -DCOLORDRIFT=30.0"
+for sc in 0 1 3 4 5; do
+    build_one "col_s${sc}" 3 $COL -DCOLOR_SCHEME=$sc -DTWOSPEEDS=0 &
+    pids+=($!)
+done
+build_one "col_s1_asym" 3 $COL -DCOLOR_SCHEME=1 -DCOLOR_PALETTE=11 -DTWOSPEEDS=1 &
+pids+=($!)
+build_one "col_s3_asym" 3 $COL -DCOLOR_SCHEME=3 -DCOLOR_PALETTE=11 -DTWOSPEEDS=0 &
+pids+=($!)
+build_one "col_s3_att" 3 $COL -DCOLOR_SCHEME=3 -DCOLOR_RANGE=1.0 -DATTENUATION=0.01 \
+    -DRESCALE_COLOR_IN_CENTER=1 -DTWOSPEEDS=1 &
 pids+=($!)
 for p in "${pids[@]}"; do wait "$p"; done; pids=()
 # a lens medium with damping and absorbing edges (energy fixture, two speeds)

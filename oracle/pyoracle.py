@@ -38,6 +38,33 @@ def make_params(nx, ny, bcond=0, twospeeds=0, stencil=5, oscillate_topbot=0, tb_
                      kappa, kappa_sides, gamma_sides, kappa_topbot, gamma_topbot)
 
 
+class OrcFrameParams(ctypes.Structure):
+    """orc_frame_params of wave_oracle.h."""
+    _fields_ = [(n, ctypes.c_int) for n in ("plot", "scheme", "palette", "asym", "size", "time", "shade",
+                                            "rescale_center", "fade_in_obstacle", "fade")] + \
+               [("nsteps", ctypes.c_long)] + \
+               [(n, ctypes.c_double) for n in ("scale", "fade_value", "slope", "color_range", "attenuation",
+                                               "vshift_amplitude", "vscale_amplitude", "log_shift", "log_scale",
+                                               "flux_scale", "avrg_e_factor", "colorhue", "colordrift",
+                                               "lummean", "lumamp", "huemean", "hueamp", "shade_scale",
+                                               "light_x", "light_y", "courant", "courantb", "e_scale")]
+
+
+# the drawing macros of the checked-in wave_billiard.c:200-241, 287
+FRAME_DEFAULTS = dict(plot=0, scheme=3, palette=0, asym=0, size=1, time=0, shade=0, rescale_center=0,
+                      fade_in_obstacle=0, fade=0, nsteps=500, scale=1.0, fade_value=1.0, slope=1.0,
+                      color_range=0.8, attenuation=0.0, vshift_amplitude=-0.2, vscale_amplitude=1.0,
+                      log_shift=0.75, log_scale=0.75, flux_scale=250.0, avrg_e_factor=0.95, colorhue=260.0,
+                      colordrift=0.0, lummean=0.5, lumamp=0.3, huemean=180.0, hueamp=-180.0, shade_scale=0.5,
+                      light_x=0.40824829, light_y=0.816496581, courant=0.1, courantb=0.05, e_scale=300.0)
+
+
+def frame_params(**kw):
+    d = dict(FRAME_DEFAULTS)
+    d.update(kw)
+    return OrcFrameParams(**d)
+
+
 class OrcExtras(ctypes.Structure):
     """orc_extras of wave_oracle.h (Neumann ghost points, driven cells, FLOOR)."""
     _fields_ = [
@@ -90,6 +117,10 @@ def lib():
         L.orc_run_table.argtypes = [P, _D, _D, _D, _I, ctypes.c_long, _S, _D, _D, _D, _I, _I,
                                     ctypes.c_int, _I, _I, _D, ctypes.c_long, ctypes.c_long,
                                     ctypes.POINTER(OrcExtras)]
+        L.orc_frame.argtypes = [P, ctypes.POINTER(OrcFrameParams), _D, _D, _S, _D, _D, _D, _D, _D, _F,
+                                ctypes.c_int, _D, _D, ctypes.POINTER(ctypes.c_ubyte)]
+        L.orc_color.argtypes = [ctypes.POINTER(OrcFrameParams), ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                ctypes.c_double, ctypes.c_int, _F, ctypes.c_int, _D]
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_split_energy_total.restype = ctypes.c_double
         L.orc_split_energy_total.argtypes = [P, _F, _F, _F, _S, _D, ctypes.c_double]
@@ -234,6 +265,33 @@ def run_table(p, phi, psi, nsteps, xy_in, tcc, tc, tgamma, nneighb, nghb, tmp=No
     return b[s], b[(s + 1) % 3], b[(s + 2) % 3]
 
 
+class FrameAccumulators:
+    """the running sums of the mean / average / flux plots"""
+
+    def __init__(self, nx, ny):
+        self.te = np.zeros((nx, ny))
+        self.ae = np.zeros((nx, ny))
+        self.tf = np.zeros(2 * nx * ny)
+
+
+def frame(p, f, phi, psi, xy_in, acc, color_scale=None, fade_table=None, lut=None):
+    """orc_frame: (values, rgb[nx, ny, 3], drawn) at block cells (others 0)"""
+    nx, ny = p.nx, p.ny
+    phi = np.ascontiguousarray(phi, np.float64)
+    psi = np.ascontiguousarray(psi, np.float64)
+    xy = np.ascontiguousarray(xy_in, np.int16)
+    cs = np.ascontiguousarray(np.ones((nx, ny)) if color_scale is None else color_scale, np.float64)
+    ft = np.ascontiguousarray(np.ones((nx, ny)) if fade_table is None else fade_table, np.float64)
+    lt = np.ascontiguousarray(np.zeros((1, 3)) if lut is None else lut, np.float32)
+    values = np.zeros((nx, ny))
+    rgb = np.zeros((nx, ny, 3))
+    drawn = np.zeros((nx, ny), np.uint8)
+    lib().orc_frame(ctypes.byref(p), ctypes.byref(f), _p(phi, _D), _p(psi, _D), _p(xy, _S), _p(acc.te, _D),
+                    _p(acc.ae, _D), _p(acc.tf, _D), _p(cs, _D), _p(ft, _D), _p(lt, _F), lt.shape[0],
+                    _p(values, _D), _p(rgb, _D), drawn.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
+    return values, rgb, drawn.astype(bool)
+
+
 def _sources(sources):
     if not sources:
         z = np.zeros(1, np.int32)
@@ -306,6 +364,14 @@ class RefBuild:
         self.ny = self.L.ref_ny()
         self.flat = hasattr(self.L, "ref_evolve_wave_half_flat")
         self.table = hasattr(self.L, "ref_evolve_wave_half_new")
+        self.color = hasattr(self.L, "ref_wave_value")
+        if self.color:
+            self.L.ref_wave_value.restype = ctypes.c_double
+            self.L.ref_wave_value.argtypes = [_D, _D, _D, _D, _D, _D, _S, ctypes.c_double, ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]
+            self.L.ref_palette_table.argtypes = [ctypes.c_int, _F]
+            self.L.ref_palette_table.restype = ctypes.c_int
+            return
         if self.table:
             self.L.ref_evolve_wave_half_new.argtypes = [_D, _D, _D, _S, _D, _D, _D]
             self.L.ref_laplacian_table.argtypes = [_I, _I]
@@ -333,6 +399,18 @@ class RefBuild:
 
     def oscillating_bc(self, time, j):
         return self.L.ref_oscillating_bc(time, j)
+
+    def wave_value(self, phi, psi, acc, cs, xy_in, scale, time, plot, palette, i, j):
+        """wave_value at (i, j): (value, rgb)"""
+        rgb = np.zeros(3)
+        v = self.L.ref_wave_value(_p(phi, _D), _p(psi, _D), _p(acc.te, _D), _p(acc.ae, _D), _p(acc.tf, _D),
+                                  _p(cs, _D), _p(xy_in, _S), scale, time, plot, palette, i, j, _p(rgb, _D))
+        return v, rgb
+
+    def palette_table(self, palette):
+        out = np.zeros((511, 3), np.float32)
+        n = self.L.ref_palette_table(palette, _p(out, _F))
+        return out[:n] if n else None
 
     def laplacian_table(self):
         """the reference's init_laplacian_coords table as counts and flat indices"""

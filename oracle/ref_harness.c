@@ -34,7 +34,43 @@ int ref_set_neumann(int n, const int *pi, const int *pj, const int *bi1, const i
 double ref_oscillating_bc(int time, int j) { return oscillating_bc(time, j); }
 int ref_beam_bc(void) { return BEAM_BC; }
 
-#if defined(TABLE_TWIN)
+#if defined(COLOR_TWIN)
+/* wave_value (wave_common.c:1053) at (i, j) on flat arrays; total_flux is
+ * flat already (index 2*j*NX + 2*i) */
+double ref_wave_value(double *phi, double *psi, double *te, double *ae, double *tf, double *cs, short *xy,
+                      double scale, int time, int plot, int palette, int i, int j, double *rgb)
+{
+    double *a[NX], *b[NX], *e1[NX], *e2[NX], *c[NX];
+    short int *m[NX];
+    for (int r = 0; r < NX; r++) {
+        a[r] = phi + (size_t)r * NY; b[r] = psi + (size_t)r * NY;
+        e1[r] = te + (size_t)r * NY; e2[r] = ae + (size_t)r * NY; c[r] = cs + (size_t)r * NY;
+        m[r] = xy + (size_t)r * NY;
+    }
+    return wave_value(i, j, a, b, e1, e2, tf, c, m, scale, time, plot, palette, rgb);
+}
+
+/* a colour table of colormaps.c / turbo_colormap.c, for the direct
+ * cross-check of the table palettes (entries, or 0 if none) */
+int ref_palette_table(int palette, float *out)
+{
+    const float(*t)[3] = NULL;
+    int n = 256;
+    switch (palette) {
+    case COL_TURBO: case COL_TURBO_CYCLIC: t = turbo_srgb_floats; break;
+    case COL_VIRIDIS: t = viridis_srgb_floats; break;
+    case COL_MAGMA: t = magma_srgb_floats; break;
+    case COL_INFERNO: t = inferno_srgb_floats; break;
+    case COL_PLASMA: t = plasma_srgb_floats; break;
+    case COL_CIVIDIS: t = cividis_srgb_floats; break;
+    case COL_PARULA: t = parula_srgb_floats; break;
+    case COL_TWILIGHT: case COL_TWILIGHT_SHIFTED: t = twilight_srgb_floats; n = 511; break;
+    default: return 0;
+    }
+    memcpy(out, t, sizeof(float) * 3 * n);
+    return n;
+}
+#elif defined(TABLE_TWIN)
 #include <fcntl.h>
 #include <unistd.h>
 /* init_laplacian_coords (sub_wave.c:11555) on a zeroed table for the field at

@@ -86,5 +86,64 @@
 #ifndef VMAX
 #define VMAX 2.5
 #endif
+#ifndef NSTEPS
 #define NSTEPS 10
+#endif
 #define BEAM_BC ((OSCILLATION_SCHEDULE == OSC_BEAM_SINE)||(OSCILLATION_SCHEDULE == OSC_BEAM_SINE_DECREASING)||(OSCILLATION_SCHEDULE == OSC_BEAM_SINE_TWOPERIODS)||(OSCILLATION_SCHEDULE == OSC_BEAM_SINE_CHIRP))
+
+/* colour macros of the drawing path (wave_billiard.c:200-241), for the
+ * colour builds */
+#ifndef COLOR_SCHEME
+#define COLOR_SCHEME 3
+#endif
+#ifndef COLOR_PALETTE
+#define COLOR_PALETTE 0
+#endif
+#ifndef SLOPE
+#define SLOPE 1.0
+#endif
+#ifndef COLOR_RANGE
+#define COLOR_RANGE 0.8
+#endif
+#ifndef ATTENUATION
+#define ATTENUATION 0.0
+#endif
+#ifndef VSHIFT_AMPLITUDE
+#define VSHIFT_AMPLITUDE -0.2
+#endif
+#ifndef VSCALE_AMPLITUDE
+#define VSCALE_AMPLITUDE 1.0
+#endif
+#ifndef LOG_SCALE
+#define LOG_SCALE 0.75
+#endif
+#ifndef LOG_SHIFT
+#define LOG_SHIFT 0.75
+#endif
+#ifndef FLUX_SCALE
+#define FLUX_SCALE 250.0
+#endif
+#ifndef AVRG_E_FACTOR
+#define AVRG_E_FACTOR 0.95
+#endif
+#ifndef COLORHUE
+#define COLORHUE 260
+#endif
+#ifndef COLORDRIFT
+#define COLORDRIFT 0.0
+#endif
+#ifndef LUMMEAN
+#define LUMMEAN 0.5
+#endif
+#ifndef LUMAMP
+#define LUMAMP 0.3
+#endif
+#ifndef HUEMEAN
+#define HUEMEAN 180.0
+#endif
+#ifndef HUEAMP
+#define HUEAMP -180.0
+#endif
+#ifndef RESCALE_COLOR_IN_CENTER
+#define RESCALE_COLOR_IN_CENTER 0
+#endif

@@ -142,6 +142,53 @@ void orc_run_table(const orc_params *p, double *b0, double *b1, double *b2, int 
                    const int *si, const int *sj, const double *amp, long amp_len, long step0,
                    const orc_extras *x);
 
+/* ---- frame colours (the drawing path's colour computation) ---------------
+ * orc_frame restates what draw_wave_highres_palette (wave_common.c:1748-1847)
+ * computes before it hands colours to OpenGL: for every block cell (i, j),
+ * i and j multiples of `size`, the plotted value and colour of wave_value
+ * (wave_common.c:1053-1171: plot types P_AMPLITUDE .. P_LOG_AVERAGE_ENERGY,
+ * with the running accumulators of the mean / average / flux plots) through
+ * color_scheme_palette / color_scheme_asym_palette (colors_waves.c:161-290)
+ * and the palettes of colormaps.c (hsl_to_rgb_jet 2077-2114,
+ * hsl_to_rgb_palette 2343-2447, amp_to_rgb_palette 2449-2548); then the
+ * SHADE_2D pseudo-3D shading, FADE_IN_OBSTACLE and fade, and the
+ * (TWOSPEEDS || xy_in) draw test.
+ * Table palettes (COL_TURBO .. COL_PARULA: 256 entries, COL_TWILIGHT and
+ * COL_TWILIGHT_SHIFTED: 511, COL_TURBO_CYCLIC: the 256-entry Turbo table)
+ * read the caller's table lut[3*n] (the program's own colour table).
+ * Shading takes the gradient between block cells (i+size, j) and (i, j+size);
+ * the reference reads cells (i+1, j) and (i, j+1), which it only fills when
+ * size is 1.  Indices below 0 that amp_to_rgb_palette would form are
+ * clamped to 0. */
+typedef struct {
+    int plot, scheme, palette;   /* PLOT, COLOR_SCHEME, palette argument       */
+    int asym;                    /* COLOR_PALETTE >= COL_TURBO (wave_value)    */
+    int size, time, shade, rescale_center, fade_in_obstacle, fade;
+    long nsteps;                 /* NSTEPS (C_LUM hue drift)                   */
+    double scale, fade_value;
+    double slope, color_range, attenuation;        /* SLOPE, COLOR_RANGE, ATTENUATION */
+    double vshift_amplitude, vscale_amplitude;     /* VSHIFT/VSCALE_AMPLITUDE   */
+    double log_shift, log_scale, flux_scale, avrg_e_factor;
+    double colorhue, colordrift, lummean, lumamp, huemean, hueamp;
+    double shade_scale, light_x, light_y;          /* SHADE_SCALE_2D, light[2]  */
+    double courant, courantb, e_scale;             /* compute_energy constants  */
+} orc_frame_params;
+
+/* Colours of one frame on flat k = i*NY + j arrays.  total_energy,
+ * average_energy (NX*NY) and total_flux (2*NX*NY, index 2*j*NX + 2*i) are
+ * the running accumulators (updated); color_scale and fade_table may be
+ * NULL when unused.  Outputs at block cells: values[k], rgb[3k..3k+2]
+ * (double, before the float conversion of glColor3f) and drawn[k]. */
+void orc_frame(const orc_params *p, const orc_frame_params *f, const double *phi, const double *psi,
+               const short *xy_in, double *total_energy, double *average_energy, double *total_flux,
+               const double *color_scale, const double *fade_table, const float *lut, int lut_n,
+               double *values, double *rgb, unsigned char *drawn);
+
+/* the colour of one value: color_scheme_palette (asym = 0) or
+ * color_scheme_asym_palette (asym = 1) */
+void orc_color(const orc_frame_params *f, int scheme, int asym, double value, double scale, int time,
+               const float *lut, int lut_n, double rgb[3]);
+
 /* Per-cell energy, compute_energy wave_common.c:453-472, with the Courant
  * constants given explicitly (c2_in = COURANT^2 for xy_in != 0, c2_out =
  * COURANTB^2 used when TWOSPEEDS). */

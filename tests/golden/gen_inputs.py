@@ -134,3 +134,47 @@ def table_inputs(tag, nx, ny, twospeeds, extras):
             d[k] = d[k] * 0.5
         d["drive_cells"] = [(0, j) for j in range(1, ny - 1)] + [(0, 0), (0, ny - 1)]
     return d
+
+
+def color_builds():
+    """(tag, frame parameters) of the reference colour builds (oracle/build_ref.sh:
+    24x17, E_SCALE 50, COURANT 0.1, COURANTB 0.05, NSTEPS 500, COLORDRIFT 30)."""
+    base = dict(e_scale=50.0, courant=0.1, courantb=0.05, nsteps=500, colordrift=30.0)
+    out = []
+    for sc in (0, 1, 3, 4, 5):
+        out.append(("col_s%d" % sc, dict(base, scheme=sc, asym=0, twospeeds=0)))
+    out.append(("col_s1_asym", dict(base, scheme=1, asym=1, twospeeds=1)))
+    out.append(("col_s3_asym", dict(base, scheme=3, asym=1, twospeeds=0)))
+    out.append(("col_s3_att", dict(base, scheme=3, asym=0, twospeeds=1, color_range=1.0, attenuation=0.01,
+                                   rescale_center=1)))
+    return out
+
+
+COLOR_FRAMES = 3
+
+
+def color_inputs(tag, nx=24, ny=17):
+    """mask, color_scale and the fields of frames 0..COLOR_FRAMES-1: phi
+    U[-0.05, 0.05], psi = phi + U[-0.01, 0.01]"""
+    s = case_seed(tag)
+    r = uniform(s, (nx, ny), 0.0, 1.0)
+    xy = np.where(r < 0.1, -1, np.where(r < 0.3, 0, np.where(r < 0.8, 1, 2))).astype(np.int16)
+    frames = []
+    for t in range(COLOR_FRAMES):
+        phi = uniform(s + 10 + 2 * t, (nx, ny), -0.05, 0.05)
+        frames.append((phi, phi + uniform(s + 11 + 2 * t, (nx, ny), -0.01, 0.01)))
+    return dict(xy_in=xy, color_scale=uniform(s + 1, (nx, ny), 0.2, 1.0), frames=frames)
+
+
+def color_cases():
+    """(tag, params, palette, plot) with a colour for every cell: the schemes
+    C_ONEDIM_LINEAR / C_BASIC_LINEAR set none for the asymmetric scheme
+    (P_ENERGY_FLUX)"""
+    out = []
+    for tag, prm in color_builds():
+        for pal in (0, 2):
+            for plot in range(10):
+                if plot == 6 and prm["scheme"] in (4, 5):
+                    continue
+                out.append((tag, prm, pal, plot))
+    return out

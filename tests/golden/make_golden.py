@@ -14,6 +14,10 @@ Outputs (committed):
   ext_cases.npz     Neumann ghost points + oscillating left edge + FLOOR:
                     the drive values oscillating_bc(time, j) and phi, psi,
                     tmp after EXT_STEPS steps, for every gen_inputs.ext_cases()
+  frame_digests.json  the frame colours of wave_value (wave_common.c:1053)
+                    through the colour schemes and palettes COL_JET, COL_GRAY:
+                    SHA-256 of the values and colours of every cell, frames
+                    0..COLOR_FRAMES-1, for every gen_inputs.color_cases()
   table_cases.npz   the neighbour-table stepper (evolve_wave_half_new): the
                     reference's table (counts, flat indices), the L-shape
                     lines, drive values where driven, and phi, psi, tmp after
@@ -33,8 +37,8 @@ sys.path.insert(0, os.path.join(ROOT, "youtube-simulations_amd"))
 sys.path.insert(0, HERE)
 
 import pyoracle  # noqa: E402
-from gen_inputs import (EXT_STEPS, TABLE_STEPS, case_inputs, ext_cases, ext_inputs, small_cases,  # noqa: E402
-                        table_cases, table_inputs)
+from gen_inputs import (COLOR_FRAMES, EXT_STEPS, TABLE_STEPS, case_inputs, color_cases,  # noqa: E402
+                        color_inputs, ext_cases, ext_inputs, small_cases, table_cases, table_inputs)
 from wavehip import workloads  # noqa: E402
 
 NSTEPS_SMALL = 7
@@ -111,6 +115,24 @@ def main():
         phi, psi, tmp = R.run(b[0], b[1], b[2], TABLE_STEPS, d["xy_in"], d["tcc"], d["tc"], d["tgamma"])
         out[tag + "/phi"], out[tag + "/psi"], out[tag + "/tmp"] = phi, psi, tmp
     np.savez_compressed(os.path.join(HERE, "table_cases.npz"), **out)
+    # frame colours: wave_value cell by cell (size 1), COL_JET and COL_GRAY
+    dig = {}
+    for (tag, prm, pal, plot) in color_cases():
+        R = pyoracle.RefBuild(tag)
+        d = color_inputs(tag)
+        acc = pyoracle.FrameAccumulators(R.nx, R.ny)
+        frames = []
+        for t, (phi, psi) in enumerate(d["frames"]):
+            vals = np.zeros((R.nx, R.ny))
+            rgb = np.zeros((R.nx, R.ny, 3))
+            for i in range(R.nx):
+                for j in range(R.ny):
+                    vals[i, j], rgb[i, j] = R.wave_value(phi, psi, acc, d["color_scale"], d["xy_in"], 1.0, t, plot,
+                                                         pal, i, j)
+            frames.append(dict(values=sha(vals), rgb=sha(rgb), probe=[float(vals[5, 7]).hex(), float(rgb[5, 7, 0]).hex()]))
+        dig["%s/pal%d/plot%d" % (tag, pal, plot)] = frames
+    with open(os.path.join(HERE, "frame_digests.json"), "w") as f:
+        json.dump(dig, f, indent=0)
     print("fixtures written")
 
 

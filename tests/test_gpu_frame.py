@@ -1,0 +1,123 @@
+"""Frame colours on the GPU (wv_frame) against the oracle (orc_frame, pinned to
+the reference's wave_value and colour functions).
+
+The GPU evaluates the same FP64 expressions; tanh, exp, log and atan come
+from the device library, within an ulp or two of glibc's, so the float
+colours glColor3f would receive agree to within 2e-7 (a float ulp near 1 is
+6e-8; a palette index or branch would have to sit within ~1e-13 of a
+boundary to differ).  The drawn flag (alpha) is exact.
+"""
+import numpy as np
+import pytest
+
+from conftest import say
+
+pytestmark = pytest.mark.gpu
+TOL = 2e-7
+
+CONFIGS = [
+    # (plot, scheme, palette, asym, size, shade, extras)
+    (0, 3, 0, 0, 1, 1, {}),
+    (0, 0, 0, 0, 2, 1, {}),
+    (0, 1, 17, 0, 1, 0, {}),
+    (1, 3, 10, 1, 1, 1, {}),
+    (1, 1, 0, 1, 3, 0, {}),
+    (2, 4, 2, 0, 1, 1, {}),
+    (3, 3, 11, 1, 2, 1, {}),
+    (4, 0, 0, 0, 1, 1, {}),
+    (5, 5, 0, 0, 1, 0, {}),
+    (6, 3, 101, 0, 1, 1, {}),
+    (7, 4, 0, 0, 1, 1, {}),
+    (8, 1, 18, 1, 1, 0, {}),
+    (9, 3, 16, 0, 2, 1, {}),
+    (0, 3, 0, 0, 1, 1, dict(rescale_center=1, fade_in_obstacle=1, fade=1, fade_value=0.7)),
+    (1, 0, 10, 0, 1, 1, dict(attenuation=0.02, color_range=1.0, colordrift=40.0)),
+]
+
+
+@pytest.fixture(scope="module")
+def wh():
+    import wavehip
+    wavehip.load()
+    return wavehip
+
+
+def luts():
+    rng = np.random.default_rng(9)
+    return {256: rng.uniform(0, 1, (256, 3)).astype(np.float32), 511: rng.uniform(0, 1, (511, 3)).astype(np.float32)}
+
+
+@pytest.mark.parametrize("cfg", CONFIGS, ids=lambda c: "plot%d-s%d-pal%d-a%d-z%d-sh%d%s" % (c[:6] + ("x" if c[6] else "",)))
+@pytest.mark.parametrize("form", [0, 1])
+def test_frame_matches_oracle(wh, oracle, cfg, form):
+    from wavehip import workloads
+    plot, scheme, pal, asym, size, shade, extra = cfg
+    if form == 1 and (plot not in (0, 1, 3) or extra):
+        pytest.skip("split form: a few plots")
+    nx, ny = 150, 97
+    say("frame", form, cfg)
+    w = workloads.random_case(nx, ny, 40 + plot, bcond=2, twospeeds=int(plot == 2))
+    w["phi"] *= 0.05
+    w["psi"] = w["phi"] + 0.01 * np.random.default_rng(plot).standard_normal((nx, ny))
+    params = dict(w["params"], e_scale=50.0)
+    lut = luts()[511 if pal in (17, 18) else 256] if pal >= 10 else None
+    rng = np.random.default_rng(5)
+    cs, ft = rng.uniform(0.2, 1.0, (nx, ny)), rng.uniform(0.2, 1.0, (nx, ny))
+    W = wh.Wave(params, form=form)
+    W.set_tables(w["xy_in"], w["tcc"], w["tc"], w["tgamma"])
+    W.set_state(w["phi"], w["psi"])
+    W.set_frame_tables(lut=lut, color_scale=cs, fade_table=ft)
+    p = oracle.make_params(nx, ny, twospeeds=params["twospeeds"])
+    acc = oracle.FrameAccumulators(nx, ny)
+    kw = dict(plot=plot, scheme=scheme, palette=pal, asym=asym, size=size, shade=shade, **extra)
+    for t in range(3):
+        W.step(4)
+        phi, psi = W.get_state()
+        f = W.frame_params(time=t, **kw)
+        got = W.frame(f)
+        fo = oracle.frame_params(time=t, e_scale=50.0, **kw)
+        _, rgb, drawn = oracle.frame(p, fo, phi, psi, w["xy_in"], acc, color_scale=cs, fade_table=ft, lut=lut)
+        exp_rgb = rgb[::size, ::size].astype(np.float32)
+        exp_drawn = drawn[::size, ::size]
+        assert got.shape == exp_rgb.shape[:2] + (4,)
+        assert np.array_equal(got[..., 3] == 1.0, exp_drawn) and np.all((got[..., 3] == 0) | (got[..., 3] == 1))
+        err = np.abs(got[..., :3] - exp_rgb)
+        assert err.max() <= TOL, (t, err.max(), np.unravel_index(err.argmax(), err.shape))
+    W.close()
+
+
+def test_frame_argument_checks(wh):
+    from wavehip import workloads
+    w = workloads.random_case(40, 30, 3)
+    W = wh.Wave(w["params"])
+    W.set_tables(w["xy_in"], w["tcc"], w["tc"], w["tgamma"])
+    W.set_state(w["phi"], w["psi"])
+    for bad in (dict(plot=10), dict(scheme=2), dict(palette=1), dict(palette=10), dict(size=0),
+                dict(plot=6, scheme=4), dict(rescale_center=1), dict(fade_in_obstacle=1)):
+        with pytest.raises(wh.WaveError):
+            W.frame(W.frame_params(**bad))
+    with pytest.raises(wh.WaveError):
+        W.set_frame_tables(lut=np.zeros((100, 3), np.float32))
+    W.set_frame_tables(lut=np.zeros((511, 3), np.float32))
+    with pytest.raises(wh.WaveError):
+        W.frame(W.frame_params(palette=10))  # needs 256 entries
+    W.frame(W.frame_params(palette=17))
+    W.close()
+
+
+def test_frame_running_sums_reset(wh):
+    """wv_frame_reset zeroes the running sums: the first mean-energy frame
+    after a reset equals the first frame of a fresh context."""
+    from wavehip import workloads
+    w = workloads.random_case(64, 48, 4)
+    w["phi"] *= 0.05
+    W = wh.Wave(dict(w["params"], e_scale=50.0))
+    W.set_tables(w["xy_in"], w["tcc"], w["tc"], w["tgamma"])
+    W.set_state(w["phi"], w["psi"] * 0.05)
+    f = W.frame_params(plot=3, time=0)
+    a = W.frame(f)
+    b = W.frame(f)
+    W.frame_reset()
+    c = W.frame(f)
+    W.close()
+    assert not np.array_equal(a, b) and np.array_equal(a, c)

@@ -62,6 +62,14 @@ struct Grid {
     int *tbl_ptr = nullptr, *tbl_col = nullptr, *tbl_n = nullptr;
     long long *tbl_off = nullptr;
     std::vector<uint64_t> h_act_plain;
+    // frame colours: palette table, color_scale and fade_table (NX x ldy),
+    // double-buffered running sums per block cell, output blocks
+    float *fr_lut = nullptr;
+    int fr_lut_n = 0;
+    double *fr_cs = nullptr, *fr_ft = nullptr;
+    double *fr_te[2] = {nullptr, nullptr}, *fr_ae[2] = {nullptr, nullptr}, *fr_tf[2] = {nullptr, nullptr};
+    int fr_te_cur = 0, fr_ae_cur = 0, fr_tf_cur = 0, fr_size = 0;
+    float4 *fr_out = nullptr;
 };
 
 struct Engine {
@@ -181,7 +189,8 @@ void free_grid(Grid &G)
                     G.act, G.eact, G.med, G.tccf, G.tgf, G.tc, G.tccf_hi, G.tccf_lo, G.tgf32,
                     G.src_ptr, G.src_col, G.src_id, G.amp, G.nz, G.neu_ptr, G.neu_col, G.neu_code,
                     G.neu_w, G.drv_ptr, G.drv_col, G.drv_id, G.drv_val, G.tbl_ptr, G.tbl_col, G.tbl_n,
-                    G.tbl_off};
+                    G.tbl_off, G.fr_lut, G.fr_cs, G.fr_ft, G.fr_te[0], G.fr_te[1], G.fr_ae[0], G.fr_ae[1],
+                    G.fr_tf[0], G.fr_tf[1], G.fr_out};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
 }
@@ -1138,6 +1147,124 @@ int engine_step(Engine *e, long nsteps, bool force_k1)
     return WV_OK;
 }
 
+// ------------------------------------------------------------ frame colours
+
+int frame_set_tables(Engine *e, int k, const float *lut, int lut_n, const double *cs, const double *ft)
+{
+    if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
+    Grid &G = e->g[k];
+    if (lut && lut_n != 256 && lut_n != 511) return fail(e, WV_EINVAL, "palette table of %d entries (256 or 511)", lut_n);
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    for (void **q : {(void **)&G.fr_lut, (void **)&G.fr_cs, (void **)&G.fr_ft})
+        if (*q) { (void)hipFree(*q); *q = nullptr; }
+    G.fr_lut_n = 0;
+    const int nx = G.p.nx, ny = G.p.ny;
+    if (lut) {
+        HIPCHK(e, hipMalloc(&G.fr_lut, sizeof(float) * 3 * lut_n));
+        HIPCHK(e, hipMemcpy(G.fr_lut, lut, sizeof(float) * 3 * lut_n, hipMemcpyHostToDevice));
+        G.fr_lut_n = lut_n;
+    }
+    for (auto pr : {std::make_pair(&G.fr_cs, cs), std::make_pair(&G.fr_ft, ft)}) {
+        if (!pr.second) continue;
+        HIPCHK(e, hipMalloc(pr.first, sizeof(double) * (size_t)nx * G.ldy));
+        HIPCHK(e, hipMemcpy2D(*pr.first, G.ldy * sizeof(double), pr.second, ny * sizeof(double), ny * sizeof(double),
+                              nx, hipMemcpyHostToDevice));
+    }
+    return WV_OK;
+}
+
+int frame_reset(Engine *e, int k)
+{
+    if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
+    Grid &G = e->g[k];
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    for (double **q : {&G.fr_te[0], &G.fr_te[1], &G.fr_ae[0], &G.fr_ae[1], &G.fr_tf[0], &G.fr_tf[1]})
+        if (*q) { (void)hipFree(*q); *q = nullptr; }
+    if (G.fr_out) { (void)hipFree(G.fr_out); G.fr_out = nullptr; }
+    G.fr_size = 0;
+    G.fr_te_cur = G.fr_ae_cur = G.fr_tf_cur = 0;
+    return WV_OK;
+}
+
+int frame(Engine *e, int k, const wv_frame_params *f, float *rgba)
+{
+    if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
+    if (!f || !rgba) return fail(e, WV_EINVAL, "frame parameters or output is NULL");
+    int rc = ready(e);
+    if (rc) return rc;
+    Grid &G = e->g[k];
+    const int pal = f->palette, sch = f->scheme, plot = f->plot;
+    if (plot < 0 || plot > 9) return fail(e, WV_EINVAL, "plot %d is not a wave plot type 0..9", plot);
+    if (sch != 0 && sch != 1 && sch != 3 && sch != 4 && sch != 5)
+        return fail(e, WV_EUNSUPPORTED, "colour scheme %d (0, 1, 3, 4, 5 are provided)", sch);
+    const bool table = (pal >= 10 && pal <= 18) || pal == 101;
+    if (pal != 0 && pal != 2 && !table)
+        return fail(e, WV_EUNSUPPORTED, "palette %d (COL_JET, COL_GRAY and the table palettes are provided)", pal);
+    const int need = (pal == 17 || pal == 18) ? 511 : 256;
+    if (table && G.fr_lut_n != need)
+        return fail(e, WV_ESTATE, "palette %d needs its table of %d entries (wv_frame_set_tables)", pal, need);
+    // the asymmetric schemes leave the colour unset for the linear schemes
+    const bool asym_used = plot == 6 || (f->asym && (plot == 1 || plot == 3 || plot == 8));
+    if (asym_used && (sch == 4 || sch == 5))
+        return fail(e, WV_EUNSUPPORTED, "plot %d with scheme %d: color_scheme_asym_palette sets no colour", plot, sch);
+    if (f->size < 1) return fail(e, WV_EINVAL, "block size must be >= 1");
+    if (f->rescale_center && !G.fr_cs) return fail(e, WV_ESTATE, "RESCALE_COLOR_IN_CENTER needs color_scale");
+    if (f->fade_in_obstacle && !G.fr_ft) return fail(e, WV_ESTATE, "FADE_IN_OBSTACLE needs fade_table");
+    if (sch == 0 && f->nsteps == 0) return fail(e, WV_EINVAL, "C_LUM divides by nsteps");
+    const int nx = G.p.nx, ny = G.p.ny, s = f->size;
+    const int nbx = (nx + s - 1) / s, nby = (ny + s - 1) / s;
+    if (nbx > 65535) return fail(e, WV_EINVAL, "more than 65535 block rows");
+    HIPCHK(e, hipSetDevice(e->device));
+    const size_t nb = (size_t)nbx * nby;
+    if (G.fr_size != s) {
+        if ((rc = frame_reset(e, k))) return rc;
+        for (double **q : {&G.fr_te[0], &G.fr_te[1], &G.fr_ae[0], &G.fr_ae[1]}) {
+            HIPCHK(e, hipMalloc(q, sizeof(double) * nb));
+            HIPCHK(e, hipMemset(*q, 0, sizeof(double) * nb));
+        }
+        for (double **q : {&G.fr_tf[0], &G.fr_tf[1]}) {
+            HIPCHK(e, hipMalloc(q, sizeof(double) * 2 * nb));
+            HIPCHK(e, hipMemset(*q, 0, sizeof(double) * 2 * nb));
+        }
+        HIPCHK(e, hipMalloc(&G.fr_out, sizeof(float4) * nb));
+        G.fr_size = s;
+    }
+    WvFrame a{};
+    a.plot = plot; a.scheme = sch; a.palette = pal; a.asym = f->asym != 0; a.size = s; a.time = f->time;
+    a.shade = f->shade != 0; a.rescale = f->rescale_center != 0; a.fadeobs = f->fade_in_obstacle != 0;
+    a.fade = f->fade != 0; a.nsteps = f->nsteps;
+    a.scale = f->scale; a.fade_value = f->fade_value; a.slope = f->slope; a.color_range = f->color_range;
+    a.attenuation = f->attenuation; a.vshift = f->vshift_amplitude; a.vscale = f->vscale_amplitude;
+    a.log_shift = f->log_shift; a.log_scale = f->log_scale; a.flux_scale = f->flux_scale;
+    a.avrg = f->avrg_e_factor; a.colorhue = f->colorhue; a.colordrift = f->colordrift; a.lummean = f->lummean;
+    a.lumamp = f->lumamp; a.huemean = f->huemean; a.hueamp = f->hueamp; a.shade_scale = f->shade_scale;
+    a.light_x = f->light_x; a.light_y = f->light_y; a.courant = f->courant; a.courantb = f->courantb;
+    a.es = G.p.e_scale;
+    a.nx = nx; a.ny = ny; a.ldy = G.ldy; a.ldw = G.ldw; a.nbx = nbx; a.nby = nby; a.twospeeds = G.p.twospeeds != 0;
+    a.lut_n = G.fr_lut_n;
+    if (e->form == WV_FORM_FP64) {
+        a.phi = G.u[e->role[0]];
+        a.psi = G.u[e->role[1]];
+    } else {
+        a.hi = G.hi[e->scur]; a.lo = G.lo[e->scur]; a.v = G.v[e->scur];
+    }
+    a.nz = G.nz; a.cs = G.fr_cs; a.ft = G.fr_ft; a.lut = G.fr_lut;
+    a.te0 = G.fr_te[G.fr_te_cur]; a.te1 = G.fr_te[1 - G.fr_te_cur];
+    a.ae0 = G.fr_ae[G.fr_ae_cur]; a.ae1 = G.fr_ae[1 - G.fr_ae_cur];
+    a.tf0 = G.fr_tf[G.fr_tf_cur]; a.tf1 = G.fr_tf[1 - G.fr_tf_cur];
+    a.out = G.fr_out;
+    int lr = wv_launch_frame(a, e->stream);
+    if (lr) return fail(e, WV_EHIP, "kernel launch failed: %s", hipGetErrorString((hipError_t)lr));
+    if (plot == 3 || plot == 5) G.fr_te_cur ^= 1;
+    if (plot == 8 || plot == 9) G.fr_ae_cur ^= 1;
+    if (plot == 7) G.fr_tf_cur ^= 1;
+    HIPCHK(e, hipMemcpyAsync(rgba, G.fr_out, sizeof(float4) * nb, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return WV_OK;
+}
+
 int engine_energy(Engine *e, double *out)
 {
     int rc = ready(e);
@@ -1322,6 +1449,56 @@ int wv_energy(wv_ctx *c, double *e)
 {
     if (!c) return WV_EINVAL;
     return engine_energy(c->e, e);
+}
+
+void wv_frame_params_default(wv_frame_params *f)
+{
+    if (!f) return;
+    memset(f, 0, sizeof *f);
+    // the drawing macros of the checked-in wave_billiard.c:200-241, 287
+    f->plot = 0;
+    f->scheme = 3;
+    f->palette = 0;
+    f->size = 1;
+    f->nsteps = 500;
+    f->scale = 1.0;
+    f->fade_value = 1.0;
+    f->slope = 1.0;
+    f->color_range = 0.8;
+    f->vshift_amplitude = -0.2;
+    f->vscale_amplitude = 1.0;
+    f->log_shift = 0.75;
+    f->log_scale = 0.75;
+    f->flux_scale = 250.0;
+    f->avrg_e_factor = 0.95;
+    f->colorhue = 260.0;
+    f->lummean = 0.5;
+    f->lumamp = 0.3;
+    f->huemean = 180.0;
+    f->hueamp = -180.0;
+    f->shade_scale = 0.5;
+    f->light_x = 0.40824829;
+    f->light_y = 0.816496581;
+    f->courant = 0.1;
+    f->courantb = 0.05;
+}
+
+int wv_frame_set_tables(wv_ctx *c, const float *lut, int lut_n, const double *color_scale, const double *fade_table)
+{
+    if (!c) return WV_EINVAL;
+    return frame_set_tables(c->e, 0, lut, lut_n, color_scale, fade_table);
+}
+
+int wv_frame_reset(wv_ctx *c)
+{
+    if (!c) return WV_EINVAL;
+    return frame_reset(c->e, 0);
+}
+
+int wv_frame(wv_ctx *c, const wv_frame_params *f, float *rgba)
+{
+    if (!c) return WV_EINVAL;
+    return frame(c->e, 0, f, rgba);
 }
 
 int wv_get_tiling(const wv_ctx *c, int *tb_depth, int *rows_per_tile, int *tile_cols)

@@ -1,0 +1,363 @@
+// wv_frame.hip -- the colours of one frame of the drawing path on the GPU.
+//
+// draw_wave_highres_palette (wave_common.c:1748-1847) computes, for every
+// block cell (i, j) of a size x size grid, the plotted value and colour of
+// wave_value (wave_common.c:1053-1171), shades the colours with the value
+// gradient (SHADE_2D), fades them and hands them to OpenGL.  Here one thread
+// owns one block cell and writes (r, g, b, drawn) as floats, the values
+// glColor3f receives, so a frame leaves the GPU as 16 bytes per block
+// instead of the 16 bytes per cell of phi and psi.
+//
+// The running sums of the mean / average / flux plots are read from one
+// buffer and written to the other, so the shading gradient can evaluate the
+// neighbouring blocks' values (from their old sums and this frame's field)
+// in the same pass without racing with their owners.
+//
+// Arithmetic follows the reference expression by expression (no FMA
+// contraction); tanh, exp, log, atan, sqrt and fmod are the device library's,
+// which may differ from glibc's in the last bit.
+#include "wv_device.h"
+
+using namespace wv;
+
+namespace {
+
+// global_pdes.c:318-327, 354-359, 364-378 and the constants of global_pdes.c:8-10
+constexpr int P_AMPLITUDE = 0, P_ENERGY = 1, P_MIXED = 2, P_MEAN_ENERGY = 3, P_LOG_ENERGY = 4,
+              P_LOG_MEAN_ENERGY = 5, P_ENERGY_FLUX = 6, P_TOTAL_ENERGY_FLUX = 7, P_AVERAGE_ENERGY = 8;
+constexpr int C_LUM = 0, C_HUE = 1, C_ONEDIM = 3, C_ONEDIM_LINEAR = 4, C_BASIC_LINEAR = 5;
+constexpr int COL_JET = 0, COL_GRAY = 2, COL_TURBO = 10, COL_PARULA = 16, COL_TWILIGHT = 17,
+              COL_TWILIGHT_SHIFTED = 18, COL_TURBO_CYCLIC = 101;
+constexpr double REF_PI = 3.141592654, REF_DPI = 6.283185307, REF_PID = 1.570796327;
+
+struct Rgb {
+    double c[3];
+};
+
+__device__ inline double lut_at(const WvFrame &f, int idx, int c)
+{
+    idx = idx < 0 ? 0 : (idx > f.lut_n - 1 ? f.lut_n - 1 : idx);
+    return (double)gc(f.lut)[3 * idx + c];
+}
+
+// colormaps.c:2077-2114
+__device__ inline Rgb hsl_jet(double h, double s, double l)
+{
+    const double c = (1.0 - fabs(2.0 * l - 1.0)) * s;
+    const double m = 1.0 * (l - 0.5 * c);
+    const double x = c * (1.0 - fabs(fmod(h / 60.0, 2.0) - 1.0));
+    if (h >= 0.0 && h < 60.0) return {{c + m, x + m, m}};
+    if (h < 120.0) return {{x + m, c + m, m}};
+    if (h < 180.0) return {{m, c + m, x + m}};
+    if (h < 240.0) return {{m, x + m, c + m}};
+    if (h < 300.0) return {{x + m, m, c + m}};
+    if (h < 360.0) return {{c + m, m, x + m}};
+    return {{m, m, m}};
+}
+
+__device__ inline Rgb lut_rgb(const WvFrame &f, int idx, double mul, bool scaled)
+{
+    Rgb r;
+    for (int c = 0; c < 3; c++) r.c[c] = scaled ? mul * lut_at(f, idx, c) : lut_at(f, idx, c);
+    return r;
+}
+
+__device__ inline Rgb turbo_blend(const WvFrame &f, double t)
+{
+    Rgb r;
+    for (int c = 0; c < 3; c++) r.c[c] = (1.0 - t) * lut_at(f, 255, c) + t * lut_at(f, 0, c);
+    return r;
+}
+
+// hsl_to_rgb_palette, colormaps.c:2343-2447 (hue in degrees)
+__device__ inline Rgb hsl_palette(const WvFrame &f, double h, double s, double l)
+{
+    const double ratio = 0.711111111;
+    const int pal = f.palette;
+    if (pal == COL_JET) return hsl_jet(h, s, l);
+    if (pal == COL_GRAY) {
+        const int ch = (int)(h / 360.0);
+        return {{(double)ch, (double)ch, (double)ch}};
+    }
+    const double two_l = 2.0 * l;
+    if (pal >= COL_TURBO && pal <= COL_PARULA) return lut_rgb(f, 255 - (int)(ratio * h), two_l, true);
+    if (pal == COL_TWILIGHT) {
+        int ch = 510 - (int)(2.0 * ratio * h);
+        ch = ch > 510 ? 510 : (ch < 0 ? 0 : ch);
+        return lut_rgb(f, ch, two_l, true);
+    }
+    if (pal == COL_TWILIGHT_SHIFTED) {
+        int ch = (int)(2.0 * ratio * h) - 255;
+        if (ch > 510) ch -= 510;
+        else if (ch < 0) ch += 510;
+        return lut_rgb(f, ch, two_l, true);
+    }
+    // COL_TURBO_CYCLIC
+    if (h < 330.0) return lut_rgb(f, 255 - (int)(ratio * h * 12.0 / 11.0), two_l, true);
+    if (h > 360.0) h = 360.0;
+    return turbo_blend(f, (h - 330.0) / 30.0);
+}
+
+// amp_to_rgb_palette, colormaps.c:2449-2548 (amplitude in [0, 1])
+__device__ inline Rgb amp_palette(const WvFrame &f, double h)
+{
+    const int pal = f.palette;
+    int ch = (int)(256.0 * h);
+    if (ch > 255) ch = 255;
+    if (pal == COL_JET) return hsl_jet(360.0 * (1.0 - h), 0.9, 0.5);
+    if (pal == COL_GRAY) return {{h, h, h}};
+    if (pal >= COL_TURBO && pal <= COL_PARULA) return lut_rgb(f, ch, 1.0, false);
+    if (pal == COL_TWILIGHT) {
+        ch = (int)(511.0 * h);
+        ch = ch > 510 ? 510 : (ch < 0 ? 0 : ch);
+        return lut_rgb(f, ch, 1.0, false);
+    }
+    if (pal == COL_TWILIGHT_SHIFTED) {
+        ch = 255 - (int)(511.0 * h);
+        if (ch > 510) ch -= 510;
+        else if (ch < 0) ch += 510;
+        return lut_rgb(f, ch, 1.0, false);
+    }
+    if (h < 0.9) return lut_rgb(f, (int)(256.0 * h / 0.9), 1.0, false);
+    if (h > 1.0) h = 1.0;
+    return turbo_blend(f, (h - 0.9) / 0.1);
+}
+
+// color_amplitude, colors_waves.c:3-16
+__device__ inline double amplitude(const WvFrame &f, double value, double scale)
+{
+    const double a = tanh(f.slope * value / scale);
+    if (f.attenuation == 0.0) return f.color_range == 1.0 ? a : f.color_range * a;
+    const double d = exp(-((double)f.time * f.attenuation));
+    return f.color_range == 1.0 ? a * d : f.color_range * a * d;
+}
+
+// color_scheme_palette (colors_waves.c:161-218), color_scheme_asym_palette (220-290)
+__device__ inline Rgb colour(const WvFrame &f, int scheme, bool asym, double value, double scale, int time)
+{
+    double hue, y, amp;
+    switch (scheme) {
+    case C_LUM:
+        hue = f.colorhue + (double)time * f.colordrift / (double)f.nsteps;
+        if (hue < 0.0) hue += 360.0;
+        if (hue >= 360.0) hue -= 360.0;
+        amp = amplitude(f, value, scale);
+        y = f.lummean + amp * f.lumamp;
+        y -= (double)(int)y;
+        return hsl_palette(f, hue, 0.9, y);
+    case C_HUE:
+        if (asym) {
+            amp = 2.0 * tanh(f.slope * value / scale) * exp(-((double)time * f.attenuation)) - 1.0;
+            hue = f.huemean + 0.8 * amp * f.hueamp;
+        } else {
+            amp = amplitude(f, value, scale);
+            hue = f.huemean + amp * f.hueamp;
+        }
+        if (hue < 0.0) hue += 360.0;
+        if (hue >= 360.0) hue -= 360.0;
+        return hsl_palette(f, hue, 0.9, 0.5);
+    case C_ONEDIM:
+        amp = amplitude(f, value, scale);
+        return amp_palette(f, asym ? amp : 0.5 * (1.0 + amp));
+    case C_ONEDIM_LINEAR:
+        amp = f.slope * value / scale;
+        if (amp > 1.0) amp -= 1.0;
+        else if (amp < 0.0) amp += 1.0;
+        return amp_palette(f, amp);
+    default:  // C_BASIC_LINEAR
+        if (value > 1.0) value -= 1.0;
+        else if (value < 0.0) value += 1.0;
+        return amp_palette(f, value);
+    }
+}
+
+// phi and psi of a cell in either form
+__device__ inline double phi_at(const WvFrame &f, int i, int j)
+{
+    const size_t k = (size_t)i * f.ldy + j;
+    if (f.phi) return gc(f.phi)[k];
+    return (double)gc(f.hi)[k] + (double)gc(f.lo)[k];
+}
+__device__ inline double psi_at(const WvFrame &f, int i, int j, double phi)
+{
+    const size_t k = (size_t)i * f.ldy + j;
+    if (f.psi) return gc(f.psi)[k];
+    return phi - (double)gc(f.v)[k];
+}
+__device__ inline bool nz_at(const WvFrame &f, int i, int j)
+{
+    return (gc(f.nz)[(size_t)i * f.ldw + (j >> 6)] >> (j & 63)) & 1u;
+}
+
+// compute_energy, wave_common.c:453-472
+__device__ inline double energy(const WvFrame &f, int i, int j, double x, double y)
+{
+    const int ip = i + 1 == f.nx ? f.nx - 1 : i + 1, im = i == 0 ? 0 : i - 1;
+    const int jp = j + 1 == f.ny ? f.ny - 1 : j + 1, jm = j == 0 ? 0 : j - 1;
+    const double e = phi_at(f, ip, j), w = phi_at(f, im, j), n = phi_at(f, i, jp), s = phi_at(f, i, jm);
+    const double v = x - y;
+    const double gx2 = (e - x) * (e - x) + (x - w) * (x - w);
+    const double gy2 = (n - x) * (n - x) + (x - s) * (x - s);
+    if (nz_at(f, i, j)) return f.es * f.es * (v * v + 0.5 * f.courant * f.courant * (gx2 + gy2));
+    if (f.twospeeds) return f.es * f.es * (v * v + 0.5 * f.courantb * f.courantb * (gx2 + gy2));
+    return 0.0;
+}
+
+// argument, sub_wave.c:287-304
+__device__ inline double argument(double x, double y)
+{
+    double a;
+    if (x != 0.0) {
+        a = atan(y / x);
+        if (x < 0.0) a += REF_PI;
+    } else {
+        a = REF_PID;
+        if (y < 0.0) a = REF_PI * 1.5;
+    }
+    return a;
+}
+
+// The value and colour of block (bi, bj), wave_value (wave_common.c:1053-1171),
+// from the old running sums; the new sums go to te / ae / tf.
+__device__ double wave_value(const WvFrame &f, int bi, int bj, Rgb &rgb, double &te, double &ae, double &tfx,
+                             double &tfy)
+{
+    const int i = bi * f.size, j = bj * f.size;
+    const size_t b = (size_t)bi * f.nby + bj;
+    const double x = phi_at(f, i, j), y = psi_at(f, i, j, x);
+    const double sc = f.scale;
+    const int t = f.time;
+    double value, e;
+    switch (f.plot) {
+    case P_AMPLITUDE:
+        value = x;
+        rgb = colour(f, f.scheme, false, f.vshift + f.vscale * value, sc, t);
+        break;
+    case P_ENERGY:
+        value = energy(f, i, j, x, y);
+        rgb = colour(f, f.scheme, f.asym, value, sc, t);
+        break;
+    case P_MIXED:
+        value = j > f.ny / 2 ? x : energy(f, i, j, x, y);
+        rgb = colour(f, f.scheme, false, value, sc, t);
+        break;
+    case P_MEAN_ENERGY:
+        te = gc(f.te0)[b] + energy(f, i, j, x, y);
+        value = te / (double)(t + 1);
+        rgb = colour(f, f.scheme, f.asym, value, sc, t);
+        break;
+    case P_LOG_ENERGY:
+        e = energy(f, i, j, x, y);
+        if (e < 1.0e-20) e = 1.0e-20;
+        value = f.log_shift + f.log_scale * log(e);
+        rgb = colour(f, f.scheme, false, value, sc, t);
+        break;
+    case P_LOG_MEAN_ENERGY:
+        e = energy(f, i, j, x, y);
+        if (e == 0.0) e = 1.0e-20;
+        te = gc(f.te0)[b] + e;
+        value = f.log_shift + f.log_scale * log(te / (double)(t + 1));
+        rgb = colour(f, f.scheme, false, value, sc, t);
+        break;
+    case P_ENERGY_FLUX:
+    case P_TOTAL_ENERGY_FLUX: {
+        // compute_energy_flux, wave_common.c:475-512
+        const double vel = (x - y) < 0.0 ? -(x - y) : (x - y);
+        const int ip = i + 1 == f.nx ? f.nx - 1 : i + 1, im = i == 0 ? 0 : i - 1;
+        const int jp = j + 1 == f.ny ? f.ny - 1 : j + 1, jm = j == 0 ? 0 : j - 1;
+        const double gx = phi_at(f, ip, j) - phi_at(f, im, j);
+        const double gy = phi_at(f, i, jp) - phi_at(f, i, jm);
+        const bool on = nz_at(f, i, j) || f.twospeeds;
+        const double mod = on ? vel * sqrt(gx * gx + gy * gy) : 0.0;
+        if (f.plot == P_ENERGY_FLUX) {
+            value = mod * f.flux_scale;
+            rgb = colour(f, f.scheme, true, value, sc, t);
+            break;
+        }
+        tfx = gc(f.tf0)[2 * b] * 0.99;
+        tfy = gc(f.tf0)[2 * b + 1] * 0.99;
+        tfx = tfx + (on ? vel * gx : 0.0);
+        tfy = tfy + (on ? vel * gy : 0.0);
+        const double m = sqrt(tfx * tfx + tfy * tfy);
+        double arg = argument(tfx, tfy);
+        if (arg < 0.0) arg += REF_DPI;
+        value = arg / REF_DPI;
+        rgb = colour(f, C_ONEDIM_LINEAR, false, value, 1.0, 1);
+        const double ff = tanh(m * f.flux_scale);
+        for (int c = 0; c < 3; c++) rgb.c[c] *= ff;
+        break;
+    }
+    default:  // P_AVERAGE_ENERGY, P_LOG_AVERAGE_ENERGY
+        e = energy(f, i, j, x, y);
+        ae = f.avrg * gc(f.ae0)[b] + (1.0 - f.avrg) * e;
+        if (f.plot == P_AVERAGE_ENERGY) {
+            value = ae;
+            rgb = colour(f, f.scheme, f.asym, value, sc, t);
+        } else {
+            value = ae == 0.0 ? f.log_shift - 10.0 * f.log_scale : f.log_shift + f.log_scale * log(ae);
+            rgb = colour(f, f.scheme, false, value, sc, t);
+        }
+        break;
+    }
+    if (f.rescale) {
+        const double c = gc(f.cs)[(size_t)i * f.ldy + j];
+        value *= c;
+        for (int q = 0; q < 3; q++) rgb.c[q] = 1.0 + c * (rgb.c[q] - 1.0);
+    }
+    return value;
+}
+
+__global__ void __launch_bounds__(256) k_frame(const WvFrame f)
+{
+    const int bj = blockIdx.x * blockDim.x + threadIdx.x, bi = blockIdx.y;
+    if (bj >= f.nby || bi >= f.nbx) return;
+    const size_t b = (size_t)bi * f.nby + bj;
+    Rgb rgb;
+    double te = 0, ae = 0, tfx = 0, tfy = 0;
+    const double value = wave_value(f, bi, bj, rgb, te, ae, tfx, tfy);
+    if (f.plot == P_MEAN_ENERGY || f.plot == P_LOG_MEAN_ENERGY) gm(f.te1)[b] = te;
+    if (f.plot >= P_AVERAGE_ENERGY) gm(f.ae1)[b] = ae;
+    if (f.plot == P_TOTAL_ENERGY_FLUX) {
+        gm(f.tf1)[2 * b] = tfx;
+        gm(f.tf1)[2 * b + 1] = tfy;
+    }
+    const int i = bi * f.size, j = bj * f.size;
+    // SHADE_2D, wave_common.c:1772-1791: gradient towards the next block cells
+    if (f.shade && i < f.nx - f.size && j < f.ny - f.size) {
+        Rgb dummy;
+        double d0 = 0, d1 = 0, d2 = 0, d3 = 0;
+        const double ve = wave_value(f, bi + 1, bj, dummy, d0, d1, d2, d3);
+        const double vn = wave_value(f, bi, bj + 1, dummy, d0, d1, d2, d3);
+        const double gx = ve - value, gy = vn - value;
+        const double norm = sqrt(f.shade_scale * f.shade_scale + gx * gx + gy * gy);
+        const double pscal = -gx * f.light_x - gy * f.light_y + f.shade_scale;
+        double ca = pscal / norm;
+        ca = (ca + 1.0) * 0.4 + 0.2;
+        for (int q = 0; q < 3; q++) {
+            if (f.rescale)
+                rgb.c[q] = 1.0 + (ca * rgb.c[q] - 1.0) * (gc(f.cs)[(size_t)i * f.ldy + j] + 0.01) * (1.0 / 1.01);
+            else
+                rgb.c[q] *= ca;
+        }
+    }
+    // drawing, wave_common.c:1797-1820
+    const bool drawn = f.twospeeds || nz_at(f, i, j);
+    if (drawn) {
+        if (f.fadeobs) {
+            const double fd = gc(f.ft)[(size_t)i * f.ldy + j];
+            for (int q = 0; q < 3; q++) rgb.c[q] *= fd;
+        }
+        if (f.fade)
+            for (int q = 0; q < 3; q++) rgb.c[q] *= f.fade_value;
+    }
+    f.out[b] = make_float4((float)rgb.c[0], (float)rgb.c[1], (float)rgb.c[2], drawn ? 1.0f : 0.0f);
+}
+
+}  // namespace
+
+int wv_launch_frame(const WvFrame &f, hipStream_t s)
+{
+    const dim3 block(256), grid((f.nby + 255) / 256, f.nbx);
+    hipLaunchKernelGGL(k_frame, grid, block, 0, s, f);
+    return (int)hipGetLastError();
+}

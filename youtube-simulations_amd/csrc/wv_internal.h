@@ -117,3 +117,25 @@ int wv_launch_energy(int split, const WvInst *d_inst, const int2 *d_rows, int nr
                      double *d_partial, const int *d_row0, int ninst, double *d_out, int ia, int ib,
                      hipStream_t s);
 int wv_launch_bump(long *gstep, long n, hipStream_t s);
+
+// Frame colours (wv_frame.hip): one frame of the drawing path's colours for
+// the block cells (i, j) = (bi*size, bj*size), bi < nbx, bj < nby.  Running
+// sums are double-buffered: read from *0, written to *1 (compact block
+// layout, tf two per block).  out[bi*nby + bj] = (r, g, b, drawn).
+struct WvFrame {
+    int plot, scheme, palette, asym, size, time, shade, rescale, fadeobs, fade;
+    long nsteps;
+    double scale, fade_value, slope, color_range, attenuation, vshift, vscale, log_shift, log_scale;
+    double flux_scale, avrg, colorhue, colordrift, lummean, lumamp, huemean, hueamp;
+    double shade_scale, light_x, light_y, courant, courantb, es;
+    int nx, ny, ldy, ldw, nbx, nby, twospeeds, lut_n;
+    const double *phi, *psi;             // FP64 form
+    const float *hi, *lo, *v;            // split form (phi = hi + lo, psi = phi - v)
+    const uint64_t *nz;                  // xy_in != 0
+    const double *cs, *ft;               // color_scale, fade_table (NX x ldy) or null
+    const float *lut;                    // palette table, lut_n x 3
+    const double *te0, *ae0, *tf0;
+    double *te1, *ae1, *tf1;
+    float4 *out;
+};
+int wv_launch_frame(const WvFrame &f, hipStream_t s);

@@ -30,7 +30,7 @@ EXPORTS = [
     "wv_set_state", "wv_set_state3", "wv_get_state", "wv_get_state3", "wv_set_sources",
     "wv_set_neumann", "wv_set_drive", "wv_set_floor", "wv_set_laplacian",
     "wv_step", "wv_step_timed", "wv_sync", "wv_steps_done", "wv_last_kernel_ms", "wv_energy",
-    "wv_get_tiling",
+    "wv_get_tiling", "wv_frame_params_default", "wv_frame_set_tables", "wv_frame_reset", "wv_frame",
     "wv_set_tiling", "wv_autotune", "wv_evolve_wave_half_rows", "wv_evolve_wave_rows",
     "wv_evolve_wave_half_flat", "wv_evolve_wave_flat", "wv_batch_create", "wv_batch_destroy",
     "wv_batch_last_error", "wv_batch_set_tables", "wv_batch_set_state", "wv_batch_get_state",
@@ -52,6 +52,18 @@ class WvParams(ctypes.Structure):
         ("tb_depth", ctypes.c_int), ("rows_per_tile", ctypes.c_int), ("device", ctypes.c_int),
         ("flags", ctypes.c_int),
     ]
+
+
+class WvFrameParams(ctypes.Structure):
+    """wv_frame_params of include/wavehip.h"""
+    _fields_ = [(n, ctypes.c_int) for n in ("plot", "scheme", "palette", "asym", "size", "time", "shade",
+                                            "rescale_center", "fade_in_obstacle", "fade")] + \
+               [("nsteps", ctypes.c_long)] + \
+               [(n, ctypes.c_double) for n in ("scale", "fade_value", "slope", "color_range", "attenuation",
+                                               "vshift_amplitude", "vscale_amplitude", "log_shift", "log_scale",
+                                               "flux_scale", "avrg_e_factor", "colorhue", "colordrift",
+                                               "lummean", "lumamp", "huemean", "hueamp", "shade_scale",
+                                               "light_x", "light_y", "courant", "courantb")]
 
 
 class WaveError(RuntimeError):
@@ -98,6 +110,10 @@ def load():
         "wv_last_kernel_ms": (ctypes.c_int, [vp, _D, _D]),
         "wv_batch_last_kernel_ms": (ctypes.c_int, [vp, _D, _D]),
         "wv_energy": (ctypes.c_int, [vp, _D]),
+        "wv_frame_params_default": (None, [ctypes.POINTER(WvFrameParams)]),
+        "wv_frame_set_tables": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int, _D, _D]),
+        "wv_frame_reset": (ctypes.c_int, [vp]),
+        "wv_frame": (ctypes.c_int, [vp, ctypes.POINTER(WvFrameParams), ctypes.POINTER(ctypes.c_float)]),
         "wv_get_tiling": (ctypes.c_int, [vp, _I, _I, _I]),
         "wv_set_tiling": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
         "wv_autotune": (ctypes.c_int, [vp, ctypes.c_long, _D]),
@@ -239,6 +255,37 @@ class Wave:
 
     def set_floor(self, vmax):
         self._chk(self._L.wv_set_floor(self._h, float(vmax)), "wv_set_floor")
+
+    def frame_params(self, **kw):
+        """wv_frame_params with the reference defaults, fields overridden by kw"""
+        f = WvFrameParams()
+        self._L.wv_frame_params_default(ctypes.byref(f))
+        for k, v in kw.items():
+            setattr(f, k, v)
+        return f
+
+    def set_frame_tables(self, lut=None, color_scale=None, fade_table=None):
+        F = ctypes.POINTER(ctypes.c_float)
+        lt = None if lut is None else np.ascontiguousarray(lut, np.float32)
+        cs = None if color_scale is None else np.ascontiguousarray(color_scale, np.float64)
+        ft = None if fade_table is None else np.ascontiguousarray(fade_table, np.float64)
+        self._chk(self._L.wv_frame_set_tables(self._h, None if lt is None else lt.ctypes.data_as(F),
+                                              0 if lt is None else lt.shape[0],
+                                              None if cs is None else cs.ctypes.data_as(_D),
+                                              None if ft is None else ft.ctypes.data_as(_D)), "wv_frame_set_tables")
+
+    def frame_reset(self):
+        self._chk(self._L.wv_frame_reset(self._h), "wv_frame_reset")
+
+    def frame(self, f):
+        """one frame: rgba[nbx, nby, 4] float32 (alpha 1 where drawn)"""
+        z = max(f.size, 1)  # the library rejects size < 1
+        nbx = -(-self.p.nx // z)
+        nby = -(-self.p.ny // z)
+        out = np.empty((nbx, nby, 4), np.float32)
+        self._chk(self._L.wv_frame(self._h, ctypes.byref(f), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+                  "wv_frame")
+        return out
 
     def set_laplacian(self, table):
         """neighbour-table stepping: table = (nneighb[NX,NY], nghb[NX,NY,4]) of
