@@ -61,6 +61,44 @@ def run_form(wh, w, form, steps, warmup, tb_depth=0, rows=0, tune=False):
                 u=u, k=k, L=L, cols=cols)
 
 
+def frame_bench(wh, w, reps=10):
+    """The frame consumer on the same grid: the drawing configuration of the
+    checked-in wave_billiard.c (HIGHRES blocks of 2, P_AMPLITUDE, C_ONEDIM,
+    COL_TWILIGHT_SHIFTED, SHADE_2D, FADE_IN_OBSTACLE with the fade table
+    pow(tcc / courant^2, 0.1) of wave_billiard.c:1745) as an 8-bit frame,
+    against downloading phi and psi for the host to colour; wall clock per
+    frame, host buffers preallocated (the table is a synthetic 511-entry one:
+    colours do not change the work)."""
+    nx, ny = w["params"]["nx"], w["params"]["ny"]
+    W = wh.Wave(w["params"], form=wh.FORM_FP64)
+    W.load(w)
+    W.step(100)
+    W.sync()
+    lut = np.random.default_rng(0).uniform(0, 1, (511, 3)).astype(np.float32)
+    fade = np.where(w["xy_in"] != 0, 1.0, np.power(w["tcc"] / 0.01, 0.1))
+    W.set_frame_tables(lut=lut, fade_table=fade)
+    out = {}
+    for size in (2, 1):
+        f = W.frame_params(plot=0, scheme=3, palette=18, size=size, shade=1, fade_in_obstacle=1)
+        buf = W.frame(f, u8=True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            W.frame(f, u8=True, out=buf)
+        out["rgba8_ms_size%d" % size] = (time.perf_counter() - t0) / reps * 1e3
+    phi, psi = np.empty((nx, ny)), np.empty((nx, ny))
+    W.get_state(out=(phi, psi))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        W.get_state(out=(phi, psi))
+    out["get_state_ms"] = (time.perf_counter() - t0) / reps * 1e3
+    W.close()
+    out["bytes_rgba8_size2"] = 4 * (-(-nx // 2)) * (-(-ny // 2))
+    out["bytes_state"] = 16 * nx * ny
+    out["workload"] = ("%dx%d FP64 state, P_AMPLITUDE / C_ONEDIM / COL_TWILIGHT_SHIFTED, SHADE_2D, "
+                       "FADE_IN_OBSTACLE, 8-bit RGBA" % (nx, ny))
+    return out
+
+
 def cpu_baseline(w, target_s=10.0, chunk=40):
     """the oracle on the host cores, a bounded sample of the FP64 workload:
     chunks of steps until target_s seconds of CPU work have been timed"""
@@ -241,6 +279,7 @@ def main():
     ap.add_argument("--tune", action="store_true", help="autotune tiling before timing")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-frame", action="store_true")
     ap.add_argument("--only-fp64", action="store_true")
     ap.add_argument("--configs", default="", help="comma list of c1..c5: run those workloads in both "
                     "forms and print their report instead of the benchmark line")
@@ -348,6 +387,12 @@ def main():
             "energy_rel_diff": (s["e1"] - f["e1"]) / f["e0"],
             "steps": args.steps,
         }
+    if not args.no_frame:
+        log("timing the frame consumer ...")
+        try:
+            out["frame"] = frame_bench(wh, w)
+        except Exception as ex:  # the frame is an extra: report, do not lose the line
+            out["frame"] = {"error": str(ex)}
     if not args.no_cpu:
         log("timing the CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(w)

@@ -212,6 +212,10 @@ int wv_frame_reset(wv_ctx *c);
 /* one frame (updates the running sums of its plot type; a change of size
  * starts them from zero); rgba: 4*nbx*nby floats */
 int wv_frame(wv_ctx *c, const wv_frame_params *f, float *rgba);
+/* the same frame as an 8-bit colour buffer holds it: rgba[4*(bi*nby + bj) ..]
+ * = (q(r), q(g), q(b), drawn ? 255 : 0), q(c) = (int)(min(max(c, 0), 1)*255 + 0.5)
+ * in binary32 on the float colour; 4 bytes per block */
+int wv_frame_rgba8(wv_ctx *c, const wv_frame_params *f, unsigned char *rgba);
 
 /* What wv_step launches: fused steps per launch, rows per tile, tile columns */
 int wv_get_tiling(const wv_ctx *c, int *tb_depth, int *rows_per_tile, int *tile_cols);

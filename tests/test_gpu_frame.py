@@ -121,3 +121,23 @@ def test_frame_running_sums_reset(wh):
     c = W.frame(f)
     W.close()
     assert not np.array_equal(a, b) and np.array_equal(a, c)
+
+
+def test_frame_rgba8_quantizes_the_float_frame(wh):
+    """wv_frame_rgba8 holds the float frame quantized as an 8-bit colour
+    buffer: (int)(min(max(c, 0), 1)*255 + 0.5) in binary32, alpha 255 where
+    drawn."""
+    from wavehip import workloads
+    w = workloads.random_case(130, 70, 6)
+    w["phi"] *= 0.05
+    W = wh.Wave(dict(w["params"], e_scale=50.0))
+    W.set_tables(w["xy_in"], w["tcc"], w["tc"], w["tgamma"])
+    W.set_state(w["phi"], w["psi"] * 0.05)
+    for kw in (dict(plot=0, shade=1), dict(plot=1, scheme=0, size=2), dict(plot=4, scheme=1, size=3)):
+        f = W.frame_params(**kw)
+        a = W.frame(f)
+        b = W.frame(f, u8=True)
+        q = (np.clip(a[..., :3], np.float32(0), np.float32(1)) * np.float32(255) + np.float32(0.5)).astype(np.uint8)
+        assert np.array_equal(b[..., :3], q), kw
+        assert np.array_equal(b[..., 3] == 255, a[..., 3] == 1.0) and set(np.unique(b[..., 3])) <= {0, 255}
+    W.close()

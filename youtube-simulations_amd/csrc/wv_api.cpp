@@ -1188,7 +1188,7 @@ int frame_reset(Engine *e, int k)
     return WV_OK;
 }
 
-int frame(Engine *e, int k, const wv_frame_params *f, float *rgba)
+int frame(Engine *e, int k, const wv_frame_params *f, void *rgba, bool u8)
 {
     if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
     if (!f || !rgba) return fail(e, WV_EINVAL, "frame parameters or output is NULL");
@@ -1215,7 +1215,7 @@ int frame(Engine *e, int k, const wv_frame_params *f, float *rgba)
     if (sch == 0 && f->nsteps == 0) return fail(e, WV_EINVAL, "C_LUM divides by nsteps");
     const int nx = G.p.nx, ny = G.p.ny, s = f->size;
     const int nbx = (nx + s - 1) / s, nby = (ny + s - 1) / s;
-    if (nbx > 65535) return fail(e, WV_EINVAL, "more than 65535 block rows");
+    if (nbx > 4 * 65535) return fail(e, WV_EINVAL, "more than 262140 block rows");
     HIPCHK(e, hipSetDevice(e->device));
     const size_t nb = (size_t)nbx * nby;
     if (G.fr_size != s) {
@@ -1255,12 +1255,14 @@ int frame(Engine *e, int k, const wv_frame_params *f, float *rgba)
     a.ae0 = G.fr_ae[G.fr_ae_cur]; a.ae1 = G.fr_ae[1 - G.fr_ae_cur];
     a.tf0 = G.fr_tf[G.fr_tf_cur]; a.tf1 = G.fr_tf[1 - G.fr_tf_cur];
     a.out = G.fr_out;
+    a.out8 = u8 ? (uchar4 *)G.fr_out : nullptr;  // the float buffer holds the 8-bit colours too
     int lr = wv_launch_frame(a, e->stream);
     if (lr) return fail(e, WV_EHIP, "kernel launch failed: %s", hipGetErrorString((hipError_t)lr));
     if (plot == 3 || plot == 5) G.fr_te_cur ^= 1;
     if (plot == 8 || plot == 9) G.fr_ae_cur ^= 1;
     if (plot == 7) G.fr_tf_cur ^= 1;
-    HIPCHK(e, hipMemcpyAsync(rgba, G.fr_out, sizeof(float4) * nb, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipMemcpyAsync(rgba, G.fr_out, (u8 ? sizeof(uchar4) : sizeof(float4)) * nb, hipMemcpyDeviceToHost,
+                             e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
     return WV_OK;
 }
@@ -1498,7 +1500,13 @@ int wv_frame_reset(wv_ctx *c)
 int wv_frame(wv_ctx *c, const wv_frame_params *f, float *rgba)
 {
     if (!c) return WV_EINVAL;
-    return frame(c->e, 0, f, rgba);
+    return frame(c->e, 0, f, rgba, false);
+}
+
+int wv_frame_rgba8(wv_ctx *c, const wv_frame_params *f, unsigned char *rgba)
+{
+    if (!c) return WV_EINVAL;
+    return frame(c->e, 0, f, rgba, true);
 }
 
 int wv_get_tiling(const wv_ctx *c, int *tb_depth, int *rows_per_tile, int *tile_cols)

@@ -5,13 +5,14 @@
 // wave_value (wave_common.c:1053-1171), shades the colours with the value
 // gradient (SHADE_2D), fades them and hands them to OpenGL.  Here one thread
 // owns one block cell and writes (r, g, b, drawn) as floats, the values
-// glColor3f receives, so a frame leaves the GPU as 16 bytes per block
-// instead of the 16 bytes per cell of phi and psi.
+// glColor3f receives, or as the 8-bit colours a frame buffer holds: a frame
+// leaves the GPU as 16 or 4 bytes per block instead of the 16 bytes per cell
+// of phi and psi.
 //
 // The running sums of the mean / average / flux plots are read from one
-// buffer and written to the other, so the shading gradient can evaluate the
-// neighbouring blocks' values (from their old sums and this frame's field)
-// in the same pass without racing with their owners.
+// buffer and written to the other, so a tile can evaluate the blocks of its
+// halo (from their old sums and this frame's field) in the same pass without
+// racing with their owners.
 //
 // Arithmetic follows the reference expression by expression (no FMA
 // contraction); tanh, exp, log, atan, sqrt and fmod are the device library's,
@@ -219,8 +220,9 @@ __device__ inline double argument(double x, double y)
 
 // The value and colour of block (bi, bj), wave_value (wave_common.c:1053-1171),
 // from the old running sums; the new sums go to te / ae / tf.
-__device__ double wave_value(const WvFrame &f, int bi, int bj, Rgb &rgb, double &te, double &ae, double &tfx,
-                             double &tfy)
+template <int PLOT>
+__device__ __forceinline__ double wave_value(const WvFrame &f, int bi, int bj, Rgb &rgb, double &te, double &ae,
+                                             double &tfx, double &tfy)
 {
     const int i = bi * f.size, j = bj * f.size;
     const size_t b = (size_t)bi * f.nby + bj;
@@ -228,7 +230,7 @@ __device__ double wave_value(const WvFrame &f, int bi, int bj, Rgb &rgb, double 
     const double sc = f.scale;
     const int t = f.time;
     double value, e;
-    switch (f.plot) {
+    switch (PLOT) {
     case P_AMPLITUDE:
         value = x;
         rgb = colour(f, f.scheme, false, f.vshift + f.vscale * value, sc, t);
@@ -269,7 +271,7 @@ __device__ double wave_value(const WvFrame &f, int bi, int bj, Rgb &rgb, double 
         const double gy = phi_at(f, i, jp) - phi_at(f, i, jm);
         const bool on = nz_at(f, i, j) || f.twospeeds;
         const double mod = on ? vel * sqrt(gx * gx + gy * gy) : 0.0;
-        if (f.plot == P_ENERGY_FLUX) {
+        if (PLOT == P_ENERGY_FLUX) {
             value = mod * f.flux_scale;
             rgb = colour(f, f.scheme, true, value, sc, t);
             break;
@@ -290,7 +292,7 @@ __device__ double wave_value(const WvFrame &f, int bi, int bj, Rgb &rgb, double 
     default:  // P_AVERAGE_ENERGY, P_LOG_AVERAGE_ENERGY
         e = energy(f, i, j, x, y);
         ae = f.avrg * gc(f.ae0)[b] + (1.0 - f.avrg) * e;
-        if (f.plot == P_AVERAGE_ENERGY) {
+        if (PLOT == P_AVERAGE_ENERGY) {
             value = ae;
             rgb = colour(f, f.scheme, f.asym, value, sc, t);
         } else {
@@ -307,57 +309,104 @@ __device__ double wave_value(const WvFrame &f, int bi, int bj, Rgb &rgb, double 
     return value;
 }
 
-__global__ void __launch_bounds__(256) k_frame(const WvFrame f)
+// A workgroup owns a tile of TY x (TX*NC) block cells, each thread NC
+// consecutive cells of one block row: the thread computes all of its values
+// and colours before it stores anything, so their loads are in flight
+// together (one cell per thread leaves too few bytes in flight to stream at
+// HBM rate).  Values go to LDS; the shading gradient reads the next block
+// row / column from there, and only the tile's halo (the row below, the
+// column to the right) is evaluated a second time, by the tile that needs
+// it, from the old running sums.
+constexpr int TX = 64, TY = 4, NC = 4, TW = TX * NC;
+
+template <int PLOT>
+__global__ void __launch_bounds__(TX * TY) k_frame(const WvFrame f)
 {
-    const int bj = blockIdx.x * blockDim.x + threadIdx.x, bi = blockIdx.y;
-    if (bj >= f.nby || bi >= f.nbx) return;
-    const size_t b = (size_t)bi * f.nby + bj;
-    Rgb rgb;
-    double te = 0, ae = 0, tfx = 0, tfy = 0;
-    const double value = wave_value(f, bi, bj, rgb, te, ae, tfx, tfy);
-    if (f.plot == P_MEAN_ENERGY || f.plot == P_LOG_MEAN_ENERGY) gm(f.te1)[b] = te;
-    if (f.plot >= P_AVERAGE_ENERGY) gm(f.ae1)[b] = ae;
-    if (f.plot == P_TOTAL_ENERGY_FLUX) {
-        gm(f.tf1)[2 * b] = tfx;
-        gm(f.tf1)[2 * b + 1] = tfy;
+    __shared__ double sv[TY + 1][TW + 1];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int bi = blockIdx.y * TY + ty, bj0 = blockIdx.x * TW + tx * NC;
+    Rgb rgb[NC];
+    double val[NC], te[NC], ae[NC], tfx[NC], tfy[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        rgb[c] = Rgb{};
+        val[c] = te[c] = ae[c] = tfx[c] = tfy[c] = 0.0;
+        if (bi < f.nbx && bj0 + c < f.nby) val[c] = wave_value<PLOT>(f, bi, bj0 + c, rgb[c], te[c], ae[c], tfx[c], tfy[c]);
     }
-    const int i = bi * f.size, j = bj * f.size;
-    // SHADE_2D, wave_common.c:1772-1791: gradient towards the next block cells
-    if (f.shade && i < f.nx - f.size && j < f.ny - f.size) {
+    double hcol = 0.0;
+    if (f.shade) {
+        // halo: row TY of the tile (threads of row 0), column TW (thread 0 of each row)
         Rgb dummy;
         double d0 = 0, d1 = 0, d2 = 0, d3 = 0;
-        const double ve = wave_value(f, bi + 1, bj, dummy, d0, d1, d2, d3);
-        const double vn = wave_value(f, bi, bj + 1, dummy, d0, d1, d2, d3);
-        const double gx = ve - value, gy = vn - value;
-        const double norm = sqrt(f.shade_scale * f.shade_scale + gx * gx + gy * gy);
-        const double pscal = -gx * f.light_x - gy * f.light_y + f.shade_scale;
-        double ca = pscal / norm;
-        ca = (ca + 1.0) * 0.4 + 0.2;
-        for (int q = 0; q < 3; q++) {
-            if (f.rescale)
-                rgb.c[q] = 1.0 + (ca * rgb.c[q] - 1.0) * (gc(f.cs)[(size_t)i * f.ldy + j] + 0.01) * (1.0 / 1.01);
-            else
-                rgb.c[q] *= ca;
+        const int hbi = blockIdx.y * TY + TY, hbj = blockIdx.x * TW + TW;
+        if (ty == 0 && hbi < f.nbx)
+            for (int c = 0; c < NC; c++)
+                if (bj0 + c < f.nby) sv[TY][tx * NC + c] = wave_value<PLOT>(f, hbi, bj0 + c, dummy, d0, d1, d2, d3);
+        if (tx == 0 && bi < f.nbx && hbj < f.nby) hcol = wave_value<PLOT>(f, bi, hbj, dummy, d0, d1, d2, d3);
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) sv[ty][tx * NC + c] = val[c];
+    if (tx == 0) sv[ty][TW] = hcol;
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const int bj = bj0 + c;
+        if (bi >= f.nbx || bj >= f.nby) continue;
+        const size_t b = (size_t)bi * f.nby + bj;
+        if (PLOT == P_MEAN_ENERGY || PLOT == P_LOG_MEAN_ENERGY) gm(f.te1)[b] = te[c];
+        if (PLOT >= P_AVERAGE_ENERGY) gm(f.ae1)[b] = ae[c];
+        if (PLOT == P_TOTAL_ENERGY_FLUX) {
+            gm(f.tf1)[2 * b] = tfx[c];
+            gm(f.tf1)[2 * b + 1] = tfy[c];
+        }
+        const int i = bi * f.size, j = bj * f.size;
+        // SHADE_2D, wave_common.c:1772-1791: gradient towards the next block cells
+        if (f.shade && i < f.nx - f.size && j < f.ny - f.size) {
+            const double gx = sv[ty + 1][tx * NC + c] - val[c], gy = sv[ty][tx * NC + c + 1] - val[c];
+            const double norm = sqrt(f.shade_scale * f.shade_scale + gx * gx + gy * gy);
+            const double pscal = -gx * f.light_x - gy * f.light_y + f.shade_scale;
+            double ca = pscal / norm;
+            ca = (ca + 1.0) * 0.4 + 0.2;
+            for (int q = 0; q < 3; q++) {
+                if (f.rescale)
+                    rgb[c].c[q] =
+                        1.0 + (ca * rgb[c].c[q] - 1.0) * (gc(f.cs)[(size_t)i * f.ldy + j] + 0.01) * (1.0 / 1.01);
+                else
+                    rgb[c].c[q] *= ca;
+            }
+        }
+        // drawing, wave_common.c:1797-1820
+        const bool drawn = f.twospeeds || nz_at(f, i, j);
+        if (drawn) {
+            if (f.fadeobs) {
+                const double fd = gc(f.ft)[(size_t)i * f.ldy + j];
+                for (int q = 0; q < 3; q++) rgb[c].c[q] *= fd;
+            }
+            if (f.fade)
+                for (int q = 0; q < 3; q++) rgb[c].c[q] *= f.fade_value;
+        }
+        const float r = (float)rgb[c].c[0], g = (float)rgb[c].c[1], bl = (float)rgb[c].c[2];
+        if (f.out8) {
+            // an 8-bit colour buffer: clamp to [0, 1], scale by 255, round to nearest
+            auto q = [](float x) { return (unsigned char)(fminf(fmaxf(x, 0.0f), 1.0f) * 255.0f + 0.5f); };
+            f.out8[b] = make_uchar4(q(r), q(g), q(bl), drawn ? 255 : 0);
+        } else {
+            f.out[b] = make_float4(r, g, bl, drawn ? 1.0f : 0.0f);
         }
     }
-    // drawing, wave_common.c:1797-1820
-    const bool drawn = f.twospeeds || nz_at(f, i, j);
-    if (drawn) {
-        if (f.fadeobs) {
-            const double fd = gc(f.ft)[(size_t)i * f.ldy + j];
-            for (int q = 0; q < 3; q++) rgb.c[q] *= fd;
-        }
-        if (f.fade)
-            for (int q = 0; q < 3; q++) rgb.c[q] *= f.fade_value;
-    }
-    f.out[b] = make_float4((float)rgb.c[0], (float)rgb.c[1], (float)rgb.c[2], drawn ? 1.0f : 0.0f);
 }
 
 }  // namespace
 
 int wv_launch_frame(const WvFrame &f, hipStream_t s)
 {
-    const dim3 block(256), grid((f.nby + 255) / 256, f.nbx);
-    hipLaunchKernelGGL(k_frame, grid, block, 0, s, f);
+    const dim3 block(TX, TY), grid((f.nby + TW - 1) / TW, (f.nbx + TY - 1) / TY);
+    switch (f.plot) {
+#define WV_PLOT(P) case P: hipLaunchKernelGGL(k_frame<P>, grid, block, 0, s, f); break;
+    WV_PLOT(0) WV_PLOT(1) WV_PLOT(2) WV_PLOT(3) WV_PLOT(4) WV_PLOT(5) WV_PLOT(6) WV_PLOT(7) WV_PLOT(8)
+    WV_PLOT(9)
+#undef WV_PLOT
+    default: return (int)hipErrorInvalidValue;
+    }
     return (int)hipGetLastError();
 }

@@ -121,7 +121,8 @@ int wv_launch_bump(long *gstep, long n, hipStream_t s);
 // Frame colours (wv_frame.hip): one frame of the drawing path's colours for
 // the block cells (i, j) = (bi*size, bj*size), bi < nbx, bj < nby.  Running
 // sums are double-buffered: read from *0, written to *1 (compact block
-// layout, tf two per block).  out[bi*nby + bj] = (r, g, b, drawn).
+// layout, tf two per block).  out[bi*nby + bj] = (r, g, b, drawn), or
+// out8[...] = the same quantized to 8 bits.
 struct WvFrame {
     int plot, scheme, palette, asym, size, time, shade, rescale, fadeobs, fade;
     long nsteps;
@@ -137,5 +138,6 @@ struct WvFrame {
     const double *te0, *ae0, *tf0;
     double *te1, *ae1, *tf1;
     float4 *out;
+    uchar4 *out8;                        // if set: 8-bit colours instead of out
 };
 int wv_launch_frame(const WvFrame &f, hipStream_t s);

@@ -31,6 +31,7 @@ EXPORTS = [
     "wv_set_neumann", "wv_set_drive", "wv_set_floor", "wv_set_laplacian",
     "wv_step", "wv_step_timed", "wv_sync", "wv_steps_done", "wv_last_kernel_ms", "wv_energy",
     "wv_get_tiling", "wv_frame_params_default", "wv_frame_set_tables", "wv_frame_reset", "wv_frame",
+    "wv_frame_rgba8",
     "wv_set_tiling", "wv_autotune", "wv_evolve_wave_half_rows", "wv_evolve_wave_rows",
     "wv_evolve_wave_half_flat", "wv_evolve_wave_flat", "wv_batch_create", "wv_batch_destroy",
     "wv_batch_last_error", "wv_batch_set_tables", "wv_batch_set_state", "wv_batch_get_state",
@@ -114,6 +115,7 @@ def load():
         "wv_frame_set_tables": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int, _D, _D]),
         "wv_frame_reset": (ctypes.c_int, [vp]),
         "wv_frame": (ctypes.c_int, [vp, ctypes.POINTER(WvFrameParams), ctypes.POINTER(ctypes.c_float)]),
+        "wv_frame_rgba8": (ctypes.c_int, [vp, ctypes.POINTER(WvFrameParams), ctypes.POINTER(ctypes.c_ubyte)]),
         "wv_get_tiling": (ctypes.c_int, [vp, _I, _I, _I]),
         "wv_set_tiling": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
         "wv_autotune": (ctypes.c_int, [vp, ctypes.c_long, _D]),
@@ -210,9 +212,10 @@ class Wave:
             t = _f64(tmp, self.shape)
             self._chk(self._L.wv_set_state3(self._h, _d(a), _d(b), _d(t)), "wv_set_state3")
 
-    def get_state(self, three=False):
-        a = np.empty(self.shape)
-        b = np.empty(self.shape)
+    def get_state(self, three=False, out=None):
+        """(phi, psi) [, tmp] as new arrays, or into the C-contiguous float64
+        arrays of out"""
+        a, b = (np.empty(self.shape), np.empty(self.shape)) if out is None else out[:2]
         if three:
             t = np.empty(self.shape)
             self._chk(self._L.wv_get_state3(self._h, _d(a), _d(b), _d(t)), "wv_get_state3")
@@ -277,12 +280,21 @@ class Wave:
     def frame_reset(self):
         self._chk(self._L.wv_frame_reset(self._h), "wv_frame_reset")
 
-    def frame(self, f):
-        """one frame: rgba[nbx, nby, 4] float32 (alpha 1 where drawn)"""
+    def frame(self, f, u8=False, out=None):
+        """one frame: rgba[nbx, nby, 4] float32 (alpha 1 where drawn), or
+        uint8 (alpha 255 where drawn) with u8; into out if given"""
         z = max(f.size, 1)  # the library rejects size < 1
         nbx = -(-self.p.nx // z)
         nby = -(-self.p.ny // z)
-        out = np.empty((nbx, nby, 4), np.float32)
+        if out is not None:
+            assert out.shape == (nbx, nby, 4) and out.dtype == (np.uint8 if u8 else np.float32)
+            assert out.flags.c_contiguous
+        if u8:
+            out = np.empty((nbx, nby, 4), np.uint8) if out is None else out
+            rc = self._L.wv_frame_rgba8(self._h, ctypes.byref(f), out.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
+            self._chk(rc, "wv_frame_rgba8")
+            return out
+        out = np.empty((nbx, nby, 4), np.float32) if out is None else out
         self._chk(self._L.wv_frame(self._h, ctypes.byref(f), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
                   "wv_frame")
         return out
