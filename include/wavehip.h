@@ -130,6 +130,18 @@ int wv_set_drive(wv_ctx *c, int n, const int *i, const int *j, const double *val
  * +0); vmax = 0 turns it off. */
 int wv_set_floor(wv_ctx *c, double vmax);
 
+/* Probes -- the time series of wave_billiard.c:900-908 (SAVE_TIME_SERIES
+ * writes phi at two cells after every evolve_wave call): after every
+ * every-th step, counted from this call, the value of each cell (i[k],
+ * j[k]) is recorded on the GPU, whatever the number of steps fused per
+ * launch; up to `capacity` samples per probe (FP64 form).  n = 0 removes
+ * them. */
+int wv_set_probes(wv_ctx *c, int n, const int *i, const int *j, long every, long capacity);
+/* the samples recorded since wv_set_probes or the last call, out[s*n + k]
+ * for s < *count (at most max_samples), then counting restarts after them;
+ * *lost (may be NULL): samples that did not fit the capacity */
+int wv_get_probes(wv_ctx *c, double *out, long max_samples, long *count, long *lost);
+
 /* Neighbour-table stepping, the reference's PRECOMPUTE_BC path:
  * evolve_wave_half_new (wave_3d.c:345-597) with the t_laplacian table built
  * by init_laplacian_coords (sub_wave.c:11555-11892) and read by

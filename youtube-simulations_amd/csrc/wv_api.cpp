@@ -70,6 +70,12 @@ struct Grid {
     double *fr_te[2] = {nullptr, nullptr}, *fr_ae[2] = {nullptr, nullptr}, *fr_tf[2] = {nullptr, nullptr};
     int fr_te_cur = 0, fr_ae_cur = 0, fr_tf_cur = 0, fr_size = 0;
     float4 *fr_out = nullptr;
+    // probes: CSR by row (column, id), sample buffer, sampling period and
+    // capacity, and where counting starts (device step counter; host steps)
+    int *prb_ptr = nullptr, *prb_col = nullptr, *prb_id = nullptr;
+    double *prb_buf = nullptr;
+    std::vector<int> h_pi, h_pj;
+    long prb_every = 0, prb_cap = 0, prb_base = 0, prb_host0 = 0;
 };
 
 struct Engine {
@@ -190,7 +196,7 @@ void free_grid(Grid &G)
                     G.src_ptr, G.src_col, G.src_id, G.amp, G.nz, G.neu_ptr, G.neu_col, G.neu_code,
                     G.neu_w, G.drv_ptr, G.drv_col, G.drv_id, G.drv_val, G.tbl_ptr, G.tbl_col, G.tbl_n,
                     G.tbl_off, G.fr_lut, G.fr_cs, G.fr_ft, G.fr_te[0], G.fr_te[1], G.fr_ae[0], G.fr_ae[1],
-                    G.fr_tf[0], G.fr_tf[1], G.fr_out};
+                    G.fr_tf[0], G.fr_tf[1], G.fr_out, G.prb_ptr, G.prb_col, G.prb_id, G.prb_buf};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
 }
@@ -210,6 +216,8 @@ WvInst make_inst(const Engine *e, const Grid &G)
     d.drv_len = G.drv_len; d.drv_base = G.drv_base;
     d.nz = G.nz;
     d.vmax = G.vmax;
+    d.prb_ptr = G.prb_ptr; d.prb_col = G.prb_col; d.prb_id = G.prb_id; d.prb_buf = G.prb_buf;
+    d.prb_n = (int)G.h_pi.size(); d.prb_every = G.prb_every; d.prb_cap = G.prb_cap; d.prb_base = G.prb_base;
     d.tbl = G.tbl;
     d.tbl_ptr = G.tbl_ptr; d.tbl_col = G.tbl_col; d.tbl_n = G.tbl_n; d.tbl_off = G.tbl_off;
     d.nx = G.p.nx; d.ny = G.p.ny; d.ldy = G.ldy; d.ldw = G.ldw;
@@ -287,7 +295,7 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
         const bool interior = js >= 1 && js + 64 * WV_C <= ny - 1 && ibeg >= k && iend <= nx - k && G.tbl != 2;
         // (wrapped halos: any source, Neumann point or driven cell of the grid)
         const int bflags = (G.h_si.empty() ? 0 : WV_TILE_SRC) | (G.h_ni.empty() ? 0 : WV_TILE_NEU) |
-                           (G.h_di.empty() ? 0 : WV_TILE_DRV);
+                           (G.h_di.empty() ? 0 : WV_TILE_DRV) | (G.h_pi.empty() ? 0 : WV_TILE_PRB);
         if (!interior) return bflags;
         int xflags = 0;
         for (size_t q = 0; q < G.h_ni.size() && !(xflags & WV_TILE_NEU); q++)
@@ -296,6 +304,9 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
         for (size_t q = 0; q < G.h_di.size() && !(xflags & WV_TILE_DRV); q++)
             if (G.h_di[q] >= ibeg - k && G.h_di[q] < iend + k && G.h_dj[q] >= js && G.h_dj[q] < js + 64 * WV_C)
                 xflags |= WV_TILE_DRV;
+        for (size_t q = 0; q < G.h_pi.size() && !(xflags & WV_TILE_PRB); q++)
+            if (G.h_pi[q] >= ibeg && G.h_pi[q] < iend && G.h_pj[q] >= js && G.h_pj[q] < js + 64 * WV_C)
+                xflags |= WV_TILE_PRB;
         // FLOOR and the two lists run on the boundary path of the kernels
         if (xflags || G.vmax > 0) return xflags | (G.h_si.empty() ? 0 : WV_TILE_SRC);
         bool all = !G.h_wr.empty();
@@ -966,6 +977,70 @@ int set_laplacian(Engine *e, int k, const int *nn, const int *nb)
     return sync_inst(e);
 }
 
+// ------------------------------------------------------------------ probes
+
+int set_probes(Engine *e, int k, int n, const int *pi, const int *pj, long every, long cap)
+{
+    if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
+    Grid &G = e->g[k];
+    if (e->form != WV_FORM_FP64 || WV_C != 1) return fail(e, WV_EUNSUPPORTED, "probes are provided for the FP64 form");
+    if (n < 0) return fail(e, WV_EINVAL, "negative probe count");
+    if (n > 0 && (!pi || !pj || every < 1 || cap < 1)) return fail(e, WV_EINVAL, "probe arrays, period or capacity");
+    if (n > 0 && (double)n * (double)cap > 1e9) return fail(e, WV_EINVAL, "probe buffer above 8 GB");
+    const int nx = G.p.nx, ny = G.p.ny;
+    for (int q = 0; q < n; q++)
+        if (pi[q] < 0 || pi[q] >= nx || pj[q] < 0 || pj[q] >= ny)
+            return fail(e, WV_EINVAL, "probe %d at (%d,%d) is outside the grid", q, pi[q], pj[q]);
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    std::vector<int> ptr, order, col(n), id(n);
+    csr_by_row(nx, n, pi, pj, ptr, order);
+    for (int q = 0; q < n; q++) { col[q] = pj[order[q]]; id[q] = order[q]; }
+    if (n == 0) ptr.clear();
+    int rc;
+    if ((rc = upload_vec(e, &G.prb_ptr, ptr)) || (rc = upload_vec(e, &G.prb_col, col)) ||
+        (rc = upload_vec(e, &G.prb_id, id)))
+        return rc;
+    if (G.prb_buf) { (void)hipFree(G.prb_buf); G.prb_buf = nullptr; }
+    if (n > 0) {
+        HIPCHK(e, hipMalloc(&G.prb_buf, sizeof(double) * (size_t)n * cap));
+        HIPCHK(e, hipMemset(G.prb_buf, 0, sizeof(double) * (size_t)n * cap));
+    }
+    long gnow = 0;
+    HIPCHK(e, hipMemcpy(&gnow, e->d_gstep, sizeof(long), hipMemcpyDeviceToHost));
+    G.h_pi.assign(pi, pi + n);
+    G.h_pj.assign(pj, pj + n);
+    G.prb_every = n ? every : 0;
+    G.prb_cap = n ? cap : 0;
+    G.prb_base = gnow;
+    G.prb_host0 = e->steps_done;
+    drop_tiles(e);
+    return sync_inst(e);
+}
+
+// copy the samples recorded since the probes were set or last read
+// (out[s*n + k], s < *count) and restart the numbering after them; *lost:
+// samples beyond the capacity, not recorded
+int get_probes(Engine *e, int k, double *out, long max_samples, long *count, long *lost)
+{
+    if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
+    Grid &G = e->g[k];
+    const long n = (long)G.h_pi.size();
+    if (!n) return fail(e, WV_ESTATE, "no probes set");
+    if (!out || !count || max_samples < 0) return fail(e, WV_EINVAL, "output is NULL");
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    const long taken = (e->steps_done - G.prb_host0) / G.prb_every;
+    const long have = std::min(taken, G.prb_cap);
+    if (have > max_samples) return fail(e, WV_EINVAL, "%ld samples recorded, room for %ld", have, max_samples);
+    if (have) HIPCHK(e, hipMemcpy(out, G.prb_buf, sizeof(double) * n * have, hipMemcpyDeviceToHost));
+    *count = have;
+    if (lost) *lost = taken - have;
+    G.prb_base += taken * G.prb_every;
+    G.prb_host0 += taken * G.prb_every;
+    return sync_inst(e);
+}
+
 bool any_exact(const Engine *e)
 {
     for (const auto &G : e->g)
@@ -1043,7 +1118,7 @@ int enqueue_steps(Engine *e, long nsteps, bool force_k1, long *launches)
     bool extras = false;
     for (const Grid &G : e->g) {
         if (G.coef != cm) cm = WV_CM_ALL;
-        extras = extras || !G.h_ni.empty() || !G.h_di.empty() || G.vmax > 0;
+        extras = extras || !G.h_ni.empty() || !G.h_di.empty() || G.vmax > 0 || !G.h_pi.empty();
     }
     if (extras && e->form == WV_FORM_FP64) cm = WV_CM_ALLX;  // the variant with those paths
     if (e->form == WV_FORM_FP64 && any_table(e)) cm = WV_CM_TBL;
@@ -1814,6 +1889,18 @@ extern "C" int wv_set_floor(wv_ctx *c, double vmax)
 {
     if (!c) return WV_EINVAL;
     return set_floor(c->e, 0, vmax);
+}
+
+extern "C" int wv_set_probes(wv_ctx *c, int n, const int *i, const int *j, long every, long capacity)
+{
+    if (!c) return WV_EINVAL;
+    return set_probes(c->e, 0, n, i, j, every, capacity);
+}
+
+extern "C" int wv_get_probes(wv_ctx *c, double *out, long max_samples, long *count, long *lost)
+{
+    if (!c) return WV_EINVAL;
+    return get_probes(c->e, 0, out, max_samples, count, lost);
 }
 
 extern "C" int wv_set_laplacian(wv_ctx *c, const int *nneighb, const int *nghb)

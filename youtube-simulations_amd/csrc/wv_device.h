@@ -318,10 +318,16 @@ struct Xtra {
     double vmax;
     const int *tptr, *tcol, *tn;
     const long long *toff;
+    const int *pptr, *pcol, *pid;
+    double *pbuf;
+    int pn;
+    long pevery, pcap, pbase;
     __device__ __forceinline__ explicit Xtra(const WvInst &d)
         : nptr(d.neu_ptr), ncol(d.neu_col), ncode(d.neu_code), nw(d.neu_w), dptr(d.drv_ptr),
           dcol(d.drv_col), did(d.drv_id), dval(d.drv_val), dlen(d.drv_len), dbase(d.drv_base), nz(d.nz),
-          ldw(d.ldw), vmax(d.vmax), tptr(d.tbl_ptr), tcol(d.tbl_col), tn(d.tbl_n), toff(d.tbl_off)
+          ldw(d.ldw), vmax(d.vmax), tptr(d.tbl_ptr), tcol(d.tbl_col), tn(d.tbl_n), toff(d.tbl_off),
+          pptr(d.prb_ptr), pcol(d.prb_col), pid(d.prb_id), pbuf(d.prb_buf), pn(d.prb_n), pevery(d.prb_every),
+          pcap(d.prb_cap), pbase(d.prb_base)
     {
     }
 };
@@ -360,6 +366,18 @@ __device__ inline bool drv_find(const Xtra &q, int rm, int jm, long g, double &v
     if (k < 0) return false;
     val = gc(q.dval)[(size_t)gc(q.did)[k] * q.dlen + g];
     return true;
+}
+
+// Probe at (rm, jm): record value v of step g (0-based, counted from the
+// probes' start) when g+1 is a multiple of the sampling period
+__device__ inline void prb_record(const Xtra &q, int rm, int jm, long g, double v)
+{
+    const long n = g + 1;
+    if (n <= 0 || n % q.pevery) return;
+    const long s = n / q.pevery - 1;
+    if (s >= q.pcap) return;
+    const int k = row_find(q.pptr, q.pcol, rm, jm);
+    if (k >= 0) gm(q.pbuf)[(size_t)s * q.pn + gc(q.pid)[k]] = v;
 }
 
 // Table Laplacian of cell (rm, jm), sub_wave.c:11908-11962: -n*x, then the

@@ -81,6 +81,13 @@ struct WvInst {
     int tbl;
     const int *tbl_ptr, *tbl_col, *tbl_n;
     const long long *tbl_off;
+    // probes (FP64 form): cells whose value is recorded after every
+    // prb_every-th step counted from prb_base, CSR by row (column, id); sample
+    // s of probe k at prb_buf[s*prb_n + k] while s < prb_cap
+    const int *prb_ptr, *prb_col, *prb_id;
+    double *prb_buf;
+    int prb_n;
+    long prb_every, prb_cap, prb_base;
     double tcc0, tcc1, tg0, tg1;         // coefficient pairs (modes UNIFORM / TWO)
     float tcch0, tccl0, tcch1, tccl1, tgf0, tgf1;
     double kappa, ks, gs, kt, gt, es2;
@@ -92,7 +99,7 @@ struct WvInst {
 #define WV_TILE_SRC 4       // a point source lies in the cells the tile computes
 #define WV_TILE_NEU 8       // ... a Neumann ghost point (never an INTERIOR tile)
 #define WV_TILE_DRV 16      // ... a driven cell (never an INTERIOR tile)
-#define WV_TILE_TBL 32      // ... a cell with a table Laplacian (never an INTERIOR tile)
+#define WV_TILE_PRB 32      // ... a probe (never an INTERIOR tile)
 struct WvTile {
     int inst, cb, sb, flags;
 };

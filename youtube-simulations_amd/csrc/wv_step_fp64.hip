@@ -76,6 +76,7 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
     const bool xneu = XT && !INTERIOR && WV_C == 1 && (t.flags & WV_TILE_NEU);
     const bool xdrv = XT && !INTERIOR && (t.flags & WV_TILE_DRV);
     const bool xfloor = XT && !INTERIOR && xq.vmax > 0.0;
+    const bool xprb = XT && !INTERIOR && (t.flags & WV_TILE_PRB);
     static_assert(!TB || (XT && K == 1 && ST == 5), "table Laplacians: 5-point, one step per launch");
     const int tblm = TB && !INTERIOR ? d.tbl : 0;  // 1: table Laplacians, 2: BC_LSHAPE
     const double *u0 = d.u[ia], *um1 = d.u[ib];
@@ -241,6 +242,17 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
                     wr[c] = w;
                     nu[c] = w ? v : x;  // cells the step never writes keep u
                     np[c] = x;
+                    // probes: the tile that writes the cell records it (at every level)
+                    if (xprb && r >= ibeg && r < iend && L.wcol[c]) {
+                        double q = nu[c];
+                        if (K == 1 && !w) {
+                            // one step per launch (three buffers): the cell keeps what the
+                            // output buffer holds, clamped by FLOOR where it applies
+                            q = out0[(size_t)r * ldy + L.jm[c]];
+                            if (nzc[c]) q = q > xq.vmax ? xq.vmax : (q < -xq.vmax ? -xq.vmax : q);
+                        }
+                        prb_record(xq, rm, L.jm[c], gbase + lv - 1 - xq.pbase, q);
+                    }
                 }
                 if (lv == K) {
                     // level K at row s-2K is final (the live test puts it inside the strip)

@@ -28,7 +28,7 @@ FLAG_FULL_FORMULA = 2
 EXPORTS = [
     "wv_params_default", "wv_create", "wv_destroy", "wv_last_error", "wv_set_tables",
     "wv_set_state", "wv_set_state3", "wv_get_state", "wv_get_state3", "wv_set_sources",
-    "wv_set_neumann", "wv_set_drive", "wv_set_floor", "wv_set_laplacian",
+    "wv_set_neumann", "wv_set_drive", "wv_set_floor", "wv_set_laplacian", "wv_set_probes", "wv_get_probes",
     "wv_step", "wv_step_timed", "wv_sync", "wv_steps_done", "wv_last_kernel_ms", "wv_energy",
     "wv_get_tiling", "wv_frame_params_default", "wv_frame_set_tables", "wv_frame_reset", "wv_frame",
     "wv_frame_rgba8",
@@ -104,6 +104,9 @@ def load():
         "wv_set_drive": (ctypes.c_int, [vp, ctypes.c_int, _I, _I, _D, ctypes.c_long]),
         "wv_set_floor": (ctypes.c_int, [vp, ctypes.c_double]),
         "wv_set_laplacian": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+        "wv_set_probes": (ctypes.c_int, [vp, ctypes.c_int, _I, _I, ctypes.c_long, ctypes.c_long]),
+        "wv_get_probes": (ctypes.c_int, [vp, _D, ctypes.c_long, ctypes.POINTER(ctypes.c_long),
+                                         ctypes.POINTER(ctypes.c_long)]),
         "wv_step": (ctypes.c_int, [vp, ctypes.c_long]),
         "wv_step_timed": (ctypes.c_int, [vp, ctypes.c_long, _D, ctypes.POINTER(ctypes.c_long)]),
         "wv_sync": (ctypes.c_int, [vp]),
@@ -298,6 +301,23 @@ class Wave:
         self._chk(self._L.wv_frame(self._h, ctypes.byref(f), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
                   "wv_frame")
         return out
+
+    def set_probes(self, i, j, every=3, capacity=4096):
+        """record u at cells (i[k], j[k]) after every `every`-th step"""
+        ii = np.ascontiguousarray(i, np.int32)
+        jj = np.ascontiguousarray(j, np.int32)
+        self._keep_probe_n = len(ii)
+        self._chk(self._L.wv_set_probes(self._h, len(ii), ii.ctypes.data_as(_I), jj.ctypes.data_as(_I), every,
+                                        capacity), "wv_set_probes")
+
+    def get_probes(self, max_samples=1 << 20):
+        """(samples[count, n], lost) recorded since the probes were set or last read"""
+        n = getattr(self, "_keep_probe_n", 0)
+        out = np.empty(max(max_samples * n, 1))
+        cnt, lost = ctypes.c_long(0), ctypes.c_long(0)
+        self._chk(self._L.wv_get_probes(self._h, _d(out), max_samples, ctypes.byref(cnt), ctypes.byref(lost)),
+                  "wv_get_probes")
+        return out[:cnt.value * n].reshape(cnt.value, n), lost.value
 
     def set_laplacian(self, table):
         """neighbour-table stepping: table = (nneighb[NX,NY], nghb[NX,NY,4]) of
