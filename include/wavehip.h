@@ -91,6 +91,21 @@ const char *wv_last_error(const wv_ctx *c);  /* never NULL; c may be NULL */
 int wv_set_tables(wv_ctx *c, const short *xy_in, const double *tcc, const double *tc,
                   const double *tgamma);
 
+/* Variable media of init_ior_2d's Gaussian-well models (IOR_PERIODIC_WELLS,
+ * IOR_RANDOM_WELLS, IOR_POISSON_WELLS, IOR_PPP_WELLS, sub_wave.c:12315-12555),
+ * the sum computed on the GPU: at cell (i, j), (x, y) of ij_to_xy
+ * (sub_wave.c:469-476) over box = (XMIN, XMAX, YMIN, YMAX),
+ *   s = sum_n exp(-r_n^2/sigma) * height[n]   (height NULL: without the factor),
+ *   s = tanh(s) if use_tanh, tcc = courant*s + courantb*(1 - s), tgamma = gamma,
+ * then tc = sqrt(tcc) where xy_in != 0, courantb where TWOSPEEDS (0 elsewhere),
+ * sub_wave.c:12970-12981.  The well centres and heights are the program's
+ * (drawn with its own generator).  The tables are installed as by
+ * wv_set_tables and copied to tcc / tc / tgamma (NX*NY) when not NULL.
+ * exp and tanh are the GPU's: tcc agrees with the reference to about 1e-16. */
+int wv_set_wells_tables(wv_ctx *c, const short *xy_in, int nwells, const double *xc, const double *yc,
+                        const double *height, double sigma, int use_tanh, double courant, double courantb,
+                        double gamma, const double box[4], double *tcc, double *tc, double *tgamma);
+
 /* u^n (phi) and u^{n-1} (psi); the third buffer of the rotation starts at 0. */
 int wv_set_state(wv_ctx *c, const double *phi, const double *psi);
 /* all three buffers of evolve_wave's rotation (phi, psi, tmp) */

@@ -7,8 +7,9 @@
 # compute_energy (wave_common.c:453-472) and the stepper (wave_billiard.c:302-631
 # or wave_3d.c:600-1007; for the neighbour-table stepper sub_wave.c:11555-11963
 # and wave_3d.c:345-597; for the frame colours sub_wave.c:270-304, colors_waves.c
-# with the colour tables it includes, wave_common.c:453-512 and 1053-1171),
-# then oracle/ref_harness.c.  Flags follow makefile:1-4
+# with the colour tables it includes, wave_common.c:453-512 and 1053-1171; for
+# the media tables sub_wave.c:270-304, 332-357, 469-476, 721-1387,
+# 12164-13004), then oracle/ref_harness.c.  Flags follow makefile:1-4
 # (gcc -O3 -fopenmp) plus -ffp-contract=off.  Needs the reference sources at
 # /root/reference; machines without them use the oracle alone.
 set -euo pipefail
@@ -18,7 +19,7 @@ OUT="$HERE/_ref"
 [ -f "$REF/wave_billiard.c" ] || { echo "reference not found at $REF" >&2; exit 1; }
 mkdir -p "$OUT"
 
-build_one() {  # build_one <tag> <0 row pointers | 1 flat | 2 neighbour table | 3 colours> <gcc -D flags...>
+build_one() {  # build_one <tag> <0 row pointers | 1 flat | 2 neighbour table | 3 colours | 4 media> <gcc -D flags...>
     local tag=$1 flat=$2; shift 2
     local so="$OUT/libref_${tag}.so"
     {
@@ -26,7 +27,10 @@ build_one() {  # build_one <tag> <0 row pointers | 1 flat | 2 neighbour table | 
         echo "#include \"$REF/global_pdes.c\""
         sed -n '412,422p' "$REF/sub_wave.c"
         sed -n '11965,12162p' "$REF/sub_wave.c"
-        if [ "$flat" = 3 ]; then
+        if [ "$flat" = 4 ]; then
+            echo "#define IOR_TWIN 1"
+            for r in 270,304 332,357 469,476 721,1387 12164,13004; do sed -n "${r}p" "$REF/sub_wave.c"; done
+        elif [ "$flat" = 3 ]; then
             echo "#define COLOR_TWIN 1"
             sed -n '270,304p' "$REF/sub_wave.c"
             echo "#include \"$REF/colors_waves.c\""
@@ -121,6 +125,12 @@ build_one "col_s3_asym" 3 $COL -DCOLOR_SCHEME=3 -DCOLOR_PALETTE=11 -DTWOSPEEDS=0
 pids+=($!)
 build_one "col_s3_att" 3 $COL -DCOLOR_SCHEME=3 -DCOLOR_RANGE=1.0 -DATTENUATION=0.01 \
     -DRESCALE_COLOR_IN_CENTER=1 -DTWOSPEEDS=1 &
+pids+=($!)
+for p in "${pids[@]}"; do wait "$p"; done; pids=()
+# the Gaussian-well media of init_ior_2d (sub_wave.c:12315-12389)
+build_one "ior_wells_p" 4 -DNX=40 -DNY=30 -DIOR=4 -DNGRIDX=6 -DNGRIDY=5 -DTWOSPEEDS=0 -DCOURANTB=0.05 &
+pids+=($!)
+build_one "ior_wells_r" 4 -DNX=40 -DNY=30 -DIOR=5 -DNGRIDX=6 -DNGRIDY=5 -DTWOSPEEDS=1 -DCOURANTB=0.05 &
 pids+=($!)
 for p in "${pids[@]}"; do wait "$p"; done; pids=()
 # a lens medium with damping and absorbing edges (energy fixture, two speeds)

@@ -1,0 +1,40 @@
+/*
+ * media_oracle.c -- CPU oracle of the Gaussian-well media tables (test
+ * infrastructure, see wave_oracle.h).  Same flags as wave_oracle.c; exp,
+ * tanh and sqrt are glibc's, as for the reference.
+ */
+#include <math.h>
+#include <stddef.h>
+
+#include "wave_oracle.h"
+
+/* init_ior_2d, sub_wave.c:12315-12389 (IOR_PERIODIC_WELLS, IOR_RANDOM_WELLS;
+ * IOR_POISSON_WELLS and IOR_PPP_WELLS 12482-12555 sum the same way over
+ * their circles), then the tc rule of sub_wave.c:12970-12981 */
+void orc_wells_tables(const orc_params *p, const short *xy, int nw, const double *xc, const double *yc,
+                      const double *h, double sigma, int use_tanh, double courant, double courantb,
+                      double gamma, const double box[4], double *tcc, double *tc, double *tg)
+{
+    const int NX = p->nx, NY = p->ny;
+    const double xmin = box[0], xmax = box[1], ymin = box[2], ymax = box[3];
+    int i, j;
+#pragma omp parallel for private(i, j)
+    for (i = 0; i < NX; i++)
+        for (j = 0; j < NY; j++) {
+            /* ij_to_xy, sub_wave.c:469-476 */
+            const double x = xmin + ((double)i) * (xmax - xmin) / ((double)NX);
+            const double y = ymin + ((double)j) * (ymax - ymin) / ((double)NY);
+            double sum = 0.0;
+            for (int n = 0; n < nw; n++) {
+                const double r2 = (x - xc[n]) * (x - xc[n]) + (y - yc[n]) * (y - yc[n]);
+                if (h) sum += exp(-r2 / (sigma)) * h[n];
+                else sum += exp(-r2 / (sigma));
+            }
+            if (use_tanh) sum = tanh(sum);
+            const size_t k = (size_t)i * NY + j;
+            tcc[k] = courant * sum + courantb * (1.0 - sum);
+            tg[k] = gamma;
+            if (xy[k] != 0) tc[k] = sqrt(tcc[k]);
+            else if (p->twospeeds) tc[k] = courantb;
+        }
+}

@@ -121,6 +121,8 @@ def lib():
                                 ctypes.c_int, _D, _D, ctypes.POINTER(ctypes.c_ubyte)]
         L.orc_color.argtypes = [ctypes.POINTER(OrcFrameParams), ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                 ctypes.c_double, ctypes.c_int, _F, ctypes.c_int, _D]
+        L.orc_wells_tables.argtypes = [P, _S, ctypes.c_int, _D, _D, _D, ctypes.c_double, ctypes.c_int,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _D, _D, _D]
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_split_energy_total.restype = ctypes.c_double
         L.orc_split_energy_total.argtypes = [P, _F, _F, _F, _S, _D, ctypes.c_double]
@@ -292,6 +294,21 @@ def frame(p, f, phi, psi, xy_in, acc, color_scale=None, fade_table=None, lut=Non
     return values, rgb, drawn.astype(bool)
 
 
+def wells_tables(p, xy_in, xc, yc, h, sigma, use_tanh, courant, courantb, gamma, box):
+    """orc_wells_tables: (tcc, tc, tgamma); tc is 0 where the rule leaves it"""
+    nx, ny = p.nx, p.ny
+    xy = np.ascontiguousarray(xy_in, np.int16)
+    xc = np.ascontiguousarray(xc, np.float64)
+    yc = np.ascontiguousarray(yc, np.float64)
+    hh = None if h is None else np.ascontiguousarray(h, np.float64)
+    bx = np.ascontiguousarray(box, np.float64)
+    tcc, tc, tg = np.zeros((nx, ny)), np.zeros((nx, ny)), np.zeros((nx, ny))
+    lib().orc_wells_tables(ctypes.byref(p), _p(xy, _S), len(xc), _p(xc, _D), _p(yc, _D),
+                           None if hh is None else _p(hh, _D), sigma, int(use_tanh), courant, courantb, gamma,
+                           _p(bx, _D), _p(tcc, _D), _p(tc, _D), _p(tg, _D))
+    return tcc, tc, tg
+
+
 def _sources(sources):
     if not sources:
         z = np.zeros(1, np.int32)
@@ -364,6 +381,11 @@ class RefBuild:
         self.ny = self.L.ref_ny()
         self.flat = hasattr(self.L, "ref_evolve_wave_half_flat")
         self.table = hasattr(self.L, "ref_evolve_wave_half_new")
+        if hasattr(self.L, "ref_init_ior"):
+            self.L.ref_init_ior.argtypes = [_S, _D, _D, _D, ctypes.c_uint]
+            self.L.ref_wells.restype = ctypes.c_double
+            self.L.ref_wells.argtypes = [ctypes.c_uint, _D, _D, _D]
+            return
         self.color = hasattr(self.L, "ref_wave_value")
         if self.color:
             self.L.ref_wave_value.restype = ctypes.c_double
@@ -406,6 +428,20 @@ class RefBuild:
         v = self.L.ref_wave_value(_p(phi, _D), _p(psi, _D), _p(acc.te, _D), _p(acc.ae, _D), _p(acc.tf, _D),
                                   _p(cs, _D), _p(xy_in, _S), scale, time, plot, palette, i, j, _p(rgb, _D))
         return v, rgb
+
+    def init_ior(self, xy_in, seed):
+        """init_ior_2d's tables (tcc, tc, tgamma), tables zeroed first"""
+        xy = np.ascontiguousarray(xy_in, np.int16)
+        t = [np.zeros((self.nx, self.ny)) for _ in range(3)]
+        self.L.ref_init_ior(_p(xy, _S), _p(t[0], _D), _p(t[1], _D), _p(t[2], _D), seed)
+        return tuple(t)
+
+    def wells(self, seed):
+        """the well centres, heights and sigma init_ior_2d uses"""
+        n = self.L.ref_nwells()
+        xc, yc, h = np.zeros(n), np.zeros(n), np.zeros(n)
+        sigma = self.L.ref_wells(seed, _p(xc, _D), _p(yc, _D), _p(h, _D))
+        return xc, yc, h, sigma
 
     def palette_table(self, palette):
         out = np.zeros((511, 3), np.float32)

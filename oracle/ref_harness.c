@@ -34,7 +34,48 @@ int ref_set_neumann(int n, const int *pi, const int *pj, const int *bi1, const i
 double ref_oscillating_bc(int time, int j) { return oscillating_bc(time, j); }
 int ref_beam_bc(void) { return BEAM_BC; }
 
-#if defined(COLOR_TWIN)
+#if defined(IOR_TWIN)
+/* init_ior_2d (sub_wave.c:12164) on flat tables, after srand(seed) */
+void ref_init_ior(short *xy_in, double *tcc, double *tc, double *tgamma, unsigned seed)
+{
+    double *t1[NX], *t2[NX], *t3[NX];
+    short int *m[NX];
+    for (int i = 0; i < NX; i++) {
+        m[i] = xy_in + (size_t)i * NY;
+        t1[i] = tcc + (size_t)i * NY; t2[i] = tc + (size_t)i * NY; t3[i] = tgamma + (size_t)i * NY;
+    }
+    srand(seed);
+    init_ior_2d(m, t1, t2, t3, 0.0);
+}
+
+int ref_nwells(void) { return NGRIDX * NGRIDY; }
+
+/* the wells init_ior_2d places: IOR_PERIODIC_WELLS (sub_wave.c:12317-12328) or
+ * IOR_RANDOM_WELLS (12352-12365, drawn with the reference's gaussian() from
+ * the rand() sequence of srand(seed)); height 1 for the periodic wells */
+double ref_wells(unsigned seed, double *xc, double *yc, double *h)
+{
+    const double dx = (XMAX - XMIN) / (double)NGRIDX, dy = (YMAX - YMIN) / (double)NGRIDY;
+    srand(seed);
+    for (int i = 0; i < NGRIDX; i++)
+        for (int j = 0; j < NGRIDY; j++) {
+            const int n = j * NGRIDX + i;
+            if (IOR == IOR_RANDOM_WELLS) {
+                xc[n] = XMIN + dx * ((double)i + 0.5 + 0.1 * gaussian());
+                yc[n] = YMIN + dy * ((double)j + 0.5 + 0.1 * gaussian());
+                h[n] = 0.5 + 0.5 * gaussian();
+                if (h[n] > 1.0) h[n] = 1.0;
+                if (h[n] < 0.0) h[n] = 0.0;
+            } else {
+                xc[n] = XMIN + dx * ((double)i + 0.5);
+                yc[n] = YMIN + dy * ((double)j + 0.5);
+                if (j % 2 == 1) yc[n] += 0.5 * dx;
+                h[n] = 1.0;
+            }
+        }
+    return 0.2 * dx * dx; /* sigma */
+}
+#elif defined(COLOR_TWIN)
 /* wave_value (wave_common.c:1053) at (i, j) on flat arrays; total_flux is
  * flat already (index 2*j*NX + 2*i) */
 double ref_wave_value(double *phi, double *psi, double *te, double *ae, double *tf, double *cs, short *xy,

@@ -147,3 +147,46 @@
 #ifndef RESCALE_COLOR_IN_CENTER
 #define RESCALE_COLOR_IN_CENTER 0
 #endif
+
+/* media models of init_ior_2d (sub_wave.c:12164-13004), for the media builds;
+ * the macros its other branches and init_circle_config_pattern read are set
+ * to the checked-in wave_billiard.c values or to any valid value (those
+ * branches do not run) */
+#ifndef VARIABLE_IOR
+#define VARIABLE_IOR 1
+#endif
+#ifndef IOR
+#define IOR 4
+#endif
+#ifndef NGRIDX
+#define NGRIDX 10
+#endif
+#ifndef NGRIDY
+#define NGRIDY 10
+#endif
+#ifndef GAMMA
+#define GAMMA 1.0e-4
+#endif
+#ifndef GAMMAB
+#define GAMMAB 2.0e-4
+#endif
+#define MANDELLEVEL 100
+#define MANDELLIMIT 10.0
+#define MANDEL_IOR_SCALE -0.05
+#define NMAXCIRCLES 10000
+#define HRES 1
+#define INITIAL_TIME 0
+#define WALL_WIDTH 0.1
+#define MU 0.1
+#define PDISC_FACTOR 3.0
+#define P_PERCOL 0.25
+#define NPOISSON 300
+#define X_SHOOTER -0.2
+#define X_TARGET 0.4
+#define Y_SHOOTER -0.6
+#define Y_TARGET 0.0
+#define RADIUS_FACTOR 0.3
+#define NPOLY 6
+#define APOLY 0.0
+#define MDEPTH 4
+

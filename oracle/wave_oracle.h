@@ -189,6 +189,17 @@ void orc_frame(const orc_params *p, const orc_frame_params *f, const double *phi
 void orc_color(const orc_frame_params *f, int scheme, int asym, double value, double scale, int time,
                const float *lut, int lut_n, double rgb[3]);
 
+/* ---- media tables ----------------------------------------------------------
+ * init_ior_2d's Gaussian-well models (sub_wave.c:12315-12555): at cell (i, j),
+ * (x, y) from ij_to_xy over box = (XMIN, XMAX, YMIN, YMAX),
+ *   s = sum_n exp(-r_n^2/sigma) * h[n]   (h = NULL: sum_n exp(-r_n^2/sigma)),
+ *   s = tanh(s) if use_tanh, tcc = courant*s + courantb*(1 - s), tgamma = gamma,
+ * then tc = sqrt(tcc) where xy_in != 0, courantb where TWOSPEEDS and
+ * xy_in == 0, untouched elsewhere (sub_wave.c:12970-12981). */
+void orc_wells_tables(const orc_params *p, const short *xy, int nw, const double *xc, const double *yc,
+                      const double *h, double sigma, int use_tanh, double courant, double courantb,
+                      double gamma, const double box[4], double *tcc, double *tc, double *tg);
+
 /* Per-cell energy, compute_energy wave_common.c:453-472, with the Courant
  * constants given explicitly (c2_in = COURANT^2 for xy_in != 0, c2_out =
  * COURANTB^2 used when TWOSPEEDS). */

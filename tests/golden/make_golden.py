@@ -18,6 +18,9 @@ Outputs (committed):
                     through the colour schemes and palettes COL_JET, COL_GRAY:
                     SHA-256 of the values and colours of every cell, frames
                     0..COLOR_FRAMES-1, for every gen_inputs.color_cases()
+  media_cases.npz   init_ior_2d's Gaussian-well media (IOR_PERIODIC_WELLS,
+                    IOR_RANDOM_WELLS): the wells it placed and its tcc, tc,
+                    tgamma tables
   table_cases.npz   the neighbour-table stepper (evolve_wave_half_new): the
                     reference's table (counts, flat indices), the L-shape
                     lines, drive values where driven, and phi, psi, tmp after
@@ -115,6 +118,14 @@ def main():
         phi, psi, tmp = R.run(b[0], b[1], b[2], TABLE_STEPS, d["xy_in"], d["tcc"], d["tc"], d["tgamma"])
         out[tag + "/phi"], out[tag + "/psi"], out[tag + "/tmp"] = phi, psi, tmp
     np.savez_compressed(os.path.join(HERE, "table_cases.npz"), **out)
+    # Gaussian-well media
+    out = {}
+    for tag in ("ior_wells_p", "ior_wells_r"):
+        R = pyoracle.RefBuild(tag)
+        d = case_inputs(tag, R.nx, R.ny)
+        out[tag + "/tcc"], out[tag + "/tc"], out[tag + "/tgamma"] = R.init_ior(d["xy_in"], 7)
+        out[tag + "/xc"], out[tag + "/yc"], out[tag + "/h"], out[tag + "/sigma"] = R.wells(7)
+    np.savez_compressed(os.path.join(HERE, "media_cases.npz"), **out)
     # frame colours: wave_value cell by cell (size 1), COL_JET and COL_GRAY
     dig = {}
     for (tag, prm, pal, plot) in color_cases():

@@ -1222,6 +1222,58 @@ int engine_step(Engine *e, long nsteps, bool force_k1)
     return WV_OK;
 }
 
+// ------------------------------------------------------------ media tables
+
+// init_ior_2d's Gaussian-well media: the sum on the GPU, the tc rule
+// (sub_wave.c:12970-12981) on the host, then set_tables
+int set_wells_tables(Engine *e, int k, const short *xy, int nw, const double *xc, const double *yc, const double *h,
+                     double sigma, int use_tanh, double courant, double courantb, double gamma, const double *box,
+                     double *tcc_out, double *tc_out, double *tg_out)
+{
+    if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
+    if (!xy || !box || (nw > 0 && (!xc || !yc))) return fail(e, WV_EINVAL, "mask, box or well arrays missing");
+    if (nw < 0) return fail(e, WV_EINVAL, "negative well count");
+    if (!(sigma > 0) || !(box[1] > box[0]) || !(box[3] > box[2])) return fail(e, WV_EINVAL, "sigma or box");
+    const Grid &G = e->g[k];
+    const int nx = G.p.nx, ny = G.p.ny;
+    const size_t cells = (size_t)nx * ny;
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    double *d_tcc = nullptr, *d_w = nullptr;
+    auto release = [&]() {
+        if (d_tcc) (void)hipFree(d_tcc);
+        if (d_w) (void)hipFree(d_w);
+    };
+    std::vector<double> w(3 * (size_t)std::max(nw, 1), 0.0);
+    for (int n = 0; n < nw; n++) {
+        w[n] = xc[n];
+        w[nw + n] = yc[n];
+        w[2 * nw + n] = h ? h[n] : 1.0;
+    }
+    std::vector<double> tcc(cells), tc(cells, 0.0), tg(cells, gamma);
+    hipError_t err = hipMalloc(&d_tcc, sizeof(double) * cells);
+    if (err == hipSuccess) err = hipMalloc(&d_w, sizeof(double) * w.size());
+    if (err == hipSuccess) err = hipMemcpy(d_w, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice);
+    if (err == hipSuccess) {
+        const int lr = wv_launch_wells(nx, ny, box, nw, d_w, d_w + nw, h ? d_w + 2 * nw : nullptr, sigma, use_tanh,
+                                       courant, courantb, d_tcc, e->stream);
+        err = (hipError_t)lr;
+    }
+    if (err == hipSuccess) err = hipMemcpyAsync(tcc.data(), d_tcc, sizeof(double) * cells, hipMemcpyDeviceToHost,
+                                                e->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+    release();
+    if (err != hipSuccess) return fail(e, WV_EHIP, "wells media: %s", hipGetErrorString(err));
+    for (size_t c = 0; c < cells; c++) {
+        if (xy[c] != 0) tc[c] = std::sqrt(tcc[c]);
+        else if (G.p.twospeeds) tc[c] = courantb;
+    }
+    if (tcc_out) memcpy(tcc_out, tcc.data(), sizeof(double) * cells);
+    if (tc_out) memcpy(tc_out, tc.data(), sizeof(double) * cells);
+    if (tg_out) memcpy(tg_out, tg.data(), sizeof(double) * cells);
+    return set_tables(e, k, xy, tcc.data(), tc.data(), tg.data());
+}
+
 // ------------------------------------------------------------ frame colours
 
 int frame_set_tables(Engine *e, int k, const float *lut, int lut_n, const double *cs, const double *ft)
@@ -1889,6 +1941,16 @@ extern "C" int wv_set_floor(wv_ctx *c, double vmax)
 {
     if (!c) return WV_EINVAL;
     return set_floor(c->e, 0, vmax);
+}
+
+extern "C" int wv_set_wells_tables(wv_ctx *c, const short *xy_in, int nwells, const double *xc, const double *yc,
+                                   const double *height, double sigma, int use_tanh, double courant,
+                                   double courantb, double gamma, const double box[4], double *tcc, double *tc,
+                                   double *tgamma)
+{
+    if (!c) return WV_EINVAL;
+    return set_wells_tables(c->e, 0, xy_in, nwells, xc, yc, height, sigma, use_tanh, courant, courantb, gamma, box,
+                            tcc, tc, tgamma);
 }
 
 extern "C" int wv_set_probes(wv_ctx *c, int n, const int *i, const int *j, long every, long capacity)

@@ -148,3 +148,7 @@ struct WvFrame {
     uchar4 *out8;                        // if set: 8-bit colours instead of out
 };
 int wv_launch_frame(const WvFrame &f, hipStream_t s);
+// Gaussian-well media (wv_media.hip): tcc[i*ny + j] on an NX x NY grid over
+// box = (xmin, xmax, ymin, ymax); h may be null (unit heights)
+int wv_launch_wells(int nx, int ny, const double box[4], int nw, const double *xc, const double *yc, const double *h,
+                    double sigma, int use_tanh, double courant, double courantb, double *tcc, hipStream_t s);

@@ -29,6 +29,7 @@ EXPORTS = [
     "wv_params_default", "wv_create", "wv_destroy", "wv_last_error", "wv_set_tables",
     "wv_set_state", "wv_set_state3", "wv_get_state", "wv_get_state3", "wv_set_sources",
     "wv_set_neumann", "wv_set_drive", "wv_set_floor", "wv_set_laplacian", "wv_set_probes", "wv_get_probes",
+    "wv_set_wells_tables",
     "wv_step", "wv_step_timed", "wv_sync", "wv_steps_done", "wv_last_kernel_ms", "wv_energy",
     "wv_get_tiling", "wv_frame_params_default", "wv_frame_set_tables", "wv_frame_reset", "wv_frame",
     "wv_frame_rgba8",
@@ -105,6 +106,8 @@ def load():
         "wv_set_floor": (ctypes.c_int, [vp, ctypes.c_double]),
         "wv_set_laplacian": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "wv_set_probes": (ctypes.c_int, [vp, ctypes.c_int, _I, _I, ctypes.c_long, ctypes.c_long]),
+        "wv_set_wells_tables": (ctypes.c_int, [vp, _S, ctypes.c_int, _D, _D, _D, ctypes.c_double, ctypes.c_int,
+                                               ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _D, _D, _D]),
         "wv_get_probes": (ctypes.c_int, [vp, _D, ctypes.c_long, ctypes.POINTER(ctypes.c_long),
                                          ctypes.POINTER(ctypes.c_long)]),
         "wv_step": (ctypes.c_int, [vp, ctypes.c_long]),
@@ -301,6 +304,20 @@ class Wave:
         self._chk(self._L.wv_frame(self._h, ctypes.byref(f), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
                   "wv_frame")
         return out
+
+    def set_wells_tables(self, xy_in, xc, yc, height, sigma, use_tanh, courant, courantb, gamma, box):
+        """Gaussian-well media built on the GPU; returns (tcc, tc, tgamma)"""
+        xy = np.ascontiguousarray(xy_in, np.int16)
+        xc = np.ascontiguousarray(xc, np.float64)
+        yc = np.ascontiguousarray(yc, np.float64)
+        hh = None if height is None else np.ascontiguousarray(height, np.float64)
+        bx = np.ascontiguousarray(box, np.float64)
+        t = [np.empty(self.shape) for _ in range(3)]
+        self._chk(self._L.wv_set_wells_tables(self._h, xy.ctypes.data_as(_S), len(xc), _d(xc), _d(yc),
+                                              None if hh is None else _d(hh), sigma, int(use_tanh), courant,
+                                              courantb, gamma, _d(bx), _d(t[0]), _d(t[1]), _d(t[2])),
+                  "wv_set_wells_tables")
+        return tuple(t)
 
     def set_probes(self, i, j, every=3, capacity=4096):
         """record u at cells (i[k], j[k]) after every `every`-th step"""
