@@ -99,6 +99,39 @@ def frame_bench(wh, w, reps=10):
     return out
 
 
+def media_bench(wh, n=4096, nwells=100):
+    """init_ior_2d's IOR_RANDOM_WELLS media for an n x n grid (10 x 10 wells,
+    the checked-in NGRIDX, NGRIDY) built by wv_set_wells_tables, wall clock
+    including the table upload; beside it the oracle (OpenMP, all host
+    threads) on a band of rows of the same grid, scaled to the grid."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    rng = np.random.default_rng(5)
+    g = int(round(nwells ** 0.5))
+    dx, dy = 4.0 / g, 4.0 / g
+    ii, jj = np.meshgrid(np.arange(g), np.arange(g), indexing="ij")
+    xc = (-2.0 + dx * (ii + 0.5 + 0.1 * rng.standard_normal(ii.shape))).ravel()
+    yc = (-2.0 + dy * (jj + 0.5 + 0.1 * rng.standard_normal(jj.shape))).ravel()
+    h = np.clip(0.5 + 0.5 * rng.standard_normal(xc.size), 0, 1)
+    sigma = 0.2 * dx * dx
+    xy = np.ones((n, n), np.int16)
+    box = (-2.0, 2.0, -2.0, 2.0)
+    W = wh.Wave(dict(nx=n, ny=n, bcond=2))
+    t0 = time.perf_counter()
+    W.set_wells_tables(xy, xc, yc, h, sigma, 1, 0.1, 0.05, 0.0, box, copy=False)
+    gpu_ms = (time.perf_counter() - t0) * 1e3
+    W.close()
+    rows = 64
+    p = pyoracle.make_params(rows, n)
+    band = (-2.0, -2.0 + 4.0 * rows / n, -2.0, 2.0)
+    t0 = time.perf_counter()
+    pyoracle.wells_tables(p, xy[:rows], xc, yc, h, sigma, 1, 0.1, 0.05, 0.0, band)
+    cpu_ms = (time.perf_counter() - t0) * 1e3 * n / rows
+    return {"workload": "%dx%d IOR_RANDOM_WELLS, %d wells, tanh" % (n, n, xc.size), "gpu_ms": gpu_ms,
+            "cpu_oracle_ms_scaled": cpu_ms, "cpu_threads": pyoracle.num_threads(),
+            "cpu_sample": "%d of %d rows" % (rows, n)}
+
+
 def cpu_baseline(w, target_s=10.0, chunk=40):
     """the oracle on the host cores, a bounded sample of the FP64 workload:
     chunks of steps until target_s seconds of CPU work have been timed"""
@@ -393,6 +426,12 @@ def main():
             out["frame"] = frame_bench(wh, w)
         except Exception as ex:  # the frame is an extra: report, do not lose the line
             out["frame"] = {"error": str(ex)}
+    if not args.no_frame:
+        log("timing the media builder ...")
+        try:
+            out["media"] = media_bench(wh)
+        except Exception as ex:  # an extra: report, do not lose the line
+            out["media"] = {"error": str(ex)}
     if not args.no_cpu:
         log("timing the CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(w)

@@ -1224,8 +1224,9 @@ int engine_step(Engine *e, long nsteps, bool force_k1)
 
 // ------------------------------------------------------------ media tables
 
-// init_ior_2d's Gaussian-well media: the sum on the GPU, the tc rule
-// (sub_wave.c:12970-12981) on the host, then set_tables
+// init_ior_2d's Gaussian-well media, built in place: the mask planes on the
+// host (as wv_set_tables), tcc and tc by the GPU into the engine's per-cell
+// tables (field coefficient mode, one tgamma)
 int set_wells_tables(Engine *e, int k, const short *xy, int nw, const double *xc, const double *yc, const double *h,
                      double sigma, int use_tanh, double courant, double courantb, double gamma, const double *box,
                      double *tcc_out, double *tc_out, double *tg_out)
@@ -1234,44 +1235,77 @@ int set_wells_tables(Engine *e, int k, const short *xy, int nw, const double *xc
     if (!xy || !box || (nw > 0 && (!xc || !yc))) return fail(e, WV_EINVAL, "mask, box or well arrays missing");
     if (nw < 0) return fail(e, WV_EINVAL, "negative well count");
     if (!(sigma > 0) || !(box[1] > box[0]) || !(box[3] > box[2])) return fail(e, WV_EINVAL, "sigma or box");
-    const Grid &G = e->g[k];
-    const int nx = G.p.nx, ny = G.p.ny;
-    const size_t cells = (size_t)nx * ny;
+    Grid &G = e->g[k];
+    const int nx = G.p.nx, ny = G.p.ny, ldw = G.ldw, ldy = G.ldy;
+    const bool ts = G.p.twospeeds != 0;
+    // mask planes, the rules of set_tables
+    std::vector<uint64_t> act((size_t)nx * ldw, 0), eact((size_t)nx * ldw, 0), nzp((size_t)nx * ldw, 0);
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++) {
+            const size_t c = (size_t)i * ny + j;
+            const bool edge = i == 0 || i == nx - 1 || j == 0 || j == ny - 1;
+            const size_t w = (size_t)i * ldw + (j >> 6);
+            const uint64_t b = 1ull << (j & 63);
+            if (ts || (edge ? xy[c] != 0 : xy[c] > 0)) act[w] |= b;
+            if (ts || xy[c] != 0) eact[w] |= b;
+            if (xy[c] != 0) nzp[w] |= b;
+        }
     HIPCHK(e, hipSetDevice(e->device));
     HIPCHK(e, hipStreamSynchronize(e->stream));
-    double *d_tcc = nullptr, *d_w = nullptr;
-    auto release = [&]() {
-        if (d_tcc) (void)hipFree(d_tcc);
-        if (d_w) (void)hipFree(d_w);
-    };
-    std::vector<double> w(3 * (size_t)std::max(nw, 1), 0.0);
+    const size_t wb = (size_t)nx * ldw * 8, cells = (size_t)nx * ldy;
+    HIPCHK(e, hipMemcpy(G.eact, eact.data(), wb, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemset(G.med, 0, wb));
+    HIPCHK(e, hipMemcpy(G.nz, nzp.data(), wb, hipMemcpyHostToDevice));
+    G.h_nz.swap(nzp);
+    for (double **q : {&G.tgf}) if (*q) { (void)hipFree(*q); *q = nullptr; }
+    for (float **q : {&G.tgf32}) if (*q) { (void)hipFree(*q); *q = nullptr; }
+    if (!G.tccf) HIPCHK(e, hipMalloc(&G.tccf, cells * sizeof(double)));
+    const bool split = e->form == WV_FORM_F32SPLIT;
+    if (split) {
+        if (!G.tccf_hi) HIPCHK(e, hipMalloc(&G.tccf_hi, cells * sizeof(float)));
+        if (!G.tccf_lo) HIPCHK(e, hipMalloc(&G.tccf_lo, cells * sizeof(float)));
+    }
+    double *d_w = nullptr;
+    const size_t nwv = 3 * (size_t)std::max(nw, 1);
+    std::vector<double> w(nwv, 1.0);
     for (int n = 0; n < nw; n++) {
         w[n] = xc[n];
         w[nw + n] = yc[n];
-        w[2 * nw + n] = h ? h[n] : 1.0;
+        if (h) w[2 * nw + n] = h[n];
     }
-    std::vector<double> tcc(cells), tc(cells, 0.0), tg(cells, gamma);
-    hipError_t err = hipMalloc(&d_tcc, sizeof(double) * cells);
-    if (err == hipSuccess) err = hipMalloc(&d_w, sizeof(double) * w.size());
-    if (err == hipSuccess) err = hipMemcpy(d_w, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice);
+    HIPCHK(e, hipMalloc(&d_w, sizeof(double) * nwv));
+    hipError_t err = hipMemcpy(d_w, w.data(), sizeof(double) * nwv, hipMemcpyHostToDevice);
     if (err == hipSuccess) {
-        const int lr = wv_launch_wells(nx, ny, box, nw, d_w, d_w + nw, h ? d_w + 2 * nw : nullptr, sigma, use_tanh,
-                                       courant, courantb, d_tcc, e->stream);
-        err = (hipError_t)lr;
+        WvWells a{};
+        a.nx = nx; a.ny = ny; a.ldy = ldy; a.ldw = ldw; a.nw = nw; a.use_tanh = use_tanh != 0; a.twospeeds = ts;
+        a.xmin = box[0]; a.xmax = box[1]; a.ymin = box[2]; a.ymax = box[3];
+        a.sigma = sigma; a.courant = courant; a.courantb = courantb;
+        a.xc = d_w; a.yc = d_w + nw; a.h = h ? d_w + 2 * nw : nullptr;
+        a.nz = G.nz; a.tcc = G.tccf; a.tc = G.tc;
+        a.tcc_hi = split ? G.tccf_hi : nullptr; a.tcc_lo = split ? G.tccf_lo : nullptr;
+        err = (hipError_t)wv_launch_wells(a, e->stream);
     }
-    if (err == hipSuccess) err = hipMemcpyAsync(tcc.data(), d_tcc, sizeof(double) * cells, hipMemcpyDeviceToHost,
-                                                e->stream);
     if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
-    release();
+    (void)hipFree(d_w);
     if (err != hipSuccess) return fail(e, WV_EHIP, "wells media: %s", hipGetErrorString(err));
-    for (size_t c = 0; c < cells; c++) {
-        if (xy[c] != 0) tc[c] = std::sqrt(tcc[c]);
-        else if (G.p.twospeeds) tc[c] = courantb;
-    }
-    if (tcc_out) memcpy(tcc_out, tcc.data(), sizeof(double) * cells);
-    if (tc_out) memcpy(tc_out, tc.data(), sizeof(double) * cells);
-    if (tg_out) memcpy(tg_out, tg.data(), sizeof(double) * cells);
-    return set_tables(e, k, xy, tcc.data(), tc.data(), tg.data());
+    if (tcc_out) HIPCHK(e, hipMemcpy2D(tcc_out, ny * sizeof(double), G.tccf, ldy * sizeof(double), ny * sizeof(double),
+                                      nx, hipMemcpyDeviceToHost));
+    if (tc_out) HIPCHK(e, hipMemcpy2D(tc_out, ny * sizeof(double), G.tc, ldy * sizeof(double), ny * sizeof(double), nx,
+                                     hipMemcpyDeviceToHost));
+    if (tg_out) std::fill(tg_out, tg_out + (size_t)nx * ny, gamma);
+    G.coef = WV_COEF_FIELD;
+    G.tcc0 = G.tcc1 = courantb;
+    G.tg0 = G.tg1 = gamma;
+    // as set_tables: a state set under another mask keeps its values, one step per launch
+    G.h_act_plain = act;
+    if (G.tbl) act = table_act(G);
+    if (G.state && G.h_act != act) G.exact = true;
+    HIPCHK(e, hipMemcpy(G.act, act.data(), wb, hipMemcpyHostToDevice));
+    G.h_act.swap(act);
+    rebuild_written(G);
+    G.tables = true;
+    drop_tiles(e);
+    return sync_inst(e);
 }
 
 // ------------------------------------------------------------ frame colours

@@ -148,7 +148,15 @@ struct WvFrame {
     uchar4 *out8;                        // if set: 8-bit colours instead of out
 };
 int wv_launch_frame(const WvFrame &f, hipStream_t s);
-// Gaussian-well media (wv_media.hip): tcc[i*ny + j] on an NX x NY grid over
-// box = (xmin, xmax, ymin, ymax); h may be null (unit heights)
-int wv_launch_wells(int nx, int ny, const double box[4], int nw, const double *xc, const double *yc, const double *h,
-                    double sigma, int use_tanh, double courant, double courantb, double *tcc, hipStream_t s);
+// Gaussian-well media (wv_media.hip): tcc and tc of every cell of an NX x NY
+// grid over (xmin, xmax, ymin, ymax), into arrays of row stride ldy; h may be
+// null (unit heights); tcc_hi / tcc_lo (split form) may be null
+struct WvWells {
+    int nx, ny, ldy, ldw, nw, use_tanh, twospeeds;
+    double xmin, xmax, ymin, ymax, sigma, courant, courantb;
+    const double *xc, *yc, *h;
+    const uint64_t *nz;
+    double *tcc, *tc;
+    float *tcc_hi, *tcc_lo;
+};
+int wv_launch_wells(const WvWells &a, hipStream_t s);

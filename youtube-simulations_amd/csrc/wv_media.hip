@@ -12,34 +12,41 @@ using namespace wv;
 
 namespace {
 
-__global__ void __launch_bounds__(256) k_wells(int nx, int ny, double xmin, double xmax, double ymin, double ymax,
-                                               int nw, const double *__restrict__ xc, const double *__restrict__ yc,
-                                               const double *__restrict__ h, double sigma, int use_tanh,
-                                               double courant, double courantb, double *__restrict__ tcc)
+// tcc and tc of cell (i, j) into the engine's per-cell tables (row stride
+// ldy); tc = sqrt(tcc) where xy_in != 0 (nz plane), courantb where TWOSPEEDS,
+// 0 elsewhere (sub_wave.c:12970-12981); the split form's binary32 pair too
+__global__ void __launch_bounds__(256) k_wells(WvWells a)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
-    if (j >= ny || i >= nx) return;
+    if (j >= a.ny || i >= a.nx) return;
     // ij_to_xy, sub_wave.c:469-476
-    const double x = xmin + ((double)i) * (xmax - xmin) / ((double)nx);
-    const double y = ymin + ((double)j) * (ymax - ymin) / ((double)ny);
+    const double x = a.xmin + ((double)i) * (a.xmax - a.xmin) / ((double)a.nx);
+    const double y = a.ymin + ((double)j) * (a.ymax - a.ymin) / ((double)a.ny);
     double sum = 0.0;
-    for (int n = 0; n < nw; n++) {
-        const double dx = x - gc(xc)[n], dy = y - gc(yc)[n];
+    for (int n = 0; n < a.nw; n++) {
+        const double dx = x - gc(a.xc)[n], dy = y - gc(a.yc)[n];
         const double r2 = dx * dx + dy * dy;
-        const double e = exp(-r2 / sigma);
-        sum = h ? sum + e * gc(h)[n] : sum + e;
+        const double e = exp(-r2 / a.sigma);
+        sum = a.h ? sum + e * gc(a.h)[n] : sum + e;
     }
-    if (use_tanh) sum = tanh(sum);
-    gm(tcc)[(size_t)i * ny + j] = courant * sum + courantb * (1.0 - sum);
+    if (a.use_tanh) sum = tanh(sum);
+    const double tcc = a.courant * sum + a.courantb * (1.0 - sum);
+    const size_t k = (size_t)i * a.ldy + j;
+    gm(a.tcc)[k] = tcc;
+    const bool nz = (gc(a.nz)[(size_t)i * a.ldw + (j >> 6)] >> (j & 63)) & 1u;
+    gm(a.tc)[k] = nz ? sqrt(tcc) : (a.twospeeds ? a.courantb : 0.0);
+    if (a.tcc_hi) {
+        const float hi = (float)tcc;
+        gm(a.tcc_hi)[k] = hi;
+        gm(a.tcc_lo)[k] = (float)(tcc - (double)hi);
+    }
 }
 
 }  // namespace
 
-int wv_launch_wells(int nx, int ny, const double box[4], int nw, const double *xc, const double *yc, const double *h,
-                    double sigma, int use_tanh, double courant, double courantb, double *tcc, hipStream_t s)
+int wv_launch_wells(const WvWells &a, hipStream_t s)
 {
-    const dim3 block(256), grid((ny + 255) / 256, nx);
-    hipLaunchKernelGGL(k_wells, grid, block, 0, s, nx, ny, box[0], box[1], box[2], box[3], nw, xc, yc, h, sigma,
-                       use_tanh, courant, courantb, tcc);
+    const dim3 block(256), grid((a.ny + 255) / 256, a.nx);
+    hipLaunchKernelGGL(k_wells, grid, block, 0, s, a);
     return (int)hipGetLastError();
 }

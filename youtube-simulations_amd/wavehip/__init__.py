@@ -305,19 +305,20 @@ class Wave:
                   "wv_frame")
         return out
 
-    def set_wells_tables(self, xy_in, xc, yc, height, sigma, use_tanh, courant, courantb, gamma, box):
-        """Gaussian-well media built on the GPU; returns (tcc, tc, tgamma)"""
+    def set_wells_tables(self, xy_in, xc, yc, height, sigma, use_tanh, courant, courantb, gamma, box, copy=True):
+        """Gaussian-well media built on the GPU; returns (tcc, tc, tgamma), or
+        None without copy"""
         xy = np.ascontiguousarray(xy_in, np.int16)
         xc = np.ascontiguousarray(xc, np.float64)
         yc = np.ascontiguousarray(yc, np.float64)
         hh = None if height is None else np.ascontiguousarray(height, np.float64)
         bx = np.ascontiguousarray(box, np.float64)
-        t = [np.empty(self.shape) for _ in range(3)]
+        t = [np.empty(self.shape) for _ in range(3)] if copy else None
+        outs = [_d(a) for a in t] if copy else [None, None, None]
         self._chk(self._L.wv_set_wells_tables(self._h, xy.ctypes.data_as(_S), len(xc), _d(xc), _d(yc),
                                               None if hh is None else _d(hh), sigma, int(use_tanh), courant,
-                                              courantb, gamma, _d(bx), _d(t[0]), _d(t[1]), _d(t[2])),
-                  "wv_set_wells_tables")
-        return tuple(t)
+                                              courantb, gamma, _d(bx), *outs), "wv_set_wells_tables")
+        return tuple(t) if copy else None
 
     def set_probes(self, i, j, every=3, capacity=4096):
         """record u at cells (i[k], j[k]) after every `every`-th step"""
