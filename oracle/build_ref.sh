@@ -89,6 +89,16 @@ build_one "row_16x29_bc2_ts1_ext" 0 -DNX=16 -DNY=29 -DB_COND=2 -DTWOSPEEDS=1 -DB
     -DOSCILLATE_LEFT=1 -DOSCILLATION_SCHEDULE=0 -DFLOOR=1 -DVMAX=0.9 $KS &
 pids+=($!)
 for p in "${pids[@]}"; do wait "$p"; done; pids=()
+# COMPARISON: the flat twin split into two halves (wave_3d.c:736-990)
+for g in "24 17" "16 29"; do
+    set -- $g; nx=$1; ny=$2
+    for bc in 0 1 2 3 4; do
+        build_one "flat_${nx}x${ny}_bc${bc}_cmp" 1 -DNX=$nx -DNY=$ny -DB_COND=$bc -DTWOSPEEDS=$(( bc % 2 )) \
+            -DCOMPARISON=1 $KS &
+        pids+=($!)
+    done
+    for p in "${pids[@]}"; do wait "$p"; done; pids=()
+done
 # the neighbour-table stepper (PRECOMPUTE_BC): evolve_wave_half_new
 # (wave_3d.c:345-597) with init_laplacian_coords / compute_laplacian
 # (sub_wave.c:11555-11963), B_COND 0-4 and BC_LSHAPE (10)

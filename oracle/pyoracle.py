@@ -26,16 +26,16 @@ class OrcParams(ctypes.Structure):
         ("oscillate_topbot", ctypes.c_int), ("tb_shift", ctypes.c_int),
         ("kappa", ctypes.c_double), ("kappa_sides", ctypes.c_double),
         ("gamma_sides", ctypes.c_double), ("kappa_topbot", ctypes.c_double),
-        ("gamma_topbot", ctypes.c_double),
+        ("gamma_topbot", ctypes.c_double), ("comparison", ctypes.c_int),
     ]
 
 
 def make_params(nx, ny, bcond=0, twospeeds=0, stencil=5, oscillate_topbot=0, tb_shift=None,
-                kappa=0.0, kappa_sides=0.0, gamma_sides=0.0, kappa_topbot=0.0, gamma_topbot=0.0):
+                kappa=0.0, kappa_sides=0.0, gamma_sides=0.0, kappa_topbot=0.0, gamma_topbot=0.0, comparison=0):
     if tb_shift is None:
         tb_shift = nx
     return OrcParams(nx, ny, bcond, twospeeds, stencil, oscillate_topbot, tb_shift,
-                     kappa, kappa_sides, gamma_sides, kappa_topbot, gamma_topbot)
+                     kappa, kappa_sides, gamma_sides, kappa_topbot, gamma_topbot, comparison)
 
 
 class OrcFrameParams(ctypes.Structure):

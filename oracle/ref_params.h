@@ -79,7 +79,9 @@
 #ifndef BC_NEUMANN
 #define BC_NEUMANN 0
 #endif
+#ifndef COMPARISON
 #define COMPARISON 0
+#endif
 #ifndef FLOOR
 #define FLOOR 0
 #endif

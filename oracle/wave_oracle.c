@@ -77,6 +77,9 @@ static void edge_step(const orc_params *p, double *const *phi_in, double *const 
     const double KT = p->kappa_topbot, GT = p->gamma_topbot;
     int i, j, iplus, iminus;
     double delta, x, y;
+    /* COMPARISON (wave_3d.c:737-738, 806-807): the periodic wraps of the top and
+     * bottom rows stay inside their half */
+    const int jbot = p->comparison ? NY / 2 : 0, jtop = p->comparison ? NY / 2 - 1 : NY - 1;
 
     /* left boundary, wave_billiard.c:393-430 (OSCILLATE_LEFT not supported) */
     for (j = 1; j < NY - 1; j++)
@@ -137,13 +140,13 @@ static void edge_step(const orc_params *p, double *const *phi_in, double *const 
                 case ORC_BC_PERIODIC:
                     iplus = (i + 1) % NX;
                     iminus = (i - 1) % NX; if (iminus < 0) iminus += NX;
-                    delta = phi_in[iplus][NY - 1] + phi_in[iminus][NY - 1] + phi_in[i][NY - 2] + phi_in[i][0] - 4.0 * x;
+                    delta = phi_in[iplus][NY - 1] + phi_in[iminus][NY - 1] + phi_in[i][NY - 2] + phi_in[i][jbot] - 4.0 * x;
                     phi_out[i][NY - 1] = leap(x, y, delta, tcc[i][NY - 1], KAPPA, tgamma[i][NY - 1]);
                     break;
                 case ORC_BC_VPER_HABS:
                     iplus = i + 1;  if (iplus == NX) iplus = NX - 1;
                     iminus = i - 1; if (iminus == -1) iminus = 0;
-                    delta = phi_in[iplus][NY - 1] + phi_in[iminus][NY - 1] + phi_in[i][NY - 2] + phi_in[i][0] - 4.0 * x;
+                    delta = phi_in[iplus][NY - 1] + phi_in[iminus][NY - 1] + phi_in[i][NY - 2] + phi_in[i][jbot] - 4.0 * x;
                     if (i == 0)
                         phi_out[0][NY - 1] = absorb(x, y, phi_in[1][NY - 1], tc[0][NY - 1], KS, GS);
                     else
@@ -174,13 +177,13 @@ static void edge_step(const orc_params *p, double *const *phi_in, double *const 
                 case ORC_BC_PERIODIC:
                     iplus = (i + 1) % NX;
                     iminus = (i - 1) % NX; if (iminus < 0) iminus += NX;
-                    delta = phi_in[iplus][0] + phi_in[iminus][0] + phi_in[i][1] + phi_in[i][NY - 1] - 4.0 * x;
+                    delta = phi_in[iplus][0] + phi_in[iminus][0] + phi_in[i][1] + phi_in[i][jtop] - 4.0 * x;
                     phi_out[i][0] = leap(x, y, delta, tcc[i][0], KAPPA, tgamma[i][0]);
                     break;
                 case ORC_BC_VPER_HABS:
                     iplus = i + 1;  if (iplus == NX) iplus = NX - 1;
                     iminus = i - 1; if (iminus == -1) iminus = 0;
-                    delta = phi_in[iplus][0] + phi_in[iminus][0] + phi_in[i][1] + phi_in[i][NY - 1] - 4.0 * x;
+                    delta = phi_in[iplus][0] + phi_in[iminus][0] + phi_in[i][1] + phi_in[i][jtop] - 4.0 * x;
                     if (i == 0)
                         phi_out[0][0] = absorb(x, y, phi_in[1][0], tc[0][0], KS, GS);
                     else
@@ -190,6 +193,71 @@ static void edge_step(const orc_params *p, double *const *phi_in, double *const 
                     phi_out[i][0] = absorb(x, y, phi_in[i][1], tc[i][0], KT, GT);
                     break;
                 }
+        }
+    if (!p->comparison) return;
+
+    /* COMPARISON, top of the bottom half j = jtop, wave_3d.c:873-929 */
+    for (i = 0; i < NX; i++)
+        if (ACTIVE_EDGE(p, xy_in[i][jtop])) {
+            x = phi_in[i][jtop];
+            y = psi_in[i][jtop];
+            switch (B) {
+            case ORC_BC_PERIODIC:
+                iplus = (i + 1) % NX;
+                iminus = (i - 1) % NX; if (iminus < 0) iminus += NX;
+                delta = phi_in[iplus][jtop] + phi_in[iminus][jtop] + phi_in[i][jtop - 1] + phi_in[i][0] - 4.0 * x;
+                phi_out[i][jtop] = leap(x, y, delta, tcc[i][jtop], KAPPA, tgamma[i][jtop]);
+                break;
+            case ORC_BC_ABSORBING:
+                phi_out[i][jtop] = absorb(x, y, phi_in[i][jtop - 1], tc[i][jtop], KT, GT);
+                break;
+            case ORC_BC_VPER_HABS:
+                iplus = i + 1;  if (iplus == NX) iplus = NX - 1;
+                iminus = i - 1; if (iminus == -1) iminus = 0;
+                delta = phi_in[iplus][jtop] + phi_in[iminus][jtop] + phi_in[i][jtop - 1] + phi_in[i][0] - 4.0 * x;
+                if (i == 0) phi_out[0][jtop] = absorb(x, y, phi_in[1][jtop], tc[0][jtop], KS, GS);
+                else phi_out[i][jtop] = leap(x, y, delta, tcc[i][jtop], KAPPA, tgamma[i][jtop]);
+                break;
+            default: /* DIRICHLET, ABS_REFLECT */
+                iplus = i + 1;  if (iplus == NX) iplus = NX - 1;
+                iminus = i - 1; if (iminus == -1) iminus = 0;
+                delta = phi_in[iplus][jtop] + phi_in[iminus][jtop] + phi_in[i][jtop - 1] - 3.0 * x;
+                phi_out[i][jtop] = leap(x, y, delta, tcc[i][jtop], KAPPA, tgamma[i][jtop]);
+                break;
+            }
+        }
+
+    /* COMPARISON, bottom of the top half j = jbot, wave_3d.c:931-982; the
+     * periodic branch wraps to the cell itself and the VPER_HABS corner reads
+     * phi_in[NY] (cell (1, 0)), as the reference does */
+    for (i = 0; i < NX; i++)
+        if (ACTIVE_EDGE(p, xy_in[i][jbot])) {
+            x = phi_in[i][jbot];
+            y = psi_in[i][jbot];
+            switch (B) {
+            case ORC_BC_PERIODIC:
+                iplus = (i + 1) % NX;
+                iminus = (i - 1) % NX; if (iminus < 0) iminus += NX;
+                delta = phi_in[iplus][jbot] + phi_in[iminus][jbot] + phi_in[i][jbot + 1] + phi_in[i][jbot] - 4.0 * x;
+                phi_out[i][jbot] = leap(x, y, delta, tcc[i][jbot], KAPPA, tgamma[i][jbot]);
+                break;
+            case ORC_BC_ABSORBING:
+                phi_out[i][jbot] = absorb(x, y, phi_in[i][jbot + 1], tc[i][jbot], KT, GT);
+                break;
+            case ORC_BC_VPER_HABS:
+                iplus = i + 1;  if (iplus == NX) iplus = NX - 1;
+                iminus = i - 1; if (iminus == -1) iminus = 0;
+                delta = phi_in[iplus][jbot] + phi_in[iminus][jbot] + phi_in[i][jbot + 1] + phi_in[i][NY - 1] - 4.0 * x;
+                if (i == 0) phi_out[0][jbot] = absorb(x, y, phi_in[1][0], tc[0][jbot], KS, GS);
+                else phi_out[i][jbot] = leap(x, y, delta, tcc[i][jbot], KAPPA, tgamma[i][jbot]);
+                break;
+            default: /* DIRICHLET, ABS_REFLECT */
+                iplus = i + 1;  if (iplus == NX) iplus = NX - 1;
+                iminus = i - 1; if (iminus == -1) iminus = 0;
+                delta = phi_in[iplus][jbot] + phi_in[iminus][jbot] + phi_in[i][jbot + 1] - 3.0 * x;
+                phi_out[i][jbot] = leap(x, y, delta, tcc[i][jbot], KAPPA, tgamma[i][jbot]);
+                break;
+            }
         }
 }
 

@@ -47,6 +47,8 @@ typedef struct {
     double gamma_sides;       /* GAMMA_SIDES                                        */
     double kappa_topbot;      /* KAPPA_TOPBOT                                       */
     double gamma_topbot;      /* GAMMA_TOPBOT                                       */
+    int comparison;           /* COMPARISON (flat twin, wave_3d.c:736-990): two     */
+                              /* halves, interface columns NY/2-1 and NY/2          */
 } orc_params;
 
 /* 1.0/6.0: weight of the build-defined 9-point Laplacian

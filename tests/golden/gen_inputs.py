@@ -178,3 +178,10 @@ def color_cases():
                     continue
                 out.append((tag, prm, pal, plot))
     return out
+
+
+def cmp_cases():
+    """(tag, nx, ny, bcond, twospeeds) of the flat reference builds with
+    COMPARISON (two halves; TWOSPEEDS on for odd B_COND)"""
+    return [("flat_%dx%d_bc%d_cmp" % (nx, ny, bc), nx, ny, bc, bc % 2) for nx, ny in ((24, 17), (16, 29))
+            for bc in range(5)]

@@ -18,6 +18,8 @@ Outputs (committed):
                     through the colour schemes and palettes COL_JET, COL_GRAY:
                     SHA-256 of the values and colours of every cell, frames
                     0..COLOR_FRAMES-1, for every gen_inputs.color_cases()
+  cmp_cases.npz     the flat twin with COMPARISON (two halves): phi, psi, tmp
+                    after 7 steps, for every gen_inputs.cmp_cases()
   media_cases.npz   init_ior_2d's Gaussian-well media (IOR_PERIODIC_WELLS,
                     IOR_RANDOM_WELLS): the wells it placed and its tcc, tc,
                     tgamma tables
@@ -40,7 +42,7 @@ sys.path.insert(0, os.path.join(ROOT, "youtube-simulations_amd"))
 sys.path.insert(0, HERE)
 
 import pyoracle  # noqa: E402
-from gen_inputs import (COLOR_FRAMES, EXT_STEPS, TABLE_STEPS, case_inputs, color_cases,  # noqa: E402
+from gen_inputs import (COLOR_FRAMES, EXT_STEPS, TABLE_STEPS, case_inputs, cmp_cases, color_cases,  # noqa: E402
                         color_inputs, ext_cases, ext_inputs, small_cases, table_cases, table_inputs)
 from wavehip import workloads  # noqa: E402
 
@@ -118,6 +120,15 @@ def main():
         phi, psi, tmp = R.run(b[0], b[1], b[2], TABLE_STEPS, d["xy_in"], d["tcc"], d["tc"], d["tgamma"])
         out[tag + "/phi"], out[tag + "/psi"], out[tag + "/tmp"] = phi, psi, tmp
     np.savez_compressed(os.path.join(HERE, "table_cases.npz"), **out)
+    # COMPARISON
+    out = {}
+    for (tag, nx, ny, bc, ts) in cmp_cases():
+        R = pyoracle.RefBuild(tag)
+        d = case_inputs(tag, nx, ny)
+        b = [d["phi"].copy(), d["psi"].copy(), d["tmp"].copy()]
+        out[tag + "/phi"], out[tag + "/psi"], out[tag + "/tmp"] = R.run(b[0], b[1], b[2], NSTEPS_SMALL, d["xy_in"],
+                                                                        d["tcc"], d["tc"], d["tgamma"])
+    np.savez_compressed(os.path.join(HERE, "cmp_cases.npz"), **out)
     # Gaussian-well media
     out = {}
     for tag in ("ior_wells_p", "ior_wells_r"):

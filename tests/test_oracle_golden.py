@@ -12,7 +12,7 @@ import os
 import numpy as np
 import pytest
 
-from gen_inputs import (EXT_STEPS, EXT_VMAX, KAPPAS, TABLE_STEPS, case_inputs, ext_cases, ext_inputs,
+from gen_inputs import (EXT_STEPS, EXT_VMAX, KAPPAS, TABLE_STEPS, case_inputs, cmp_cases, ext_cases, ext_inputs,
                         small_cases, table_cases, table_inputs)
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -213,6 +213,22 @@ def test_oracle_table_matches_reference_build_random(oracle):
         got = oracle.run_table(p, s[0], s[1], 5, xy, t[0], t[1], t[2], nn, nb, tmp=s[2])
         for a, b in zip(ref, got):
             assert bits_equal(a, b), tag
+
+
+@pytest.mark.parametrize("case", cmp_cases(), ids=lambda c: c[0])
+def test_oracle_comparison_matches_reference_fixture(oracle, case):
+    """COMPARISON, the flat twin split into two halves (wave_3d.c:736-990):
+    the interface rows of both halves with their quirks, bit for bit; the
+    plain stepper on the same inputs differs."""
+    tag, nx, ny, bc, ts = case
+    g = np.load(os.path.join(GOLD, "cmp_cases.npz"))
+    d = case_inputs(tag, nx, ny)
+    p = oracle.make_params(nx, ny, bcond=bc, twospeeds=ts, tb_shift=ny - 2, comparison=1, **KAPPAS)
+    phi, psi, tmp = oracle.run(p, d["phi"], d["psi"], 7, d["xy_in"], d["tcc"], d["tc"], d["tgamma"], tmp=d["tmp"])
+    assert bits_equal(phi, g[tag + "/phi"]) and bits_equal(psi, g[tag + "/psi"]) and bits_equal(tmp, g[tag + "/tmp"])
+    p0 = oracle.make_params(nx, ny, bcond=bc, twospeeds=ts, tb_shift=ny - 2, **KAPPAS)
+    phi0, _, _ = oracle.run(p0, d["phi"], d["psi"], 7, d["xy_in"], d["tcc"], d["tc"], d["tgamma"], tmp=d["tmp"])
+    assert not bits_equal(phi0, g[tag + "/phi"])
 
 
 def test_c1_plain_workload_digests(oracle):
