@@ -75,6 +75,12 @@ typedef struct wv_params {
  * bits for every finite field; the flag also reproduces the reference's NaN
  * where a field that overflows meets a zero coefficient. */
 #define WV_FLAG_FULL_FORMULA 2
+/* COMPARISON of the flat program (wave_3d.c:736-990): the grid is two halves,
+ * j < NY/2 and j >= NY/2, each with its own top and bottom rows (the
+ * interface columns NY/2-1 and NY/2 take the B_COND rules of wave_3d.c:873-982,
+ * the periodic wraps of the outer rows stay inside their half).  FP64 form,
+ * 5-point, NY >= 4; one step per launch. */
+#define WV_FLAG_COMPARISON 4
 
 /* Fill *p with the reference's defaults for an nx x ny grid (B_COND 0, 5-point,
  * FP64, all constants 0, e_scale 1, tb_shift nx, tuned tiling). */

@@ -41,8 +41,10 @@
  * corners (0, 0), (0, NY-1) at AMPLITUDE*cos(time*OMEGA), wave_3d.c:358-376,
  * 575-579.
  *
- * Not provided (the program aborts with a message on the first call):
- * COMPARISON (wave_3d.c:805-976).
+ * COMPARISON (with WAVEHIP_FLAT, wave_3d.c:736-990): the grid steps as two
+ * halves (WV_FLAG_COMPARISON); with OSCILLATE_LEFT the interface cells
+ * (0, NY/2-1), (0, NY/2) keep their interface formula where it applies, as
+ * in the reference.
  *
  * Faster path for the frame loop (wave_billiard.c:897-899): keep the state on
  * the GPU and call wv_step(wavehip_ctx(), 3*NVID) plus one wv_get_state per
@@ -83,11 +85,10 @@ static wv_ctx *wavehip_ctx(void)
     if (!ctx) {
         wv_params p;
         int rc;
-#if defined(COMPARISON) && COMPARISON
-        fprintf(stderr, "wavehip: COMPARISON is not supported by libwavehip\n");
-        abort();
-#endif
         wv_params_default(&p, NX, NY);
+#if defined(COMPARISON) && COMPARISON && defined(WAVEHIP_FLAT)
+        p.flags |= WV_FLAG_COMPARISON; /* the flat stepper's two halves, wave_3d.c:736-990 */
+#endif
         p.bcond = B_COND;
         p.twospeeds = TWOSPEEDS;
         p.kappa = KAPPA;
@@ -213,8 +214,9 @@ static void wavehip_laplacian(const t_laplacian *lap, const double *base, const 
  * stepper: (0, j) for bc_jmin < j < bc_jmax, 0 < j < NY-1, then the corners
  * (0, 0), (0, NY-1) at AMPLITUDE*cos(time*OMEGA) (wave_3d.c:358-376, 413,
  * 575-579) */
-static void wavehip_drive(int nsteps)
+static void wavehip_drive(int nsteps, const short *xy0)
 {
+    (void)xy0;
     static long time = 0;  /* the reference's step counter, wave_billiard.c:310,315 */
     static int cells[NY + 2], ncell = -1;
     static double vals[(NY + 2) * 3];
@@ -235,7 +237,13 @@ static void wavehip_drive(int nsteps)
         cells[ncell++] = NY - 1;
 #elif defined(WAVEHIP_FLAT)
         ncell = 0;
-        for (q = 1; q < NY - 1; q++) cells[ncell++] = q;
+        for (q = 1; q < NY - 1; q++) {
+#if defined(COMPARISON) && COMPARISON
+            /* the interface rows overwrite the drive where they apply (wave_3d.c:873-982) */
+            if ((q == NY / 2 - 1 || q == NY / 2) && (TWOSPEEDS || xy0[q] != 0)) continue;
+#endif
+            cells[ncell++] = q;
+        }
         cells[ncell++] = 0;
         cells[ncell++] = NY - 1;
         (void)bc_jmin; (void)bc_jmax; (void)ij;
@@ -284,7 +292,7 @@ static void evolve_wave_half(double *phi_in[NX], double *psi_in[NX], double *phi
     wavehip_neumann(neumann_bc);
 #endif
 #if OSCILLATE_LEFT
-    wavehip_drive(1);
+    wavehip_drive(1, xy_in[0]);
 #endif
     rc = wv_evolve_wave_half_rows(wavehip_ctx(), phi_in, psi_in, phi_out, xy_in, tcc, tc, tgamma);
     (void)neumann_bc;
@@ -301,7 +309,7 @@ static void evolve_wave(double *phi[NX], double *psi[NX], double *tmp[NX], short
     wavehip_neumann(neumann_bc);
 #endif
 #if OSCILLATE_LEFT
-    wavehip_drive(3);
+    wavehip_drive(3, xy_in[0]);
 #endif
     rc = wv_evolve_wave_rows(wavehip_ctx(), phi, psi, tmp, xy_in, tcc_table, tc_table, tgamma_table);
     (void)neumann_bc;
@@ -322,7 +330,7 @@ static void evolve_wave_half(double phi_in[NX * NY], double psi_in[NX * NY], dou
     wavehip_neumann(neumann_bc);
 #endif
 #if OSCILLATE_LEFT
-    wavehip_drive(1);
+    wavehip_drive(1, xy_in);
 #endif
     rc = wv_evolve_wave_half_flat(wavehip_ctx(), phi_in, psi_in, phi_out, xy_in, tc, tcc, tgamma);
     (void)neumann_bc;
@@ -338,7 +346,7 @@ static void evolve_wave_half_new(double phi_in[NX * NY], double psi_in[NX * NY],
     int rc;
     wavehip_laplacian(laplace, phi_in, NULL, NULL, NULL, NULL);
 #if OSCILLATE_LEFT
-    wavehip_drive(1);
+    wavehip_drive(1, xy_in);
 #endif
     rc = wv_evolve_wave_half_flat(wavehip_ctx(), phi_in, psi_in, phi_out, xy_in, tc, tcc, tgamma);
     if (rc) wavehip_die(wavehip_ctx(), "evolve_wave_half_new", rc);
@@ -360,7 +368,7 @@ static void evolve_wave(double phi[NX * NY], double psi[NX * NY], double tmp[NX 
     wavehip_neumann(neumann_bc);
 #endif
 #if OSCILLATE_LEFT
-    wavehip_drive(3);
+    wavehip_drive(3, xy_in);
 #endif
     rc = wv_evolve_wave_flat(wavehip_ctx(), phi, psi, tmp, xy_in, tc, tcc, tgamma);
     (void)laplace; (void)laplace1; (void)laplace2; (void)neumann_bc;

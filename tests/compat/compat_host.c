@@ -57,6 +57,9 @@
 #ifndef PRECOMPUTE_BC
 #define PRECOMPUTE_BC 0
 #endif
+#ifndef COMPARISON
+#define COMPARISON 0
+#endif
 
 /* the fields of global_pdes.c:518-528 the stepper reads */
 typedef struct {
@@ -121,10 +124,11 @@ int main(int argc, char **argv)
 {
     if (argc == 2 && strcmp(argv[1], "--params") == 0) {
         printf("nx=%d ny=%d bcond=%d twospeeds=%d oscillate_topbot=%d flat=%d fixed=%d neumann=%d "
-               "oscillate_left=%d floor=%d vmax=%.17g table=%d "
+               "oscillate_left=%d floor=%d vmax=%.17g table=%d comparison=%d "
                "kappa=%.17g kappa_sides=%.17g gamma_sides=%.17g kappa_topbot=%.17g gamma_topbot=%.17g\n",
                NX, NY, B_COND, TWOSPEEDS, OSCILLATE_TOPBOT, HOST_FLAT, HOST_FIXED, BC_NEUMANN, OSCILLATE_LEFT,
-               FLOOR, (double)VMAX, PRECOMPUTE_BC, KAPPA, KAPPA_SIDES, GAMMA_SIDES, KAPPA_TOPBOT, GAMMA_TOPBOT);
+               FLOOR, (double)VMAX, PRECOMPUTE_BC, COMPARISON, KAPPA, KAPPA_SIDES, GAMMA_SIDES, KAPPA_TOPBOT,
+               GAMMA_TOPBOT);
         return 0;
     }
     if (argc != 5) {

@@ -18,7 +18,7 @@ from conftest import say
 HERE = os.path.dirname(os.path.abspath(__file__))
 BIN = os.path.join(HERE, "compat", "bin")
 VARIANTS = ["rows_abs", "rows_per", "rows_dir_tb", "flat_vper", "flat_abs_fixed", "rows_ext", "rows_ext_wide",
-            "flat_ext", "flat_tbl_dir", "flat_tbl_lshape"]
+            "flat_ext", "flat_tbl_dir", "flat_tbl_lshape", "flat_cmp", "flat_cmp_ext"]
 
 
 def host_params(name):
@@ -47,7 +47,7 @@ def oracle_params(oracle, d):
                               oscillate_topbot=d["oscillate_topbot"], tb_shift=tbs,
                               kappa=d["kappa"], kappa_sides=d["kappa_sides"],
                               gamma_sides=d["gamma_sides"], kappa_topbot=d["kappa_topbot"],
-                              gamma_topbot=d["gamma_topbot"])
+                              gamma_topbot=d["gamma_topbot"], comparison=d["comparison"])
 
 
 def drive_cells(d):
@@ -56,6 +56,10 @@ def drive_cells(d):
     wave_3d.c:651-654, 989-993 (flat) or 358-376, 413, 575-579 (table stepper)"""
     nx, ny = d["nx"], d["ny"]
     cells = [(0, j) for j in range(1, ny - 1)]
+    if d["comparison"]:
+        # the interface rows overwrite the drive where they apply (wave_3d.c:873-982)
+        xy = d["xy0"]
+        cells = [(0, j) for j in range(1, ny - 1) if not (j in (ny // 2 - 1, ny // 2) and (d["twospeeds"] or xy[j]))]
     if d["flat"]:
         return cells + [(0, 0), (0, ny - 1)]
     return cells + [(0, 0)] + ([(0, nx - 1)] if nx - 1 < ny else [])
@@ -75,6 +79,7 @@ def oracle_expect(oracle, d, w, ncalls, mode):
         if d["floor"]:
             kw["vmax"] = d["vmax"]
         return oracle.run_table(p, phi, psi, n, xy, tcc, tc, tg, w["lap"][0], w["lap"][1], tmp=tmp, **kw)
+    d = dict(d, xy0=w["xy_in"][0])
     if d["neumann"] or d["oscillate_left"] or d["floor"]:
         n = 3 * ncalls if mode == "wave" else ncalls
         kw = {}

@@ -150,6 +150,9 @@ int check_params(Engine *e, const wv_params *p)
         return fail(e, WV_EINVAL, "bcond %d is not a B_COND code 0..4 or 10", p->bcond);
     if (p->bcond == WV_BC_LSHAPE && (p->form != WV_FORM_FP64 || p->stencil != 5))
         return fail(e, WV_EUNSUPPORTED, "BC_LSHAPE is provided for the FP64 form and the 5-point stencil");
+    if ((p->flags & WV_FLAG_COMPARISON) && (p->form != WV_FORM_FP64 || p->stencil != 5 || p->ny < 4 ||
+                                            p->bcond == WV_BC_LSHAPE))
+        return fail(e, WV_EUNSUPPORTED, "COMPARISON is provided for the FP64 form, 5-point, NY >= 4");
     if (p->stencil != 5 && p->stencil != 9) return fail(e, WV_EINVAL, "stencil must be 5 or 9");
     if (p->form != WV_FORM_FP64 && p->form != WV_FORM_F32SPLIT) return fail(e, WV_EINVAL, "unknown form %d", p->form);
     if (p->tb_depth < 0 || p->tb_depth > 8 || (p->tb_depth && !wv_have_depth(p->tb_depth)))
@@ -219,6 +222,7 @@ WvInst make_inst(const Engine *e, const Grid &G)
     d.prb_ptr = G.prb_ptr; d.prb_col = G.prb_col; d.prb_id = G.prb_id; d.prb_buf = G.prb_buf;
     d.prb_n = (int)G.h_pi.size(); d.prb_every = G.prb_every; d.prb_cap = G.prb_cap; d.prb_base = G.prb_base;
     d.tbl = G.tbl;
+    d.cmp = (G.p.flags & WV_FLAG_COMPARISON) != 0;
     d.tbl_ptr = G.tbl_ptr; d.tbl_col = G.tbl_col; d.tbl_n = G.tbl_n; d.tbl_off = G.tbl_off;
     d.nx = G.p.nx; d.ny = G.p.ny; d.ldy = G.ldy; d.ldw = G.ldw;
     d.bc = G.p.bcond; d.osc_tb = G.p.oscillate_topbot; d.tb_shift = G.p.tb_shift; d.coef = G.coef;
@@ -292,7 +296,9 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
         const int js = cb * wout - wv_halo(k);
         const int ibeg = sb * L, iend = std::min(ibeg + L, nx);
         // (BC_LSHAPE: every bulk cell sums its Laplacian in table order)
-        const bool interior = js >= 1 && js + 64 * WV_C <= ny - 1 && ibeg >= k && iend <= nx - k && G.tbl != 2;
+        // (COMPARISON: neighbours across the grid, one step per launch)
+        const bool interior = js >= 1 && js + 64 * WV_C <= ny - 1 && ibeg >= k && iend <= nx - k && G.tbl != 2 &&
+                              !(G.p.flags & WV_FLAG_COMPARISON);
         // (wrapped halos: any source, Neumann point or driven cell of the grid)
         const int bflags = (G.h_si.empty() ? 0 : WV_TILE_SRC) | (G.h_ni.empty() ? 0 : WV_TILE_NEU) |
                            (G.h_di.empty() ? 0 : WV_TILE_DRV) | (G.h_pi.empty() ? 0 : WV_TILE_PRB);
@@ -492,7 +498,7 @@ int set_tables(Engine *e, int k, const short *xy, const double *tcc, const doubl
     if (!xy || !tcc || !tc || !tg) return fail(e, WV_EINVAL, "table pointer is NULL");
     Grid &G = e->g[k];
     const int nx = G.p.nx, ny = G.p.ny, ldw = G.ldw, ldy = G.ldy;
-    const bool ts = G.p.twospeeds != 0;
+    const bool ts = G.p.twospeeds != 0, cmp = (G.p.flags & WV_FLAG_COMPARISON) != 0;
     std::vector<uint64_t> act((size_t)nx * ldw, 0), eact((size_t)nx * ldw, 0), med((size_t)nx * ldw, 0);
     // distinct (tcc, tgamma) pairs over the cells a step or the energy reads
     std::vector<std::pair<uint64_t, uint64_t>> cls;
@@ -501,7 +507,8 @@ int set_tables(Engine *e, int k, const short *xy, const double *tcc, const doubl
     for (int i = 0; i < nx; i++)
         for (int j = 0; j < ny; j++) {
             const size_t c = (size_t)i * ny + j;
-            const bool edge = i == 0 || i == nx - 1 || j == 0 || j == ny - 1;
+            // (COMPARISON: the interface columns take the edge rule, wave_3d.c:875, 933)
+            const bool edge = i == 0 || i == nx - 1 || j == 0 || j == ny - 1 || (cmp && (j == ny / 2 - 1 || j == ny / 2));
             const bool a = ts || (edge ? xy[c] != 0 : xy[c] > 0);
             const bool en = ts || xy[c] != 0;
             const size_t w = (size_t)i * ldw + (j >> 6);
@@ -908,6 +915,8 @@ int set_laplacian(Engine *e, int k, const int *nn, const int *nb)
     if (nn && !nb) return fail(e, WV_EINVAL, "neighbour index table is NULL");
     if (nn && !G.h_ni.empty())
         return fail(e, WV_EUNSUPPORTED, "the neighbour-table stepper has no Neumann points (wave_3d.c:345-597)");
+    if (nn && (G.p.flags & WV_FLAG_COMPARISON))
+        return fail(e, WV_EUNSUPPORTED, "the neighbour-table stepper has no COMPARISON (wave_3d.c:345-597)");
     const int nx = G.p.nx, ny = G.p.ny;
     const long ncell = (long)nx * ny;
     const bool lshape = G.p.bcond == WV_BC_LSHAPE;
@@ -1059,10 +1068,12 @@ int ready(Engine *e)
     return WV_OK;
 }
 
+// grids whose steps read neighbours anywhere in the field: the
+// neighbour-table stepper and COMPARISON
 bool any_table(const Engine *e)
 {
     for (const auto &G : e->g)
-        if (G.tbl) return true;
+        if (G.tbl || (G.p.flags & WV_FLAG_COMPARISON)) return true;
     return false;
 }
 
@@ -1237,13 +1248,13 @@ int set_wells_tables(Engine *e, int k, const short *xy, int nw, const double *xc
     if (!(sigma > 0) || !(box[1] > box[0]) || !(box[3] > box[2])) return fail(e, WV_EINVAL, "sigma or box");
     Grid &G = e->g[k];
     const int nx = G.p.nx, ny = G.p.ny, ldw = G.ldw, ldy = G.ldy;
-    const bool ts = G.p.twospeeds != 0;
+    const bool ts = G.p.twospeeds != 0, cmp = (G.p.flags & WV_FLAG_COMPARISON) != 0;
     // mask planes, the rules of set_tables
     std::vector<uint64_t> act((size_t)nx * ldw, 0), eact((size_t)nx * ldw, 0), nzp((size_t)nx * ldw, 0);
     for (int i = 0; i < nx; i++)
         for (int j = 0; j < ny; j++) {
             const size_t c = (size_t)i * ny + j;
-            const bool edge = i == 0 || i == nx - 1 || j == 0 || j == ny - 1;
+            const bool edge = i == 0 || i == nx - 1 || j == 0 || j == ny - 1 || (cmp && (j == ny / 2 - 1 || j == ny / 2));
             const size_t w = (size_t)i * ldw + (j >> 6);
             const uint64_t b = 1ull << (j & 63);
             if (ts || (edge ? xy[c] != 0 : xy[c] > 0)) act[w] |= b;

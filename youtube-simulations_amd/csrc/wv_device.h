@@ -177,6 +177,61 @@ __device__ inline double edge_fp64(const Par &d, int rm, int jm, double x, doubl
     }
 }
 
+// COMPARISON, wave_3d.c:736-990: the flat grid in two halves, j < NY/2 and
+// j >= NY/2.  Cells of the top and bottom rows (periodic wraps stay inside
+// their half) and of the interface columns jtop = NY/2-1 and jbot = NY/2, with
+// every neighbour read from the step's input field u0 (one step per launch).
+// The reference's quirks are kept: the periodic jbot branch adds the cell
+// itself, and the VPER_HABS corner (0, jbot) reads cell (1, 0).
+__device__ inline double cmp_fp64(const Par &d, const double *u0, int rm, int jm, double x, double y, double cc,
+                                  double g, double tcv)
+{
+    const int nx = d.nx, ny = d.ny, bc = d.bc, jtop = ny / 2 - 1, jbot = ny / 2;
+    auto U = [&](int i, int j) { return gc(u0)[(size_t)i * d.ldy + j]; };
+    const int ip = rm + 1 == nx ? nx - 1 : rm + 1, im = rm == 0 ? 0 : rm - 1;
+    const int ipw = rm + 1 == nx ? 0 : rm + 1, imw = rm == 0 ? nx - 1 : rm - 1;
+    double delta;
+    if (jm == ny - 1 || jm == 0) {
+        const bool top = jm == ny - 1;
+        const int jin = top ? ny - 2 : 1, jseam = top ? jbot : jtop;
+        if (d.osc_tb && rm < d.tb_shift && rm < nx - 1 && rm > 0) {
+            delta = (U(rm + 1, jm) + U(rm - 1, jm)) + -2.0 * x;
+            return leap(x, y, delta, cc, d.kappa, g);
+        }
+        switch (bc) {
+        case 0:
+            delta = ((U(ip, jm) + U(im, jm)) + U(rm, jin)) - 3.0 * x;
+            return leap(x, y, delta, cc, d.kappa, g);
+        case 1:
+            delta = (((U(ipw, jm) + U(imw, jm)) + U(rm, jin)) + U(rm, jseam)) - 4.0 * x;
+            return leap(x, y, delta, cc, d.kappa, g);
+        case 3:
+            if (rm == 0) return absorb(x, y, U(1, jm), tcv, d.ks, d.gs);
+            delta = (((U(ip, jm) + U(im, jm)) + U(rm, jin)) + U(rm, jseam)) - 4.0 * x;
+            return leap(x, y, delta, cc, d.kappa, g);
+        default:
+            return absorb(x, y, U(rm, jin), tcv, d.kt, d.gt);
+        }
+    }
+    // interface columns: the neighbour inside the half, the seam of the half
+    const bool low = jm == jtop;
+    const int jin = low ? jtop - 1 : jbot + 1;
+    switch (bc) {
+    case 1:
+        delta = (((U(ipw, jm) + U(imw, jm)) + U(rm, jin)) + U(rm, low ? 0 : jbot)) - 4.0 * x;
+        return leap(x, y, delta, cc, d.kappa, g);
+    case 2:
+        return absorb(x, y, U(rm, jin), tcv, d.kt, d.gt);
+    case 3:
+        if (rm == 0) return absorb(x, y, low ? U(1, jtop) : U(1, 0), tcv, d.ks, d.gs);
+        delta = (((U(ip, jm) + U(im, jm)) + U(rm, jin)) + U(rm, low ? 0 : ny - 1)) - 4.0 * x;
+        return leap(x, y, delta, cc, d.kappa, g);
+    default:  // DIRICHLET, ABS_REFLECT
+        delta = ((U(ip, jm) + U(im, jm)) + U(rm, jin)) - 3.0 * x;
+        return leap(x, y, delta, cc, d.kappa, g);
+    }
+}
+
 // Column geometry of one lane: FAST lanes lie in 1..ny-2 and read/write
 // contiguously; other lanes map every column (wrap or clamp) one by one.
 template <bool FAST>

@@ -33,8 +33,8 @@
 #define WV_CM_ALL 3
 // any mixture, plus Neumann ghost points, driven cells and FLOOR (FP64 form)
 #define WV_CM_ALLX 4
-// WV_CM_ALLX plus the neighbour-table Laplacian (FP64 form, one step per
-// launch, 5-point)
+// WV_CM_ALLX plus the neighbour-table Laplacian and COMPARISON (FP64 form,
+// one step per launch, 5-point)
 #define WV_CM_TBL 5
 
 // rows in flight in a stepping kernel's prefetch ring (3 or 6)
@@ -79,6 +79,7 @@ struct WvInst {
     // 2 BC_LSHAPE.  Cells whose Laplacian is not the plain one, CSR by row:
     // column, neighbour count, four neighbour offsets i*ldy + j.
     int tbl;
+    int cmp;                             // COMPARISON: two halves (one step per launch)
     const int *tbl_ptr, *tbl_col, *tbl_n;
     const long long *tbl_off;
     // probes (FP64 form): cells whose value is recorded after every

@@ -79,6 +79,8 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
     const bool xprb = XT && !INTERIOR && (t.flags & WV_TILE_PRB);
     static_assert(!TB || (XT && K == 1 && ST == 5), "table Laplacians: 5-point, one step per launch");
     const int tblm = TB && !INTERIOR ? d.tbl : 0;  // 1: table Laplacians, 2: BC_LSHAPE
+    const bool cmpm = TB && !INTERIOR && d.cmp;    // COMPARISON
+    const int jtop = ny / 2 - 1, jbot = ny / 2;
     const double *u0 = d.u[ia], *um1 = d.u[ib];
     const uint64_t *act = d.act, *med = d.med;
     const double *tcp = d.tc, *tccf = d.tccf, *tgf = d.tgf;
@@ -175,7 +177,9 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
                         coef_fp64<COEF>(p, tccf, tgf, rm, L.jm[c], (mb >> c) & 1u, cc, g);
                     }
                     double v;
-                    if (rowint && L.colint[c]) {
+                    const bool cmpcol = cmpm && (L.jm[c] == jtop || L.jm[c] == jbot);
+                    const bool bulk = rowint && L.colint[c] && !cmpcol;
+                    if (bulk) {
                         double delta;
                         if (ST == 9) {
                             const double NE = (c < C - 1) ? Xe[cn1] : en;
@@ -206,6 +210,8 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
                             double delta = 0.0;
                             (void)tbl_delta(xq, u0, rm, L.jm[c], x, 0, delta);
                             v = edge_table_fp64(p, rm, L.jm[c], x, y, E, Wv, N, S, delta, cc, g, tcv);
+                        } else if (cmpm && (cmpcol || L.jm[c] == 0 || L.jm[c] == ny - 1)) {
+                            v = cmp_fp64(p, u0, rm, L.jm[c], x, y, cc, g, tcv);
                         } else {
                             v = edge_fp64(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv);
                         }
@@ -216,7 +222,7 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
                         // two neighbours unless it is an edge cell the edge loops write
                         int code = 0;
                         double w1 = 0, w2 = 0;
-                        if (xneu && neu_find(xq, rm, L.jm[c], code, w1, w2) && !(w && !(rowint && L.colint[c]))) {
+                        if (xneu && neu_find(xq, rm, L.jm[c], code, w1, w2) && !(w && !bulk)) {
                             double t = w1 * pick9(code & 15, wp, Wv, wn, S, x, N, ep, E, en);
                             t = t + w2 * pick9(code >> 4, wp, Wv, wn, S, x, N, ep, E, en);
                             v = t;

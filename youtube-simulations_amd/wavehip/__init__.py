@@ -23,6 +23,7 @@ BC_LSHAPE = 10
 FORM_FP64, FORM_F32SPLIT = 0, 1
 FLAG_TABLES_FIXED = 1
 FLAG_FULL_FORMULA = 2
+FLAG_COMPARISON = 4
 
 # every symbol include/wavehip.h declares
 EXPORTS = [
