@@ -14,7 +14,9 @@ reported.  Inputs are resident in HBM before the timed region starts.
 Prints one JSON line.  The CPU baseline is the oracle (this repository's C +
 OpenMP restatement of the reference evolve_wave_half, pinned bit-exact to the
 reference's own code) timed on a bounded number of steps of the same FP64
-workload on the host cores.
+workload on the host cores.  Two extra legs (off with --no-frame) time the
+frame consumer (8-bit frames of the drawing path's colours against
+downloading the field) and the media builder (init_ior_2d's random wells).
 """
 import argparse
 import json
