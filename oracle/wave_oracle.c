@@ -379,6 +379,10 @@ static void energy_field(const orc_params *p, const double *phi, const double *p
             if (!(xy_in[k] != 0 || p->twospeeds)) { e[k] = 0.0; continue; }
             int ip = i + 1 == NX ? NX - 1 : i + 1, im = i == 0 ? 0 : i - 1;
             int jp = j + 1 == NY ? NY - 1 : j + 1, jm = j == 0 ? 0 : j - 1;
+            /* COMPARISON: no gradient across the halves, compute_energy_mod
+             * wave_common.c:1965-1971 */
+            if (p->comparison && jp == NY / 2) jp = NY / 2 - 1;
+            if (p->comparison && jm == NY / 2 - 1) jm = NY / 2;
             double c = phi[k], v = c - psi[k];
             double a1 = phi[(size_t)ip * NY + j] - c, a2 = c - phi[(size_t)im * NY + j];
             double b1 = phi[(size_t)i * NY + jp] - c, b2 = c - phi[(size_t)i * NY + jm];

@@ -210,7 +210,8 @@ double orc_energy_cell(const orc_params *p, const double *phi, const double *psi
                        double e_scale, int i, int j);
 
 /* Build-defined global energy: per-cell energy with c^2 taken from tcc
- * (cells with xy_in != 0, or all cells if TWOSPEEDS; 0 elsewhere), summed
+ * (cells with xy_in != 0, or all cells if TWOSPEEDS; 0 elsewhere; with
+ * COMPARISON no gradient across the halves, wave_common.c:1965-1971), summed
  * in a fixed order: for each row, 64 lane-strided partials then a halving
  * tree (32,16,...,1); the NX row sums are reduced the same way. */
 double orc_energy_total(const orc_params *p, const double *phi, const double *psi,

@@ -95,3 +95,19 @@ def test_comparison_argument_checks(wh):
     with pytest.raises(wh.WaveError):
         W.set_laplacian(workloads.laplacian_table(40, 30, 0))
     W.close()
+
+
+def test_comparison_energy(wh, oracle):
+    """wv_energy with COMPARISON takes no gradient across the halves, as
+    compute_energy_mod (wave_common.c:1965-1971); same bits as the oracle."""
+    from wavehip import workloads
+    w = workloads.random_case(90, 64, 8, bcond=1)
+    W = wh.Wave(dict(w["params"], flags=wh.FLAG_COMPARISON, e_scale=3.0))
+    W.set_tables(w["xy_in"], w["tcc"], w["tc"], w["tgamma"])
+    W.set_state(w["phi"], w["psi"])
+    e = W.energy()
+    W.close()
+    p = oracle.make_params(90, 64, bcond=1, comparison=1)
+    ref = oracle.energy_total(p, w["phi"], w["psi"], w["xy_in"], w["tcc"], 3.0)
+    p0 = oracle.make_params(90, 64, bcond=1)
+    assert e == ref and e != oracle.energy_total(p0, w["phi"], w["psi"], w["xy_in"], w["tcc"], 3.0)

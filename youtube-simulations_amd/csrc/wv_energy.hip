@@ -24,7 +24,11 @@ k_energy_rows(const WvInst *__restrict__ I, const int2 *__restrict__ rows, doubl
     for (int j = lane; j < ny; j += 64) {
         double e = 0.0;
         if ((gc(d.eact)[(size_t)i * d.ldw + (j >> 6)] >> (j & 63)) & 1u) {
-            const int jp = j + 1 == ny ? ny - 1 : j + 1, jn = j == 0 ? 0 : j - 1;
+            int jp = j + 1 == ny ? ny - 1 : j + 1, jn = j == 0 ? 0 : j - 1;
+            // COMPARISON: no gradient across the halves (compute_energy_mod,
+            // wave_common.c:1965-1971)
+            if (d.cmp && jp == ny / 2) jp = ny / 2 - 1;
+            if (d.cmp && jn == ny / 2 - 1) jn = ny / 2;
             double c, v, uip, uim, ujp, ujm;
             if (SPLIT) {
                 auto U = [&](int a, int b) {
