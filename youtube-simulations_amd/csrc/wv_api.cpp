@@ -319,10 +319,13 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
         for (int i = ibeg - k; all && i < iend + k; i++)
             for (int j = js; j < js + 64 * WV_C; j++)
                 if (!((G.h_wr[(size_t)i * G.ldw + (j >> 6)] >> (j & 63)) & 1u)) { all = false; break; }
+        // a tile that adds a point source runs on the boundary path, so the
+        // interior variants carry no source code at all
         bool src = false;
         for (size_t q = 0; q < G.h_si.size() && !src; q++)
             src = G.h_si[q] >= ibeg - k && G.h_si[q] < iend + k && G.h_sj[q] >= js && G.h_sj[q] < js + 64 * WV_C;
-        return WV_TILE_INTERIOR | (all ? WV_TILE_ALLACT : 0) | (src ? WV_TILE_SRC : 0);
+        if (src) return (int)WV_TILE_SRC;
+        return WV_TILE_INTERIOR | (all ? WV_TILE_ALLACT : 0);
     };
     // one list in XCD-interleave order; boundary tiles keep their place so
     // that they run among the interior tiles instead of as a tail

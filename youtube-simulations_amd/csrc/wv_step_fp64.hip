@@ -69,7 +69,7 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
     const Cols<FAST> L(jb - H + (int)threadIdx.x * C, jb, WOUT, ny, wrapj);
     const int ibeg = t.sb * d.rows_per_tile;
     const int iend = min(ibeg + d.rows_per_tile, nx);
-    const bool has_src = (t.flags & WV_TILE_SRC) && src.any();
+    const bool has_src = !INTERIOR && (t.flags & WV_TILE_SRC) && src.any();  // (source tiles are boundary tiles)
     // Neumann ghost points, driven cells and FLOOR (XT: built into the kernel
     // variant for grids that use them; boundary-path tiles only)
     const Xtra xq(d);
@@ -111,7 +111,14 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
         qm = COEF == WV_COEF_TWO ? lane_word(med, ldw, rm, L) : 0u;
     };
 #pragma unroll
-    for (int q = 0; q < WV_RING; q++) fetch(min(s0 + q, slast), QU[q], QP[q], QB[q], QM[q]);
+    for (int q = 0; q < WV_RING; q++) {
+        fetch(min(s0 + q, slast), QU[q], QP[q], QB[q], QM[q]);
+        // issue the ring slots in order: the loop waits for slot q with a
+        // count of the loads issued after it, and a preheader that issued them
+        // in another order would make that count (the minimum over both ways
+        // into the loop) drain the ring one slot early
+        __builtin_amdgcn_sched_barrier(0);
+    }
 
     // per-cell coefficients: level lv computes row s-2lv at iteration s and
     // row s+1-2lv at the next one, so each level loads its next row's values
@@ -140,8 +147,9 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
         // before level lv-1 is advanced -- no temporaries, no copies
 #pragma unroll
         for (int lv = K; lv >= 1; lv--) {
-            // level lv is needed at rows [ibeg-K+lv, iend+K-lv) only (wave-uniform)
-            const bool live = s >= ibeg - K + 3 * lv && s < iend + K + lv;
+            // level lv is needed at rows [ibeg-K+lv, iend+K-lv) only (wave-uniform),
+            // i.e. at iterations s in [ibeg-K+3lv, iend+K+lv): one unsigned compare
+            const bool live = (unsigned)(s - s0 - 3 * lv) < (unsigned)(iend - ibeg + 2 * (K - lv));
             if (live) {
                 const int r = s - 2 * lv;
                 // rows outside the dependence cone of the strip's output are never

@@ -67,7 +67,7 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
     const Cols<FAST> L(jb - H + (int)threadIdx.x * C, jb, WOUT, ny, wrapj);
     const int ibeg = t.sb * d.rows_per_tile;
     const int iend = min(ibeg + d.rows_per_tile, nx);
-    const bool has_src = (t.flags & WV_TILE_SRC) && src.any();
+    const bool has_src = !INTERIOR && (t.flags & WV_TILE_SRC) && src.any();  // (source tiles are boundary tiles)
     const uint64_t *act = d.act, *med = d.med;
     const double *tcp = d.tc, *tccf = d.tccf, *tgf = d.tgf;
     const float *tccf_hi = d.tccf_hi, *tccf_lo = d.tccf_lo, *tgf32 = d.tgf32;
@@ -102,7 +102,14 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
         qm = COEF == WV_COEF_TWO ? lane_word(med, ldw, rm, L) : 0u;
     };
 #pragma unroll
-    for (int q = 0; q < WV_RING; q++) fetch(min(s0 + q, slast), QH[q], QL[q], QV[q], QB[q], QM[q]);
+    for (int q = 0; q < WV_RING; q++) {
+        fetch(min(s0 + q, slast), QH[q], QL[q], QV[q], QB[q], QM[q]);
+        // issue the ring slots in order: the loop waits for slot q with a
+        // count of the loads issued after it, and a preheader that issued them
+        // in another order would make that count (the minimum over both ways
+        // into the loop) drain the ring one slot early
+        __builtin_amdgcn_sched_barrier(0);
+    }
 
     // per-cell coefficients, one iteration ahead per level (see the FP64 kernel)
     float TH[K][C], TL[K][C], TG[K][C];
@@ -150,7 +157,8 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
                     Dse_l = from_prev(le0) - l0;
                 }
             }
-            const bool live = s >= ibeg - K + 3 * lv && s < iend + K + lv;
+            // level lv is live at iterations s in [ibeg-K+3lv, iend+K+lv)
+            const bool live = (unsigned)(s - s0 - 3 * lv) < (unsigned)(iend - ibeg + 2 * (K - lv));
             if (live) {
                 const int r = s - 2 * lv;
                 const int rm = INTERIOR ? min(max(r, 0), nx - 1) : map_idx(r, nx, wrapi);
