@@ -103,6 +103,7 @@ struct Engine {
     std::vector<short> stm;
     bool dropin_tables = false;
     bool xcd_order = true;  // WAVEHIP_TILE_ORDER=linear turns the XCD interleave off
+    bool slow_first = true;  // WAVEHIP_TILE_ORDER=inplace: slower tiles keep their place in the strip order
     // per-kernel timing of the last wv_step_timed: an event pair around each
     // stepping launch
     bool time_kernels = false;
@@ -137,7 +138,21 @@ int round_up(int a, int b) { return (a + b - 1) / b * b; }
 // measured on MI355X at 4096^2 (5-point, uniform medium): split form 6 steps
 // per launch in 12-row strips, FP64 8 steps in 16-row strips
 int default_depth(const Engine *e) { return e->form == WV_FORM_F32SPLIT ? 6 : 8; }
-int default_rows(const Engine *e) { return e->form == WV_FORM_F32SPLIT ? 12 : 16; }
+// Rows per strip without tuning: the measured best at 4096^2 (split 16,
+// FP64 24), shortened for smaller grids until the launch has enough tiles to
+// fill the chip several times over (256 CUs x 4 SIMDs x 4 waves)
+int default_rows(const Engine *e)
+{
+    const int pref = e->form == WV_FORM_F32SPLIT ? 16 : 24;
+    const int wout = wv_tile_cols(e->K > 0 ? e->K : default_depth(e));
+    for (int L : {24, 16, 12}) {
+        if (L > pref) continue;
+        long tiles = 0;
+        for (const Grid &G : e->g) tiles += (long)((G.p.ny + wout - 1) / wout) * ((G.p.nx + L - 1) / L);
+        if (tiles >= 8192) return L;
+    }
+    return 8;
+}
 int depth(const Engine *e) { return e->K > 0 ? e->K : default_depth(e); }
 int rows(const Engine *e) { return e->L > 0 ? e->L : default_rows(e); }
 
@@ -327,8 +342,9 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
         if (src) return (int)WV_TILE_SRC;
         return WV_TILE_INTERIOR | (all ? WV_TILE_ALLACT : 0);
     };
-    // one list in XCD-interleave order; boundary tiles keep their place so
-    // that they run among the interior tiles instead of as a tail
+    // one list in XCD-interleave order.  Each XCD starts with its slowest tiles
+    // (boundary tiles take several times as long as interior ones): started
+    // last, they would run on after every interior tile had finished
     std::vector<WvTile> h, hb;
     if (e->xcd_order) {
         const int X = 8;
@@ -339,6 +355,14 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
             for (size_t q = S * x / X; q < S * (x + 1) / X; q++)
                 for (int cb = 0; cb < strips[q].ncb; cb++)
                     per[x].push_back(WvTile{strips[q].inst, cb, strips[q].sb, flags_of(strips[q].inst, cb, strips[q].sb)});
+        if (e->slow_first) {
+            // boundary tiles, then interior tiles with inactive cells, then the rest
+            auto rank = [](const WvTile &t) {
+                return !(t.flags & WV_TILE_INTERIOR) ? 0 : (!(t.flags & WV_TILE_ALLACT) ? 1 : 2);
+            };
+            for (auto &v : per)
+                std::stable_sort(v.begin(), v.end(), [&](const WvTile &a, const WvTile &b) { return rank(a) < rank(b); });
+        }
         size_t len = 0;
         for (const auto &v : per) len = std::max(len, v.size());
         for (size_t j = 0; j < len; j++)
@@ -378,7 +402,10 @@ int engine_create(Engine **out, int n, const wv_params *ps)
     e->stencil = ps[0].stencil;
     e->K = ps[0].tb_depth;
     e->L = ps[0].rows_per_tile;
-    if (const char *o = getenv("WAVEHIP_TILE_ORDER")) e->xcd_order = strcmp(o, "linear") != 0;
+    if (const char *o = getenv("WAVEHIP_TILE_ORDER")) {
+        e->xcd_order = strcmp(o, "linear") != 0;
+        e->slow_first = strcmp(o, "inplace") != 0;
+    }
     int ndev = 0;
     HIPCHK(e, hipGetDeviceCount(&ndev));
     if (e->device < 0 || e->device >= ndev) return fail(e, WV_EINVAL, "device %d not present (%d devices)", e->device, ndev);

@@ -138,6 +138,20 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
         for (int lv = 1; lv <= K; lv++) fetch_coef(lv, s0 - 2 * lv);
     }
 
+    // boundary tiles: the tc value of each level's next row, loaded one
+    // iteration ahead (a load inside the edge branch would make the level wait
+    // for it, and for every load in flight, at the branch join)
+    constexpr bool PTV = !INTERIOR;
+    double TV[K][C];
+    auto fetch_tv = [&](int lv, int row) {
+        const int rm = map_idx(row, nx, wrapi);
+#pragma unroll
+        for (int c = 0; c < C; c++) TV[lv - 1][c] = gc(tcp)[(size_t)rm * ldy + L.jm[c]];
+    };
+    if (PTV) {
+#pragma unroll
+        for (int lv = 1; lv <= K; lv++) fetch_tv(lv, s0 - 2 * lv);
+    }
     auto iter = [&](int s, auto phase) {
         constexpr int PR = decltype(phase)::value;  // ring slot of row s
         constexpr int P = PR % 3;
@@ -212,7 +226,7 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
                         }
                         v = DAMP ? leap(x, y, delta, cc, p.kappa, g) : leap_undamped(x, y, delta, cc);
                     } else {
-                        const double tcv = gc(tcp)[(size_t)rm * ldy + L.jm[c]];
+                        const double tcv = PTV ? TV[lv - 1][c] : gc(tcp)[(size_t)rm * ldy + L.jm[c]];
                         if (TB && tblm) {
                             // an edge formula that uses no Laplacian has no table entry
                             double delta = 0.0;
@@ -306,6 +320,7 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
             // unconditional, like the ring (a load under the branch would
             // make the compiler wait for every load in flight at the join)
             if (PREF) fetch_coef(lv, s + 1 - 2 * lv);
+            if (PTV) fetch_tv(lv, s + 1 - 2 * lv);
         }
         // level 0 from the prefetch ring, then request row s+3 into the freed slot
 #pragma unroll

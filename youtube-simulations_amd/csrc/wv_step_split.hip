@@ -127,6 +127,20 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
 #pragma unroll
         for (int lv = 1; lv <= K; lv++) fetch_coef(lv, s0 - 2 * lv);
     }
+    // boundary tiles: the tc value of each level's next row, loaded one
+    // iteration ahead (a load inside the edge branch would make the level wait
+    // for it, and for every load in flight, at the branch join)
+    constexpr bool PTV = !INTERIOR;
+    double TV[K][C];
+    auto fetch_tv = [&](int lv, int row) {
+        const int rm = map_idx(row, nx, wrapi);
+#pragma unroll
+        for (int c = 0; c < C; c++) TV[lv - 1][c] = gc(tcp)[(size_t)rm * ldy + L.jm[c]];
+    };
+    if (PTV) {
+#pragma unroll
+        for (int lv = 1; lv <= K; lv++) fetch_tv(lv, s0 - 2 * lv);
+    }
     // differences kept from the previous iteration of each level (SHARE):
     // row r+1 minus row r of hi and lo, and the two diagonal ones (9-point)
     float DEh[K], DEl[K], DNEh[K], DNEl[K], DSEh[K], DSEl[K];
@@ -290,7 +304,7 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
                         const double S = (double)hS + (double)lS;
                         double cc, g;
                         coef_fp64<COEF>(p, tccf, tgf, rm, L.jm[c], mbit, cc, g);
-                        const double tcv = gc(tcp)[(size_t)rm * ldy + L.jm[c]];
+                        const double tcv = PTV ? TV[lv - 1][c] : gc(tcp)[(size_t)rm * ldy + L.jm[c]];
                         double out = edge_fp64(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv);
                         if (has_src) {
                             const long g1 = gbase + lv - 1;
@@ -342,6 +356,7 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
             }
             // unconditional, like the ring
             if (PREF) fetch_coef(lv, s + 1 - 2 * lv);
+            if (PTV) fetch_tv(lv, s + 1 - 2 * lv);
         }
 #pragma unroll
         for (int c = 0; c < C; c++) {
