@@ -278,15 +278,15 @@ void drop_tiles(Engine *e)
     drop_graphs(e);
 }
 
-// Tile table of a launch of depth k: every instance, row strips of rows(e)
-// rows, column blocks of wv_tile_cols(k) columns.  Workgroups are dealt to
-// the 8 XCDs round-robin by workgroup id, so position 8j + x of the table is
-// the j-th tile XCD x runs.  Each XCD gets one contiguous block of strips and
-// runs it strip after strip: the K halo rows a tile reads above and below
-// its strip were read moments before by the neighbouring strip on the same
-// XCD, so they come from that XCD's L2 instead of HBM, and so do the halo
-// columns two neighbouring tiles of a strip share.  Tiles with inst = -1 only
-// pad the shorter blocks.
+// Tile table of a launch of depth k: every instance, row strips (of rows(e)
+// rows where a tile can be interior), column blocks of wv_tile_cols(k)
+// columns.  Workgroups are dealt to the 8 XCDs round-robin by workgroup id,
+// so position 8j + x of the table is the j-th tile XCD x runs.  Each XCD gets
+// one contiguous run of strips in row order: the K halo rows a tile reads
+// above and below its strip were read shortly before by the neighbouring
+// strip on the same XCD, so they come from that XCD's L2 instead of HBM, and
+// so do the halo columns two neighbouring tiles of a strip share.  Tiles with
+// inst = -1 only pad the shorter runs.
 int get_tiles(Engine *e, int k, Engine::Tiles *out)
 {
     auto it = e->tiles.find(k);
@@ -294,22 +294,21 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
         *out = it->second;
         return WV_OK;
     }
-    const int wout = wv_tile_cols(k), L = rows(e);
-    struct Strip { int inst, sb, ncb; };
-    std::vector<Strip> strips;
-    for (int q = 0; q < e->n; q++) {
-        const Grid &G = e->g[q];
-        const int ncb = (G.p.ny + wout - 1) / wout, nsb = (G.p.nx + L - 1) / L;
-        for (int sb = 0; sb < nsb; sb++) strips.push_back(Strip{q, sb, ncb});
-    }
+    // (a tile's row count lives in the upper bits of its flags)
+    const int wout = wv_tile_cols(k), L = std::min(rows(e), 32767);
+    // Strips of L rows, except where a tile cannot be interior for its
+    // position: the first and last rows of a grid and the column blocks that
+    // reach its outer columns get strips of at most 8 rows.  Those tiles run
+    // the edge formulas and take several times as long per row; short, they
+    // finish within the launch instead of setting its length.
+    const int Lb = std::min(L, 8);
     // flags: INTERIOR when every row and column the tile reads lies inside the
     // outer box (no edge formula, no wrapping); ALLACT when, in addition,
     // every one of those cells is written by a step
-    auto flags_of = [&](int q, int cb, int sb) {
+    auto flags_of = [&](int q, int cb, int ibeg, int iend) {
         const Grid &G = e->g[q];
         const int nx = G.p.nx, ny = G.p.ny;
         const int js = cb * wout - wv_halo(k);
-        const int ibeg = sb * L, iend = std::min(ibeg + L, nx);
         // (BC_LSHAPE: every bulk cell sums its Laplacian in table order)
         // (COMPARISON: neighbours across the grid, one step per launch)
         const bool interior = js >= 1 && js + 64 * WV_C <= ny - 1 && ibeg >= k && iend <= nx - k && G.tbl != 2 &&
@@ -342,21 +341,58 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
         if (src) return (int)WV_TILE_SRC;
         return WV_TILE_INTERIOR | (all ? WV_TILE_ALLACT : 0);
     };
-    // one list in XCD-interleave order.  Each XCD starts with its slowest tiles
-    // (boundary tiles take several times as long as interior ones): started
-    // last, they would run on after every interior tile had finished
+    // every tile in grid order: by grid, first row, column block
+    std::vector<WvTile> tl;
+    for (int q = 0; q < e->n; q++) {
+        const Grid &G = e->g[q];
+        const int nx = G.p.nx, ny = G.p.ny;
+        const int ncb = (ny + wout - 1) / wout;
+        // grids whose every tile is a boundary tile keep strips of L rows
+        const bool allb = G.tbl == 2 || (G.p.flags & WV_FLAG_COMPARISON) || G.vmax > 0;
+        const int lb = allb ? L : Lb;
+        // row cuts of the inner column blocks: lb rows, strips of L, lb rows
+        std::vector<int> cut{0};
+        const int head = std::min(lb, nx), tail = std::max(head, nx - lb);
+        if (head > 0) cut.push_back(head);
+        for (int i = head + L; i < tail; i += L) cut.push_back(i);
+        if (tail > cut.back()) cut.push_back(tail);
+        if (nx > cut.back()) cut.push_back(nx);
+        // row cuts of the outer column blocks: strips of lb rows
+        std::vector<int> cutb;
+        for (int i = 0; i < nx; i += lb) cutb.push_back(i);
+        cutb.push_back(nx);
+        auto outer = [&](int cb) {
+            const int js = cb * wout - wv_halo(k);
+            return !(js >= 1 && js + 64 * WV_C <= ny - 1);
+        };
+        // merge the two cut lists so tiles come out ordered by first row
+        size_t a = 0, b = 0;
+        while (a + 1 < cut.size() || b + 1 < cutb.size()) {
+            const bool takeb = b + 1 < cutb.size() && (a + 1 >= cut.size() || cutb[b] < cut[a]);
+            const int i0 = takeb ? cutb[b] : cut[a], i1 = takeb ? cutb[b + 1] : cut[a + 1];
+            for (int cb = 0; cb < ncb; cb++)
+                if (outer(cb) == takeb)
+                    tl.push_back(WvTile{q, cb, i0, flags_of(q, cb, i0, i1) | ((i1 - i0) << WV_TILE_ROWS_SHIFT)});
+            (takeb ? b : a)++;
+        }
+    }
+    // one list in XCD-interleave order.  Each XCD takes one contiguous run of
+    // the grid order holding an eighth of the rows x columns, and starts with
+    // its slowest tiles (boundary tiles, then interior tiles with inactive
+    // cells): started last, they would run on after the rest had finished
     std::vector<WvTile> h, hb;
     if (e->xcd_order) {
         const int X = 8;
-        // XCD x: strips [x*S/X, (x+1)*S/X), strip-major
         std::vector<std::vector<WvTile>> per(X);
-        const size_t S = strips.size();
-        for (int x = 0; x < X; x++)
-            for (size_t q = S * x / X; q < S * (x + 1) / X; q++)
-                for (int cb = 0; cb < strips[q].ncb; cb++)
-                    per[x].push_back(WvTile{strips[q].inst, cb, strips[q].sb, flags_of(strips[q].inst, cb, strips[q].sb)});
+        double total = 0;
+        for (const WvTile &t : tl) total += t.flags >> WV_TILE_ROWS_SHIFT;
+        double acc = 0;
+        for (const WvTile &t : tl) {
+            const int x = std::min(X - 1, (int)(acc * X / std::max(total, 1.0)));
+            per[x].push_back(t);
+            acc += t.flags >> WV_TILE_ROWS_SHIFT;
+        }
         if (e->slow_first) {
-            // boundary tiles, then interior tiles with inactive cells, then the rest
             auto rank = [](const WvTile &t) {
                 return !(t.flags & WV_TILE_INTERIOR) ? 0 : (!(t.flags & WV_TILE_ALLACT) ? 1 : 2);
             };
@@ -369,8 +405,7 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
             for (int x = 0; x < X; x++) h.push_back(j < per[x].size() ? per[x][j] : WvTile{-1, 0, 0, 0});
         while (!h.empty() && h.back().inst < 0) h.pop_back();
     } else {
-        for (const Strip &st : strips)
-            for (int cb = 0; cb < st.ncb; cb++) h.push_back(WvTile{st.inst, cb, st.sb, flags_of(st.inst, cb, st.sb)});
+        h = tl;
     }
     Engine::Tiles T;
     T.ni = (int)h.size();

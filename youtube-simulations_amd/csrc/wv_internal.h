@@ -101,8 +101,11 @@ struct WvInst {
 #define WV_TILE_NEU 8       // ... a Neumann ghost point (never an INTERIOR tile)
 #define WV_TILE_DRV 16      // ... a driven cell (never an INTERIOR tile)
 #define WV_TILE_PRB 32      // ... a probe (never an INTERIOR tile)
+// One wave's work: column block cb of grid inst, rows [i0, i0 + rows) with
+// rows = flags >> WV_TILE_ROWS_SHIFT (the low bits hold WV_TILE_*)
+#define WV_TILE_ROWS_SHIFT 16
 struct WvTile {
-    int inst, cb, sb, flags;
+    int inst, cb, i0, flags;
 };
 
 // number of output columns of one wave for a fused depth k

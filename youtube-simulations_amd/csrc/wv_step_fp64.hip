@@ -67,8 +67,8 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
     const bool wrapi = p.bc == 1, wrapj = (p.bc == 1 || p.bc == 3);
     const int jb = t.cb * WOUT;
     const Cols<FAST> L(jb - H + (int)threadIdx.x * C, jb, WOUT, ny, wrapj);
-    const int ibeg = t.sb * d.rows_per_tile;
-    const int iend = min(ibeg + d.rows_per_tile, nx);
+    const int ibeg = t.i0;
+    const int iend = min(ibeg + (t.flags >> WV_TILE_ROWS_SHIFT), nx);
     const bool has_src = !INTERIOR && (t.flags & WV_TILE_SRC) && src.any();  // (source tiles are boundary tiles)
     // Neumann ghost points, driven cells and FLOOR (XT: built into the kernel
     // variant for grids that use them; boundary-path tiles only)

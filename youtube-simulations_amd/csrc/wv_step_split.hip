@@ -65,8 +65,8 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
     const bool wrapi = p.bc == 1, wrapj = (p.bc == 1 || p.bc == 3);
     const int jb = t.cb * WOUT;
     const Cols<FAST> L(jb - H + (int)threadIdx.x * C, jb, WOUT, ny, wrapj);
-    const int ibeg = t.sb * d.rows_per_tile;
-    const int iend = min(ibeg + d.rows_per_tile, nx);
+    const int ibeg = t.i0;
+    const int iend = min(ibeg + (t.flags >> WV_TILE_ROWS_SHIFT), nx);
     const bool has_src = !INTERIOR && (t.flags & WV_TILE_SRC) && src.any();  // (source tiles are boundary tiles)
     const uint64_t *act = d.act, *med = d.med;
     const double *tcp = d.tc, *tccf = d.tccf, *tgf = d.tgf;
