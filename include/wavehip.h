@@ -62,7 +62,7 @@ typedef struct wv_params {
     double gamma_topbot;   /* GAMMA_TOPBOT                                         */
     double e_scale;        /* E_SCALE of compute_energy (wave_common.c:469)        */
     int tb_depth;          /* steps fused per launch: 0 = tuned default, 1..8 fixed */
-    int rows_per_tile;     /* rows per tile of a fused launch: 0 = tuned default   */
+    int rows_per_tile;     /* rows per tile of a fused launch: 0 = default (boundary tiles may take fewer) */
     int device;            /* HIP device ordinal                                   */
     int flags;             /* WV_FLAG_*                                            */
 } wv_params;

@@ -137,19 +137,27 @@ int round_up(int a, int b) { return (a + b - 1) / b * b; }
 // strips, FP64 form 6 fused steps in 16-row strips
 // measured on MI355X at 4096^2 (5-point, uniform medium): split form 6 steps
 // per launch in 12-row strips, FP64 8 steps in 16-row strips
-int default_depth(const Engine *e) { return e->form == WV_FORM_F32SPLIT ? 6 : 8; }
-// Rows per strip without tuning: the measured best at 4096^2 (split 16,
-// FP64 24), shortened for smaller grids until the launch has enough tiles to
-// fill the chip several times over (256 CUs x 4 SIMDs x 4 waves)
+// Fused steps per launch without tuning: 8 (measured best at 4096^2 for
+// both forms); the split form takes 6 on grids under 8 M cells, where its
+// wider halo columns and fewer tiles per launch cost more than they save
+int default_depth(const Engine *e)
+{
+    long cells = 0;
+    for (const Grid &G : e->g) cells += (long)G.p.nx * G.p.ny;
+    return e->form == WV_FORM_F32SPLIT && cells < (8L << 20) ? 6 : 8;
+}
+// Rows per strip without tuning: the measured best at 4096^2 (split 32,
+// FP64 48), shortened for smaller grids until the launch has enough tiles to
+// fill the chip one and a half times over (256 CUs x 4 SIMDs x 4 waves)
 int default_rows(const Engine *e)
 {
-    const int pref = e->form == WV_FORM_F32SPLIT ? 16 : 24;
+    const int pref = e->form == WV_FORM_F32SPLIT ? 32 : 48;
     const int wout = wv_tile_cols(e->K > 0 ? e->K : default_depth(e));
-    for (int L : {24, 16, 12}) {
+    for (int L : {48, 32, 24, 16, 12}) {
         if (L > pref) continue;
         long tiles = 0;
         for (const Grid &G : e->g) tiles += (long)((G.p.ny + wout - 1) / wout) * ((G.p.nx + L - 1) / L);
-        if (tiles >= 8192) return L;
+        if (tiles >= 6144) return L;
     }
     return 8;
 }
@@ -301,7 +309,29 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
     // reach its outer columns get strips of at most 8 rows.  Those tiles run
     // the edge formulas and take several times as long per row; short, they
     // finish within the launch instead of setting its length.
-    const int Lb = std::min(L, 8);
+    //
+    // Short strips cost redundant rows (2K per strip), so they are used only
+    // where one boundary tile of L rows would run for more than half a slot's
+    // share of the launch: an estimate from the geometry alone, counting a
+    // boundary row as 5 interior rows (measured at 4096^2, K = 6 and 8) and
+    // 4,096 resident waves.
+    int Lb = std::min(L, 8);
+    {
+        double tot = 0;
+        for (int q = 0; q < e->n; q++) {
+            const Grid &G = e->g[q];
+            const int ncb = (G.p.ny + wout - 1) / wout, nsb = (G.p.nx + L - 1) / L;
+            int ob = 0;
+            for (int cb = 0; cb < ncb; cb++) {
+                const int js = cb * wout - wv_halo(k);
+                ob += !(js >= 1 && js + 64 * WV_C <= G.p.ny - 1);
+            }
+            const double nb = (double)ob * nsb + (double)(ncb - ob) * std::min(nsb, 2);
+            const double ni = (double)(ncb - ob) * std::max(nsb - 2, 0);
+            tot += (double)(L + 2 * k) * (ni + 5.0 * nb);
+        }
+        if (5.0 * (L + 2 * k) <= 0.5 * tot / 4096.0) Lb = L;
+    }
     // flags: INTERIOR when every row and column the tile reads lies inside the
     // outer box (no edge formula, no wrapping); ALLACT when, in addition,
     // every one of those cells is written by a step
