@@ -352,18 +352,23 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
 
 // Waves per SIMD the register allocation must allow: the rarely taken
 // boundary path (per-lane column mapping, edge formulas) would otherwise set
-// the register count of the whole kernel.
-template <int K>
+// the register count of the whole kernel.  The uniform-medium kernel at K = 8
+// fits three waves (168 registers, two spilled on the boundary path): +16 %
+// at 4096^2 over two, which its own allocation (170) would give.
+#ifndef WV_OCC_FP64_U8
+#define WV_OCC_FP64_U8 3
+#endif
+template <int K, int CM>
 struct Occ {
 #ifdef WV_OCC_FP64
     static constexpr int waves = WV_OCC_FP64;
 #else
-    static constexpr int waves = K <= 4 ? 3 : 2;
+    static constexpr int waves = K <= 4 ? 3 : (K == 8 && CM == WV_COEF_UNIFORM ? WV_OCC_FP64_U8 : 2);
 #endif
 };
 
 template <int K, int ST, int CM>
-__global__ void __launch_bounds__(64, Occ<K>::waves)
+__global__ void __launch_bounds__(64, (Occ<K, CM>::waves))
 k_step_fp64(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, const long *gstep,
             long g0, int ia, int ib, int oc, int od)
 {
