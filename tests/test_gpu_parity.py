@@ -230,6 +230,43 @@ def test_split_matches_oracle_split(wh, oracle, bc, stencil):
         assert e == e_ref
 
 
+@pytest.mark.parametrize("bc", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("variant", ["plain", "damped", "topbot"])
+def test_split_edge_column_tiles(wh, oracle, bc, variant):
+    """Uniform medium, 5-point, no point lists: the tiles whose rows lie
+    inside the box but whose columns reach column 0 or NY-1 take the
+    interior evaluation plus the FP64 edge formula on the outer columns
+    (K = 8).  Random state, obstacle cells, every B_COND, damping and
+    OSCILLATE_TOPBOT; bit-exact against the oracle's split form."""
+    nx, ny, n = 120, 150, 21
+    rng = np.random.default_rng(40 + bc)
+    xy = np.ones((nx, ny), np.int16)
+    xy[rng.integers(0, nx, 300), rng.integers(0, ny, 300)] = 0
+    xy[50:60, 0:5] = 0          # obstacle cells on an outer column
+    g = 2e-4 if variant == "damped" else 0.0
+    tcc, tc, tg = np.full((nx, ny), 0.01), np.full((nx, ny), 0.1), np.full((nx, ny), g)
+    kw = dict(kappa=1e-4, kappa_sides=5e-4, gamma_sides=1e-4, kappa_topbot=2e-4, gamma_topbot=1e-7) \
+        if variant == "damped" else {}
+    if variant == "topbot":
+        kw = dict(oscillate_topbot=1, tb_shift=nx // 2)
+    phi, psi = rng.standard_normal((nx, ny)), rng.standard_normal((nx, ny))
+    phi[xy == 0] = psi[xy == 0] = 0.0
+    p = oracle.make_params(nx, ny, bcond=bc, **kw)
+    hi, lo, v = oracle.split_from_fp64(phi, psi)
+    hi, lo, v = oracle.split_run(p, hi, lo, v, n, xy, tcc, tc, tg)
+    u_ref, um_ref = oracle.split_to_fp64(hi, lo, v)
+    for L in (8, 24):
+        say("edge columns", bc, variant, L)
+        W = wh.Wave(dict(nx=nx, ny=ny, bcond=bc, **kw), form=1, tb_depth=8, rows_per_tile=L)
+        W.set_tables(xy, tcc, tc, tg)
+        W.set_state(phi, psi)
+        W.step(n)
+        gs = W.get_state()
+        W.close()
+        assert bits_equal(gs[0], u_ref), first_diff(gs[0], u_ref)
+        assert bits_equal(gs[1], um_ref)
+
+
 def test_split_field_coefficients(wh, oracle):
     """per-cell tcc and tgamma (smooth medium): split and FP64 both exact"""
     from wavehip import workloads

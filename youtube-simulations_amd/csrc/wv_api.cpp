@@ -146,18 +146,16 @@ int default_depth(const Engine *e)
     for (const Grid &G : e->g) cells += (long)G.p.nx * G.p.ny;
     return e->form == WV_FORM_F32SPLIT && cells < (8L << 20) ? 6 : 8;
 }
-// Rows per strip without tuning: the measured best at 4096^2 (split 32,
-// FP64 48), shortened for smaller grids until the launch has enough tiles to
-// fill the chip one and a half times over (256 CUs x 4 SIMDs x 4 waves)
+// Rows per strip without tuning: the measured best at 4096^2 (64 for both
+// forms), shortened for smaller grids until the launch has enough tiles to
+// fill the chip one and a quarter times over (256 CUs x 4 SIMDs x 4 waves)
 int default_rows(const Engine *e)
 {
-    const int pref = e->form == WV_FORM_F32SPLIT ? 32 : 48;
     const int wout = wv_tile_cols(e->K > 0 ? e->K : default_depth(e));
-    for (int L : {48, 32, 24, 16, 12}) {
-        if (L > pref) continue;
+    for (int L : {64, 48, 32, 24, 16, 12}) {
         long tiles = 0;
         for (const Grid &G : e->g) tiles += (long)((G.p.ny + wout - 1) / wout) * ((G.p.nx + L - 1) / L);
-        if (tiles >= 6144) return L;
+        if (tiles >= 5120) return L;
     }
     return 8;
 }
@@ -315,7 +313,10 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
     // share of the launch: an estimate from the geometry alone, counting a
     // boundary row as 5 interior rows (measured at 4096^2, K = 6 and 8) and
     // 4,096 resident waves.
-    int Lb = std::min(L, 8);
+#ifndef WV_LB_MAX
+#define WV_LB_MAX 8  // rows of a short strip (4 and 16 measured slower at 4096^2)
+#endif
+    int Lb = std::min(L, WV_LB_MAX);
     {
         double tot = 0;
         for (int q = 0; q < e->n; q++) {
@@ -346,7 +347,14 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
         // (wrapped halos: any source, Neumann point or driven cell of the grid)
         const int bflags = (G.h_si.empty() ? 0 : WV_TILE_SRC) | (G.h_ni.empty() ? 0 : WV_TILE_NEU) |
                            (G.h_di.empty() ? 0 : WV_TILE_DRV) | (G.h_pi.empty() ? 0 : WV_TILE_PRB);
-        if (!interior) return bflags;
+        if (!interior) {
+            // EDGEJ: rows inside the box, columns reaching column 0 or NY-1, on a
+            // grid with no point lists and no FLOOR (the split kernel evaluates
+            // such a tile as interior and its outer columns by the edge formulas)
+            const bool edgej = WV_C == 1 && ibeg >= k && iend <= nx - k && G.tbl != 2 &&
+                               !(G.p.flags & WV_FLAG_COMPARISON) && bflags == 0 && !(G.vmax > 0);
+            return edgej ? (int)WV_TILE_EDGEJ : bflags;
+        }
         int xflags = 0;
         for (size_t q = 0; q < G.h_ni.size() && !(xflags & WV_TILE_NEU); q++)
             if (G.h_ni[q] >= ibeg - k && G.h_ni[q] < iend + k && G.h_nj[q] >= js && G.h_nj[q] < js + 64 * WV_C)

@@ -177,6 +177,35 @@ __device__ inline double edge_fp64(const Par &d, int rm, int jm, double x, doubl
     }
 }
 
+// edge_fp64() for a cell of the top or bottom row (jm = 0 or NY-1) whose row
+// lies inside the box (1 <= rm <= NX-2): the corner tests resolve, B_COND 1
+// and 3 take the same four-neighbour sum, and what is left is one uniform
+// switch on B_COND (same arithmetic, same order as edge_fp64)
+__device__ __forceinline__ double edge_tb_inside(const Par &d, int rm, int jm, double x, double y, double E,
+                                                 double Wv, double N, double S, double cc, double g,
+                                                 double tcv)
+{
+    const bool top = (jm == d.ny - 1);
+    const double V = top ? S : N;  // inward neighbour along j
+    const double O = top ? N : S;  // neighbour across the periodic seam
+    double delta;
+    if (d.osc_tb && rm < d.tb_shift) {
+        delta = (E + Wv) + -2.0 * x;
+        return leap(x, y, delta, cc, d.kappa, g);
+    }
+    switch (d.bc) {
+    case 0:
+        delta = ((E + Wv) + V) - 3.0 * x;
+        return leap(x, y, delta, cc, d.kappa, g);
+    case 1:
+    case 3:
+        delta = (((E + Wv) + V) + O) - 4.0 * x;
+        return leap(x, y, delta, cc, d.kappa, g);
+    default:
+        return absorb(x, y, V, tcv, d.kt, d.gt);
+    }
+}
+
 // COMPARISON, wave_3d.c:736-990: the flat grid in two halves, j < NY/2 and
 // j >= NY/2.  Cells of the top and bottom rows (periodic wraps stay inside
 // their half) and of the interface columns jtop = NY/2-1 and jbot = NY/2, with

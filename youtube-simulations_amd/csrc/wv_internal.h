@@ -101,6 +101,7 @@ struct WvInst {
 #define WV_TILE_NEU 8       // ... a Neumann ghost point (never an INTERIOR tile)
 #define WV_TILE_DRV 16      // ... a driven cell (never an INTERIOR tile)
 #define WV_TILE_PRB 32      // ... a probe (never an INTERIOR tile)
+#define WV_TILE_EDGEJ 64    // not INTERIOR only because its columns reach column 0 or NY-1
 // One wave's work: column block cb of grid inst, rows [i0, i0 + rows) with
 // rows = flags >> WV_TILE_ROWS_SHIFT (the low bits hold WV_TILE_*)
 #define WV_TILE_ROWS_SHIFT 16

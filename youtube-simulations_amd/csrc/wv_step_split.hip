@@ -45,14 +45,24 @@ constexpr int CF_ANY = 6;      // per-cell tables read in place (boundary tiles)
 // difference once: along i it is kept for the next iteration of the same
 // level, along j and the diagonals it comes from the adjacent lane.  Boundary
 // tiles evaluate the same subtractions directly; the results are the same.
-template <int K, int ST, bool INTERIOR, int CF, bool ALLACT, bool DAMP>
+//
+// EJ (edge columns): the tile's rows never reach the outer box but its
+// columns do.  Every cell takes the interior evaluation and the cells of the
+// outer columns (j = 0, NY-1, and the halo lanes mapped onto them) then take
+// the FP64 edge formula instead: a bulk cell's neighbours are its true
+// neighbours whatever lane holds them, and an edge cell's own value is
+// replaced, not blended.
+template <int K, int ST, bool INTERIOR, int CF, bool ALLACT, bool DAMP, bool EJ = false>
 __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long gbase, int ia, int oc)
 {
+    constexpr bool ROWI = INTERIOR || EJ;  // rows inside the outer box
     constexpr bool PREF = CF >= CF_FIELD && CF <= CF_FIELDLG;
     constexpr bool PG = CF == CF_FIELDG || CF == CF_FIELDLG, PL = CF == CF_FIELDL || CF == CF_FIELDLG;
     constexpr int COEF = CF == CF_UNIFORM ? WV_COEF_UNIFORM : (CF == CF_TWO ? WV_COEF_TWO : WV_COEF_FIELD);
-    constexpr bool SHARE = INTERIOR && WV_C == 1;
-    static_assert(INTERIOR || !PREF, "prefetched coefficients are for interior tiles");
+    constexpr bool SHARE = ROWI && WV_C == 1;
+    static_assert(!(INTERIOR && EJ), "edge-column tiles are not interior tiles");
+    static_assert(!EJ || WV_C == 1, "edge-column tiles: one cell per lane");
+    static_assert(ROWI || !PREF, "prefetched coefficients are for tiles with interior rows");
     constexpr int C = WV_C;
     constexpr int H = Tile<K>::H, WOUT = Tile<K>::WOUT;
     constexpr unsigned CMASK = (1u << C) - 1u;
@@ -67,7 +77,7 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
     const Cols<FAST> L(jb - H + (int)threadIdx.x * C, jb, WOUT, ny, wrapj);
     const int ibeg = t.i0;
     const int iend = min(ibeg + (t.flags >> WV_TILE_ROWS_SHIFT), nx);
-    const bool has_src = !INTERIOR && (t.flags & WV_TILE_SRC) && src.any();  // (source tiles are boundary tiles)
+    const bool has_src = !ROWI && (t.flags & WV_TILE_SRC) && src.any();  // (source tiles are boundary tiles)
     const uint64_t *act = d.act, *med = d.med;
     const double *tcp = d.tc, *tccf = d.tccf, *tgf = d.tgf;
     const float *tccf_hi = d.tccf_hi, *tccf_lo = d.tccf_lo, *tgf32 = d.tgf32;
@@ -92,7 +102,7 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
     const int send = iend + 2 * K;
 
     auto fetch = [&](int row, float(&qh)[C], float(&ql)[C], float(&qv)[C], unsigned &qb, unsigned &qm) {
-        const int rm = INTERIOR ? min(max(row, 0), nx - 1) : map_idx(row, nx, wrapi);
+        const int rm = ROWI ? min(max(row, 0), nx - 1) : map_idx(row, nx, wrapi);
         load_row(hin, ldy, rm, L, qh);
         load_row(lin, ldy, rm, L, ql);
         load_row(vin, ldy, rm, L, qv);
@@ -175,8 +185,8 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
             const bool live = (unsigned)(s - s0 - 3 * lv) < (unsigned)(iend - ibeg + 2 * (K - lv));
             if (live) {
                 const int r = s - 2 * lv;
-                const int rm = INTERIOR ? min(max(r, 0), nx - 1) : map_idx(r, nx, wrapi);
-                const bool rowint = INTERIOR || (rm >= 1 && rm <= nx - 2);
+                const int rm = ROWI ? min(max(r, 0), nx - 1) : map_idx(r, nx, wrapi);
+                const bool rowint = ROWI || (rm >= 1 && rm <= nx - 2);
                 const float(&he)[C] = Hw[lv - 1][PE];
                 const float(&hc)[C] = Hw[lv - 1][PC];
                 const float(&hw)[C] = Hw[lv - 1][PW];
@@ -203,6 +213,27 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
                     const float hN = (c < C - 1) ? hc[cn] : hcn, hS = (c > 0) ? hc[cp] : hcp;
                     const float lN = (c < C - 1) ? lc[cn] : lcn, lS = (c > 0) ? lc[cp] : lcp;
                     const unsigned mbit = (mb >> c) & 1u;
+                    // edge cells in FP64 with the reference formulas
+                    auto edge_cell = [&](float &oh, float &ol, float &ov) {
+                        const double x = (double)hc[c] + (double)lc[c];
+                        const double y = x - (double)vc[c];
+                        const double E = (double)he[c] + (double)le[c];
+                        const double Wv = (double)hw[c] + (double)lw[c];
+                        const double N = (double)hN + (double)lN;
+                        const double S = (double)hS + (double)lS;
+                        double cc, g;
+                        coef_fp64<COEF>(p, tccf, tgf, rm, L.jm[c], mbit, cc, g);
+                        const double tcv = PTV ? TV[lv - 1][c] : gc(tcp)[(size_t)rm * ldy + L.jm[c]];
+                        double out = EJ ? edge_tb_inside(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv)
+                                        : edge_fp64(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv);
+                        if (has_src) {
+                            const long g1 = gbase + lv - 1;
+                            if (g1 < src.len) out = add_sources(src, rm, L.jm[c], g1, out);
+                        }
+                        oh = (float)out;
+                        ol = (float)(out - (double)oh);
+                        ov = (float)(out - x);
+                    };
                     float oh, ol, ov;
                     if (SHARE) {
                         // h0 - hw is the east difference of row r-1 (previous iteration),
@@ -247,6 +278,7 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
                         }
                         dd_add_f(h0, l0, vn, oh, ol);
                         ov = vn;
+                        if (EJ && !L.colint[c]) edge_cell(oh, ol, ov);
                     } else if (rowint && L.colint[c]) {
                         const float h0 = hc[c], l0 = lc[c];
                         const float dh = ((he[c] - h0) - (h0 - hw[c])) + ((hN - h0) - (h0 - hS));
@@ -295,24 +327,7 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
                         dd_add_f(h0, l0, vn, oh, ol);
                         ov = vn;
                     } else {
-                        // edge cells in FP64 with the reference formulas
-                        const double x = (double)hc[c] + (double)lc[c];
-                        const double y = x - (double)vc[c];
-                        const double E = (double)he[c] + (double)le[c];
-                        const double Wv = (double)hw[c] + (double)lw[c];
-                        const double N = (double)hN + (double)lN;
-                        const double S = (double)hS + (double)lS;
-                        double cc, g;
-                        coef_fp64<COEF>(p, tccf, tgf, rm, L.jm[c], mbit, cc, g);
-                        const double tcv = PTV ? TV[lv - 1][c] : gc(tcp)[(size_t)rm * ldy + L.jm[c]];
-                        double out = edge_fp64(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv);
-                        if (has_src) {
-                            const long g1 = gbase + lv - 1;
-                            if (g1 < src.len) out = add_sources(src, rm, L.jm[c], g1, out);
-                        }
-                        oh = (float)out;
-                        ol = (float)(out - (double)oh);
-                        ov = (float)(out - x);
+                        edge_cell(oh, ol, ov);
                     }
                     const bool live = ALLACT || ((ab >> c) & 1u);  // cells the step never writes keep their state
                     nh[c] = live ? oh : hc[c];
@@ -389,17 +404,24 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
 // Waves per SIMD the register allocation must allow: the rarely taken
 // boundary path (per-lane column mapping, edge formulas) would otherwise set
 // the register count of the whole kernel.
-template <int K>
+// The edge-column variant is built into the uniform-medium 5-point kernel at
+// K = 8 only (elsewhere it would cost a resident wave per SIMD); there the
+// kernel fits three waves (168 registers, two spilled outside the loops).
+template <int K, int ST, int CM>
+struct EdgeJ {
+    static constexpr bool on = K == 8 && ST == 5 && CM == WV_COEF_UNIFORM;
+};
+template <int K, int ST, int CM>
 struct Occ {
 #ifdef WV_OCC_SPLIT
     static constexpr int waves = WV_OCC_SPLIT;
 #else
-    static constexpr int waves = K <= 4 ? 3 : 2;
+    static constexpr int waves = K <= 4 ? 3 : (EdgeJ<K, ST, CM>::on ? 3 : 2);
 #endif
 };
 
 template <int K, int ST, int CM>
-__global__ void __launch_bounds__(64, Occ<K>::waves)
+__global__ void __launch_bounds__(64, (Occ<K, ST, CM>::waves))
 k_step_split(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, const long *gstep,
             long g0, int ia, int oc)
 {
@@ -409,8 +431,12 @@ k_step_split(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, con
     const long gbase = (gstep ? *gstep : 0) + g0;
     const bool interior = t.flags & WV_TILE_INTERIOR;
     const bool allact = t.flags & WV_TILE_ALLACT;
+    const bool edgej = t.flags & WV_TILE_EDGEJ;
     const bool damp = d.damped;
 #define WV_RUN(CF, A, D) run_split<K, ST, true, CF, A, D>(d, t, gbase, ia, oc)
+#define WV_EDGEJ(CF)                                                        \
+    if (damp) run_split<K, ST, false, CF, false, true, true>(d, t, gbase, ia, oc); \
+    else run_split<K, ST, false, CF, false, false, true>(d, t, gbase, ia, oc);
 #define WV_INTERIOR(CF)                            \
     if (allact) {                                  \
         if (damp) WV_RUN(CF, true, true);          \
@@ -426,12 +452,13 @@ k_step_split(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, con
     const int coef = CM == WV_CM_ALL ? d.coef : CM;
     switch (coef) {
     case WV_COEF_UNIFORM:
-        if (!interior) run_split<K, ST, false, CF_UNIFORM, false, true>(d, t, gbase, ia, oc);
-        else WV_INTERIOR(CF_UNIFORM)
+        if (interior) { WV_INTERIOR(CF_UNIFORM) }
+        else if (EdgeJ<K, ST, CM>::on && edgej) { WV_EDGEJ(CF_UNIFORM) }
+        else run_split<K, ST, false, CF_UNIFORM, false, true>(d, t, gbase, ia, oc);
         break;
     case WV_COEF_TWO:
-        if (!interior) run_split<K, ST, false, CF_TWO, false, true>(d, t, gbase, ia, oc);
-        else WV_INTERIOR(CF_TWO)
+        if (interior) { WV_INTERIOR(CF_TWO) }
+        else run_split<K, ST, false, CF_TWO, false, true>(d, t, gbase, ia, oc);
         break;
     default:
         if (!interior) run_split<K, ST, false, CF_ANY, false, true>(d, t, gbase, ia, oc);
@@ -444,6 +471,7 @@ k_step_split(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, con
         }
         break;
     }
+#undef WV_EDGEJ
 #undef WV_DAMPED
 #undef WV_INTERIOR
 #undef WV_RUN
