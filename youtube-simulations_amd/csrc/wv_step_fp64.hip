@@ -164,7 +164,10 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
             // level lv is needed at rows [ibeg-K+lv, iend+K-lv) only (wave-uniform),
             // i.e. at iterations s in [ibeg-K+3lv, iend+K+lv): one unsigned compare
             const bool live = (unsigned)(s - s0 - 3 * lv) < (unsigned)(iend - ibeg + 2 * (K - lv));
-            if (live) {
+            // rows outside the box (plain boundary tiles of grids not periodic
+            // along i): no edge formula of rows 0 and NX-1 reads them
+            const bool inbox = INTERIOR || XT || TB || wrapi || (unsigned)(s - 2 * lv) < (unsigned)nx;
+            if (live && inbox) {
                 const int r = s - 2 * lv;
                 // rows outside the dependence cone of the strip's output are never
                 // used: INTERIOR tiles only clamp them so table reads stay in bounds

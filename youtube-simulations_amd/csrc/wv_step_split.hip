@@ -183,7 +183,10 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
             }
             // level lv is live at iterations s in [ibeg-K+3lv, iend+K+lv)
             const bool live = (unsigned)(s - s0 - 3 * lv) < (unsigned)(iend - ibeg + 2 * (K - lv));
-            if (live) {
+            // rows outside the box (boundary tiles of grids not periodic along i):
+            // no edge formula of rows 0 and NX-1 reads them, so they are skipped
+            const bool inbox = ROWI || wrapi || (unsigned)(s - 2 * lv) < (unsigned)nx;
+            if (live && inbox) {
                 const int r = s - 2 * lv;
                 const int rm = ROWI ? min(max(r, 0), nx - 1) : map_idx(r, nx, wrapi);
                 const bool rowint = ROWI || (rm >= 1 && rm <= nx - 2);
