@@ -232,12 +232,15 @@ def test_split_matches_oracle_split(wh, oracle, bc, stencil):
 
 @pytest.mark.parametrize("bc", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("variant", ["plain", "damped", "topbot"])
-def test_split_edge_column_tiles(wh, oracle, bc, variant):
+@pytest.mark.parametrize("form", [0, 1])
+def test_edge_column_tiles(wh, oracle, bc, variant, form):
     """Uniform medium, 5-point, no point lists: the tiles whose rows lie
     inside the box but whose columns reach column 0 or NY-1 take the
-    interior evaluation plus the FP64 edge formula on the outer columns
-    (K = 8).  Random state, obstacle cells, every B_COND, damping and
-    OSCILLATE_TOPBOT; bit-exact against the oracle's split form."""
+    interior evaluation plus the edge formula of the outer columns with the
+    row tests resolved (K = 8), and the first and last strips skip the rows
+    outside the box.  Random state, obstacle cells, every B_COND, damping
+    and OSCILLATE_TOPBOT; bit-exact against the oracle (FP64 form against
+    the reference's arithmetic, split form against its split restatement)."""
     nx, ny, n = 120, 150, 21
     rng = np.random.default_rng(40 + bc)
     xy = np.ones((nx, ny), np.int16)
@@ -252,12 +255,15 @@ def test_split_edge_column_tiles(wh, oracle, bc, variant):
     phi, psi = rng.standard_normal((nx, ny)), rng.standard_normal((nx, ny))
     phi[xy == 0] = psi[xy == 0] = 0.0
     p = oracle.make_params(nx, ny, bcond=bc, **kw)
-    hi, lo, v = oracle.split_from_fp64(phi, psi)
-    hi, lo, v = oracle.split_run(p, hi, lo, v, n, xy, tcc, tc, tg)
-    u_ref, um_ref = oracle.split_to_fp64(hi, lo, v)
+    if form == 0:
+        u_ref, um_ref = oracle.run(p, phi, psi, n, xy, tcc, tc, tg)[:2]
+    else:
+        hi, lo, v = oracle.split_from_fp64(phi, psi)
+        hi, lo, v = oracle.split_run(p, hi, lo, v, n, xy, tcc, tc, tg)
+        u_ref, um_ref = oracle.split_to_fp64(hi, lo, v)
     for L in (8, 24):
-        say("edge columns", bc, variant, L)
-        W = wh.Wave(dict(nx=nx, ny=ny, bcond=bc, **kw), form=1, tb_depth=8, rows_per_tile=L)
+        say("edge columns", form, bc, variant, L)
+        W = wh.Wave(dict(nx=nx, ny=ny, bcond=bc, **kw), form=form, tb_depth=8, rows_per_tile=L)
         W.set_tables(xy, tcc, tc, tg)
         W.set_state(phi, psi)
         W.step(n)

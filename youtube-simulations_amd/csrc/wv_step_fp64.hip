@@ -47,14 +47,21 @@ constexpr int CF_ANY = 4;      // per-cell tables read in place (boundary tiles)
 // XT: Neumann ghost points, driven cells and FLOOR on boundary tiles.  TB
 // (with XT, K = 1): the neighbour-table Laplacian of evolve_wave_half_new on
 // boundary tiles of grids that use it.
-template <int K, int ST, bool INTERIOR, int CF, bool ALLACT, bool DAMP, bool XT = false, bool TB = false>
+//
+// EJ (edge columns): rows inside the box, columns reaching column 0 or NY-1;
+// bulk cells as on interior rows, outer-column cells by the top/bottom-row
+// edge formula with the row tests resolved (edge_tb_inside).
+template <int K, int ST, bool INTERIOR, int CF, bool ALLACT, bool DAMP, bool XT = false, bool TB = false,
+          bool EJ = false>
 __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long gbase, int ia, int ib,
                                          int oc, int od)
 {
     constexpr bool PREF = CF == CF_FIELD || CF == CF_FIELDG;
     constexpr int COEF = CF == CF_UNIFORM ? WV_COEF_UNIFORM : (CF == CF_TWO ? WV_COEF_TWO : WV_COEF_FIELD);
+    constexpr bool ROWI = INTERIOR || EJ;  // rows inside the outer box
+    static_assert(!(INTERIOR && EJ) && !(EJ && (XT || TB)), "edge-column tiles: plain variant only");
     static_assert(INTERIOR || !PREF, "prefetched coefficients are for interior tiles");
-    static_assert(INTERIOR || DAMP, "boundary tiles evaluate the full formula");
+    static_assert(ROWI || DAMP, "boundary tiles evaluate the full formula");
     constexpr int C = WV_C;
     constexpr int H = Tile<K>::H, WOUT = Tile<K>::WOUT;
     constexpr unsigned CMASK = (1u << C) - 1u;
@@ -69,7 +76,7 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
     const Cols<FAST> L(jb - H + (int)threadIdx.x * C, jb, WOUT, ny, wrapj);
     const int ibeg = t.i0;
     const int iend = min(ibeg + (t.flags >> WV_TILE_ROWS_SHIFT), nx);
-    const bool has_src = !INTERIOR && (t.flags & WV_TILE_SRC) && src.any();  // (source tiles are boundary tiles)
+    const bool has_src = !ROWI && (t.flags & WV_TILE_SRC) && src.any();  // (source tiles are boundary tiles)
     // Neumann ghost points, driven cells and FLOOR (XT: built into the kernel
     // variant for grids that use them; boundary-path tiles only)
     const Xtra xq(d);
@@ -102,7 +109,7 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
     const int send = iend + 2 * K;   // iterations s0 .. send-1
 
     auto fetch = [&](int row, double(&qu)[C], double(&qp)[C], unsigned &qb, unsigned &qm) {
-        const int rm = INTERIOR ? min(max(row, 0), nx - 1) : map_idx(row, nx, wrapi);
+        const int rm = ROWI ? min(max(row, 0), nx - 1) : map_idx(row, nx, wrapi);
         load_row(u0, ldy, rm, L, qu);
         load_row(um1, ldy, rm, L, qp);
         // unconditional loads: a load under a runtime branch makes the compiler
@@ -166,13 +173,13 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
             const bool live = (unsigned)(s - s0 - 3 * lv) < (unsigned)(iend - ibeg + 2 * (K - lv));
             // rows outside the box (plain boundary tiles of grids not periodic
             // along i): no edge formula of rows 0 and NX-1 reads them
-            const bool inbox = INTERIOR || XT || TB || wrapi || (unsigned)(s - 2 * lv) < (unsigned)nx;
+            const bool inbox = ROWI || XT || TB || wrapi || (unsigned)(s - 2 * lv) < (unsigned)nx;
             if (live && inbox) {
                 const int r = s - 2 * lv;
                 // rows outside the dependence cone of the strip's output are never
                 // used: INTERIOR tiles only clamp them so table reads stay in bounds
-                const int rm = INTERIOR ? min(max(r, 0), nx - 1) : map_idx(r, nx, wrapi);
-                const bool rowint = INTERIOR || (rm >= 1 && rm <= nx - 2);
+                const int rm = ROWI ? min(max(r, 0), nx - 1) : map_idx(r, nx, wrapi);
+                const bool rowint = ROWI || (rm >= 1 && rm <= nx - 2);
                 const double(&Xe)[C] = U[lv - 1][PE];
                 const double(&Xc)[C] = U[lv - 1][PC];
                 const double(&Xw)[C] = U[lv - 1][PW];
@@ -238,7 +245,8 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
                         } else if (cmpm && (cmpcol || L.jm[c] == 0 || L.jm[c] == ny - 1)) {
                             v = cmp_fp64(p, u0, rm, L.jm[c], x, y, cc, g, tcv);
                         } else {
-                            v = edge_fp64(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv);
+                            v = EJ ? edge_tb_inside(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv)
+                                   : edge_fp64(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv);
                         }
                     }
                     bool w = ALLACT || ((ab >> c) & 1u);
@@ -361,6 +369,12 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
 #ifndef WV_OCC_FP64_U8
 #define WV_OCC_FP64_U8 3
 #endif
+// The edge-column variant is built into the uniform-medium 5-point kernel at
+// K = 8 only, like the split kernel's.
+template <int K, int ST, int CM>
+struct EdgeJ {
+    static constexpr bool on = K == 8 && ST == 5 && CM == WV_COEF_UNIFORM;
+};
 template <int K, int CM>
 struct Occ {
 #ifdef WV_OCC_FP64
@@ -381,6 +395,7 @@ k_step_fp64(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, cons
     const long gbase = (gstep ? *gstep : 0) + g0;
     const bool interior = t.flags & WV_TILE_INTERIOR;
     const bool allact = t.flags & WV_TILE_ALLACT;
+    const bool edgej = t.flags & WV_TILE_EDGEJ;
     const bool damp = d.fp64_full;
 #define WV_RUN(CF, A, D) run_fp64<K, ST, true, CF, A, D>(d, t, gbase, ia, ib, oc, od)
 #define WV_INTERIOR(CF)                            \
@@ -397,8 +412,11 @@ k_step_fp64(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, cons
     const int coef = CM >= WV_CM_ALL ? d.coef : CM;
     switch (coef) {
     case WV_COEF_UNIFORM:
-        if (!interior) run_fp64<K, ST, false, CF_UNIFORM, false, true, XT, TB>(d, t, gbase, ia, ib, oc, od);
-        else WV_INTERIOR(CF_UNIFORM)
+        if (interior) { WV_INTERIOR(CF_UNIFORM) }
+        else if (EdgeJ<K, ST, CM>::on && edgej) {
+            if (damp) run_fp64<K, ST, false, CF_UNIFORM, false, true, false, false, true>(d, t, gbase, ia, ib, oc, od);
+            else run_fp64<K, ST, false, CF_UNIFORM, false, false, false, false, true>(d, t, gbase, ia, ib, oc, od);
+        } else run_fp64<K, ST, false, CF_UNIFORM, false, true, XT, TB>(d, t, gbase, ia, ib, oc, od);
         break;
     case WV_COEF_TWO:
         if (!interior) run_fp64<K, ST, false, CF_TWO, false, true, XT, TB>(d, t, gbase, ia, ib, oc, od);
