@@ -112,6 +112,25 @@ int wv_set_wells_tables(wv_ctx *c, const short *xy_in, int nwells, const double 
                         const double *height, double sigma, int use_tanh, double courant, double courantb,
                         double gamma, const double box[4], double *tcc, double *tc, double *tgamma);
 
+/* Domain codes of xy_in_billiard_single_domain (global_pdes.c:18-50) that
+ * wv_build_mask evaluates */
+#define WV_D_SINAI        3
+#define WV_D_EXT_ELLIPSE  199
+#define WV_D_CIRCLES      20
+
+/* The billiard mask of init_wave_flat (wave_common.c:294-319), computed on
+ * the GPU for the context's (first) grid: xy_in[i*NY + j] =
+ * xy_in_billiard_single_domain(x, y) (sub_wave.c:5712) at the cell corner
+ * (x, y) of ij_to_xy (sub_wave.c:469-476) over box = (XMIN, XMAX, YMIN, YMAX):
+ * D_SINAI 1 outside the disc of radius lambda (5774-5779), D_EXT_ELLIPSE 1
+ * outside the ellipse of half-axes lambda, mu (5744-5749), D_CIRCLES 0 inside
+ * any active circle of the list (6216-6227; circles = xc[n], yc[n],
+ * radius[n], active[n] for n < ncircles, 4*ncircles doubles, active != 0).
+ * The mask is returned, not installed: pass it on to wv_set_tables or
+ * wv_set_wells_tables with the media tables. */
+int wv_build_mask(wv_ctx *c, int domain, const double box[4], double lambda, double mu, int ncircles,
+                  const double *circles, short *xy_in);
+
 /* u^n (phi) and u^{n-1} (psi); the third buffer of the rotation starts at 0. */
 int wv_set_state(wv_ctx *c, const double *phi, const double *psi);
 /* all three buffers of evolve_wave's rotation (phi, psi, tmp) */

@@ -9,7 +9,8 @@
 # and wave_3d.c:345-597; for the frame colours sub_wave.c:270-304, colors_waves.c
 # with the colour tables it includes, wave_common.c:453-512 and 1053-1171; for
 # the media tables sub_wave.c:270-304, 332-357, 469-476, 721-1387,
-# 12164-13004), then oracle/ref_harness.c.  Flags follow makefile:1-4
+# 12164-13004; for the masks sub_wave.c:270-304, 321-330, 332-357, 362-411,
+# 469-476, 721-1387, 5299-5486, 5682-7602), then oracle/ref_harness.c.  Flags follow makefile:1-4
 # (gcc -O3 -fopenmp) plus -ffp-contract=off.  Needs the reference sources at
 # /root/reference; machines without them use the oracle alone.
 set -euo pipefail
@@ -19,7 +20,7 @@ OUT="$HERE/_ref"
 [ -f "$REF/wave_billiard.c" ] || { echo "reference not found at $REF" >&2; exit 1; }
 mkdir -p "$OUT"
 
-build_one() {  # build_one <tag> <0 row pointers | 1 flat | 2 neighbour table | 3 colours | 4 media> <gcc -D flags...>
+build_one() {  # build_one <tag> <0 row pointers | 1 flat | 2 neighbour table | 3 colours | 4 media | 5 masks> <gcc -D flags...>
     local tag=$1 flat=$2; shift 2
     local so="$OUT/libref_${tag}.so"
     {
@@ -27,7 +28,10 @@ build_one() {  # build_one <tag> <0 row pointers | 1 flat | 2 neighbour table | 
         echo "#include \"$REF/global_pdes.c\""
         sed -n '412,422p' "$REF/sub_wave.c"
         sed -n '11965,12162p' "$REF/sub_wave.c"
-        if [ "$flat" = 4 ]; then
+        if [ "$flat" = 5 ]; then
+            echo "#define MASK_TWIN 1"
+            for r in 270,304 321,330 332,357 362,411 469,476 721,1387 5299,5486 5682,7602; do sed -n "${r}p" "$REF/sub_wave.c"; done
+        elif [ "$flat" = 4 ]; then
             echo "#define IOR_TWIN 1"
             for r in 270,304 332,357 469,476 721,1387 12164,13004; do sed -n "${r}p" "$REF/sub_wave.c"; done
         elif [ "$flat" = 3 ]; then
@@ -141,6 +145,15 @@ for p in "${pids[@]}"; do wait "$p"; done; pids=()
 build_one "ior_wells_p" 4 -DNX=40 -DNY=30 -DIOR=4 -DNGRIDX=6 -DNGRIDY=5 -DTWOSPEEDS=0 -DCOURANTB=0.05 &
 pids+=($!)
 build_one "ior_wells_r" 4 -DNX=40 -DNY=30 -DIOR=5 -DNGRIDX=6 -DNGRIDY=5 -DTWOSPEEDS=1 -DCOURANTB=0.05 &
+pids+=($!)
+for p in "${pids[@]}"; do wait "$p"; done; pids=()
+# the billiard masks of init_wave_flat (xy_in_billiard_single_domain): the
+# Sinai and exterior-ellipse domains of the plain workloads and D_CIRCLES
+build_one "mask_97x61_l03" 5 -DNX=97 -DNY=61 -DLAMBDA=0.3 -DMU=0.1 &
+pids+=($!)
+build_one "mask_97x61_l06_m03" 5 -DNX=97 -DNY=61 -DLAMBDA=0.6 -DMU=0.3 &
+pids+=($!)
+build_one "mask_1280x720_l03" 5 -DNX=1280 -DNY=720 -DLAMBDA=0.3 -DMU=0.1 &
 pids+=($!)
 for p in "${pids[@]}"; do wait "$p"; done; pids=()
 # a lens medium with damping and absorbing edges (energy fixture, two speeds)

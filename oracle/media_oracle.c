@@ -1,7 +1,7 @@
 /*
- * media_oracle.c -- CPU oracle of the Gaussian-well media tables (test
- * infrastructure, see wave_oracle.h).  Same flags as wave_oracle.c; exp,
- * tanh and sqrt are glibc's, as for the reference.
+ * media_oracle.c -- CPU oracle of the Gaussian-well media tables and of the
+ * billiard masks (test infrastructure, see wave_oracle.h).  Same flags as
+ * wave_oracle.c; exp, tanh and sqrt are glibc's, as for the reference.
  */
 #include <math.h>
 #include <stddef.h>
@@ -37,4 +37,41 @@ void orc_wells_tables(const orc_params *p, const short *xy, int nw, const double
             if (xy[k] != 0) tc[k] = sqrt(tcc[k]);
             else if (p->twospeeds) tc[k] = courantb;
         }
+}
+
+/* init_wave_flat's mask, wave_common.c:294-319: xy_in[i][j] =
+ * xy_in_billiard_single_domain(ij_to_xy(i, j)), sub_wave.c:5712 with
+ * ij_to_xy 469-476, for D_SINAI (5774-5779), D_EXT_ELLIPSE (5744-5749) and
+ * D_CIRCLES (6216-6227; circ = xc[n], yc[n], radius[n], active[n]).
+ * Returns -1 for another domain. */
+int orc_mask(int NX, int NY, const double box[4], int domain, double lambda, double mu, int ncirc,
+             const double *circ, short *out)
+{
+    if (domain != 3 && domain != 199 && domain != 20) return -1;
+    const double xmin = box[0], xmax = box[1], ymin = box[2], ymax = box[3];
+    int i, j;
+#pragma omp parallel for private(i, j)
+    for (i = 0; i < NX; i++)
+        for (j = 0; j < NY; j++) {
+            const double x = xmin + ((double)i) * (xmax - xmin) / ((double)NX);
+            const double y = ymin + ((double)j) * (ymax - ymin) / ((double)NY);
+            short v = 1;
+            if (domain == 3) {
+                v = x * x + y * y > lambda * lambda;
+            } else if (domain == 199) {
+                v = x * x / (lambda * lambda) + y * y / (mu * mu) > 1.0;
+            } else {
+                for (int k = 0; k < ncirc; k++)
+                    if (circ[3 * ncirc + k] != 0.0) {
+                        const double x1 = circ[k], y1 = circ[ncirc + k];
+                        const double r2 = circ[2 * ncirc + k] * circ[2 * ncirc + k];
+                        if ((x - x1) * (x - x1) + (y - y1) * (y - y1) < r2) {
+                            v = 0;
+                            break;
+                        }
+                    }
+            }
+            out[(size_t)i * NY + j] = v;
+        }
+    return 0;
 }

@@ -121,6 +121,9 @@ def lib():
                                 ctypes.c_int, _D, _D, ctypes.POINTER(ctypes.c_ubyte)]
         L.orc_color.argtypes = [ctypes.POINTER(OrcFrameParams), ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                 ctypes.c_double, ctypes.c_int, _F, ctypes.c_int, _D]
+        L.orc_mask.argtypes = [ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                               ctypes.c_int, _D, _S]
+        L.orc_mask.restype = ctypes.c_int
         L.orc_wells_tables.argtypes = [P, _S, ctypes.c_int, _D, _D, _D, ctypes.c_double, ctypes.c_int,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _D, _D, _D]
         L.orc_num_threads.restype = ctypes.c_int
@@ -309,6 +312,26 @@ def wells_tables(p, xy_in, xc, yc, h, sigma, use_tanh, courant, courantb, gamma,
     return tcc, tc, tg
 
 
+def circle_list(xc, yc, r, active=None):
+    """the (4, n) circle array of the mask builders: xc, yc, radius, active"""
+    xc = np.asarray(xc, np.float64)
+    a = np.ones(len(xc)) if active is None else np.asarray(active, np.float64)
+    return np.ascontiguousarray(np.vstack([xc, np.asarray(yc, np.float64), np.asarray(r, np.float64), a]))
+
+
+def mask(nx, ny, box, domain, lam, mu=0.0, circles=None):
+    """orc_mask: init_wave_flat's xy_in (int16 (nx, ny)) for D_SINAI (3),
+    D_EXT_ELLIPSE (199), D_CIRCLES (20, circles from circle_list)"""
+    c = np.zeros((4, 1)) if circles is None else np.ascontiguousarray(circles, np.float64)
+    n = 0 if circles is None else c.shape[1]
+    out = np.zeros((nx, ny), np.int16)
+    bx = np.ascontiguousarray(box, np.float64)
+    rc = lib().orc_mask(nx, ny, _p(bx, _D), domain, lam, mu, n, _p(c, _D), _p(out, _S))
+    if rc:
+        raise ValueError("domain %d not provided" % domain)
+    return out
+
+
 def _sources(sources):
     if not sources:
         z = np.zeros(1, np.int32)
@@ -381,6 +404,10 @@ class RefBuild:
         self.ny = self.L.ref_ny()
         self.flat = hasattr(self.L, "ref_evolve_wave_half_flat")
         self.table = hasattr(self.L, "ref_evolve_wave_half_new")
+        if hasattr(self.L, "ref_mask"):
+            self.L.ref_mask.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D, _I, _S]
+            self.L.ref_mask.restype = ctypes.c_int
+            return
         if hasattr(self.L, "ref_init_ior"):
             self.L.ref_init_ior.argtypes = [_S, _D, _D, _D, ctypes.c_uint]
             self.L.ref_wells.restype = ctypes.c_double
@@ -435,6 +462,18 @@ class RefBuild:
         t = [np.zeros((self.nx, self.ny)) for _ in range(3)]
         self.L.ref_init_ior(_p(xy, _S), _p(t[0], _D), _p(t[1], _D), _p(t[2], _D), seed)
         return tuple(t)
+
+    def mask(self, domain, circles=None):
+        """xy_in_billiard_single_domain over the build's grid (its LAMBDA, MU,
+        XMIN..YMAX), circles from circle_list for D_CIRCLES"""
+        c = np.zeros((4, 1)) if circles is None else np.ascontiguousarray(circles, np.float64)
+        n = 0 if circles is None else c.shape[1]
+        a = np.ascontiguousarray(c[3], np.int32)
+        out = np.zeros((self.nx, self.ny), np.int16)
+        cc = [np.ascontiguousarray(c[k]) for k in range(3)]
+        rc = self.L.ref_mask(domain, n, _p(cc[0], _D), _p(cc[1], _D), _p(cc[2], _D), _p(a, _I), _p(out, _S))
+        assert rc == 0
+        return out
 
     def wells(self, seed):
         """the well centres, heights and sigma init_ior_2d uses"""

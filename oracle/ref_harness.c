@@ -75,6 +75,28 @@ double ref_wells(unsigned seed, double *xc, double *yc, double *h)
         }
     return 0.2 * dx * dx; /* sigma */
 }
+#elif defined(MASK_TWIN)
+/* xy_in_billiard_single_domain (sub_wave.c:5712) at the cell corners
+ * ij_to_xy gives (sub_wave.c:469-476), flat out[i*NY + j], with the circle
+ * list of the caller (D_CIRCLES) */
+int ref_mask(int domain, int ncirc, const double *xc, const double *yc, const double *r, const int *active,
+             short *out)
+{
+    if (ncirc < 0 || ncirc > NMAXCIRCLES) return -1;
+    for (int k = 0; k < ncirc; k++) {
+        circles[k].xc = xc[k];
+        circles[k].yc = yc[k];
+        circles[k].radius = r[k];
+        circles[k].active = (short)active[k];
+    }
+    for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++) {
+            double xy[2];
+            ij_to_xy(i, j, xy);
+            out[(size_t)i * NY + j] = (short)xy_in_billiard_single_domain(xy[0], xy[1], domain, ncirc, circles);
+        }
+    return 0;
+}
 #elif defined(COLOR_TWIN)
 /* wave_value (wave_common.c:1053) at (i, j) on flat arrays; total_flux is
  * flat already (index 2*j*NX + 2*i) */

@@ -179,7 +179,19 @@
 #define HRES 1
 #define INITIAL_TIME 0
 #define WALL_WIDTH 0.1
+#ifndef MU
 #define MU 0.1
+#endif
+/* domain parameters xy_in_billiard_single_domain reads (sub_wave.c:5712),
+ * at the values of the checked-in wave_billiard.c:73-109 */
+#define B_DOMAIN 20
+#define B_DOMAIN_B 20
+#define MU_B 0.08
+#define JULIA_SCALE 0.8
+#define MRATIO 3
+#define WALL_WIDTH_B 0.1
+#define WALL_WIDTH_ASYM 0.75
+#define WALL_WIDTH_ASYM_B 0.75
 #define PDISC_FACTOR 3.0
 #define P_PERCOL 0.25
 #define NPOISSON 300

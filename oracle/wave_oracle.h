@@ -202,6 +202,11 @@ void orc_wells_tables(const orc_params *p, const short *xy, int nw, const double
                       const double *h, double sigma, int use_tanh, double courant, double courantb,
                       double gamma, const double box[4], double *tcc, double *tc, double *tg);
 
+/* the billiard mask of init_wave_flat (D_SINAI 3, D_EXT_ELLIPSE 199,
+ * D_CIRCLES 20), flat out[i*NY + j]; -1 for another domain */
+int orc_mask(int NX, int NY, const double box[4], int domain, double lambda, double mu, int ncirc,
+             const double *circ, short *out);
+
 /* Per-cell energy, compute_energy wave_common.c:453-472, with the Courant
  * constants given explicitly (c2_in = COURANT^2 for xy_in != 0, c2_out =
  * COURANTB^2 used when TWOSPEEDS). */

@@ -185,3 +185,23 @@ def cmp_cases():
     COMPARISON (two halves; TWOSPEEDS on for odd B_COND)"""
     return [("flat_%dx%d_bc%d_cmp" % (nx, ny, bc), nx, ny, bc, bc % 2) for nx, ny in ((24, 17), (16, 29))
             for bc in range(5)]
+
+
+# billiard masks (xy_in_billiard_single_domain): reference builds of a
+# 97x61 and a 1280x720 grid on the box of oracle/ref_params.h, with their
+# LAMBDA and MU; the circle list of D_CIRCLES is drawn here
+MASK_BOX = (-2.0, 2.0, -1.125, 1.125)
+
+
+def mask_cases():
+    """(build tag, lambda, mu)"""
+    return [("mask_97x61_l03", 0.3, 0.1), ("mask_97x61_l06_m03", 0.6, 0.3), ("mask_1280x720_l03", 0.3, 0.1)]
+
+
+def mask_circles(n=40):
+    """(4, n): centres in the box, radii 0.02..0.3, every fifth circle inactive"""
+    xc = uniform(case_seed("mask_xc"), n, -1.9, 1.9)
+    yc = uniform(case_seed("mask_yc"), n, -1.1, 1.1)
+    r = uniform(case_seed("mask_r"), n, 0.02, 0.3)
+    act = np.array([0.0 if k % 5 == 4 else 1.0 for k in range(n)])
+    return np.ascontiguousarray(np.vstack([xc, yc, r, act]))

@@ -23,6 +23,10 @@ Outputs (committed):
   media_cases.npz   init_ior_2d's Gaussian-well media (IOR_PERIODIC_WELLS,
                     IOR_RANDOM_WELLS): the wells it placed and its tcc, tc,
                     tgamma tables
+  mask_cases.npz    the billiard masks of xy_in_billiard_single_domain
+                    (D_SINAI, D_EXT_ELLIPSE, D_CIRCLES) for every
+                    gen_inputs.mask_cases() build (python make_golden.py masks
+                    writes this file alone)
   table_cases.npz   the neighbour-table stepper (evolve_wave_half_new): the
                     reference's table (counts, flat indices), the L-shape
                     lines, drive values where driven, and phi, psi, tmp after
@@ -43,7 +47,8 @@ sys.path.insert(0, HERE)
 
 import pyoracle  # noqa: E402
 from gen_inputs import (COLOR_FRAMES, EXT_STEPS, TABLE_STEPS, case_inputs, cmp_cases, color_cases,  # noqa: E402
-                        color_inputs, ext_cases, ext_inputs, small_cases, table_cases, table_inputs)
+                        color_inputs, ext_cases, ext_inputs, mask_cases, mask_circles, small_cases, table_cases,
+                        table_inputs)
 from wavehip import workloads  # noqa: E402
 
 NSTEPS_SMALL = 7
@@ -53,7 +58,18 @@ def sha(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
+def masks():
+    out = {}
+    circ = mask_circles()
+    for tag, lam, mu in mask_cases():
+        R = pyoracle.RefBuild(tag)
+        for dom in (3, 199, 20):
+            out["%s/%d" % (tag, dom)] = R.mask(dom, circ if dom == 20 else None)
+    np.savez_compressed(os.path.join(HERE, "mask_cases.npz"), **out)
+
+
 def main():
+    masks()
     out = {}
     for (tag, nx, ny, bc, ts, tb, tbs) in small_cases():
         R = pyoracle.RefBuild(tag)
@@ -159,4 +175,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["masks"]:
+        masks()
+    else:
+        main()

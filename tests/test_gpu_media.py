@@ -62,3 +62,42 @@ def test_wells_tables_match_oracle_and_step(wh, oracle):
     W.close()
     r = oracle.run(p, phi, phi, 40, xy, got[0], got[1], got[2])
     assert np.array_equal(u[0].view(np.uint64), r[0].view(np.uint64))
+
+
+def test_mask_matches_reference_fixture(wh):
+    """wv_build_mask against the masks of the reference's own
+    xy_in_billiard_single_domain (tests/golden/mask_cases.npz): exact"""
+    from gen_inputs import MASK_BOX, mask_cases, mask_circles
+    g = np.load(os.path.join(GOLD, "mask_cases.npz"))
+    sizes = {"mask_97x61_l03": (97, 61), "mask_97x61_l06_m03": (97, 61), "mask_1280x720_l03": (1280, 720)}
+    for tag, lam, mu in mask_cases():
+        nx, ny = sizes[tag]
+        W = wh.Wave(dict(nx=nx, ny=ny))
+        for dom in (wh.D_SINAI, wh.D_EXT_ELLIPSE, wh.D_CIRCLES):
+            say("mask fixture", tag, dom)
+            got = W.build_mask(dom, MASK_BOX, lam, mu, mask_circles() if dom == wh.D_CIRCLES else None)
+            assert np.array_equal(got, g["%s/%d" % (tag, dom)]), (tag, dom)
+        W.close()
+
+
+def test_mask_matches_oracle(wh, oracle):
+    """other sizes, boxes and circle lists (one longer than a wave), and a
+    mask that steps: the GPU mask fed to wv_set_tables equals the oracle's"""
+    rng = np.random.default_rng(11)
+    n = 150
+    circ = oracle.circle_list(rng.uniform(-1.5, 3.0, n), rng.uniform(-1.2, 1.2, n), rng.uniform(0.01, 0.2, n),
+                              (rng.uniform(0, 1, n) > 0.1).astype(float))
+    box = (-0.75, 3.25, -1.197916667, 1.197916667)
+    for nx, ny in ((300, 203), (64, 4100)):
+        W = wh.Wave(dict(nx=nx, ny=ny))
+        for dom, lam, mu in ((wh.D_SINAI, 0.7, 0.0), (wh.D_EXT_ELLIPSE, 1.1, 0.45), (wh.D_CIRCLES, 0.0, 0.0)):
+            say("mask", nx, ny, dom)
+            c = circ if dom == wh.D_CIRCLES else None
+            got = W.build_mask(dom, box, lam, mu, c)
+            assert np.array_equal(got, oracle.mask(nx, ny, box, dom, lam, mu, c))
+        W.close()
+    W = wh.Wave(dict(nx=40, ny=30))
+    for bad in ((1, BOX), (wh.D_SINAI, (1.0, 1.0, 0.0, 1.0))):
+        with pytest.raises(wh.WaveError):
+            W.build_mask(bad[0], bad[1], 0.5)
+    W.close()

@@ -2101,6 +2101,52 @@ extern "C" int wv_set_wells_tables(wv_ctx *c, const short *xy_in, int nwells, co
                             tcc, tc, tgamma);
 }
 
+// The billiard mask of init_wave_flat on the GPU (see wavehip.h)
+static int build_mask(Engine *e, int domain, const double *box, double lambda, double mu, int ncirc, const double *circ,
+                      short *xy)
+{
+    int kind;
+    switch (domain) {
+    case WV_D_SINAI: kind = 0; break;
+    case WV_D_EXT_ELLIPSE: kind = 1; break;
+    case WV_D_CIRCLES: kind = 2; break;
+    default: return fail(e, WV_EUNSUPPORTED, "domain %d: D_SINAI, D_EXT_ELLIPSE and D_CIRCLES are provided", domain);
+    }
+    if (!box || !xy) return fail(e, WV_EINVAL, "box or output is NULL");
+    if (!(box[1] > box[0]) || !(box[3] > box[2])) return fail(e, WV_EINVAL, "empty box");
+    if (kind == 2 && (ncirc < 0 || (ncirc > 0 && !circ))) return fail(e, WV_EINVAL, "circle list");
+    const Grid &G = e->g[0];
+    const int nx = G.p.nx, ny = G.p.ny;
+    const size_t cells = (size_t)nx * ny;
+    HIPCHK(e, hipSetDevice(e->device));
+    short *d_out = nullptr;
+    double *d_c = nullptr;
+    const int nc = kind == 2 ? ncirc : 0;
+    hipError_t err = hipMalloc(&d_out, cells * sizeof(short));
+    if (err == hipSuccess) err = hipMalloc(&d_c, sizeof(double) * 4 * (size_t)std::max(nc, 1));
+    if (err == hipSuccess && nc > 0) err = hipMemcpy(d_c, circ, sizeof(double) * 4 * (size_t)nc, hipMemcpyHostToDevice);
+    if (err == hipSuccess) {
+        WvMask a{};
+        a.nx = nx; a.ny = ny; a.kind = kind; a.ncirc = nc;
+        a.xmin = box[0]; a.xmax = box[1]; a.ymin = box[2]; a.ymax = box[3];
+        a.lambda = lambda; a.mu = mu; a.circ = d_c; a.out = d_out;
+        err = (hipError_t)wv_launch_mask(a, e->stream);
+    }
+    if (err == hipSuccess) err = hipMemcpyAsync(xy, d_out, cells * sizeof(short), hipMemcpyDeviceToHost, e->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+    (void)hipFree(d_out);
+    (void)hipFree(d_c);
+    if (err != hipSuccess) return fail(e, WV_EHIP, "mask: %s", hipGetErrorString(err));
+    return WV_OK;
+}
+
+extern "C" int wv_build_mask(wv_ctx *c, int domain, const double box[4], double lambda, double mu, int ncircles,
+                             const double *circles, short *xy_in)
+{
+    if (!c) return WV_EINVAL;
+    return build_mask(c->e, domain, box, lambda, mu, ncircles, circles, xy_in);
+}
+
 extern "C" int wv_set_probes(wv_ctx *c, int n, const int *i, const int *j, long every, long capacity)
 {
     if (!c) return WV_EINVAL;

@@ -165,3 +165,13 @@ struct WvWells {
     float *tcc_hi, *tcc_lo;
 };
 int wv_launch_wells(const WvWells &a, hipStream_t s);
+// Billiard masks (wv_media.hip): xy_in of every cell of an NX x NY grid over
+// (xmin, xmax, ymin, ymax), flat out[i*NY + j]; kind 0 D_SINAI, 1
+// D_EXT_ELLIPSE, 2 D_CIRCLES with circ = ncirc each of xc, yc, radius, active
+struct WvMask {
+    int nx, ny, kind, ncirc;
+    double xmin, xmax, ymin, ymax, lambda, mu;
+    const double *circ;
+    short *out;
+};
+int wv_launch_mask(const WvMask &a, hipStream_t s);

@@ -1,4 +1,4 @@
-// wv_media.hip -- media tables built on the GPU.
+// wv_media.hip -- media tables and billiard masks built on the GPU.
 //
 // init_ior_2d's Gaussian-well models (sub_wave.c:12315-12555:
 // IOR_PERIODIC_WELLS, IOR_RANDOM_WELLS, IOR_POISSON_WELLS, IOR_PPP_WELLS) sum
@@ -42,7 +42,45 @@ __global__ void __launch_bounds__(256) k_wells(WvWells a)
     }
 }
 
+// xy_in_billiard_single_domain (sub_wave.c:5712) at the cell corner of
+// ij_to_xy (sub_wave.c:469-476), for the domains of the plain workloads:
+// D_SINAI (5774-5779), D_EXT_ELLIPSE (5744-5749) and D_CIRCLES (6216-6227:
+// 0 inside any active circle, tested in list order).  The circle list is
+// read with wave-uniform loads.
+__global__ void __launch_bounds__(256) k_mask(WvMask a)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (j >= a.ny || i >= a.nx) return;
+    const double x = a.xmin + ((double)i) * (a.xmax - a.xmin) / ((double)a.nx);
+    const double y = a.ymin + ((double)j) * (a.ymax - a.ymin) / ((double)a.ny);
+    short v = 1;
+    if (a.kind == 0) {  // D_SINAI
+        v = x * x + y * y > a.lambda * a.lambda;
+    } else if (a.kind == 1) {  // D_EXT_ELLIPSE
+        v = x * x / (a.lambda * a.lambda) + y * y / (a.mu * a.mu) > 1.0;
+    } else {  // D_CIRCLES
+        const int n = a.ncirc;
+        for (int k = 0; k < n; k++) {
+            if (gc(a.circ)[3 * n + k] == 0.0) continue;  // inactive
+            const double x1 = gc(a.circ)[k], y1 = gc(a.circ)[n + k], r = gc(a.circ)[2 * n + k];
+            const double r2 = r * r;
+            if ((x - x1) * (x - x1) + (y - y1) * (y - y1) < r2) {
+                v = 0;
+                break;
+            }
+        }
+    }
+    gm(a.out)[(size_t)i * a.ny + j] = v;
+}
+
 }  // namespace
+
+int wv_launch_mask(const WvMask &a, hipStream_t s)
+{
+    const dim3 block(256), grid((a.ny + 255) / 256, a.nx);
+    hipLaunchKernelGGL(k_mask, grid, block, 0, s, a);
+    return (int)hipGetLastError();
+}
 
 int wv_launch_wells(const WvWells &a, hipStream_t s)
 {

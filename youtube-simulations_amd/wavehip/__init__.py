@@ -24,13 +24,14 @@ FORM_FP64, FORM_F32SPLIT = 0, 1
 FLAG_TABLES_FIXED = 1
 FLAG_FULL_FORMULA = 2
 FLAG_COMPARISON = 4
+D_SINAI, D_EXT_ELLIPSE, D_CIRCLES = 3, 199, 20
 
 # every symbol include/wavehip.h declares
 EXPORTS = [
     "wv_params_default", "wv_create", "wv_destroy", "wv_last_error", "wv_set_tables",
     "wv_set_state", "wv_set_state3", "wv_get_state", "wv_get_state3", "wv_set_sources",
     "wv_set_neumann", "wv_set_drive", "wv_set_floor", "wv_set_laplacian", "wv_set_probes", "wv_get_probes",
-    "wv_set_wells_tables",
+    "wv_set_wells_tables", "wv_build_mask",
     "wv_step", "wv_step_timed", "wv_sync", "wv_steps_done", "wv_last_kernel_ms", "wv_energy",
     "wv_get_tiling", "wv_frame_params_default", "wv_frame_set_tables", "wv_frame_reset", "wv_frame",
     "wv_frame_rgba8",
@@ -109,6 +110,8 @@ def load():
         "wv_set_probes": (ctypes.c_int, [vp, ctypes.c_int, _I, _I, ctypes.c_long, ctypes.c_long]),
         "wv_set_wells_tables": (ctypes.c_int, [vp, _S, ctypes.c_int, _D, _D, _D, ctypes.c_double, ctypes.c_int,
                                                ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _D, _D, _D]),
+        "wv_build_mask": (ctypes.c_int, [vp, ctypes.c_int, _D, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                         _D, _S]),
         "wv_get_probes": (ctypes.c_int, [vp, _D, ctypes.c_long, ctypes.POINTER(ctypes.c_long),
                                          ctypes.POINTER(ctypes.c_long)]),
         "wv_step": (ctypes.c_int, [vp, ctypes.c_long]),
@@ -320,6 +323,18 @@ class Wave:
                                               None if hh is None else _d(hh), sigma, int(use_tanh), courant,
                                               courantb, gamma, _d(bx), *outs), "wv_set_wells_tables")
         return tuple(t) if copy else None
+
+    def build_mask(self, domain, box, lam, mu=0.0, circles=None):
+        """init_wave_flat's xy_in on the GPU (D_SINAI 3, D_EXT_ELLIPSE 199,
+        D_CIRCLES 20 with circles = (4, n): xc, yc, radius, active); returns
+        int16 (nx, ny)"""
+        c = np.zeros((4, 1)) if circles is None else np.ascontiguousarray(circles, np.float64)
+        n = 0 if circles is None else c.shape[1]
+        bx = np.ascontiguousarray(box, np.float64)
+        out = np.empty(self.shape, np.int16)
+        self._chk(self._L.wv_build_mask(self._h, int(domain), _d(bx), lam, mu, n, _d(c), out.ctypes.data_as(_S)),
+                  "wv_build_mask")
+        return out
 
     def set_probes(self, i, j, every=3, capacity=4096):
         """record u at cells (i[k], j[k]) after every `every`-th step"""
