@@ -104,7 +104,8 @@ def frame_bench(wh, w, reps=10):
 def media_bench(wh, n=4096, nwells=100):
     """init_ior_2d's IOR_RANDOM_WELLS media for an n x n grid (10 x 10 wells,
     the checked-in NGRIDX, NGRIDY) built by wv_set_wells_tables, wall clock
-    including the table upload; beside it the oracle (OpenMP, all host
+    including the table upload, and the D_CIRCLES mask of the precision
+    workload built by wv_build_mask; beside each the oracle (OpenMP, all host
     threads) on a band of rows of the same grid, scaled to the grid."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -129,9 +130,26 @@ def media_bench(wh, n=4096, nwells=100):
     t0 = time.perf_counter()
     pyoracle.wells_tables(p, xy[:rows], xc, yc, h, sigma, 1, 0.1, 0.05, 0.0, band)
     cpu_ms = (time.perf_counter() - t0) * 1e3 * n / rows
+    # the D_CIRCLES mask of the precision workload (64 seeded discs), built by
+    # wv_build_mask including the copy of the int16 mask to the host
+    from wavehip import workloads
+    discs = np.array(workloads.random_discs(workloads.SplitMix64(3), 64, box, 0.02, 0.1))
+    circ = pyoracle.circle_list(discs[:, 0], discs[:, 1], discs[:, 2])
+    W = wh.Wave(dict(nx=n, ny=n, bcond=2))
+    W.build_mask(wh.D_CIRCLES, box, 0.0, 0.0, circ)  # first call: code load
+    t0 = time.perf_counter()
+    m = W.build_mask(wh.D_CIRCLES, box, 0.0, 0.0, circ)
+    mask_ms = (time.perf_counter() - t0) * 1e3
+    W.close()
+    t0 = time.perf_counter()
+    mo = pyoracle.mask(rows, n, band, wh.D_CIRCLES, 0.0, 0.0, circ)
+    mask_cpu_ms = (time.perf_counter() - t0) * 1e3 * n / rows
+    assert np.array_equal(m[:rows], mo)
     return {"workload": "%dx%d IOR_RANDOM_WELLS, %d wells, tanh" % (n, n, xc.size), "gpu_ms": gpu_ms,
             "cpu_oracle_ms_scaled": cpu_ms, "cpu_threads": pyoracle.num_threads(),
-            "cpu_sample": "%d of %d rows" % (rows, n)}
+            "cpu_sample": "%d of %d rows" % (rows, n),
+            "mask": {"workload": "%dx%d D_CIRCLES mask, 64 discs, int16 copied to the host" % (n, n),
+                     "gpu_ms": mask_ms, "cpu_oracle_ms_scaled": mask_cpu_ms}}
 
 
 def cpu_baseline(w, target_s=10.0, chunk=40):
