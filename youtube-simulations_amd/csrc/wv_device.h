@@ -262,13 +262,15 @@ __device__ inline double cmp_fp64(const Par &d, const double *u0, int rm, int jm
 }
 
 // Column geometry of one lane: FAST lanes lie in 1..ny-2 and read/write
-// contiguously; other lanes map every column (wrap or clamp) one by one.
+// contiguously; other lanes map every column (wrap or clamp) one by one, on
+// every lane, with no branch on the lane's position: a load under a
+// per-lane branch would make the compiler wait for every load in flight (the
+// prefetch ring) where the branch joins.
 template <bool FAST>
 struct Cols {
     int j0;
     int jm[WV_C];
     bool colint[WV_C], wcol[WV_C];
-    bool inside;
     __device__ __forceinline__ Cols(int j0_, int jb, int wout, int ny, bool wrapj) : j0(j0_)
     {
 #pragma unroll
@@ -278,19 +280,18 @@ struct Cols {
             colint[c] = FAST ? true : (jm[c] >= 1 && jm[c] <= ny - 2);
             wcol[c] = jj >= jb && jj < jb + wout && jj < ny;
         }
-        inside = FAST ? true : (j0 >= 0 && j0 + WV_C <= ny);
     }
 };
 
 // The 32-bit word of a bit plane that holds the lane's WV_C bits, returned
 // raw (bits at position j0 & 31) so that a prefetched word is not touched
 // until it is consumed; bits_of() extracts them.  j0 is a multiple of WV_C
-// (1 or 2), so the lane's bits never straddle a word.  Lanes with mapped columns assemble their bits
-// at the same position.
+// (1 or 2), so the lane's bits never straddle a word.  Lanes with mapped
+// columns assemble their bits at the same position.
 template <bool FAST>
 __device__ __forceinline__ unsigned lane_word(const uint64_t *plane, int ldw, int rm, const Cols<FAST> &L)
 {
-    if (FAST || L.inside) {
+    if (FAST) {
         gptr<const uint32_t> row = (gptr<const uint32_t>)(gc(plane) + (size_t)rm * ldw);
         return row[L.j0 >> 5];
     }
@@ -312,7 +313,7 @@ __device__ __forceinline__ void load_row(const double *base, int ldy, int rm, co
                                          double (&out)[WV_C])
 {
     gptr<const double> row = gc(base) + (size_t)rm * ldy;
-    if (FAST || L.inside) {
+    if (FAST) {
         if (WV_C == 2) {
             const dvec2 v = *(gptr<const dvec2>)(row + L.j0);
             out[0] = v.x;
@@ -332,7 +333,7 @@ __device__ __forceinline__ void load_row(const float *base, int ldy, int rm, con
                                          float (&out)[WV_C])
 {
     gptr<const float> row = gc(base) + (size_t)rm * ldy;
-    if (FAST || L.inside) {
+    if (FAST) {
         if (WV_C == 2) {
             const fvec2 v = *(gptr<const fvec2>)(row + L.j0);
             out[0] = v.x;
