@@ -138,13 +138,13 @@ int round_up(int a, int b) { return (a + b - 1) / b * b; }
 // measured on MI355X at 4096^2 (5-point, uniform medium): split form 6 steps
 // per launch in 12-row strips, FP64 8 steps in 16-row strips
 // Fused steps per launch without tuning: 8 (measured best at 4096^2 for
-// both forms); the split form takes 6 on grids under 8 M cells, where its
-// wider halo columns and fewer tiles per launch cost more than they save
+// both forms); 6 on grids under 8 M cells and 4 under 2 M (at 1280x720 a
+// launch lasts as long as its slowest boundary tile, which grows with K)
 int default_depth(const Engine *e)
 {
     long cells = 0;
     for (const Grid &G : e->g) cells += (long)G.p.nx * G.p.ny;
-    return e->form == WV_FORM_F32SPLIT && cells < (8L << 20) ? 6 : 8;
+    return cells < (2L << 20) ? 4 : (cells < (8L << 20) ? 6 : 8);
 }
 // Rows per strip without tuning: the measured best at 4096^2 (64 for both
 // forms), shortened for smaller grids until the launch has enough tiles to
