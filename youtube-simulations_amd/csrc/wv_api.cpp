@@ -316,12 +316,18 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
 #ifndef WV_LB_MAX
 #define WV_LB_MAX 8  // rows of a short strip (4 and 16 measured slower at 4096^2)
 #endif
+    //
+    // A launch with fewer tiles than resident waves lasts as long as its
+    // slowest tile: there the short strips have 4 rows (at 1280x720, K = 4:
+    // +30 % over 8), but at least K so that the next strip can be interior.
     int Lb = std::min(L, WV_LB_MAX);
     {
         double tot = 0;
+        long ntiles = 0;
         for (int q = 0; q < e->n; q++) {
             const Grid &G = e->g[q];
             const int ncb = (G.p.ny + wout - 1) / wout, nsb = (G.p.nx + L - 1) / L;
+            ntiles += (long)ncb * nsb;
             int ob = 0;
             for (int cb = 0; cb < ncb; cb++) {
                 const int js = cb * wout - wv_halo(k);
@@ -331,6 +337,7 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
             const double ni = (double)(ncb - ob) * std::max(nsb - 2, 0);
             tot += (double)(L + 2 * k) * (ni + 5.0 * nb);
         }
+        if (ntiles < 4096) Lb = std::min(L, std::max(k, std::min(4, WV_LB_MAX)));
         if (5.0 * (L + 2 * k) <= 0.5 * tot / 4096.0) Lb = L;
     }
     // flags: INTERIOR when every row and column the tile reads lies inside the
