@@ -21,9 +21,14 @@ k_energy_rows(const WvInst *__restrict__ I, const int2 *__restrict__ rows, doubl
     const int nx = d.nx, ny = d.ny;
     const int ip = i + 1 == nx ? nx - 1 : i + 1, im = i == 0 ? 0 : i - 1;
     double acc = 0.0;
+    // every load unconditional (clamped indices stay in the grid) so that the
+    // loads of several columns are in flight together; cells off the energy
+    // plane add +0, as before, in the same order
+#pragma unroll 4
     for (int j = lane; j < ny; j += 64) {
-        double e = 0.0;
-        if ((gc(d.eact)[(size_t)i * d.ldw + (j >> 6)] >> (j & 63)) & 1u) {
+        const bool on = (gc(d.eact)[(size_t)i * d.ldw + (j >> 6)] >> (j & 63)) & 1u;
+        double e;
+        {
             int jp = j + 1 == ny ? ny - 1 : j + 1, jn = j == 0 ? 0 : j - 1;
             // COMPARISON: no gradient across the halves (compute_energy_mod,
             // wave_common.c:1965-1971)
@@ -56,7 +61,7 @@ k_energy_rows(const WvInst *__restrict__ I, const int2 *__restrict__ rows, doubl
             const double gx2 = a1 * a1 + a2 * a2, gy2 = b1 * b1 + b2 * b2;
             e = d.es2 * (v * v + 0.5 * cc * (gx2 + gy2));
         }
-        acc += e;
+        acc += on ? e : 0.0;
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) acc = acc + __shfl_down(acc, off, 64);
