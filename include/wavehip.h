@@ -22,6 +22,8 @@
 #ifndef WAVEHIP_H
 #define WAVEHIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -32,6 +34,7 @@ extern "C" {
 #define WV_ESTATE       -3   /* call out of order (no tables / no state)     */
 #define WV_EUNSUPPORTED -4   /* reference feature not provided (see text)    */
 #define WV_ENOMEM       -5
+#define WV_EIO          -6   /* file could not be written                    */
 
 /* B_COND codes, global_pdes.c:270-274 */
 #define WV_BC_DIRICHLET   0  /* zero-flux outer edges (3 neighbours - 3x)  */
@@ -130,6 +133,27 @@ int wv_set_wells_tables(wv_ctx *c, const short *xy_in, int nwells, const double 
  * wv_set_wells_tables with the media tables. */
 int wv_build_mask(wv_ctx *c, int domain, const double box[4], double lambda, double mu, int ncircles,
                   const double *circles, short *xy_in);
+
+/* ---- TIFF frames: writetiff, sub_wave.c:62-123 ----------------------------
+ * The file writetiff(filename, description, 0, 0, width, height, compression)
+ * writes through libtiff (4.x): RGB, 8 bits a sample, one row per strip,
+ * Orientation BOTLEFT, the rows of glReadPixels (bottom row first, 3 bytes a
+ * pixel, GL_PACK_ALIGNMENT 1) stored as lines height-1 .. 0, in libtiff's
+ * layout, byte for byte.  rgb is that host buffer.  Under WV_TIFF_LZW (what
+ * every call site of the reference asks for) the strips are LZW-coded on the
+ * device, one workgroup a strip; WV_TIFF_NONE stores the rows as they are.
+ * wv_tiff_bound gives the output size to provide (0 for a frame or width out
+ * of range: 1..WV_TIFF_MAX_WIDTH x 1..65535); *len receives the file size.
+ * wv_tiff_write writes the file (WV_EIO when it cannot), where writetiff
+ * returns 1. */
+#define WV_TIFF_NONE      1
+#define WV_TIFF_LZW       5
+#define WV_TIFF_MAX_WIDTH 8192
+size_t wv_tiff_bound(int width, int height, const char *description);
+int wv_tiff_encode(wv_ctx *c, const unsigned char *rgb, int width, int height, const char *description,
+                   int compression, unsigned char *out, size_t cap, size_t *len);
+int wv_tiff_write(wv_ctx *c, const char *filename, const char *description, const unsigned char *rgb, int width,
+                  int height, int compression);
 
 /* u^n (phi) and u^{n-1} (psi); the third buffer of the rotation starts at 0. */
 int wv_set_state(wv_ctx *c, const double *phi, const double *psi);

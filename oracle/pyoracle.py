@@ -126,6 +126,13 @@ def lib():
         L.orc_mask.restype = ctypes.c_int
         L.orc_wells_tables.argtypes = [P, _S, ctypes.c_int, _D, _D, _D, ctypes.c_double, ctypes.c_int,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _D, _D, _D]
+        L.orc_lzw_strip.restype = ctypes.c_long
+        L.orc_lzw_strip.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p]
+        L.orc_tiff_bound.restype = ctypes.c_long
+        L.orc_tiff_bound.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_char_p]
+        L.orc_tiff.restype = ctypes.c_long
+        L.orc_tiff.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                               ctypes.c_void_p, ctypes.c_long]
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_split_energy_total.restype = ctypes.c_double
         L.orc_split_energy_total.argtypes = [P, _F, _F, _F, _S, _D, ctypes.c_double]
@@ -330,6 +337,28 @@ def mask(nx, ny, box, domain, lam, mu=0.0, circles=None):
     if rc:
         raise ValueError("domain %d not provided" % domain)
     return out
+
+
+def lzw_strip(data):
+    """orc_lzw_strip: one strip coded as libtiff's LZW coder codes it"""
+    src = np.frombuffer(bytes(data), np.uint8).copy()
+    out = np.zeros(len(src) + len(src) // 2 + 64, np.uint8)
+    n = lib().orc_lzw_strip(src.ctypes.data if len(src) else None, len(src), out.ctypes.data)
+    return out[:n].tobytes()
+
+
+def tiff(rgb, description=b"Wave equation in a planar domain", compression=5):
+    """orc_tiff: the file writetiff writes for the glReadPixels rows rgb
+    (uint8 (height, width, 3), bottom row first)"""
+    rgb = np.ascontiguousarray(rgb, np.uint8)
+    h, w = rgb.shape[0], rgb.shape[1]
+    d = description.encode() if isinstance(description, str) else bytes(description)
+    cap = lib().orc_tiff_bound(w, h, d)
+    out = np.zeros(cap, np.uint8)
+    n = lib().orc_tiff(rgb.ctypes.data, w, h, d, compression, out.ctypes.data, cap)
+    if n < 0:
+        raise ValueError("frame %dx%d, compression %d not provided" % (w, h, compression))
+    return out[:n].tobytes()
 
 
 def _sources(sources):

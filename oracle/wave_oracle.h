@@ -249,4 +249,16 @@ double orc_split_energy_total(const orc_params *p, const float *hi, const float 
 #ifdef __cplusplus
 }
 #endif
+/* ---- TIFF frames (the frame writer writetiff, sub_wave.c:62-123) --------
+ * orc_lzw_strip: one strip coded as libtiff's LZW coder does (out needs
+ * 3n/2 + 16 bytes); returns its length.  orc_tiff: the whole file that
+ * writetiff(name, description, 0, 0, width, height, compression) writes for
+ * the glReadPixels rows rgb (bottom row first, 3 bytes a pixel, no row
+ * padding); compression 1 (none) or 5 (LZW); out holds orc_tiff_bound()
+ * bytes; returns the file length or -1. */
+long orc_lzw_strip(const unsigned char *in, long n, unsigned char *out);
+long orc_tiff_bound(int width, int height, const char *description);
+long orc_tiff(const unsigned char *rgb, int width, int height, const char *description, int compression,
+              unsigned char *out, long cap);
+
 #endif

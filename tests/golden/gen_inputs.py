@@ -205,3 +205,39 @@ def mask_circles(n=40):
     r = uniform(case_seed("mask_r"), n, 0.02, 0.3)
     act = np.array([0.0 if k % 5 == 4 else 1.0 for k in range(n)])
     return np.ascontiguousarray(np.vstack([xc, yc, r, act]))
+
+
+TIFF_DESC = b"Wave equation in a planar domain"
+
+
+def tiff_cases():
+    """(tag, height, width, content, description, compression): widths around
+    the LZW code-width steps, the SHORT/LONG StripByteCounts switch (2184/2185
+    pixels), the 10000-byte ratio checkpoint and the table reset (4096 and
+    more), one to three rows, inline and out-of-line descriptions"""
+    out = []
+    for w in (1, 2, 85, 86, 171, 1365, 2184, 2185, 3334, 4096):
+        for content in ("rand", "low", "const", "smooth"):
+            if content == "rand" and w not in (1, 2, 85, 86, 171, 1365, 4096):
+                continue
+            rows = [(1, TIFF_DESC, 5), (2, b"abc", 5)]
+            if w <= 171 or content in ("const", "low"):
+                rows += [(3, b"Wave", 5), (3, b"x", 1)]
+            for h, desc, comp in rows:
+                out.append(("tiff_%s_%dx%d_c%d_d%d" % (content, h, w, comp, len(desc)), h, w, content, desc, comp))
+    return out
+
+
+def tiff_rgb(tag, h, w, content):
+    """uint8 (h, w, 3) rows of a frame: uniform bytes, bytes 0..2, a constant,
+    or a smooth colour ramp"""
+    if content == "rand":
+        return np.floor(uniform(case_seed(tag), (h, w, 3), 0.0, 256.0)).astype(np.uint8)
+    if content == "low":
+        return np.floor(uniform(case_seed(tag), (h, w, 3), 0.0, 3.0)).astype(np.uint8)
+    if content == "const":
+        return np.full((h, w, 3), 7, np.uint8)
+    x = np.arange(w)[None, :] * (18.0 / max(w, 1))
+    y = np.arange(h)[:, None] * 0.5
+    v = np.floor(127.0 + 120.0 * np.sin(x + y)).astype(np.uint8)
+    return np.ascontiguousarray(np.stack([v, 255 - v, v // 2], -1))

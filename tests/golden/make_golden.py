@@ -174,8 +174,21 @@ def main():
     print("fixtures written")
 
 
+def tiffs():
+    """tiff_cases.npz: the files libtiff writes for writetiff's calls"""
+    import libtiff_writer
+    from gen_inputs import tiff_cases, tiff_rgb
+    out = {"libtiff_version": np.array(libtiff_writer.version())}
+    for tag, h, w, content, desc, comp in tiff_cases():
+        out[tag] = np.frombuffer(libtiff_writer.encode(tiff_rgb(tag, h, w, content), desc, comp), np.uint8)
+    np.savez_compressed(os.path.join(HERE, "tiff_cases.npz"), **out)
+    print("tiff fixtures written:", len(out) - 1)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["masks"]:
         masks()
+    elif sys.argv[1:] == ["tiffs"]:
+        tiffs()
     else:
         main()

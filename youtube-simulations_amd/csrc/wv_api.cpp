@@ -111,6 +111,13 @@ struct Engine {
     std::vector<int> kev_depth;  // fused depth of each timed launch
     size_t kev_used = 0;
     double last_avg_ms = 0, last_timed = 0;
+    // TIFF frames: device rows, coded strips, their lengths and offsets, the
+    // packed strips (grown to the largest frame seen)
+    uint8_t *tf_rgb = nullptr, *tf_strips = nullptr, *tf_body = nullptr;
+    int *tf_lens = nullptr;
+    long *tf_off = nullptr;
+    size_t tf_cap_rgb = 0, tf_cap_strips = 0, tf_cap_body = 0;
+    int tf_cap_rows = 0;
 };
 
 int fail(Engine *e, int code, const char *fmt, ...)
@@ -531,7 +538,8 @@ void engine_destroy(Engine *e)
     drop_tiles(e);
     for (hipEvent_t ev : e->kev) (void)hipEventDestroy(ev);
     for (auto &G : e->g) free_grid(G);
-    void *ptrs[] = {e->d_inst, e->d_gstep, e->d_partial, e->d_eout, e->d_erows, e->d_row0};
+    void *ptrs[] = {e->d_inst, e->d_gstep, e->d_partial, e->d_eout, e->d_erows, e->d_row0,
+                    e->tf_rgb, e->tf_strips, e->tf_body, e->tf_lens, e->tf_off};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -2152,6 +2160,190 @@ extern "C" int wv_build_mask(wv_ctx *c, int domain, const double box[4], double 
 {
     if (!c) return WV_EINVAL;
     return build_mask(c->e, domain, box, lambda, mu, ncircles, circles, xy_in);
+}
+
+// ---- TIFF frames (writetiff, sub_wave.c:62-123) ------------------------------
+
+static uint8_t *le16(uint8_t *p, unsigned v)
+{
+    p[0] = (uint8_t)v;
+    p[1] = (uint8_t)(v >> 8);
+    return p + 2;
+}
+
+static uint8_t *le32(uint8_t *p, uint32_t v)
+{
+    for (int b = 0; b < 4; b++) p[b] = (uint8_t)(v >> (8 * b));
+    return p + 4;
+}
+
+static uint8_t *ifd_entry(uint8_t *p, unsigned tag, unsigned type, uint32_t count, uint32_t value)
+{
+    return le32(le32(le16(le16(p, tag), type), count), value);
+}
+
+extern "C" size_t wv_tiff_bound(int width, int height, const char *description)
+{
+    if (width < 1 || width > WV_TIFF_MAX_WIDTH || height < 1 || height > 65535 || !description) return 0;
+    const size_t strip = (size_t)wv_lzw_stride(3 * width);
+    return 8 + (size_t)height * strip + 1 + 2 + 12 * 12 + 4 + 6 + 8 * (size_t)height + strlen(description) + 1;
+}
+
+// grow a device buffer to at least want bytes
+static int grow(Engine *e, void **q, size_t *cap, size_t want)
+{
+    if (*cap >= want) return WV_OK;
+    if (*q) (void)hipFree(*q);
+    *q = nullptr;
+    *cap = 0;
+    HIPCHK(e, hipMalloc(q, want));
+    *cap = want;
+    return WV_OK;
+}
+
+static int tiff_encode(Engine *e, const unsigned char *rgb, int width, int height, const char *description,
+                       int compression, unsigned char *out, size_t cap, size_t *len)
+{
+    if (!rgb || !description || !out || !len) return fail(e, WV_EINVAL, "NULL argument");
+    if (width < 1 || width > WV_TIFF_MAX_WIDTH || height < 1 || height > 65535)
+        return fail(e, WV_EINVAL, "frame %dx%d: width 1..%d and height 1..65535 are provided", width, height,
+                    WV_TIFF_MAX_WIDTH);
+    if (compression != WV_TIFF_NONE && compression != WV_TIFF_LZW)
+        return fail(e, WV_EUNSUPPORTED, "compression %d (1 = none and 5 = LZW are provided)", compression);
+    const size_t need = wv_tiff_bound(width, height, description);
+    if (cap < need) return fail(e, WV_EINVAL, "output holds %zu bytes, %zu needed (wv_tiff_bound)", cap, need);
+    const int n = 3 * width;
+    std::vector<uint32_t> soff(height), scnt(height);  // by TIFF line
+    size_t body = 0;
+    if (compression == WV_TIFF_NONE) {
+        // rows are stored as they are: a copy, nothing to compute
+        for (int r = 0; r < height; r++) {
+            memcpy(out + 8 + body, rgb + (size_t)r * n, n);
+            soff[height - 1 - r] = (uint32_t)(8 + body);
+            scnt[height - 1 - r] = (uint32_t)n;
+            body += n;
+        }
+    } else {
+        HIPCHK(e, hipSetDevice(e->device));
+        const long stride = wv_lzw_stride(n);
+        int rc;
+        size_t cap_rows = (size_t)e->tf_cap_rows;
+        if ((rc = grow(e, (void **)&e->tf_rgb, &e->tf_cap_rgb, (size_t)height * n))) return rc;
+        if ((rc = grow(e, (void **)&e->tf_strips, &e->tf_cap_strips, (size_t)height * stride))) return rc;
+        if (cap_rows < (size_t)height) {
+            size_t c1 = 0, c2 = 0;
+            if (e->tf_lens) (void)hipFree(e->tf_lens);
+            if (e->tf_off) (void)hipFree(e->tf_off);
+            e->tf_lens = nullptr;
+            e->tf_off = nullptr;
+            e->tf_cap_rows = 0;
+            if ((rc = grow(e, (void **)&e->tf_lens, &c1, sizeof(int) * height))) return rc;
+            if ((rc = grow(e, (void **)&e->tf_off, &c2, sizeof(long) * height))) return rc;
+            e->tf_cap_rows = height;
+        }
+        HIPCHK(e, hipMemcpyAsync(e->tf_rgb, rgb, (size_t)height * n, hipMemcpyHostToDevice, e->stream));
+        int lr = wv_launch_lzw(e->tf_rgb, n, height, e->tf_strips, stride, e->tf_lens, e->stream);
+        if (lr) return fail(e, WV_EHIP, "kernel launch failed: %s", hipGetErrorString((hipError_t)lr));
+        std::vector<int> lens(height);
+        std::vector<long> off(height);
+        HIPCHK(e, hipMemcpyAsync(lens.data(), e->tf_lens, sizeof(int) * height, hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+        // strips go to the file in the order they are written: row 0 (the
+        // bottom row) first, as line height-1 (sub_wave.c:113-122)
+        for (int r = 0; r < height; r++) {
+            if (lens[r] < 3 || lens[r] > stride) return fail(e, WV_EHIP, "strip %d coded to %d bytes", r, lens[r]);
+            off[r] = (long)body;
+            soff[height - 1 - r] = (uint32_t)(8 + body);
+            scnt[height - 1 - r] = (uint32_t)lens[r];
+            body += (size_t)lens[r];
+        }
+        if ((rc = grow(e, (void **)&e->tf_body, &e->tf_cap_body, body))) return rc;
+        HIPCHK(e, hipMemcpyAsync(e->tf_off, off.data(), sizeof(long) * height, hipMemcpyHostToDevice, e->stream));
+        lr = wv_launch_tiff_pack(e->tf_strips, stride, e->tf_lens, e->tf_off, height, e->tf_body, e->stream);
+        if (lr) return fail(e, WV_EHIP, "kernel launch failed: %s", hipGetErrorString((hipError_t)lr));
+        HIPCHK(e, hipMemcpyAsync(out + 8, e->tf_body, body, hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+    }
+    // header, then the IFD at the next even offset and the values that do not
+    // fit in an entry, laid out as libtiff lays out this tag set
+    memcpy(out, "II*\0", 4);
+    size_t at = 8 + body;
+    if (at & 1) out[at++] = 0;
+    const uint32_t ifd = (uint32_t)at;
+    le32(out + 4, ifd);
+    const size_t dlen = strlen(description) + 1;
+    const bool one = height == 1;
+    // StripByteCounts are LONG for one strip or when the uncompressed strip
+    // could code to 65535 bytes or more (LZW allowed ten times its input)
+    const bool sbc_long = one || (compression == WV_TIFF_LZW ? n >= 65535 / 10 : n > 65535);
+    const size_t sbc_bytes = (sbc_long ? 4 : 2) * (size_t)height;
+    uint32_t data = ifd + 2 + 12 * 12 + 4;
+    const uint32_t bps_at = data;
+    data += 6;
+    const uint32_t sbc_at = data;
+    if (sbc_bytes > 4) data += (uint32_t)sbc_bytes;
+    const uint32_t so_at = data;
+    if (!one) data += 4 * (uint32_t)height;
+    const uint32_t desc_at = data;
+    uint32_t dval = desc_at;
+    if (dlen <= 4) {
+        uint8_t b[4] = {0, 0, 0, 0};
+        memcpy(b, description, dlen);
+        dval = b[0] | (uint32_t)b[1] << 8 | (uint32_t)b[2] << 16 | (uint32_t)b[3] << 24;
+    }
+    uint8_t *p = le16(out + at, 12);
+    p = ifd_entry(p, 256, 3, 1, (uint32_t)width);         // ImageWidth
+    p = ifd_entry(p, 257, 3, 1, (uint32_t)height);        // ImageLength
+    p = ifd_entry(p, 258, 3, 3, bps_at);                  // BitsPerSample 8,8,8
+    p = ifd_entry(p, 259, 3, 1, (uint32_t)compression);   // Compression
+    p = ifd_entry(p, 262, 3, 1, 2);                       // Photometric RGB
+    p = ifd_entry(p, 270, 2, (uint32_t)dlen, dval);       // ImageDescription
+    p = ifd_entry(p, 273, 4, (uint32_t)height, one ? soff[0] : so_at);  // StripOffsets
+    p = ifd_entry(p, 274, 3, 1, 4);                       // Orientation BOTLEFT
+    p = ifd_entry(p, 277, 3, 1, 3);                       // SamplesPerPixel
+    p = ifd_entry(p, 278, 3, 1, 1);                       // RowsPerStrip
+    if (sbc_bytes <= 4)                                   // StripByteCounts
+        p = ifd_entry(p, 279, sbc_long ? 4 : 3, (uint32_t)height, sbc_long ? scnt[0] : scnt[0] | scnt[1] << 16);
+    else
+        p = ifd_entry(p, 279, sbc_long ? 4 : 3, (uint32_t)height, sbc_at);
+    p = ifd_entry(p, 284, 3, 1, 1);                       // PlanarConfig CONTIG
+    p = le32(p, 0);
+    p = le16(le16(le16(p, 8), 8), 8);
+    if (sbc_bytes > 4)
+        for (int l = 0; l < height; l++) p = sbc_long ? le32(p, scnt[l]) : le16(p, scnt[l]);
+    if (!one)
+        for (int l = 0; l < height; l++) p = le32(p, soff[l]);
+    if (dlen > 4) {
+        memcpy(p, description, dlen);
+        p += dlen;
+    }
+    *len = (size_t)(p - out);
+    return WV_OK;
+}
+
+extern "C" int wv_tiff_encode(wv_ctx *c, const unsigned char *rgb, int width, int height, const char *description,
+                              int compression, unsigned char *out, size_t cap, size_t *len)
+{
+    if (!c) return WV_EINVAL;
+    return tiff_encode(c->e, rgb, width, height, description, compression, out, cap, len);
+}
+
+extern "C" int wv_tiff_write(wv_ctx *c, const char *filename, const char *description, const unsigned char *rgb,
+                             int width, int height, int compression)
+{
+    if (!c) return WV_EINVAL;
+    if (!filename) return fail(c->e, WV_EINVAL, "NULL file name");
+    const size_t cap = wv_tiff_bound(width, height, description);
+    if (!cap) return fail(c->e, WV_EINVAL, "frame %dx%d or description not provided", width, height);
+    std::vector<unsigned char> buf(cap);
+    size_t len = 0;
+    int rc = tiff_encode(c->e, rgb, width, height, description, compression, buf.data(), cap, &len);
+    if (rc) return rc;
+    FILE *f = fopen(filename, "wb");
+    if (!f) return fail(c->e, WV_EIO, "cannot open %s for writing", filename);
+    const bool ok = fwrite(buf.data(), 1, len, f) == len;
+    if (fclose(f) != 0 || !ok) return fail(c->e, WV_EIO, "short write to %s", filename);
+    return WV_OK;
 }
 
 extern "C" int wv_set_probes(wv_ctx *c, int n, const int *i, const int *j, long every, long capacity)

@@ -175,3 +175,11 @@ struct WvMask {
     short *out;
 };
 int wv_launch_mask(const WvMask &a, hipStream_t s);
+
+// TIFF frames: LZW strips of n bytes (3*width), one per row, each into
+// strips + r*stride (stride = wv_lzw_stride(n), a bound on one coded strip),
+// lens[r] its length; then the strips packed at body + off[r]
+long wv_lzw_stride(int n);
+int wv_launch_lzw(const uint8_t *rgb, int n, int rows, uint8_t *strips, long stride, int *lens, hipStream_t s);
+int wv_launch_tiff_pack(const uint8_t *strips, long stride, const int *lens, const long *off, int rows, uint8_t *body,
+                        hipStream_t s);

@@ -17,7 +17,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libwavehip.so")
 
-WV_OK, WV_EINVAL, WV_EHIP, WV_ESTATE, WV_EUNSUPPORTED, WV_ENOMEM = 0, -1, -2, -3, -4, -5
+WV_OK, WV_EINVAL, WV_EHIP, WV_ESTATE, WV_EUNSUPPORTED, WV_ENOMEM, WV_EIO = 0, -1, -2, -3, -4, -5, -6
 BC_DIRICHLET, BC_PERIODIC, BC_ABSORBING, BC_VPER_HABS, BC_ABS_REFLECT = 0, 1, 2, 3, 4
 BC_LSHAPE = 10
 FORM_FP64, FORM_F32SPLIT = 0, 1
@@ -25,13 +25,14 @@ FLAG_TABLES_FIXED = 1
 FLAG_FULL_FORMULA = 2
 FLAG_COMPARISON = 4
 D_SINAI, D_EXT_ELLIPSE, D_CIRCLES = 3, 199, 20
+TIFF_NONE, TIFF_LZW, TIFF_MAX_WIDTH = 1, 5, 8192
 
 # every symbol include/wavehip.h declares
 EXPORTS = [
     "wv_params_default", "wv_create", "wv_destroy", "wv_last_error", "wv_set_tables",
     "wv_set_state", "wv_set_state3", "wv_get_state", "wv_get_state3", "wv_set_sources",
     "wv_set_neumann", "wv_set_drive", "wv_set_floor", "wv_set_laplacian", "wv_set_probes", "wv_get_probes",
-    "wv_set_wells_tables", "wv_build_mask",
+    "wv_set_wells_tables", "wv_build_mask", "wv_tiff_bound", "wv_tiff_encode", "wv_tiff_write",
     "wv_step", "wv_step_timed", "wv_sync", "wv_steps_done", "wv_last_kernel_ms", "wv_energy",
     "wv_get_tiling", "wv_frame_params_default", "wv_frame_set_tables", "wv_frame_reset", "wv_frame",
     "wv_frame_rgba8",
@@ -112,6 +113,12 @@ def load():
                                                ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _D, _D, _D]),
         "wv_build_mask": (ctypes.c_int, [vp, ctypes.c_int, _D, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                          _D, _S]),
+        "wv_tiff_bound": (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
+        "wv_tiff_encode": (ctypes.c_int, [vp, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
+                                          ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
+                                          ctypes.POINTER(ctypes.c_size_t)]),
+        "wv_tiff_write": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int]),
         "wv_get_probes": (ctypes.c_int, [vp, _D, ctypes.c_long, ctypes.POINTER(ctypes.c_long),
                                          ctypes.POINTER(ctypes.c_long)]),
         "wv_step": (ctypes.c_int, [vp, ctypes.c_long]),
@@ -335,6 +342,33 @@ class Wave:
         self._chk(self._L.wv_build_mask(self._h, int(domain), _d(bx), lam, mu, n, _d(c), out.ctypes.data_as(_S)),
                   "wv_build_mask")
         return out
+
+    def tiff(self, rgb, description="Wave equation in a planar domain", compression=TIFF_LZW):
+        """the TIFF file (bytes) writetiff writes for the glReadPixels rows
+        rgb, uint8 (height, width, 3) with the bottom row first"""
+        rgb = np.ascontiguousarray(rgb, np.uint8)
+        if rgb.ndim != 3 or rgb.shape[2] != 3:
+            raise ValueError("rgb must be (height, width, 3)")
+        h, w = rgb.shape[0], rgb.shape[1]
+        d = description.encode() if isinstance(description, str) else bytes(description)
+        cap = self._L.wv_tiff_bound(w, h, d)
+        if cap == 0:
+            raise ValueError("frame %dx%d out of range (width 1..%d, height 1..65535)" % (w, h, TIFF_MAX_WIDTH))
+        out = np.empty(cap, np.uint8)
+        n = ctypes.c_size_t(0)
+        self._chk(self._L.wv_tiff_encode(self._h, rgb.ctypes.data, w, h, d, int(compression), out.ctypes.data, cap,
+                                         ctypes.byref(n)), "wv_tiff_encode")
+        return out[:n.value].tobytes()
+
+    def write_tiff(self, filename, rgb, description="Wave equation in a planar domain", compression=TIFF_LZW):
+        """writetiff(filename, description, 0, 0, width, height, compression)
+        for the glReadPixels rows rgb (bottom row first)"""
+        rgb = np.ascontiguousarray(rgb, np.uint8)
+        if rgb.ndim != 3 or rgb.shape[2] != 3:
+            raise ValueError("rgb must be (height, width, 3)")
+        d = description.encode() if isinstance(description, str) else bytes(description)
+        self._chk(self._L.wv_tiff_write(self._h, str(filename).encode(), d, rgb.ctypes.data, rgb.shape[1],
+                                        rgb.shape[0], int(compression)), "wv_tiff_write")
 
     def set_probes(self, i, j, every=3, capacity=4096):
         """record u at cells (i[k], j[k]) after every `every`-th step"""
