@@ -106,7 +106,9 @@ def media_bench(wh, n=4096, nwells=100):
     the checked-in NGRIDX, NGRIDY) built by wv_set_wells_tables, wall clock
     including the table upload, and the D_CIRCLES mask of the precision
     workload built by wv_build_mask; beside each the oracle (OpenMP, all host
-    threads) on a band of rows of the same grid, scaled to the grid."""
+    threads) on a band of rows of the same grid, scaled to the grid.  Also a
+    1920x1080 frame saved as writetiff saves it (LZW strips coded on the
+    device) beside the oracle's coder on one host thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     rng = np.random.default_rng(5)
@@ -140,7 +142,24 @@ def media_bench(wh, n=4096, nwells=100):
     t0 = time.perf_counter()
     m = W.build_mask(wh.D_CIRCLES, box, 0.0, 0.0, circ)
     mask_ms = (time.perf_counter() - t0) * 1e3
+    # a 1920x1080 frame (the glReadPixels rows) saved as writetiff saves it:
+    # LZW strips coded on the device; wall clock from host rows to file bytes
+    fy, fx = np.meshgrid(np.linspace(-1.125, 1.125, 1080), np.linspace(-2.0, 2.0, 1920), indexing="ij")
+    r = np.hypot(fx + 1.0, fy)
+    u = np.exp(-r * r) * np.cos(40.0 * r)
+    frame = np.stack([127.5 + 127.0 * u, 127.5 - 90.0 * u, 127.5 + 60.0 * np.sin(3.0 * u)], -1)
+    frame[np.hypot(fx - 0.5, fy - 0.3) < 0.3] = 0.0
+    frame = np.floor(frame).astype(np.uint8)
+    W.tiff(frame)  # first call: code load, buffers
+    t0 = time.perf_counter()
+    for _ in range(5):
+        tf = W.tiff(frame)
+    tiff_ms = (time.perf_counter() - t0) * 1e3 / 5
     W.close()
+    t0 = time.perf_counter()
+    tfo = pyoracle.tiff(frame)
+    tiff_cpu_ms = (time.perf_counter() - t0) * 1e3
+    assert tf == tfo
     t0 = time.perf_counter()
     mo = pyoracle.mask(rows, n, band, wh.D_CIRCLES, 0.0, 0.0, circ)
     mask_cpu_ms = (time.perf_counter() - t0) * 1e3 * n / rows
@@ -149,7 +168,10 @@ def media_bench(wh, n=4096, nwells=100):
             "cpu_oracle_ms_scaled": cpu_ms, "cpu_threads": pyoracle.num_threads(),
             "cpu_sample": "%d of %d rows" % (rows, n),
             "mask": {"workload": "%dx%d D_CIRCLES mask, 64 discs, int16 copied to the host" % (n, n),
-                     "gpu_ms": mask_ms, "cpu_oracle_ms_scaled": mask_cpu_ms}}
+                     "gpu_ms": mask_ms, "cpu_oracle_ms_scaled": mask_cpu_ms},
+            "tiff": {"workload": "1920x1080 RGB frame to an LZW TIFF file image (one row a strip), host rows in, "
+                                 "file bytes out", "gpu_ms": tiff_ms, "cpu_oracle_ms": tiff_cpu_ms,
+                     "cpu_threads": 1, "file_bytes": len(tf)}}
 
 
 def cpu_baseline(w, target_s=10.0, chunk=40):
