@@ -46,23 +46,19 @@ __global__ __launch_bounds__(64) void k_lzw(const uint8_t *__restrict__ rgb, int
     for (uint32_t k = lane; k < HSIZE; k += 64) tab[k] = EMPTY;
     __syncthreads();
 
+    // codes accumulate MSB-first in acc (nacc < 32 bits pending between
+    // calls); each full 32 bits go out as one word, byte-swapped so that the
+    // stream's first byte lands at the lowest address
     uint32_t *out = reinterpret_cast<uint32_t *>(strips + (size_t)r * (size_t)stride);
-    uint32_t data = 0, word = 0;
-    int bits = 0, nbits = 9, pos = 0, outbits = 0;
-    auto byte = [&](uint32_t b) {
-        word |= (b & 0xffu) << (8 * (pos & 3));
-        if ((pos & 3) == 3) {
-            if (w0) out[pos >> 2] = word;
-            word = 0;
-        }
-        pos++;
-    };
+    uint64_t acc = 0;
+    int nacc = 0, nbits = 9, nword = 0, outbits = 0;
     auto put = [&](int code) {
-        data = (data << nbits) | (uint32_t)code;  // at most 7 + 12 live bits
-        bits += nbits;
-        while (bits >= 8) {
-            byte(data >> (bits - 8));
-            bits -= 8;
+        acc = (acc << nbits) | (uint32_t)code;
+        nacc += nbits;
+        if (nacc >= 32) {
+            nacc -= 32;
+            if (w0) out[nword] = __builtin_bswap32((uint32_t)(acc >> nacc));
+            nword++;
         }
         outbits += nbits;
     };
@@ -133,9 +129,10 @@ __global__ __launch_bounds__(64) void k_lzw(const uint8_t *__restrict__ rgb, int
         nbits++;
     }
     put(257);
-    if (bits > 0) byte(data << (8 - bits));
-    if ((pos & 3) && w0) out[pos >> 2] = word;
-    if (w0) lens[r] = pos;
+    // the pending bits, zero-padded to a byte; the word may run up to three
+    // bytes past the strip, inside its stride
+    if (nacc > 0 && w0) out[nword] = __builtin_bswap32((uint32_t)(acc << (32 - nacc)));
+    if (w0) lens[r] = 4 * nword + (nacc + 7) / 8;
 }
 
 // strip r to body + off[r] (off[r] = bytes of the strips before it)
