@@ -30,9 +30,33 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "youtube-simulations_amd"))
 
 HBM_PEAK = 8.0e12              # MI355X HBM3E, bytes/s
+# wave64 VALU instructions the chip can issue per second: 256 CUs x 4 SIMDs,
+# one wave64 instruction per SIMD every 4 cycles (16 lanes), 2.4 GHz
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 B_ALG = {"f32split": 24.125,   # read u_hi, u_lo, v + write them (3 x 4 B each way) + 1-bit mask
          "fp64": 24.125}       # read u^n, u^{n-1} + write u^{n+1} (3 x 8 B) + 1-bit mask
 METRIC = "Gcell-updates/s at 4096^2 (FP64 plain vs FP32 split) + % of HBM roofline"
+KERNEL_SOURCES = {"f32split": ["wv_step_split.hip", "wv_device.h", "wv_internal.h"],
+                  "fp64": ["wv_step_fp64.hip", "wv_device.h", "wv_internal.h"]}
+
+
+def compulsory_bytes_per_cell(form, k):
+    """HBM bytes one fused launch of depth k must move per cell: the state read
+    once and written once, plus the 1-bit mask.  Split: hi, lo, v in and out
+    (24 B).  FP64: u^n and u^{n-1} in; u^{n+k} and u^{n+k-1} out when k > 1
+    (the next launch needs both), u^{n+1} alone when k == 1."""
+    if form == "f32split":
+        return 24.125
+    return 32.125 if k > 1 else 24.125
+
+
+def kernel_source_sha(form):
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES[form]:
+        with open(os.path.join(ROOT, "youtube-simulations_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
 
 
 def log(*a):
@@ -49,12 +73,16 @@ def run_form(wh, w, form, steps, warmup, tb_depth=0, rows=0, tune=False):
         W.step(warmup)
         W.sync()
         W.set_state(w["phi"], w["psi"])   # the timed run starts from the initial state
+    # set-up of the timed sequence outside the timed region: the tile tables of
+    # every fused depth it uses and the first load of every kernel it launches
+    # (no step is taken)
+    W.prepare(steps)
     W.sync()
     t0 = time.perf_counter()
     ms_dev, launches = W.step_timed(steps)   # HIP events on the stepping stream, waits
     W.sync()
     wall = time.perf_counter() - t0
-    avg_ms, n_timed = W.kernel_ms()          # event pairs around one launch in eight of the same run
+    avg_ms, n_timed = W.kernel_ms()          # every full-depth launch of the same run, event-timed
     e1 = W.energy()
     u, um = W.get_state()
     k, L, cols = W.tiling()
@@ -174,40 +202,143 @@ def media_bench(wh, n=4096, nwells=100):
                      "cpu_threads": 1, "file_bytes": len(tf)}}
 
 
-def cpu_baseline(w, target_s=10.0, chunk=40):
-    """the oracle on the host cores, a bounded sample of the FP64 workload:
-    chunks of steps until target_s seconds of CPU work have been timed"""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle
-    pw = w["params"]
-    p = pyoracle.make_params(pw["nx"], pw["ny"], bcond=pw["bcond"])
-    args = (w["xy_in"], w["tcc"], w["tc"], w["tgamma"])
-    phi, psi, tmp = pyoracle.run(p, w["phi"], w["psi"], 2, *args)   # touch the buffers
+def host_cpu():
+    """model name, logical CPUs, physical cores (lscpu) and the CPUs this
+    process may run on"""
+    import subprocess
+    info = {"model": None, "nproc": os.cpu_count(), "physical_cores": None,
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {}
+        for line in out.splitlines():
+            if ":" in line:
+                a, b = line.split(":", 1)
+                kv[a.strip()] = b.strip()
+        info["model"] = kv.get("Model name")
+        cps, sockets = kv.get("Core(s) per socket"), kv.get("Socket(s)")
+        if cps and sockets and cps.isdigit() and sockets.isdigit():
+            info["physical_cores"] = int(cps) * int(sockets)
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def _time_steps(step, target_s, chunk):
+    """run step(chunk) until target_s seconds have been timed: (steps, seconds)"""
     n, dt = 0, 0.0
     while dt < target_s:
         t0 = time.perf_counter()
-        phi, psi, tmp = pyoracle.run(p, phi, psi, chunk, *args, tmp=tmp)
+        step(chunk)
         dt += time.perf_counter() - t0
         n += chunk
+    return n, dt
+
+
+def cpu_baseline(w, target_s=10.0, chunk=40, target_1t=4.0, chunk_1t=4):
+    """the reference's OpenMP evolve_wave_half on the host cores, a bounded
+    sample of the FP64 workload: chunks of steps until target_s seconds of CPU
+    work have been timed; then the same on one thread.  The reference's own
+    code (oracle/_ref, compiled from its sources for this grid) when it was
+    built, else the oracle's restatement of it (bit-identical results).
+    Threads: the OpenMP default of this process (OMP_NUM_THREADS)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    pw = w["params"]
     cells = pw["nx"] * pw["ny"]
-    return dict(value=cells * n / dt / 1e9, unit="Gcell-updates/s", cores=pyoracle.num_threads(),
-                kind="port",
-                sample="%d steps of the %dx%d FP64 5-point workload (oracle restatement of "
-                       "evolve_wave_half, row pointers, OpenMP parallel for over i), %.1f s"
-                       % (n, pw["nx"], pw["ny"], dt))
+    args = (w["xy_in"], w["tcc"], w["tc"], w["tgamma"])
+    ref = pyoracle.ref_stepper(pw["nx"], pw["ny"], pw["bcond"])
+    if ref is not None:
+        kind = "reference"
+        what = ("the reference's evolve_wave_half (wave_billiard.c:302-610, row pointers, OpenMP parallel for over i) "
+                "compiled from its sources by oracle/build_ref.sh, gcc -O3 -fopenmp")
+        bufs = [np.array(w["phi"]), np.array(w["psi"]), np.zeros_like(w["phi"])]
+
+        def step(n):
+            ref.run_rows(bufs, n, *args)
+    else:
+        kind = "port"
+        what = ("the oracle's restatement of evolve_wave_half (row pointers, OpenMP parallel for over i, "
+                "bit-identical to the reference)")
+        p = pyoracle.make_params(pw["nx"], pw["ny"], bcond=pw["bcond"])
+        st = list(pyoracle.run(p, w["phi"], w["psi"], 2, *args))
+
+        def step(n):
+            st[:] = pyoracle.run(p, st[0], st[1], n, *args, tmp=st[2])
+    threads = pyoracle.num_threads()
+    step(2)   # touch the buffers
+    n, dt = _time_steps(step, target_s, chunk)
+    prev = pyoracle.set_num_threads(1)
+    try:
+        n1, dt1 = _time_steps(step, target_1t, chunk_1t)
+    finally:
+        pyoracle.set_num_threads(prev)
+    cpu = host_cpu()
+    return dict(value=cells * n / dt / 1e9, unit="Gcell-updates/s", cores=threads, kind=kind,
+                sample="%d steps of the %dx%d FP64 5-point workload of the timed run (%s), %.1f s on %d threads"
+                       % (n, pw["nx"], pw["ny"], what, dt, threads),
+                value_1thread=cells * n1 / dt1 / 1e9, sample_1thread="%d steps, %.1f s" % (n1, dt1),
+                cpu_model=cpu["model"], nproc=cpu["nproc"], physical_cores=cpu["physical_cores"],
+                cpus_allowed=cpu["affinity"],
+                threads_note="OMP_NUM_THREADS=%s (this process's CPU share of the machine)"
+                             % os.environ.get("OMP_NUM_THREADS", "unset"))
 
 
-def traffic_from_profile(form, k, L, size):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when it was measured on this
-    workload with the same tiling; otherwise None"""
-    f = os.path.join(ROOT, "profiles", "traffic_%s_c3_4096.json" % ("split" if form == "f32split" else "fp64"))
+def counters_from_profile(form, k, L, size):
+    """Counter figures of the stepping kernel from the committed rocprofv3
+    --pmc summary (HBM bytes per launch from FETCH_SIZE + WRITE_SIZE,
+    copy-calibrated; SQ_INSTS_VALU per launch), when it was measured on this
+    workload with the same tiling and the same kernel sources; otherwise None.
+    A profiler run of its own: these are not measured by this run."""
+    name = "counters_%s_c3_4096.json" % ("split" if form == "f32split" else "fp64")
+    f = os.path.join(ROOT, "profiles", name)
     if size != 4096 or not os.path.exists(f):
         return None
     d = json.load(open(f))
-    if d.get("tb_depth") == k and d.get("rows_per_tile") == L:
-        return d.get("bytes_per_launch")
-    return None
+    if d.get("tb_depth") != k or d.get("rows_per_tile") != L or d.get("kernel_src_sha16") != kernel_source_sha(form):
+        return None
+    d["source"] = "profiles/" + name
+    return d
+
+
+def roofline(r, form, cells):
+    """the roofline object of the stepping kernel of one form.  frac: the
+    bytes a fused launch must move (state read once, written once) over the
+    launch time, against HBM peak -- at most 1, and low when the launch is
+    bound by something else.  effective: the no-reuse bytes of the same steps
+    (B_alg x cells x K), which temporal blocking lets exceed the peak.
+    Counter figures (measured HBM traffic, VALU issue) come from the
+    committed rocprofv3 summary of the same kernel sources and tiling."""
+    t = r["avg_launch_ms"] * 1e-3
+    c = r["counters"]
+    ro = {
+        "bound": "hbm",
+        "achieved": r["achieved_Bps"] / 1e9,
+        "peak": HBM_PEAK / 1e9,
+        "unit": "GB/s",
+        "frac": r["achieved_Bps"] / HBM_PEAK,
+        "traffic": c["bytes_per_launch"] if c else None,
+        "traffic_source": c["source"] if c else None,
+        "traffic_frac": c["bytes_per_launch"] / t / HBM_PEAK if c and t > 0 else None,
+        "kernel": ("k_step_split<%d,5,0>" if form == "f32split" else "k_step_fp64<%d,5,0>") % r["k"],
+        "compulsory_bytes_per_launch": r["comp_bytes_per_launch"],
+        "compulsory_bytes_per_cell": compulsory_bytes_per_cell(form, r["k"]),
+        "avg_launch_ms": r["avg_launch_ms"],
+        "timed_launches": r["n_timed"],
+        "launches": r["launches"],
+        "effective": {
+            "note": "B_alg x cells x fused steps per launch over the launch time: counts temporal reuse, may exceed 1",
+            "b_alg_per_cell_update": B_ALG[form],
+            "bytes_per_launch": r["alg_bytes_per_launch"],
+            "achieved": r["effective_Bps"] / 1e9,
+            "frac": r["effective_Bps"] / HBM_PEAK,
+        },
+    }
+    if c and c.get("valu_insts_per_launch") and t > 0:
+        ro["valu"] = {"insts_per_launch": c["valu_insts_per_launch"],
+                      "issue_frac": c["valu_insts_per_launch"] / t / VALU_ISSUE_PEAK,
+                      "peak_insts_per_s": VALU_ISSUE_PEAK, "source": c["source"]}
+    return ro
 
 
 def rel_l2(a, b):
@@ -390,10 +521,14 @@ def main():
         # in eight of the timed run is bracketed by a HIP event pair on the
         # stepping stream
         r["avg_launch_ms"] = r["avg_ms"]
-        # algorithmic bytes per launch: B_alg x cells x fused steps per launch
+        # compulsory bytes per launch: the state read once and written once
+        r["comp_bytes_per_launch"] = compulsory_bytes_per_cell(name, r["k"]) * cells
+        r["achieved_Bps"] = r["comp_bytes_per_launch"] / (r["avg_launch_ms"] * 1e-3)
+        # B_alg x cells x fused steps: the bytes the same steps would move
+        # without temporal reuse (one sweep per step)
         r["alg_bytes_per_launch"] = B_ALG[name] * cells * r["k"]
-        r["achieved_Bps"] = r["alg_bytes_per_launch"] / (r["avg_launch_ms"] * 1e-3)
-        r["traffic"] = traffic_from_profile(name, r["k"], r["L"], args.size)
+        r["effective_Bps"] = r["alg_bytes_per_launch"] / (r["avg_launch_ms"] * 1e-3)
+        r["counters"] = counters_from_profile(name, r["k"], r["L"], args.size)
         log("%s: %.1f Gcell/s (wall), %.1f (events), k=%d rows=%d, %d launches, %.4f ms/launch"
             % (name, r["gcells"], r["gcells_dev"], r["k"], r["L"], r["launches"], r["avg_launch_ms"]))
         res[name] = r
@@ -424,32 +559,16 @@ def main():
             "rows_per_tile": head["L"],
             "tile_cols": head["cols"],
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": head["achieved_Bps"] / 1e9,
-            "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s",
-            "frac": head["achieved_Bps"] / HBM_PEAK,
-            "traffic": head["traffic"],
-            "traffic_frac": (head["traffic"] / (head["avg_launch_ms"] * 1e-3) / HBM_PEAK
-                             if head["traffic"] else None),
-            "kernel": ("k_step_split<%d,5,0>" if hname == "f32split" else "k_step_fp64<%d,5,0>") % head["k"],
-            "alg_bytes_per_launch": head["alg_bytes_per_launch"],
-            "avg_launch_ms": head["avg_launch_ms"],
-            "timed_launches": head["n_timed"],
-            "launches": head["launches"],
-            "b_alg_per_cell_update": B_ALG[hname],
-        },
+        "roofline": roofline(head, hname, cells),
     }
     if "fp64" in res:
         f = res["fp64"]
         out["fp64"] = {
             "gcells": round(f["gcells"], 3),
             "gcells_events": round(f["gcells_dev"], 3),
+            "ms_per_step": f["wall"] * 1e3 / args.steps,
             "tb_depth": f["k"], "rows_per_tile": f["L"],
-            "roofline_frac": f["achieved_Bps"] / HBM_PEAK,
-            "traffic_bytes_per_launch": f["traffic"],
-            "avg_launch_ms": f["avg_launch_ms"],
+            "roofline": roofline(f, "fp64", cells),
             "energy_drift": (f["e1"] - f["e0"]) / f["e0"],
         }
     if "fp64" in res and "f32split" in res:

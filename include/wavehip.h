@@ -223,17 +223,26 @@ int wv_get_probes(wv_ctx *c, double *out, long max_samples, long *count, long *l
  * nneighb = NULL returns to the plain stepper (not for BC_LSHAPE). */
 int wv_set_laplacian(wv_ctx *c, const int *nneighb, const int *nghb);
 
+/* Advance nsteps leapfrog steps: the 3-buffer rotation of evolve_wave
+ * (wave_billiard.c:613-631) applied nsteps times, in fused launches of the
+ * tiling's depth; asynchronous on the context stream. */
 int wv_step(wv_ctx *c, long nsteps);
-/* wv_step bracketed by HIP events on the context stream; waits; *ms is the
- * device time of the whole sequence and *launches the kernel launches in it */
+/* Do now the host-side work the first wv_step(c, nsteps) after a change of
+ * tables, sources, point lists or tiling would do before its first launch:
+ * build the tile tables of every fused depth the sequence uses and load
+ * every stepping kernel it launches.  The state is unchanged; waits.  (No
+ * counterpart in the reference, whose stepper has no set-up.) */
+int wv_prepare(wv_ctx *c, long nsteps);
+/* wv_step with a HIP event on the context stream before the first launch and
+ * after every launch; waits; *ms is the device time from the first event to
+ * the last and *launches the kernel launches in it */
 int wv_step_timed(wv_ctx *c, long nsteps, double *ms, long *launches);
 int wv_sync(wv_ctx *c);
 long wv_steps_done(const wv_ctx *c);
-/* Kernel time of the last wv_step_timed: one stepping launch in eight (the
- * first included) is bracketed by a HIP event pair on the context stream;
- * *avg_ms is the average duration of those at the full fused depth and
- * *timed_launches their number.  A launch covers every tile for one fused
- * block of steps. */
+/* Kernel time of the last wv_step_timed: launch q lasts from event q to event
+ * q+1 (the gap between two launches counts to the later one); *avg_ms is the
+ * average over every launch of the full fused depth and *timed_launches
+ * their number.  A launch covers every tile for one fused block of steps. */
 int wv_last_kernel_ms(const wv_ctx *c, double *avg_ms, double *timed_launches);
 
 /* Total energy: compute_energy (wave_common.c:453-472) per cell with c^2 from
@@ -334,6 +343,8 @@ int wv_batch_set_tables(wv_batch *b, int k, const short *xy_in, const double *tc
 int wv_batch_set_state(wv_batch *b, int k, const double *phi, const double *psi);
 int wv_batch_get_state(wv_batch *b, int k, double *phi, double *psi);
 int wv_batch_step(wv_batch *b, long nsteps);
+/* wv_prepare for a batch; also captures the graph of an nsteps sequence */
+int wv_batch_prepare(wv_batch *b, long nsteps);
 int wv_batch_step_timed(wv_batch *b, long nsteps, double *ms, long *launches);
 int wv_batch_last_kernel_ms(const wv_batch *b, double *avg_ms, double *timed_launches);
 int wv_batch_energy(wv_batch *b, double *e /* n values */);

@@ -156,6 +156,11 @@ pids+=($!)
 build_one "mask_1280x720_l03" 5 -DNX=1280 -DNY=720 -DLAMBDA=0.3 -DMU=0.1 &
 pids+=($!)
 for p in "${pids[@]}"; do wait "$p"; done; pids=()
+# the precision workload's grid (4096 x 4096, absorbing edges, courant 0.1, no
+# damping): the reference stepper timed as the CPU baseline of the benchmark
+build_one "row_4096x4096_bc2_ts0" 0 -DNX=4096 -DNY=4096 -DB_COND=2 -DTWOSPEEDS=0 &
+pids+=($!)
 # a lens medium with damping and absorbing edges (energy fixture, two speeds)
 build_one "row_64x48_bc2_ts1" 0 -DNX=64 -DNY=48 -DB_COND=2 -DTWOSPEEDS=1 -DCOURANTB=0.0666666666666666667 $KS
+for p in "${pids[@]}"; do wait "$p"; done; pids=()
 ls "$OUT" | wc -l

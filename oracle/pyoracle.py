@@ -144,6 +144,15 @@ def num_threads():
     return lib().orc_num_threads()
 
 
+def set_num_threads(n):
+    """OpenMP threads of the oracle and of the reference builds loaded in this
+    process (one OpenMP runtime); returns the previous count"""
+    L = lib()
+    L.orc_set_num_threads.restype = ctypes.c_int
+    L.orc_set_num_threads.argtypes = [ctypes.c_int]
+    return L.orc_set_num_threads(int(n))
+
+
 def evolve_wave_half_flat(p, phi_in, psi_in, phi_out, xy_in, tc, tcc, tgamma):
     """One step in place on phi_out (flat twin signature, wave_3d.c:600)."""
     lib().orc_evolve_wave_half_flat(ctypes.byref(p), _p(phi_in, _D), _p(psi_in, _D),
@@ -417,6 +426,15 @@ def split_energy_total(p, hi, lo, v, xy_in, tcc, e_scale=1.0):
 
 # ---------------------------------------------------------------- reference --
 
+def ref_stepper(nx, ny, bcond):
+    """the reference's row-pointer stepper built for an nx x ny grid with this
+    B_COND (no damping, one speed: the precision workload's build), or None"""
+    try:
+        return RefBuild("row_%dx%d_bc%d_ts0" % (nx, ny, bcond))
+    except FileNotFoundError:
+        return None
+
+
 def ref_available():
     return os.path.isdir(REF_DIR) and any(f.endswith(".so") for f in os.listdir(REF_DIR))
 
@@ -545,6 +563,14 @@ class RefBuild:
     def evolve_wave(self, phi, psi, tmp, xy_in, tcc, tc, tgamma):
         self.L.ref_evolve_wave(_p(phi, _D), _p(psi, _D), _p(tmp, _D), _p(xy_in, _S), _p(tcc, _D),
                                _p(tc, _D), _p(tgamma, _D))
+
+    def run_rows(self, bufs, nsteps, xy_in, tcc, tc, tgamma):
+        """nsteps evolve_wave_half calls with the evolve_wave rotation on the
+        three C-contiguous buffers of bufs (phi, psi, tmp), rotated in place"""
+        for _ in range(nsteps):
+            self.L.ref_evolve_wave_half(_p(bufs[0], _D), _p(bufs[1], _D), _p(bufs[2], _D), _p(xy_in, _S),
+                                        _p(tcc, _D), _p(tc, _D), _p(tgamma, _D))
+            bufs[:] = [bufs[2], bufs[0], bufs[1]]
 
     def run(self, phi, psi, tmp, nsteps, xy_in, tcc, tc, tgamma):
         """nsteps half-steps with the evolve_wave rotation; returns (phi, psi, tmp)."""

@@ -13,6 +13,13 @@
 #include <string.h>
 
 int orc_num_threads(void) { return omp_get_max_threads(); }
+/* OpenMP threads of later parallel regions; returns the previous count */
+int orc_set_num_threads(int n)
+{
+    const int prev = omp_get_max_threads();
+    if (n > 0) omp_set_num_threads(n);
+    return prev;
+}
 
 #define ACTIVE_BULK(p, v)  ((p)->twospeeds || (v) > 0)     /* wave_billiard.c:365 */
 #define ACTIVE_EDGE(p, v)  ((p)->twospeeds || (v) != 0)    /* wave_billiard.c:395,436,467,525 */
@@ -682,7 +689,10 @@ void orc_split_run(const orc_params *p, float *hi, float *lo, float *v, long nst
     double *U = (double *)malloc(sizeof(double) * n), *Y = (double *)malloc(sizeof(double) * n),
            *O = (double *)malloc(sizeof(double) * n);
     const float kf = (float)p->kappa;
-    const float w9 = (float)ORC_W9;
+    /* the coefficient pair holds tcc times the stencil weight (1/6 for the
+     * 9-point Laplacian, folded in binary64); the sum of differences is not
+     * scaled */
+    const double sw = p->stencil == 9 ? ORC_W9 : 1.0;
     ROWS(Ur, U, double); ROWS(Yr, Y, double); ROWS(Or, O, double);
     ROWS(m, xy_in, short); ROWS(t1, tcc, double); ROWS(t2, tc, double); ROWS(t3, tgamma, double);
 
@@ -709,12 +719,13 @@ void orc_split_run(const orc_params *p, float *hi, float *lo, float *v, long nst
                                 + ((hi[k + NY - 1] - hc) - (hc - hi[k - NY - 1]));
                     float dld = ((lo[k + NY + 1] - lc) - (lc - lo[k - NY + 1]))
                                 + ((lo[k + NY - 1] - lc) - (lc - lo[k - NY - 1]));
-                    lap = ((4.0f * dh + dhd) + (4.0f * dl + dld)) * w9;
+                    lap = (4.0f * dh + dhd) + (4.0f * dl + dld);
                 } else {
                     lap = dh + dl;
                 }
-                const float ch = (float)tcc[k];
-                const float cl = (float)(tcc[k] - (double)ch);
+                const double cw = tcc[k] * sw;
+                const float ch = (float)cw;
+                const float cl = (float)(cw - (double)ch);
                 const float gf = (float)tgamma[k];
                 /* a term whose coefficient is zero is not evaluated */
                 float vn = vc + fmaf(ch, lap, cl * lap);

@@ -57,6 +57,8 @@ typedef struct {
 
 /* OpenMP threads the parallel loops use */
 int orc_num_threads(void);
+/* OpenMP threads of later parallel regions; returns the previous count */
+int orc_set_num_threads(int n);
 
 /* One leapfrog step on row pointers (wave_billiard.c:302-610). */
 void orc_evolve_wave_half(const orc_params *p,
@@ -227,8 +229,12 @@ double orc_fixed_order_sum(int nx, int ny, const double *e);
 /* ---- FP32 double-float ("split") form, build-defined ----------------------
  * State per cell: u = hi + lo (two floats, |lo| <= ulp(hi)/2) and the
  * velocity v = u^n - u^{n-1} as one float.  Interior active cells:
- *   dh   = ((hE-hc) - (hc-hW)) + ((hN-hc) - (hc-hS))   (9-pt: see .c)
+ *   dh   = ((hE-hc) - (hc-hW)) + ((hN-hc) - (hc-hS))
  *   dl   = same on lo;  lap = dh + dl
+ *          (9-point: lap = (4*dh + dhd) + (4*dl + dld), dhd/dld the same
+ *          over the diagonal neighbours, unscaled)
+ *   (tcc_hi, tcc_lo) = binary32 pair of tcc*w, w = 1 (5-point) or ORC_W9
+ *          (9-point), w applied in binary64
  *   vn   = ((vc + fmaf(tcc_hi, lap, tcc_lo*lap)) - kappa*hc) - gamma*vc
  *          (a term whose coefficient is zero is not evaluated)
  *   t = lc + vn;  hi' = hc + t;  lo' = t - (hi' - hc)   (Fast2Sum)

@@ -437,3 +437,35 @@ def test_error_paths(wh):
     with pytest.raises(wh.WaveError):
         W.set_tiling(5, 64)
     W.close()
+
+
+def test_rejected_sources_leave_state(wh, oracle):
+    """A set_sources call that fails its checks (a source off the grid, or on
+    a cell the step never writes) changes nothing: the grid keeps stepping
+    with its old sources, bit-identical to the oracle."""
+    from wavehip import workloads
+    w = workloads.random_case(60, 50, 7, bcond=2, neg_cells=False)
+    w["phi"][w["xy_in"] == 0] = 0
+    w["psi"][w["xy_in"] == 0] = 0
+    i, j = np.argwhere(w["xy_in"][1:-1, 1:-1] > 0)[5] + 1
+    amp = np.sin(np.arange(1, 41) * 0.3)[None, :]
+    src = dict(i=[int(i)], j=[int(j)], amp=amp)
+    p = oracle.make_params(60, 50, bcond=2)
+    ref = oracle.run(p, w["phi"], w["psi"], 40, w["xy_in"], w["tcc"], w["tc"], w["tgamma"], sources=src)
+    zi, zj = np.argwhere(w["xy_in"][1:-1, 1:-1] == 0)[0] + 1
+    for k in (1, 4):
+        say("rejected sources, depth", k)
+        W = wh.Wave(w["params"], tb_depth=k)
+        W.load(w)
+        W.set_sources(src)
+        W.step(12)
+        with pytest.raises(wh.WaveError):
+            W.set_sources(dict(i=[70], j=[3], amp=np.ones((1, 5))))
+        with pytest.raises(wh.WaveError):
+            W.set_sources(dict(i=[int(zi)], j=[int(zj)], amp=np.ones((1, 5))))
+        with pytest.raises(ValueError):
+            W.set_sources(dict(i=[1, 2], j=[3], amp=np.ones((2, 5))))
+        W.step(28)
+        g = W.get_state()
+        W.close()
+        assert bits_equal(g[0], ref[0]) and bits_equal(g[1], ref[1]), k

@@ -104,13 +104,17 @@ struct Engine {
     bool dropin_tables = false;
     bool xcd_order = true;  // WAVEHIP_TILE_ORDER=linear turns the XCD interleave off
     bool slow_first = true;  // WAVEHIP_TILE_ORDER=inplace: slower tiles keep their place in the strip order
-    // per-kernel timing of the last wv_step_timed: an event pair around each
-    // stepping launch
+    // per-kernel timing of the last wv_step_timed: one event before the first
+    // stepping launch and one after every launch (kev[q], kev[q+1] bracket
+    // launch q; consecutive launches share an event)
     bool time_kernels = false;
     std::vector<hipEvent_t> kev;
-    std::vector<int> kev_depth;  // fused depth of each timed launch
-    size_t kev_used = 0;
+    std::vector<int> kev_depth;  // fused depth of launch q
+    size_t kev_used = 0;         // events recorded (the last one follows the step counter's bump)
+    bool kev_pending = false;    // per-launch durations of the last timed run not yet read
     double last_avg_ms = 0, last_timed = 0;
+    // one tile of instance -1: a launch over it loads the kernel and returns
+    WvTile *d_pad = nullptr;
     // TIFF frames: device rows, coded strips, their lengths and offsets, the
     // packed strips (grown to the largest frame seen)
     uint8_t *tf_rgb = nullptr, *tf_strips = nullptr, *tf_body = nullptr;
@@ -232,6 +236,10 @@ void free_grid(Grid &G)
         if (q) (void)hipFree(q);
 }
 
+// weight the split form folds into its coefficient pair: 1 for the 5-point
+// Laplacian, the 9-point weight 1/6 (as the FP64 stencil's W9) otherwise
+double split_weight(const Engine *e) { return e->stencil == 9 ? 1.0 / 6.0 : 1.0; }
+
 WvInst make_inst(const Engine *e, const Grid &G)
 {
     WvInst d{};
@@ -256,8 +264,12 @@ WvInst make_inst(const Engine *e, const Grid &G)
     d.bc = G.p.bcond; d.osc_tb = G.p.oscillate_topbot; d.tb_shift = G.p.tb_shift; d.coef = G.coef;
     d.rows_per_tile = rows(e);
     d.tcc0 = G.tcc0; d.tcc1 = G.tcc1; d.tg0 = G.tg0; d.tg1 = G.tg1;
-    d.tcch0 = (float)G.tcc0; d.tccl0 = (float)(G.tcc0 - (double)d.tcch0);
-    d.tcch1 = (float)G.tcc1; d.tccl1 = (float)(G.tcc1 - (double)d.tcch1);
+    // split form: the binary32 pair of tcc times the stencil's weight (tcc/6
+    // for the 9-point stencil, applied to an unscaled sum of differences)
+    const double sw = split_weight(e);
+    const double c0 = G.tcc0 * sw, c1 = G.tcc1 * sw;
+    d.tcch0 = (float)c0; d.tccl0 = (float)(c0 - (double)d.tcch0);
+    d.tcch1 = (float)c1; d.tccl1 = (float)(c1 - (double)d.tcch1);
     d.tgf0 = (float)G.tg0; d.tgf1 = (float)G.tg1;
     d.kappa = G.p.kappa; d.ks = G.p.kappa_sides; d.gs = G.p.gamma_sides;
     d.kt = G.p.kappa_topbot; d.gt = G.p.gamma_topbot;
@@ -527,6 +539,9 @@ int engine_create(Engine **out, int n, const wv_params *ps)
     HIPCHK(e, hipMemcpy(e->d_row0, row0.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
     HIPCHK(e, hipMalloc(&e->d_partial, sizeof(double) * rowsv.size()));
     HIPCHK(e, hipMalloc(&e->d_eout, sizeof(double) * n));
+    const WvTile pad{-1, 0, 0, 0};
+    HIPCHK(e, hipMalloc(&e->d_pad, sizeof(WvTile)));
+    HIPCHK(e, hipMemcpy(e->d_pad, &pad, sizeof(WvTile), hipMemcpyHostToDevice));
     return sync_inst(e);
 }
 
@@ -539,7 +554,7 @@ void engine_destroy(Engine *e)
     for (hipEvent_t ev : e->kev) (void)hipEventDestroy(ev);
     for (auto &G : e->g) free_grid(G);
     void *ptrs[] = {e->d_inst, e->d_gstep, e->d_partial, e->d_eout, e->d_erows, e->d_row0,
-                    e->tf_rgb, e->tf_strips, e->tf_body, e->tf_lens, e->tf_off};
+                    e->tf_rgb, e->tf_strips, e->tf_body, e->tf_lens, e->tf_off, e->d_pad};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -696,9 +711,10 @@ int set_tables(Engine *e, int k, const short *xy, const double *tcc, const doubl
         if (e->form == WV_FORM_F32SPLIT) {
             std::vector<float> h((size_t)nx * ldy, 0.f), l((size_t)nx * ldy, 0.f);
             bool anylo = false;
+            const double sw = split_weight(e);
             for (int i = 0; i < nx; i++)
                 for (int j = 0; j < ny; j++) {
-                    const double t = tcc[(size_t)i * ny + j];
+                    const double t = tcc[(size_t)i * ny + j] * sw;
                     const float th = (float)t;
                     const float tl = (float)(t - (double)th);
                     h[(size_t)i * ldy + j] = th;
@@ -830,6 +846,18 @@ int set_sources(Engine *e, int k, int n, const int *si, const int *sj, const dou
     if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
     Grid &G = e->g[k];
     if (!G.tables) return fail(e, WV_ESTATE, "set the tables before the sources");
+    // every argument is checked before anything changes: a rejected call
+    // leaves the sources (and the step counter) as they were
+    if (n < 0) return fail(e, WV_EINVAL, "negative source count");
+    if (n > 0) {
+        if (!si || !sj || !amp || amp_len <= 0) return fail(e, WV_EINVAL, "source arrays missing");
+        for (int q = 0; q < n; q++) {
+            if (si[q] < 0 || si[q] >= G.p.nx || sj[q] < 0 || sj[q] >= G.p.ny)
+                return fail(e, WV_EINVAL, "source %d at (%d,%d) is outside the grid", q, si[q], sj[q]);
+            if (!bit(G.h_wr, G.ldw, si[q], sj[q]))
+                return fail(e, WV_EINVAL, "source %d at (%d,%d) is on a cell the step never writes", q, si[q], sj[q]);
+        }
+    }
     HIPCHK(e, hipSetDevice(e->device));
     HIPCHK(e, hipStreamSynchronize(e->stream));
     for (int **q : {&G.src_ptr, &G.src_col, &G.src_id})
@@ -840,15 +868,7 @@ int set_sources(Engine *e, int k, int n, const int *si, const int *sj, const dou
     G.h_sj.clear();
     G.clean_amp = true;
     drop_tiles(e);  // tile flags mark the tiles that hold a source
-    if (n < 0) return fail(e, WV_EINVAL, "negative source count");
     if (n > 0) {
-        if (!si || !sj || !amp || amp_len <= 0) return fail(e, WV_EINVAL, "source arrays missing");
-        for (int q = 0; q < n; q++) {
-            if (si[q] < 0 || si[q] >= G.p.nx || sj[q] < 0 || sj[q] >= G.p.ny)
-                return fail(e, WV_EINVAL, "source %d at (%d,%d) is outside the grid", q, si[q], sj[q]);
-            if (!bit(G.h_wr, G.ldw, si[q], sj[q]))
-                return fail(e, WV_EINVAL, "source %d at (%d,%d) is on a cell the step never writes", q, si[q], sj[q]);
-        }
         std::vector<int> order(n);
         for (int q = 0; q < n; q++) order[q] = q;
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return si[a] < si[b]; });
@@ -856,14 +876,24 @@ int set_sources(Engine *e, int k, int n, const int *si, const int *sj, const dou
         for (int q = 0; q < n; q++) ptr[si[q] + 1]++;
         for (int i = 0; i < G.p.nx; i++) ptr[i + 1] += ptr[i];
         for (int q = 0; q < n; q++) { col[q] = sj[order[q]]; id[q] = order[q]; }
-        HIPCHK(e, hipMalloc(&G.src_ptr, sizeof(int) * ptr.size()));
-        HIPCHK(e, hipMalloc(&G.src_col, sizeof(int) * n));
-        HIPCHK(e, hipMalloc(&G.src_id, sizeof(int) * n));
-        HIPCHK(e, hipMalloc(&G.amp, sizeof(double) * (size_t)n * amp_len));
-        HIPCHK(e, hipMemcpy(G.src_ptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice));
-        HIPCHK(e, hipMemcpy(G.src_col, col.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-        HIPCHK(e, hipMemcpy(G.src_id, id.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-        HIPCHK(e, hipMemcpy(G.amp, amp, sizeof(double) * (size_t)n * amp_len, hipMemcpyHostToDevice));
+        hipError_t err = hipMalloc(&G.src_ptr, sizeof(int) * ptr.size());
+        if (err == hipSuccess) err = hipMalloc(&G.src_col, sizeof(int) * n);
+        if (err == hipSuccess) err = hipMalloc(&G.src_id, sizeof(int) * n);
+        if (err == hipSuccess) err = hipMalloc(&G.amp, sizeof(double) * (size_t)n * amp_len);
+        if (err == hipSuccess)
+            err = hipMemcpy(G.src_ptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice);
+        if (err == hipSuccess) err = hipMemcpy(G.src_col, col.data(), sizeof(int) * n, hipMemcpyHostToDevice);
+        if (err == hipSuccess) err = hipMemcpy(G.src_id, id.data(), sizeof(int) * n, hipMemcpyHostToDevice);
+        if (err == hipSuccess)
+            err = hipMemcpy(G.amp, amp, sizeof(double) * (size_t)n * amp_len, hipMemcpyHostToDevice);
+        if (err != hipSuccess) {
+            // the grid is left with no sources, and the device table says so
+            for (int **q : {&G.src_ptr, &G.src_col, &G.src_id})
+                if (*q) { (void)hipFree(*q); *q = nullptr; }
+            if (G.amp) { (void)hipFree(G.amp); G.amp = nullptr; }
+            (void)sync_inst(e);
+            return fail(e, WV_EHIP, "source upload failed: %s", hipGetErrorString(err));
+        }
         G.amp_len = amp_len;
         G.h_si.assign(si, si + n);
         G.h_sj.assign(sj, sj + n);
@@ -1237,12 +1267,9 @@ int prepare_tiles(Engine *e, long nsteps, int kmax)
     return WV_OK;
 }
 
-// enqueue nsteps steps; force_k1 keeps one step per launch (drop-in forms)
-int enqueue_steps(Engine *e, long nsteps, bool force_k1, long *launches)
+// coefficient modes present in the engine: the kernel variant built for just those
+int step_cm(const Engine *e)
 {
-    long done = 0, nl = 0;
-    const int kmax = max_depth(e, force_k1);
-    // coefficient modes present: the kernel built for just those
     int cm = e->g[0].coef;
     bool extras = false;
     for (const Grid &G : e->g) {
@@ -1251,44 +1278,65 @@ int enqueue_steps(Engine *e, long nsteps, bool force_k1, long *launches)
     }
     if (extras && e->form == WV_FORM_FP64) cm = WV_CM_ALLX;  // the variant with those paths
     if (e->form == WV_FORM_FP64 && any_table(e)) cm = WV_CM_TBL;
+    return cm;
+}
+
+// one stepping launch of depth k over ni + nb tiles starting at T
+int launch_depth(Engine *e, int k, int cm, const WvTile *T, int ni, int nb, long done)
+{
+    if (e->form == WV_FORM_FP64)
+        return wv_launch_fp64(k, e->stencil, cm, e->d_inst, T, ni, T + ni, nb, e->d_gstep, done, e->role[0],
+                              e->role[1], e->role[2], e->role[3], e->stream);
+    return wv_launch_split(k, e->stencil, cm, e->d_inst, T, ni, T + ni, nb, e->d_gstep, done, e->scur,
+                           1 - e->scur, e->stream);
+}
+
+// at least n timing events exist
+int reserve_kev(Engine *e, size_t n)
+{
+    while (e->kev.size() < n) {
+        hipEvent_t ev;
+        HIPCHK(e, hipEventCreate(&ev));
+        e->kev.push_back(ev);
+        e->kev_depth.push_back(0);
+    }
+    return WV_OK;
+}
+
+// record timing event number q on the stepping stream (timed runs)
+int record_kev(Engine *e, int depth)
+{
+    int rc;
+    if ((rc = reserve_kev(e, e->kev_used + 1))) return rc;
+    e->kev_depth[e->kev_used] = depth;
+    HIPCHK(e, hipEventRecord(e->kev[e->kev_used], e->stream));
+    e->kev_used++;
+    return WV_OK;
+}
+
+// enqueue nsteps steps; force_k1 keeps one step per launch (drop-in forms)
+int enqueue_steps(Engine *e, long nsteps, bool force_k1, long *launches)
+{
+    long done = 0, nl = 0;
+    const int kmax = max_depth(e, force_k1);
+    const int cm = step_cm(e);
+    int rc;
+    // timed runs: an event before the first launch and one after each launch
+    if (e->time_kernels && (rc = record_kev(e, 0))) return rc;
     while (done < nsteps) {
         const int k = next_depth(kmax, nsteps - done);
         Engine::Tiles t;
-        int rc;
         if ((rc = get_tiles(e, k, &t))) return rc;
-        auto launch = [&](int ni, int nb) {
-            if (e->form == WV_FORM_FP64)
-                return wv_launch_fp64(k, e->stencil, cm, e->d_inst, t.d, ni, t.d + t.ni, nb, e->d_gstep, done,
-                                      e->role[0], e->role[1], e->role[2], e->role[3], e->stream);
-            return wv_launch_split(k, e->stencil, cm, e->d_inst, t.d, ni, t.d + t.ni, nb, e->d_gstep, done,
-                                   e->scur, 1 - e->scur, e->stream);
-        };
-        int lr;
-        // one launch in eight is bracketed by an event pair: enough samples of
-        // identical launches, without the gaps a pair around every launch adds
-        if (e->time_kernels && nl % 8 == 0) {
-            while (e->kev.size() < e->kev_used + 2) {
-                hipEvent_t ev;
-                HIPCHK(e, hipEventCreate(&ev));
-                e->kev.push_back(ev);
-                e->kev_depth.push_back(0);
-            }
-            hipEvent_t *ev = &e->kev[e->kev_used];
-            e->kev_depth[e->kev_used] = k;
-            e->kev_used += 2;
-            HIPCHK(e, hipEventRecord(ev[0], e->stream));
-            lr = launch(t.ni, t.nb);
-            HIPCHK(e, hipEventRecord(ev[1], e->stream));
-        } else {
-            lr = launch(t.ni, t.nb);
-        }
+        const int lr = launch_depth(e, k, cm, t.d, t.ni, t.nb, done);
         advance_roles(e, k);
         if (lr) return fail(e, WV_EHIP, "kernel launch failed: %s", hipGetErrorString((hipError_t)lr));
+        if (e->time_kernels && (rc = record_kev(e, k))) return rc;
         done += k;
         nl++;
     }
     int lr = wv_launch_bump(e->d_gstep, nsteps, e->stream);
     if (lr) return fail(e, WV_EHIP, "kernel launch failed: %s", hipGetErrorString((hipError_t)lr));
+    if (e->time_kernels && (rc = record_kev(e, -1))) return rc;  // end of the sequence
     e->steps_done += nsteps;
     if (launches) *launches = nl;
     return WV_OK;
@@ -1299,27 +1347,20 @@ long rot_key(const Engine *e)
     return e->form == WV_FORM_FP64 ? e->role[0] + 4 * e->role[1] + 16 * e->role[2] + 64 * e->role[3] : e->scur;
 }
 
-int engine_step(Engine *e, long nsteps, bool force_k1)
+// the executable graph of an nsteps sequence from the current rotation
+// state (batches), captured on first use
+int step_graph(Engine *e, long nsteps, hipGraphExec_t *out)
 {
-    if (nsteps < 0) return fail(e, WV_EINVAL, "negative step count");
-    int rc = ready(e);
-    if (rc) return rc;
-    HIPCHK(e, hipSetDevice(e->device));
-    if (nsteps == 0) return WV_OK;
-    if (!e->use_graph || force_k1) {
-        rc = enqueue_steps(e, nsteps, force_k1, &e->last_launches);
-        return rc;
-    }
-    // batch: capture the launch sequence once per (nsteps, rotation) and replay
     const int kmax = max_depth(e, false);
+    int rc;
     if ((rc = prepare_tiles(e, nsteps, kmax))) return rc;
     const auto key = std::make_pair(nsteps, rot_key(e));
     auto it = e->graphs.find(key);
-    int role0[4];
-    memcpy(role0, e->role, sizeof role0);
-    const int s0 = e->scur;
-    const long sd0 = e->steps_done;
     if (it == e->graphs.end()) {
+        int role0[4];
+        memcpy(role0, e->role, sizeof role0);
+        const int s0 = e->scur;
+        const long sd0 = e->steps_done;
         hipGraph_t graph;
         HIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
         long nl = 0;
@@ -1337,8 +1378,27 @@ int engine_step(Engine *e, long nsteps, bool force_k1)
         if (ie != hipSuccess) return fail(e, WV_EHIP, "graph instantiate failed: %s", hipGetErrorString(ie));
         it = e->graphs.emplace(key, ex).first;
     }
-    HIPCHK(e, hipGraphLaunch(it->second, e->stream));
+    *out = it->second;
+    return WV_OK;
+}
+
+int engine_step(Engine *e, long nsteps, bool force_k1)
+{
+    if (nsteps < 0) return fail(e, WV_EINVAL, "negative step count");
+    int rc = ready(e);
+    if (rc) return rc;
+    HIPCHK(e, hipSetDevice(e->device));
+    if (nsteps == 0) return WV_OK;
+    if (!e->use_graph || force_k1) {
+        rc = enqueue_steps(e, nsteps, force_k1, &e->last_launches);
+        return rc;
+    }
+    // batch: capture the launch sequence once per (nsteps, rotation) and replay
+    hipGraphExec_t ex;
+    if ((rc = step_graph(e, nsteps, &ex))) return rc;
+    HIPCHK(e, hipGraphLaunch(ex, e->stream));
     // advance the host-side rotation exactly as the captured sequence does
+    const int kmax = max_depth(e, false);
     long done = 0, nl = 0;
     while (done < nsteps) {
         const int k = next_depth(kmax, nsteps - done);
@@ -1348,6 +1408,47 @@ int engine_step(Engine *e, long nsteps, bool force_k1)
     }
     e->steps_done += nsteps;
     e->last_launches = nl;
+    return WV_OK;
+}
+
+// Everything a later engine_step(nsteps) would do on the host before its
+// first launch, done now: the tile tables of every fused depth the sequence
+// uses, the first load of every stepping kernel it launches (one launch over a
+// padding tile, which returns at once), the energy kernel, and for batches the
+// captured graph.  The state does not change.
+int engine_prepare(Engine *e, long nsteps)
+{
+    if (nsteps < 0) return fail(e, WV_EINVAL, "negative step count");
+    int rc = ready(e);
+    if (rc) return rc;
+    HIPCHK(e, hipSetDevice(e->device));
+    if (nsteps == 0) return WV_OK;
+    const int kmax = max_depth(e, false);
+    if ((rc = prepare_tiles(e, nsteps, kmax))) return rc;
+    const int cm = step_cm(e);
+    bool seen[9] = {false};
+    long done = 0;
+    while (done < nsteps) {
+        const int k = next_depth(kmax, nsteps - done);
+        if (!seen[k]) {
+            seen[k] = true;
+            const int lr = launch_depth(e, k, cm, e->d_pad, 1, 0, 0);
+            if (lr) return fail(e, WV_EHIP, "kernel launch failed: %s", hipGetErrorString((hipError_t)lr));
+        }
+        done += k;
+        if (k == kmax) done += (nsteps - done) / kmax * kmax;
+    }
+    int lr = wv_launch_bump(e->d_gstep, 0, e->stream);
+    if (lr) return fail(e, WV_EHIP, "kernel launch failed: %s", hipGetErrorString((hipError_t)lr));
+    // the events a timed run of this sequence records (one per launch and two more)
+    long nl = 0;
+    for (long d = 0; d < nsteps; nl++) d += next_depth(kmax, nsteps - d);
+    if ((rc = reserve_kev(e, (size_t)nl + 2))) return rc;
+    if (e->use_graph) {
+        hipGraphExec_t ex;
+        if ((rc = step_graph(e, nsteps, &ex))) return rc;
+    }
+    HIPCHK(e, hipStreamSynchronize(e->stream));
     return WV_OK;
 }
 
@@ -1412,6 +1513,7 @@ int set_wells_tables(Engine *e, int k, const short *xy, int nw, const double *xc
         a.xc = d_w; a.yc = d_w + nw; a.h = h ? d_w + 2 * nw : nullptr;
         a.nz = G.nz; a.tcc = G.tccf; a.tc = G.tc;
         a.tcc_hi = split ? G.tccf_hi : nullptr; a.tcc_lo = split ? G.tccf_lo : nullptr;
+        a.split_weight = split_weight(e);
         err = (hipError_t)wv_launch_wells(a, e->stream);
     }
     if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
@@ -1577,40 +1679,50 @@ int engine_step_timed(Engine *e, long nsteps, double *ms, long *launches)
     int rc = ready(e);
     if (rc) return rc;
     HIPCHK(e, hipSetDevice(e->device));
-    hipEvent_t a, b;
-    HIPCHK(e, hipEventCreate(&a));
-    HIPCHK(e, hipEventCreate(&b));
-    HIPCHK(e, hipEventRecord(a, e->stream));
     // per-kernel events need the launches themselves, not a replayed graph
     const bool graph = e->use_graph;
     e->use_graph = false;
     e->time_kernels = true;
     e->kev_used = 0;
+    e->kev_pending = false;
+    e->last_avg_ms = e->last_timed = 0;
     rc = engine_step(e, nsteps, false);
     e->time_kernels = false;
     e->use_graph = graph;
-    if (rc) { (void)hipEventDestroy(a); (void)hipEventDestroy(b); return rc; }
-    HIPCHK(e, hipEventRecord(b, e->stream));
-    HIPCHK(e, hipEventSynchronize(b));
+    if (rc) return rc;
+    if (e->kev_used < 3) {  // no launch (nsteps == 0)
+        if (ms) *ms = 0;
+        if (launches) *launches = 0;
+        return WV_OK;
+    }
+    HIPCHK(e, hipEventSynchronize(e->kev[e->kev_used - 1]));
     float t = 0;
-    HIPCHK(e, hipEventElapsedTime(&t, a, b));
-    // average over the timed launches of full depth (a shorter last one differs)
+    HIPCHK(e, hipEventElapsedTime(&t, e->kev[0], e->kev[e->kev_used - 1]));
+    e->kev_pending = true;  // per-launch durations are read when asked for
+    if (ms) *ms = t;
+    if (launches) *launches = e->last_launches;
+    return WV_OK;
+}
+
+// per-launch durations of the last timed run: launch q runs between events q
+// and q+1; the average is over the launches of full depth (a shorter last one
+// differs)
+int kernel_times(Engine *e)
+{
+    if (!e->kev_pending) return WV_OK;
+    e->kev_pending = false;
     const int kmax = max_depth(e, false);
     double ti = 0;
     int nt = 0;
-    for (size_t q = 0; q + 2 <= e->kev_used; q += 2) {
+    for (size_t q = 1; q + 1 < e->kev_used; q++) {
         if (e->kev_depth[q] != kmax) continue;
         float x = 0;
-        HIPCHK(e, hipEventElapsedTime(&x, e->kev[q], e->kev[q + 1]));
+        HIPCHK(e, hipEventElapsedTime(&x, e->kev[q - 1], e->kev[q]));
         ti += x;
         nt++;
     }
     e->last_avg_ms = nt ? ti / nt : 0.0;
     e->last_timed = nt;
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    if (ms) *ms = t;
-    if (launches) *launches = e->last_launches;
     return WV_OK;
 }
 
@@ -1712,6 +1824,11 @@ int wv_step(wv_ctx *c, long nsteps)
     return engine_step(c->e, nsteps, false);
 }
 
+int wv_prepare(wv_ctx *c, long nsteps)
+{
+    if (!c) return WV_EINVAL;
+    return engine_prepare(c->e, nsteps);
+}
 int wv_step_timed(wv_ctx *c, long nsteps, double *ms, long *launches)
 {
     if (!c) return WV_EINVAL;
@@ -1732,6 +1849,8 @@ long wv_steps_done(const wv_ctx *c) { return c ? c->e->steps_done : -1; }
 int wv_last_kernel_ms(const wv_ctx *c, double *avg_ms, double *timed_launches)
 {
     if (!c) return WV_EINVAL;
+    int rc = kernel_times(c->e);
+    if (rc) return rc;
     if (avg_ms) *avg_ms = c->e->last_avg_ms;
     if (timed_launches) *timed_launches = c->e->last_timed;
     return WV_OK;
@@ -1962,6 +2081,11 @@ int wv_batch_step(wv_batch *b, long nsteps)
     return engine_step(b->e, nsteps, false);
 }
 
+int wv_batch_prepare(wv_batch *b, long nsteps)
+{
+    if (!b) return WV_EINVAL;
+    return engine_prepare(b->e, nsteps);
+}
 int wv_batch_step_timed(wv_batch *b, long nsteps, double *ms, long *launches)
 {
     if (!b) return WV_EINVAL;
@@ -1971,6 +2095,8 @@ int wv_batch_step_timed(wv_batch *b, long nsteps, double *ms, long *launches)
 int wv_batch_last_kernel_ms(const wv_batch *b, double *avg_ms, double *timed_launches)
 {
     if (!b) return WV_EINVAL;
+    int rc = kernel_times(b->e);
+    if (rc) return rc;
     if (avg_ms) *avg_ms = b->e->last_avg_ms;
     if (timed_launches) *timed_launches = b->e->last_timed;
     return WV_OK;

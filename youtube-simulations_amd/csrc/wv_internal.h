@@ -90,6 +90,8 @@ struct WvInst {
     int prb_n;
     long prb_every, prb_cap, prb_base;
     double tcc0, tcc1, tg0, tg1;         // coefficient pairs (modes UNIFORM / TWO)
+    // split form: binary32 pairs of tcc0 and tcc1 times the stencil weight
+    // (1 for 5-point, 1/6 for 9-point), and tgamma in binary32
     float tcch0, tccl0, tcch1, tccl1, tgf0, tgf1;
     double kappa, ks, gs, kt, gt, es2;
 };
@@ -155,10 +157,11 @@ struct WvFrame {
 int wv_launch_frame(const WvFrame &f, hipStream_t s);
 // Gaussian-well media (wv_media.hip): tcc and tc of every cell of an NX x NY
 // grid over (xmin, xmax, ymin, ymax), into arrays of row stride ldy; h may be
-// null (unit heights); tcc_hi / tcc_lo (split form) may be null
+// null (unit heights); tcc_hi / tcc_lo (split form: the binary32 pair of
+// tcc * split_weight) may be null
 struct WvWells {
     int nx, ny, ldy, ldw, nw, use_tanh, twospeeds;
-    double xmin, xmax, ymin, ymax, sigma, courant, courantb;
+    double xmin, xmax, ymin, ymax, sigma, courant, courantb, split_weight;
     const double *xc, *yc, *h;
     const uint64_t *nz;
     double *tcc, *tc;

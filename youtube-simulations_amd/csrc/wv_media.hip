@@ -36,9 +36,10 @@ __global__ void __launch_bounds__(256) k_wells(WvWells a)
     const bool nz = (gc(a.nz)[(size_t)i * a.ldw + (j >> 6)] >> (j & 63)) & 1u;
     gm(a.tc)[k] = nz ? sqrt(tcc) : (a.twospeeds ? a.courantb : 0.0);
     if (a.tcc_hi) {
-        const float hi = (float)tcc;
+        const double ts = tcc * a.split_weight;
+        const float hi = (float)ts;
         gm(a.tcc_hi)[k] = hi;
-        gm(a.tcc_lo)[k] = (float)(tcc - (double)hi);
+        gm(a.tcc_lo)[k] = (float)(ts - (double)hi);
     }
 }
 

@@ -7,7 +7,11 @@
 // which is the reference's -y + 2x + tcc*delta - KAPPA*x - tgamma*(x-y)
 // rearranged so that no 2x - y cancellation happens in binary32: lap(u) is
 // summed from neighbour differences of hi and of lo separately (differences
-// of nearby values are exact), tcc is applied as a hi + lo pair, and u' is
+// of nearby values are exact), tcc is applied as a hi + lo pair (for the
+// 9-point stencil the pair holds tcc/6, rounded once from binary64, and the
+// sum 4*(edge differences) + (diagonal differences) is not scaled: a binary32
+// 1/6 would change the wave speed by 3e-8 relative, a phase error that grows
+// with every step), and u' is
 // formed by adding v' to lo and that sum to hi with a Fast2Sum.  Edge cells
 // (outer rows and columns) are evaluated in FP64 with the reference's own
 // formulas.
@@ -82,7 +86,6 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
     const double *tcp = d.tc, *tccf = d.tccf, *tgf = d.tgf;
     const float *tccf_hi = d.tccf_hi, *tccf_lo = d.tccf_lo, *tgf32 = d.tgf32;
     const float kf = (float)p.kappa;
-    const float w9 = (float)W9;
     const float *hin = d.hi[ia], *lin = d.lo[ia], *vin = d.v[ia];
     gptr<float> hout = gm(d.hi[oc]), lout = gm(d.lo[oc]), vout = gm(d.v[oc]);
 
@@ -251,7 +254,7 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
                             // h0 - hSW the NE difference of lane j-1 in row r-1
                             const float dhd = (Dne_h - from_next(DSEh[lv - 1])) + (Dse_h - from_prev(DNEh[lv - 1]));
                             const float dld = (Dne_l - from_next(DSEl[lv - 1])) + (Dse_l - from_prev(DNEl[lv - 1]));
-                            lap = ((4.0f * dh + dhd) + (4.0f * dl + dld)) * w9;
+                            lap = (4.0f * dh + dhd) + (4.0f * dl + dld);
                         } else {
                             lap = dh + dl;
                         }
@@ -294,7 +297,7 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
                             const float lNW = (c < C - 1) ? lw[cn] : lwn, lSW = (c > 0) ? lw[cp] : lwp;
                             const float dhd = ((hNE - h0) - (h0 - hNW)) + ((hSE - h0) - (h0 - hSW));
                             const float dld = ((lNE - l0) - (l0 - lNW)) + ((lSE - l0) - (l0 - lSW));
-                            lap = ((4.0f * dh + dhd) + (4.0f * dl + dld)) * w9;
+                            lap = (4.0f * dh + dhd) + (4.0f * dl + dld);
                         } else {
                             lap = dh + dl;
                         }

@@ -33,13 +33,13 @@ EXPORTS = [
     "wv_set_state", "wv_set_state3", "wv_get_state", "wv_get_state3", "wv_set_sources",
     "wv_set_neumann", "wv_set_drive", "wv_set_floor", "wv_set_laplacian", "wv_set_probes", "wv_get_probes",
     "wv_set_wells_tables", "wv_build_mask", "wv_tiff_bound", "wv_tiff_encode", "wv_tiff_write",
-    "wv_step", "wv_step_timed", "wv_sync", "wv_steps_done", "wv_last_kernel_ms", "wv_energy",
+    "wv_step", "wv_prepare", "wv_step_timed", "wv_sync", "wv_steps_done", "wv_last_kernel_ms", "wv_energy",
     "wv_get_tiling", "wv_frame_params_default", "wv_frame_set_tables", "wv_frame_reset", "wv_frame",
     "wv_frame_rgba8",
     "wv_set_tiling", "wv_autotune", "wv_evolve_wave_half_rows", "wv_evolve_wave_rows",
     "wv_evolve_wave_half_flat", "wv_evolve_wave_flat", "wv_batch_create", "wv_batch_destroy",
     "wv_batch_last_error", "wv_batch_set_tables", "wv_batch_set_state", "wv_batch_get_state",
-    "wv_batch_step", "wv_batch_step_timed", "wv_batch_last_kernel_ms", "wv_batch_energy",
+    "wv_batch_step", "wv_batch_prepare", "wv_batch_step_timed", "wv_batch_last_kernel_ms", "wv_batch_energy",
     "wv_batch_set_tiling",
     "wv_batch_get_tiling", "wv_batch_autotune",
 ]
@@ -122,6 +122,7 @@ def load():
         "wv_get_probes": (ctypes.c_int, [vp, _D, ctypes.c_long, ctypes.POINTER(ctypes.c_long),
                                          ctypes.POINTER(ctypes.c_long)]),
         "wv_step": (ctypes.c_int, [vp, ctypes.c_long]),
+        "wv_prepare": (ctypes.c_int, [vp, ctypes.c_long]),
         "wv_step_timed": (ctypes.c_int, [vp, ctypes.c_long, _D, ctypes.POINTER(ctypes.c_long)]),
         "wv_sync": (ctypes.c_int, [vp]),
         "wv_steps_done": (ctypes.c_long, [vp]),
@@ -147,6 +148,7 @@ def load():
         "wv_batch_set_state": (ctypes.c_int, [vp, ctypes.c_int, _D, _D]),
         "wv_batch_get_state": (ctypes.c_int, [vp, ctypes.c_int, _D, _D]),
         "wv_batch_step": (ctypes.c_int, [vp, ctypes.c_long]),
+        "wv_batch_prepare": (ctypes.c_int, [vp, ctypes.c_long]),
         "wv_batch_step_timed": (ctypes.c_int, [vp, ctypes.c_long, _D, ctypes.POINTER(ctypes.c_long)]),
         "wv_batch_energy": (ctypes.c_int, [vp, _D]),
         "wv_batch_set_tiling": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
@@ -179,6 +181,31 @@ def _f64(a, shape):
     a = np.ascontiguousarray(a, dtype=np.float64)
     if a.shape != shape:
         raise ValueError("expected shape %s, got %s" % (shape, a.shape))
+    return a
+
+
+def _i16(a, shape):
+    a = np.ascontiguousarray(a, dtype=np.int16)
+    if a.shape != shape:
+        raise ValueError("expected shape %s, got %s" % (shape, a.shape))
+    return a
+
+
+def _out(a, shape, dtype):
+    """a caller's output array the library writes in place: exact shape,
+    dtype and C-contiguity, never a converted copy"""
+    if not isinstance(a, np.ndarray) or a.shape != tuple(shape) or a.dtype != dtype or not a.flags.c_contiguous \
+            or not a.flags.writeable:
+        raise ValueError("output must be a writeable C-contiguous %s array of shape %s" % (np.dtype(dtype), shape))
+    return a
+
+
+def _inplace(a, shape, dtype, what):
+    """an array the library reads and writes in place (drop-in forms)"""
+    if not isinstance(a, np.ndarray) or a.dtype != dtype or not a.flags.c_contiguous:
+        raise ValueError("%s must be a C-contiguous %s array" % (what, np.dtype(dtype)))
+    if a.size != shape[0] * shape[1] or (a.ndim == 2 and a.shape != tuple(shape)):
+        raise ValueError("%s must hold %d x %d values, got shape %s" % (what, shape[0], shape[1], a.shape))
     return a
 
 
@@ -216,7 +243,7 @@ class Wave:
             raise WaveError("%s: %d %s" % (what, rc, self._L.wv_last_error(self._h).decode()))
 
     def set_tables(self, xy_in, tcc, tc, tgamma):
-        xy = np.ascontiguousarray(xy_in, dtype=np.int16)
+        xy = _i16(xy_in, self.shape)
         a, b, c = (_f64(t, self.shape) for t in (tcc, tc, tgamma))
         self._chk(self._L.wv_set_tables(self._h, xy.ctypes.data_as(_S), _d(a), _d(b), _d(c)),
                   "wv_set_tables")
@@ -232,9 +259,12 @@ class Wave:
     def get_state(self, three=False, out=None):
         """(phi, psi) [, tmp] as new arrays, or into the C-contiguous float64
         arrays of out"""
-        a, b = (np.empty(self.shape), np.empty(self.shape)) if out is None else out[:2]
+        if out is None:
+            a, b = np.empty(self.shape), np.empty(self.shape)
+        else:
+            a, b = (_out(x, self.shape, np.float64) for x in out[:2])
         if three:
-            t = np.empty(self.shape)
+            t = np.empty(self.shape) if out is None or len(out) < 3 else _out(out[2], self.shape, np.float64)
             self._chk(self._L.wv_get_state3(self._h, _d(a), _d(b), _d(t)), "wv_get_state3")
             return a, b, t
         self._chk(self._L.wv_get_state(self._h, _d(a), _d(b)), "wv_get_state")
@@ -244,9 +274,11 @@ class Wave:
         if not sources:
             self._chk(self._L.wv_set_sources(self._h, 0, None, None, None, 0), "wv_set_sources")
             return
-        si = np.ascontiguousarray(sources["i"], np.int32)
-        sj = np.ascontiguousarray(sources["j"], np.int32)
+        si = np.ascontiguousarray(sources["i"], np.int32).ravel()
+        sj = np.ascontiguousarray(sources["j"], np.int32).ravel()
         amp = np.ascontiguousarray(sources["amp"], np.float64)
+        if sj.size != si.size or amp.ndim != 2 or amp.shape[0] != si.size or amp.shape[1] < 1:
+            raise ValueError("sources: i and j of one length n, amp of shape (n, steps)")
         self._chk(self._L.wv_set_sources(self._h, len(si), si.ctypes.data_as(_I), sj.ctypes.data_as(_I),
                                          _d(amp), amp.shape[1]), "wv_set_sources")
 
@@ -257,8 +289,10 @@ class Wave:
             self._chk(self._L.wv_set_neumann(self._h, 0, None, None, None, None, None, None, None, None),
                       "wv_set_neumann")
             return
-        ia = [np.ascontiguousarray(nb[k], np.int32) for k in ("i", "j", "nbi1", "nbj1", "nbi2", "nbj2")]
-        fa = [np.ascontiguousarray(nb[k], np.float64) for k in ("nx", "ny")]
+        ia = [np.ascontiguousarray(nb[k], np.int32).ravel() for k in ("i", "j", "nbi1", "nbj1", "nbi2", "nbj2")]
+        fa = [np.ascontiguousarray(nb[k], np.float64).ravel() for k in ("nx", "ny")]
+        if any(a.size != ia[0].size for a in ia + fa):
+            raise ValueError("Neumann points: every array needs one entry per point")
         self._chk(self._L.wv_set_neumann(self._h, len(ia[0]), *[a.ctypes.data_as(_I) for a in ia],
                                          *[_d(a) for a in fa]), "wv_set_neumann")
 
@@ -267,9 +301,11 @@ class Wave:
         if not drive:
             self._chk(self._L.wv_set_drive(self._h, 0, None, None, None, 0), "wv_set_drive")
             return
-        i = np.ascontiguousarray(drive["i"], np.int32)
-        j = np.ascontiguousarray(drive["j"], np.int32)
+        i = np.ascontiguousarray(drive["i"], np.int32).ravel()
+        j = np.ascontiguousarray(drive["j"], np.int32).ravel()
         val = np.ascontiguousarray(drive["val"], np.float64)
+        if j.size != i.size or val.ndim != 2 or val.shape[0] != i.size or val.shape[1] < 1:
+            raise ValueError("driven cells: i and j of one length n, val of shape (n, steps)")
         self._chk(self._L.wv_set_drive(self._h, len(i), i.ctypes.data_as(_I), j.ctypes.data_as(_I), _d(val),
                                        val.shape[1]), "wv_set_drive")
 
@@ -287,8 +323,10 @@ class Wave:
     def set_frame_tables(self, lut=None, color_scale=None, fade_table=None):
         F = ctypes.POINTER(ctypes.c_float)
         lt = None if lut is None else np.ascontiguousarray(lut, np.float32)
-        cs = None if color_scale is None else np.ascontiguousarray(color_scale, np.float64)
-        ft = None if fade_table is None else np.ascontiguousarray(fade_table, np.float64)
+        if lt is not None and (lt.ndim != 2 or lt.shape[1] != 3):
+            raise ValueError("palette table must be (n, 3)")
+        cs = None if color_scale is None else _f64(color_scale, self.shape)
+        ft = None if fade_table is None else _f64(fade_table, self.shape)
         self._chk(self._L.wv_frame_set_tables(self._h, None if lt is None else lt.ctypes.data_as(F),
                                               0 if lt is None else lt.shape[0],
                                               None if cs is None else cs.ctypes.data_as(_D),
@@ -304,8 +342,7 @@ class Wave:
         nbx = -(-self.p.nx // z)
         nby = -(-self.p.ny // z)
         if out is not None:
-            assert out.shape == (nbx, nby, 4) and out.dtype == (np.uint8 if u8 else np.float32)
-            assert out.flags.c_contiguous
+            _out(out, (nbx, nby, 4), np.uint8 if u8 else np.float32)
         if u8:
             out = np.empty((nbx, nby, 4), np.uint8) if out is None else out
             rc = self._L.wv_frame_rgba8(self._h, ctypes.byref(f), out.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
@@ -319,11 +356,13 @@ class Wave:
     def set_wells_tables(self, xy_in, xc, yc, height, sigma, use_tanh, courant, courantb, gamma, box, copy=True):
         """Gaussian-well media built on the GPU; returns (tcc, tc, tgamma), or
         None without copy"""
-        xy = np.ascontiguousarray(xy_in, np.int16)
-        xc = np.ascontiguousarray(xc, np.float64)
-        yc = np.ascontiguousarray(yc, np.float64)
-        hh = None if height is None else np.ascontiguousarray(height, np.float64)
-        bx = np.ascontiguousarray(box, np.float64)
+        xy = _i16(xy_in, self.shape)
+        xc = np.ascontiguousarray(xc, np.float64).ravel()
+        yc = np.ascontiguousarray(yc, np.float64).ravel()
+        hh = None if height is None else np.ascontiguousarray(height, np.float64).ravel()
+        bx = np.ascontiguousarray(box, np.float64).ravel()
+        if yc.size != xc.size or (hh is not None and hh.size != xc.size) or bx.size != 4:
+            raise ValueError("wells: xc, yc (and height) of one length, box of 4 values")
         t = [np.empty(self.shape) for _ in range(3)] if copy else None
         outs = [_d(a) for a in t] if copy else [None, None, None]
         self._chk(self._L.wv_set_wells_tables(self._h, xy.ctypes.data_as(_S), len(xc), _d(xc), _d(yc),
@@ -336,8 +375,12 @@ class Wave:
         D_CIRCLES 20 with circles = (4, n): xc, yc, radius, active); returns
         int16 (nx, ny)"""
         c = np.zeros((4, 1)) if circles is None else np.ascontiguousarray(circles, np.float64)
+        if c.ndim != 2 or c.shape[0] != 4:
+            raise ValueError("circles must be (4, n): xc, yc, radius, active")
         n = 0 if circles is None else c.shape[1]
-        bx = np.ascontiguousarray(box, np.float64)
+        bx = np.ascontiguousarray(box, np.float64).ravel()
+        if bx.size != 4:
+            raise ValueError("box must hold XMIN, XMAX, YMIN, YMAX")
         out = np.empty(self.shape, np.int16)
         self._chk(self._L.wv_build_mask(self._h, int(domain), _d(bx), lam, mu, n, _d(c), out.ctypes.data_as(_S)),
                   "wv_build_mask")
@@ -372,8 +415,10 @@ class Wave:
 
     def set_probes(self, i, j, every=3, capacity=4096):
         """record u at cells (i[k], j[k]) after every `every`-th step"""
-        ii = np.ascontiguousarray(i, np.int32)
-        jj = np.ascontiguousarray(j, np.int32)
+        ii = np.ascontiguousarray(i, np.int32).ravel()
+        jj = np.ascontiguousarray(j, np.int32).ravel()
+        if jj.size != ii.size:
+            raise ValueError("probes: i and j of one length")
         self._keep_probe_n = len(ii)
         self._chk(self._L.wv_set_probes(self._h, len(ii), ii.ctypes.data_as(_I), jj.ctypes.data_as(_I), every,
                                         capacity), "wv_set_probes")
@@ -396,6 +441,8 @@ class Wave:
         I = ctypes.POINTER(ctypes.c_int)
         nn = np.ascontiguousarray(table[0], np.int32)
         nb = np.ascontiguousarray(table[1], np.int32)
+        if nn.size != self.shape[0] * self.shape[1] or nb.size != 4 * nn.size:
+            raise ValueError("neighbour table: nneighb (NX, NY) and nghb (NX, NY, 4)")
         self._chk(self._L.wv_set_laplacian(self._h, nn.ctypes.data_as(I), nb.ctypes.data_as(I)),
                   "wv_set_laplacian")
 
@@ -409,6 +456,10 @@ class Wave:
     def step(self, n):
         self._chk(self._L.wv_step(self._h, int(n)), "wv_step")
 
+    def prepare(self, n):
+        """tile tables and kernel loads of a later step(n), done now"""
+        self._chk(self._L.wv_prepare(self._h, int(n)), "wv_prepare")
+
     def step_timed(self, n):
         ms = ctypes.c_double()
         nl = ctypes.c_long()
@@ -417,8 +468,8 @@ class Wave:
         return ms.value, nl.value
 
     def kernel_ms(self):
-        """(average duration in ms, number) of the event-timed launches of the
-        last step_timed (one launch in eight, full fused depth)"""
+        """(average duration in ms, number) of the launches of the last
+        step_timed at the full fused depth (every launch is event-timed)"""
         a, b = ctypes.c_double(), ctypes.c_double()
         self._chk(self._L.wv_last_kernel_ms(self._h, ctypes.byref(a), ctypes.byref(b)), "wv_last_kernel_ms")
         return a.value, int(b.value)
@@ -451,21 +502,28 @@ class Wave:
     # ---- drop-in forms (reference signatures) --------------------------------
     def evolve_wave_half_flat(self, phi_in, psi_in, phi_out, xy_in, tc, tcc, tgamma):
         """evolve_wave_half of wave_3d.c:600 -- in place on phi_out."""
-        arrs = [phi_in, psi_in, phi_out, tc, tcc, tgamma]
-        for a in arrs:
-            assert a.dtype == np.float64 and a.flags.c_contiguous
+        for a, nm in ((phi_in, "phi_in"), (psi_in, "psi_in"), (phi_out, "phi_out"), (tc, "tc"), (tcc, "tcc"),
+                      (tgamma, "tgamma")):
+            _inplace(a, self.shape, np.float64, nm)
+        _inplace(xy_in, self.shape, np.int16, "xy_in")
         self._chk(self._L.wv_evolve_wave_half_flat(self._h, _d(phi_in), _d(psi_in), _d(phi_out),
                                                    xy_in.ctypes.data_as(_S), _d(tc), _d(tcc),
                                                    _d(tgamma)), "wv_evolve_wave_half_flat")
 
     def evolve_wave_flat(self, phi, psi, tmp, xy_in, tc, tcc, tgamma):
+        """evolve_wave of wave_3d.c:1010 -- three steps in place."""
+        for a, nm in ((phi, "phi"), (psi, "psi"), (tmp, "tmp"), (tc, "tc"), (tcc, "tcc"), (tgamma, "tgamma")):
+            _inplace(a, self.shape, np.float64, nm)
+        _inplace(xy_in, self.shape, np.int16, "xy_in")
         self._chk(self._L.wv_evolve_wave_flat(self._h, _d(phi), _d(psi), _d(tmp),
                                               xy_in.ctypes.data_as(_S), _d(tc), _d(tcc), _d(tgamma)),
                   "wv_evolve_wave_flat")
 
-    @staticmethod
-    def _rows(a, t):
+    def _rows(self, a, t):
         """array of row pointers into a C-contiguous 2D array (double *a[NX])"""
+        dt = np.int16 if t is ctypes.c_short else np.float64
+        if not isinstance(a, np.ndarray) or a.dtype != dt or a.shape != self.shape or not a.flags.c_contiguous:
+            raise ValueError("row-pointer arrays must be C-contiguous %s of shape %s" % (np.dtype(dt), self.shape))
         n = a.shape[0]
         base = a.ctypes.data
         stride = a.strides[0]
@@ -526,7 +584,7 @@ class Batch:
             raise WaveError("%s: %d %s" % (what, rc, self._L.wv_batch_last_error(self._h).decode()))
 
     def set_tables(self, k, xy_in, tcc, tc, tgamma):
-        xy = np.ascontiguousarray(xy_in, dtype=np.int16)
+        xy = _i16(xy_in, self.shapes[k])
         a, b, c = (_f64(t, self.shapes[k]) for t in (tcc, tc, tgamma))
         self._chk(self._L.wv_batch_set_tables(self._h, k, xy.ctypes.data_as(_S), _d(a), _d(b), _d(c)),
                   "wv_batch_set_tables")
@@ -547,6 +605,10 @@ class Batch:
 
     def step(self, n):
         self._chk(self._L.wv_batch_step(self._h, int(n)), "wv_batch_step")
+
+    def prepare(self, n):
+        """tile tables, kernel loads and the graph of a later step(n), done now"""
+        self._chk(self._L.wv_batch_prepare(self._h, int(n)), "wv_batch_prepare")
 
     def step_timed(self, n):
         ms = ctypes.c_double()
