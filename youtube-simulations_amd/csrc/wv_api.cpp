@@ -1695,7 +1695,13 @@ int engine_step_timed(Engine *e, long nsteps, double *ms, long *launches)
         if (launches) *launches = 0;
         return WV_OK;
     }
-    HIPCHK(e, hipEventSynchronize(e->kev[e->kev_used - 1]));
+    // wait by polling the last event: a blocking wait can sleep and wake up
+    // tens of microseconds after the work is done, which a caller timing a
+    // short sequence by wall clock would count
+    hipError_t q;
+    while ((q = hipEventQuery(e->kev[e->kev_used - 1])) == hipErrorNotReady) {
+    }
+    if (q != hipSuccess) return fail(e, WV_EHIP, "waiting for the timed steps: %s", hipGetErrorString(q));
     float t = 0;
     HIPCHK(e, hipEventElapsedTime(&t, e->kev[0], e->kev[e->kev_used - 1]));
     e->kev_pending = true;  // per-launch durations are read when asked for
