@@ -261,14 +261,21 @@ def test_edge_column_tiles(wh, oracle, bc, variant, form):
         hi, lo, v = oracle.split_from_fp64(phi, psi)
         hi, lo, v = oracle.split_run(p, hi, lo, v, n, xy, tcc, tc, tg)
         u_ref, um_ref = oracle.split_to_fp64(hi, lo, v)
-    for L in (8, 24):
-        say("edge columns", form, bc, variant, L)
-        W = wh.Wave(dict(nx=nx, ny=ny, bcond=bc, **kw), form=form, tb_depth=8, rows_per_tile=L)
-        W.set_tables(xy, tcc, tc, tg)
-        W.set_state(phi, psi)
-        W.step(n)
-        gs = W.get_state()
-        W.close()
+    # outer column blocks in strips of the default length, and (edge rows) of
+    # 16 and 20 rows between their first and last 8 rows
+    for L, edge_rows in ((8, None), (24, None), (24, "16"), (32, "20")):
+        say("edge columns", form, bc, variant, L, edge_rows)
+        if edge_rows:
+            os.environ["WAVEHIP_EDGE_ROWS"] = edge_rows
+        try:
+            W = wh.Wave(dict(nx=nx, ny=ny, bcond=bc, **kw), form=form, tb_depth=8, rows_per_tile=L)
+            W.set_tables(xy, tcc, tc, tg)
+            W.set_state(phi, psi)
+            W.step(n)
+            gs = W.get_state()
+            W.close()
+        finally:
+            os.environ.pop("WAVEHIP_EDGE_ROWS", None)
         assert bits_equal(gs[0], u_ref), first_diff(gs[0], u_ref)
         assert bits_equal(gs[1], um_ref)
 

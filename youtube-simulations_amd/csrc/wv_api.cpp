@@ -340,8 +340,8 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
     // slowest tile: there the short strips have 4 rows (at 1280x720, K = 4:
     // +30 % over 8), but at least K so that the next strip can be interior.
     int Lb = std::min(L, WV_LB_MAX);
+    double tot = 0;  // launch work in interior-row units
     {
-        double tot = 0;
         long ntiles = 0;
         for (int q = 0; q < e->n; q++) {
             const Grid &G = e->g[q];
@@ -359,6 +359,23 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
         if (ntiles < 4096) Lb = std::min(L, std::max(k, std::min(4, WV_LB_MAX)));
         if (5.0 * (L + 2 * k) <= 0.5 * tot / 4096.0) Lb = L;
     }
+    // Strips of the outer column blocks between their first and last rows.
+    // Where the kernel has the edge-column path (K = 8, 5-point, uniform
+    // medium: interior evaluation plus the edge formula of the outer columns)
+    // such a row costs about 3 interior rows, and the strips are as long as
+    // keeps one tile within a slot's share of the launch (at 4096^2: 24 rows,
+    // +8 % for the split form over 8; 48 rows and more make the edge tiles the
+    // critical path).  Otherwise they are boundary strips of Lb rows.
+    // WAVEHIP_EDGE_ROWS fixes the length (tuning).
+    int Lj = Lb;
+    {
+        bool edgej = k == 8 && e->stencil == 5 && WV_C == 1;
+        for (int q = 0; q < e->n; q++) edgej = edgej && e->g[q].coef == WV_COEF_UNIFORM;
+        if (edgej && Lb < L)
+            for (int c : {16, 24, 32, 48, 64, 96, 128})
+                if (c <= L && 3.0 * (c + 2 * k) <= tot / 4096.0) Lj = c;
+    }
+    if (const char *v = getenv("WAVEHIP_EDGE_ROWS")) Lj = std::max(1, std::min(L, atoi(v)));
     // flags: INTERIOR when every row and column the tile reads lies inside the
     // outer box (no edge formula, no wrapping); ALLACT when, in addition,
     // every one of those cells is written by a step
@@ -421,10 +438,14 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
         for (int i = head + L; i < tail; i += L) cut.push_back(i);
         if (tail > cut.back()) cut.push_back(tail);
         if (nx > cut.back()) cut.push_back(nx);
-        // row cuts of the outer column blocks: strips of lb rows
-        std::vector<int> cutb;
-        for (int i = 0; i < nx; i += lb) cutb.push_back(i);
-        cutb.push_back(nx);
+        // row cuts of the outer column blocks: lb rows at either end, strips
+        // of lj rows between
+        const int lj = allb ? L : Lj;
+        std::vector<int> cutb{0};
+        if (head > 0) cutb.push_back(head);
+        for (int i = head + lj; i < tail; i += lj) cutb.push_back(i);
+        if (tail > cutb.back()) cutb.push_back(tail);
+        if (nx > cutb.back()) cutb.push_back(nx);
         auto outer = [&](int cb) {
             const int js = cb * wout - wv_halo(k);
             return !(js >= 1 && js + 64 * WV_C <= ny - 1);
