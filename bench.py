@@ -79,9 +79,12 @@ def run_form(wh, w, form, steps, warmup, tb_depth=0, rows=0, tune=False):
     W.prepare(steps)
     W.sync()
     t0 = time.perf_counter()
-    ms_dev, launches = W.step_timed(steps)   # HIP events on the stepping stream, waits
-    W.sync()
+    # HIP events on the stepping stream: one before the first launch, one
+    # after each launch and one after the step counter's update, the last
+    # waited for -- when it returns every step of the sequence has completed
+    ms_dev, launches = W.step_timed(steps)
     wall = time.perf_counter() - t0
+    W.sync()
     avg_ms, n_timed = W.kernel_ms()          # every full-depth launch of the same run, event-timed
     e1 = W.energy()
     u, um = W.get_state()

@@ -708,21 +708,21 @@ void orc_split_run(const orc_params *p, float *hi, float *lo, float *v, long nst
                 size_t k = (size_t)i * NY + j;
                 if (!ACTIVE_BULK(p, xy_in[k])) continue;
                 const float hc = hi[k], lc = lo[k], vc = v[k];
-                /* every neighbour difference is (east/north/diagonal neighbour)
-                 * minus (cell), evaluated once per pair of cells: the west and
-                 * south ones enter as minus the difference their neighbour takes */
-                float dh = ((hi[k + NY] - hc) - (hc - hi[k - NY])) + ((hi[k + 1] - hc) - (hc - hi[k - 1]));
-                float dl = ((lo[k + NY] - lc) - (lc - lo[k - NY])) + ((lo[k + 1] - lc) - (lc - lo[k - 1]));
+                /* every neighbour difference is a difference of u = hi + lo,
+                 * (hi[a] - hi[b]) + (lo[a] - lo[b]) with a the east / north /
+                 * diagonal neighbour and b the cell, evaluated once per pair of
+                 * cells: the west and south ones enter as minus the difference
+                 * their neighbour takes */
+#define DIF(a, b) ((hi[a] - hi[b]) + (lo[a] - lo[b]))
+                const float d = (DIF(k + NY, k) - DIF(k, k - NY)) + (DIF(k + 1, k) - DIF(k, k - 1));
                 float lap;
                 if (p->stencil == 9) {
-                    float dhd = ((hi[k + NY + 1] - hc) - (hc - hi[k - NY + 1]))
-                                + ((hi[k + NY - 1] - hc) - (hc - hi[k - NY - 1]));
-                    float dld = ((lo[k + NY + 1] - lc) - (lc - lo[k - NY + 1]))
-                                + ((lo[k + NY - 1] - lc) - (lc - lo[k - NY - 1]));
-                    lap = (4.0f * dh + dhd) + (4.0f * dl + dld);
+                    const float dd = (DIF(k + NY + 1, k) - DIF(k, k - NY + 1)) + (DIF(k + NY - 1, k) - DIF(k, k - NY - 1));
+                    lap = 4.0f * d + dd;
                 } else {
-                    lap = dh + dl;
+                    lap = d;
                 }
+#undef DIF
                 const double cw = tcc[k] * sw;
                 const float ch = (float)cw;
                 const float cl = (float)(cw - (double)ch);

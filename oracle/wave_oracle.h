@@ -229,10 +229,12 @@ double orc_fixed_order_sum(int nx, int ny, const double *e);
 /* ---- FP32 double-float ("split") form, build-defined ----------------------
  * State per cell: u = hi + lo (two floats, |lo| <= ulp(hi)/2) and the
  * velocity v = u^n - u^{n-1} as one float.  Interior active cells:
- *   dh   = ((hE-hc) - (hc-hW)) + ((hN-hc) - (hc-hS))
- *   dl   = same on lo;  lap = dh + dl
- *          (9-point: lap = (4*dh + dhd) + (4*dl + dld), dhd/dld the same
- *          over the diagonal neighbours, unscaled)
+ *   D(a) = (h_a - hc) + (l_a - lc)   (a difference of u = h + l, one
+ *          rounding); D(W), D(S) are taken by the west / south neighbour
+ *          towards the cell: D(W) = (hc - hW) + (lc - lW)
+ *   lap  = (D(E) - D(W)) + (D(N) - D(S))
+ *          (9-point: lap = 4*that + ((D(NE) - D(NW)) + (D(SE) - D(SW))),
+ *          NE = (i+1, j+1), NW = (i-1, j+1), unscaled)
  *   (tcc_hi, tcc_lo) = binary32 pair of tcc*w, w = 1 (5-point) or ORC_W9
  *          (9-point), w applied in binary64
  *   vn   = ((vc + fmaf(tcc_hi, lap, tcc_lo*lap)) - kappa*hc) - gamma*vc

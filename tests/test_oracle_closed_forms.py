@@ -157,9 +157,12 @@ def test_sources_are_linear_and_reciprocal(oracle):
 
 
 def test_split_form_tracks_fp64(oracle):
-    """The FP32 double-float form stays within 1e-6 of FP64 (L2, relative
-    to the field's norm; with absorbing edges, relative to the initial norm,
-    since the field itself drains out of the box)."""
+    """The FP32 double-float form stays within 2e-6 of FP64 after 3,000 steps
+    (L2, relative to the field's norm; with absorbing edges, relative to the
+    initial norm, since the field itself drains out of the box).  The pulse
+    is 7 cells wide, so the velocity, whose binary32 rounding is the form's
+    main error, is a larger part of the field than in the 4096^2 workload
+    (1e-7 there after 10,000 steps)."""
     nx = ny = 96
     x, y = workloads.grid_xy(nx, ny, -1, 1, -1, 1)
     f = workloads.gaussian_bumps(x, y, [(-0.2, 0.1)], 0.15)
@@ -173,8 +176,34 @@ def test_split_form_tracks_fp64(oracle):
         us, ums = oracle.split_to_fp64(hi, lo, v)
         ref = np.linalg.norm(f) if bc in (2, 3) else np.linalg.norm(u)
         rel = np.linalg.norm(us - u) / ref
-        assert rel < 1e-6, (bc, st, rel)
+        assert rel < 2e-6, (bc, st, rel)
         assert np.linalg.norm((us - ums) - (u - um)) / np.linalg.norm(f) < 1e-7
+
+
+def test_split_nine_point_weight_in_binary64(oracle):
+    """The split form applies the 9-point weight 1/6 folded into the binary64
+    coefficient (tcc/6 split into a binary32 pair).  A binary32 1/6 instead
+    changes the wave speed by 1.5e-8 relative, a phase error that grows with
+    every step: a driven source of period 40 (waves of 4 cells) for 5,000
+    steps shows it.  Planted mistake: the same run with tcc scaled by
+    6*float(1/6) (what a binary32 weight does) must drift measurably
+    further from FP64 than the form itself."""
+    nx, ny, n = 160, 90, 5000
+    xy = np.ones((nx, ny), np.int16)
+    tcc, tc, tg = workloads.constant_media_tables(xy, 0.1, 0.1, 0.0, 0.0, 0)
+    amp = np.sin(2.0 * math.pi * np.arange(1, n + 1) / 40.0)[None, :]
+    src = dict(i=[40], j=[45], amp=amp)
+    z = np.zeros((nx, ny))
+    p = oracle.make_params(nx, ny, bcond=2, stencil=9)
+    u = oracle.run(p, z, z, n, xy, tcc, tc, tg, sources=src)[0]
+    rels = []
+    for scale in (1.0, 6.0 * float(np.float32(1.0 / 6.0))):
+        hi, lo, v = oracle.split_from_fp64(z, z)
+        hi, lo, v = oracle.split_run(p, hi, lo, v, n, xy, tcc * scale, tc, tg, sources=src)
+        us = oracle.split_to_fp64(hi, lo, v)[0]
+        rels.append(np.linalg.norm(us - u) / np.linalg.norm(u))
+    assert rels[0] < 3e-6, rels
+    assert rels[1] > 2.0 * rels[0], rels
 
 
 def test_split_conversion_round_trip(oracle):

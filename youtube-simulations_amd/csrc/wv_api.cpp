@@ -461,28 +461,59 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
             (takeb ? b : a)++;
         }
     }
+    // Estimated run time of a tile in units of one interior level-row (one
+    // level of one row of 64 lanes): K*rows + K*(K-1) level-rows (the skewed
+    // pipeline fills and drains over 2K halo rows; half of that at the first
+    // and last rows of a grid not periodic along i, whose rows outside the box
+    // are skipped), times the cost of a level-row on the tile's path, measured
+    // per tile at 4096^2 (K = 8, both forms): interior 1 (1.15 with inactive
+    // cells), edge-column tiles 4.5, tiles of the first or last rows 8, and
+    // the corner tiles of both 16.
+    auto tile_cost = [&](const WvTile &t) {
+        const Grid &G = e->g[t.inst];
+        const int n = t.flags >> WV_TILE_ROWS_SHIFT, nx = G.p.nx;
+        const bool wrapi = G.p.bcond == 1;
+        const bool rowedge = t.i0 < k || t.i0 + n > nx - k;
+        const double lr = (double)k * n + (rowedge && !wrapi ? 0.5 : 1.0) * k * (k - 1);
+        const int js = t.cb * wout - wv_halo(k);
+        const bool coledge = !(js >= 1 && js + 64 * WV_C <= G.p.ny - 1);
+        double f;
+        if (t.flags & WV_TILE_INTERIOR) f = (t.flags & WV_TILE_ALLACT) ? 1.0 : 1.15;
+        else if (t.flags & WV_TILE_EDGEJ) f = 4.5;
+        else f = (rowedge && coledge) ? 16.0 : 8.0;
+        return lr * f;
+    };
     // one list in XCD-interleave order.  Each XCD takes one contiguous run of
-    // the grid order holding an eighth of the rows x columns, and starts with
-    // its slowest tiles (boundary tiles, then interior tiles with inactive
-    // cells): started last, they would run on after the rest had finished
+    // the grid order holding an eighth of the estimated work, and starts with
+    // its longest tiles: started last, they would run on after the rest had
+    // finished.  (Balancing rows instead of work gave the two XCDs holding
+    // the first and last rows, with their slow boundary tiles, 12 % more work
+    // and a launch 20 % longer than the other six needed.)
     std::vector<WvTile> h, hb;
     if (e->xcd_order) {
         const int X = 8;
         std::vector<std::vector<WvTile>> per(X);
+        std::vector<double> cost(tl.size());
         double total = 0;
-        for (const WvTile &t : tl) total += t.flags >> WV_TILE_ROWS_SHIFT;
+        for (size_t q = 0; q < tl.size(); q++) total += (cost[q] = tile_cost(tl[q]));
         double acc = 0;
-        for (const WvTile &t : tl) {
-            const int x = std::min(X - 1, (int)(acc * X / std::max(total, 1.0)));
-            per[x].push_back(t);
-            acc += t.flags >> WV_TILE_ROWS_SHIFT;
+        std::vector<std::vector<double>> pc(X);
+        for (size_t q = 0; q < tl.size(); q++) {
+            const int x = std::min(X - 1, (int)((acc + 0.5 * cost[q]) * X / std::max(total, 1e-30)));
+            per[x].push_back(tl[q]);
+            pc[x].push_back(cost[q]);
+            acc += cost[q];
         }
         if (e->slow_first) {
-            auto rank = [](const WvTile &t) {
-                return !(t.flags & WV_TILE_INTERIOR) ? 0 : (!(t.flags & WV_TILE_ALLACT) ? 1 : 2);
-            };
-            for (auto &v : per)
-                std::stable_sort(v.begin(), v.end(), [&](const WvTile &a, const WvTile &b) { return rank(a) < rank(b); });
+            for (int x = 0; x < X; x++) {
+                std::vector<size_t> idx(per[x].size());
+                for (size_t q = 0; q < idx.size(); q++) idx[q] = q;
+                // longest first; tiles of equal estimate keep their grid order
+                std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return pc[x][a] > pc[x][b]; });
+                std::vector<WvTile> v(idx.size());
+                for (size_t q = 0; q < idx.size(); q++) v[q] = per[x][idx[q]];
+                per[x].swap(v);
+            }
         }
         size_t len = 0;
         for (const auto &v : per) len = std::max(len, v.size());
@@ -1465,6 +1496,8 @@ int engine_prepare(Engine *e, long nsteps)
     long nl = 0;
     for (long d = 0; d < nsteps; nl++) d += next_depth(kmax, nsteps - d);
     if ((rc = reserve_kev(e, (size_t)nl + 2))) return rc;
+    // record each once: the runtime sets an event up on its first record
+    for (size_t q = 0; q < (size_t)nl + 2; q++) HIPCHK(e, hipEventRecord(e->kev[q], e->stream));
     if (e->use_graph) {
         hipGraphExec_t ex;
         if ((rc = step_graph(e, nsteps, &ex))) return rc;

@@ -6,8 +6,9 @@
 //   v' = v + tcc*lap(u) - KAPPA*u - tgamma*v,   u' = u + v'
 // which is the reference's -y + 2x + tcc*delta - KAPPA*x - tgamma*(x-y)
 // rearranged so that no 2x - y cancellation happens in binary32: lap(u) is
-// summed from neighbour differences of hi and of lo separately (differences
-// of nearby values are exact), tcc is applied as a hi + lo pair (for the
+// built from neighbour differences of u, each the sum of the hi difference
+// and the lo difference (differences of nearby values are exact), tcc is
+// applied as a hi + lo pair (for the
 // 9-point stencil the pair holds tcc/6, rounded once from binary64, and the
 // sum 4*(edge differences) + (diagonal differences) is not scaled: a binary32
 // 1/6 would change the wave speed by 3e-8 relative, a phase error that grows
@@ -43,12 +44,17 @@ constexpr int CF_ANY = 6;      // per-cell tables read in place (boundary tiles)
 // tile reads is written by a step.  DAMP: the grid has a non-zero KAPPA or
 // tgamma (terms with a zero coefficient are not evaluated).
 //
-// Every neighbour difference is (east / north / diagonal neighbour) minus
+// Every neighbour difference is a difference of u = hi + lo, taken as (hi
+// difference) + (lo difference): (east / north / diagonal neighbour) minus
 // (cell); a cell's west, south and two western diagonal terms are minus the
 // difference that neighbour takes towards it.  Interior tiles evaluate each
 // difference once: along i it is kept for the next iteration of the same
 // level, along j and the diagonals it comes from the adjacent lane.  Boundary
 // tiles evaluate the same subtractions directly; the results are the same.
+// (Adding the hi and lo differences before the second difference rounds each
+// first difference once, at 2^-24 of its size; that error is a fraction
+// 2*courant of the velocity's own rounding per step, and it saves two
+// additions per cell and level against keeping the two sums apart.)
 //
 // EJ (edge columns): the tile's rows never reach the outer box but its
 // columns do.  Every cell takes the interior evaluation and the cells of the
@@ -155,10 +161,10 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
         for (int lv = 1; lv <= K; lv++) fetch_tv(lv, s0 - 2 * lv);
     }
     // differences kept from the previous iteration of each level (SHARE):
-    // row r+1 minus row r of hi and lo, and the two diagonal ones (9-point)
-    float DEh[K], DEl[K], DNEh[K], DNEl[K], DSEh[K], DSEl[K];
+    // row r+1 minus row r, and the two diagonal ones (9-point)
+    float DE[K], DNE[K], DSE[K];
 #pragma unroll
-    for (int a = 0; a < K; a++) DEh[a] = DEl[a] = DNEh[a] = DNEl[a] = DSEh[a] = DSEl[a] = 0.0f;
+    for (int a = 0; a < K; a++) DE[a] = DNE[a] = DSE[a] = 0.0f;
 
     auto iter = [&](int s, auto phase) {
         constexpr int PR = decltype(phase)::value;  // ring slot of row s
@@ -171,17 +177,14 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
             // differences of this iteration's rows (east minus centre), also in
             // iterations where the level is not needed: the first needed one
             // takes its west difference from here
-            float De_h = 0, De_l = 0, Dne_h = 0, Dne_l = 0, Dse_h = 0, Dse_l = 0;
+            float De = 0, Dne = 0, Dse = 0;
             if (SHARE) {
                 const float h0 = Hw[lv - 1][PC][0], l0 = Lw[lv - 1][PC][0];
                 const float he0 = Hw[lv - 1][PE][0], le0 = Lw[lv - 1][PE][0];
-                De_h = he0 - h0;
-                De_l = le0 - l0;
+                De = (he0 - h0) + (le0 - l0);
                 if (ST == 9) {
-                    Dne_h = from_next(he0) - h0;
-                    Dne_l = from_next(le0) - l0;
-                    Dse_h = from_prev(he0) - h0;
-                    Dse_l = from_prev(le0) - l0;
+                    Dne = (from_next(he0) - h0) + (from_next(le0) - l0);
+                    Dse = (from_prev(he0) - h0) + (from_prev(le0) - l0);
                 }
             }
             // level lv is live at iterations s in [ibeg-K+3lv, iend+K+lv)
@@ -245,18 +248,16 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
                         // h0 - hw is the east difference of row r-1 (previous iteration),
                         // h0 - hS the north difference of lane j-1
                         const float h0 = hc[c], l0 = lc[c];
-                        const float Dn_h = hN - h0, Dn_l = lN - l0;
-                        const float dh = (De_h - DEh[lv - 1]) + (Dn_h - from_prev(Dn_h));
-                        const float dl = (De_l - DEl[lv - 1]) + (Dn_l - from_prev(Dn_l));
+                        const float Dn = (hN - h0) + (lN - l0);
+                        const float dsum = (De - DE[lv - 1]) + (Dn - from_prev(Dn));
                         float lap;
                         if (ST == 9) {
-                            // h0 - hNW is the SE difference of lane j+1 in row r-1,
-                            // h0 - hSW the NE difference of lane j-1 in row r-1
-                            const float dhd = (Dne_h - from_next(DSEh[lv - 1])) + (Dse_h - from_prev(DNEh[lv - 1]));
-                            const float dld = (Dne_l - from_next(DSEl[lv - 1])) + (Dse_l - from_prev(DNEl[lv - 1]));
-                            lap = (4.0f * dh + dhd) + (4.0f * dl + dld);
+                            // u0 - uNW is the SE difference of lane j+1 in row r-1,
+                            // u0 - uSW the NE difference of lane j-1 in row r-1
+                            const float dd = (Dne - from_next(DSE[lv - 1])) + (Dse - from_prev(DNE[lv - 1]));
+                            lap = 4.0f * dsum + dd;
                         } else {
-                            lap = dh + dl;
+                            lap = dsum;
                         }
                         float ch, cl, gf;
                         if (PREF) {
@@ -287,19 +288,21 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
                         if (EJ && !L.colint[c]) edge_cell(oh, ol, ov);
                     } else if (rowint && L.colint[c]) {
                         const float h0 = hc[c], l0 = lc[c];
-                        const float dh = ((he[c] - h0) - (h0 - hw[c])) + ((hN - h0) - (h0 - hS));
-                        const float dl = ((le[c] - l0) - (l0 - lw[c])) + ((lN - l0) - (l0 - lS));
+                        // differences of u = hi + lo: (hi difference) + (lo difference)
+                        const float dE = (he[c] - h0) + (le[c] - l0), dW = (h0 - hw[c]) + (l0 - lw[c]);
+                        const float dN = (hN - h0) + (lN - l0), dS = (h0 - hS) + (l0 - lS);
+                        const float dsum = (dE - dW) + (dN - dS);
                         float lap;
                         if (ST == 9) {
                             const float hNE = (c < C - 1) ? he[cn] : hen, hSE = (c > 0) ? he[cp] : hep;
                             const float hNW = (c < C - 1) ? hw[cn] : hwn, hSW = (c > 0) ? hw[cp] : hwp;
                             const float lNE = (c < C - 1) ? le[cn] : len_, lSE = (c > 0) ? le[cp] : lep;
                             const float lNW = (c < C - 1) ? lw[cn] : lwn, lSW = (c > 0) ? lw[cp] : lwp;
-                            const float dhd = ((hNE - h0) - (h0 - hNW)) + ((hSE - h0) - (h0 - hSW));
-                            const float dld = ((lNE - l0) - (l0 - lNW)) + ((lSE - l0) - (l0 - lSW));
-                            lap = (4.0f * dh + dhd) + (4.0f * dl + dld);
+                            const float dNE = (hNE - h0) + (lNE - l0), dNW = (h0 - hNW) + (l0 - lNW);
+                            const float dSE = (hSE - h0) + (lSE - l0), dSW = (h0 - hSW) + (l0 - lSW);
+                            lap = 4.0f * dsum + ((dNE - dNW) + (dSE - dSW));
                         } else {
-                            lap = dh + dl;
+                            lap = dsum;
                         }
                         float ch, cl, gf;
                         if (PREF) {
@@ -366,13 +369,10 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
                 }
             }
             if (SHARE) {
-                DEh[lv - 1] = De_h;
-                DEl[lv - 1] = De_l;
+                DE[lv - 1] = De;
                 if (ST == 9) {
-                    DNEh[lv - 1] = Dne_h;
-                    DNEl[lv - 1] = Dne_l;
-                    DSEh[lv - 1] = Dse_h;
-                    DSEl[lv - 1] = Dse_l;
+                    DNE[lv - 1] = Dne;
+                    DSE[lv - 1] = Dse;
                 }
             }
             // unconditional, like the ring
