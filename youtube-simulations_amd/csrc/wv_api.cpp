@@ -327,11 +327,11 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
     // the edge formulas and take several times as long per row; short, they
     // finish within the launch instead of setting its length.
     //
-    // Short strips cost redundant rows (2K per strip), so they are used only
-    // where one boundary tile of L rows would run for more than half a slot's
-    // share of the launch: an estimate from the geometry alone, counting a
-    // boundary row as 5 interior rows (measured at 4096^2, K = 6 and 8) and
-    // 4,096 resident waves.
+    // Short strips cost redundant rows (2K per strip), so they are only as
+    // short as keeps one boundary tile within a slot's share of the launch
+    // (half a share for strips of the full length L): an estimate from the
+    // geometry alone, counting a boundary row as 8 interior rows (measured
+    // per tile at 4096^2, K = 8) and 4,096 resident waves.
 #ifndef WV_LB_MAX
 #define WV_LB_MAX 8  // rows of a short strip (4 and 16 measured slower at 4096^2)
 #endif
@@ -354,15 +354,22 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
             }
             const double nb = (double)ob * nsb + (double)(ncb - ob) * std::min(nsb, 2);
             const double ni = (double)(ncb - ob) * std::max(nsb - 2, 0);
-            tot += (double)(L + 2 * k) * (ni + 5.0 * nb);
+            tot += (double)(L + 2 * k) * (ni + 8.0 * nb);
         }
         if (ntiles < 4096) Lb = std::min(L, std::max(k, std::min(4, WV_LB_MAX)));
-        if (5.0 * (L + 2 * k) <= 0.5 * tot / 4096.0) Lb = L;
+        // on grids large enough, longer boundary strips (fewer halo rows)
+        // still finish within a slot's share of the launch (they start
+        // first): at 8192^2 32 rows (+5 % over 8); at 4096^2 even 16 rows
+        // would set the launch length
+        for (int c : {16, 24, 32, 48, 64})
+            if (c <= L && 8.0 * (c + 2 * k) <= tot / 4096.0) Lb = c;
+        if (8.0 * (L + 2 * k) <= 0.5 * tot / 4096.0) Lb = L;
     }
+    if (const char *v = getenv("WAVEHIP_BOUNDARY_ROWS")) Lb = std::max(k, std::min(L, atoi(v)));
     // Strips of the outer column blocks between their first and last rows.
     // Where the kernel has the edge-column path (K = 8, 5-point, uniform
     // medium: interior evaluation plus the edge formula of the outer columns)
-    // such a row costs about 3 interior rows, and the strips are as long as
+    // such a row costs about 3.5 interior rows, and the strips are as long as
     // keeps one tile within a slot's share of the launch (at 4096^2: 24 rows,
     // +8 % for the split form over 8; 48 rows and more make the edge tiles the
     // critical path).  Otherwise they are boundary strips of Lb rows.
@@ -373,7 +380,7 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
         for (int q = 0; q < e->n; q++) edgej = edgej && e->g[q].coef == WV_COEF_UNIFORM;
         if (edgej && Lb < L)
             for (int c : {16, 24, 32, 48, 64, 96, 128})
-                if (c <= L && 3.0 * (c + 2 * k) <= tot / 4096.0) Lj = c;
+                if (c <= L && 3.5 * (c + 2 * k) <= tot / 4096.0) Lj = c;
     }
     if (const char *v = getenv("WAVEHIP_EDGE_ROWS")) Lj = std::max(1, std::min(L, atoi(v)));
     // flags: INTERIOR when every row and column the tile reads lies inside the
@@ -387,9 +394,26 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
         // (COMPARISON: neighbours across the grid, one step per launch)
         const bool interior = js >= 1 && js + 64 * WV_C <= ny - 1 && ibeg >= k && iend <= nx - k && G.tbl != 2 &&
                               !(G.p.flags & WV_FLAG_COMPARISON);
-        // (wrapped halos: any source, Neumann point or driven cell of the grid)
-        const int bflags = (G.h_si.empty() ? 0 : WV_TILE_SRC) | (G.h_ni.empty() ? 0 : WV_TILE_NEU) |
-                           (G.h_di.empty() ? 0 : WV_TILE_DRV) | (G.h_pi.empty() ? 0 : WV_TILE_PRB);
+        // boundary tiles: the point lists whose cells lie in the rows and
+        // columns the tile computes, wrapped where the boundary wraps (rows
+        // and columns clamped onto the edge are halo cells outside the
+        // dependence cone of the tile's output, and rows outside the box are
+        // skipped)
+        const bool wrapi = G.p.bcond == 1, wrapj = G.p.bcond == 1 || G.p.bcond == 3;
+        auto in_range = [](int p, int lo, int hi, int n, bool wrap) {
+            if (!wrap) return p >= lo && p < hi;
+            if (hi - lo >= n) return true;
+            return ((p - lo) % n + n) % n < hi - lo;
+        };
+        auto any_hit = [&](const std::vector<int> &pi, const std::vector<int> &pj, int rlo, int rhi) {
+            for (size_t q = 0; q < pi.size(); q++)
+                if (in_range(pi[q], rlo, rhi, nx, wrapi) && in_range(pj[q], js, js + 64 * WV_C, ny, wrapj)) return true;
+            return false;
+        };
+        const int bflags = (any_hit(G.h_si, G.h_sj, ibeg - k, iend + k) ? WV_TILE_SRC : 0) |
+                           (any_hit(G.h_ni, G.h_nj, ibeg - k, iend + k) ? WV_TILE_NEU : 0) |
+                           (any_hit(G.h_di, G.h_dj, ibeg - k, iend + k) ? WV_TILE_DRV : 0) |
+                           (any_hit(G.h_pi, G.h_pj, ibeg, iend) ? WV_TILE_PRB : 0);
         if (!interior) {
             // EDGEJ: rows inside the box, columns reaching column 0 or NY-1, on a
             // grid with no point lists and no FLOOR (the split kernel evaluates

@@ -413,6 +413,8 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
 // The edge-column variant is built into the uniform-medium 5-point kernel at
 // K = 8 only (elsewhere it would cost a resident wave per SIMD); there the
 // kernel fits three waves (168 registers, two spilled outside the loops).
+// The 9-point kernels also run three waves at K = 8 (8192^2 smooth medium:
+// +7 % over two, with some spills on paths that medium does not take).
 template <int K, int ST, int CM>
 struct EdgeJ {
     static constexpr bool on = K == 8 && ST == 5 && CM == WV_COEF_UNIFORM;
@@ -422,7 +424,7 @@ struct Occ {
 #ifdef WV_OCC_SPLIT
     static constexpr int waves = WV_OCC_SPLIT;
 #else
-    static constexpr int waves = K <= 4 ? 3 : (EdgeJ<K, ST, CM>::on ? 3 : 2);
+    static constexpr int waves = K <= 4 ? 3 : ((EdgeJ<K, ST, CM>::on || ST == 9) ? 3 : 2);
 #endif
 };
 
