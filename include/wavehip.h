@@ -134,6 +134,18 @@ int wv_set_wells_tables(wv_ctx *c, const short *xy_in, int nwells, const double 
 int wv_build_mask(wv_ctx *c, int domain, const double box[4], double lambda, double mu, int ncircles,
                   const double *circles, short *xy_in);
 
+/* wv_build_mask with no device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight frame loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The circle list and the mask
+ * pass through one fixed scratch array of 8 MiB per device, kept until the
+ * process ends (at most 32,768 circles; rows of at most 2,097,152 cells; any
+ * NX, in bands of rows); larger inputs are refused with WV_EINVAL.  Every
+ * call on a device shares that array, so calls that overlap on one device
+ * (several threads) race; a call waits for its own work before it returns.
+ * Same arguments and results as wv_build_mask. */
+int wv_build_mask_static(wv_ctx *c, int domain, const double box[4], double lambda, double mu, int ncircles,
+                         const double *circles, short *xy_in);
+
 /* ---- TIFF frames: writetiff, sub_wave.c:62-123 ----------------------------
  * The file writetiff(filename, description, 0, 0, width, height, compression)
  * writes through libtiff (4.x): RGB, 8 bits a sample, one row per strip,

@@ -2352,7 +2352,7 @@ static int build_mask(Engine *e, int domain, const double *box, double lambda, d
     if (err == hipSuccess && nc > 0) err = hipMemcpy(d_c, circ, sizeof(double) * 4 * (size_t)nc, hipMemcpyHostToDevice);
     if (err == hipSuccess) {
         WvMask a{};
-        a.nx = nx; a.ny = ny; a.kind = kind; a.ncirc = nc;
+        a.nx = nx; a.ny = ny; a.kind = kind; a.ncirc = nc; a.i0 = 0; a.rows = nx;
         a.xmin = box[0]; a.xmax = box[1]; a.ymin = box[2]; a.ymax = box[3];
         a.lambda = lambda; a.mu = mu; a.circ = d_c; a.out = d_out;
         err = (hipError_t)wv_launch_mask(a, e->stream);
@@ -2370,6 +2370,67 @@ extern "C" int wv_build_mask(wv_ctx *c, int domain, const double box[4], double 
 {
     if (!c) return WV_EINVAL;
     return build_mask(c->e, domain, box, lambda, mu, ncircles, circles, xy_in);
+}
+
+// build_mask without a device allocation: the circle list and the mask go
+// through the library's fixed scratch array (WV_MASK_SCRATCH_BYTES on each
+// device), the mask in bands of rows copied out one after the other on the
+// context's stream.  Calls that overlap on one device race for the array.
+static int build_mask_static(Engine *e, int domain, const double *box, double lambda, double mu, int ncirc,
+                             const double *circ, short *xy)
+{
+    int kind;
+    switch (domain) {
+    case WV_D_SINAI: kind = 0; break;
+    case WV_D_EXT_ELLIPSE: kind = 1; break;
+    case WV_D_CIRCLES: kind = 2; break;
+    default: return fail(e, WV_EUNSUPPORTED, "domain %d: D_SINAI, D_EXT_ELLIPSE and D_CIRCLES are provided", domain);
+    }
+    if (!box || !xy) return fail(e, WV_EINVAL, "box or output is NULL");
+    if (!(box[1] > box[0]) || !(box[3] > box[2])) return fail(e, WV_EINVAL, "empty box");
+    if (kind == 2 && (ncirc < 0 || (ncirc > 0 && !circ))) return fail(e, WV_EINVAL, "circle list");
+    const int nc = kind == 2 ? ncirc : 0;
+    if (nc > WV_MASK_SCRATCH_MAX_CIRCLES)
+        return fail(e, WV_EINVAL, "%d circles: the allocation-free mask builder takes at most %d", nc,
+                    WV_MASK_SCRATCH_MAX_CIRCLES);
+    const Grid &G = e->g[0];
+    const int nx = G.p.nx, ny = G.p.ny;
+    // scratch layout: circle list (fixed room), then the row band
+    const size_t circ_room = sizeof(double) * 4 * (size_t)WV_MASK_SCRATCH_MAX_CIRCLES;
+    const size_t band_room = (size_t)WV_MASK_SCRATCH_BYTES - circ_room;
+    const size_t row_bytes = sizeof(short) * (size_t)ny;
+    if (row_bytes > band_room)
+        return fail(e, WV_EINVAL, "NY = %d: the allocation-free mask builder takes rows of at most %zu cells", ny,
+                    band_room / sizeof(short));
+    const int band = (int)std::min<size_t>((size_t)nx, band_room / row_bytes);
+    HIPCHK(e, hipSetDevice(e->device));
+    void *base = nullptr;
+    const int se = wv_mask_scratch(e->device, &base);
+    if (se) return fail(e, WV_EHIP, "mask scratch: %s", hipGetErrorString((hipError_t)se));
+    double *d_c = (double *)base;
+    short *d_band = (short *)((unsigned char *)base + circ_room);
+    hipError_t err = hipSuccess;
+    if (nc > 0) err = hipMemcpyAsync(d_c, circ, sizeof(double) * 4 * (size_t)nc, hipMemcpyHostToDevice, e->stream);
+    for (int i0 = 0; i0 < nx && err == hipSuccess; i0 += band) {
+        WvMask a{};
+        a.nx = nx; a.ny = ny; a.kind = kind; a.ncirc = nc; a.i0 = i0; a.rows = std::min(band, nx - i0);
+        a.xmin = box[0]; a.xmax = box[1]; a.ymin = box[2]; a.ymax = box[3];
+        a.lambda = lambda; a.mu = mu; a.circ = d_c; a.out = d_band;
+        err = (hipError_t)wv_launch_mask(a, e->stream);
+        // (the next band's kernel follows this copy on the same stream)
+        if (err == hipSuccess)
+            err = hipMemcpyAsync(xy + (size_t)i0 * ny, d_band, row_bytes * a.rows, hipMemcpyDeviceToHost, e->stream);
+    }
+    if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+    if (err != hipSuccess) return fail(e, WV_EHIP, "mask: %s", hipGetErrorString(err));
+    return WV_OK;
+}
+
+extern "C" int wv_build_mask_static(wv_ctx *c, int domain, const double box[4], double lambda, double mu,
+                                    int ncircles, const double *circles, short *xy_in)
+{
+    if (!c) return WV_EINVAL;
+    return build_mask_static(c->e, domain, box, lambda, mu, ncircles, circles, xy_in);
 }
 
 // ---- TIFF frames (writetiff, sub_wave.c:62-123) ------------------------------

@@ -171,13 +171,20 @@ int wv_launch_wells(const WvWells &a, hipStream_t s);
 // Billiard masks (wv_media.hip): xy_in of every cell of an NX x NY grid over
 // (xmin, xmax, ymin, ymax), flat out[i*NY + j]; kind 0 D_SINAI, 1
 // D_EXT_ELLIPSE, 2 D_CIRCLES with circ = ncirc each of xc, yc, radius, active
+// Rows [i0, i0 + rows) only (the whole grid: i0 = 0, rows = nx), into
+// out[(i - i0)*NY + j].
 struct WvMask {
-    int nx, ny, kind, ncirc;
+    int nx, ny, kind, ncirc, i0, rows;
     double xmin, xmax, ymin, ymax, lambda, mu;
     const double *circ;
     short *out;
 };
 int wv_launch_mask(const WvMask &a, hipStream_t s);
+// The mask builder's fixed device scratch (wv_build_mask_static): its
+// address on the current device (resolved once per device) and its size.
+#define WV_MASK_SCRATCH_BYTES (8u << 20)
+#define WV_MASK_SCRATCH_MAX_CIRCLES 32768
+int wv_mask_scratch(int device, void **ptr);
 
 // TIFF frames: LZW strips of n bytes (3*width), one per row, each into
 // strips + r*stride (stride = wv_lzw_stride(n), a bound on one coded strip),

@@ -6,6 +6,8 @@
 // the reference on one host thread before the first frame.  Here one thread
 // per cell walks the well list (wave-uniform loads, summed in the
 // reference's order) in FP64; exp and tanh are the device library's.
+#include <mutex>
+
 #include "wv_device.h"
 
 using namespace wv;
@@ -50,8 +52,8 @@ __global__ void __launch_bounds__(256) k_wells(WvWells a)
 // read with wave-uniform loads.
 __global__ void __launch_bounds__(256) k_mask(WvMask a)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
-    if (j >= a.ny || i >= a.nx) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = a.i0 + (int)blockIdx.y;
+    if (j >= a.ny || i >= a.nx || i >= a.i0 + a.rows) return;
     const double x = a.xmin + ((double)i) * (a.xmax - a.xmin) / ((double)a.nx);
     const double y = a.ymin + ((double)j) * (a.ymax - a.ymin) / ((double)a.ny);
     short v = 1;
@@ -71,14 +73,43 @@ __global__ void __launch_bounds__(256) k_mask(WvMask a)
             }
         }
     }
-    gm(a.out)[(size_t)i * a.ny + j] = v;
+    gm(a.out)[(size_t)(i - a.i0) * a.ny + j] = v;
 }
+
+// Fixed device scratch of the allocation-free mask builder: the circle list
+// (at most WV_MASK_SCRATCH_MAX_CIRCLES x 4 doubles) and then bands of mask
+// rows.  One copy per device, alive until the process ends; every call on a
+// device shares it.
+static __device__ __align__(256) unsigned char scratch[WV_MASK_SCRATCH_BYTES];
 
 }  // namespace
 
+// device address of the scratch array, looked up once per device with that
+// device current (the array's name is a host-side stand-in, not an address)
+int wv_mask_scratch(int device, void **ptr)
+{
+    constexpr int MAXDEV = 64;
+    static void *addr[MAXDEV];
+    static std::mutex mu;
+    if (device < 0 || device >= MAXDEV) return (int)hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        int cur = 0;
+        hipError_t err = hipGetDevice(&cur);
+        if (err == hipSuccess && cur != device) err = hipSetDevice(device);
+        void *p = nullptr;
+        if (err == hipSuccess) err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (err != hipSuccess) return (int)err;
+        addr[device] = p;
+    }
+    *ptr = addr[device];
+    return 0;
+}
+
 int wv_launch_mask(const WvMask &a, hipStream_t s)
 {
-    const dim3 block(256), grid((a.ny + 255) / 256, a.nx);
+    const dim3 block(256), grid((a.ny + 255) / 256, a.rows);
     hipLaunchKernelGGL(k_mask, grid, block, 0, s, a);
     return (int)hipGetLastError();
 }

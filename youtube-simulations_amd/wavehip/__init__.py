@@ -32,7 +32,7 @@ EXPORTS = [
     "wv_params_default", "wv_create", "wv_destroy", "wv_last_error", "wv_set_tables",
     "wv_set_state", "wv_set_state3", "wv_get_state", "wv_get_state3", "wv_set_sources",
     "wv_set_neumann", "wv_set_drive", "wv_set_floor", "wv_set_laplacian", "wv_set_probes", "wv_get_probes",
-    "wv_set_wells_tables", "wv_build_mask", "wv_tiff_bound", "wv_tiff_encode", "wv_tiff_write",
+    "wv_set_wells_tables", "wv_build_mask", "wv_build_mask_static", "wv_tiff_bound", "wv_tiff_encode", "wv_tiff_write",
     "wv_step", "wv_prepare", "wv_step_timed", "wv_sync", "wv_steps_done", "wv_last_kernel_ms", "wv_energy",
     "wv_get_tiling", "wv_frame_params_default", "wv_frame_set_tables", "wv_frame_reset", "wv_frame",
     "wv_frame_rgba8",
@@ -113,6 +113,8 @@ def load():
                                                ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _D, _D, _D]),
         "wv_build_mask": (ctypes.c_int, [vp, ctypes.c_int, _D, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                          _D, _S]),
+        "wv_build_mask_static": (ctypes.c_int, [vp, ctypes.c_int, _D, ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_int, _D, _S]),
         "wv_tiff_bound": (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
         "wv_tiff_encode": (ctypes.c_int, [vp, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
                                           ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
@@ -370,10 +372,11 @@ class Wave:
                                               courantb, gamma, _d(bx), *outs), "wv_set_wells_tables")
         return tuple(t) if copy else None
 
-    def build_mask(self, domain, box, lam, mu=0.0, circles=None):
+    def build_mask(self, domain, box, lam, mu=0.0, circles=None, static=False):
         """init_wave_flat's xy_in on the GPU (D_SINAI 3, D_EXT_ELLIPSE 199,
         D_CIRCLES 20 with circles = (4, n): xc, yc, radius, active); returns
-        int16 (nx, ny)"""
+        int16 (nx, ny).  static: wv_build_mask_static (no device allocation;
+        the library's fixed scratch array)"""
         c = np.zeros((4, 1)) if circles is None else np.ascontiguousarray(circles, np.float64)
         if c.ndim != 2 or c.shape[0] != 4:
             raise ValueError("circles must be (4, n): xc, yc, radius, active")
@@ -382,8 +385,9 @@ class Wave:
         if bx.size != 4:
             raise ValueError("box must hold XMIN, XMAX, YMIN, YMAX")
         out = np.empty(self.shape, np.int16)
-        self._chk(self._L.wv_build_mask(self._h, int(domain), _d(bx), lam, mu, n, _d(c), out.ctypes.data_as(_S)),
-                  "wv_build_mask")
+        f = self._L.wv_build_mask_static if static else self._L.wv_build_mask
+        self._chk(f(self._h, int(domain), _d(bx), lam, mu, n, _d(c), out.ctypes.data_as(_S)),
+                  "wv_build_mask_static" if static else "wv_build_mask")
         return out
 
     def tiff(self, rgb, description="Wave equation in a planar domain", compression=TIFF_LZW):
