@@ -115,6 +115,28 @@ int wv_set_wells_tables(wv_ctx *c, const short *xy_in, int nwells, const double 
                         const double *height, double sigma, int use_tanh, double courant, double courantb,
                         double gamma, const double box[4], double *tcc, double *tc, double *tgamma);
 
+/* The initial drop of init_circular_wave (wave_common.c:89-111) and the
+ * pulses of add_circular_wave (wave_common.c:337-351), computed on the GPU in
+ * the context's state, so a program that keeps its field on the GPU adds the
+ * frame loop's pulses (wave_billiard.c:916-934) without a round trip.  At the
+ * cell corner (x, y) of ij_to_xy (sub_wave.c:469-476) over box = (XMIN,
+ * XMAX, YMIN, YMAX), d2 = (x - x0)^2 + (y - y0)^2, on the cells the drawing
+ * rule reaches (xy_in != 0 of the installed mask, or TWOSPEEDS):
+ *   init: phi = amp*exp(-d2/variance)*cos(-sqrt(d2)/wavelength), 0 on the
+ *         other cells, psi = 0 (a state, as wv_set_state would install it);
+ *   add:  phi += amp*factor*exp(-d2/variance)*cos(-sqrt(d2)/wavelength),
+ *         psi unchanged (a kick of the velocity).
+ * amp, variance, wavelength are INITIAL_AMP, INITIAL_VARIANCE,
+ * INITIAL_WAVELENGTH.  The mask must be installed (wv_set_tables, unlike the
+ * reference's init, which also computes xy_in: wv_build_mask does that).
+ * exp, cos and sqrt are the GPU's (within an ulp or two of glibc's).  Split
+ * form: applied to u = hi + lo in binary64 and to the velocity.  Asynchronous
+ * on the context stream. */
+int wv_init_circular_wave(wv_ctx *c, double x0, double y0, double amp, double variance, double wavelength,
+                          const double box[4]);
+int wv_add_circular_wave(wv_ctx *c, double factor, double x0, double y0, double amp, double variance,
+                         double wavelength, const double box[4]);
+
 /* Domain codes of xy_in_billiard_single_domain (global_pdes.c:18-50) that
  * wv_build_mask evaluates */
 #define WV_D_SINAI        3
@@ -362,6 +384,11 @@ int wv_batch_last_kernel_ms(const wv_batch *b, double *avg_ms, double *timed_lau
 int wv_batch_energy(wv_batch *b, double *e /* n values */);
 int wv_batch_set_tiling(wv_batch *b, int tb_depth, int rows_per_tile);
 int wv_batch_get_tiling(const wv_batch *b, int *tb_depth, int *rows_per_tile, int *tile_cols);
+/* wv_init_circular_wave / wv_add_circular_wave on grid k of a batch */
+int wv_batch_init_circular_wave(wv_batch *b, int k, double x0, double y0, double amp, double variance,
+                                double wavelength, const double box[4]);
+int wv_batch_add_circular_wave(wv_batch *b, int k, double factor, double x0, double y0, double amp,
+                               double variance, double wavelength, const double box[4]);
 /* wv_autotune over all grids of the batch (one tiling for every launch) */
 int wv_batch_autotune(wv_batch *b, long steps_per_trial, double *best_gcells);
 

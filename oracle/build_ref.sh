@@ -10,7 +10,8 @@
 # with the colour tables it includes, wave_common.c:453-512 and 1053-1171; for
 # the media tables sub_wave.c:270-304, 332-357, 469-476, 721-1387,
 # 12164-13004; for the masks sub_wave.c:270-304, 321-330, 332-357, 362-411,
-# 469-476, 721-1387, 5299-5486, 5682-7602), then oracle/ref_harness.c.  Flags follow makefile:1-4
+# 469-476, 721-1387, 5299-5486, 5682-7602; for the initial drop and pulses the
+# same plus wave_common.c:89-111 and 337-351), then oracle/ref_harness.c.  Flags follow makefile:1-4
 # (gcc -O3 -fopenmp) plus -ffp-contract=off.  Needs the reference sources at
 # /root/reference; machines without them use the oracle alone.
 set -euo pipefail
@@ -20,7 +21,7 @@ OUT="$HERE/_ref"
 [ -f "$REF/wave_billiard.c" ] || { echo "reference not found at $REF" >&2; exit 1; }
 mkdir -p "$OUT"
 
-build_one() {  # build_one <tag> <0 row pointers | 1 flat | 2 neighbour table | 3 colours | 4 media | 5 masks> <gcc -D flags...>
+build_one() {  # build_one <tag> <0 row pointers | 1 flat | 2 neighbour table | 3 colours | 4 media | 5 masks | 6 pulses> <gcc -D flags...>
     local tag=$1 flat=$2; shift 2
     local so="$OUT/libref_${tag}.so"
     {
@@ -28,7 +29,12 @@ build_one() {  # build_one <tag> <0 row pointers | 1 flat | 2 neighbour table | 
         echo "#include \"$REF/global_pdes.c\""
         sed -n '412,422p' "$REF/sub_wave.c"
         sed -n '11965,12162p' "$REF/sub_wave.c"
-        if [ "$flat" = 5 ]; then
+        if [ "$flat" = 6 ]; then
+            echo "#define PULSE_TWIN 1"
+            for r in 270,304 321,330 332,357 362,411 469,476 721,1387 5299,5486 5682,7602; do sed -n "${r}p" "$REF/sub_wave.c"; done
+            sed -n '89,111p' "$REF/wave_common.c"
+            sed -n '337,351p' "$REF/wave_common.c"
+        elif [ "$flat" = 5 ]; then
             echo "#define MASK_TWIN 1"
             for r in 270,304 321,330 332,357 362,411 469,476 721,1387 5299,5486 5682,7602; do sed -n "${r}p" "$REF/sub_wave.c"; done
         elif [ "$flat" = 4 ]; then
@@ -159,6 +165,14 @@ for p in "${pids[@]}"; do wait "$p"; done; pids=()
 # the precision workload's grid (4096 x 4096, absorbing edges, courant 0.1, no
 # damping): the reference stepper timed as the CPU baseline of the benchmark
 build_one "row_4096x4096_bc2_ts0" 0 -DNX=4096 -DNY=4096 -DB_COND=2 -DTWOSPEEDS=0 &
+pids+=($!)
+# the initial drop and the added pulses of init_circular_wave / add_circular_wave
+# (wave_common.c:89-111, 337-351) on a Sinai billiard, one speed and two
+build_one "pulse_97x61_sinai_ts0" 6 -DNX=97 -DNY=61 -DB_DOMAIN=3 -DLAMBDA=0.3 -DINITIAL_VARIANCE=0.05 \
+    -DINITIAL_WAVELENGTH=0.1 -DINITIAL_AMP=0.75 &
+pids+=($!)
+build_one "pulse_97x61_sinai_ts1" 6 -DNX=97 -DNY=61 -DB_DOMAIN=3 -DLAMBDA=0.3 -DINITIAL_VARIANCE=0.05 \
+    -DINITIAL_WAVELENGTH=0.1 -DINITIAL_AMP=0.75 -DTWOSPEEDS=1 &
 pids+=($!)
 # a lens medium with damping and absorbing edges (energy fixture, two speeds)
 build_one "row_64x48_bc2_ts1" 0 -DNX=64 -DNY=48 -DB_COND=2 -DTWOSPEEDS=1 -DCOURANTB=0.0666666666666666667 $KS

@@ -75,3 +75,31 @@ int orc_mask(int NX, int NY, const double box[4], int domain, double lambda, dou
         }
     return 0;
 }
+
+/* init_circular_wave (wave_common.c:89-111; mode 0: phi set, psi = 0, with
+ * the mask xy given instead of computed) and add_circular_wave (337-351;
+ * mode 1: phi += ..., psi untouched), at the cell corners of ij_to_xy
+ * (sub_wave.c:469-476) over box, on cells with xy != 0 or TWOSPEEDS */
+void orc_circular_wave(const orc_params *p, int mode, double factor, double x0, double y0, double amp,
+                       double variance, double wavelength, const double box[4], double *phi, double *psi,
+                       const short *xy)
+{
+    const int NX = p->nx, NY = p->ny;
+    const double xmin = box[0], xmax = box[1], ymin = box[2], ymax = box[3];
+    int i, j;
+#pragma omp parallel for private(i, j)
+    for (i = 0; i < NX; i++)
+        for (j = 0; j < NY; j++) {
+            const double x = xmin + ((double)i) * (xmax - xmin) / ((double)NX);
+            const double y = ymin + ((double)j) * (ymax - ymin) / ((double)NY);
+            const double dist2 = (x - x0) * (x - x0) + (y - y0) * (y - y0);
+            const size_t k = (size_t)i * NY + j;
+            const int drawn = xy[k] != 0 || p->twospeeds;
+            if (mode == 0) {
+                phi[k] = drawn ? amp * exp(-dist2 / variance) * cos(-sqrt(dist2) / wavelength) : 0.0;
+                psi[k] = 0.0;
+            } else if (drawn) {
+                phi[k] += amp * factor * exp(-dist2 / variance) * cos(-sqrt(dist2) / wavelength);
+            }
+        }
+}

@@ -328,6 +328,21 @@ def wells_tables(p, xy_in, xc, yc, h, sigma, use_tanh, courant, courantb, gamma,
     return tcc, tc, tg
 
 
+def circular_wave(p, mode, factor, x0, y0, amp, variance, wavelength, box, phi, psi, xy_in):
+    """init_circular_wave (mode 0) / add_circular_wave (mode 1) on copies of
+    phi, psi; returns (phi, psi)"""
+    L = lib()
+    L.orc_circular_wave.argtypes = [ctypes.POINTER(OrcParams), ctypes.c_int] + [ctypes.c_double] * 6 + \
+        [_D, _D, _D, _S]
+    a = np.array(phi, dtype=np.float64, copy=True)
+    b = np.array(psi, dtype=np.float64, copy=True)
+    bx = np.ascontiguousarray(box, np.float64)
+    xy = np.ascontiguousarray(xy_in, np.int16)
+    L.orc_circular_wave(ctypes.byref(p), mode, factor, x0, y0, amp, variance, wavelength, _p(bx, _D), _p(a, _D),
+                        _p(b, _D), _p(xy, _S))
+    return a, b
+
+
 def circle_list(xc, yc, r, active=None):
     """the (4, n) circle array of the mask builders: xc, yc, radius, active"""
     xc = np.asarray(xc, np.float64)
@@ -451,6 +466,10 @@ class RefBuild:
         self.ny = self.L.ref_ny()
         self.flat = hasattr(self.L, "ref_evolve_wave_half_flat")
         self.table = hasattr(self.L, "ref_evolve_wave_half_new")
+        if hasattr(self.L, "ref_init_circular_wave"):
+            self.L.ref_init_circular_wave.argtypes = [ctypes.c_double, ctypes.c_double, _D, _D, _S]
+            self.L.ref_add_circular_wave.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _D, _S]
+            return
         if hasattr(self.L, "ref_mask"):
             self.L.ref_mask.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D, _I, _S]
             self.L.ref_mask.restype = ctypes.c_int
@@ -509,6 +528,20 @@ class RefBuild:
         t = [np.zeros((self.nx, self.ny)) for _ in range(3)]
         self.L.ref_init_ior(_p(xy, _S), _p(t[0], _D), _p(t[1], _D), _p(t[2], _D), seed)
         return tuple(t)
+
+    def init_circular_wave(self, x, y):
+        """init_circular_wave(x, y, phi, psi, xy_in): (phi, psi, xy_in)"""
+        phi, psi = np.zeros((self.nx, self.ny)), np.zeros((self.nx, self.ny))
+        xy = np.zeros((self.nx, self.ny), np.int16)
+        self.L.ref_init_circular_wave(x, y, _p(phi, _D), _p(psi, _D), _p(xy, _S))
+        return phi, psi, xy
+
+    def add_circular_wave(self, factor, x, y, phi, psi, xy):
+        """add_circular_wave(factor, x, y, phi, psi, xy_in) on copies"""
+        a, b = np.array(phi, copy=True), np.array(psi, copy=True)
+        xy = np.ascontiguousarray(xy, np.int16)
+        self.L.ref_add_circular_wave(factor, x, y, _p(a, _D), _p(b, _D), _p(xy, _S))
+        return a, b
 
     def mask(self, domain, circles=None):
         """xy_in_billiard_single_domain over the build's grid (its LAMBDA, MU,

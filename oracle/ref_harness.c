@@ -75,6 +75,27 @@ double ref_wells(unsigned seed, double *xc, double *yc, double *h)
         }
     return 0.2 * dx * dx; /* sigma */
 }
+#elif defined(PULSE_TWIN)
+/* init_circular_wave (wave_common.c:89) and add_circular_wave (337) on flat
+ * arrays through row pointers; init writes the mask xy_in_billiard gives */
+void ref_init_circular_wave(double x, double y, double *phi, double *psi, short *xy)
+{
+    double *a[NX], *b[NX];
+    short int *m[NX];
+    for (int i = 0; i < NX; i++) {
+        a[i] = phi + (size_t)i * NY; b[i] = psi + (size_t)i * NY; m[i] = xy + (size_t)i * NY;
+    }
+    init_circular_wave(x, y, a, b, m);
+}
+void ref_add_circular_wave(double factor, double x, double y, double *phi, double *psi, short *xy)
+{
+    double *a[NX], *b[NX];
+    short int *m[NX];
+    for (int i = 0; i < NX; i++) {
+        a[i] = phi + (size_t)i * NY; b[i] = psi + (size_t)i * NY; m[i] = xy + (size_t)i * NY;
+    }
+    add_circular_wave(factor, x, y, a, b, m);
+}
 #elif defined(MASK_TWIN)
 /* xy_in_billiard_single_domain (sub_wave.c:5712) at the cell corners
  * ij_to_xy gives (sub_wave.c:469-476), flat out[i*NY + j], with the circle

@@ -75,7 +75,15 @@
 #endif
 #define ACHIRP 0.25
 #define DAMPING 0.0
+#ifndef INITIAL_VARIANCE
 #define INITIAL_VARIANCE 0.0003
+#endif
+#ifndef INITIAL_AMP
+#define INITIAL_AMP 1.0
+#endif
+#ifndef INITIAL_WAVELENGTH
+#define INITIAL_WAVELENGTH 0.01
+#endif
 #ifndef BC_NEUMANN
 #define BC_NEUMANN 0
 #endif
@@ -184,7 +192,9 @@
 #endif
 /* domain parameters xy_in_billiard_single_domain reads (sub_wave.c:5712),
  * at the values of the checked-in wave_billiard.c:73-109 */
+#ifndef B_DOMAIN
 #define B_DOMAIN 20
+#endif
 #define B_DOMAIN_B 20
 #define MU_B 0.08
 #define JULIA_SCALE 0.8

@@ -200,6 +200,14 @@ void orc_color(const orc_frame_params *f, int scheme, int asym, double value, do
  *   s = tanh(s) if use_tanh, tcc = courant*s + courantb*(1 - s), tgamma = gamma,
  * then tc = sqrt(tcc) where xy_in != 0, courantb where TWOSPEEDS and
  * xy_in == 0, untouched elsewhere (sub_wave.c:12970-12981). */
+/* init_circular_wave (wave_common.c:89-111) with mode 0: phi =
+ * amp*exp(-d2/variance)*cos(-sqrt(d2)/wavelength) where xy != 0 or
+ * TWOSPEEDS, 0 elsewhere, psi = 0; add_circular_wave (337-351) with mode 1:
+ * phi += amp*factor*exp(...)*cos(...) on the same cells, psi untouched.
+ * d2 from the cell corner of ij_to_xy over box to (x0, y0); glibc libm. */
+void orc_circular_wave(const orc_params *p, int mode, double factor, double x0, double y0, double amp,
+                       double variance, double wavelength, const double box[4], double *phi, double *psi,
+                       const short *xy);
 void orc_wells_tables(const orc_params *p, const short *xy, int nw, const double *xc, const double *yc,
                       const double *h, double sigma, int use_tanh, double courant, double courantb,
                       double gamma, const double box[4], double *tcc, double *tc, double *tg);

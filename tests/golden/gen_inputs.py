@@ -241,3 +241,10 @@ def tiff_rgb(tag, h, w, content):
     y = np.arange(h)[:, None] * 0.5
     v = np.floor(127.0 + 120.0 * np.sin(x + y)).astype(np.uint8)
     return np.ascontiguousarray(np.stack([v, 255 - v, v // 2], -1))
+
+
+# init_circular_wave / add_circular_wave fixtures (oracle/build_ref.sh builds
+# pulse_97x61_sinai_ts0/1): the centre of the initial drop and the (factor,
+# x, y) of the pulses added after it
+PULSE_INIT = (-0.5, 0.2)
+PULSES = ((1.0, 0.7, -0.4), (-1.3, -1.6, 0.9), (0.35, 0.05, 0.02))

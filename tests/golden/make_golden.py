@@ -27,6 +27,11 @@ Outputs (committed):
                     (D_SINAI, D_EXT_ELLIPSE, D_CIRCLES) for every
                     gen_inputs.mask_cases() build (python make_golden.py masks
                     writes this file alone)
+  pulse_cases.npz   init_circular_wave / add_circular_wave (wave_common.c:
+                    89-111, 337-351) on a Sinai billiard, one speed and two:
+                    the mask init writes, phi and psi after init and after
+                    each of PULSES (python make_golden.py pulses writes this
+                    file alone)
   table_cases.npz   the neighbour-table stepper (evolve_wave_half_new): the
                     reference's table (counts, flat indices), the L-shape
                     lines, drive values where driven, and phi, psi, tmp after
@@ -46,9 +51,9 @@ sys.path.insert(0, os.path.join(ROOT, "youtube-simulations_amd"))
 sys.path.insert(0, HERE)
 
 import pyoracle  # noqa: E402
-from gen_inputs import (COLOR_FRAMES, EXT_STEPS, TABLE_STEPS, case_inputs, cmp_cases, color_cases,  # noqa: E402
-                        color_inputs, ext_cases, ext_inputs, mask_cases, mask_circles, small_cases, table_cases,
-                        table_inputs)
+from gen_inputs import (COLOR_FRAMES, EXT_STEPS, PULSE_INIT, PULSES, TABLE_STEPS, case_inputs, cmp_cases,  # noqa: E402
+                        color_cases, color_inputs, ext_cases, ext_inputs, mask_cases, mask_circles, small_cases,
+                        table_cases, table_inputs)
 from wavehip import workloads  # noqa: E402
 
 NSTEPS_SMALL = 7
@@ -56,6 +61,20 @@ NSTEPS_SMALL = 7
 
 def sha(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def pulses():
+    out = {}
+    for ts in (0, 1):
+        R = pyoracle.RefBuild("pulse_97x61_sinai_ts%d" % ts)
+        phi, psi, xy = R.init_circular_wave(*PULSE_INIT)
+        pre = "ts%d/" % ts
+        out[pre + "xy_in"] = xy
+        out[pre + "phi0"], out[pre + "psi0"] = phi, psi
+        for n, (f, x, y) in enumerate(PULSES):
+            phi, psi = R.add_circular_wave(f, x, y, phi, psi, xy)
+            out[pre + "phi%d" % (n + 1)], out[pre + "psi%d" % (n + 1)] = phi, psi
+    np.savez_compressed(os.path.join(HERE, "pulse_cases.npz"), **out)
 
 
 def masks():
@@ -190,5 +209,7 @@ if __name__ == "__main__":
         masks()
     elif sys.argv[1:] == ["tiffs"]:
         tiffs()
+    elif sys.argv[1:] == ["pulses"]:
+        pulses()
     else:
         main()

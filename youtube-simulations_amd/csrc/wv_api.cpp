@@ -1617,6 +1617,73 @@ int set_wells_tables(Engine *e, int k, const short *xy, int nw, const double *xc
     return sync_inst(e);
 }
 
+// --------------------------------------------------- initial drop, pulses
+
+// is there a cell the drawing rule reaches (xy_in != 0 or TWOSPEEDS) that no
+// step writes?  A drop or pulse that lands there leaves a non-zero value on
+// a never-written cell, which the two-level fused launches would not keep
+bool drawn_unwritten(const Grid &G)
+{
+    const int nx = G.p.nx, ny = G.p.ny, ldw = G.ldw;
+    if (G.h_wr.empty()) return false;
+    for (int i = 0; i < nx; i++)
+        for (int w = 0; w < ldw; w++) {
+            const int nb = std::min(64, ny - 64 * w);
+            if (nb <= 0) break;
+            const uint64_t valid = nb == 64 ? ~0ull : ((1ull << nb) - 1);
+            const uint64_t drawn = G.p.twospeeds ? valid : (G.h_nz[(size_t)i * ldw + w] & valid);
+            if (drawn & ~G.h_wr[(size_t)i * ldw + w]) return true;
+        }
+    return false;
+}
+
+// init_circular_wave (mode 0) / add_circular_wave (mode 1) on grid k
+int circular_wave(Engine *e, int k, int mode, double factor, double x0, double y0, double amp, double variance,
+                  double wavelength, const double *box)
+{
+    if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
+    if (!box || !(box[1] > box[0]) || !(box[3] > box[2])) return fail(e, WV_EINVAL, "box missing or empty");
+    if (!std::isfinite(factor) || !std::isfinite(x0) || !std::isfinite(y0) || !std::isfinite(amp) ||
+        !(variance > 0) || !std::isfinite(variance) || !(wavelength != 0) || !std::isfinite(wavelength))
+        return fail(e, WV_EINVAL, "drop parameters must be finite, with variance > 0 and wavelength != 0");
+    Grid &G = e->g[k];
+    if (!G.tables) return fail(e, WV_ESTATE, "set the tables (the mask) before a drop");
+    if (mode == 1 && !G.state) return fail(e, WV_ESTATE, "no state to add a pulse to");
+    HIPCHK(e, hipSetDevice(e->device));
+    WvPulse a{};
+    a.nx = G.p.nx; a.ny = G.p.ny; a.ldy = G.ldy; a.ldw = G.ldw; a.mode = mode; a.twospeeds = G.p.twospeeds != 0;
+    a.xmin = box[0]; a.xmax = box[1]; a.ymin = box[2]; a.ymax = box[3];
+    a.x0 = x0; a.y0 = y0; a.amp = amp; a.factor = factor; a.variance = variance; a.wavelength = wavelength;
+    a.nz = G.nz;
+    const size_t bytes = (size_t)G.p.nx * G.ldy * sizeof(double);
+    if (e->form == WV_FORM_FP64) {
+        a.phi = G.u[e->role[0]];
+        a.psi = G.u[e->role[1]];
+        if (mode == 0) {
+            // the other two buffers start at zero, as wv_set_state leaves them
+            HIPCHK(e, hipMemsetAsync(G.u[e->role[2]], 0, bytes, e->stream));
+            HIPCHK(e, hipMemsetAsync(G.u[e->role[3]], 0, bytes, e->stream));
+        }
+    } else {
+        for (int s = 0; s < 2; s++) { a.hi[s] = G.hi[s]; a.lo[s] = G.lo[s]; a.v[s] = G.v[s]; }
+    }
+    const int lr = wv_launch_pulse(a, e->stream);
+    if (lr) return fail(e, WV_EHIP, "kernel launch failed: %s", hipGetErrorString((hipError_t)lr));
+    if (e->form == WV_FORM_FP64) {
+        // finite values, no -0 (x + t is -0 only if both are): the state stays
+        // clean; values on never-written cells need one step per launch
+        const bool ex = drawn_unwritten(G);
+        G.exact = mode == 0 ? ex : (G.exact || ex);
+        if (mode == 0 && !G.clean_state) {
+            G.clean_state = true;
+            int rc = sync_inst(e);
+            if (rc) return rc;
+        }
+    }
+    G.state = true;
+    return WV_OK;
+}
+
 // ------------------------------------------------------------ frame colours
 
 int frame_set_tables(Engine *e, int k, const float *lut, int lut_n, const double *cs, const double *ft)
@@ -2424,6 +2491,34 @@ static int build_mask_static(Engine *e, int domain, const double *box, double la
     if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
     if (err != hipSuccess) return fail(e, WV_EHIP, "mask: %s", hipGetErrorString(err));
     return WV_OK;
+}
+
+extern "C" int wv_init_circular_wave(wv_ctx *c, double x, double y, double amp, double variance, double wavelength,
+                                     const double box[4])
+{
+    if (!c) return WV_EINVAL;
+    return circular_wave(c->e, 0, 0, 0.0, x, y, amp, variance, wavelength, box);
+}
+
+extern "C" int wv_add_circular_wave(wv_ctx *c, double factor, double x, double y, double amp, double variance,
+                                    double wavelength, const double box[4])
+{
+    if (!c) return WV_EINVAL;
+    return circular_wave(c->e, 0, 1, factor, x, y, amp, variance, wavelength, box);
+}
+
+extern "C" int wv_batch_init_circular_wave(wv_batch *b, int k, double x, double y, double amp, double variance,
+                                           double wavelength, const double box[4])
+{
+    if (!b) return WV_EINVAL;
+    return circular_wave(b->e, k, 0, 0.0, x, y, amp, variance, wavelength, box);
+}
+
+extern "C" int wv_batch_add_circular_wave(wv_batch *b, int k, double factor, double x, double y, double amp,
+                                          double variance, double wavelength, const double box[4])
+{
+    if (!b) return WV_EINVAL;
+    return circular_wave(b->e, k, 1, factor, x, y, amp, variance, wavelength, box);
 }
 
 extern "C" int wv_build_mask_static(wv_ctx *c, int domain, const double box[4], double lambda, double mu,

@@ -171,6 +171,19 @@ int wv_launch_wells(const WvWells &a, hipStream_t s);
 // Billiard masks (wv_media.hip): xy_in of every cell of an NX x NY grid over
 // (xmin, xmax, ymin, ymax), flat out[i*NY + j]; kind 0 D_SINAI, 1
 // D_EXT_ELLIPSE, 2 D_CIRCLES with circ = ncirc each of xc, yc, radius, active
+// init_circular_wave / add_circular_wave (wv_media.hip) on a grid's device
+// state: mode 0 sets phi = amp*exp(-d2/variance)*cos(-sqrt(d2)/wavelength)
+// on drawn cells (xy_in != 0: nz plane, or TWOSPEEDS), 0 elsewhere, and psi
+// = 0; mode 1 adds amp*factor*exp(...)*cos(...) to phi on drawn cells.
+// FP64 form: phi, psi (row stride ldy); split form: both sets of hi, lo, v.
+struct WvPulse {
+    int nx, ny, ldy, ldw, mode, twospeeds;
+    double xmin, xmax, ymin, ymax, x0, y0, amp, factor, variance, wavelength;
+    const uint64_t *nz;
+    double *phi, *psi;
+    float *hi[2], *lo[2], *v[2];
+};
+int wv_launch_pulse(const WvPulse &a, hipStream_t s);
 // Rows [i0, i0 + rows) only (the whole grid: i0 = 0, rows = nx), into
 // out[(i - i0)*NY + j].
 struct WvMask {

@@ -45,6 +45,50 @@ __global__ void __launch_bounds__(256) k_wells(WvWells a)
     }
 }
 
+// init_circular_wave (wave_common.c:89-111) and add_circular_wave (337-351)
+// on the device state, one thread per cell, the reference's expressions in
+// its order (exp, cos and sqrt are the device library's)
+__global__ void __launch_bounds__(256) k_pulse(WvPulse a)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (j >= a.ny || i >= a.nx) return;
+    // ij_to_xy, sub_wave.c:469-476
+    const double x = a.xmin + ((double)i) * (a.xmax - a.xmin) / ((double)a.nx);
+    const double y = a.ymin + ((double)j) * (a.ymax - a.ymin) / ((double)a.ny);
+    const double dist2 = (x - a.x0) * (x - a.x0) + (y - a.y0) * (y - a.y0);
+    const bool drawn = a.twospeeds || ((gc(a.nz)[(size_t)i * a.ldw + (j >> 6)] >> (j & 63)) & 1u);
+    const size_t k = (size_t)i * a.ldy + j;
+    double term = 0.0;
+    if (drawn) {
+        const double pre = a.mode == 0 ? a.amp : a.amp * a.factor;
+        term = pre * exp(-dist2 / a.variance) * cos(-sqrt(dist2) / a.wavelength);
+    }
+    if (a.phi) {
+        if (a.mode == 0) {
+            gm(a.phi)[k] = term;   // 0 on cells not drawn
+            gm(a.psi)[k] = 0.0;
+        } else if (drawn) {
+            gm(a.phi)[k] = gc(a.phi)[k] + term;
+        }
+        return;
+    }
+    // split form: u = hi + lo, v = u - psi; both sets
+    for (int s = 0; s < 2; s++) {
+        if (a.mode == 0) {
+            const float h = (float)term;
+            gm(a.hi[s])[k] = h;
+            gm(a.lo[s])[k] = (float)(term - (double)h);
+            gm(a.v[s])[k] = (float)term;
+        } else if (drawn) {
+            const double u = ((double)gc(a.hi[s])[k] + (double)gc(a.lo[s])[k]) + term;
+            const float h = (float)u;
+            gm(a.hi[s])[k] = h;
+            gm(a.lo[s])[k] = (float)(u - (double)h);
+            gm(a.v[s])[k] = (float)((double)gc(a.v[s])[k] + term);   // psi unchanged
+        }
+    }
+}
+
 // xy_in_billiard_single_domain (sub_wave.c:5712) at the cell corner of
 // ij_to_xy (sub_wave.c:469-476), for the domains of the plain workloads:
 // D_SINAI (5774-5779), D_EXT_ELLIPSE (5744-5749) and D_CIRCLES (6216-6227:
@@ -105,6 +149,13 @@ int wv_mask_scratch(int device, void **ptr)
     }
     *ptr = addr[device];
     return 0;
+}
+
+int wv_launch_pulse(const WvPulse &a, hipStream_t s)
+{
+    const dim3 block(256), grid((a.ny + 255) / 256, a.nx);
+    hipLaunchKernelGGL(k_pulse, grid, block, 0, s, a);
+    return (int)hipGetLastError();
 }
 
 int wv_launch_mask(const WvMask &a, hipStream_t s)

@@ -32,7 +32,8 @@ EXPORTS = [
     "wv_params_default", "wv_create", "wv_destroy", "wv_last_error", "wv_set_tables",
     "wv_set_state", "wv_set_state3", "wv_get_state", "wv_get_state3", "wv_set_sources",
     "wv_set_neumann", "wv_set_drive", "wv_set_floor", "wv_set_laplacian", "wv_set_probes", "wv_get_probes",
-    "wv_set_wells_tables", "wv_build_mask", "wv_build_mask_static", "wv_tiff_bound", "wv_tiff_encode", "wv_tiff_write",
+    "wv_set_wells_tables", "wv_build_mask", "wv_build_mask_static", "wv_init_circular_wave",
+    "wv_add_circular_wave", "wv_batch_init_circular_wave", "wv_batch_add_circular_wave", "wv_tiff_bound", "wv_tiff_encode", "wv_tiff_write",
     "wv_step", "wv_prepare", "wv_step_timed", "wv_sync", "wv_steps_done", "wv_last_kernel_ms", "wv_energy",
     "wv_get_tiling", "wv_frame_params_default", "wv_frame_set_tables", "wv_frame_reset", "wv_frame",
     "wv_frame_rgba8",
@@ -115,6 +116,10 @@ def load():
                                          _D, _S]),
         "wv_build_mask_static": (ctypes.c_int, [vp, ctypes.c_int, _D, ctypes.c_double, ctypes.c_double,
                                                 ctypes.c_int, _D, _S]),
+        "wv_init_circular_wave": (ctypes.c_int, [vp] + [ctypes.c_double] * 5 + [_D]),
+        "wv_add_circular_wave": (ctypes.c_int, [vp] + [ctypes.c_double] * 6 + [_D]),
+        "wv_batch_init_circular_wave": (ctypes.c_int, [vp, ctypes.c_int] + [ctypes.c_double] * 5 + [_D]),
+        "wv_batch_add_circular_wave": (ctypes.c_int, [vp, ctypes.c_int] + [ctypes.c_double] * 6 + [_D]),
         "wv_tiff_bound": (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
         "wv_tiff_encode": (ctypes.c_int, [vp, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
                                           ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
@@ -390,6 +395,22 @@ class Wave:
                   "wv_build_mask_static" if static else "wv_build_mask")
         return out
 
+    def init_circular_wave(self, x0, y0, amp, variance, wavelength, box):
+        """init_circular_wave (wave_common.c:89) on the GPU state"""
+        bx = np.ascontiguousarray(box, np.float64).ravel()
+        if bx.size != 4:
+            raise ValueError("box must hold XMIN, XMAX, YMIN, YMAX")
+        self._chk(self._L.wv_init_circular_wave(self._h, x0, y0, amp, variance, wavelength, _d(bx)),
+                  "wv_init_circular_wave")
+
+    def add_circular_wave(self, factor, x0, y0, amp, variance, wavelength, box):
+        """add_circular_wave (wave_common.c:337) on the GPU state"""
+        bx = np.ascontiguousarray(box, np.float64).ravel()
+        if bx.size != 4:
+            raise ValueError("box must hold XMIN, XMAX, YMIN, YMAX")
+        self._chk(self._L.wv_add_circular_wave(self._h, factor, x0, y0, amp, variance, wavelength, _d(bx)),
+                  "wv_add_circular_wave")
+
     def tiff(self, rgb, description="Wave equation in a planar domain", compression=TIFF_LZW):
         """the TIFF file (bytes) writetiff writes for the glReadPixels rows
         rgb, uint8 (height, width, 3) with the bottom row first"""
@@ -606,6 +627,16 @@ class Batch:
     def load(self, k, w):
         self.set_tables(k, w["xy_in"], w["tcc"], w["tc"], w["tgamma"])
         self.set_state(k, w["phi"], w["psi"])
+
+    def init_circular_wave(self, k, x0, y0, amp, variance, wavelength, box):
+        bx = np.ascontiguousarray(box, np.float64).ravel()
+        self._chk(self._L.wv_batch_init_circular_wave(self._h, k, x0, y0, amp, variance, wavelength, _d(bx)),
+                  "wv_batch_init_circular_wave")
+
+    def add_circular_wave(self, k, factor, x0, y0, amp, variance, wavelength, box):
+        bx = np.ascontiguousarray(box, np.float64).ravel()
+        self._chk(self._L.wv_batch_add_circular_wave(self._h, k, factor, x0, y0, amp, variance, wavelength,
+                                                     _d(bx)), "wv_batch_add_circular_wave")
 
     def step(self, n):
         self._chk(self._L.wv_batch_step(self._h, int(n)), "wv_batch_step")
