@@ -339,6 +339,12 @@ int wv_frame_rgba8(wv_ctx *c, const wv_frame_params *f, unsigned char *rgba);
 /* What wv_step launches: fused steps per launch, rows per tile, tile columns */
 int wv_get_tiling(const wv_ctx *c, int *tb_depth, int *rows_per_tile, int *tile_cols);
 int wv_set_tiling(wv_ctx *c, int tb_depth, int rows_per_tile);
+/* Tiles of one stepping launch of fused depth tb_depth (0: the current
+ * one): wide tiles (split form: a workgroup of waves sharing their column
+ * halos through LDS), one-wave tiles, workgroups launched, and the output
+ * cells the tiles cover (a cell two tiles write counts twice).  No
+ * counterpart in the reference (a tuning and test aid). */
+int wv_get_tile_counts(wv_ctx *c, int tb_depth, int *wide, int *narrow, int *groups, long *cells);
 /* Time candidate tilings on the device for the current state (state is
  * restored afterwards) and keep the fastest; *best_gcells is its throughput. */
 int wv_autotune(wv_ctx *c, long steps_per_trial, double *best_gcells);

@@ -85,7 +85,8 @@ struct Engine {
     int role[4] = {0, 1, 2, 3};  // FP64: phi, psi, free1 (tmp), free2
     int scur = 0;                // split: current set
     WvInst *d_inst = nullptr;
-    struct Tiles { WvTile *d = nullptr; int ni = 0, nb = 0; };  // interior tiles, then boundary tiles
+    // ni workgroups of tiles (WV_WG entries each in the split form, one in the FP64 form); nb unused (0)
+    struct Tiles { WvTile *d = nullptr; int ni = 0, nb = 0; };
     std::map<int, Tiles> tiles;  // depth -> device tile tables
     int K = 0, L = 0;            // tiling (0 = untuned default)
     long steps_done = 0, last_launches = 0;
@@ -104,6 +105,9 @@ struct Engine {
     bool dropin_tables = false;
     bool xcd_order = true;  // WAVEHIP_TILE_ORDER=linear turns the XCD interleave off
     bool slow_first = true;  // WAVEHIP_TILE_ORDER=inplace: slower tiles keep their place in the strip order
+    // split form: interior strips in wide tiles (WAVEHIP_WIDE=1); measured
+    // slower than one-wave tiles on every workload, so off by default
+    bool wide = false;
     // per-kernel timing of the last wv_step_timed: one event before the first
     // stepping launch and one after every launch (kev[q], kev[q+1] bracket
     // launch q; consecutive launches share an event)
@@ -113,7 +117,7 @@ struct Engine {
     size_t kev_used = 0;         // events recorded (the last one follows the step counter's bump)
     bool kev_pending = false;    // per-launch durations of the last timed run not yet read
     double last_avg_ms = 0, last_timed = 0;
-    // one tile of instance -1: a launch over it loads the kernel and returns
+    // one workgroup of tiles of instance -1: a launch over it loads the kernel and returns
     WvTile *d_pad = nullptr;
     // TIFF frames: device rows, coded strips, their lengths and offsets, the
     // packed strips (grown to the largest frame seen)
@@ -305,13 +309,18 @@ void drop_tiles(Engine *e)
 
 // Tile table of a launch of depth k: every instance, row strips (of rows(e)
 // rows where a tile can be interior), column blocks of wv_tile_cols(k)
-// columns.  Workgroups are dealt to the 8 XCDs round-robin by workgroup id,
-// so position 8j + x of the table is the j-th tile XCD x runs.  Each XCD gets
-// one contiguous run of strips in row order: the K halo rows a tile reads
-// above and below its strip were read shortly before by the neighbouring
-// strip on the same XCD, so they come from that XCD's L2 instead of HBM, and
-// so do the halo columns two neighbouring tiles of a strip share.  Tiles with
-// inst = -1 only pad the shorter runs.
+// columns for one-wave tiles; in the split form the inner columns of strips
+// that can be interior go to wide tiles (a workgroup of WV_WG waves sharing
+// their j-halos through LDS, wv_wide_cols(k) columns).  The table lists
+// workgroups: WV_WG tile entries each in the split form (a wide tile
+// repeated in every entry, or up to WV_WG one-wave tiles, padded with
+// inst = -1), one in the FP64 form.  Workgroups are dealt to the 8 XCDs
+// round-robin by workgroup id, so workgroup 8j + x of the table is the j-th
+// one XCD x runs.  Each XCD gets one contiguous run of strips in row order:
+// the K halo rows a tile reads above and below its strip were read shortly
+// before by the neighbouring strip on the same XCD, so they come from that
+// XCD's L2 instead of HBM, and so do the halo columns two neighbouring tiles
+// of a strip share.  Workgroups of inst = -1 only pad the shorter runs.
 int get_tiles(Engine *e, int k, Engine::Tiles *out)
 {
     auto it = e->tiles.find(k);
@@ -320,7 +329,10 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
         return WV_OK;
     }
     // (a tile's row count lives in the upper bits of its flags)
-    const int wout = wv_tile_cols(k), L = std::min(rows(e), 32767);
+    const int wout = wv_tile_cols(k), L = std::min(rows(e), 32767), H = wv_halo(k);
+    const int G = e->form == WV_FORM_F32SPLIT ? WV_WG : 1;  // tile entries per workgroup
+    const bool wide = e->form == WV_FORM_F32SPLIT && WV_C == 1 && e->wide;
+    const int ww = wv_wide_cols(k);
     // Strips of L rows, except where a tile cannot be interior for its
     // position: the first and last rows of a grid and the column blocks that
     // reach its outer columns get strips of at most 8 rows.  Those tiles run
@@ -340,21 +352,28 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
     // slowest tile: there the short strips have 4 rows (at 1280x720, K = 4:
     // +30 % over 8), but at least K so that the next strip can be interior.
     int Lb = std::min(L, WV_LB_MAX);
+    // the kernel variant of this launch has the edge-column path (K = 8,
+    // 5-point, uniform medium): tiles of the outer columns take it, and so do
+    // the tiles of the first and last rows of grids not periodic along i
+    bool ej_on = k == 8 && e->stencil == 5 && WV_C == 1;
+    for (int q = 0; q < e->n; q++) ej_on = ej_on && e->g[q].coef == WV_COEF_UNIFORM;
     double tot = 0;  // launch work in interior-row units
+    long ntiles = 0;  // one-wave tiles at strips of L rows
     {
-        long ntiles = 0;
+        // wide tiles cover an inner column with 64 WV_WG / ww lanes instead of 64 / wout
+        const double fw = wide && ww > 0 ? (double)(64 * WV_WG) / ww / (64.0 / wout) : 1.0;
         for (int q = 0; q < e->n; q++) {
             const Grid &G = e->g[q];
             const int ncb = (G.p.ny + wout - 1) / wout, nsb = (G.p.nx + L - 1) / L;
             ntiles += (long)ncb * nsb;
             int ob = 0;
             for (int cb = 0; cb < ncb; cb++) {
-                const int js = cb * wout - wv_halo(k);
+                const int js = cb * wout - H;
                 ob += !(js >= 1 && js + 64 * WV_C <= G.p.ny - 1);
             }
             const double nb = (double)ob * nsb + (double)(ncb - ob) * std::min(nsb, 2);
             const double ni = (double)(ncb - ob) * std::max(nsb - 2, 0);
-            tot += (double)(L + 2 * k) * (ni + 8.0 * nb);
+            tot += (double)(L + 2 * k) * (ni * fw + 8.0 * nb);
         }
         if (ntiles < 4096) Lb = std::min(L, std::max(k, std::min(4, WV_LB_MAX)));
         // on grids large enough, longer boundary strips (fewer halo rows)
@@ -376,23 +395,22 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
     // WAVEHIP_EDGE_ROWS fixes the length (tuning).
     int Lj = Lb;
     {
-        bool edgej = k == 8 && e->stencil == 5 && WV_C == 1;
-        for (int q = 0; q < e->n; q++) edgej = edgej && e->g[q].coef == WV_COEF_UNIFORM;
-        if (edgej && Lb < L)
+        if (ej_on && Lb < L)
             for (int c : {16, 24, 32, 48, 64, 96, 128})
                 if (c <= L && 3.5 * (c + 2 * k) <= tot / 4096.0) Lj = c;
     }
     if (const char *v = getenv("WAVEHIP_EDGE_ROWS")) Lj = std::max(1, std::min(L, atoi(v)));
-    // flags: INTERIOR when every row and column the tile reads lies inside the
-    // outer box (no edge formula, no wrapping); ALLACT when, in addition,
-    // every one of those cells is written by a step
-    auto flags_of = [&](int q, int cb, int ibeg, int iend) {
+    // flags of a tile reading columns [jlo, jhi) (wrapped where the boundary
+    // wraps) for output rows [ibeg, iend): INTERIOR when every row and column
+    // the tile reads lies inside the outer box (no edge formula, no
+    // wrapping); ALLACT when, in addition, every one of those cells is written
+    // by a step
+    auto flags_of = [&](int q, int jlo, int jhi, int ibeg, int iend) {
         const Grid &G = e->g[q];
         const int nx = G.p.nx, ny = G.p.ny;
-        const int js = cb * wout - wv_halo(k);
         // (BC_LSHAPE: every bulk cell sums its Laplacian in table order)
         // (COMPARISON: neighbours across the grid, one step per launch)
-        const bool interior = js >= 1 && js + 64 * WV_C <= ny - 1 && ibeg >= k && iend <= nx - k && G.tbl != 2 &&
+        const bool interior = jlo >= 1 && jhi <= ny - 1 && ibeg >= k && iend <= nx - k && G.tbl != 2 &&
                               !(G.p.flags & WV_FLAG_COMPARISON);
         // boundary tiles: the point lists whose cells lie in the rows and
         // columns the tile computes, wrapped where the boundary wraps (rows
@@ -407,7 +425,7 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
         };
         auto any_hit = [&](const std::vector<int> &pi, const std::vector<int> &pj, int rlo, int rhi) {
             for (size_t q = 0; q < pi.size(); q++)
-                if (in_range(pi[q], rlo, rhi, nx, wrapi) && in_range(pj[q], js, js + 64 * WV_C, ny, wrapj)) return true;
+                if (in_range(pi[q], rlo, rhi, nx, wrapi) && in_range(pj[q], jlo, jhi, ny, wrapj)) return true;
             return false;
         };
         const int bflags = (any_hit(G.h_si, G.h_sj, ibeg - k, iend + k) ? WV_TILE_SRC : 0) |
@@ -415,51 +433,87 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
                            (any_hit(G.h_di, G.h_dj, ibeg - k, iend + k) ? WV_TILE_DRV : 0) |
                            (any_hit(G.h_pi, G.h_pj, ibeg, iend) ? WV_TILE_PRB : 0);
         if (!interior) {
-            // EDGEJ: rows inside the box, columns reaching column 0 or NY-1, on a
-            // grid with no point lists and no FLOOR (the split kernel evaluates
-            // such a tile as interior and its outer columns by the edge formulas)
-            const bool edgej = WV_C == 1 && ibeg >= k && iend <= nx - k && G.tbl != 2 &&
-                               !(G.p.flags & WV_FLAG_COMPARISON) && bflags == 0 && !(G.vmax > 0);
+            // EDGEJ: rows inside the box (columns reaching column 0 or NY-1), or
+            // rows reaching row 0 or NX-1 of a grid not periodic along i, on a
+            // grid with no point lists and no FLOOR (the kernels with the edge-
+            // column path evaluate such a tile as interior, its outer columns
+            // and edge rows by the edge formulas)
+            const bool rows_ok = (ibeg >= k && iend <= nx - k) || G.p.bcond != 1;
+            const bool edgej = WV_C == 1 && rows_ok && G.tbl != 2 && !(G.p.flags & WV_FLAG_COMPARISON) &&
+                               bflags == 0 && !(G.vmax > 0);
             return edgej ? (int)WV_TILE_EDGEJ : bflags;
         }
         int xflags = 0;
         for (size_t q = 0; q < G.h_ni.size() && !(xflags & WV_TILE_NEU); q++)
-            if (G.h_ni[q] >= ibeg - k && G.h_ni[q] < iend + k && G.h_nj[q] >= js && G.h_nj[q] < js + 64 * WV_C)
+            if (G.h_ni[q] >= ibeg - k && G.h_ni[q] < iend + k && G.h_nj[q] >= jlo && G.h_nj[q] < jhi)
                 xflags |= WV_TILE_NEU;
         for (size_t q = 0; q < G.h_di.size() && !(xflags & WV_TILE_DRV); q++)
-            if (G.h_di[q] >= ibeg - k && G.h_di[q] < iend + k && G.h_dj[q] >= js && G.h_dj[q] < js + 64 * WV_C)
+            if (G.h_di[q] >= ibeg - k && G.h_di[q] < iend + k && G.h_dj[q] >= jlo && G.h_dj[q] < jhi)
                 xflags |= WV_TILE_DRV;
         for (size_t q = 0; q < G.h_pi.size() && !(xflags & WV_TILE_PRB); q++)
-            if (G.h_pi[q] >= ibeg && G.h_pi[q] < iend && G.h_pj[q] >= js && G.h_pj[q] < js + 64 * WV_C)
+            if (G.h_pi[q] >= ibeg && G.h_pi[q] < iend && G.h_pj[q] >= jlo && G.h_pj[q] < jhi)
                 xflags |= WV_TILE_PRB;
         // FLOOR and the two lists run on the boundary path of the kernels
         if (xflags || G.vmax > 0) return xflags | (G.h_si.empty() ? 0 : WV_TILE_SRC);
         bool all = !G.h_wr.empty();
         for (int i = ibeg - k; all && i < iend + k; i++)
-            for (int j = js; j < js + 64 * WV_C; j++)
+            for (int j = jlo; j < jhi; j++)
                 if (!((G.h_wr[(size_t)i * G.ldw + (j >> 6)] >> (j & 63)) & 1u)) { all = false; break; }
         // a tile that adds a point source runs on the boundary path, so the
         // interior variants carry no source code at all
         bool src = false;
         for (size_t q = 0; q < G.h_si.size() && !src; q++)
-            src = G.h_si[q] >= ibeg - k && G.h_si[q] < iend + k && G.h_sj[q] >= js && G.h_sj[q] < js + 64 * WV_C;
+            src = G.h_si[q] >= ibeg - k && G.h_si[q] < iend + k && G.h_sj[q] >= jlo && G.h_sj[q] < jhi;
         if (src) return (int)WV_TILE_SRC;
         return WV_TILE_INTERIOR | (all ? WV_TILE_ALLACT : 0);
     };
-    // every tile in grid order: by grid, first row, column block
+    // a one-wave tile with output columns [jb, jb + w) (w <= wout) reads
+    // [jb - H, jb - H + 64C)
+    auto narrow = [&](int q, int jb, int w, int i0, int i1) {
+        return WvTile{q, jb, i0,
+                      flags_of(q, jb - H, jb - H + 64 * WV_C, i0, i1) | (w << WV_TILE_WIDTH_SHIFT) |
+                          ((i1 - i0) << WV_TILE_ROWS_SHIFT)};
+    };
+    // every tile in grid order: by grid, first row, columns
     std::vector<WvTile> tl;
     for (int q = 0; q < e->n; q++) {
-        const Grid &G = e->g[q];
-        const int nx = G.p.nx, ny = G.p.ny;
+        const Grid &Gq = e->g[q];
+        const int nx = Gq.p.nx, ny = Gq.p.ny;
         const int ncb = (ny + wout - 1) / wout;
         // grids whose every tile is a boundary tile keep strips of L rows
-        const bool allb = G.tbl == 2 || (G.p.flags & WV_FLAG_COMPARISON) || G.vmax > 0;
+        const bool allb = Gq.tbl == 2 || (Gq.p.flags & WV_FLAG_COMPARISON) || Gq.vmax > 0;
         const int lb = allb ? L : Lb;
         // row cuts of the inner column blocks: lb rows, strips of L, lb rows
         std::vector<int> cut{0};
         const int head = std::min(lb, nx), tail = std::max(head, nx - lb);
         if (head > 0) cut.push_back(head);
-        for (int i = head + L; i < tail; i += L) cut.push_back(i);
+        {
+            // Strip lengths of the inner rows.  A launch of fewer than three
+            // tiles per resident wave (3,072 at three waves per SIMD) ends with
+            // the last strips to start, so strips of 64 rows and more come in a
+            // repeating mix of 2L, L/2, L/2 rows instead: the same redundant
+            // halo rows per row (2K per strip), and the longest-first order
+            // runs the long strips first and the short ones last (4096^2,
+            // K = 8: split 142 -> 127 us, FP64 160 -> 151 us per launch).  With
+            // more tiles the tail is short, and the mixed strips lose the L2
+            // reuse of the halo rows two neighbouring strips share (8192^2
+            // slower by a fifth).  WAVEHIP_STRIP_MIX sets the mix (a comma list
+            // of row counts; one count for strips of one length).
+            std::vector<int> mix{L};
+            if (L >= 64 && ntiles < 3 * 3072) mix = {2 * L, L / 2, L / 2};
+            if (const char *v = getenv("WAVEHIP_STRIP_MIX")) {
+                mix.clear();
+                for (const char *q = v; *q;) {
+                    const int r = atoi(q);
+                    if (r > 0) mix.push_back(std::min(r, 32767));
+                    while (*q && *q != ',') q++;
+                    if (*q == ',') q++;
+                }
+                if (mix.empty()) mix.push_back(L);
+            }
+            size_t m = 0;
+            for (int i = head + mix[0]; i < tail; i += mix[++m % mix.size()]) cut.push_back(i);
+        }
         if (tail > cut.back()) cut.push_back(tail);
         if (nx > cut.back()) cut.push_back(nx);
         // row cuts of the outer column blocks: lb rows at either end, strips
@@ -471,62 +525,113 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
         if (tail > cutb.back()) cutb.push_back(tail);
         if (nx > cutb.back()) cutb.push_back(nx);
         auto outer = [&](int cb) {
-            const int js = cb * wout - wv_halo(k);
+            const int js = cb * wout - H;
             return !(js >= 1 && js + 64 * WV_C <= ny - 1);
         };
+        // the inner column blocks form one run [cb_lo, cb_hi]: output columns [A, B)
+        int cb_lo = ncb, cb_hi = -1;
+        for (int cb = 0; cb < ncb; cb++)
+            if (!outer(cb)) cb_lo = std::min(cb_lo, cb), cb_hi = cb;
+        const int A = cb_lo * wout, B = std::min((cb_hi + 1) * wout, ny);
         // merge the two cut lists so tiles come out ordered by first row
         size_t a = 0, b = 0;
         while (a + 1 < cut.size() || b + 1 < cutb.size()) {
             const bool takeb = b + 1 < cutb.size() && (a + 1 >= cut.size() || cutb[b] < cut[a]);
             const int i0 = takeb ? cutb[b] : cut[a], i1 = takeb ? cutb[b + 1] : cut[a + 1];
-            for (int cb = 0; cb < ncb; cb++)
-                if (outer(cb) == takeb)
-                    tl.push_back(WvTile{q, cb, i0, flags_of(q, cb, i0, i1) | ((i1 - i0) << WV_TILE_ROWS_SHIFT)});
+            if (takeb || !wide || cb_hi < cb_lo || B - A < ww) {
+                for (int cb = 0; cb < ncb; cb++)
+                    if (outer(cb) == takeb) tl.push_back(narrow(q, cb * wout, wout, i0, i1));
+            } else {
+                // inner columns of an inner strip: wide tiles from A (each
+                // reading [jb - K, jb + ww + K), inside the box since the
+                // inner blocks are), the rest [A + m ww, B) in one-wave tiles
+                // ending at B (the first may overlap the last wide tile:
+                // those columns get the same values from both)
+                const int m = (B - A) / ww, rest = B - A - m * ww;
+                for (int t = 0; t < m; t++) {
+                    const int jb = A + t * ww;
+                    const int f = flags_of(q, jb - k, jb + ww + k, i0, i1);
+                    if (f & WV_TILE_INTERIOR) {
+                        tl.push_back(WvTile{q, jb, i0, f | WV_TILE_WIDE | (ww << WV_TILE_WIDTH_SHIFT) |
+                                                           ((i1 - i0) << WV_TILE_ROWS_SHIFT)});
+                    } else {
+                        for (int j = jb; j < jb + ww; j += wout) tl.push_back(narrow(q, j, std::min(wout, jb + ww - j), i0, i1));
+                    }
+                }
+                const int nr = (rest + wout - 1) / wout;
+                for (int r = 0; r < nr; r++) tl.push_back(narrow(q, B - (nr - r) * wout, wout, i0, i1));
+            }
             (takeb ? b : a)++;
         }
     }
-    // Estimated run time of a tile in units of one interior level-row (one
-    // level of one row of 64 lanes): K*rows + K*(K-1) level-rows (the skewed
-    // pipeline fills and drains over 2K halo rows; half of that at the first
-    // and last rows of a grid not periodic along i, whose rows outside the box
-    // are skipped), times the cost of a level-row on the tile's path, measured
-    // per tile at 4096^2 (K = 8, both forms): interior 1 (1.15 with inactive
-    // cells), edge-column tiles 4.5, tiles of the first or last rows 8, and
-    // the corner tiles of both 16.
+    // Estimated run time of a tile (of each of its waves) in units of one
+    // interior level-row (one level of one row of 64 lanes): K*rows +
+    // K*(K-1) level-rows (the skewed pipeline fills and drains over 2K halo
+    // rows; half of that at the first and last rows of a grid not periodic
+    // along i, whose rows outside the box are skipped), times the cost of a
+    // level-row on the tile's path, measured per tile at 4096^2 (K = 8, both
+    // forms): interior 1 (1.15 with inactive cells), edge-column tiles 4.5,
+    // tiles of the first or last rows 8 (measured 6, corners included, on the
+    // edge-column path; counted as 8 so that they start before the interior
+    // tiles: started last, they set the launch length), the corner tiles on
+    // the boundary path 16.
     auto tile_cost = [&](const WvTile &t) {
-        const Grid &G = e->g[t.inst];
-        const int n = t.flags >> WV_TILE_ROWS_SHIFT, nx = G.p.nx;
-        const bool wrapi = G.p.bcond == 1;
+        const Grid &Gq = e->g[t.inst];
+        const int n = t.flags >> WV_TILE_ROWS_SHIFT, nx = Gq.p.nx;
+        const bool wrapi = Gq.p.bcond == 1;
         const bool rowedge = t.i0 < k || t.i0 + n > nx - k;
         const double lr = (double)k * n + (rowedge && !wrapi ? 0.5 : 1.0) * k * (k - 1);
-        const int js = t.cb * wout - wv_halo(k);
-        const bool coledge = !(js >= 1 && js + 64 * WV_C <= G.p.ny - 1);
+        const int js = t.jb - H;
+        const bool coledge = !(js >= 1 && js + 64 * WV_C <= Gq.p.ny - 1);
         double f;
         if (t.flags & WV_TILE_INTERIOR) f = (t.flags & WV_TILE_ALLACT) ? 1.0 : 1.15;
-        else if (t.flags & WV_TILE_EDGEJ) f = 4.5;
+        else if (ej_on && (t.flags & WV_TILE_EDGEJ)) f = rowedge ? 8.0 : 4.5;
         else f = (rowedge && coledge) ? 16.0 : 8.0;
         return lr * f;
     };
-    // one list in XCD-interleave order.  Each XCD takes one contiguous run of
-    // the grid order holding an eighth of the estimated work, and starts with
-    // its longest tiles: started last, they would run on after the rest had
-    // finished.  (Balancing rows instead of work gave the two XCDs holding
-    // the first and last rows, with their slow boundary tiles, 12 % more work
-    // and a launch 20 % longer than the other six needed.)
-    std::vector<WvTile> h, hb;
+    auto slots = [&](const WvTile &t) { return (t.flags & WV_TILE_WIDE) ? WV_WG : 1; };
+    // one list of workgroups in XCD-interleave order.  Each XCD takes one
+    // contiguous run of the grid order holding an eighth of the estimated
+    // work (wave-slots times run time), and starts with its longest tiles:
+    // started last, they would run on after the rest had finished.
+    // (Balancing rows instead of work gave the two XCDs holding the first and
+    // last rows, with their slow boundary tiles, 12 % more work and a launch
+    // 20 % longer than the other six needed.)  One-wave tiles of the split
+    // form fill workgroups WV_WG at a time in that order.
+    const WvTile padt{-1, 0, 0, 0};
+    auto to_groups = [&](const std::vector<WvTile> &v, std::vector<WvTile> &dst) {
+        std::vector<WvTile> pack;
+        for (const WvTile &t : v) {
+            if (t.flags & WV_TILE_WIDE) {
+                for (int w = 0; w < G; w++) dst.push_back(t);
+                continue;
+            }
+            pack.push_back(t);
+            if ((int)pack.size() == G) {
+                dst.insert(dst.end(), pack.begin(), pack.end());
+                pack.clear();
+            }
+        }
+        if (!pack.empty()) {
+            while ((int)pack.size() < G) pack.push_back(padt);
+            dst.insert(dst.end(), pack.begin(), pack.end());
+        }
+    };
+    std::vector<WvTile> h;
     if (e->xcd_order) {
         const int X = 8;
         std::vector<std::vector<WvTile>> per(X);
         std::vector<double> cost(tl.size());
         double total = 0;
-        for (size_t q = 0; q < tl.size(); q++) total += (cost[q] = tile_cost(tl[q]));
+        for (size_t q = 0; q < tl.size(); q++) total += (cost[q] = tile_cost(tl[q])) * slots(tl[q]);
         double acc = 0;
         std::vector<std::vector<double>> pc(X);
         for (size_t q = 0; q < tl.size(); q++) {
-            const int x = std::min(X - 1, (int)((acc + 0.5 * cost[q]) * X / std::max(total, 1e-30)));
+            const double w = cost[q] * slots(tl[q]);
+            const int x = std::min(X - 1, (int)((acc + 0.5 * w) * X / std::max(total, 1e-30)));
             per[x].push_back(tl[q]);
             pc[x].push_back(cost[q]);
-            acc += cost[q];
+            acc += w;
         }
         if (e->slow_first) {
             for (int x = 0; x < X; x++) {
@@ -539,18 +644,27 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
                 per[x].swap(v);
             }
         }
+        std::vector<std::vector<WvTile>> grp(X);
         size_t len = 0;
-        for (const auto &v : per) len = std::max(len, v.size());
+        for (int x = 0; x < X; x++) {
+            to_groups(per[x], grp[x]);
+            len = std::max(len, grp[x].size() / G);
+        }
         for (size_t j = 0; j < len; j++)
-            for (int x = 0; x < X; x++) h.push_back(j < per[x].size() ? per[x][j] : WvTile{-1, 0, 0, 0});
-        while (!h.empty() && h.back().inst < 0) h.pop_back();
+            for (int x = 0; x < X; x++)
+                for (int w = 0; w < G; w++) h.push_back(j * G < grp[x].size() ? grp[x][j * G + w] : padt);
+        while (h.size() >= (size_t)G) {
+            bool pad = true;
+            for (int w = 0; w < G; w++) pad = pad && h[h.size() - G + w].inst < 0;
+            if (!pad) break;
+            h.resize(h.size() - G);
+        }
     } else {
-        h = tl;
+        to_groups(tl, h);
     }
     Engine::Tiles T;
-    T.ni = (int)h.size();
-    T.nb = (int)hb.size();
-    h.insert(h.end(), hb.begin(), hb.end());
+    T.ni = (int)(h.size() / G);
+    T.nb = 0;
     if (!h.empty()) {
         HIPCHK(e, hipMalloc(&T.d, sizeof(WvTile) * h.size()));
         HIPCHK(e, hipMemcpy(T.d, h.data(), sizeof(WvTile) * h.size(), hipMemcpyHostToDevice));
@@ -581,6 +695,7 @@ int engine_create(Engine **out, int n, const wv_params *ps)
         e->xcd_order = strcmp(o, "linear") != 0;
         e->slow_first = strcmp(o, "inplace") != 0;
     }
+    if (const char *o = getenv("WAVEHIP_WIDE")) e->wide = atoi(o) != 0;
     int ndev = 0;
     HIPCHK(e, hipGetDeviceCount(&ndev));
     if (e->device < 0 || e->device >= ndev) return fail(e, WV_EINVAL, "device %d not present (%d devices)", e->device, ndev);
@@ -615,9 +730,10 @@ int engine_create(Engine **out, int n, const wv_params *ps)
     HIPCHK(e, hipMemcpy(e->d_row0, row0.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
     HIPCHK(e, hipMalloc(&e->d_partial, sizeof(double) * rowsv.size()));
     HIPCHK(e, hipMalloc(&e->d_eout, sizeof(double) * n));
-    const WvTile pad{-1, 0, 0, 0};
-    HIPCHK(e, hipMalloc(&e->d_pad, sizeof(WvTile)));
-    HIPCHK(e, hipMemcpy(e->d_pad, &pad, sizeof(WvTile), hipMemcpyHostToDevice));
+    const WvTile pad[WV_WG] = {{-1, 0, 0, 0}, {-1, 0, 0, 0}, {-1, 0, 0, 0}, {-1, 0, 0, 0}};
+    static_assert(WV_WG == 4, "padding workgroup");
+    HIPCHK(e, hipMalloc(&e->d_pad, sizeof(pad)));
+    HIPCHK(e, hipMemcpy(e->d_pad, pad, sizeof(pad), hipMemcpyHostToDevice));
     return sync_inst(e);
 }
 
@@ -2077,6 +2193,46 @@ int wv_get_tiling(const wv_ctx *c, int *tb_depth, int *rows_per_tile, int *tile_
     if (rows_per_tile) *rows_per_tile = rows(e);
     if (tile_cols) *tile_cols = wv_tile_cols(depth(e));
     return WV_OK;
+}
+
+// tiles of one launch of depth tb_depth (0: the current depth), by kind, and
+// the output cells they cover (a cell written by two tiles counts twice)
+static int tile_counts(Engine *e, int k, int *wide, int *narrow, int *groups, long *cells)
+{
+    if (k == 0) k = depth(e);
+    if (!wv_have_depth(k)) return fail(e, WV_EINVAL, "no kernel for fused depth %d", k);
+    int rc = ready(e);
+    if (rc) return rc;
+    HIPCHK(e, hipSetDevice(e->device));
+    Engine::Tiles t;
+    if ((rc = get_tiles(e, k, &t))) return rc;
+    const int G = e->form == WV_FORM_F32SPLIT ? WV_WG : 1;
+    std::vector<WvTile> h((size_t)t.ni * G);
+    if (!h.empty()) HIPCHK(e, hipMemcpy(h.data(), t.d, sizeof(WvTile) * h.size(), hipMemcpyDeviceToHost));
+    int nw = 0, nn = 0;
+    long nc = 0;
+    for (size_t q = 0; q < h.size(); q++) {
+        const WvTile &u = h[q];
+        if (u.inst < 0) continue;
+        const bool w = u.flags & WV_TILE_WIDE;
+        if (w && q % G) continue;  // a wide tile fills its workgroup's entries
+        (w ? nw : nn)++;
+        const Grid &Gq = e->g[u.inst];
+        const int j1 = std::min(u.jb + ((u.flags >> WV_TILE_WIDTH_SHIFT) & 255), Gq.p.ny);
+        const int i1 = std::min(u.i0 + (u.flags >> WV_TILE_ROWS_SHIFT), Gq.p.nx);
+        nc += (long)std::max(0, j1 - std::max(u.jb, 0)) * std::max(0, i1 - u.i0);
+    }
+    if (wide) *wide = nw;
+    if (narrow) *narrow = nn;
+    if (groups) *groups = t.ni;
+    if (cells) *cells = nc;
+    return WV_OK;
+}
+
+int wv_get_tile_counts(wv_ctx *c, int tb_depth, int *wide, int *narrow, int *groups, long *cells)
+{
+    if (!c) return WV_EINVAL;
+    return tile_counts(c->e, tb_depth, wide, narrow, groups, cells);
 }
 
 int wv_set_tiling(wv_ctx *c, int tb_depth, int rows_per_tile)

@@ -24,6 +24,14 @@ __device__ __forceinline__ gptr<T> gm(T *p) { return (gptr<T>)p; }
 // native two-element vectors (usable in the global address space)
 typedef double dvec2 __attribute__((ext_vector_type(2)));
 typedef float fvec2 __attribute__((ext_vector_type(2)));
+// LDS view of a workgroup array
+typedef __attribute__((address_space(3))) fvec2 *lptr2;
+
+// Workgroup barrier for LDS traffic only: this wave's LDS writes are done
+// (lgkmcnt) and every wave has arrived.  Global loads in flight stay in
+// flight (a fence would wait for them, draining a prefetch ring), and the
+// memory clobber keeps the compiler from moving LDS accesses across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 constexpr double W9 = 1.0 / 6.0;  // 9-point weight: ((4*s4 + sd) - 20x) * W9
 
@@ -206,6 +214,41 @@ __device__ __forceinline__ double edge_tb_inside(const Par &d, int rm, int jm, d
     }
 }
 
+// edge_fp64() for a cell of row 0 or NX-1 (rm) of a grid not periodic along
+// i (B_COND != 1): the corner cells (jm = 0 or NY-1) by the top/bottom-row
+// formulas, the others by the left/right-column ones; OSCILLATE_TOPBOT does
+// not reach rows 0 and NX-1 (same arithmetic, same order as edge_fp64)
+__device__ __forceinline__ double edge_row_box(const Par &d, int rm, int jm, double x, double y, double E,
+                                               double Wv, double N, double S, double cc, double g, double tcv)
+{
+    const bool first = rm == 0;
+    double delta;
+    if (jm == d.ny - 1 || jm == 0) {
+        const bool top = (jm == d.ny - 1);
+        const double V = top ? S : N;  // inward neighbour along j
+        const double O = top ? N : S;  // neighbour across the periodic seam
+        switch (d.bc) {
+        case 0: {
+            const double Ec = first ? E : x;
+            const double Wc = first ? x : Wv;
+            delta = ((Ec + Wc) + V) - 3.0 * x;
+            return leap(x, y, delta, cc, d.kappa, g);
+        }
+        case 3:
+            if (first) return absorb(x, y, E, tcv, d.ks, d.gs);
+            delta = (((x + Wv) + V) + O) - 4.0 * x;
+            return leap(x, y, delta, cc, d.kappa, g);
+        default:
+            return absorb(x, y, V, tcv, d.kt, d.gt);
+        }
+    }
+    if (d.bc == 0) {
+        delta = first ? ((E + N) + S) - 3.0 * x : ((Wv + N) + S) - 3.0 * x;
+        return leap(x, y, delta, cc, d.kappa, g);
+    }
+    return absorb(x, y, first ? E : Wv, tcv, d.ks, d.gs);
+}
+
 // COMPARISON, wave_3d.c:736-990: the flat grid in two halves, j < NY/2 and
 // j >= NY/2.  Cells of the top and bottom rows (periodic wraps stay inside
 // their half) and of the interface columns jtop = NY/2-1 and jbot = NY/2, with
@@ -271,14 +314,16 @@ struct Cols {
     int j0;
     int jm[WV_C];
     bool colint[WV_C], wcol[WV_C];
-    __device__ __forceinline__ Cols(int j0_, int jb, int wout, int ny, bool wrapj) : j0(j0_)
+    // own: the lane may write its output columns (the mirrored lanes of a
+    // wide tile do not)
+    __device__ __forceinline__ Cols(int j0_, int jb, int wout, int ny, bool wrapj, bool own = true) : j0(j0_)
     {
 #pragma unroll
         for (int c = 0; c < WV_C; c++) {
             const int jj = j0 + c;
             jm[c] = FAST ? jj : map_idx(jj, ny, wrapj);
             colint[c] = FAST ? true : (jm[c] >= 1 && jm[c] <= ny - 2);
-            wcol[c] = jj >= jb && jj < jb + wout && jj < ny;
+            wcol[c] = own && jj >= jb && jj < jb + wout && jj < ny;
         }
     }
 };

@@ -104,16 +104,30 @@ struct WvInst {
 #define WV_TILE_DRV 16      // ... a driven cell (never an INTERIOR tile)
 #define WV_TILE_PRB 32      // ... a probe (never an INTERIOR tile)
 #define WV_TILE_EDGEJ 64    // not INTERIOR only because its columns reach column 0 or NY-1
-// One wave's work: column block cb of grid inst, rows [i0, i0 + rows) with
-// rows = flags >> WV_TILE_ROWS_SHIFT (the low bits hold WV_TILE_*)
+#define WV_TILE_WIDE 128    // an INTERIOR tile computed by the WV_WG waves of one workgroup
+// One tile: output columns [jb, jb + width) of grid inst, rows [i0, i0 +
+// rows), with width = (flags >> WV_TILE_WIDTH_SHIFT) & 255 and rows = flags
+// >> WV_TILE_ROWS_SHIFT (the low bits hold WV_TILE_*).  A one-wave tile's
+// lanes cover columns jb - H ... jb - H + 64C - 1 (H = wv_halo(K) halo
+// columns per side); a WIDE tile's wave w covers jb - K + 62w ... jb - K +
+// 62w + 63, its lanes 0 and 63 mirroring lane 62 of wave w-1 and lane 1 of
+// wave w+1 (exchanged through LDS at every time level).
+#define WV_TILE_WIDTH_SHIFT 8
 #define WV_TILE_ROWS_SHIFT 16
 struct WvTile {
-    int inst, cb, i0, flags;
+    int inst, jb, i0, flags;
 };
+
+// waves per workgroup of the split-form stepping kernel: a WIDE tile, or up
+// to WV_WG one-wave tiles side by side (independent of each other)
+#define WV_WG 4
 
 // number of output columns of one wave for a fused depth k
 static inline int wv_halo(int k) { return (k + WV_C - 1) / WV_C * WV_C; }
 static inline int wv_tile_cols(int k) { return 64 * WV_C - 2 * wv_halo(k); }
+// number of output columns of a WIDE tile (WV_C == 1 only): 62 per wave,
+// less K - 1 halo columns on either side
+static inline int wv_wide_cols(int k) { return 62 * WV_WG - 2 * k + 2; }
 
 // Launchers (wv_step_*.hip, dispatched in wv_energy.hip).  cm: WV_COEF_* of
 // every grid of the launch, or WV_CM_ALL.  Ti/ni: interior

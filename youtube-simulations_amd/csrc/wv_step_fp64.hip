@@ -48,11 +48,17 @@ constexpr int CF_ANY = 4;      // per-cell tables read in place (boundary tiles)
 // (with XT, K = 1): the neighbour-table Laplacian of evolve_wave_half_new on
 // boundary tiles of grids that use it.
 //
-// EJ (edge columns): rows inside the box, columns reaching column 0 or NY-1;
-// bulk cells as on interior rows, outer-column cells by the top/bottom-row
-// edge formula with the row tests resolved (edge_tb_inside).
+// EJ (edge columns): columns reaching column 0 or NY-1, rows inside the box
+// or reaching row 0 or NX-1 of a grid not periodic along i; bulk cells as on
+// interior rows, outer-column cells of rows inside the box by the top/bottom-
+// row edge formula with the row tests resolved (edge_tb_inside), rows 0 and
+// NX-1 by the full edge formula, rows outside the box skipped.
+//
+// EI (with EJ): the tile's rows reach row 0 or NX-1 of a grid not periodic
+// along i (a variant of its own, so that the edge-column loop of the other
+// tiles keeps its size).
 template <int K, int ST, bool INTERIOR, int CF, bool ALLACT, bool DAMP, bool XT = false, bool TB = false,
-          bool EJ = false>
+          bool EJ = false, bool EI = false>
 __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long gbase, int ia, int ib,
                                          int oc, int od)
 {
@@ -63,7 +69,7 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
     static_assert(INTERIOR || !PREF, "prefetched coefficients are for interior tiles");
     static_assert(ROWI || DAMP, "boundary tiles evaluate the full formula");
     constexpr int C = WV_C;
-    constexpr int H = Tile<K>::H, WOUT = Tile<K>::WOUT;
+    constexpr int H = Tile<K>::H;
     constexpr unsigned CMASK = (1u << C) - 1u;
     constexpr bool FAST = INTERIOR;
     static_assert(2 * K * C <= 32, "bit shift register holds 2K rows of C bits");
@@ -72,8 +78,8 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
     const Src src(d);
     const int nx = p.nx, ny = p.ny, ldy = p.ldy, ldw = p.ldw;
     const bool wrapi = p.bc == 1, wrapj = (p.bc == 1 || p.bc == 3);
-    const int jb = t.cb * WOUT;
-    const Cols<FAST> L(jb - H + (int)threadIdx.x * C, jb, WOUT, ny, wrapj);
+    const int jb = t.jb;
+    const Cols<FAST> L(jb - H + (int)threadIdx.x * C, jb, (t.flags >> WV_TILE_WIDTH_SHIFT) & 255, ny, wrapj);
     const int ibeg = t.i0;
     const int iend = min(ibeg + (t.flags >> WV_TILE_ROWS_SHIFT), nx);
     const bool has_src = !ROWI && (t.flags & WV_TILE_SRC) && src.any();  // (source tiles are boundary tiles)
@@ -173,13 +179,15 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
             const bool live = (unsigned)(s - s0 - 3 * lv) < (unsigned)(iend - ibeg + 2 * (K - lv));
             // rows outside the box (plain boundary tiles of grids not periodic
             // along i): no edge formula of rows 0 and NX-1 reads them
-            const bool inbox = ROWI || XT || TB || wrapi || (unsigned)(s - 2 * lv) < (unsigned)nx;
+            const bool inbox = (ROWI && !EI) || XT || TB || wrapi || (unsigned)(s - 2 * lv) < (unsigned)nx;
             if (live && inbox) {
                 const int r = s - 2 * lv;
                 // rows outside the dependence cone of the strip's output are never
                 // used: INTERIOR tiles only clamp them so table reads stay in bounds
                 const int rm = ROWI ? min(max(r, 0), nx - 1) : map_idx(r, nx, wrapi);
-                const bool rowint = ROWI || (rm >= 1 && rm <= nx - 2);
+                // (an edge-column tile may reach row 0 or NX-1 of a grid not
+                // periodic along i: the row is wave-uniform)
+                const bool rowint = (ROWI && !EI) || (rm >= 1 && rm <= nx - 2);
                 const double(&Xe)[C] = U[lv - 1][PE];
                 const double(&Xc)[C] = U[lv - 1][PC];
                 const double(&Xw)[C] = U[lv - 1][PW];
@@ -245,8 +253,9 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
                         } else if (cmpm && (cmpcol || L.jm[c] == 0 || L.jm[c] == ny - 1)) {
                             v = cmp_fp64(p, u0, rm, L.jm[c], x, y, cc, g, tcv);
                         } else {
-                            v = EJ ? edge_tb_inside(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv)
-                                   : edge_fp64(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv);
+                            v = !EJ ? edge_fp64(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv)
+                                    : (!EI || rowint ? edge_tb_inside(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv)
+                                                     : edge_row_box(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv));
                         }
                     }
                     bool w = ALLACT || ((ab >> c) & 1u);
@@ -414,8 +423,15 @@ k_step_fp64(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, cons
     case WV_COEF_UNIFORM:
         if (interior) { WV_INTERIOR(CF_UNIFORM) }
         else if (EdgeJ<K, ST, CM>::on && edgej) {
-            if (damp) run_fp64<K, ST, false, CF_UNIFORM, false, true, false, false, true>(d, t, gbase, ia, ib, oc, od);
-            else run_fp64<K, ST, false, CF_UNIFORM, false, false, false, false, true>(d, t, gbase, ia, ib, oc, od);
+            // edge-column tiles that reach row 0 or NX-1 (of a grid not periodic along i)
+            const bool edgei = t.i0 < K || t.i0 + (t.flags >> WV_TILE_ROWS_SHIFT) > d.nx - K;
+            if (edgei) {
+                if (damp) run_fp64<K, ST, false, CF_UNIFORM, false, true, false, false, true, true>(d, t, gbase, ia, ib, oc, od);
+                else run_fp64<K, ST, false, CF_UNIFORM, false, false, false, false, true, true>(d, t, gbase, ia, ib, oc, od);
+            } else {
+                if (damp) run_fp64<K, ST, false, CF_UNIFORM, false, true, false, false, true>(d, t, gbase, ia, ib, oc, od);
+                else run_fp64<K, ST, false, CF_UNIFORM, false, false, false, false, true>(d, t, gbase, ia, ib, oc, od);
+            }
         } else run_fp64<K, ST, false, CF_UNIFORM, false, true, XT, TB>(d, t, gbase, ia, ib, oc, od);
         break;
     case WV_COEF_TWO:

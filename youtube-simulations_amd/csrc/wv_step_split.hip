@@ -56,14 +56,35 @@ constexpr int CF_ANY = 6;      // per-cell tables read in place (boundary tiles)
 // 2*courant of the velocity's own rounding per step, and it saves two
 // additions per cell and level against keeping the two sums apart.)
 //
-// EJ (edge columns): the tile's rows never reach the outer box but its
-// columns do.  Every cell takes the interior evaluation and the cells of the
-// outer columns (j = 0, NY-1, and the halo lanes mapped onto them) then take
-// the FP64 edge formula instead: a bulk cell's neighbours are its true
-// neighbours whatever lane holds them, and an edge cell's own value is
-// replaced, not blended.
-template <int K, int ST, bool INTERIOR, int CF, bool ALLACT, bool DAMP, bool EJ = false>
-__device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long gbase, int ia, int oc)
+// EJ (edge columns): the tile's columns reach the outer box, and its rows
+// either stay inside it or reach row 0 or NX-1 of a grid not periodic along i.
+// On rows inside the box every cell takes the interior evaluation and the
+// cells of the outer columns (j = 0, NY-1, and the halo lanes mapped onto
+// them) then take the FP64 edge formula instead: a bulk cell's neighbours are
+// its true neighbours whatever lane holds them, and an edge cell's own value
+// is replaced, not blended.  Rows 0 and NX-1 take the edge formula on every
+// lane (a branch the whole wave takes), and rows outside the box are skipped
+// (no edge formula reads them).
+//
+// WIDE (interior tiles only): the WV_WG waves of the workgroup compute one
+// tile side by side, wave w on columns jb - K + 62w + lane.  Lanes 1..62 of
+// a wave are its own; lane 0 holds the column of lane 62 of wave w-1 and
+// lane 63 that of lane 1 of wave w+1.  After every level lanes 1 and 62 put
+// their new (hi, lo) into LDS, and before the next level reads that row
+// lanes 0 and 63 take their neighbours' values from there: the j-halo is
+// one column per wave instead of K, and only the workgroup's two outer
+// waves carry the K-column dependence cone.  One barrier per row: a level's
+// row is written in one iteration and read in the next, from one of three
+// buffers (by iteration phase), so a buffer is rewritten two barriers after
+// it was read.  The same cells get the same operations in the same order as
+// in a one-wave tile.
+//
+// EI (with EJ): the tile's rows reach row 0 or NX-1 of a grid not periodic
+// along i (a variant of its own, so that the edge-column loop of the other
+// tiles keeps its size).
+template <int K, int ST, bool INTERIOR, int CF, bool ALLACT, bool DAMP, bool EJ = false, bool WIDE = false,
+          bool EI = false>
+__device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long gbase, int ia, int oc, lptr2 gh)
 {
     constexpr bool ROWI = INTERIOR || EJ;  // rows inside the outer box
     constexpr bool PREF = CF >= CF_FIELD && CF <= CF_FIELDLG;
@@ -73,8 +94,10 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
     static_assert(!(INTERIOR && EJ), "edge-column tiles are not interior tiles");
     static_assert(!EJ || WV_C == 1, "edge-column tiles: one cell per lane");
     static_assert(ROWI || !PREF, "prefetched coefficients are for tiles with interior rows");
+    static_assert(!WIDE || (INTERIOR && WV_C == 1), "wide tiles: interior, one cell per lane");
+    static_assert(!EI || EJ, "edge rows: edge-column tiles only");
     constexpr int C = WV_C;
-    constexpr int H = Tile<K>::H, WOUT = Tile<K>::WOUT;
+    constexpr int H = Tile<K>::H;
     constexpr unsigned CMASK = (1u << C) - 1u;
     constexpr bool FAST = INTERIOR;
     static_assert(2 * K * C <= 32, "bit shift register holds 2K rows of C bits");
@@ -83,8 +106,24 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
     const Src src(d);
     const int nx = p.nx, ny = p.ny, ldy = p.ldy, ldw = p.ldw;
     const bool wrapi = p.bc == 1, wrapj = (p.bc == 1 || p.bc == 3);
-    const int jb = t.cb * WOUT;
-    const Cols<FAST> L(jb - H + (int)threadIdx.x * C, jb, WOUT, ny, wrapj);
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int jb = t.jb;
+    const Cols<FAST> L(WIDE ? jb - K + 62 * wave + lane : jb - H + lane * C, jb, (t.flags >> WV_TILE_WIDTH_SHIFT) & 255,
+                       ny, wrapj, !WIDE || (lane >= 1 && lane <= 62));
+    // LDS slots of the exchange: [phase][wave][level-1][side] (float2 hi,
+    // lo), side 0 written by lane 1, side 1 by lane 62.  Lane 0 reads side 1
+    // of wave w-1 and lane 63 side 0 of wave w+1 (not at the workgroup's outer
+    // edges).  Only those lanes touch LDS (writing or reading whole rows costs
+    // the wave an eighth of its time).
+    constexpr int GL = K > 1 ? K - 1 : 1;
+    const bool gwr = lane == 1 || lane == 62;
+    const bool grd = (lane == 0 && wave > 0) || (lane == 63 && wave < WV_WG - 1);
+    const int gw = wave * GL * 2 + (lane == 62);
+    const int gr = grd ? (lane == 0 ? wave - 1 : wave + 1) * GL * 2 + (lane == 0) : 0;
+    (void)gwr;
+    (void)grd;
+    (void)gw;
+    (void)gr;
     const int ibeg = t.i0;
     const int iend = min(ibeg + (t.flags >> WV_TILE_ROWS_SHIFT), nx);
     const bool has_src = !ROWI && (t.flags & WV_TILE_SRC) && src.any();  // (source tiles are boundary tiles)
@@ -191,11 +230,14 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
             const bool live = (unsigned)(s - s0 - 3 * lv) < (unsigned)(iend - ibeg + 2 * (K - lv));
             // rows outside the box (boundary tiles of grids not periodic along i):
             // no edge formula of rows 0 and NX-1 reads them, so they are skipped
-            const bool inbox = ROWI || wrapi || (unsigned)(s - 2 * lv) < (unsigned)nx;
+            const bool inbox = (ROWI && !EI) || wrapi || (unsigned)(s - 2 * lv) < (unsigned)nx;
             if (live && inbox) {
                 const int r = s - 2 * lv;
                 const int rm = ROWI ? min(max(r, 0), nx - 1) : map_idx(r, nx, wrapi);
-                const bool rowint = ROWI || (rm >= 1 && rm <= nx - 2);
+                // (wave-uniform; an edge-column tile may also reach row 0 or NX-1
+                // of a grid not periodic along i, whose rows take the edge formula
+                // on every lane)
+                const bool rowint = (ROWI && !EI) || (rm >= 1 && rm <= nx - 2);
                 const float(&he)[C] = Hw[lv - 1][PE];
                 const float(&hc)[C] = Hw[lv - 1][PC];
                 const float(&hw)[C] = Hw[lv - 1][PW];
@@ -233,8 +275,9 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
                         double cc, g;
                         coef_fp64<COEF>(p, tccf, tgf, rm, L.jm[c], mbit, cc, g);
                         const double tcv = PTV ? TV[lv - 1][c] : gc(tcp)[(size_t)rm * ldy + L.jm[c]];
-                        double out = EJ ? edge_tb_inside(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv)
-                                        : edge_fp64(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv);
+                        double out = !EJ ? edge_fp64(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv)
+                                         : (!EI || rowint ? edge_tb_inside(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv)
+                                                          : edge_row_box(p, rm, L.jm[c], x, y, E, Wv, N, S, cc, g, tcv));
                         if (has_src) {
                             const long g1 = gbase + lv - 1;
                             if (g1 < src.len) out = add_sources(src, rm, L.jm[c], g1, out);
@@ -285,7 +328,8 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
                         }
                         dd_add_f(h0, l0, vn, oh, ol);
                         ov = vn;
-                        if (EJ && !L.colint[c]) edge_cell(oh, ol, ov);
+                        // (rows 0 and NX-1: every lane, the interior values discarded)
+                        if (EJ && (!rowint || !L.colint[c])) edge_cell(oh, ol, ov);
                     } else if (rowint && L.colint[c]) {
                         const float h0 = hc[c], l0 = lc[c];
                         // differences of u = hi + lo: (hi difference) + (lo difference)
@@ -389,12 +433,39 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
         medreg = (medreg << C) | bits_of(QM[PR], L);
         // unconditional (past the last needed row it re-reads that cached row)
         fetch(min(s + WV_RING, slast), QH[PR], QL[PR], QV[PR], QB[PR], QM[PR]);
+        if (WIDE) {
+            // The rows of levels 1..K-1 this iteration computed (slot P): lanes
+            // 1 and 62 publish theirs, and after the barrier lanes 0 and 63 take
+            // their neighbours'.  A level's row is first read with its ghost
+            // lanes in the next iteration (9-point diagonals, slot PE) or the one
+            // after (slot PC), so the loads complete in the shadow of the next
+            // row's work.  (All levels together, once per row: exec-masked
+            // accesses next to the levels' liveness branches make the compiler
+            // restructure the whole loop.  Every level writes its slot, live or
+            // not: one that was not live left the slot as it was, and nobody
+            // reads it where it is not needed.)
+            if (gwr) {
+#pragma unroll
+                for (int lv = 1; lv < K; lv++) gh[gw + (P * WV_WG * GL + lv - 1) * 2] = fvec2{Hw[lv][P][0], Lw[lv][P][0]};
+            }
+            // no wait for the global loads in flight
+            lds_barrier();
+            if (grd) {
+#pragma unroll
+                for (int lv = 1; lv < K; lv++) {
+                    const fvec2 g = gh[gr + (P * WV_WG * GL + lv - 1) * 2];
+                    Hw[lv][P][0] = g.x;
+                    Lw[lv][P][0] = g.y;
+                }
+            }
+        }
     };
 
     // one exit per trip (up to two trailing iterations past send-1 find every
     // level dead and re-read the last row): an exit between the phases would
     // make the compiler copy ring registers at the loop edges
     static_assert(WV_RING == 3 || WV_RING == 6, "ring of 3 or 6 rows");
+    static_assert(!WIDE || WV_RING == 3, "wide tiles: the exchange buffers follow the 3-row ring");
     for (int s = s0; s < send; s += WV_RING) {
         iter(s, std::integral_constant<int, 0>{});
         iter(s + 1, std::integral_constant<int, 1>{});
@@ -429,47 +500,62 @@ struct Occ {
 };
 
 template <int K, int ST, int CM>
-__global__ void __launch_bounds__(64, (Occ<K, ST, CM>::waves))
+__global__ void __launch_bounds__(64 * WV_WG, (Occ<K, ST, CM>::waves))
 k_step_split(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, const long *gstep,
             long g0, int ia, int oc)
 {
-    const WvTile t = tiles[blockIdx.x];
-    if (t.inst < 0) return;  // padding of the XCD interleave
+    // exchange rows of a wide tile (see run_split)
+    __shared__ fvec2 gh[3 * WV_WG * (K > 1 ? K - 1 : 1) * 2];
+    // (the wave index through a scalar register: the tile and everything
+    // derived from it stay wave-uniform)
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const WvTile t = tiles[blockIdx.x * WV_WG + wave];
+    if (t.inst < 0) return;  // padding (one-wave tiles only: a wide tile fills its workgroup)
     const WvInst &d = I[t.inst];
     const long gbase = (gstep ? *gstep : 0) + g0;
     const bool interior = t.flags & WV_TILE_INTERIOR;
     const bool allact = t.flags & WV_TILE_ALLACT;
     const bool edgej = t.flags & WV_TILE_EDGEJ;
+    const bool wide = t.flags & WV_TILE_WIDE;
     const bool damp = d.damped;
-#define WV_RUN(CF, A, D) run_split<K, ST, true, CF, A, D>(d, t, gbase, ia, oc)
-#define WV_EDGEJ(CF)                                                        \
-    if (damp) run_split<K, ST, false, CF, false, true, true>(d, t, gbase, ia, oc); \
-    else run_split<K, ST, false, CF, false, false, true>(d, t, gbase, ia, oc);
+    const lptr2 lg = (lptr2)gh;
+#define WV_RUNW(CF, A, D, W) run_split<K, ST, true, CF, A, D, false, W>(d, t, gbase, ia, oc, lg)
+#define WV_RUN(CF, A, D)                  \
+    if (wide) WV_RUNW(CF, A, D, WV_C == 1);    \
+    else WV_RUNW(CF, A, D, false);
+    // edge-column tiles that reach row 0 or NX-1 (of a grid not periodic along i)
+    const bool edgei = t.i0 < K || t.i0 + (t.flags >> WV_TILE_ROWS_SHIFT) > d.nx - K;
+#define WV_EDGEJ1(CF, EI)                                                                      \
+    if (damp) run_split<K, ST, false, CF, false, true, true, false, EI>(d, t, gbase, ia, oc, lg); \
+    else run_split<K, ST, false, CF, false, false, true, false, EI>(d, t, gbase, ia, oc, lg);
+#define WV_EDGEJ(CF)               \
+    if (edgei) { WV_EDGEJ1(CF, true) } \
+    else { WV_EDGEJ1(CF, false) }
 #define WV_INTERIOR(CF)                            \
     if (allact) {                                  \
-        if (damp) WV_RUN(CF, true, true);          \
-        else WV_RUN(CF, true, false);              \
+        if (damp) { WV_RUN(CF, true, true) }       \
+        else { WV_RUN(CF, true, false) }           \
     } else {                                       \
-        if (damp) WV_RUN(CF, false, true);         \
-        else WV_RUN(CF, false, false);             \
+        if (damp) { WV_RUN(CF, false, true) }      \
+        else { WV_RUN(CF, false, false) }          \
     }
 #define WV_DAMPED(CF)                              \
-    if (allact) WV_RUN(CF, true, true);            \
-    else WV_RUN(CF, false, true);
+    if (allact) { WV_RUN(CF, true, true) }         \
+    else { WV_RUN(CF, false, true) }
     // the modes this kernel is not built for are folded away
     const int coef = CM == WV_CM_ALL ? d.coef : CM;
     switch (coef) {
     case WV_COEF_UNIFORM:
         if (interior) { WV_INTERIOR(CF_UNIFORM) }
         else if (EdgeJ<K, ST, CM>::on && edgej) { WV_EDGEJ(CF_UNIFORM) }
-        else run_split<K, ST, false, CF_UNIFORM, false, true>(d, t, gbase, ia, oc);
+        else run_split<K, ST, false, CF_UNIFORM, false, true>(d, t, gbase, ia, oc, lg);
         break;
     case WV_COEF_TWO:
         if (interior) { WV_INTERIOR(CF_TWO) }
-        else run_split<K, ST, false, CF_TWO, false, true>(d, t, gbase, ia, oc);
+        else run_split<K, ST, false, CF_TWO, false, true>(d, t, gbase, ia, oc, lg);
         break;
     default:
-        if (!interior) run_split<K, ST, false, CF_ANY, false, true>(d, t, gbase, ia, oc);
+        if (!interior) run_split<K, ST, false, CF_ANY, false, true>(d, t, gbase, ia, oc, lg);
         else if (d.tccf_lo) {
             if (d.tgf32) { WV_DAMPED(CF_FIELDLG) }  // a tgamma field implies damping
             else WV_INTERIOR(CF_FIELDL)
@@ -480,9 +566,11 @@ k_step_split(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, con
         break;
     }
 #undef WV_EDGEJ
+#undef WV_EDGEJ1
 #undef WV_DAMPED
 #undef WV_INTERIOR
 #undef WV_RUN
+#undef WV_RUNW
 }
 
 }  // namespace
@@ -493,11 +581,12 @@ k_step_split(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, con
 int WV_CAT(wv_launch_split_k, WV_K)(int stencil, int cm, const WvInst *I, const WvTile *Ti, int ni, const WvTile *Tb,
                                      int nb, const long *gs, long g0, int ia, int oc, hipStream_t s)
 {
-    // the two tile lists are contiguous in memory: one launch covers both
+    // the two tile lists are contiguous in memory: one launch covers both;
+    // ni and nb count workgroups (WV_WG tile entries each)
     const WvTile *T = ni ? Ti : Tb;
     const int n = ni + nb;
     if (!n) return 0;
-    const dim3 grid(n), block(64);
+    const dim3 grid(n), block(64 * WV_WG);
 #define WV_GO(ST, CM) hipLaunchKernelGGL((k_step_split<WV_K, ST, CM>), grid, block, 0, s, I, T, gs, g0, ia, oc)
 #define WV_ST(ST)                                 \
     switch (cm) {                                 \

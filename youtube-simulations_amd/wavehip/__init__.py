@@ -35,7 +35,7 @@ EXPORTS = [
     "wv_set_wells_tables", "wv_build_mask", "wv_build_mask_static", "wv_init_circular_wave",
     "wv_add_circular_wave", "wv_batch_init_circular_wave", "wv_batch_add_circular_wave", "wv_tiff_bound", "wv_tiff_encode", "wv_tiff_write",
     "wv_step", "wv_prepare", "wv_step_timed", "wv_sync", "wv_steps_done", "wv_last_kernel_ms", "wv_energy",
-    "wv_get_tiling", "wv_frame_params_default", "wv_frame_set_tables", "wv_frame_reset", "wv_frame",
+    "wv_get_tiling", "wv_get_tile_counts", "wv_frame_params_default", "wv_frame_set_tables", "wv_frame_reset", "wv_frame",
     "wv_frame_rgba8",
     "wv_set_tiling", "wv_autotune", "wv_evolve_wave_half_rows", "wv_evolve_wave_rows",
     "wv_evolve_wave_half_flat", "wv_evolve_wave_flat", "wv_batch_create", "wv_batch_destroy",
@@ -142,6 +142,7 @@ def load():
         "wv_frame": (ctypes.c_int, [vp, ctypes.POINTER(WvFrameParams), ctypes.POINTER(ctypes.c_float)]),
         "wv_frame_rgba8": (ctypes.c_int, [vp, ctypes.POINTER(WvFrameParams), ctypes.POINTER(ctypes.c_ubyte)]),
         "wv_get_tiling": (ctypes.c_int, [vp, _I, _I, _I]),
+        "wv_get_tile_counts": (ctypes.c_int, [vp, ctypes.c_int, _I, _I, _I, ctypes.POINTER(ctypes.c_long)]),
         "wv_set_tiling": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
         "wv_autotune": (ctypes.c_int, [vp, ctypes.c_long, _D]),
         "wv_evolve_wave_half_rows": (ctypes.c_int, [vp, _DD, _DD, _DD, _SS, _DD, _DD, _DD]),
@@ -515,6 +516,14 @@ class Wave:
         self._chk(self._L.wv_get_tiling(self._h, ctypes.byref(k), ctypes.byref(L), ctypes.byref(w)),
                   "wv_get_tiling")
         return k.value, L.value, w.value
+
+    def tile_counts(self, tb_depth=0):
+        """(wide tiles, one-wave tiles, workgroups, output cells covered) of
+        one launch of fused depth tb_depth (0: the current depth)"""
+        a, b, c, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_long()
+        self._chk(self._L.wv_get_tile_counts(self._h, int(tb_depth), ctypes.byref(a), ctypes.byref(b),
+                                             ctypes.byref(c), ctypes.byref(n)), "wv_get_tile_counts")
+        return a.value, b.value, c.value, n.value
 
     def set_tiling(self, tb_depth, rows_per_tile):
         self._chk(self._L.wv_set_tiling(self._h, int(tb_depth), int(rows_per_tile)), "wv_set_tiling")
