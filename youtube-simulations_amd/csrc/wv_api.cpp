@@ -321,6 +321,8 @@ void drop_tiles(Engine *e)
 // before by the neighbouring strip on the same XCD, so they come from that
 // XCD's L2 instead of HBM, and so do the halo columns two neighbouring tiles
 // of a strip share.  Workgroups of inst = -1 only pad the shorter runs.
+int step_cm(const Engine *e);
+
 int get_tiles(Engine *e, int k, Engine::Tiles *out)
 {
     auto it = e->tiles.find(k);
@@ -352,11 +354,11 @@ int get_tiles(Engine *e, int k, Engine::Tiles *out)
     // slowest tile: there the short strips have 4 rows (at 1280x720, K = 4:
     // +30 % over 8), but at least K so that the next strip can be interior.
     int Lb = std::min(L, WV_LB_MAX);
-    // the kernel variant of this launch has the edge-column path (K = 8,
-    // 5-point, uniform medium): tiles of the outer columns take it, and so do
-    // the tiles of the first and last rows of grids not periodic along i
-    bool ej_on = k == 8 && e->stencil == 5 && WV_C == 1;
-    for (int q = 0; q < e->n; q++) ej_on = ej_on && e->g[q].coef == WV_COEF_UNIFORM;
+    // the kernel variant of this launch has the edge-column path: tiles of
+    // the outer columns take it, and so do the tiles of the first and last
+    // rows of grids not periodic along i (grids of one or two media)
+    bool ej_on = WV_C == 1 && wv_edgej_kernel(k, e->stencil, step_cm(e));
+    for (int q = 0; q < e->n; q++) ej_on = ej_on && e->g[q].coef != WV_COEF_FIELD;
     double tot = 0;  // launch work in interior-row units
     long ntiles = 0;  // one-wave tiles at strips of L rows
     {

@@ -103,7 +103,8 @@ struct WvInst {
 #define WV_TILE_NEU 8       // ... a Neumann ghost point (never an INTERIOR tile)
 #define WV_TILE_DRV 16      // ... a driven cell (never an INTERIOR tile)
 #define WV_TILE_PRB 32      // ... a probe (never an INTERIOR tile)
-#define WV_TILE_EDGEJ 64    // not INTERIOR only because its columns reach column 0 or NY-1
+#define WV_TILE_EDGEJ 64    // not INTERIOR only because its columns reach column 0 or NY-1, or
+                            // its rows row 0 or NX-1 of a grid not periodic along i
 #define WV_TILE_WIDE 128    // an INTERIOR tile computed by the WV_WG waves of one workgroup
 // One tile: output columns [jb, jb + width) of grid inst, rows [i0, i0 +
 // rows), with width = (flags >> WV_TILE_WIDTH_SHIFT) & 255 and rows = flags
@@ -121,6 +122,16 @@ struct WvTile {
 // waves per workgroup of the split-form stepping kernel: a WIDE tile, or up
 // to WV_WG one-wave tiles side by side (independent of each other)
 #define WV_WG 4
+
+// Stepping kernels with the edge-column path (tiles flagged WV_TILE_EDGEJ
+// evaluate their cells as interior cells and then their edge cells by the
+// edge formulas; elsewhere such tiles take the boundary path): fused depth 4
+// and more, the 5-point stencil, grids of one or two media (cm: the
+// kernel's coefficient modes)
+static constexpr bool wv_edgej_kernel(int k, int stencil, int cm)
+{
+    return k >= 4 && stencil == 5 && (cm == WV_COEF_UNIFORM || cm == WV_COEF_TWO || cm == WV_CM_ALL);
+}
 
 // number of output columns of one wave for a fused depth k
 static inline int wv_halo(int k) { return (k + WV_C - 1) / WV_C * WV_C; }

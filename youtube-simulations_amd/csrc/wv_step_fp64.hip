@@ -382,7 +382,7 @@ __device__ __forceinline__ void run_fp64(const WvInst &d, const WvTile t, long g
 // K = 8 only, like the split kernel's.
 template <int K, int ST, int CM>
 struct EdgeJ {
-    static constexpr bool on = K == 8 && ST == 5 && CM == WV_COEF_UNIFORM;
+    static constexpr bool on = wv_edgej_kernel(K, ST, CM);
 };
 template <int K, int CM>
 struct Occ {
@@ -435,8 +435,17 @@ k_step_fp64(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, cons
         } else run_fp64<K, ST, false, CF_UNIFORM, false, true, XT, TB>(d, t, gbase, ia, ib, oc, od);
         break;
     case WV_COEF_TWO:
-        if (!interior) run_fp64<K, ST, false, CF_TWO, false, true, XT, TB>(d, t, gbase, ia, ib, oc, od);
-        else WV_INTERIOR(CF_TWO)
+        if (interior) { WV_INTERIOR(CF_TWO) }
+        else if (EdgeJ<K, ST, CM>::on && edgej) {
+            const bool edgei = t.i0 < K || t.i0 + (t.flags >> WV_TILE_ROWS_SHIFT) > d.nx - K;
+            if (edgei) {
+                if (damp) run_fp64<K, ST, false, CF_TWO, false, true, false, false, true, true>(d, t, gbase, ia, ib, oc, od);
+                else run_fp64<K, ST, false, CF_TWO, false, false, false, false, true, true>(d, t, gbase, ia, ib, oc, od);
+            } else {
+                if (damp) run_fp64<K, ST, false, CF_TWO, false, true, false, false, true>(d, t, gbase, ia, ib, oc, od);
+                else run_fp64<K, ST, false, CF_TWO, false, false, false, false, true>(d, t, gbase, ia, ib, oc, od);
+            }
+        } else run_fp64<K, ST, false, CF_TWO, false, true, XT, TB>(d, t, gbase, ia, ib, oc, od);
         break;
     default:
         if (!interior) run_fp64<K, ST, false, CF_ANY, false, true, XT, TB>(d, t, gbase, ia, ib, oc, od);

@@ -481,21 +481,22 @@ __device__ __forceinline__ void run_split(const WvInst &d, const WvTile t, long 
 // Waves per SIMD the register allocation must allow: the rarely taken
 // boundary path (per-lane column mapping, edge formulas) would otherwise set
 // the register count of the whole kernel.
-// The edge-column variant is built into the uniform-medium 5-point kernel at
-// K = 8 only (elsewhere it would cost a resident wave per SIMD); there the
-// kernel fits three waves (168 registers, two spilled outside the loops).
+// The kernels with the edge-column variant at K = 8 (5-point, one or two
+// media) are held to three waves (168 registers): the variant alone would
+// take the two-medium kernel to 177 and two waves.
 // The 9-point kernels also run three waves at K = 8 (8192^2 smooth medium:
 // +7 % over two, with some spills on paths that medium does not take).
 template <int K, int ST, int CM>
 struct EdgeJ {
-    static constexpr bool on = K == 8 && ST == 5 && CM == WV_COEF_UNIFORM;
+    static constexpr bool on = wv_edgej_kernel(K, ST, CM);
 };
 template <int K, int ST, int CM>
 struct Occ {
 #ifdef WV_OCC_SPLIT
     static constexpr int waves = WV_OCC_SPLIT;
 #else
-    static constexpr int waves = K <= 4 ? 3 : ((EdgeJ<K, ST, CM>::on || ST == 9) ? 3 : 2);
+    static constexpr int waves =
+        K <= 4 ? 3 : (((K == 8 && ST == 5 && (CM == WV_COEF_UNIFORM || CM == WV_COEF_TWO)) || ST == 9) ? 3 : 2);
 #endif
 };
 
@@ -552,6 +553,7 @@ k_step_split(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, con
         break;
     case WV_COEF_TWO:
         if (interior) { WV_INTERIOR(CF_TWO) }
+        else if (EdgeJ<K, ST, CM>::on && edgej) { WV_EDGEJ(CF_TWO) }
         else run_split<K, ST, false, CF_TWO, false, true>(d, t, gbase, ia, oc, lg);
         break;
     default:
