@@ -36,8 +36,10 @@ VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 B_ALG = {"f32split": 24.125,   # read u_hi, u_lo, v + write them (3 x 4 B each way) + 1-bit mask
          "fp64": 24.125}       # read u^n, u^{n-1} + write u^{n+1} (3 x 8 B) + 1-bit mask
 METRIC = "Gcell-updates/s at 4096^2 (FP64 plain vs FP32 split) + % of HBM roofline"
-KERNEL_SOURCES = {"f32split": ["wv_step_split.hip", "wv_device.h", "wv_internal.h"],
-                  "fp64": ["wv_step_fp64.hip", "wv_device.h", "wv_internal.h"]}
+# sources a stored counter profile must match: the kernel and the host code
+# that builds its tile tables (tiling, order and strip lengths shape the launch)
+KERNEL_SOURCES = {"f32split": ["wv_step_split.hip", "wv_device.h", "wv_internal.h", "wv_api.cpp"],
+                  "fp64": ["wv_step_fp64.hip", "wv_device.h", "wv_internal.h", "wv_api.cpp"]}
 
 
 def compulsory_bytes_per_cell(form, k):
