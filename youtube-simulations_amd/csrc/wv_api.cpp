@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -105,6 +106,12 @@ struct Engine {
     bool dropin_tables = false;
     bool xcd_order = true;  // WAVEHIP_TILE_ORDER=linear turns the XCD interleave off
     bool slow_first = true;  // WAVEHIP_TILE_ORDER=inplace: slower tiles keep their place in the strip order
+    // split form: the full-depth launches of a step sequence as one persistent
+    // cooperative launch (WAVEHIP_PERSISTENT=1, one-wave tiles only; batches
+    // then step without a graph)
+    bool persistent = false;
+    int *d_pctr = nullptr;  // its work counters (8 per epoch)
+    long pctr_cap = 0;
     // split form: interior strips in wide tiles (WAVEHIP_WIDE=1); measured
     // slower than one-wave tiles on every workload, so off by default
     bool wide = false;
@@ -114,6 +121,7 @@ struct Engine {
     bool time_kernels = false;
     std::vector<hipEvent_t> kev;
     std::vector<int> kev_depth;  // fused depth of launch q
+    std::vector<int> kev_epochs;  // launches of that depth it stands for (persistent: several)
     size_t kev_used = 0;         // events recorded (the last one follows the step counter's bump)
     bool kev_pending = false;    // per-launch durations of the last timed run not yet read
     double last_avg_ms = 0, last_timed = 0;
@@ -698,6 +706,7 @@ int engine_create(Engine **out, int n, const wv_params *ps)
         e->slow_first = strcmp(o, "inplace") != 0;
     }
     if (const char *o = getenv("WAVEHIP_WIDE")) e->wide = atoi(o) != 0;
+    if (const char *o = getenv("WAVEHIP_PERSISTENT")) e->persistent = atoi(o) != 0;
     int ndev = 0;
     HIPCHK(e, hipGetDeviceCount(&ndev));
     if (e->device < 0 || e->device >= ndev) return fail(e, WV_EINVAL, "device %d not present (%d devices)", e->device, ndev);
@@ -748,7 +757,7 @@ void engine_destroy(Engine *e)
     for (hipEvent_t ev : e->kev) (void)hipEventDestroy(ev);
     for (auto &G : e->g) free_grid(G);
     void *ptrs[] = {e->d_inst, e->d_gstep, e->d_partial, e->d_eout, e->d_erows, e->d_row0,
-                    e->tf_rgb, e->tf_strips, e->tf_body, e->tf_lens, e->tf_off, e->d_pad};
+                    e->tf_rgb, e->tf_strips, e->tf_body, e->tf_lens, e->tf_off, e->d_pad, e->d_pctr};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -1493,18 +1502,35 @@ int reserve_kev(Engine *e, size_t n)
         HIPCHK(e, hipEventCreate(&ev));
         e->kev.push_back(ev);
         e->kev_depth.push_back(0);
+        e->kev_epochs.push_back(1);
     }
     return WV_OK;
 }
 
 // record timing event number q on the stepping stream (timed runs)
-int record_kev(Engine *e, int depth)
+int record_kev(Engine *e, int depth, int epochs = 1)
 {
     int rc;
     if ((rc = reserve_kev(e, e->kev_used + 1))) return rc;
     e->kev_depth[e->kev_used] = depth;
+    e->kev_epochs[e->kev_used] = epochs;
     HIPCHK(e, hipEventRecord(e->kev[e->kev_used], e->stream));
     e->kev_used++;
+    return WV_OK;
+}
+
+// the persistent launch's work counters for `epochs` epochs (grown, not
+// shrunk; wv_prepare sizes them before a timed run)
+int persist_counters(Engine *e, long epochs)
+{
+    const long n = 8 * epochs * WV_PCTR_PAD;
+    if (n <= e->pctr_cap) return WV_OK;
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    if (e->d_pctr) HIPCHK(e, hipFree(e->d_pctr));
+    e->d_pctr = nullptr;
+    e->pctr_cap = 0;
+    HIPCHK(e, hipMalloc(&e->d_pctr, sizeof(int) * n));
+    e->pctr_cap = n;
     return WV_OK;
 }
 
@@ -1521,6 +1547,20 @@ int enqueue_steps(Engine *e, long nsteps, bool force_k1, long *launches)
         const int k = next_depth(kmax, nsteps - done);
         Engine::Tiles t;
         if ((rc = get_tiles(e, k, &t))) return rc;
+        const long ep = (nsteps - done) / kmax;  // full-depth launches left
+        if (e->persistent && !e->wide && e->form == WV_FORM_F32SPLIT && !force_k1 && k == kmax && ep >= 2 &&
+            ep <= INT_MAX / 8) {
+            if ((rc = persist_counters(e, ep))) return rc;
+            int g = 0;
+            const int lr = wv_launch_split_persist(k, e->stencil, cm, e->d_inst, t.d, t.ni, e->d_gstep, done, (int)ep,
+                                                   e->scur, e->d_pctr, e->stream, &g);
+            if (lr) return fail(e, WV_EHIP, "persistent launch failed: %s", hipGetErrorString((hipError_t)lr));
+            for (long q = 0; q < ep; q++) advance_roles(e, k);
+            if (e->time_kernels && (rc = record_kev(e, k, (int)ep))) return rc;
+            done += ep * k;
+            nl += ep;
+            continue;
+        }
         const int lr = launch_depth(e, k, cm, t.d, t.ni, t.nb, done);
         advance_roles(e, k);
         if (lr) return fail(e, WV_EHIP, "kernel launch failed: %s", hipGetErrorString((hipError_t)lr));
@@ -1583,7 +1623,7 @@ int engine_step(Engine *e, long nsteps, bool force_k1)
     if (rc) return rc;
     HIPCHK(e, hipSetDevice(e->device));
     if (nsteps == 0) return WV_OK;
-    if (!e->use_graph || force_k1) {
+    if (!e->use_graph || force_k1 || e->persistent) {
         rc = enqueue_steps(e, nsteps, force_k1, &e->last_launches);
         return rc;
     }
@@ -1628,6 +1668,13 @@ int engine_prepare(Engine *e, long nsteps)
             seen[k] = true;
             const int lr = launch_depth(e, k, cm, e->d_pad, 1, 0, 0);
             if (lr) return fail(e, WV_EHIP, "kernel launch failed: %s", hipGetErrorString((hipError_t)lr));
+            if (e->persistent && !e->wide && e->form == WV_FORM_F32SPLIT && k == kmax) {
+                if ((rc = persist_counters(e, nsteps / kmax))) return rc;
+                int g = 0;
+                const int pr = wv_launch_split_persist(k, e->stencil, cm, e->d_inst, e->d_pad, 1, e->d_gstep, 0, 1,
+                                                       e->scur, e->d_pctr, e->stream, &g);
+                if (pr) return fail(e, WV_EHIP, "persistent launch failed: %s", hipGetErrorString((hipError_t)pr));
+            }
         }
         done += k;
         if (k == kmax) done += (nsteps - done) / kmax * kmax;
@@ -1640,7 +1687,7 @@ int engine_prepare(Engine *e, long nsteps)
     if ((rc = reserve_kev(e, (size_t)nl + 2))) return rc;
     // record each once: the runtime sets an event up on its first record
     for (size_t q = 0; q < (size_t)nl + 2; q++) HIPCHK(e, hipEventRecord(e->kev[q], e->stream));
-    if (e->use_graph) {
+    if (e->use_graph && !e->persistent) {
         hipGraphExec_t ex;
         if ((rc = step_graph(e, nsteps, &ex))) return rc;
     }
@@ -1988,7 +2035,7 @@ int kernel_times(Engine *e)
         float x = 0;
         HIPCHK(e, hipEventElapsedTime(&x, e->kev[q - 1], e->kev[q]));
         ti += x;
-        nt++;
+        nt += e->kev_epochs[q];
     }
     e->last_avg_ms = nt ? ti / nt : 0.0;
     e->last_timed = nt;

@@ -93,7 +93,9 @@ __global__ void k_bump(long *g, long n)
     int wv_launch_fp64_k##k(int, int, const WvInst *, const WvTile *, int, const WvTile *, int,          \
                             const long *, long, int, int, int, int, hipStream_t);                 \
     int wv_launch_split_k##k(int, int, const WvInst *, const WvTile *, int, const WvTile *, int,         \
-                             const long *, long, int, int, hipStream_t);
+                             const long *, long, int, int, hipStream_t);                        \
+    int wv_launch_split_persist_k##k(int, int, const WvInst *, const WvTile *, int, const long *, long, int, \
+                                     int, int *, hipStream_t, int *);
 WV_DEPTHS(WV_DECL)
 
 bool wv_have_depth(int k) { return k == 1 || k == 2 || k == 3 || k == 4 || k == 6 || k == 8; }
@@ -116,6 +118,18 @@ int wv_launch_split(int k, int stencil, int cm, const WvInst *I, const WvTile *T
     switch (k) {
 #define WV_CASE(kk) \
     case kk: return wv_launch_split_k##kk(stencil, cm, I, Ti, ni, Tb, nb, gs, g0, ia, oc, s);
+        WV_DEPTHS(WV_CASE)
+#undef WV_CASE
+    }
+    return -1;
+}
+
+int wv_launch_split_persist(int k, int stencil, int cm, const WvInst *I, const WvTile *T, int ng, const long *gs,
+                            long g0, int epochs, int ia0, int *ctr, hipStream_t s, int *grid)
+{
+    switch (k) {
+#define WV_CASE(kk) \
+    case kk: return wv_launch_split_persist_k##kk(stencil, cm, I, T, ng, gs, g0, epochs, ia0, ctr, s, grid);
         WV_DEPTHS(WV_CASE)
 #undef WV_CASE
     }

@@ -150,6 +150,15 @@ int wv_launch_fp64(int k, int stencil, int cm, const WvInst *d_inst, const WvTil
                    int nb, const long *gstep, long g0, int ia, int ib, int oc, int od, hipStream_t s);
 int wv_launch_split(int k, int stencil, int cm, const WvInst *d_inst, const WvTile *Ti, int ni, const WvTile *Tb,
                     int nb, const long *gstep, long g0, int ia, int oc, hipStream_t s);
+// Persistent split-form launch (wv_step_split.hip): `epochs` launches of
+// depth k over the ng workgroups of tile table T in one cooperative launch,
+// epoch e reading set (ia0 + e) & 1; ctr: 8 * epochs * WV_PCTR_PAD ints of
+// device memory (work counters, zeroed by the launch); *grid: workgroups
+// launched
+// ints per work counter (one counter per 128-byte line)
+#define WV_PCTR_PAD 32
+int wv_launch_split_persist(int k, int stencil, int cm, const WvInst *d_inst, const WvTile *T, int ng,
+                            const long *gstep, long g0, int epochs, int ia0, int *ctr, hipStream_t s, int *grid);
 bool wv_have_depth(int k);
 // energy: rows table (inst,row) pairs; partial[nrows_total]; out[ninst]
 int wv_launch_energy(int split, const WvInst *d_inst, const int2 *d_rows, int nrows,

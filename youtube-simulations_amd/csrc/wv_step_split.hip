@@ -19,6 +19,8 @@
 //
 // Same tile, skewed pipeline and prefetch ring as the FP64 kernel
 // (wv_step_fp64.hip); a window entry is (hi, lo, v) of one row.
+#include <hip/hip_cooperative_groups.h>
+
 #include "wv_device.h"
 
 #ifndef WV_K
@@ -500,26 +502,27 @@ struct Occ {
 #endif
 };
 
+// LDS of a workgroup: the exchange rows of a wide tile (see run_split)
+template <int K>
+struct GhostLds {
+    static constexpr int n = 3 * WV_WG * (K > 1 ? K - 1 : 1) * 2;
+};
+
+// Tile entry q of the table, computed by this wave (with the other waves of
+// its workgroup if the tile is wide); gbase is the amplitude index of the
+// first step.  q is wave-uniform.
 template <int K, int ST, int CM>
-__global__ void __launch_bounds__(64 * WV_WG, (Occ<K, ST, CM>::waves))
-k_step_split(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, const long *gstep,
-            long g0, int ia, int oc)
+__device__ __forceinline__ void step_tile(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, int q,
+                                          long gbase, int ia, int oc, lptr2 lg)
 {
-    // exchange rows of a wide tile (see run_split)
-    __shared__ fvec2 gh[3 * WV_WG * (K > 1 ? K - 1 : 1) * 2];
-    // (the wave index through a scalar register: the tile and everything
-    // derived from it stay wave-uniform)
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const WvTile t = tiles[blockIdx.x * WV_WG + wave];
+    const WvTile t = tiles[q];
     if (t.inst < 0) return;  // padding (one-wave tiles only: a wide tile fills its workgroup)
     const WvInst &d = I[t.inst];
-    const long gbase = (gstep ? *gstep : 0) + g0;
     const bool interior = t.flags & WV_TILE_INTERIOR;
     const bool allact = t.flags & WV_TILE_ALLACT;
     const bool edgej = t.flags & WV_TILE_EDGEJ;
     const bool wide = t.flags & WV_TILE_WIDE;
     const bool damp = d.damped;
-    const lptr2 lg = (lptr2)gh;
 #define WV_RUNW(CF, A, D, W) run_split<K, ST, true, CF, A, D, false, W>(d, t, gbase, ia, oc, lg)
 #define WV_RUN(CF, A, D)                  \
     if (wide) WV_RUNW(CF, A, D, WV_C == 1);    \
@@ -575,6 +578,67 @@ k_step_split(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, con
 #undef WV_RUNW
 }
 
+template <int K, int ST, int CM>
+__global__ void __launch_bounds__(64 * WV_WG, (Occ<K, ST, CM>::waves))
+k_step_split(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, const long *gstep,
+            long g0, int ia, int oc)
+{
+    __shared__ fvec2 gh[GhostLds<K>::n];
+    // (the wave index through a scalar register: the tile and everything
+    // derived from it stay wave-uniform)
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    step_tile<K, ST, CM>(I, tiles, blockIdx.x * WV_WG + wave, (gstep ? *gstep : 0) + g0, ia, oc, (lptr2)gh);
+}
+
+// Persistent form: `epochs` launches of depth K over the same tile table (of
+// one-wave tiles) in one cooperative launch of workgroups that all stay
+// resident.  Epoch e reads set (ia0 + e) & 1 and writes the other.  The table
+// lists XCD x's workgroups at positions 8j + x, longest first; a wave on XCD
+// x (its hardware XCC id) takes their tile entries one at a time, in that
+// order, from the counter ctr[8e + x] (zero before the launch), then helps
+// with the lists of the other XCDs in turn (so every entry is taken whatever
+// the partition mode).  Waves take tiles independently, as the hardware would
+// start them: a workgroup moving from one table workgroup to the next as a
+// whole would keep three waves waiting for the longest of four tiles.  A
+// grid-wide barrier (release / acquire at device scope) separates the
+// epochs.  Every wave runs every epoch to its end: each counter passes the
+// length of its list once, and then every wave reads a value past it.
+template <int K, int ST, int CM>
+__global__ void __launch_bounds__(64 * WV_WG, (Occ<K, ST, CM>::waves))
+k_step_split_persist(const WvInst *__restrict__ I, const WvTile *__restrict__ tiles, int ngroups,
+                     const long *gstep, long g0, int epochs, int ia0, int *ctr)
+{
+    __shared__ fvec2 gh[GhostLds<K>::n];
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    const long gb = (gstep ? *gstep : 0) + g0;
+    unsigned xcc;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+    const int x0 = (int)(xcc & 7);
+    const bool lead = (threadIdx.x & 63) == 0;
+    for (int e = 0; e < epochs; e++) {
+        const int ia = (ia0 + e) & 1;
+        for (int c = 0; c < 8; c++) {
+            const int x = (x0 + c) & 7;
+            const int ne = (ngroups - x + 7) / 8 * WV_WG;  // entries of table workgroups 8j + x
+            for (;;) {
+                int m = 0;
+                if (lead) m = atomicAdd(&ctr[(8 * e + x) * WV_PCTR_PAD], 1);
+                m = __builtin_amdgcn_readfirstlane(m);
+                if (m >= ne) break;
+                step_tile<K, ST, CM>(I, tiles, (8 * (m / WV_WG) + x) * WV_WG + m % WV_WG, gb + (long)e * K, ia, ia ^ 1,
+                                     (lptr2)gh);
+            }
+        }
+        grid.sync();
+    }
+}
+
+__global__ void k_zero_int(int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 }  // namespace
 
 #define WV_CAT2(a, b) a##b
@@ -605,4 +669,47 @@ int WV_CAT(wv_launch_split_k, WV_K)(int stencil, int cm, const WvInst *I, const 
 #undef WV_ST
 #undef WV_GO
     return (int)hipGetLastError();
+}
+
+// Persistent launch of `epochs` launches of depth WV_K over the tile table T
+// of ng workgroups: as many workgroups as stay resident together (a multiple
+// of 8), launched cooperatively, after ctr[0 .. 8 epochs) is zeroed.
+// *grid_out: the workgroups launched.
+int WV_CAT(wv_launch_split_persist_k, WV_K)(int stencil, int cm, const WvInst *I, const WvTile *T, int ng,
+                                             const long *gs, long g0, int epochs, int ia0, int *ctr,
+                                             hipStream_t s, int *grid_out)
+{
+    if (!ng || epochs <= 0) return 0;
+    const void *fn = nullptr;
+#define WV_FN(ST, CM) fn = (const void *)k_step_split_persist<WV_K, ST, CM>
+#define WV_ST(ST)                                 \
+    switch (cm) {                                 \
+    case WV_COEF_UNIFORM: WV_FN(ST, WV_COEF_UNIFORM); break; \
+    case WV_COEF_TWO: WV_FN(ST, WV_COEF_TWO); break;         \
+    case WV_COEF_FIELD: WV_FN(ST, WV_COEF_FIELD); break;     \
+    default: WV_FN(ST, WV_CM_ALL); break;                     \
+    }
+    if (stencil == 9) {
+        WV_ST(9)
+    } else {
+        WV_ST(5)
+    }
+#undef WV_ST
+#undef WV_FN
+    int dev = 0, cus = 0, per = 0;
+    hipError_t e;
+    if ((e = hipGetDevice(&dev)) || (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) ||
+        (e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, 64 * WV_WG, 0)))
+        return (int)e;
+    int g = per * cus;
+    g -= g % 8;
+    if (g < 8) return (int)hipErrorInvalidConfiguration;
+    const int nc = 8 * epochs * WV_PCTR_PAD;
+    hipLaunchKernelGGL(k_zero_int, dim3((nc + 255) / 256), dim3(256), 0, s, ctr, nc);
+    if ((e = hipGetLastError())) return (int)e;
+    void *args[] = {(void *)&I, (void *)&T, (void *)&ng, (void *)&gs, (void *)&g0, (void *)&epochs, (void *)&ia0,
+                    (void *)&ctr};
+    e = hipLaunchCooperativeKernel(fn, dim3(g), dim3(64 * WV_WG), args, 0, s);
+    if (grid_out) *grid_out = g;
+    return (int)e;
 }
