@@ -239,6 +239,13 @@ int wv_set_probes(wv_ctx *c, int n, const int *i, const int *j, long every, long
  * for s < *count (at most max_samples), then counting restarts after them;
  * *lost (may be NULL): samples that did not fit the capacity */
 int wv_get_probes(wv_ctx *c, double *out, long max_samples, long *count, long *lost);
+/* The lines SAVE_TIME_SERIES writes for one probe (wave_billiard.c:902-905:
+ * (long)(phi*1.0e16) printed with "%019ld\n"), for the samples v[s*stride],
+ * s < count (stride n picks probe k of wv_get_probes' out + k).  Writes at
+ * most cap bytes to out (no terminating zero) and the byte count to
+ * *written; WV_EINVAL if the lines do not fit (at most 21 bytes each).  Host
+ * code only: no context, no device. */
+int wv_format_time_series(const double *v, long count, long stride, char *out, long cap, long *written);
 
 /* Neighbour-table stepping, the reference's PRECOMPUTE_BC path:
  * evolve_wave_half_new (wave_3d.c:345-597) with the t_laplacian table built

@@ -711,6 +711,12 @@ int engine_create(Engine **out, int n, const wv_params *ps)
     HIPCHK(e, hipGetDeviceCount(&ndev));
     if (e->device < 0 || e->device >= ndev) return fail(e, WV_EINVAL, "device %d not present (%d devices)", e->device, ndev);
     HIPCHK(e, hipSetDevice(e->device));
+    // the tile order deals each XCD a run of the table, assuming 8 XCDs that
+    // take workgroups round-robin (the whole chip, 256 CUs); a partition with
+    // fewer CUs gets the grid order
+    int cus = 0;
+    HIPCHK(e, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device));
+    if (cus < 256) e->xcd_order = false;
     HIPCHK(e, hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     e->g.resize(n);
     int totrows = 0;
@@ -2927,6 +2933,26 @@ extern "C" int wv_get_probes(wv_ctx *c, double *out, long max_samples, long *cou
 {
     if (!c) return WV_EINVAL;
     return get_probes(c->e, 0, out, max_samples, count, lost);
+}
+
+// wave_billiard.c:902-905: the value scaled by 1e16 in binary64 and
+// truncated to a long, then "%019ld\n" (19 characters, sign included, zero
+// padded)
+extern "C" int wv_format_time_series(const double *v, long count, long stride, char *out, long cap, long *written)
+{
+    if (written) *written = 0;
+    if (count < 0 || (count > 0 && (!v || !out || stride < 1))) return WV_EINVAL;
+    long pos = 0;
+    char line[32];
+    for (long s = 0; s < count; s++) {
+        const long w = (long)(v[s * stride] * 1.0e16);
+        const int m = snprintf(line, sizeof line, "%019ld\n", w);
+        if (m <= 0 || pos + m > cap) return WV_EINVAL;
+        memcpy(out + pos, line, (size_t)m);
+        pos += m;
+    }
+    if (written) *written = pos;
+    return WV_OK;
 }
 
 extern "C" int wv_set_laplacian(wv_ctx *c, const int *nneighb, const int *nghb)

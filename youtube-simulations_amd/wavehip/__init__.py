@@ -32,7 +32,7 @@ EXPORTS = [
     "wv_params_default", "wv_create", "wv_destroy", "wv_last_error", "wv_set_tables",
     "wv_set_state", "wv_set_state3", "wv_get_state", "wv_get_state3", "wv_set_sources",
     "wv_set_neumann", "wv_set_drive", "wv_set_floor", "wv_set_laplacian", "wv_set_probes", "wv_get_probes",
-    "wv_set_wells_tables", "wv_build_mask", "wv_build_mask_static", "wv_init_circular_wave",
+    "wv_format_time_series", "wv_set_wells_tables", "wv_build_mask", "wv_build_mask_static", "wv_init_circular_wave",
     "wv_add_circular_wave", "wv_batch_init_circular_wave", "wv_batch_add_circular_wave", "wv_tiff_bound", "wv_tiff_encode", "wv_tiff_write",
     "wv_step", "wv_prepare", "wv_step_timed", "wv_sync", "wv_steps_done", "wv_last_kernel_ms", "wv_energy",
     "wv_get_tiling", "wv_get_tile_counts", "wv_frame_params_default", "wv_frame_set_tables", "wv_frame_reset", "wv_frame",
@@ -82,6 +82,19 @@ _I = ctypes.POINTER(ctypes.c_int)
 _DD = ctypes.POINTER(_D)
 _SS = ctypes.POINTER(_S)
 _lib = None
+
+
+def format_time_series(values):
+    """the SAVE_TIME_SERIES text of one probe's samples (wave_billiard.c:902-905)"""
+    v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+    cap = 21 * max(len(v), 1)
+    buf = ctypes.create_string_buffer(cap)
+    n = ctypes.c_long(0)
+    L = load()
+    rc = L.wv_format_time_series(_d(v), len(v), 1, buf, cap, ctypes.byref(n))
+    if rc:
+        raise WaveError("wv_format_time_series: %d" % rc)
+    return buf.raw[:n.value].decode()
 
 
 def load():
@@ -142,6 +155,8 @@ def load():
         "wv_frame": (ctypes.c_int, [vp, ctypes.POINTER(WvFrameParams), ctypes.POINTER(ctypes.c_float)]),
         "wv_frame_rgba8": (ctypes.c_int, [vp, ctypes.POINTER(WvFrameParams), ctypes.POINTER(ctypes.c_ubyte)]),
         "wv_get_tiling": (ctypes.c_int, [vp, _I, _I, _I]),
+        "wv_format_time_series": (ctypes.c_int, [_D, ctypes.c_long, ctypes.c_long, ctypes.c_char_p, ctypes.c_long,
+                                                 ctypes.POINTER(ctypes.c_long)]),
         "wv_get_tile_counts": (ctypes.c_int, [vp, ctypes.c_int, _I, _I, _I, ctypes.POINTER(ctypes.c_long)]),
         "wv_set_tiling": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
         "wv_autotune": (ctypes.c_int, [vp, ctypes.c_long, _D]),
