@@ -137,9 +137,10 @@ def frame_bench(wh, w, reps=10):
 def media_bench(wh, n=4096, nwells=100):
     """init_ior_2d's IOR_RANDOM_WELLS media for an n x n grid (10 x 10 wells,
     the checked-in NGRIDX, NGRIDY) built by wv_set_wells_tables, wall clock
-    including the table upload, and the D_CIRCLES mask of the precision
-    workload built by wv_build_mask; beside each the oracle (OpenMP, all host
-    threads) on a band of rows of the same grid, scaled to the grid.  Also a
+    including the table upload, the D_CIRCLES mask of the precision workload
+    built by wv_build_mask, and the IOR_MANDELBROT media and D_MANDELBROT mask
+    (wv_set_ior_tables, wv_build_domain_mask); beside each the oracle (OpenMP,
+    all host threads) on a band of rows of the same grid, scaled to the grid.  Also a
     1920x1080 frame saved as writetiff saves it (LZW strips coded on the
     device) beside the oracle's coder on one host thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -197,11 +198,45 @@ def media_bench(wh, n=4096, nwells=100):
     mo = pyoracle.mask(rows, n, band, wh.D_CIRCLES, 0.0, 0.0, circ)
     mask_cpu_ms = (time.perf_counter() - t0) * 1e3 * n / rows
     assert np.array_equal(m[:rows], mo)
+    # init_ior_2d's IOR_MANDELBROT media (escape time to MANDELLEVEL 1000, the
+    # checked-in value; wall clock including the tables' copy to the host and
+    # their install) and the D_MANDELBROT domain mask; the oracle on the band
+    # of rows through the middle of the set (dyadic grid: the band's cell
+    # corners are the grid's)
+    i0 = n // 2 - rows // 2
+    mid = (-2.0 + 4.0 * i0 / n, -2.0 + 4.0 * (i0 + rows) / n, -2.0, 2.0)
+    ip = wh.ior_params(courant=0.1, courantb=0.05, lam=0.8, mu=0.3)
+    dp = wh.domain_params()
+    W = wh.Wave(dict(nx=n, ny=n, bcond=2))
+    W.set_ior_tables(xy, 1, box, ip, copy=False)  # first call: code load
+    t0 = time.perf_counter()
+    tm = W.set_ior_tables(xy, 1, box, ip)
+    ior_ms = (time.perf_counter() - t0) * 1e3
+    W.build_domain_mask(24, box, dp)
+    t0 = time.perf_counter()
+    dm = W.build_domain_mask(24, box, dp)
+    dom_ms = (time.perf_counter() - t0) * 1e3
+    W.close()
+    oip = pyoracle.IorParams(**{f: getattr(ip, f) for f, _ in ip._fields_})
+    odp = pyoracle.DomainParams(**{f: getattr(dp, f) for f, _ in dp._fields_})
+    t0 = time.perf_counter()
+    to = pyoracle.ior_tables(pyoracle.make_params(rows, n), xy[:rows], 1, mid, oip)
+    ior_cpu_ms = (time.perf_counter() - t0) * 1e3 * n / rows
+    t0 = time.perf_counter()
+    do = pyoracle.domain(rows, n, mid, 24, odp)
+    dom_cpu_ms = (time.perf_counter() - t0) * 1e3 * n / rows
+    assert np.array_equal(dm[i0:i0 + rows], do)
+    assert np.all(np.abs(tm[0][i0:i0 + rows] - to[0]) <= 0.01 * 1e-15)
     return {"workload": "%dx%d IOR_RANDOM_WELLS, %d wells, tanh" % (n, n, xc.size), "gpu_ms": gpu_ms,
             "cpu_oracle_ms_scaled": cpu_ms, "cpu_threads": pyoracle.num_threads(),
             "cpu_sample": "%d of %d rows" % (rows, n),
             "mask": {"workload": "%dx%d D_CIRCLES mask, 64 discs, int16 copied to the host" % (n, n),
                      "gpu_ms": mask_ms, "cpu_oracle_ms_scaled": mask_cpu_ms},
+            "ior_mandelbrot": {"workload": "%dx%d IOR_MANDELBROT media, MANDELLEVEL 1000, tables copied to the host "
+                                           "and installed" % (n, n), "gpu_ms": ior_ms,
+                               "cpu_oracle_ms_scaled": ior_cpu_ms, "cpu_sample": "rows %d..%d" % (i0, i0 + rows)},
+            "domain_mandelbrot": {"workload": "%dx%d D_MANDELBROT mask, MANDELLEVEL 1000, int16 copied to the host"
+                                              % (n, n), "gpu_ms": dom_ms, "cpu_oracle_ms_scaled": dom_cpu_ms},
             "tiff": {"workload": "1920x1080 RGB frame to an LZW TIFF file image (one row a strip), host rows in, "
                                  "file bytes out", "gpu_ms": tiff_ms, "cpu_oracle_ms": tiff_cpu_ms,
                      "cpu_threads": 1, "file_bytes": len(tf)}}

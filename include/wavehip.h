@@ -168,6 +168,116 @@ int wv_build_mask(wv_ctx *c, int domain, const double box[4], double lambda, dou
 int wv_build_mask_static(wv_ctx *c, int domain, const double box[4], double lambda, double mu, int ncircles,
                          const double *circles, short *xy_in);
 
+/* The macros xy_in_billiard_single_domain (sub_wave.c:5712) reads besides
+ * LAMBDA and MU, by their reference names (wave_billiard.c:73-109), and the
+ * Julia parameter (julia_x, julia_y: global_pdes.c:567). */
+typedef struct wv_domain_params {
+    double lambda, mu, mu_b;          /* LAMBDA, MU, MU_B                       */
+    double apoly;                     /* APOLY: turn, in units of pi/2          */
+    double wall_width, wall_width_b;  /* WALL_WIDTH, WALL_WIDTH_B               */
+    double julia_scale;               /* JULIA_SCALE                            */
+    double julia_x, julia_y;          /* julia_x, julia_y                       */
+    double mandellimit;               /* MANDELLIMIT                            */
+    int npoly;                        /* NPOLY (1..4096)                        */
+    int mdepth, mratio;               /* MDEPTH (0..30), MRATIO (>= 1)          */
+    int mandellevel;                  /* MANDELLEVEL                            */
+    int ngridx, ngridy;               /* NGRIDX, NGRIDY (D_DISK_GRID, D_DISK_HEX) */
+} wv_domain_params;
+
+/* The billiard mask of init_wave_flat (wave_common.c:294-319) on the GPU for
+ * the context's (first) grid, as wv_build_mask, for every domain of
+ * xy_in_billiard_single_domain (sub_wave.c:5712-7602) that is a function of
+ * (x, y) and the parameters alone.  Codes are the reference's B_DOMAIN codes
+ * (global_pdes.c:14-236):
+ *   D_NOTHING 999, D_RECTANGLE 0, D_ELLIPSE 1, D_CIRCLE_LAYERS 196,
+ *   D_EXT_ELLIPSE 199, D_EXT_ELLIPSE_CURVED 198, D_EXT_ELLIPSE_CURVED_BDRY 197,
+ *   D_STADIUM 2, D_SINAI 3, D_DIAMOND 4, D_TRIANGLE 5, D_FLAT 6, D_ANNULUS 7,
+ *   D_POLYGON 8, D_YOUNG 9, D_GRATING 10, D_EHRENFEST 11, D_DISK_GRID 12,
+ *   D_DISK_HEX 13, D_MENGER 15, D_JULIA_INT 16, D_MENGER_ROTATED 17,
+ *   D_PARABOLA 18, D_TWO_PARABOLAS 19, D_TWO_PARABOLAS_ASYM 191,
+ *   D_TWO_PARABOLAS_CLOSED_SEMITRANS 195, D_TWO_PARABOLAS_CLOSED_LENS 1951,
+ *   D_TWO_PARABOLAS_CLOSED_LENS_R 1952, D_CIRCLES 20, D_CIRCLES_IN_RECT 201,
+ *   D_CIRCLES_NEUMANN 202, D_MANDELBROT 24, D_JULIA 25, D_MANDELBROT_CIRCLE 26,
+ *   D_FOUR_PARABOLAS 31, D_POLY_PARABOLAS 32, D_PENROSE 33, D_HYPERBOLA 34,
+ *   D_LSHAPE 50, D_WAVEGUIDE 51, D_WAVEGUIDES_COUPLED 522, D_WAVEGUIDE_S 523,
+ *   D_WAVEGUIDE_S_SHORT 524, D_WAVEGUIDES_COUPLED_N 525, D_LENS 61,
+ *   D_LENS_WALL 62, D_TWO_LENSES_WALL 63, D_TWO_LENSES_OBSTACLE 64,
+ *   D_LENS_ROTATED 67, D_LENS_CONCAVE 68, D_LENS_CONVEX_CONCAVE 69, D_TREE 73,
+ *   D_MICHELSON 74, D_GRADIENT_INDEX_LENS 76, D_TWOCIRCLES 79,
+ *   D_POLYCIRCLES 791, D_TWO_ELLIPSES 795, D_TWO_RECTANGLES 796,
+ *   D_CARDIOID 90, D_NEPHROID 91.
+ * Values are the reference's (0, 1 and, for some domains, 2).  The circle
+ * list (4*ncircles doubles, as wv_build_mask) is read by the D_CIRCLES
+ * family.  Cosines and sines of fixed angles (polygon sides, PI/6 of D_TREE)
+ * are taken on the host with the C library, as the reference does; the
+ * per-cell arithmetic is IEEE binary64 without contraction, so masks equal
+ * the reference's cell for cell.  Other codes return WV_EUNSUPPORTED (they
+ * read lists that other reference functions fill: polygons, mazes, images,
+ * random lattices). */
+int wv_build_domain_mask(wv_ctx *c, int domain, const double box[4], const wv_domain_params *p, int ncircles,
+                         const double *circles, short *xy_in);
+
+/* The macros init_ior_2d (sub_wave.c:12164-13004) reads, by reference name. */
+typedef struct wv_ior_params {
+    double courant, courantb;   /* COURANT, COURANTB                          */
+    double gamma, gammab;       /* GAMMA, GAMMAB                              */
+    double lambda, mu;          /* LAMBDA, MU                                 */
+    double mandel_ior_scale;    /* MANDEL_IOR_SCALE                           */
+    double mandellimit;         /* MANDELLIMIT                                */
+    int mandellevel;            /* MANDELLEVEL                                */
+    int mdepth;                 /* MDEPTH (IOR_LUNEBURG_LAYERS, >= 1)         */
+} wv_ior_params;
+
+/* Media codes of wv_set_ior_tables: the reference's IOR codes
+ * (global_pdes.c:239-264), and 0 for VARIABLE_IOR == 0 */
+#define WV_IOR_CONSTANT            0
+#define WV_IOR_MANDELBROT          1
+#define WV_IOR_MANDELBROT_LIN      100
+#define WV_IOR_EARTH               2
+#define WV_IOR_LENS_WALL           10
+#define WV_IOR_LENS_OBSTACLE       11
+#define WV_IOR_LENS_CONCAVE        12
+#define WV_IOR_LENS_CONVEX_CONCAVE 13
+#define WV_IOR_TREE                14
+#define WV_IOR_WAVE_GUIDES_COUPLED   15
+#define WV_IOR_WAVE_GUIDES_COUPLED_B 151
+#define WV_IOR_WAVE_GUIDE_COATED   16
+#define WV_IOR_MICHELSON           17
+#define WV_IOR_GRADIENT_INDEX_LENS   18
+#define WV_IOR_GRADIENT_INDEX_LENS_B 181
+#define WV_IOR_LUNEBURG_LENS       182
+#define WV_IOR_LUNEBURG_LAYERS     183
+#define WV_IOR_LINEAR_X_A          19
+#define WV_IOR_LINEAR_X_B          191
+
+/* init_ior_2d (sub_wave.c:12164) computed on the GPU and installed as by
+ * wv_set_tables: tcc and tgamma of every cell by the model's rule at the cell
+ * corner (x, y) of ij_to_xy (sub_wave.c:469-476) over box, reading xy_in
+ * where the model does (lens, tree, wave-guide, Michelson models); then tc =
+ * sqrt(tcc) where xy_in != 0, COURANTB where TWOSPEEDS (sub_wave.c:12970-12981).
+ * WV_IOR_CONSTANT is the VARIABLE_IOR == 0 branch (12984-13003: COURANT,
+ * COURANT^2, GAMMA or GAMMAB by xy_in; COURANTB, COURANTB^2, GAMMAB on the
+ * outside cells when TWOSPEEDS).  Cells the reference leaves unwritten get 0.
+ * The tables are copied to tcc / tc / tgamma (NX*NY) when not NULL.  All
+ * arithmetic is binary64 without contraction: tables equal the reference's
+ * except IOR_MANDELBROT's log (the device library's, within an ulp or two of
+ * glibc's; the escape count itself is exact).  Returns WV_EUNSUPPORTED for
+ * the Gaussian-well models 4-9 (wv_set_wells_tables, with wv_rotating_wells
+ * for 6 and 7), for IOR_EXPLO_LENSING (3: the reference indexes its row
+ * pointer array xy_in[i*NY+j], sub_wave.c:12292, past its NX entries) and for
+ * codes init_ior_2d does not handle. */
+int wv_set_ior_tables(wv_ctx *c, const short *xy_in, int ior, const double box[4], const wv_ior_params *p,
+                      double *tcc, double *tc, double *tgamma);
+
+/* The well centres of IOR_PERIODIC_WELLS_ROTATING (large = 0) and
+ * IOR_PERIODIC_WELLS_ROTATING_LARGE (large = 1), sub_wave.c:12390-12481, at
+ * the rotation angle ior_angle: ngridx*ngridy centres into xc, yc (index
+ * n = j*ngridx + i) and sigma_stat into *sigma.  Host code, no device; pass
+ * them to wv_set_wells_tables (height NULL, use_tanh 0) to rebuild the media
+ * every REFRESH_IOR frame (wave_billiard.c:939-945) on the GPU. */
+int wv_rotating_wells(int ngridx, int ngridy, int large, double ior_angle, const double box[4], double *xc,
+                      double *yc, double *sigma);
+
 /* ---- TIFF frames: writetiff, sub_wave.c:62-123 ----------------------------
  * The file writetiff(filename, description, 0, 0, width, height, compression)
  * writes through libtiff (4.x): RGB, 8 bits a sample, one row per strip,

@@ -153,6 +153,31 @@ pids+=($!)
 build_one "ior_wells_r" 4 -DNX=40 -DNY=30 -DIOR=5 -DNGRIDX=6 -DNGRIDY=5 -DTWOSPEEDS=1 -DCOURANTB=0.05 &
 pids+=($!)
 for p in "${pids[@]}"; do wait "$p"; done; pids=()
+# the per-cell media models of init_ior_2d (sub_wave.c:12181-12968), one
+# build per IOR code (VARIABLE_IOR 0: the constant branch 12984-13003), and
+# the rotating wells (12390-12481) for the REFRESH_IOR rebuild
+IORP="-DNX=97 -DNY=61 -DLAMBDA=0.8 -DMU=0.3 -DCOURANT=0.1 -DCOURANTB=0.05 -DGAMMA=1.0e-4 -DGAMMAB=2.0e-4"
+for m in 1 100 2 10 11 12 13 14 15 151 16 17 18 181 182 183 19 191; do
+    build_one "ior_m${m}" 4 $IORP -DIOR=$m -DTWOSPEEDS=$(( m % 2 )) -DMDEPTH=6 &
+    pids+=($!)
+    [ ${#pids[@]} -ge 8 ] && { for p in "${pids[@]}"; do wait "$p"; done; pids=(); }
+done
+build_one "ior_const_ts0" 4 $IORP -DVARIABLE_IOR=0 -DTWOSPEEDS=0 &
+pids+=($!)
+build_one "ior_const_ts1" 4 $IORP -DVARIABLE_IOR=0 -DTWOSPEEDS=1 &
+pids+=($!)
+build_one "ior_rot" 4 -DNX=40 -DNY=30 -DIOR=6 -DNGRIDX=6 -DNGRIDY=5 -DTWOSPEEDS=0 -DCOURANTB=0.05 &
+pids+=($!)
+build_one "ior_rot_large" 4 -DNX=40 -DNY=30 -DIOR=7 -DNGRIDX=6 -DNGRIDY=5 -DTWOSPEEDS=1 -DCOURANTB=0.05 &
+pids+=($!)
+for p in "${pids[@]}"; do wait "$p"; done; pids=()
+# every closed-form domain of xy_in_billiard_single_domain, two parameter sets
+build_one "dom_a" 5 -DNX=97 -DNY=61 -DLAMBDA=0.6 -DMU=0.3 -DNPOLY=6 -DAPOLY=0.5 &
+pids+=($!)
+build_one "dom_b" 5 -DNX=160 -DNY=90 -DLAMBDA=1.2 -DMU=0.15 -DNPOLY=9 -DAPOLY=1.0 -DMDEPTH=3 -DJULIA_SCALE=1.0 \
+    -DMANDELLEVEL=200 -DWALL_WIDTH=0.075 -DMU_B=0.3 -DWALL_WIDTH_B=0.15 -DNGRIDX=8 -DNGRIDY=6 &
+pids+=($!)
+for p in "${pids[@]}"; do wait "$p"; done; pids=()
 # the billiard masks of init_wave_flat (xy_in_billiard_single_domain): the
 # Sinai and exterior-ellipse domains of the plain workloads and D_CIRCLES
 build_one "mask_97x61_l03" 5 -DNX=97 -DNY=61 -DLAMBDA=0.3 -DMU=0.1 &

@@ -1,7 +1,9 @@
 /*
- * media_oracle.c -- CPU oracle of the Gaussian-well media tables and of the
- * billiard masks (test infrastructure, see wave_oracle.h).  Same flags as
- * wave_oracle.c; exp, tanh and sqrt are glibc's, as for the reference.
+ * media_oracle.c -- CPU oracle of init_ior_2d's media tables (Gaussian
+ * wells, rotating wells, the per-cell models) and of the billiard masks of
+ * xy_in_billiard_single_domain (test infrastructure, see wave_oracle.h).
+ * Same flags as wave_oracle.c; exp, tanh, log, sqrt, cos and sin are
+ * glibc's, as for the reference.
  */
 #include <math.h>
 #include <stddef.h>
@@ -102,4 +104,633 @@ void orc_circular_wave(const orc_params *p, int mode, double factor, double x0, 
                 phi[k] += amp * factor * exp(-dist2 / variance) * cos(-sqrt(dist2) / wavelength);
             }
         }
+}
+
+/* ---- closed-form domains of xy_in_billiard_single_domain ------------------
+ * One function per family, each in the reference's shape (same tests in the
+ * same order, cosines and sines of the polygon angles taken per point as the
+ * reference takes them); sub_wave.c line numbers at each case. */
+
+#define O_PI 3.141592654  /* global_pdes.c:8-10 */
+#define O_DPI 6.283185307
+#define O_PID 1.570796327
+
+static double o_abs(double x) { return x < 0.0 ? -x : x; }            /* vabs 270 */
+static double o_mod(double x, double y) { return sqrt(x * x + y * y); } /* module2 279 */
+
+static int o_tri(double x, double y, const double z1[2], const double z2[2], const double z3[2]) /* 5299 */
+{
+    double v1 = (z2[0] - z1[0]) * (y - z1[1]) - (z2[1] - z1[1]) * (x - z1[0]);
+    double v2 = (z3[0] - z2[0]) * (y - z2[1]) - (z3[1] - z2[1]) * (x - z2[0]);
+    double v3 = (z1[0] - z3[0]) * (y - z3[1]) - (z1[1] - z3[1]) * (x - z3[0]);
+    return (v1 >= 0.0) && (v2 >= 0.0) && (v3 >= 0.0);
+}
+
+static int o_escapes(double u, double v, double cx, double cy, const orc_domain_params *d)
+{
+    int i = 0;
+    while ((i < d->mandellevel) && (u * u + v * v < 1000.0 * d->mandellimit)) {
+        double u1 = u * u - v * v + cx;
+        v = 2.0 * u * v + cy;
+        u = u1;
+        i++;
+    }
+    return !(u * u + v * v < d->mandellimit);
+}
+
+static int o_domain_at(double x, double y, int dom, const orc_domain_params *d, const double box[4], int nc,
+                       const double *circ)
+{
+    const double LAMBDA = d->lambda, MU = d->mu, WALL_WIDTH = d->wall_width;
+    const double XMIN = box[0], XMAX = box[1], YMIN = box[2], YMAX = box[3];
+    double x1, y1, x2, y2, r, r2, a, b, c, width, angle, omega;
+    int i, j, k, condition;
+    switch (dom) {
+    case 999: return 1;
+    case 0: return (o_abs(x) < LAMBDA) && (o_abs(y) < 1.0);                          /* 5726 */
+    case 1: return x * x / (LAMBDA * LAMBDA) + y * y < 1.0;                          /* 5732 */
+    case 196: return x * x + y * y < LAMBDA * LAMBDA;                                /* 5738 */
+    case 199: return x * x / (LAMBDA * LAMBDA) + y * y / (MU * MU) > 1.0;            /* 5744 */
+    case 198: case 197:                                                              /* 5750, 5757 */
+        if (dom == 197 && y > YMAX - 0.05) return 0;
+        if (dom == 197 && y < YMIN + 0.05) return 0;
+        y1 = y + 0.4 * x * x;
+        return x * x / (LAMBDA * LAMBDA) + y1 * y1 / (MU * MU) > 1.0;
+    case 2:                                                                          /* 5766 */
+        if ((x > -0.5 * LAMBDA) && (x < 0.5 * LAMBDA) && (y > -1.0) && (y < 1.0)) return 1;
+        else if (o_mod(x + 0.5 * LAMBDA, y) < 1.0) return 1;
+        else if (o_mod(x - 0.5 * LAMBDA, y) < 1.0) return 1;
+        return 0;
+    case 3: return x * x + y * y > LAMBDA * LAMBDA;                                  /* 5774 */
+    case 4: {                                                                        /* 5780 */
+        double l2 = LAMBDA * LAMBDA;
+        r2 = l2 + (LAMBDA - 1.0) * (LAMBDA - 1.0);
+        return (x * x + y * y < 1.0) && ((x - LAMBDA) * (x - LAMBDA) + (y - LAMBDA) * (y - LAMBDA) > r2) &&
+               ((x - LAMBDA) * (x - LAMBDA) + (y + LAMBDA) * (y + LAMBDA) > r2) &&
+               ((x + LAMBDA) * (x + LAMBDA) + (y - LAMBDA) * (y - LAMBDA) > r2) &&
+               ((x + LAMBDA) * (x + LAMBDA) + (y + LAMBDA) * (y + LAMBDA) > r2);
+    }
+    case 5: return (x > -LAMBDA) && (y > -1.0) && (LAMBDA * y + x < 0.0);            /* 5791 */
+    case 6: return y > -LAMBDA;                                                      /* 5797 */
+    case 7: {                                                                        /* 5803 */
+        double l2 = LAMBDA * LAMBDA;
+        r2 = x * x + y * y;
+        return (r2 > l2) && (r2 < 1.0);
+    }
+    case 8:                                                                          /* 5810 */
+        condition = 1;
+        omega = O_DPI / ((double)d->npoly);
+        c = LAMBDA * cos(omega * 0.5);
+        for (k = 0; k < d->npoly; k++) {
+            angle = d->apoly * O_PID + (k + 0.5) * omega;
+            condition = condition * (x * cos(angle) + y * sin(angle) < c);
+        }
+        return condition;
+    case 9:                                                                          /* 5823 */
+        if ((x < -MU) || (x > MU)) return 1;
+        return (o_abs(y - LAMBDA) < MU) || (o_abs(y + LAMBDA) < MU);
+    case 10: {                                                                       /* 5829 */
+        int k1 = -(int)((-YMIN) / LAMBDA), k2 = (int)(YMAX / LAMBDA);
+        condition = 1;
+        for (i = k1; i <= k2; i++) {
+            double z = (double)i * LAMBDA;
+            condition = condition * (x * x + (y - z) * (y - z) > MU * MU);
+        }
+        return condition;
+    }
+    case 11:                                                                         /* 5842 */
+        return ((x - 1.0) * (x - 1.0) + y * y < LAMBDA * LAMBDA) || ((x + 1.0) * (x + 1.0) + y * y < LAMBDA * LAMBDA) ||
+               ((o_abs(x) < 1.0) && (o_abs(y) < MU));
+    case 12: case 13: {                                                              /* 5846, 5858 */
+        double dy = (YMAX - YMIN) / ((double)d->ngridy);
+        for (i = -d->ngridx / 2; i < d->ngridx / 2; i++)
+            for (j = (dom == 13 ? -1 : 0); j < d->ngridy; j++) {
+                x1 = ((double)i + 0.5) * dy;
+                y1 = YMIN + ((double)j + 0.5) * dy;
+                if (dom == 13 && (i + d->ngridx) % 2 == 1) y1 += 0.5 * dy;
+                if ((x - x1) * (x - x1) + (y - y1) * (y - y1) < MU * MU) return 0;
+            }
+        return 1;
+    }
+    case 15:                                                                         /* 7431 */
+        x1 = 0.5 * (x + 1.0);
+        y1 = 0.5 * (y + 1.0);
+        for (k = 0; k < d->mdepth; k++) {
+            x1 = x1 * (double)d->mratio;
+            y1 = y1 * (double)d->mratio;
+            if ((o_abs(x) < 1.0) && (o_abs(y) < 1.0) && (((int)x1 % d->mratio) == d->mratio / 2) &&
+                (((int)y1 % d->mratio) == d->mratio / 2))
+                return 0;
+        }
+        return 1;
+    case 17:                                                                         /* 7458 */
+        x2 = 1.0 * (x + y);
+        y2 = 1.0 * (x - y);
+        if ((o_abs(x2) < 1.0) && (o_abs(y2) < 1.0)) {
+            x1 = 0.5 * (x2 + 1.0);
+            y1 = 0.5 * (y2 + 1.0);
+            for (k = 0; k < d->mdepth; k++) {
+                x1 = x1 * (double)d->mratio;
+                y1 = y1 * (double)d->mratio;
+                if ((o_abs(x) < 1.0) && (o_abs(y) < 1.0) && (((int)x1 % d->mratio) == d->mratio / 2) &&
+                    (((int)y1 % d->mratio) == d->mratio / 2))
+                    return 0;
+            }
+        }
+        return 1;
+    case 16: return !o_escapes(x / d->julia_scale, y / d->julia_scale, d->julia_x, d->julia_y, d);  /* 7443 */
+    case 18: return x > 0.25 * y * y / LAMBDA - LAMBDA;                              /* 5872 */
+    case 19:                                                                         /* 5876 */
+        x1 = 0.25 * y * y / MU - MU - LAMBDA;
+        x2 = -x1;
+        width = 0.25 * MU;
+        if (width > 0.2) width = 0.2;
+        if (o_abs(y) > 1.5 * MU) return 1;
+        else if ((x < x1 - width) || (x > x2 + width)) return 1;
+        else if ((x > x1) && (x < x2)) return 1;
+        return 0;
+    case 191:                                                                        /* 5887 */
+        if (x > 0.0) {
+            if (o_abs(y) > 0.5 * LAMBDA) return 1;
+            x1 = -y * y / LAMBDA + 0.25 * LAMBDA;
+            return (x < x1) || (x > x1 + WALL_WIDTH);
+        }
+        if (o_abs(y) > 0.5 * MU) return 1;
+        x1 = y * y / MU - 0.25 * MU;
+        return (x > x1) || (x < x1 - WALL_WIDTH);
+    case 195:                                                                        /* 5977 */
+        if (o_abs(y) > 0.75 + WALL_WIDTH) return 1;
+        x1 = 0.5 * LAMBDA + 0.25 * MU - y * y / MU;
+        if ((o_abs(y) > 0.75) && (o_abs(x) < x1 + WALL_WIDTH)) return 2;
+        if ((o_abs(x) < x1) || (o_abs(x) > x1 + WALL_WIDTH)) return 1;
+        if ((x > 0.0) && (o_abs(y) < 0.2)) return 0;
+        return 2;
+    case 1951: case 1952: {                                                          /* 5986, 6006 */
+        double ylens = dom == 1951 ? 0.2 : d->wall_width_b;
+        if (o_abs(y) > 0.75 + WALL_WIDTH) return 1;
+        if ((x > 0.0) && (o_abs(y) < ylens)) {
+            double x3;
+            y1 = ylens;
+            x1 = 0.5 * LAMBDA + 0.25 * MU - y1 * y1 / MU;
+            x2 = x1 + WALL_WIDTH;
+            r2 = dom == 1951 ? 1.5 * 0.25 * LAMBDA : d->mu_b;
+            r2 = r2 * r2;
+            x3 = sqrt(r2 - y1 * y1) - x2;
+            if (((x + x3) * (x + x3) + y * y < r2) && (x > x1)) return 0;
+            return 1;
+        }
+        x1 = 0.5 * LAMBDA + 0.25 * MU - y * y / MU;
+        if ((o_abs(y) > 0.75) && (o_abs(x) < x1 + WALL_WIDTH)) return 2;
+        if ((o_abs(x) < x1) || (o_abs(x) > x1 + WALL_WIDTH)) return 1;
+        return 2;
+    }
+    case 20: case 201: case 202:                                                     /* 6216, 6240, 6228 */
+        for (i = 0; i < nc; i++)
+            if (circ[3 * nc + i] != 0.0) {
+                x1 = circ[i];
+                y1 = circ[nc + i];
+                r2 = circ[2 * nc + i] * circ[2 * nc + i];
+                if ((x - x1) * (x - x1) + (y - y1) * (y - y1) < r2) return 0;
+            }
+        if (dom == 201) return !((o_abs(x) >= LAMBDA) || (o_abs(y) >= YMAX - XMAX + LAMBDA));
+        return 1;
+    case 24: case 26:                                                                /* 7512, 7531 */
+        if (!o_escapes(0.0, 0.0, x, y, d)) return 0;
+        if (dom == 24) return ((x - 0.5) * (x - 0.5) / 3.0 + y * y / 1.0 > 1.2) ? 2 : 1;
+        return ((x - LAMBDA) * (x - LAMBDA) + (y - 0.5) * (y - 0.5) < MU * MU) ? 2 : 1;
+    case 25:                                                                         /* 7547 */
+        if (!o_escapes(x / d->julia_scale, y / d->julia_scale, d->julia_x, d->julia_y, d)) return 0;
+        return (x * x / 3.0 + y * y / 1.0 > 1.2) ? 2 : 1;
+    case 31:                                                                         /* 6026 */
+        x1 = MU + LAMBDA - 0.25 * y * y / MU;
+        y1 = MU + LAMBDA - 0.25 * x * x / MU;
+        return (o_abs(x) < x1) && (o_abs(y) < y1);
+    case 32:                                                                         /* 6032 */
+        condition = 1;
+        omega = O_DPI / ((double)d->npoly);
+        for (k = 0; k < d->npoly; k++) {
+            angle = d->apoly * O_PID + ((double)k + 0.5) * omega;
+            x1 = x * cos(angle) + y * sin(angle);
+            y1 = -x * sin(angle) + y * cos(angle);
+            condition = condition * (x1 < LAMBDA + MU - 0.25 * y1 * y1 / MU);
+        }
+        return condition;
+    case 33:                                                                         /* 6045 */
+        c = sqrt(LAMBDA * LAMBDA - (1.0 - MU) * (1.0 - MU));
+        width = 0.1 * MU;
+        x1 = o_abs(x);
+        y1 = o_abs(y);
+        if (o_abs(x) >= LAMBDA) return 0;
+        else if ((o_abs(y) >= MU) &&
+                 (x * x / (LAMBDA * LAMBDA) + (y1 - MU) * (y1 - MU) / ((1.0 - MU) * (1.0 - MU)) >= 1.0))
+            return 0;
+        else if ((o_abs(x) <= c) && (4.0 * (x1 - c) * (x1 - c) / (MU * MU) + y * y / (MU * MU) <= 1.0)) return 0;
+        else if ((o_abs(x) >= c) && (o_abs(y) <= width)) return 0;
+        return 1;
+    case 34:                                                                         /* 6061 */
+        b = MU * sqrt(1.0 + x * x / (LAMBDA * LAMBDA - MU * MU));
+        if (y > 1.02 * b) return 1;
+        else if (y < 0.98 * b) return 1;
+        return 0;
+    case 50:                                                                         /* 6415 */
+        if (o_abs(x) > LAMBDA) return 0;
+        else if (o_abs(y) > 1.0) return 0;
+        else if ((x > 0.0) && (y > 0.0)) return 0;
+        return 1;
+    case 51:                                                                         /* 6422 */
+        x1 = XMIN + MU;
+        x2 = XMAX - 2.0 * MU - 1.5 * LAMBDA;
+        y1 = 0.5 * LAMBDA;
+        y2 = 1.5 * LAMBDA;
+        if (x < x1) return 0;
+        if (x > x2 + 1.5 * LAMBDA) return 0;
+        if (o_abs(y) > y2) return 0;
+        if (x < x2) return o_abs(y) >= y1;
+        r = o_mod(x - x2, y);
+        if (r < 0.5 * LAMBDA) return 0;
+        if (r > 1.5 * LAMBDA) return 0;
+        return 1;
+    case 522: case 525: {                                                            /* 6501, 6518 */
+        double x5, x6, f, fp, cc = 15.0;
+        x1 = dom == 522 ? o_abs(x) * 2.0 / XMAX : o_abs(x);
+        y1 = o_abs(y);
+        width = dom == 522 ? 0.03 : 0.01;
+        b = 1.0 * (1.0 - 2.0 * MU - width);
+        a = MU + 0.5 * width - b;
+        x6 = x1 * x1 * x1;
+        x5 = x6 * x1 * x1;
+        x6 = x5 * x1;
+        if (dom == 522) {
+            f = a + b * (1.0 + 2.0 * x6) / (1.0 + x6);
+            fp = 6.0 * b * x5 / ((1.0 + x6) * (1.0 + x6));
+            return o_abs(f - y1) < MU * sqrt(1.0 + fp * fp);
+        }
+        f = a + b * (1.0 + 2.0 * cc * x6) / (1.0 + cc * x6);
+        fp = 6.0 * b * cc * x5 / ((1.0 + cc * x6) * (1.0 + cc * x6));
+        if (o_abs(f - y1) < 0.9 * MU * sqrt(1.0 + fp * fp)) return 1;
+        if (o_abs(f - y1) < MU * sqrt(1.0 + fp * fp)) return 2;
+        return 0;
+    }
+    case 523:                                                                        /* 6538 */
+        a = 0.5 * (1.0 - MU);
+        if (x < 0.0) {
+            x = -x;
+            y = -y;
+        }
+        if (x < LAMBDA) {
+            if (o_abs(y - 2.0 * a) < MU) return 1;
+            if (o_abs(y) < MU) return 1;
+            if (o_abs(y + 2.0 * a) < MU) return 1;
+            return 0;
+        }
+        r = o_mod(x - LAMBDA, y - a);
+        return o_abs(r - a) < MU;
+    case 524:                                                                        /* 6556 */
+        a = -1.0 + MU;
+        if (y < 0.0) {
+            x = -x;
+            y = -y;
+        }
+        if (x < -LAMBDA) {
+            if (o_abs(y - 1.0 + MU) > MU) return 0;
+            if (o_abs(y - 1.0 + MU) > 0.8 * MU) return 2;
+            return 1;
+        }
+        r = o_mod(x - a, y);
+        if (o_abs(r - 1.0 + MU) < 0.8 * MU) return 1;
+        if (o_abs(r - 1.0 + MU) < MU) return 2;
+        return 0;
+    case 61: case 62:                                                                /* 6701, 6708 */
+        if (o_abs(y) > MU) return 1;
+        a = LAMBDA - 0.25;
+        if (x > 0) return (x + a) * (x + a) + y * y > LAMBDA * LAMBDA;
+        return (x - a) * (x - a) + y * y > LAMBDA * LAMBDA;
+    case 63: case 64:                                                                /* 6715, 6726 */
+        width = 0.2;
+        a = 1.9;
+        b = 1.9 - 2.0 * LAMBDA + 2.0 * width;
+        x1 = o_abs(x);
+        if (dom == 63 && (x1 < WALL_WIDTH) && (o_abs(y) > MU)) return 2;
+        if (o_mod(x1 - a, y) > LAMBDA) return 1;
+        if (o_mod(x1 - b, y) > LAMBDA) return 1;
+        return 0;
+    case 67: {                                                                       /* 6749 */
+        double ca = cos(d->apoly * O_PID), sa = sin(d->apoly * O_PID);
+        x1 = x * ca + y * sa;
+        y1 = -x * sa + y * ca;
+        if (o_abs(y1) > MU) return 1;
+        a = LAMBDA - 0.25;
+        if (x1 > 0) return (x1 + a) * (x1 + a) + y1 * y1 > LAMBDA * LAMBDA;
+        return (x1 - a) * (x1 - a) + y1 * y1 > LAMBDA * LAMBDA;
+    }
+    case 68:                                                                         /* 6760 */
+        width = 0.1;
+        x1 = o_abs(x);
+        if (o_abs(y) > MU) return 1;
+        return (x1 - LAMBDA - width) * (x1 - LAMBDA - width) + y * y < LAMBDA * LAMBDA;
+    case 69:                                                                         /* 6767 */
+        if (o_abs(y) > MU) return 1;
+        if (x > 0.0) {
+            x1 = x - 0.6;
+            a = LAMBDA - 0.25;
+            if (x1 > 0) return (x1 + a) * (x1 + a) + y * y > LAMBDA * LAMBDA;
+            return (x1 - a) * (x1 - a) + y * y > LAMBDA * LAMBDA;
+        }
+        width = 0.1;
+        x1 = o_abs(x + 0.6);
+        return (x1 - LAMBDA - width) * (x1 - LAMBDA - width) + y * y < LAMBDA * LAMBDA;
+    case 73: {                                                                       /* 6785 */
+        double zz[9][2], h, y3;
+        h = 1.5 * LAMBDA;
+        r = 1.5 * LAMBDA;
+        x1 = o_abs(x);
+        y1 = y - h;
+        zz[0][0] = r * cos(O_PI / 6.0); zz[0][1] = -0.5 * r;
+        zz[1][0] = 0.0;                 zz[1][1] = r;
+        zz[2][0] = 0.0;                 zz[2][1] = -0.5 * r;
+        if (y1 > 0.95 * zz[1][1]) return 1;
+        h = 0.25 * LAMBDA;
+        r = 2.0 * LAMBDA;
+        y2 = y - h;
+        zz[3][0] = r * cos(O_PI / 6.0); zz[3][1] = -0.5 * r;
+        zz[4][0] = 0.0;                 zz[4][1] = r;
+        zz[5][0] = 0.0;                 zz[5][1] = -0.5 * r;
+        h = -1.5 * LAMBDA;
+        r = 3.0 * LAMBDA;
+        y3 = y - h;
+        zz[6][0] = r * cos(O_PI / 6.0); zz[6][1] = -0.5 * r;
+        zz[7][0] = 0.0;                 zz[7][1] = r;
+        zz[8][0] = 0.0;                 zz[8][1] = -0.5 * r;
+        if (o_tri(x1, y1, zz[0], zz[1], zz[2])) return 0;
+        if (o_tri(x1, y2, zz[3], zz[4], zz[5])) return 0;
+        if (o_tri(x1, y3, zz[6], zz[7], zz[8])) return 0;
+        if (o_tri(0.9 * x1, 0.9 * y1, zz[0], zz[1], zz[2])) return 1;
+        if (o_tri(0.9 * x1, 0.9 * y2, zz[3], zz[4], zz[5])) return 1;
+        if (o_tri(0.9 * x1, 0.9 * y3, zz[6], zz[7], zz[8])) return 1;
+        if (o_tri(0.8 * x1, 0.8 * y1, zz[0], zz[1], zz[2])) return 2;
+        if (o_tri(0.8 * x1, 0.8 * y2, zz[3], zz[4], zz[5])) return 2;
+        if (o_tri(0.8 * x1, 0.8 * y3, zz[6], zz[7], zz[8])) return 2;
+        return 1;
+    }
+    case 74: {                                                                       /* 6867 */
+        double h;
+        if ((o_abs(x) > LAMBDA) && (o_abs(y) > LAMBDA)) return 2;
+        r = 0.5 * sqrt(2.0);
+        h = YMAX - 0.5 * WALL_WIDTH;
+        x1 = (x + y) * r;
+        y1 = (-x + y) * r;
+        if ((o_abs(y1) < 0.5 * WALL_WIDTH) && (o_abs(x1) < LAMBDA * sqrt(2.0))) return 1;
+        if ((o_abs(x - h) < 0.5 * WALL_WIDTH) && (o_abs(y) < LAMBDA)) return 2;
+        if ((o_abs(x) < LAMBDA) && (o_abs(y - h) < 0.5 * WALL_WIDTH)) return 2;
+        return 0;
+    }
+    case 76: return 1;                                                               /* 6919 */
+    case 79:                                                                         /* 6966 */
+        x1 = 0.25 * (LAMBDA + 5.0 * MU);
+        x2 = -0.75 * (LAMBDA + 2.0 * MU);
+        if (o_mod(x - x1, y) < LAMBDA) return 1;
+        if (o_mod(x - x2, y) < MU) return 1;
+        if ((x < x1) && (x > x2) && (o_abs(y) < WALL_WIDTH)) return 1;
+        return 0;
+    case 791:                                                                        /* 6975 */
+        if (o_mod(x, y) < LAMBDA) return 1;
+        for (k = 0; k < d->npoly; k++) {
+            double ca, sa;
+            angle = d->apoly * O_PID + (double)k * O_DPI / (double)d->npoly;
+            ca = cos(angle);
+            sa = sin(angle);
+            x1 = (LAMBDA + 2.0 * MU) * ca;
+            y1 = (LAMBDA + 2.0 * MU) * sa;
+            if (o_mod(x - x1, y - y1) < MU) return 1;
+            x1 = x * ca + y * sa;
+            y1 = -x * sa + y * ca;
+            if ((x1 > 0.0) && (x1 < LAMBDA + 2.0 * MU) && (o_abs(y1) < WALL_WIDTH)) return 1;
+        }
+        return 0;
+    case 795:                                                                        /* 7052 */
+        x1 = o_abs(x);
+        return (x1 - LAMBDA) * (x1 - LAMBDA) / (MU * MU) + y * y < 1.0;
+    case 796:                                                                        /* 7057 */
+        x1 = o_abs(x);
+        return (o_abs(x1 - LAMBDA) < MU) && (o_abs(y) < 1.0);
+    case 90:                                                                         /* 7083 */
+        y1 = y - MU;
+        r = x * x + y1 * y1;
+        return r * r + 4.0 * LAMBDA * y1 * r - 4.0 * LAMBDA * LAMBDA * x * x < 0.0;
+    case 91: {                                                                       /* 7089, ipow 321 */
+        double l4 = LAMBDA, t = x * x + y * y - 4.0 * LAMBDA * LAMBDA, t3 = t;
+        for (k = 1; k < 4; k++) l4 *= LAMBDA;
+        for (k = 1; k < 3; k++) t3 *= t;
+        return 108 * l4 * x * x > t3;
+    }
+    }
+    return -1;
+}
+
+int orc_domain(int NX, int NY, const double box[4], int domain, const orc_domain_params *dp, int ncirc,
+               const double *circ, short *out)
+{
+    if (o_domain_at(0.0, 0.0, domain, dp, box, 0, circ) < 0) return -1;
+    int i, j;
+#pragma omp parallel for private(i, j)
+    for (i = 0; i < NX; i++)
+        for (j = 0; j < NY; j++) {
+            /* ij_to_xy, sub_wave.c:469-476 */
+            const double x = box[0] + ((double)i) * (box[1] - box[0]) / ((double)NX);
+            const double y = box[2] + ((double)j) * (box[3] - box[2]) / ((double)NY);
+            out[(size_t)i * NY + j] = (short)o_domain_at(x, y, domain, dp, box, ncirc, circ);
+        }
+    return 0;
+}
+
+/* ---- init_ior_2d's per-cell models, sub_wave.c:12181-12968 ------------------
+ * Each model as a loop over the cells in the reference's shape; 0 is the
+ * VARIABLE_IOR == 0 branch (12984-13003). */
+int orc_ior_tables(const orc_params *p, const short *xy, int ior, const double box[4], const orc_ior_params *q,
+                   double *tcc, double *tc, double *tg)
+{
+    static const int codes[] = {0, 1, 100, 2, 10, 11, 12, 13, 14, 15, 151, 16, 17, 18, 181, 182, 183, 19, 191};
+    int known = 0;
+    for (size_t n = 0; n < sizeof codes / sizeof codes[0]; n++) known |= codes[n] == ior;
+    if (!known) return -1;
+    const int NX = p->nx, NY = p->ny;
+    const double COURANT = q->courant, COURANTB = q->courantb, GAMMA = q->gamma, GAMMAB = q->gammab;
+    const double LAMBDA = q->lambda, MU = q->mu, XMIN = box[0], XMAX = box[1];
+    const double courant2 = COURANT * COURANT, courantb2 = COURANTB * COURANTB;
+    int i, j;
+    for (size_t k = 0; k < (size_t)NX * NY; k++) tcc[k] = tc[k] = tg[k] = 0.0;
+    if (ior == 0) {
+        for (i = 0; i < NX; i++)
+            for (j = 0; j < NY; j++) {
+                const size_t k = (size_t)i * NY + j;
+                if (xy[k] != 0) {
+                    tc[k] = COURANT;
+                    tcc[k] = courant2;
+                    tg[k] = xy[k] == 1 ? GAMMA : GAMMAB;
+                } else if (p->twospeeds) {
+                    tc[k] = COURANTB;
+                    tcc[k] = courantb2;
+                    tg[k] = GAMMAB;
+                }
+            }
+        return 0;
+    }
+#pragma omp parallel for private(i, j)
+    for (i = 0; i < NX; i++)
+        for (j = 0; j < NY; j++) {
+            const size_t k = (size_t)i * NY + j;
+            const double x = box[0] + ((double)i) * (box[1] - box[0]) / ((double)NX);
+            const double y = box[2] + ((double)j) * (box[3] - box[2]) / ((double)NY);
+            double u, v, u1, r, r2, c, speed, a;
+            int n;
+            switch (ior) {
+            case 1: case 100:                                          /* 12183, 12220 */
+                u = 0.0;
+                v = 0.0;
+                n = 0;
+                while ((n < q->mandellevel) && (u * u + v * v < 1000.0 * q->mandellimit)) {
+                    u1 = u * u - v * v + x;
+                    v = 2.0 * u * v + y;
+                    u = u1;
+                    n++;
+                }
+                if (ior == 1) {
+                    double norm2 = u * u + v * v;
+                    if (norm2 < q->mandellimit) tcc[k] = courant2;
+                    else {
+                        speed = 1.0 + q->mandel_ior_scale * log(1.0 + norm2 / q->mandellimit);
+                        if (speed < 0.01) speed = 0.01;
+                        tcc[k] = courant2 * speed;
+                    }
+                } else {
+                    if (n >= q->mandellevel) tcc[k] = courant2;
+                    else {
+                        speed = (double)n / (double)q->mandellevel;
+                        if (speed < 1.0e-10) speed = 1.0e-10;
+                        else if (speed > 10.0) speed = 10.0;
+                        tcc[k] = courant2 * speed;
+                    }
+                }
+                tg[k] = GAMMA;
+                break;
+            case 2:                                                    /* 12257 */
+                r2 = x * x + y * y;
+                if (r2 > 1.0) c = 0.0;
+                else if (r2 < 0.25 * 0.25) c = 0.8 * COURANT;
+                else if (r2 < 0.58 * 0.58) c = COURANT * (0.68 - 0.55 * r2);
+                else c = COURANT * (1.3 - 0.9 * r2);
+                tcc[k] = c;
+                tg[k] = GAMMA;
+                break;
+            case 10: case 11: case 12: case 13:                        /* 12556, 12593, 12623, 12653 */
+                if (ior == 10 || ior == 11) a = 0.05;
+                else a = LAMBDA + 0.1 - sqrt(LAMBDA * LAMBDA - MU * MU) + (ior == 13 ? 0.6 : 0.0);
+                if ((o_abs(x) < a) && (ior == 11 ? (o_abs(y) < MU) : (o_abs(y) > MU))) {
+                    tcc[k] = 0.0;
+                    tg[k] = GAMMA;
+                } else if (xy[k] != 0) {
+                    tcc[k] = courant2;
+                    tg[k] = GAMMA;
+                } else {
+                    tcc[k] = courantb2;
+                    tg[k] = GAMMAB;
+                }
+                break;
+            case 14:                                                   /* 12683 */
+                switch (xy[k]) {
+                case 0: tcc[k] = courantb2; tg[k] = GAMMAB; break;
+                case 1: tcc[k] = courant2; tg[k] = GAMMA; break;
+                case 2: tcc[k] = 0.0; tg[k] = GAMMA; break;
+                case 3: tcc[k] = courant2; tg[k] = GAMMA; break;
+                }
+                break;
+            case 15: case 151:                                         /* 12720, 12746 */
+                if (xy[k] == 1) {
+                    tcc[k] = courant2;
+                    tg[k] = GAMMA;
+                } else if (ior == 15 ? (o_abs(x) > LAMBDA) : ((o_abs(x) > LAMBDA) && (o_abs(y) > 0.7))) {
+                    tcc[k] = 0.0;
+                    tg[k] = 1.0;
+                } else {
+                    tcc[k] = courantb2;
+                    tg[k] = GAMMAB;
+                }
+                break;
+            case 16: {                                                 /* 12772 */
+                double cm = 0.8 * COURANT + 0.2 * COURANTB;
+                if (xy[k] == 1) { tcc[k] = courant2; tg[k] = GAMMA; }
+                else if (xy[k] == 2) { tcc[k] = cm * cm; tg[k] = GAMMA; }
+                else { tcc[k] = courantb2; tg[k] = GAMMAB; }
+                break;
+            }
+            case 17:                                                   /* 12800 */
+                if (xy[k] == 0) { tcc[k] = courant2; tg[k] = GAMMA; }
+                else if (xy[k] == 2) { tcc[k] = 0.0; tg[k] = 1.0; }
+                else { tcc[k] = courantb2; tg[k] = GAMMAB; }
+                break;
+            case 18: case 181: {                                       /* 12823, 12851 */
+                double ax = o_abs(x), ay = o_abs(y);
+                if (ax > LAMBDA) { tcc[k] = courant2; tg[k] = GAMMA; }
+                else if (ay > 1.0) { tcc[k] = 0.0; tg[k] = 0.0; }
+                else if (ior == 18) { tcc[k] = courantb2 / (1.0 - MU * ay * ay); tg[k] = GAMMAB; }
+                else {
+                    speed = 1.0 / (1.0 - MU * ay * ay);
+                    speed *= speed;
+                    tcc[k] = courantb2 * speed;
+                    tg[k] = GAMMAB;
+                }
+                break;
+            }
+            case 182:                                                  /* 12881 */
+                r = o_mod(x, y);
+                if (r > LAMBDA) { tcc[k] = courant2; tg[k] = GAMMA; }
+                else { tcc[k] = courant2 / (2.0 - r / LAMBDA); tg[k] = GAMMAB; }
+                break;
+            case 183:                                                  /* 12904 */
+                r = o_mod(x, y) / LAMBDA;
+                if (r > 1.0) { tcc[k] = courant2; tg[k] = GAMMA; }
+                else {
+                    r2 = r * (double)q->mdepth;
+                    r2 = (double)((int)r2) / (double)q->mdepth;
+                    tcc[k] = courant2 / (2.0 - r2);
+                    tg[k] = GAMMAB;
+                }
+                break;
+            case 19:                                                   /* 12929 */
+                speed = COURANT * (1.0 - x) + COURANTB * x;
+                tcc[k] = speed * speed;
+                tg[k] = GAMMA;
+                break;
+            case 191:                                                  /* 12945 */
+                a = (x - XMIN) / (XMAX - XMIN);
+                speed = COURANT * (1.0 - a) + COURANTB * a;
+                tcc[k] = speed * speed;
+                tg[k] = GAMMA;
+                break;
+            }
+            /* 12970-12981 */
+            if (xy[k] != 0) tc[k] = sqrt(tcc[k]);
+            else if (p->twospeeds) tc[k] = COURANTB;
+        }
+    return 0;
+}
+
+double orc_rotating_wells(int ngridx, int ngridy, int large, double ior_angle, const double box[4], double *xc,
+                          double *yc)
+{
+    const double s = large ? 1.5 : 1.0;
+    const double XMIN = box[0], XMAX = box[1], YMIN = box[2], YMAX = box[3];
+    const double dx = large ? (s * XMAX - s * XMIN) / (double)ngridx : (XMAX - XMIN) / (double)ngridx;
+    const double dy = large ? (s * YMAX - s * YMIN) / (double)ngridy : (YMAX - YMIN) / (double)ngridy;
+    const double ca = cos(ior_angle), sa = sin(ior_angle);
+    for (int i = 0; i < ngridx; i++)
+        for (int j = 0; j < ngridy; j++) {
+            const int n = j * ngridx + i;
+            double xs = (large ? s * XMIN : XMIN) + dx * ((double)i + 0.5);
+            double ys = (large ? s * YMIN : YMIN) + dy * ((double)j + 0.5);
+            if (j % 2 == 1) ys += 0.5 * dx;
+            xc[n] = xs * ca + ys * sa;
+            yc[n] = -xs * sa + ys * ca;
+        }
+    return 0.2 * dx * dx;
 }

@@ -363,6 +363,75 @@ def mask(nx, ny, box, domain, lam, mu=0.0, circles=None):
     return out
 
 
+class DomainParams(ctypes.Structure):
+    """orc_domain_params: the macros the closed-form domains read"""
+    _fields_ = [(n, ctypes.c_double) for n in ("lam", "mu", "mu_b", "apoly", "wall_width", "wall_width_b",
+                                                "julia_scale", "julia_x", "julia_y", "mandellimit")] + \
+               [(n, ctypes.c_int) for n in ("npoly", "mdepth", "mratio", "mandellevel", "ngridx", "ngridy")]
+
+
+# the values of the fixture builds' parameter block (oracle/ref_params.h) and
+# julia_x, julia_y of global_pdes.c:567
+DOMAIN_DEFAULTS = dict(lam=1.5, mu=0.1, mu_b=0.08, apoly=0.0, wall_width=0.1, wall_width_b=0.1, julia_scale=0.8,
+                       julia_x=0.37468, julia_y=0.21115, mandellimit=10.0, npoly=6, mdepth=4, mratio=3,
+                       mandellevel=100, ngridx=10, ngridy=10)
+
+
+def domain_params(**kw):
+    d = dict(DOMAIN_DEFAULTS)
+    bad = set(kw) - set(d)
+    if bad:
+        raise KeyError(bad)
+    d.update(kw)
+    return DomainParams(**d)
+
+
+def domain(nx, ny, box, dom, params, circles=None):
+    """orc_domain: xy_in_billiard_single_domain over the grid (int16 (nx, ny))
+    for the closed-form domains; params from domain_params"""
+    L = lib()
+    L.orc_domain.argtypes = [ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, ctypes.POINTER(DomainParams),
+                             ctypes.c_int, _D, _S]
+    c = np.zeros((4, 1)) if circles is None else np.ascontiguousarray(circles, np.float64)
+    n = 0 if circles is None else c.shape[1]
+    out = np.zeros((nx, ny), np.int16)
+    bx = np.ascontiguousarray(box, np.float64)
+    if L.orc_domain(nx, ny, _p(bx, _D), dom, ctypes.byref(params), n, _p(c, _D), _p(out, _S)):
+        raise ValueError("domain %d not provided" % dom)
+    return out
+
+
+class IorParams(ctypes.Structure):
+    """orc_ior_params: the macros init_ior_2d reads"""
+    _fields_ = [(n, ctypes.c_double) for n in ("courant", "courantb", "gamma", "gammab", "lam", "mu",
+                                                "mandel_ior_scale", "mandellimit")] + \
+               [(n, ctypes.c_int) for n in ("mandellevel", "mdepth")]
+
+
+def ior_tables(p, xy_in, ior, box, params):
+    """orc_ior_tables: (tcc, tc, tgamma) of init_ior_2d's per-cell model ior"""
+    L = lib()
+    L.orc_ior_tables.argtypes = [ctypes.POINTER(OrcParams), _S, ctypes.c_int, _D, ctypes.POINTER(IorParams),
+                                 _D, _D, _D]
+    xy = np.ascontiguousarray(xy_in, np.int16)
+    bx = np.ascontiguousarray(box, np.float64)
+    t = [np.zeros((p.nx, p.ny)) for _ in range(3)]
+    if L.orc_ior_tables(ctypes.byref(p), _p(xy, _S), ior, _p(bx, _D), ctypes.byref(params), *[_p(a, _D) for a in t]):
+        raise ValueError("IOR %d not provided" % ior)
+    return tuple(t)
+
+
+def rotating_wells(ngridx, ngridy, large, angle, box):
+    """orc_rotating_wells: (xc, yc, sigma)"""
+    L = lib()
+    L.orc_rotating_wells.restype = ctypes.c_double
+    L.orc_rotating_wells.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _D, _D, _D]
+    xc, yc = np.zeros(ngridx * ngridy), np.zeros(ngridx * ngridy)
+    bx = np.ascontiguousarray(box, np.float64)
+    s = L.orc_rotating_wells(ngridx, ngridy, int(large), angle, _p(bx, _D), _p(xc, _D), _p(yc, _D))
+    return xc, yc, s
+
+
 def lzw_strip(data):
     """orc_lzw_strip: one strip coded as libtiff's LZW coder codes it"""
     src = np.frombuffer(bytes(data), np.uint8).copy()
@@ -476,6 +545,7 @@ class RefBuild:
             return
         if hasattr(self.L, "ref_init_ior"):
             self.L.ref_init_ior.argtypes = [_S, _D, _D, _D, ctypes.c_uint]
+            self.L.ref_init_ior_at.argtypes = [_S, _D, _D, _D, ctypes.c_uint, ctypes.c_double]
             self.L.ref_wells.restype = ctypes.c_double
             self.L.ref_wells.argtypes = [ctypes.c_uint, _D, _D, _D]
             return
@@ -527,6 +597,14 @@ class RefBuild:
         xy = np.ascontiguousarray(xy_in, np.int16)
         t = [np.zeros((self.nx, self.ny)) for _ in range(3)]
         self.L.ref_init_ior(_p(xy, _S), _p(t[0], _D), _p(t[1], _D), _p(t[2], _D), seed)
+        return tuple(t)
+
+    def init_ior_at(self, xy_in, seed, angle):
+        """init_ior_2d at a rotation angle (its first call in the process
+        fixes the rotating models' lattice), tables zeroed first"""
+        xy = np.ascontiguousarray(xy_in, np.int16)
+        t = [np.zeros((self.nx, self.ny)) for _ in range(3)]
+        self.L.ref_init_ior_at(_p(xy, _S), _p(t[0], _D), _p(t[1], _D), _p(t[2], _D), seed, angle)
         return tuple(t)
 
     def init_circular_wave(self, x, y):

@@ -48,6 +48,19 @@ void ref_init_ior(short *xy_in, double *tcc, double *tc, double *tgamma, unsigne
     init_ior_2d(m, t1, t2, t3, 0.0);
 }
 
+/* the same at a rotation angle (the REFRESH_IOR call, wave_billiard.c:943) */
+void ref_init_ior_at(short *xy_in, double *tcc, double *tc, double *tgamma, unsigned seed, double angle)
+{
+    double *t1[NX], *t2[NX], *t3[NX];
+    short int *m[NX];
+    for (int i = 0; i < NX; i++) {
+        m[i] = xy_in + (size_t)i * NY;
+        t1[i] = tcc + (size_t)i * NY; t2[i] = tc + (size_t)i * NY; t3[i] = tgamma + (size_t)i * NY;
+    }
+    srand(seed);
+    init_ior_2d(m, t1, t2, t3, angle);
+}
+
 int ref_nwells(void) { return NGRIDX * NGRIDY; }
 
 /* the wells init_ior_2d places: IOR_PERIODIC_WELLS (sub_wave.c:12317-12328) or

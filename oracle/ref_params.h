@@ -180,13 +180,21 @@
 #ifndef GAMMAB
 #define GAMMAB 2.0e-4
 #endif
+#ifndef MANDELLEVEL
 #define MANDELLEVEL 100
+#endif
+#ifndef MANDELLIMIT
 #define MANDELLIMIT 10.0
+#endif
+#ifndef MANDEL_IOR_SCALE
 #define MANDEL_IOR_SCALE -0.05
+#endif
 #define NMAXCIRCLES 10000
 #define HRES 1
 #define INITIAL_TIME 0
+#ifndef WALL_WIDTH
 #define WALL_WIDTH 0.1
+#endif
 #ifndef MU
 #define MU 0.1
 #endif
@@ -196,10 +204,18 @@
 #define B_DOMAIN 20
 #endif
 #define B_DOMAIN_B 20
+#ifndef MU_B
 #define MU_B 0.08
+#endif
+#ifndef JULIA_SCALE
 #define JULIA_SCALE 0.8
+#endif
+#ifndef MRATIO
 #define MRATIO 3
+#endif
+#ifndef WALL_WIDTH_B
 #define WALL_WIDTH_B 0.1
+#endif
 #define WALL_WIDTH_ASYM 0.75
 #define WALL_WIDTH_ASYM_B 0.75
 #define PDISC_FACTOR 3.0
@@ -210,7 +226,13 @@
 #define Y_SHOOTER -0.6
 #define Y_TARGET 0.0
 #define RADIUS_FACTOR 0.3
+#ifndef NPOLY
 #define NPOLY 6
+#endif
+#ifndef APOLY
 #define APOLY 0.0
+#endif
+#ifndef MDEPTH
 #define MDEPTH 4
+#endif
 

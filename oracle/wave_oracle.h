@@ -217,6 +217,37 @@ void orc_wells_tables(const orc_params *p, const short *xy, int nw, const double
 int orc_mask(int NX, int NY, const double box[4], int domain, double lambda, double mu, int ncirc,
              const double *circ, short *out);
 
+/* The macros of xy_in_billiard_single_domain (sub_wave.c:5712) the closed-form
+ * domains read; same layout as the library's wv_domain_params. */
+typedef struct {
+    double lambda, mu, mu_b, apoly, wall_width, wall_width_b, julia_scale, julia_x, julia_y, mandellimit;
+    int npoly, mdepth, mratio, mandellevel, ngridx, ngridy;
+} orc_domain_params;
+
+/* xy_in_billiard_single_domain(x, y) at the cell corners of ij_to_xy over
+ * box (sub_wave.c:469-476), flat out[i*NY + j], for the closed-form domains
+ * (codes of global_pdes.c:14-236); -1 for another code. */
+int orc_domain(int NX, int NY, const double box[4], int domain, const orc_domain_params *dp, int ncirc,
+               const double *circ, short *out);
+
+/* The macros of init_ior_2d (sub_wave.c:12164); layout of wv_ior_params. */
+typedef struct {
+    double courant, courantb, gamma, gammab, lambda, mu, mandel_ior_scale, mandellimit;
+    int mandellevel, mdepth;
+} orc_ior_params;
+
+/* init_ior_2d's per-cell models (IOR codes of global_pdes.c:239-264; 0 is the
+ * VARIABLE_IOR == 0 branch) on zeroed flat tables, then the tc rule
+ * (sub_wave.c:12970-12981); cells the reference leaves unwritten stay 0.
+ * Returns -1 for another code. */
+int orc_ior_tables(const orc_params *p, const short *xy, int ior, const double box[4], const orc_ior_params *ip,
+                   double *tcc, double *tc, double *tg);
+
+/* the rotating-well centres (sub_wave.c:12390-12481) at ior_angle: n =
+ * j*ngridx + i; returns sigma_stat */
+double orc_rotating_wells(int ngridx, int ngridy, int large, double ior_angle, const double box[4], double *xc,
+                          double *yc);
+
 /* Per-cell energy, compute_energy wave_common.c:453-472, with the Courant
  * constants given explicitly (c2_in = COURANT^2 for xy_in != 0, c2_out =
  * COURANTB^2 used when TWOSPEEDS). */

@@ -248,3 +248,38 @@ def tiff_rgb(tag, h, w, content):
 # x, y) of the pulses added after it
 PULSE_INIT = (-0.5, 0.2)
 PULSES = ((1.0, 0.7, -0.4), (-1.3, -1.6, 0.9), (0.35, 0.05, 0.02))
+
+
+# every closed-form domain of xy_in_billiard_single_domain (the codes of
+# global_pdes.c:14-236), on two reference builds (oracle/build_ref.sh dom_a,
+# dom_b) whose parameter blocks are given here; other macros keep the values
+# of oracle/ref_params.h
+DOMAIN_CODES = (999, 0, 1, 196, 199, 198, 197, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 15, 16, 17, 18, 19, 191, 195,
+                1951, 1952, 20, 201, 202, 24, 25, 26, 31, 32, 33, 34, 50, 51, 522, 523, 524, 525, 61, 62, 63, 64, 67,
+                68, 69, 73, 74, 76, 79, 791, 795, 796, 90, 91)
+
+
+def domain_sets():
+    """build tag -> (nx, ny, parameters for pyoracle.domain_params)"""
+    return {
+        "dom_a": (97, 61, dict(lam=0.6, mu=0.3, npoly=6, apoly=0.5)),
+        "dom_b": (160, 90, dict(lam=1.2, mu=0.15, npoly=9, apoly=1.0, mdepth=3, julia_scale=1.0, mandellevel=200,
+                                wall_width=0.075, mu_b=0.3, wall_width_b=0.15, ngridx=8, ngridy=6)),
+    }
+
+
+# init_ior_2d's per-cell models (oracle/build_ref.sh ior_m<code>, ior_const_*):
+# 97 x 61 on MASK_BOX, LAMBDA 0.8, MU 0.3, COURANT 0.1, COURANTB 0.05, GAMMA
+# 1e-4, GAMMAB 2e-4, MDEPTH 6, TWOSPEEDS = code % 2; the mask they read is
+# drawn here with every value the models distinguish (-1, 0, 1, 2, 3)
+IOR_CODES = (1, 100, 2, 10, 11, 12, 13, 14, 15, 151, 16, 17, 18, 181, 182, 183, 19, 191)
+IOR_PARAMS = dict(courant=0.1, courantb=0.05, gamma=1.0e-4, gammab=2.0e-4, lam=0.8, mu=0.3, mandel_ior_scale=-0.05,
+                  mandellimit=10.0, mandellevel=100, mdepth=6)
+
+
+def ior_mask(nx=97, ny=61):
+    return (np.floor(uniform(case_seed("ior_mask"), (nx, ny), 0.0, 5.0)) - 1.0).astype(np.int16)
+
+
+# rotation angles of the rotating-well rebuilds (ior_rot, ior_rot_large)
+ROT_ANGLES = (0.0, 0.3, -1.7, 4.1)

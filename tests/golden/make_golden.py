@@ -27,6 +27,13 @@ Outputs (committed):
                     (D_SINAI, D_EXT_ELLIPSE, D_CIRCLES) for every
                     gen_inputs.mask_cases() build (python make_golden.py masks
                     writes this file alone)
+  domain_cases.npz  every closed-form domain of xy_in_billiard_single_domain
+                    (gen_inputs.DOMAIN_CODES) on the dom_a and dom_b builds
+                    (python make_golden.py domains writes this file alone)
+  ior_cases.npz     init_ior_2d's per-cell media models (gen_inputs.IOR_CODES)
+                    and its constant branch on gen_inputs.ior_mask(); the
+                    rotating wells at gen_inputs.ROT_ANGLES (tcc, tc)
+                    (python make_golden.py iors writes this file alone)
   pulse_cases.npz   init_circular_wave / add_circular_wave (wave_common.c:
                     89-111, 337-351) on a Sinai billiard, one speed and two:
                     the mask init writes, phi and psi after init and after
@@ -51,9 +58,9 @@ sys.path.insert(0, os.path.join(ROOT, "youtube-simulations_amd"))
 sys.path.insert(0, HERE)
 
 import pyoracle  # noqa: E402
-from gen_inputs import (COLOR_FRAMES, EXT_STEPS, PULSE_INIT, PULSES, TABLE_STEPS, case_inputs, cmp_cases,  # noqa: E402
-                        color_cases, color_inputs, ext_cases, ext_inputs, mask_cases, mask_circles, small_cases,
-                        table_cases, table_inputs)
+from gen_inputs import (COLOR_FRAMES, DOMAIN_CODES, EXT_STEPS, IOR_CODES, PULSE_INIT, PULSES, ROT_ANGLES,  # noqa: E402
+                        TABLE_STEPS, case_inputs, cmp_cases, color_cases, color_inputs, domain_sets, ext_cases,
+                        ext_inputs, ior_mask, mask_cases, mask_circles, small_cases, table_cases, table_inputs)
 from wavehip import workloads  # noqa: E402
 
 NSTEPS_SMALL = 7
@@ -87,8 +94,51 @@ def masks():
     np.savez_compressed(os.path.join(HERE, "mask_cases.npz"), **out)
 
 
+def domains():
+    """every closed-form domain on the dom_a / dom_b builds -> domain_cases.npz"""
+    out = {}
+    circ = mask_circles()
+    for tag, (nx, ny, kw) in domain_sets().items():
+        R = pyoracle.RefBuild(tag)
+        assert (R.nx, R.ny) == (nx, ny)
+        for dom in DOMAIN_CODES:
+            out["%s/%d" % (tag, dom)] = R.mask(dom, circ if dom in (20, 201, 202) else None)
+    np.savez_compressed(os.path.join(HERE, "domain_cases.npz"), **out)
+
+
+def iors():
+    """init_ior_2d's per-cell models, the constant branch and the rotating
+    wells at several angles -> ior_cases.npz"""
+    # the Mandelbrot models' loop is "omp parallel for private(i,j)" over
+    # variables declared at function scope (u, v, k, xy: sub_wave.c:12185,
+    # 12222), a race when several threads run it: the reference is run on one
+    import ctypes
+    ctypes.CDLL("libgomp.so.1").omp_set_num_threads(1)
+    out = {}
+    xy = ior_mask()
+    for m in IOR_CODES:
+        t = pyoracle.RefBuild("ior_m%d" % m).init_ior(xy, 7)
+        for name, a in zip(("tcc", "tc", "tgamma"), t):
+            out["m%d/%s" % (m, name)] = a
+    for ts in (0, 1):
+        t = pyoracle.RefBuild("ior_const_ts%d" % ts).init_ior(xy, 7)
+        for name, a in zip(("tcc", "tc", "tgamma"), t):
+            out["const_ts%d/%s" % (ts, name)] = a
+    for tag in ("ior_rot", "ior_rot_large"):
+        R = pyoracle.RefBuild(tag)
+        xr = np.ones((R.nx, R.ny), np.int16)
+        xr[::3, ::2] = 0
+        out[tag + "/xy_in"] = xr
+        for q, ang in enumerate(ROT_ANGLES):
+            t = R.init_ior_at(xr, 7, ang)
+            out["%s/%d/tcc" % (tag, q)], out["%s/%d/tc" % (tag, q)] = t[0], t[1]
+    np.savez_compressed(os.path.join(HERE, "ior_cases.npz"), **out)
+
+
 def main():
     masks()
+    domains()
+    iors()
     out = {}
     for (tag, nx, ny, bc, ts, tb, tbs) in small_cases():
         R = pyoracle.RefBuild(tag)
@@ -207,6 +257,10 @@ def tiffs():
 if __name__ == "__main__":
     if sys.argv[1:] == ["masks"]:
         masks()
+    elif sys.argv[1:] == ["domains"]:
+        domains()
+    elif sys.argv[1:] == ["iors"]:
+        iors()
     elif sys.argv[1:] == ["tiffs"]:
         tiffs()
     elif sys.argv[1:] == ["pulses"]:

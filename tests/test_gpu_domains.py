@@ -1,0 +1,199 @@
+"""Billiard domains and per-cell media models built on the GPU
+(wv_build_domain_mask, wv_set_ior_tables, the rotating wells through
+wv_rotating_wells + wv_set_wells_tables) against the reference's own
+functions (tests/golden/domain_cases.npz, ior_cases.npz) and the oracle.
+
+Masks are exact.  Media tables are exact except IOR_MANDELBROT, whose log is
+the device library's: the escape test itself is exact, and speed = 1 +
+MANDEL_IOR_SCALE*log(1 + |z|^2/MANDELLIMIT), with |z|^2 < 1e8 after escape
+(log < 17) and |MANDEL_IOR_SCALE| = 0.05, is off by at most 0.05*17*2 ulp +
+1 ulp < 1e-15 absolute (tcc within COURANT^2*1e-15, tc = sqrt(tcc) >=
+0.1*COURANT within COURANT*1e-14); and the rotating wells, whose exp is the
+device library's (relative 1e-14).  The
+installed media step exactly as the oracle steps with the same tables."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import say
+from gen_inputs import DOMAIN_CODES, IOR_CODES, IOR_PARAMS, MASK_BOX, ROT_ANGLES, domain_sets, ior_mask, mask_circles
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CIRCLE_CODES = (20, 201, 202)
+# the box of the checked-in wave_billiard.c (64-67)
+BOX_REF = (-0.75, 3.25, -1.197916667, 1.197916667)
+
+
+@pytest.fixture(scope="module")
+def wh():
+    import wavehip
+    wavehip.load()
+    return wavehip
+
+
+def bits_equal(a, b):
+    return np.array_equal(np.asarray(a).view(np.uint64), np.asarray(b).view(np.uint64))
+
+
+def mandel_close(got, want, courant):
+    """IOR_MANDELBROT tables within the bound of the device log (module text)"""
+    return (np.all(np.abs(got[0] - want[0]) <= courant * courant * 1e-15) and
+            np.all(np.abs(got[1] - want[1]) <= courant * 1e-14) and bits_equal(got[2], want[2]))
+
+
+def test_domain_masks_match_reference_fixture(wh, oracle):
+    """the builds' macros: their parameter block (oracle.DOMAIN_DEFAULTS)
+    with the set's own values"""
+    g = np.load(os.path.join(GOLD, "domain_cases.npz"))
+    circ = mask_circles()
+    for tag, (nx, ny, kw0) in domain_sets().items():
+        kw = dict(oracle.DOMAIN_DEFAULTS, **kw0)
+        W = wh.Wave(dict(nx=nx, ny=ny))
+        try:
+            for dom in DOMAIN_CODES:
+                say("domain fixture", tag, dom)
+                got = W.build_domain_mask(dom, MASK_BOX, circles=circ if dom in CIRCLE_CODES else None, **kw)
+                assert np.array_equal(got, g["%s/%d" % (tag, dom)]), (tag, dom)
+        finally:
+            W.close()
+
+
+@pytest.mark.parametrize("shape", [(1280, 720), (333, 257)])
+def test_domain_masks_match_oracle(wh, oracle, shape):
+    """the checked-in configuration's box and macros (MANDELLEVEL 1000, NPOLY
+    9, ...), and a second parameter set; a circle list longer than a wave"""
+    nx, ny = shape
+    rng = np.random.default_rng(31)
+    n = 90
+    circ = oracle.circle_list(rng.uniform(-0.7, 3.2, n), rng.uniform(-1.2, 1.2, n), rng.uniform(0.01, 0.2, n),
+                              (rng.uniform(0, 1, n) > 0.2).astype(float))
+    sets = [dict(wh.DOMAIN_DEFAULTS), dict(wh.DOMAIN_DEFAULTS, lam=0.9, mu=0.25, npoly=5, apoly=0.3, mdepth=5,
+                                                 julia_scale=1.2, mandellevel=300, ngridx=6, ngridy=9)]
+    W = wh.Wave(dict(nx=nx, ny=ny))
+    try:
+        for q, kw in enumerate(sets):
+            P = oracle.domain_params(**kw)
+            for dom in DOMAIN_CODES:
+                say("domain", nx, ny, q, dom)
+                c = circ if dom in CIRCLE_CODES else None
+                got = W.build_domain_mask(dom, BOX_REF, circles=c, **kw)
+                assert np.array_equal(got, oracle.domain(nx, ny, BOX_REF, dom, P, c)), (q, dom)
+    finally:
+        W.close()
+
+
+def test_domain_mask_errors(wh):
+    W = wh.Wave(dict(nx=40, ny=30))
+    try:
+        for dom, kw in ((53, {}), (8, dict(npoly=0)), (15, dict(mratio=3, mdepth=25)), (25, dict(julia_scale=0.0))):
+            with pytest.raises(wh.WaveError):
+                W.build_domain_mask(dom, MASK_BOX, **kw)
+        with pytest.raises(wh.WaveError):
+            W.build_domain_mask(3, (1.0, 1.0, 0.0, 1.0))
+    finally:
+        W.close()
+
+
+def test_ior_tables_match_reference_fixture(wh):
+    g = np.load(os.path.join(GOLD, "ior_cases.npz"))
+    xy = ior_mask()
+    for m in IOR_CODES + (0, 0):
+        ts = m % 2
+        W = wh.Wave(dict(nx=97, ny=61, twospeeds=ts))
+        try:
+            say("ior fixture", m)
+            got = W.set_ior_tables(xy, m, MASK_BOX, **IOR_PARAMS)
+        finally:
+            W.close()
+        key = "m%d" % m if m else "const_ts%d" % ts
+        want = [g["%s/%s" % (key, n)] for n in ("tcc", "tc", "tgamma")]
+        if m == 1:
+            inside = want[0] == IOR_PARAMS["courant"] ** 2
+            assert np.array_equal(got[0] == IOR_PARAMS["courant"] ** 2, inside)
+            assert mandel_close(got, want, IOR_PARAMS["courant"])
+        else:
+            for name, a, b in zip(("tcc", "tc", "tgamma"), got, want):
+                assert bits_equal(a, b), (m, name)
+    # TWOSPEEDS = 1 for the constant branch too
+    W = wh.Wave(dict(nx=97, ny=61, twospeeds=1))
+    try:
+        got = W.set_ior_tables(xy, 0, MASK_BOX, **IOR_PARAMS)
+    finally:
+        W.close()
+    for name, a in zip(("tcc", "tc", "tgamma"), got):
+        assert bits_equal(a, g["const_ts1/%s" % name]), name
+
+
+@pytest.mark.parametrize("m,dom", [(17, 74), (14, 73), (16, 524), (182, 999), (1, 999), (12, 68)])
+def test_ior_tables_install_and_step(wh, oracle, m, dom):
+    """a model on the domain it is meant for (Michelson, tree, coated guide,
+    Luneburg lens, Mandelbrot, concave lens), 640x360: tables as the oracle
+    builds them (Mandelbrot to 1e-15), and 30 steps on them equal the
+    oracle's 30 steps on the same tables, in both forms"""
+    nx, ny = 640, 360
+    kw = dict(wh.DOMAIN_DEFAULTS, lam=0.8, mu=0.3, wall_width=0.1)
+    box = MASK_BOX
+    ip = dict(IOR_PARAMS, mandellevel=500)
+    for form in (0, 1):
+        say("ior step", m, dom, form)
+        W = wh.Wave(dict(nx=nx, ny=ny, bcond=2, twospeeds=1), form=form, tb_depth=4)
+        try:
+            xy = W.build_domain_mask(dom, box, **kw)
+            got = W.set_ior_tables(xy, m, box, **ip)
+            p = oracle.make_params(nx, ny, bcond=2, twospeeds=1)
+            ref = oracle.ior_tables(p, xy, m, box, oracle.IorParams(**ip))
+            if m == 1:
+                assert mandel_close(got, ref, ip["courant"])
+            else:
+                for a, b in zip(got, ref):
+                    assert bits_equal(a, b)
+            x = box[0] + np.arange(nx) * (box[1] - box[0]) / nx
+            y = box[2] + np.arange(ny) * (box[3] - box[2]) / ny
+            X, Y = np.meshgrid(x, y, indexing="ij")
+            phi = np.exp(-((X + 1.2) ** 2 + (Y - 0.1) ** 2) / 0.02)
+            W.set_state(phi, phi)
+            W.step(30)
+            u = W.get_state()
+        finally:
+            W.close()
+        if form == 0:
+            r = oracle.run(p, phi, phi, 30, xy, got[0], got[1], got[2])
+            assert bits_equal(u[0], r[0]) and bits_equal(u[1], r[1])
+        else:
+            hi, lo, v = oracle.split_from_fp64(phi, phi)
+            hi, lo, v = oracle.split_run(p, hi, lo, v, 30, xy, got[0], got[1], got[2])
+            ur, umr = oracle.split_to_fp64(hi, lo, v)
+            assert bits_equal(u[0], ur) and bits_equal(u[1], umr)
+
+
+def test_ior_errors(wh):
+    W = wh.Wave(dict(nx=40, ny=30))
+    xy = np.ones((40, 30), np.int16)
+    try:
+        for m in (3, 4, 5, 6, 7, 8, 9, 20, 77, 101):
+            with pytest.raises(wh.WaveError):
+                W.set_ior_tables(xy, m, MASK_BOX)
+        with pytest.raises(wh.WaveError):
+            W.set_ior_tables(xy, 183, MASK_BOX, mdepth=0)
+    finally:
+        W.close()
+
+
+def test_rotating_wells_match_reference_fixture(wh):
+    """the REFRESH_IOR rebuild on the GPU: centres of wv_rotating_wells, sum of
+    wv_set_wells_tables, against init_ior_2d at each angle"""
+    g = np.load(os.path.join(GOLD, "ior_cases.npz"))
+    for tag, large, ts in (("ior_rot", 0, 0), ("ior_rot_large", 1, 1)):
+        xr = g[tag + "/xy_in"]
+        W = wh.Wave(dict(nx=40, ny=30, twospeeds=ts))
+        try:
+            for q, ang in enumerate(ROT_ANGLES):
+                say("rotating wells", tag, ang)
+                xc, yc, s = wh.rotating_wells(6, 5, large, ang, MASK_BOX)
+                tcc, tc, _ = W.set_wells_tables(xr, xc, yc, None, s, 0, 0.1, 0.05, 0.0, MASK_BOX)
+                assert np.allclose(tcc, g["%s/%d/tcc" % (tag, q)], rtol=1e-14, atol=0), (tag, q)
+                assert np.allclose(tc, g["%s/%d/tc" % (tag, q)], rtol=1e-14, atol=0), (tag, q)
+        finally:
+            W.close()

@@ -1,0 +1,94 @@
+"""The closed-form billiard domains of the oracle (orc_domain) against the
+reference's own xy_in_billiard_single_domain (sub_wave.c:5712-7602).
+
+tests/golden/domain_cases.npz holds the masks two reference builds (dom_a:
+97x61, LAMBDA 0.6, MU 0.3, NPOLY 6, APOLY 0.5; dom_b: 160x90 with another
+value for every macro the domains read) computed for every code of
+gen_inputs.DOMAIN_CODES at the cell corners of ij_to_xy.  Every code but the
+two constant domains takes more than one value in one of the builds, and a
+planted mistake -- the cell centre instead of the corner, a parameter off by
+a part in a thousand -- fails the fixtures.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from gen_inputs import DOMAIN_CODES, MASK_BOX, domain_sets, mask_circles
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CIRCLE_CODES = (20, 201, 202)
+CONSTANT = (999, 76)  # D_NOTHING and D_GRADIENT_INDEX_LENS are 1 everywhere
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return np.load(os.path.join(GOLD, "domain_cases.npz"))
+
+
+def oracle_mask(oracle, tag, dom, box=MASK_BOX, **over):
+    nx, ny, kw = domain_sets()[tag]
+    kw = dict(kw, **over)
+    return oracle.domain(nx, ny, box, dom, oracle.domain_params(**kw), mask_circles() if dom in CIRCLE_CODES else None)
+
+
+@pytest.mark.parametrize("tag", sorted(domain_sets()))
+@pytest.mark.parametrize("dom", DOMAIN_CODES)
+def test_domain_oracle_matches_reference(oracle, gold, tag, dom):
+    got = oracle_mask(oracle, tag, dom)
+    assert got.dtype == np.int16 and np.array_equal(got, gold["%s/%d" % (tag, dom)])
+
+
+def test_fixtures_are_not_trivial(gold):
+    for dom in DOMAIN_CODES:
+        vals = set()
+        for tag in domain_sets():
+            v = np.unique(gold["%s/%d" % (tag, dom)])
+            if len(v) > 1:
+                vals.add(tag)
+        assert (len(vals) == 0) == (dom in CONSTANT), dom
+
+
+def test_fixtures_catch_planted_mistakes(oracle, gold):
+    nx, ny, _ = domain_sets()["dom_b"]
+    b = MASK_BOX
+    hx, hy = 0.5 * (b[1] - b[0]) / nx, 0.5 * (b[3] - b[2]) / ny
+    centre = (b[0] + hx, b[1] + hx, b[2] + hy, b[3] + hy)
+    for dom in DOMAIN_CODES:
+        if dom in CONSTANT:
+            continue
+        hits = [not np.array_equal(oracle_mask(oracle, t, dom, centre if t == "dom_b" else
+                                               (b[0] + 0.5 * (b[1] - b[0]) / 97, b[1] + 0.5 * (b[1] - b[0]) / 97,
+                                                b[2] + 0.5 * (b[3] - b[2]) / 61, b[3] + 0.5 * (b[3] - b[2]) / 61)),
+                                   gold["%s/%d" % (t, dom)]) for t in domain_sets()]
+        assert any(hits), dom
+    # LAMBDA or MU off by a part in a thousand moves the boundary of most domains
+    moved = 0
+    for dom in DOMAIN_CODES:
+        for tag in domain_sets():
+            kw = domain_sets()[tag][2]
+            lam = kw.get("lam")
+            a = oracle_mask(oracle, tag, dom, lam=lam * 1.001)
+            m = oracle_mask(oracle, tag, dom, mu=kw.get("mu") * 1.001)
+            if not np.array_equal(a, gold["%s/%d" % (tag, dom)]) or not np.array_equal(m, gold["%s/%d" % (tag, dom)]):
+                moved += 1
+                break
+    assert moved >= 40
+
+
+def test_domain_oracle_refuses_other_codes(oracle):
+    with pytest.raises(ValueError):
+        oracle.domain(10, 10, MASK_BOX, 53, oracle.domain_params())  # D_MAZE reads a maze list
+
+
+@pytest.mark.skipif(not __import__("pyoracle").ref_available(), reason="oracle/_ref not built (needs /root/reference)")
+def test_domain_oracle_matches_reference_build_other_circles(oracle):
+    """direct comparison with the reference build on another circle list"""
+    import pyoracle
+    rng = np.random.default_rng(8)
+    c = pyoracle.circle_list(rng.uniform(-2.2, 2.2, 60), rng.uniform(-1.3, 1.3, 60), rng.uniform(0.01, 0.5, 60),
+                             (rng.uniform(0, 1, 60) > 0.3).astype(float))
+    for tag, (nx, ny, kw) in domain_sets().items():
+        R = pyoracle.RefBuild(tag)
+        for dom in CIRCLE_CODES:
+            assert np.array_equal(R.mask(dom, c), oracle.domain(nx, ny, MASK_BOX, dom, oracle.domain_params(**kw), c))

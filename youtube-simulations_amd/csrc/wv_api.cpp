@@ -2739,6 +2739,225 @@ extern "C" int wv_build_mask_static(wv_ctx *c, int domain, const double box[4], 
     return build_mask_static(c->e, domain, box, lambda, mu, ncircles, circles, xy_in);
 }
 
+// ---- every closed-form domain of xy_in_billiard_single_domain ----------------
+
+// the reference's angle constants (global_pdes.c:8-10)
+static const double REF_PI = 3.141592654, REF_DPI = 6.283185307, REF_PID = 1.570796327;
+
+static bool domain_known(int d)
+{
+    static const int codes[] = {999, 0,   1,   196, 199, 198,  197,  2,   3,   4,   5,   6,   7,   8,   9,
+                                10,  11,  12,  13,  15,  16,   17,   18,  19,  191, 195, 1951, 1952, 20,
+                                201, 202, 24,  25,  26,  31,   32,   33,  34,  50,  51,  522, 523, 524, 525,
+                                61,  62,  63,  64,  67,  68,   69,   73,  74,  76,  79,  791, 795, 796, 90, 91};
+    for (int c : codes)
+        if (c == d) return true;
+    return false;
+}
+
+static int build_domain_mask(Engine *e, int domain, const double *box, const wv_domain_params *p, int ncirc,
+                             const double *circ, short *xy)
+{
+    if (!domain_known(domain))
+        return fail(e, WV_EUNSUPPORTED, "domain %d is not among the closed-form domains provided", domain);
+    if (!box || !p || !xy) return fail(e, WV_EINVAL, "box, parameters or output is NULL");
+    if (!(box[1] > box[0]) || !(box[3] > box[2])) return fail(e, WV_EINVAL, "empty box");
+    const bool circles = domain == 20 || domain == 201 || domain == 202;
+    if (circles && (ncirc < 0 || (ncirc > 0 && !circ))) return fail(e, WV_EINVAL, "circle list");
+    const bool poly = domain == 8 || domain == 32 || domain == 791;
+    if (poly && (p->npoly < 1 || p->npoly > 4096)) return fail(e, WV_EINVAL, "NPOLY %d: 1..4096", p->npoly);
+    if (domain == 15 || domain == 17) {
+        // the digits (int)x1 % MRATIO of MRATIO^MDEPTH must fit an int
+        if (p->mratio < 1 || p->mdepth < 0 || p->mdepth > 30 ||
+            std::pow((double)p->mratio, (double)p->mdepth) >= 2147483647.0)
+            return fail(e, WV_EINVAL, "MRATIO %d, MDEPTH %d: MRATIO >= 1 and MRATIO^MDEPTH < 2^31", p->mratio,
+                        p->mdepth);
+    }
+    if ((domain == 16 || domain == 24 || domain == 25 || domain == 26) && (p->mandellevel < 0))
+        return fail(e, WV_EINVAL, "MANDELLEVEL %d", p->mandellevel);
+    if ((domain == 16 || domain == 25) && p->julia_scale == 0.0) return fail(e, WV_EINVAL, "JULIA_SCALE 0");
+    if (domain == 10) {
+        // one term per line of the grating: bounded by the box over LAMBDA
+        if (!(p->lambda > 0) || (std::fabs(box[2]) + std::fabs(box[3])) / p->lambda > 1e6)
+            return fail(e, WV_EINVAL, "D_GRATING: LAMBDA > 0 and at most 1e6 lines");
+    }
+    if ((domain == 12 || domain == 13) &&
+        (p->ngridx < 0 || p->ngridy < 1 || (double)p->ngridx * (p->ngridy + 1) > 1e6))
+        return fail(e, WV_EINVAL, "NGRIDX %d, NGRIDY %d: NGRIDY >= 1, at most 1e6 disks", p->ngridx, p->ngridy);
+    const double L = p->lambda, M = p->mu;
+    WvDomain a{};
+    // host constants, the reference's expressions in its order
+    std::vector<double> trig;
+    if (domain == 8 || domain == 32) {  // sub_wave.c:5812-5817, 6034-6041
+        const double omega = REF_DPI / ((double)p->npoly);
+        for (int k = 0; k < p->npoly; k++) {
+            const double angle = p->apoly * REF_PID + ((double)k + 0.5) * omega;
+            trig.push_back(std::cos(angle));
+            trig.push_back(std::sin(angle));
+        }
+        a.c0 = L * std::cos(omega * 0.5);
+    } else if (domain == 791) {  // sub_wave.c:6978-6981
+        for (int k = 0; k < p->npoly; k++) {
+            const double angle = p->apoly * REF_PID + (double)k * REF_DPI / (double)p->npoly;
+            trig.push_back(std::cos(angle));
+            trig.push_back(std::sin(angle));
+        }
+    } else if (domain == 67) {  // sub_wave.c:6751-6752
+        a.c0 = std::cos(p->apoly * REF_PID);
+        a.c1 = std::sin(p->apoly * REF_PID);
+    } else if (domain == 73) {  // sub_wave.c:6793
+        a.c0 = std::cos(REF_PI / 6.0);
+    } else if (domain == 74) {  // sub_wave.c:6870
+        a.c0 = std::sqrt(2.0);
+    } else if (domain == 33) {  // sub_wave.c:6047
+        a.c0 = std::sqrt(L * L - (1.0 - M) * (1.0 - M));
+    } else if (domain == 1951 || domain == 1952) {  // sub_wave.c:5991-5996, 6011-6016
+        const double y1 = domain == 1951 ? 0.2 : p->wall_width_b;
+        const double x1 = 0.5 * L + 0.25 * M - y1 * y1 / M;
+        const double x2 = x1 + p->wall_width;
+        double r2 = domain == 1951 ? 1.5 * 0.25 * L : p->mu_b;
+        r2 = r2 * r2;
+        a.c0 = std::sqrt(r2 - y1 * y1) - x2;
+        a.c1 = x1;
+        a.c2 = r2;
+    }
+    const int nc = circles ? ncirc : 0;
+    const Grid &G = e->g[0];
+    const int nx = G.p.nx, ny = G.p.ny;
+    const size_t cells = (size_t)nx * ny;
+    HIPCHK(e, hipSetDevice(e->device));
+    short *d_out = nullptr;
+    double *d_c = nullptr;
+    const size_t nd = 4 * (size_t)nc + trig.size();
+    hipError_t err = hipMalloc(&d_out, cells * sizeof(short));
+    if (err == hipSuccess) err = hipMalloc(&d_c, sizeof(double) * std::max<size_t>(nd, 1));
+    if (err == hipSuccess && nc > 0) err = hipMemcpy(d_c, circ, sizeof(double) * 4 * (size_t)nc, hipMemcpyHostToDevice);
+    if (err == hipSuccess && !trig.empty())
+        err = hipMemcpy(d_c + 4 * (size_t)nc, trig.data(), sizeof(double) * trig.size(), hipMemcpyHostToDevice);
+    if (err == hipSuccess) {
+        a.nx = nx; a.ny = ny; a.domain = domain; a.ncirc = nc; a.i0 = 0; a.rows = nx;
+        a.npoly = p->npoly; a.mdepth = p->mdepth; a.mratio = p->mratio; a.mandellevel = p->mandellevel;
+        a.ngridx = p->ngridx; a.ngridy = p->ngridy;
+        a.xmin = box[0]; a.xmax = box[1]; a.ymin = box[2]; a.ymax = box[3];
+        a.lambda = L; a.mu = M; a.mu_b = p->mu_b; a.wall_width = p->wall_width; a.wall_width_b = p->wall_width_b;
+        a.julia_scale = p->julia_scale; a.julia_x = p->julia_x; a.julia_y = p->julia_y;
+        a.mandellimit = p->mandellimit;
+        a.circ = d_c; a.trig = d_c + 4 * (size_t)nc; a.out = d_out;
+        err = (hipError_t)wv_launch_domain(a, e->stream);
+    }
+    if (err == hipSuccess) err = hipMemcpyAsync(xy, d_out, cells * sizeof(short), hipMemcpyDeviceToHost, e->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+    (void)hipFree(d_out);
+    (void)hipFree(d_c);
+    if (err != hipSuccess) return fail(e, WV_EHIP, "domain mask: %s", hipGetErrorString(err));
+    return WV_OK;
+}
+
+extern "C" int wv_build_domain_mask(wv_ctx *c, int domain, const double box[4], const wv_domain_params *p,
+                                    int ncircles, const double *circles, short *xy_in)
+{
+    if (!c) return WV_EINVAL;
+    return build_domain_mask(c->e, domain, box, p, ncircles, circles, xy_in);
+}
+
+// ---- init_ior_2d's per-cell media models -------------------------------------
+
+static bool ior_known(int m)
+{
+    static const int codes[] = {0, 1, 100, 2, 10, 11, 12, 13, 14, 15, 151, 16, 17, 18, 181, 182, 183, 19, 191};
+    for (int c : codes)
+        if (c == m) return true;
+    return false;
+}
+
+static int set_ior_tables(Engine *e, int k, const short *xy, int ior, const double *box, const wv_ior_params *p,
+                          double *tcc_out, double *tc_out, double *tg_out)
+{
+    if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
+    if (ior >= 4 && ior <= 9)
+        return fail(e, WV_EUNSUPPORTED, "IOR %d is a Gaussian-well model: wv_set_wells_tables builds it", ior);
+    if (ior == 3)
+        return fail(e, WV_EUNSUPPORTED, "IOR_EXPLO_LENSING: the reference reads xy_in[i*NY+j] of its NX row pointers");
+    if (!ior_known(ior)) return fail(e, WV_EUNSUPPORTED, "IOR %d is not a model of init_ior_2d provided here", ior);
+    if (!xy || !box || !p) return fail(e, WV_EINVAL, "mask, box or parameters NULL");
+    if (!(box[1] > box[0]) || !(box[3] > box[2])) return fail(e, WV_EINVAL, "empty box");
+    if ((ior == 1 || ior == 100) && p->mandellevel < (ior == 100 ? 1 : 0))
+        return fail(e, WV_EINVAL, "MANDELLEVEL %d", p->mandellevel);
+    if (ior == 183 && p->mdepth < 1) return fail(e, WV_EINVAL, "MDEPTH %d: >= 1", p->mdepth);
+    Grid &G = e->g[k];
+    const int nx = G.p.nx, ny = G.p.ny;
+    const size_t cells = (size_t)nx * ny;
+    WvIor a{};
+    a.nx = nx; a.ny = ny; a.model = ior; a.mandellevel = p->mandellevel; a.mdepth = p->mdepth;
+    a.twospeeds = G.p.twospeeds != 0;
+    a.xmin = box[0]; a.xmax = box[1]; a.ymin = box[2]; a.ymax = box[3];
+    a.courant = p->courant; a.courantb = p->courantb; a.gamma = p->gamma; a.gammab = p->gammab;
+    a.lambda = p->lambda; a.mu = p->mu; a.mandel_ior_scale = p->mandel_ior_scale; a.mandellimit = p->mandellimit;
+    // the lens-wall abscissa of IOR_LENS_CONCAVE / _CONVEX_CONCAVE, sub_wave.c:12625, 12655
+    if (ior == 12 || ior == 13) {
+        a.a0 = p->lambda + 0.1 - std::sqrt(p->lambda * p->lambda - p->mu * p->mu);
+        if (ior == 13) a.a0 = a.a0 + 0.6;
+    }
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    short *d_xy = nullptr;
+    double *d_t = nullptr;
+    hipError_t err = hipMalloc(&d_xy, cells * sizeof(short));
+    if (err == hipSuccess) err = hipMalloc(&d_t, 3 * cells * sizeof(double));
+    if (err == hipSuccess) err = hipMemcpy(d_xy, xy, cells * sizeof(short), hipMemcpyHostToDevice);
+    if (err == hipSuccess) {
+        a.xy = d_xy; a.tcc = d_t; a.tc = d_t + cells; a.tg = d_t + 2 * cells;
+        err = (hipError_t)wv_launch_ior(a, e->stream);
+    }
+    std::vector<double> h(3 * cells);
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(h.data(), d_t, 3 * cells * sizeof(double), hipMemcpyDeviceToHost, e->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+    (void)hipFree(d_xy);
+    (void)hipFree(d_t);
+    if (err != hipSuccess) return fail(e, WV_EHIP, "media tables: %s", hipGetErrorString(err));
+    // installed as wv_set_tables installs tables (media classes, masks, tiles)
+    const int rc = set_tables(e, k, xy, h.data(), h.data() + cells, h.data() + 2 * cells);
+    if (rc) return rc;
+    if (tcc_out) memcpy(tcc_out, h.data(), cells * sizeof(double));
+    if (tc_out) memcpy(tc_out, h.data() + cells, cells * sizeof(double));
+    if (tg_out) memcpy(tg_out, h.data() + 2 * cells, cells * sizeof(double));
+    return WV_OK;
+}
+
+extern "C" int wv_set_ior_tables(wv_ctx *c, const short *xy_in, int ior, const double box[4], const wv_ior_params *p,
+                                 double *tcc, double *tc, double *tgamma)
+{
+    if (!c) return WV_EINVAL;
+    return set_ior_tables(c->e, 0, xy_in, ior, box, p, tcc, tc, tgamma);
+}
+
+// The well centres of the rotating-well models, sub_wave.c:12390-12481:
+// the static lattice (first call of the reference) turned by ior_angle.
+extern "C" int wv_rotating_wells(int ngridx, int ngridy, int large, double ior_angle, const double box[4], double *xc,
+                                 double *yc, double *sigma)
+{
+    if (ngridx < 1 || ngridy < 1 || !box || !xc || !yc || !sigma) return WV_EINVAL;
+    if (!(box[1] > box[0]) || !(box[3] > box[2])) return WV_EINVAL;
+    const double xmin = box[0], xmax = box[1], ymin = box[2], ymax = box[3];
+    // IOR_PERIODIC_WELLS_ROTATING 12394-12403; _LARGE 12442-12451
+    const double dx = large ? (1.5 * xmax - 1.5 * xmin) / (double)ngridx : (xmax - xmin) / (double)ngridx;
+    const double x0 = large ? 1.5 * xmin : xmin, y0 = large ? 1.5 * ymin : ymin;
+    *sigma = 0.2 * dx * dx;
+    const double ca = std::cos(ior_angle), sa = std::sin(ior_angle);
+    for (int i = 0; i < ngridx; i++)
+        for (int j = 0; j < ngridy; j++) {
+            const int n = j * ngridx + i;
+            const double dy = large ? (1.5 * ymax - 1.5 * ymin) / (double)ngridy : (ymax - ymin) / (double)ngridy;
+            const double xs = x0 + dx * ((double)i + 0.5);
+            double ys = y0 + dy * ((double)j + 0.5);
+            if (j % 2 == 1) ys += 0.5 * dx;
+            xc[n] = xs * ca + ys * sa;
+            yc[n] = -xs * sa + ys * ca;
+        }
+    return WV_OK;
+}
+
 // ---- TIFF frames (writetiff, sub_wave.c:62-123) ------------------------------
 
 static uint8_t *le16(uint8_t *p, unsigned v)

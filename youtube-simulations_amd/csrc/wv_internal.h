@@ -226,6 +226,33 @@ struct WvMask {
     const double *circ;
     short *out;
 };
+// Every closed-form domain of xy_in_billiard_single_domain (wv_media.hip,
+// k_domain): domain = the reference's code, the macros it reads, and the
+// constants the host takes with the C library (cosines and sines of fixed
+// angles, square roots of parameter expressions).  trig: ntrig (cos, sin)
+// pairs of the domain's fixed angles (polygon sides), after the circle list
+// in the same device buffer.  Rows [i0, i0 + rows) into out[(i - i0)*NY + j].
+struct WvDomain {
+    int nx, ny, domain, ncirc, i0, rows;
+    int npoly, mdepth, mratio, mandellevel, ngridx, ngridy;
+    double xmin, xmax, ymin, ymax;
+    double lambda, mu, mu_b, wall_width, wall_width_b, julia_scale, julia_x, julia_y, mandellimit;
+    double c0, c1, c2;   // per-domain host constants (see k_domain)
+    const double *circ, *trig;
+    short *out;
+};
+int wv_launch_domain(const WvDomain &a, hipStream_t s);
+// init_ior_2d's per-cell media models (wv_media.hip, k_ior): model = the
+// reference's IOR code (0: VARIABLE_IOR == 0); tcc, tc, tgamma flat
+// (k = i*NY + j); xy flat; a0 the model's host constant (lens wall abscissa)
+struct WvIor {
+    int nx, ny, model, mandellevel, mdepth, twospeeds;
+    double xmin, xmax, ymin, ymax;
+    double courant, courantb, gamma, gammab, lambda, mu, mandel_ior_scale, mandellimit, a0;
+    const short *xy;
+    double *tcc, *tc, *tg;
+};
+int wv_launch_ior(const WvIor &a, hipStream_t s);
 int wv_launch_mask(const WvMask &a, hipStream_t s);
 // The mask builder's fixed device scratch (wv_build_mask_static): its
 // address on the current device (resolved once per device) and its size.

@@ -6,6 +6,10 @@
 // the reference on one host thread before the first frame.  Here one thread
 // per cell walks the well list (wave-uniform loads, summed in the
 // reference's order) in FP64; exp and tanh are the device library's.
+// init_ior_2d's other per-cell models (k_ior) and the closed-form domains of
+// xy_in_billiard_single_domain (k_domain) are one thread per cell too, in
+// binary64 without contraction and in the reference's order, so their
+// results equal the reference's (IOR_MANDELBROT's log aside).
 #include <mutex>
 
 #include "wv_device.h"
@@ -120,6 +124,556 @@ __global__ void __launch_bounds__(256) k_mask(WvMask a)
     gm(a.out)[(size_t)(i - a.i0) * a.ny + j] = v;
 }
 
+// ---- every closed-form domain of xy_in_billiard_single_domain ---------------
+
+// vabs and module2, sub_wave.c:270-285
+__device__ inline double vabs_(double x) { return x < 0.0 ? -x : x; }
+__device__ inline double module2_(double x, double y) { return sqrt(x * x + y * y); }
+
+// xy_in_triangle, sub_wave.c:5299-5311: all three wedge products >= 0
+__device__ inline bool in_triangle(double x, double y, double ax, double ay, double bx, double by, double cx,
+                                   double cy)
+{
+    const double v1 = (bx - ax) * (y - ay) - (by - ay) * (x - ax);
+    const double v2 = (cx - bx) * (y - by) - (cy - by) * (x - bx);
+    const double v3 = (ax - cx) * (y - cy) - (ay - cy) * (x - cx);
+    return v1 >= 0.0 && v2 >= 0.0 && v3 >= 0.0;
+}
+
+// z -> z^2 + c from z0 until the level or the escape radius; |z|^2 at the end
+__device__ inline double escape_norm2(double u, double v, double cx, double cy, int level, double limit)
+{
+    const double bound = 1000.0 * limit;
+    int k = 0;
+    while (k < level && u * u + v * v < bound) {
+        const double u1 = u * u - v * v + cx;
+        v = 2.0 * u * v + cy;
+        u = u1;
+        k++;
+    }
+    return u * u + v * v;
+}
+
+// the active circle the point falls in (D_CIRCLES family), in list order
+__device__ inline bool in_circles(const WvDomain &a, double x, double y)
+{
+    const int n = a.ncirc;
+    for (int k = 0; k < n; k++) {
+        if (gc(a.circ)[3 * n + k] == 0.0) continue;
+        const double x1 = gc(a.circ)[k], y1 = gc(a.circ)[n + k], r = gc(a.circ)[2 * n + k];
+        if ((x - x1) * (x - x1) + (y - y1) * (y - y1) < r * r) return true;
+    }
+    return false;
+}
+
+// xy_in_billiard_single_domain(x, y), sub_wave.c:5712-7602, for the
+// closed-form domains; each case follows the reference's expressions in its
+// order of evaluation (line numbers of sub_wave.c).  Host constants c0..c2
+// and the trig table are set per domain by the launcher.
+__device__ short domain_value(const WvDomain &a, double x, double y)
+{
+    const double L = a.lambda, M = a.mu, WW = a.wall_width;
+    switch (a.domain) {
+    case 999: return 1;                                                   // D_NOTHING 5721
+    case 0: return vabs_(x) < L && vabs_(y) < 1.0;                        // D_RECTANGLE 5726
+    case 1: return x * x / (L * L) + y * y < 1.0;                         // D_ELLIPSE 5732
+    case 196: return x * x + y * y < L * L;                               // D_CIRCLE_LAYERS 5738
+    case 199: return x * x / (L * L) + y * y / (M * M) > 1.0;             // D_EXT_ELLIPSE 5744
+    case 197:                                                             // D_EXT_ELLIPSE_CURVED_BDRY 5757
+        if (y > a.ymax - 0.05) return 0;
+        if (y < a.ymin + 0.05) return 0;
+        [[fallthrough]];
+    case 198: {                                                           // D_EXT_ELLIPSE_CURVED 5750
+        const double y1 = y + 0.4 * x * x;
+        return x * x / (L * L) + y1 * y1 / (M * M) > 1.0;
+    }
+    case 2:                                                               // D_STADIUM 5766
+        if (x > -0.5 * L && x < 0.5 * L && y > -1.0 && y < 1.0) return 1;
+        if (module2_(x + 0.5 * L, y) < 1.0) return 1;
+        return module2_(x - 0.5 * L, y) < 1.0;
+    case 3: return x * x + y * y > L * L;                                 // D_SINAI 5774
+    case 4: {                                                             // D_DIAMOND 5780
+        const double l2 = L * L, r2 = l2 + (L - 1.0) * (L - 1.0);
+        return x * x + y * y < 1.0 && (x - L) * (x - L) + (y - L) * (y - L) > r2 &&
+               (x - L) * (x - L) + (y + L) * (y + L) > r2 && (x + L) * (x + L) + (y - L) * (y - L) > r2 &&
+               (x + L) * (x + L) + (y + L) * (y + L) > r2;
+    }
+    case 5: return x > -L && y > -1.0 && L * y + x < 0.0;                 // D_TRIANGLE 5791
+    case 6: return y > -L;                                                // D_FLAT 5797
+    case 7: {                                                             // D_ANNULUS 5803
+        const double l2 = L * L, r2 = x * x + y * y;
+        return r2 > l2 && r2 < 1.0;
+    }
+    case 8: {                                                             // D_POLYGON 5810; c0 = LAMBDA*cos(omega/2)
+        int cond = 1;
+        for (int k = 0; k < a.npoly; k++)
+            cond = cond * (x * gc(a.trig)[2 * k] + y * gc(a.trig)[2 * k + 1] < a.c0);
+        return cond;
+    }
+    case 9:                                                               // D_YOUNG 5823
+        if (x < -M || x > M) return 1;
+        return vabs_(y - L) < M || vabs_(y + L) < M;
+    case 10: {                                                            // D_GRATING 5829
+        const int k1 = -(int)((-a.ymin) / L), k2 = (int)(a.ymax / L);
+        int cond = 1;
+        for (int i = k1; i <= k2; i++) {
+            const double z = (double)i * L;
+            cond = cond * (x * x + (y - z) * (y - z) > M * M);
+        }
+        return cond;
+    }
+    case 11:                                                              // D_EHRENFEST 5842
+        return (x - 1.0) * (x - 1.0) + y * y < L * L || (x + 1.0) * (x + 1.0) + y * y < L * L ||
+               (vabs_(x) < 1.0 && vabs_(y) < M);
+    case 12: {                                                            // D_DISK_GRID 5846
+        const double dy = (a.ymax - a.ymin) / ((double)a.ngridy);
+        for (int i = -a.ngridx / 2; i < a.ngridx / 2; i++)
+            for (int j = 0; j < a.ngridy; j++) {
+                const double x1 = ((double)i + 0.5) * dy, y1 = a.ymin + ((double)j + 0.5) * dy;
+                if ((x - x1) * (x - x1) + (y - y1) * (y - y1) < M * M) return 0;
+            }
+        return 1;
+    }
+    case 13: {                                                            // D_DISK_HEX 5858
+        const double dy = (a.ymax - a.ymin) / ((double)a.ngridy);
+        for (int i = -a.ngridx / 2; i < a.ngridx / 2; i++)
+            for (int j = -1; j < a.ngridy; j++) {
+                const double x1 = ((double)i + 0.5) * dy;
+                double y1 = a.ymin + ((double)j + 0.5) * dy;
+                if ((i + a.ngridx) % 2 == 1) y1 += 0.5 * dy;
+                if ((x - x1) * (x - x1) + (y - y1) * (y - y1) < M * M) return 0;
+            }
+        return 1;
+    }
+    case 15: case 17: {                                                   // D_MENGER 7431, D_MENGER_ROTATED 7458
+        double x1, y1;
+        if (a.domain == 17) {
+            const double x2 = 1.0 * (x + y), y2 = 1.0 * (x - y);
+            if (!(vabs_(x2) < 1.0 && vabs_(y2) < 1.0)) return 1;
+            x1 = 0.5 * (x2 + 1.0);
+            y1 = 0.5 * (y2 + 1.0);
+        } else {
+            x1 = 0.5 * (x + 1.0);
+            y1 = 0.5 * (y + 1.0);
+        }
+        // the test reads the unrotated point in both (7439, 7468)
+        if (!(vabs_(x) < 1.0 && vabs_(y) < 1.0)) return 1;
+        for (int k = 0; k < a.mdepth; k++) {
+            x1 = x1 * (double)a.mratio;
+            y1 = y1 * (double)a.mratio;
+            if (((int)x1 % a.mratio) == a.mratio / 2 && ((int)y1 % a.mratio) == a.mratio / 2) return 0;
+        }
+        return 1;
+    }
+    case 16:                                                              // D_JULIA_INT 7443
+        return escape_norm2(x / a.julia_scale, y / a.julia_scale, a.julia_x, a.julia_y, a.mandellevel,
+                            a.mandellimit) < a.mandellimit;
+    case 18: return x > 0.25 * y * y / L - L;                             // D_PARABOLA 5872
+    case 19: {                                                            // D_TWO_PARABOLAS 5876
+        const double x1 = 0.25 * y * y / M - M - L, x2 = -x1;
+        double width = 0.25 * M;
+        if (width > 0.2) width = 0.2;
+        if (vabs_(y) > 1.5 * M) return 1;
+        if (x < x1 - width || x > x2 + width) return 1;
+        return x > x1 && x < x2;
+    }
+    case 191:                                                             // D_TWO_PARABOLAS_ASYM 5887
+        if (x > 0.0) {
+            if (vabs_(y) > 0.5 * L) return 1;
+            const double x1 = -y * y / L + 0.25 * L;
+            return x < x1 || x > x1 + WW;
+        } else {
+            if (vabs_(y) > 0.5 * M) return 1;
+            const double x1 = y * y / M - 0.25 * M;
+            return x > x1 || x < x1 - WW;
+        }
+    case 195: case 1951: case 1952: {                                     // 5977, 5986, 6006
+        if (vabs_(y) > 0.75 + WW) return 1;
+        if (a.domain != 195) {
+            const double ylens = a.domain == 1951 ? 0.2 : a.wall_width_b;
+            if (x > 0.0 && vabs_(y) < ylens) {
+                // c0 = x3, c1 = x1, c2 = r2 of the window lens
+                if ((x + a.c0) * (x + a.c0) + y * y < a.c2 && x > a.c1) return 0;
+                return 1;
+            }
+        }
+        const double x1 = 0.5 * L + 0.25 * M - y * y / M;
+        if (vabs_(y) > 0.75 && vabs_(x) < x1 + WW) return 2;
+        if (vabs_(x) < x1 || vabs_(x) > x1 + WW) return 1;
+        if (a.domain == 195 && x > 0.0 && vabs_(y) < 0.2) return 0;
+        return 2;
+    }
+    case 20: case 202: return in_circles(a, x, y) ? 0 : 1;               // D_CIRCLES 6216, _NEUMANN 6228
+    case 201:                                                             // D_CIRCLES_IN_RECT 6240
+        if (in_circles(a, x, y)) return 0;
+        return (vabs_(x) >= L || vabs_(y) >= a.ymax - a.xmax + L) ? 0 : 1;
+    case 24: case 26: {                                                   // D_MANDELBROT 7512, _CIRCLE 7531
+        if (escape_norm2(0.0, 0.0, x, y, a.mandellevel, a.mandellimit) < a.mandellimit) return 0;
+        if (a.domain == 24) return (x - 0.5) * (x - 0.5) / 3.0 + y * y / 1.0 > 1.2 ? 2 : 1;
+        return (x - L) * (x - L) + (y - 0.5) * (y - 0.5) < M * M ? 2 : 1;
+    }
+    case 25:                                                              // D_JULIA 7547
+        if (escape_norm2(x / a.julia_scale, y / a.julia_scale, a.julia_x, a.julia_y, a.mandellevel,
+                         a.mandellimit) < a.mandellimit)
+            return 0;
+        return x * x / 3.0 + y * y / 1.0 > 1.2 ? 2 : 1;
+    case 31: {                                                            // D_FOUR_PARABOLAS 6026
+        const double x1 = M + L - 0.25 * y * y / M, y1 = M + L - 0.25 * x * x / M;
+        return vabs_(x) < x1 && vabs_(y) < y1;
+    }
+    case 32: {                                                            // D_POLY_PARABOLAS 6032
+        int cond = 1;
+        for (int k = 0; k < a.npoly; k++) {
+            const double ca = gc(a.trig)[2 * k], sa = gc(a.trig)[2 * k + 1];
+            const double x1 = x * ca + y * sa, y1 = -x * sa + y * ca;
+            cond = cond * (x1 < L + M - 0.25 * y1 * y1 / M);
+        }
+        return cond;
+    }
+    case 33: {                                                            // D_PENROSE 6045; c0 = c
+        const double c = a.c0, width = 0.1 * M, x1 = vabs_(x), y1 = vabs_(y);
+        if (vabs_(x) >= L) return 0;
+        if (vabs_(y) >= M && x * x / (L * L) + (y1 - M) * (y1 - M) / ((1.0 - M) * (1.0 - M)) >= 1.0) return 0;
+        if (vabs_(x) <= c && 4.0 * (x1 - c) * (x1 - c) / (M * M) + y * y / (M * M) <= 1.0) return 0;
+        if (vabs_(x) >= c && vabs_(y) <= width) return 0;
+        return 1;
+    }
+    case 34: {                                                            // D_HYPERBOLA 6061
+        const double b = M * sqrt(1.0 + x * x / (L * L - M * M));
+        if (y > 1.02 * b) return 1;
+        return y < 0.98 * b;
+    }
+    case 50:                                                              // D_LSHAPE 6415
+        if (vabs_(x) > L || vabs_(y) > 1.0) return 0;
+        return !(x > 0.0 && y > 0.0);
+    case 51: {                                                            // D_WAVEGUIDE 6422
+        const double x1 = a.xmin + M, x2 = a.xmax - 2.0 * M - 1.5 * L, y1 = 0.5 * L, y2 = 1.5 * L;
+        if (x < x1) return 0;
+        if (x > x2 + 1.5 * L) return 0;
+        if (vabs_(y) > y2) return 0;
+        if (x < x2) return vabs_(y) >= y1;
+        const double r = module2_(x - x2, y);
+        if (r < 0.5 * L) return 0;
+        if (r > 1.5 * L) return 0;
+        return 1;
+    }
+    case 522: case 525: {                                                 // D_WAVEGUIDES_COUPLED 6501, _N 6518
+        const bool narrow = a.domain == 525;
+        const double x1 = narrow ? vabs_(x) : vabs_(x) * 2.0 / a.xmax, y1 = vabs_(y);
+        const double width = narrow ? 0.01 : 0.03;
+        const double b = 1.0 * (1.0 - 2.0 * M - width), aa = M + 0.5 * width - b;
+        double x6 = x1 * x1 * x1;
+        const double x5 = x6 * x1 * x1;
+        x6 = x5 * x1;
+        if (!narrow) {
+            const double f = aa + b * (1.0 + 2.0 * x6) / (1.0 + x6);
+            const double fp = 6.0 * b * x5 / ((1.0 + x6) * (1.0 + x6));
+            return vabs_(f - y1) < M * sqrt(1.0 + fp * fp);
+        }
+        const double c = 15.0;
+        const double f = aa + b * (1.0 + 2.0 * c * x6) / (1.0 + c * x6);
+        const double fp = 6.0 * b * c * x5 / ((1.0 + c * x6) * (1.0 + c * x6));
+        if (vabs_(f - y1) < 0.9 * M * sqrt(1.0 + fp * fp)) return 1;
+        if (vabs_(f - y1) < M * sqrt(1.0 + fp * fp)) return 2;
+        return 0;
+    }
+    case 523: {                                                           // D_WAVEGUIDE_S 6538
+        const double aa = 0.5 * (1.0 - M);
+        if (x < 0.0) { x = -x; y = -y; }
+        if (x < L) {
+            if (vabs_(y - 2.0 * aa) < M) return 1;
+            if (vabs_(y) < M) return 1;
+            return vabs_(y + 2.0 * aa) < M;
+        }
+        const double r = module2_(x - L, y - aa);
+        return vabs_(r - aa) < M;
+    }
+    case 524: {                                                           // D_WAVEGUIDE_S_SHORT 6556
+        const double aa = -1.0 + M;
+        if (y < 0.0) { x = -x; y = -y; }
+        if (x < -L) {
+            if (vabs_(y - 1.0 + M) > M) return 0;
+            if (vabs_(y - 1.0 + M) > 0.8 * M) return 2;
+            return 1;
+        }
+        const double r = module2_(x - aa, y);
+        if (vabs_(r - 1.0 + M) < 0.8 * M) return 1;
+        if (vabs_(r - 1.0 + M) < M) return 2;
+        return 0;
+    }
+    case 61: case 62: case 67: {                                          // D_LENS 6701, _WALL 6708, _ROTATED 6749
+        double x1 = x, y1 = y;
+        if (a.domain == 67) {  // c0, c1 = cos, sin of APOLY*PID
+            x1 = x * a.c0 + y * a.c1;
+            y1 = -x * a.c1 + y * a.c0;
+        }
+        if (vabs_(y1) > M) return 1;
+        const double aa = L - 0.25;
+        if (x1 > 0) return (x1 + aa) * (x1 + aa) + y1 * y1 > L * L;
+        return (x1 - aa) * (x1 - aa) + y1 * y1 > L * L;
+    }
+    case 63: case 64: {                                                   // D_TWO_LENSES_WALL 6715, _OBSTACLE 6726
+        const double width = 0.2, aa = 1.9, b = 1.9 - 2.0 * L + 2.0 * width, x1 = vabs_(x);
+        if (a.domain == 63 && x1 < WW && vabs_(y) > M) return 2;
+        if (module2_(x1 - aa, y) > L) return 1;
+        if (module2_(x1 - b, y) > L) return 1;
+        return 0;
+    }
+    case 68: {                                                            // D_LENS_CONCAVE 6760
+        const double width = 0.1, x1 = vabs_(x);
+        if (vabs_(y) > M) return 1;
+        return (x1 - L - width) * (x1 - L - width) + y * y < L * L;
+    }
+    case 69: {                                                            // D_LENS_CONVEX_CONCAVE 6767
+        if (vabs_(y) > M) return 1;
+        const double xshift = 0.6;
+        if (x > 0.0) {
+            const double x1 = x - xshift, aa = L - 0.25;
+            if (x1 > 0) return (x1 + aa) * (x1 + aa) + y * y > L * L;
+            return (x1 - aa) * (x1 - aa) + y * y > L * L;
+        }
+        const double width = 0.1, x1 = vabs_(x + xshift);
+        return (x1 - L - width) * (x1 - L - width) + y * y < L * L;
+    }
+    case 73: {                                                            // D_TREE 6785; c0 = cos(PI/6)
+        const double x1 = vabs_(x);
+        const double h1 = 1.5 * L, r1 = 1.5 * L, y1 = y - h1;
+        if (y1 > 0.95 * r1) return 1;
+        const double h2 = 0.25 * L, r2 = 2.0 * L, y2 = y - h2;
+        const double h3 = -1.5 * L, r3 = 3.0 * L, y3 = y - h3;
+        const double rr[3] = {r1, r2, r3}, yy[3] = {y1, y2, y3};
+        const double scale[3] = {1.0, 0.9, 0.8};
+        const short val[3] = {0, 1, 2};
+        for (int s = 0; s < 3; s++)
+            for (int t = 0; t < 3; t++) {
+                const double r = rr[t];
+                const double px = s ? scale[s] * x1 : x1, py = s ? scale[s] * yy[t] : yy[t];
+                if (in_triangle(px, py, r * a.c0, -0.5 * r, 0.0, r, 0.0, -0.5 * r)) return val[s];
+            }
+        return 1;
+    }
+    case 74: {                                                            // D_MICHELSON 6867; c0 = sqrt(2)
+        if (vabs_(x) > L && vabs_(y) > L) return 2;
+        const double r = 0.5 * a.c0, h = a.ymax - 0.5 * WW;
+        const double x1 = (x + y) * r, y1 = (-x + y) * r;
+        if (vabs_(y1) < 0.5 * WW && vabs_(x1) < L * a.c0) return 1;
+        if (vabs_(x - h) < 0.5 * WW && vabs_(y) < L) return 2;
+        if (vabs_(x) < L && vabs_(y - h) < 0.5 * WW) return 2;
+        return 0;
+    }
+    case 76: return 1;                                                    // D_GRADIENT_INDEX_LENS 6919
+    case 79: {                                                            // D_TWOCIRCLES 6966
+        const double x1 = 0.25 * (L + 5.0 * M), x2 = -0.75 * (L + 2.0 * M);
+        if (module2_(x - x1, y) < L) return 1;
+        if (module2_(x - x2, y) < M) return 1;
+        return x < x1 && x > x2 && vabs_(y) < WW;
+    }
+    case 791: {                                                           // D_POLYCIRCLES 6975
+        if (module2_(x, y) < L) return 1;
+        for (int k = 0; k < a.npoly; k++) {
+            const double ca = gc(a.trig)[2 * k], sa = gc(a.trig)[2 * k + 1];
+            double x1 = (L + 2.0 * M) * ca, y1 = (L + 2.0 * M) * sa;
+            if (module2_(x - x1, y - y1) < M) return 1;
+            x1 = x * ca + y * sa;
+            y1 = -x * sa + y * ca;
+            if (x1 > 0.0 && x1 < L + 2.0 * M && vabs_(y1) < WW) return 1;
+        }
+        return 0;
+    }
+    case 795: {                                                           // D_TWO_ELLIPSES 7052
+        const double x1 = vabs_(x);
+        return (x1 - L) * (x1 - L) / (M * M) + y * y < 1.0;
+    }
+    case 796: {                                                           // D_TWO_RECTANGLES 7057
+        const double x1 = vabs_(x);
+        return vabs_(x1 - L) < M && vabs_(y) < 1.0;
+    }
+    case 90: {                                                            // D_CARDIOID 7083
+        const double y1 = y - M, r = x * x + y1 * y1;
+        return r * r + 4.0 * L * y1 * r - 4.0 * L * L * x * x < 0.0;
+    }
+    case 91: {                                                            // D_NEPHROID 7089 (ipow, sub_wave.c:321)
+        const double l4 = L * L * L * L, t = x * x + y * y - 4.0 * L * L;
+        return 108 * l4 * x * x > t * t * t;
+    }
+    default: return 1;  // not reached: the launcher admits only the codes above
+    }
+}
+
+__global__ void __launch_bounds__(256) k_domain(WvDomain a)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = a.i0 + (int)blockIdx.y;
+    if (j >= a.ny || i >= a.nx || i >= a.i0 + a.rows) return;
+    // ij_to_xy, sub_wave.c:469-476
+    const double x = a.xmin + ((double)i) * (a.xmax - a.xmin) / ((double)a.nx);
+    const double y = a.ymin + ((double)j) * (a.ymax - a.ymin) / ((double)a.ny);
+    gm(a.out)[(size_t)(i - a.i0) * a.ny + j] = domain_value(a, x, y);
+}
+
+// ---- init_ior_2d's per-cell media models -------------------------------------
+
+// tcc and tgamma of one cell, sub_wave.c:12181-12968 (case lines below);
+// false where the reference writes nothing (IOR_TREE off its four values)
+__device__ bool ior_cell(const WvIor &a, double x, double y, short m, double &tcc, double &tg)
+{
+    const double c2 = a.courant * a.courant, cb2 = a.courantb * a.courantb;
+    const double G = a.gamma, GB = a.gammab, L = a.lambda, M = a.mu;
+    switch (a.model) {
+    case 1: {                                                             // IOR_MANDELBROT 12183
+        const double norm2 = escape_norm2(0.0, 0.0, x, y, a.mandellevel, a.mandellimit);
+        if (norm2 < a.mandellimit) {
+            tcc = c2;
+        } else {
+            double speed = 1.0 + a.mandel_ior_scale * log(1.0 + norm2 / a.mandellimit);
+            if (speed < 0.01) speed = 0.01;
+            tcc = c2 * speed;
+        }
+        tg = G;
+        return true;
+    }
+    case 100: {                                                           // IOR_MANDELBROT_LIN 12220
+        double u = 0.0, v = 0.0;
+        int k = 0;
+        const double bound = 1000.0 * a.mandellimit;
+        while (k < a.mandellevel && u * u + v * v < bound) {
+            const double u1 = u * u - v * v + x;
+            v = 2.0 * u * v + y;
+            u = u1;
+            k++;
+        }
+        if (k >= a.mandellevel) {
+            tcc = c2;
+        } else {
+            double speed = (double)k / (double)a.mandellevel;
+            if (speed < 1.0e-10) speed = 1.0e-10;
+            else if (speed > 10.0) speed = 10.0;
+            tcc = c2 * speed;
+        }
+        tg = G;
+        return true;
+    }
+    case 2: {                                                             // IOR_EARTH 12257
+        const double r2 = x * x + y * y;
+        double c;
+        if (r2 > 1.0) c = 0.0;
+        else if (r2 < 0.25 * 0.25) c = 0.8 * a.courant;
+        else if (r2 < 0.58 * 0.58) c = a.courant * (0.68 - 0.55 * r2);
+        else c = a.courant * (1.3 - 0.9 * r2);
+        tcc = c;   // the reference stores c itself here
+        tg = G;
+        return true;
+    }
+    case 10: case 11: case 12: case 13: {                                 // IOR_LENS_* 12556-12682
+        // 10: |x| < 0.05, |y| > MU; 11: |x| < 0.05, |y| < MU; 12, 13: |x| < a0, |y| > MU
+        const double w = (a.model == 10 || a.model == 11) ? 0.05 : a.a0;
+        const bool wall = vabs_(x) < w && (a.model == 11 ? vabs_(y) < M : vabs_(y) > M);
+        if (wall) { tcc = 0.0; tg = G; }
+        else if (m != 0) { tcc = c2; tg = G; }
+        else { tcc = cb2; tg = GB; }
+        return true;
+    }
+    case 14:                                                              // IOR_TREE 12683
+        if (m == 0) { tcc = cb2; tg = GB; return true; }
+        if (m == 1 || m == 3) { tcc = c2; tg = G; return true; }
+        if (m == 2) { tcc = 0.0; tg = G; return true; }
+        return false;
+    case 15: case 151: {                                                  // IOR_WAVE_GUIDES_COUPLED 12720, _B 12746
+        const bool outside = a.model == 15 ? vabs_(x) > L : (vabs_(x) > L && vabs_(y) > 0.7);
+        if (m == 1) { tcc = c2; tg = G; }
+        else if (outside) { tcc = 0.0; tg = 1.0; }
+        else { tcc = cb2; tg = GB; }
+        return true;
+    }
+    case 16: {                                                            // IOR_WAVE_GUIDE_COATED 12772
+        const double cm = 0.8 * a.courant + 0.2 * a.courantb;
+        if (m == 1) { tcc = c2; tg = G; }
+        else if (m == 2) { tcc = cm * cm; tg = G; }
+        else { tcc = cb2; tg = GB; }
+        return true;
+    }
+    case 17:                                                              // IOR_MICHELSON 12800
+        if (m == 0) { tcc = c2; tg = G; }
+        else if (m == 2) { tcc = 0.0; tg = 1.0; }
+        else { tcc = cb2; tg = GB; }
+        return true;
+    case 18: case 181: {                                                  // IOR_GRADIENT_INDEX_LENS 12823, _B 12851
+        const double ax = vabs_(x), ay = vabs_(y);
+        if (ax > L) { tcc = c2; tg = G; }
+        else if (ay > 1.0) { tcc = 0.0; tg = 0.0; }
+        else if (a.model == 18) { tcc = cb2 / (1.0 - M * ay * ay); tg = GB; }
+        else {
+            double speed = 1.0 / (1.0 - M * ay * ay);
+            speed *= speed;
+            tcc = cb2 * speed;
+            tg = GB;
+        }
+        return true;
+    }
+    case 182: {                                                           // IOR_LUNEBURG_LENS 12881
+        const double r = module2_(x, y);
+        if (r > L) { tcc = c2; tg = G; }
+        else { tcc = c2 / (2.0 - r / L); tg = GB; }
+        return true;
+    }
+    case 183: {                                                           // IOR_LUNEBURG_LAYERS 12904
+        // (the layer index is only used, and so only formed, inside the lens)
+        const double r = module2_(x, y) / L;
+        if (r > 1.0) { tcc = c2; tg = G; return true; }
+        double r2 = r * (double)a.mdepth;
+        r2 = (double)((int)r2) / (double)a.mdepth;
+        tcc = c2 / (2.0 - r2);
+        tg = GB;
+        return true;
+    }
+    case 19: {                                                            // IOR_LINEAR_X_A 12929
+        const double speed = a.courant * (1.0 - x) + a.courantb * x;
+        tcc = speed * speed;
+        tg = G;
+        return true;
+    }
+    case 191: {                                                           // IOR_LINEAR_X_B 12945
+        const double s = (x - a.xmin) / (a.xmax - a.xmin);
+        const double speed = a.courant * (1.0 - s) + a.courantb * s;
+        tcc = speed * speed;
+        tg = G;
+        return true;
+    }
+    default: return false;  // not reached: the launcher admits only the codes above
+    }
+}
+
+__global__ void __launch_bounds__(256) k_ior(WvIor a)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (j >= a.ny || i >= a.nx) return;
+    const size_t k = (size_t)i * a.ny + j;
+    const short m = gc(a.xy)[k];
+    double tcc = 0.0, tc = 0.0, tg = 0.0;
+    if (a.model == 0) {
+        // VARIABLE_IOR == 0, sub_wave.c:12984-13003
+        if (m != 0) {
+            tc = a.courant;
+            tcc = a.courant * a.courant;
+            tg = m == 1 ? a.gamma : a.gammab;
+        } else if (a.twospeeds) {
+            tc = a.courantb;
+            tcc = a.courantb * a.courantb;
+            tg = a.gammab;
+        }
+    } else {
+        // ij_to_xy, sub_wave.c:469-476
+        const double x = a.xmin + ((double)i) * (a.xmax - a.xmin) / ((double)a.nx);
+        const double y = a.ymin + ((double)j) * (a.ymax - a.ymin) / ((double)a.ny);
+        if (!ior_cell(a, x, y, m, tcc, tg)) tcc = tg = 0.0;
+        // sub_wave.c:12970-12981
+        if (m != 0) tc = sqrt(tcc);
+        else if (a.twospeeds) tc = a.courantb;
+    }
+    gm(a.tcc)[k] = tcc;
+    gm(a.tc)[k] = tc;
+    gm(a.tg)[k] = tg;
+}
+
 // Fixed device scratch of the allocation-free mask builder: the circle list
 // (at most WV_MASK_SCRATCH_MAX_CIRCLES x 4 doubles) and then bands of mask
 // rows.  One copy per device, alive until the process ends; every call on a
@@ -162,6 +716,20 @@ int wv_launch_mask(const WvMask &a, hipStream_t s)
 {
     const dim3 block(256), grid((a.ny + 255) / 256, a.rows);
     hipLaunchKernelGGL(k_mask, grid, block, 0, s, a);
+    return (int)hipGetLastError();
+}
+
+int wv_launch_domain(const WvDomain &a, hipStream_t s)
+{
+    const dim3 block(256), grid((a.ny + 255) / 256, a.rows);
+    hipLaunchKernelGGL(k_domain, grid, block, 0, s, a);
+    return (int)hipGetLastError();
+}
+
+int wv_launch_ior(const WvIor &a, hipStream_t s)
+{
+    const dim3 block(256), grid((a.ny + 255) / 256, a.nx);
+    hipLaunchKernelGGL(k_ior, grid, block, 0, s, a);
     return (int)hipGetLastError();
 }
 

@@ -32,7 +32,8 @@ EXPORTS = [
     "wv_params_default", "wv_create", "wv_destroy", "wv_last_error", "wv_set_tables",
     "wv_set_state", "wv_set_state3", "wv_get_state", "wv_get_state3", "wv_set_sources",
     "wv_set_neumann", "wv_set_drive", "wv_set_floor", "wv_set_laplacian", "wv_set_probes", "wv_get_probes",
-    "wv_format_time_series", "wv_set_wells_tables", "wv_build_mask", "wv_build_mask_static", "wv_init_circular_wave",
+    "wv_format_time_series", "wv_set_wells_tables", "wv_build_mask", "wv_build_mask_static", "wv_build_domain_mask",
+    "wv_set_ior_tables", "wv_rotating_wells", "wv_init_circular_wave",
     "wv_add_circular_wave", "wv_batch_init_circular_wave", "wv_batch_add_circular_wave", "wv_tiff_bound", "wv_tiff_encode", "wv_tiff_write",
     "wv_step", "wv_prepare", "wv_step_timed", "wv_sync", "wv_steps_done", "wv_last_kernel_ms", "wv_energy",
     "wv_get_tiling", "wv_get_tile_counts", "wv_frame_params_default", "wv_frame_set_tables", "wv_frame_reset", "wv_frame",
@@ -84,6 +85,62 @@ _SS = ctypes.POINTER(_S)
 _lib = None
 
 
+class WvDomainParams(ctypes.Structure):
+    """struct wv_domain_params of include/wavehip.h"""
+    _fields_ = [(n, ctypes.c_double) for n in ("lam", "mu", "mu_b", "apoly", "wall_width", "wall_width_b",
+                                                "julia_scale", "julia_x", "julia_y", "mandellimit")] + \
+               [(n, ctypes.c_int) for n in ("npoly", "mdepth", "mratio", "mandellevel", "ngridx", "ngridy")]
+
+
+class WvIorParams(ctypes.Structure):
+    """struct wv_ior_params of include/wavehip.h"""
+    _fields_ = [(n, ctypes.c_double) for n in ("courant", "courantb", "gamma", "gammab", "lam", "mu",
+                                                "mandel_ior_scale", "mandellimit")] + \
+               [(n, ctypes.c_int) for n in ("mandellevel", "mdepth")]
+
+
+# the domain macros of the checked-in wave_billiard.c (73-109) and julia_x,
+# julia_y of global_pdes.c:567
+DOMAIN_DEFAULTS = dict(lam=0.2, mu=0.008, mu_b=0.08, apoly=1.0, wall_width=0.075, wall_width_b=0.1,
+                       julia_scale=0.8, julia_x=0.37468, julia_y=0.21115, mandellimit=10.0, npoly=9, mdepth=6,
+                       mratio=3, mandellevel=1000, ngridx=10, ngridy=10)
+# init_ior_2d's macros of the checked-in wave_billiard.c (54, 92-100, 140-143)
+IOR_DEFAULTS = dict(courant=0.09, courantb=0.25, gamma=0.0, gammab=0.0, lam=0.2, mu=0.008, mandel_ior_scale=-0.05,
+                    mandellimit=10.0, mandellevel=1000, mdepth=6)
+
+
+def _struct(cls, defaults, kw):
+    d = dict(defaults)
+    bad = set(kw) - set(d)
+    if bad:
+        raise KeyError("unknown parameters: %s" % sorted(bad))
+    d.update(kw)
+    return cls(**d)
+
+
+def domain_params(**kw):
+    """wv_domain_params from DOMAIN_DEFAULTS updated with kw"""
+    return _struct(WvDomainParams, DOMAIN_DEFAULTS, kw)
+
+
+def ior_params(**kw):
+    """wv_ior_params from IOR_DEFAULTS updated with kw"""
+    return _struct(WvIorParams, IOR_DEFAULTS, kw)
+
+
+def rotating_wells(ngridx, ngridy, large, angle, box):
+    """the rotating-well centres at angle (wv_rotating_wells): (xc, yc, sigma)"""
+    bx = np.ascontiguousarray(box, np.float64).ravel()
+    if bx.size != 4:
+        raise ValueError("box must hold XMIN, XMAX, YMIN, YMAX")
+    xc, yc = np.empty(ngridx * ngridy), np.empty(ngridx * ngridy)
+    s = ctypes.c_double(0)
+    rc = load().wv_rotating_wells(ngridx, ngridy, int(large), angle, _d(bx), _d(xc), _d(yc), ctypes.byref(s))
+    if rc:
+        raise WaveError("wv_rotating_wells: %d" % rc)
+    return xc, yc, s.value
+
+
 def format_time_series(values):
     """the SAVE_TIME_SERIES text of one probe's samples (wave_billiard.c:902-905)"""
     v = np.ascontiguousarray(values, dtype=np.float64).ravel()
@@ -129,6 +186,11 @@ def load():
                                          _D, _S]),
         "wv_build_mask_static": (ctypes.c_int, [vp, ctypes.c_int, _D, ctypes.c_double, ctypes.c_double,
                                                 ctypes.c_int, _D, _S]),
+        "wv_build_domain_mask": (ctypes.c_int, [vp, ctypes.c_int, _D, ctypes.POINTER(WvDomainParams), ctypes.c_int,
+                                                _D, _S]),
+        "wv_set_ior_tables": (ctypes.c_int, [vp, _S, ctypes.c_int, _D, ctypes.POINTER(WvIorParams), _D, _D, _D]),
+        "wv_rotating_wells": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _D, _D, _D,
+                                             _D]),
         "wv_init_circular_wave": (ctypes.c_int, [vp] + [ctypes.c_double] * 5 + [_D]),
         "wv_add_circular_wave": (ctypes.c_int, [vp] + [ctypes.c_double] * 6 + [_D]),
         "wv_batch_init_circular_wave": (ctypes.c_int, [vp, ctypes.c_int] + [ctypes.c_double] * 5 + [_D]),
@@ -410,6 +472,39 @@ class Wave:
         self._chk(f(self._h, int(domain), _d(bx), lam, mu, n, _d(c), out.ctypes.data_as(_S)),
                   "wv_build_mask_static" if static else "wv_build_mask")
         return out
+
+    def build_domain_mask(self, domain, box, params=None, circles=None, **kw):
+        """xy_in_billiard_single_domain over the grid on the GPU for any
+        closed-form B_DOMAIN code (wv_build_domain_mask); params: a
+        WvDomainParams, or keywords over DOMAIN_DEFAULTS; circles (4, n) for
+        the D_CIRCLES family.  Returns int16 (nx, ny)."""
+        P = params if params is not None else domain_params(**kw)
+        c = np.zeros((4, 1)) if circles is None else np.ascontiguousarray(circles, np.float64)
+        if c.ndim != 2 or c.shape[0] != 4:
+            raise ValueError("circles must be (4, n): xc, yc, radius, active")
+        n = 0 if circles is None else c.shape[1]
+        bx = np.ascontiguousarray(box, np.float64).ravel()
+        if bx.size != 4:
+            raise ValueError("box must hold XMIN, XMAX, YMIN, YMAX")
+        out = np.empty(self.shape, np.int16)
+        self._chk(self._L.wv_build_domain_mask(self._h, int(domain), _d(bx), ctypes.byref(P), n, _d(c),
+                                               out.ctypes.data_as(_S)), "wv_build_domain_mask")
+        return out
+
+    def set_ior_tables(self, xy_in, ior, box, params=None, copy=True, **kw):
+        """init_ior_2d's per-cell media model ior built on the GPU and
+        installed (wv_set_ior_tables); params: a WvIorParams, or keywords over
+        IOR_DEFAULTS.  Returns (tcc, tc, tgamma), or None without copy."""
+        P = params if params is not None else ior_params(**kw)
+        xy = _i16(xy_in, self.shape)
+        bx = np.ascontiguousarray(box, np.float64).ravel()
+        if bx.size != 4:
+            raise ValueError("box must hold XMIN, XMAX, YMIN, YMAX")
+        t = [np.empty(self.shape) for _ in range(3)] if copy else None
+        outs = [_d(a) for a in t] if copy else [None, None, None]
+        self._chk(self._L.wv_set_ior_tables(self._h, xy.ctypes.data_as(_S), int(ior), _d(bx), ctypes.byref(P),
+                                            *outs), "wv_set_ior_tables")
+        return tuple(t) if copy else None
 
     def init_circular_wave(self, x0, y0, amp, variance, wavelength, box):
         """init_circular_wave (wave_common.c:89) on the GPU state"""
