@@ -407,7 +407,8 @@ int wv_energy(wv_ctx *c, double *e);
  * (wave_common.c:1053-1171, plot types P_AMPLITUDE 0 .. P_LOG_AVERAGE_ENERGY 9
  * with their running sums), colour schemes C_LUM 0, C_HUE 1, C_ONEDIM 3,
  * C_ONEDIM_LINEAR 4, C_BASIC_LINEAR 5 (colors_waves.c:161-290), palettes
- * COL_JET 0, COL_GRAY 2 and the table palettes COL_TURBO 10 .. COL_PARULA 16,
+ * COL_JET 0, COL_HSLUV 1 (hsluv2rgb of hsluv.c:396-410, the HSLuv library the
+ * reference vendors), COL_GRAY 2 and the table palettes COL_TURBO 10 .. COL_PARULA 16,
  * COL_TWILIGHT 17, COL_TWILIGHT_SHIFTED 18, COL_TURBO_CYCLIC 101
  * (colormaps.c:2077-2548; the program's own table, 256 or 511 entries, via
  * wv_frame_set_tables), then SHADE_2D shading (gradient towards blocks

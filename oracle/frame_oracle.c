@@ -15,7 +15,7 @@ enum { P_AMPLITUDE, P_ENERGY, P_MIXED, P_MEAN_ENERGY, P_LOG_ENERGY, P_LOG_MEAN_E
 /* colour schemes, global_pdes.c:354-359 */
 enum { C_LUM = 0, C_HUE = 1, C_ONEDIM = 3, C_ONEDIM_LINEAR = 4, C_BASIC_LINEAR = 5 };
 /* palettes, global_pdes.c:364-378 */
-enum { COL_JET = 0, COL_GRAY = 2, COL_TURBO = 10, COL_PARULA = 16, COL_TWILIGHT = 17,
+enum { COL_JET = 0, COL_HSLUV = 1, COL_GRAY = 2, COL_TURBO = 10, COL_PARULA = 16, COL_TWILIGHT = 17,
        COL_TWILIGHT_SHIFTED = 18, COL_TURBO_CYCLIC = 101 };
 
 /* the reference's own constants, global_pdes.c:8-10 */
@@ -45,6 +45,72 @@ static void hsl_jet(double h, double s, double l, double rgb[3])
     else { rgb[0] = m; rgb[1] = m; rgb[2] = m; }
 }
 
+/* HSLuv to sRGB, the published HSLuv algorithm (hsluv.org) that the
+ * reference vendors as hsluv-c (hsluv.c:396-410): HSLuv -> LCh with the
+ * chroma scaled to the largest one inside the sRGB gamut at that lightness
+ * and hue -> CIELUV -> CIEXYZ (D65) -> sRGB.  The sRGB matrix, the white
+ * point's (u', v') and CIE's kappa and epsilon are the library's values
+ * (hsluv.c:43-58); every operation is in the library's order, so results
+ * are the library's bit for bit. */
+static const double HS_M[3][3] = {
+    {3.24096994190452134377, -1.53738317757009345794, -0.49861076029300328366},
+    {-0.96924363628087982613, 1.87596750150772066772, 0.04155505740717561247},
+    {0.05563007969699360846, -0.20397695888897656435, 1.05697151424287856072}};
+#define HS_REF_U 0.19783000664283680764
+#define HS_REF_V 0.46831999493879100370
+#define HS_KAPPA 903.29629629629629629630
+#define HS_EPS 0.00885645167903563082
+#define HS_DEG 0.01745329251994329577
+
+/* the shortest non-negative distance from the L axis along hue h to the six
+ * gamut lines (one per channel at 0 and at 1) of lightness l */
+static double hs_max_chroma(double l, double h)
+{
+    const double tl = l + 16.0, sub1 = (tl * tl * tl) / 1560896.0;
+    const double sub2 = sub1 > HS_EPS ? sub1 : (l / HS_KAPPA);
+    const double hrad = h * HS_DEG, sh = sin(hrad), chh = cos(hrad);
+    double best = 1.7976931348623157e308;
+    for (int ch = 0; ch < 3; ch++)
+        for (int t = 0; t < 2; t++) {
+            const double m1 = HS_M[ch][0], m2 = HS_M[ch][1], m3 = HS_M[ch][2];
+            const double top1 = (284517.0 * m1 - 94839.0 * m3) * sub2;
+            const double top2 = (838422.0 * m3 + 769860.0 * m2 + 731718.0 * m1) * l * sub2 - 769860.0 * t * l;
+            const double bottom = (632260.0 * m3 - 126452.0 * m2) * sub2 + 126452.0 * t;
+            const double slope = top1 / bottom, icpt = top2 / bottom;
+            const double len = icpt / (sh - slope * chh);
+            if (len >= 0 && len < best) best = len;
+        }
+    return best;
+}
+
+static double hs_gamma(double c)
+{
+    return c <= 0.0031308 ? 12.92 * c : 1.055 * pow(c, 1.0 / 2.4) - 0.055;
+}
+
+static void hsluv_rgb(double h, double s, double l, double rgb[3])
+{
+    /* HSLuv -> LCh */
+    const double c = (l > 99.9999999 || l < 0.00000001) ? 0.0 : hs_max_chroma(l, h) / 100.0 * s;
+    if (s < 0.00000001) h = 0.0;
+    /* LCh -> Luv */
+    const double hrad = h * HS_DEG, u = cos(hrad) * c, v = sin(hrad) * c;
+    /* Luv -> XYZ */
+    double X = 0.0, Y = 0.0, Z = 0.0;
+    if (l > 0.00000001) {
+        const double vu = u / (13.0 * l) + HS_REF_U, vv = v / (13.0 * l) + HS_REF_V;
+        if (l <= 8.0) Y = l / HS_KAPPA;
+        else {
+            const double q = (l + 16.0) / 116.0;
+            Y = q * q * q;
+        }
+        X = -(9.0 * Y * vu) / ((vu - 4.0) * vv - vu * vv);
+        Z = (9.0 * Y - (15.0 * vv * Y) - (vv * X)) / (3.0 * vv);
+    }
+    /* XYZ -> sRGB */
+    for (int k = 0; k < 3; k++) rgb[k] = hs_gamma(HS_M[k][0] * X + HS_M[k][1] * Y + HS_M[k][2] * Z);
+}
+
 /* colormaps.c:2343-2447 (hue in degrees) */
 static void hsl_palette(double h, double s, double l, double rgb[3], int palette, const float *lut, int n)
 {
@@ -52,6 +118,10 @@ static void hsl_palette(double h, double s, double l, double rgb[3], int palette
     int ch;
     if (palette == COL_JET) {
         hsl_jet(h, s, l, rgb);
+    } else if (palette == COL_HSLUV) {  /* colormaps.c:2355-2362 */
+        double q[3];
+        hsluv_rgb(h, 100.0 * s, l, q);
+        for (int c = 0; c < 3; c++) rgb[c] = 2.0 * l * q[c] * 100.0;
     } else if (palette == COL_GRAY) {
         ch = h / 360.0;
         rgb[0] = rgb[1] = rgb[2] = ch;
@@ -87,6 +157,10 @@ static void amp_palette(double h, double rgb[3], int palette, const float *lut, 
     if (ch > 255) ch = 255;
     if (palette == COL_JET) {
         hsl_jet(360.0 * (1.0 - h), 0.9, 0.5, rgb);
+    } else if (palette == COL_HSLUV) {  /* colormaps.c:2463-2470 */
+        double q[3];
+        hsluv_rgb(360.0 * (1.0 - h), 90.0, 0.5, q);
+        for (int c = 0; c < 3; c++) rgb[c] = q[c] * 100.0;
     } else if (palette == COL_GRAY) {
         rgb[0] = rgb[1] = rgb[2] = h;
     } else if (palette >= COL_TURBO && palette <= COL_PARULA) {
@@ -348,3 +422,5 @@ void orc_frame(const orc_params *p, const orc_frame_params *f, const double *phi
         }
     free(rv);
 }
+
+void orc_hsluv_rgb(double h, double s, double l, double rgb[3]) { hsluv_rgb(h, s, l, rgb); }

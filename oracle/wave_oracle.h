@@ -217,6 +217,10 @@ void orc_wells_tables(const orc_params *p, const short *xy, int nw, const double
 int orc_mask(int NX, int NY, const double box[4], int domain, double lambda, double mu, int ncirc,
              const double *circ, short *out);
 
+/* HSLuv to sRGB (hsluv2rgb of the hsluv-c library the reference vendors,
+ * hsluv.c:396-410): h in degrees, s and l in 0..100, rgb in 0..1 */
+void orc_hsluv_rgb(double h, double s, double l, double rgb[3]);
+
 /* The macros of xy_in_billiard_single_domain (sub_wave.c:5712) the closed-form
  * domains read; same layout as the library's wv_domain_params. */
 typedef struct {

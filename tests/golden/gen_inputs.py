@@ -169,10 +169,10 @@ def color_inputs(tag, nx=24, ny=17):
 def color_cases():
     """(tag, params, palette, plot) with a colour for every cell: the schemes
     C_ONEDIM_LINEAR / C_BASIC_LINEAR set none for the asymmetric scheme
-    (P_ENERGY_FLUX)"""
+    (P_ENERGY_FLUX); palettes COL_JET, COL_HSLUV, COL_GRAY"""
     out = []
     for tag, prm in color_builds():
-        for pal in (0, 2):
+        for pal in (0, 1, 2):
             for plot in range(10):
                 if plot == 6 and prm["scheme"] in (4, 5):
                     continue

@@ -15,7 +15,8 @@ Outputs (committed):
                     the drive values oscillating_bc(time, j) and phi, psi,
                     tmp after EXT_STEPS steps, for every gen_inputs.ext_cases()
   frame_digests.json  the frame colours of wave_value (wave_common.c:1053)
-                    through the colour schemes and palettes COL_JET, COL_GRAY:
+                    through the colour schemes and palettes COL_JET,
+                    COL_HSLUV, COL_GRAY (python make_golden.py frames):
                     SHA-256 of the values and colours of every cell, frames
                     0..COLOR_FRAMES-1, for every gen_inputs.color_cases()
   cmp_cases.npz     the flat twin with COMPARISON (two halves): phi, psi, tmp
@@ -222,7 +223,13 @@ def main():
         out[tag + "/tcc"], out[tag + "/tc"], out[tag + "/tgamma"] = R.init_ior(d["xy_in"], 7)
         out[tag + "/xc"], out[tag + "/yc"], out[tag + "/h"], out[tag + "/sigma"] = R.wells(7)
     np.savez_compressed(os.path.join(HERE, "media_cases.npz"), **out)
-    # frame colours: wave_value cell by cell (size 1), COL_JET and COL_GRAY
+    frames()
+    print("fixtures written")
+
+
+def frames():
+    """frame colours: wave_value cell by cell (size 1), COL_JET, COL_HSLUV and
+    COL_GRAY -> frame_digests.json"""
     dig = {}
     for (tag, prm, pal, plot) in color_cases():
         R = pyoracle.RefBuild(tag)
@@ -240,7 +247,6 @@ def main():
         dig["%s/pal%d/plot%d" % (tag, pal, plot)] = frames
     with open(os.path.join(HERE, "frame_digests.json"), "w") as f:
         json.dump(dig, f, indent=0)
-    print("fixtures written")
 
 
 def tiffs():
@@ -257,6 +263,8 @@ def tiffs():
 if __name__ == "__main__":
     if sys.argv[1:] == ["masks"]:
         masks()
+    elif sys.argv[1:] == ["frames"]:
+        frames()
     elif sys.argv[1:] == ["domains"]:
         domains()
     elif sys.argv[1:] == ["iors"]:

@@ -32,6 +32,10 @@ CONFIGS = [
     (9, 3, 16, 0, 2, 1, {}),
     (0, 3, 0, 0, 1, 1, dict(rescale_center=1, fade_in_obstacle=1, fade=1, fade_value=0.7)),
     (1, 0, 10, 0, 1, 1, dict(attenuation=0.02, color_range=1.0, colordrift=40.0)),
+    (0, 3, 1, 0, 1, 1, {}),
+    (1, 1, 1, 1, 2, 0, {}),
+    (3, 0, 1, 0, 1, 1, {}),
+    (4, 3, 1, 1, 1, 1, {}),
 ]
 
 
@@ -92,7 +96,7 @@ def test_frame_argument_checks(wh):
     W = wh.Wave(w["params"])
     W.set_tables(w["xy_in"], w["tcc"], w["tc"], w["tgamma"])
     W.set_state(w["phi"], w["psi"])
-    for bad in (dict(plot=10), dict(scheme=2), dict(palette=1), dict(palette=10), dict(size=0),
+    for bad in (dict(plot=10), dict(scheme=2), dict(palette=3), dict(palette=10), dict(size=0),
                 dict(plot=6, scheme=4), dict(rescale_center=1), dict(fade_in_obstacle=1)):
         with pytest.raises(wh.WaveError):
             W.frame(W.frame_params(**bad))

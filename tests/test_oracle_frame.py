@@ -3,7 +3,8 @@
 tests/golden/frame_digests.json holds SHA-256 digests of the values and
 colours wave_value (wave_common.c:1053-1171) computes cell by cell, through
 color_scheme_palette / color_scheme_asym_palette (colors_waves.c) and the
-palettes COL_JET and COL_GRAY (colormaps.c), for 8 reference builds (every
+palettes COL_JET, COL_HSLUV and COL_GRAY (colormaps.c; COL_HSLUV through
+hsluv2rgb of the hsluv-c library the reference vendors), for 8 reference builds (every
 colour scheme, the asymmetric energy scheme, attenuation and
 RESCALE_COLOR_IN_CENTER), 10 plot types and 3 frames of running sums.  The
 table palettes are checked against the reference build directly when
@@ -162,3 +163,55 @@ def test_block_size_uses_block_cells(oracle):
     S, lx, ly = 0.5, 0.40824829, 0.816496581
     ca = ((-gx * lx - gy * ly + S) / np.sqrt(S * S + gx * gx + gy * gy) + 1.0) * 0.4 + 0.2
     assert np.allclose(s2[::2, ::2][:-1, :-1], c1[::2, ::2][:-1, :-1] * ca[..., None], rtol=1e-14, atol=0)
+
+
+def hsluv(oracle, h, s, l):
+    import ctypes
+    L = oracle.lib()
+    L.orc_hsluv_rgb.argtypes = [ctypes.c_double] * 3 + [ctypes.POINTER(ctypes.c_double)]
+    out = (ctypes.c_double * 3)()
+    L.orc_hsluv_rgb(h, s, l, out)
+    return np.array(out[:])
+
+
+def test_hsluv_closed_forms(oracle):
+    """HSLuv's defining property: at saturation 100 the colour lies on the
+    sRGB gamut boundary (the largest in-gamut chroma), so a channel is 0 or
+    1; below it the colour is inside.  Lightness 0 is black, 100 is white
+    (the D65 white point through the sRGB matrix), saturation 0 is grey."""
+    rng = np.random.default_rng(4)
+    for _ in range(400):
+        h, l = rng.uniform(0, 360), rng.uniform(1, 99)
+        c = hsluv(oracle, h, 100.0, l)
+        assert min(abs(c.min()), abs(c.max() - 1.0)) < 1e-9, (h, l, c)
+        inner = hsluv(oracle, h, 60.0, l)
+        assert inner.min() > 1e-9 and inner.max() < 1.0 - 1e-9, (h, l, inner)
+    for h in (0.0, 77.0, 300.0):
+        assert np.array_equal(hsluv(oracle, h, 50.0, 0.0), np.zeros(3))
+        assert np.allclose(hsluv(oracle, h, 50.0, 100.0), 1.0, atol=1e-12)
+        g = hsluv(oracle, h, 0.0, 37.0)
+        assert np.allclose(g, g[0], atol=1e-14) and 0.0 < g[0] < 1.0
+    # lightness is monotone along a grey ramp and the hue turns the colour
+    ramp = [hsluv(oracle, 0.0, 0.0, l)[0] for l in np.linspace(1, 99, 50)]
+    assert np.all(np.diff(ramp) > 0)
+    assert not np.allclose(hsluv(oracle, 10.0, 80.0, 50.0), hsluv(oracle, 130.0, 80.0, 50.0))
+
+
+@pytest.mark.skipif(not __import__("pyoracle").ref_available(), reason="oracle/_ref not built (needs /root/reference)")
+def test_hsluv_matches_reference_build(oracle):
+    """orc_hsluv_rgb against hsluv2rgb of the reference's colour build, bit for
+    bit, over hues beyond [0, 360) and the lightness edges of the library"""
+    import ctypes
+    import pyoracle
+    R = pyoracle.RefBuild("col_s3")
+    D = ctypes.c_double
+    R.L.hsluv2rgb.argtypes = [D, D, D, ctypes.POINTER(D), ctypes.POINTER(D), ctypes.POINTER(D)]
+    rng = np.random.default_rng(6)
+    for _ in range(3000):
+        h = rng.uniform(-20, 380)
+        s = rng.choice([0.0, 1e-9, 90.0, 100.0, rng.uniform(0, 100)])
+        l = rng.choice([0.0, 1e-9, 0.5, 7.99, 8.0, 8.01, 99.99999995, 100.0, rng.uniform(0, 1), rng.uniform(0, 100)])
+        r, g, b = D(), D(), D()
+        R.L.hsluv2rgb(h, s, l, ctypes.byref(r), ctypes.byref(g), ctypes.byref(b))
+        assert np.array_equal(np.array([r.value, g.value, b.value]).view(np.uint64),
+                              hsluv(oracle, h, s, l).view(np.uint64)), (h, s, l)

@@ -1908,8 +1908,9 @@ int frame(Engine *e, int k, const wv_frame_params *f, void *rgba, bool u8)
     if (sch != 0 && sch != 1 && sch != 3 && sch != 4 && sch != 5)
         return fail(e, WV_EUNSUPPORTED, "colour scheme %d (0, 1, 3, 4, 5 are provided)", sch);
     const bool table = (pal >= 10 && pal <= 18) || pal == 101;
-    if (pal != 0 && pal != 2 && !table)
-        return fail(e, WV_EUNSUPPORTED, "palette %d (COL_JET, COL_GRAY and the table palettes are provided)", pal);
+    if (pal != 0 && pal != 1 && pal != 2 && !table)
+        return fail(e, WV_EUNSUPPORTED, "palette %d (COL_JET, COL_HSLUV, COL_GRAY and the table palettes are provided)",
+                    pal);
     const int need = (pal == 17 || pal == 18) ? 511 : 256;
     if (table && G.fr_lut_n != need)
         return fail(e, WV_ESTATE, "palette %d needs its table of %d entries (wv_frame_set_tables)", pal, need);

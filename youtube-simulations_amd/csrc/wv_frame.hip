@@ -27,7 +27,7 @@ namespace {
 constexpr int P_AMPLITUDE = 0, P_ENERGY = 1, P_MIXED = 2, P_MEAN_ENERGY = 3, P_LOG_ENERGY = 4,
               P_LOG_MEAN_ENERGY = 5, P_ENERGY_FLUX = 6, P_TOTAL_ENERGY_FLUX = 7, P_AVERAGE_ENERGY = 8;
 constexpr int C_LUM = 0, C_HUE = 1, C_ONEDIM = 3, C_ONEDIM_LINEAR = 4, C_BASIC_LINEAR = 5;
-constexpr int COL_JET = 0, COL_GRAY = 2, COL_TURBO = 10, COL_PARULA = 16, COL_TWILIGHT = 17,
+constexpr int COL_JET = 0, COL_HSLUV = 1, COL_GRAY = 2, COL_TURBO = 10, COL_PARULA = 16, COL_TWILIGHT = 17,
               COL_TWILIGHT_SHIFTED = 18, COL_TURBO_CYCLIC = 101;
 constexpr double REF_PI = 3.141592654, REF_DPI = 6.283185307, REF_PID = 1.570796327;
 
@@ -56,6 +56,65 @@ __device__ inline Rgb hsl_jet(double h, double s, double l)
     return {{m, m, m}};
 }
 
+// HSLuv to sRGB (the published HSLuv algorithm, hsluv.org, which the
+// reference vendors as hsluv-c: hsluv2rgb, hsluv.c:396-410).  The chroma of
+// (L, H) is the HSLuv saturation times the largest chroma inside the sRGB
+// gamut there: the nearest of six lines (each channel at 0 and at 1) along
+// the hue's ray.  Then CIELUV -> CIEXYZ (D65) -> linear RGB -> sRGB gamma.
+// The sRGB matrix, the white point's (u', v') and CIE's kappa and epsilon are
+// the library's values; the library's order of operations is kept (sin, cos
+// and pow are the device's).
+__constant__ const double HS_M[9] = {
+    3.24096994190452134377,  -1.53738317757009345794, -0.49861076029300328366,
+    -0.96924363628087982613, 1.87596750150772066772,  0.04155505740717561247,
+    0.05563007969699360846,  -0.20397695888897656435, 1.05697151424287856072};
+constexpr double HS_REF_U = 0.19783000664283680764, HS_REF_V = 0.46831999493879100370;
+constexpr double HS_KAPPA = 903.29629629629629629630, HS_EPS = 0.00885645167903563082;
+constexpr double HS_DEG = 0.01745329251994329577;
+
+__device__ inline double hs_gamma(double c)
+{
+    return c <= 0.0031308 ? 12.92 * c : 1.055 * pow(c, 1.0 / 2.4) - 0.055;
+}
+
+__device__ Rgb hsluv(double h, double s, double l)
+{
+    double c = 0.0;
+    if (!(l > 99.9999999 || l < 0.00000001)) {
+        const double tl = l + 16.0, sub1 = (tl * tl * tl) / 1560896.0;
+        const double sub2 = sub1 > HS_EPS ? sub1 : (l / HS_KAPPA);
+        const double hr = h * HS_DEG, sh = sin(hr), ch = cos(hr);
+        double best = 1.7976931348623157e308;
+        for (int q = 0; q < 6; q++) {
+            const double m1 = HS_M[3 * (q >> 1)], m2 = HS_M[3 * (q >> 1) + 1], m3 = HS_M[3 * (q >> 1) + 2];
+            const double t = (double)(q & 1);
+            const double top1 = (284517.0 * m1 - 94839.0 * m3) * sub2;
+            const double top2 = (838422.0 * m3 + 769860.0 * m2 + 731718.0 * m1) * l * sub2 - 769860.0 * t * l;
+            const double bottom = (632260.0 * m3 - 126452.0 * m2) * sub2 + 126452.0 * t;
+            const double len = (top2 / bottom) / (sh - (top1 / bottom) * ch);
+            if (len >= 0 && len < best) best = len;
+        }
+        c = best / 100.0 * s;
+    }
+    if (s < 0.00000001) h = 0.0;
+    const double hr = h * HS_DEG, u = cos(hr) * c, v = sin(hr) * c;
+    double X = 0.0, Y = 0.0, Z = 0.0;
+    if (l > 0.00000001) {
+        const double vu = u / (13.0 * l) + HS_REF_U, vv = v / (13.0 * l) + HS_REF_V;
+        if (l <= 8.0) {
+            Y = l / HS_KAPPA;
+        } else {
+            const double w = (l + 16.0) / 116.0;
+            Y = w * w * w;
+        }
+        X = -(9.0 * Y * vu) / ((vu - 4.0) * vv - vu * vv);
+        Z = (9.0 * Y - (15.0 * vv * Y) - (vv * X)) / (3.0 * vv);
+    }
+    Rgb r;
+    for (int k = 0; k < 3; k++) r.c[k] = hs_gamma(HS_M[3 * k] * X + HS_M[3 * k + 1] * Y + HS_M[3 * k + 2] * Z);
+    return r;
+}
+
 __device__ inline Rgb lut_rgb(const WvFrame &f, int idx, double mul, bool scaled)
 {
     Rgb r;
@@ -76,6 +135,11 @@ __device__ inline Rgb hsl_palette(const WvFrame &f, double h, double s, double l
     const double ratio = 0.711111111;
     const int pal = f.palette;
     if (pal == COL_JET) return hsl_jet(h, s, l);
+    if (pal == COL_HSLUV) {  // colormaps.c:2355-2362
+        Rgb r = hsluv(h, 100.0 * s, l);
+        for (int c = 0; c < 3; c++) r.c[c] = 2.0 * l * r.c[c] * 100.0;
+        return r;
+    }
     if (pal == COL_GRAY) {
         const int ch = (int)(h / 360.0);
         return {{(double)ch, (double)ch, (double)ch}};
@@ -106,6 +170,11 @@ __device__ inline Rgb amp_palette(const WvFrame &f, double h)
     int ch = (int)(256.0 * h);
     if (ch > 255) ch = 255;
     if (pal == COL_JET) return hsl_jet(360.0 * (1.0 - h), 0.9, 0.5);
+    if (pal == COL_HSLUV) {  // colormaps.c:2463-2470
+        Rgb r = hsluv(360.0 * (1.0 - h), 90.0, 0.5);
+        for (int c = 0; c < 3; c++) r.c[c] = r.c[c] * 100.0;
+        return r;
+    }
     if (pal == COL_GRAY) return {{h, h, h}};
     if (pal >= COL_TURBO && pal <= COL_PARULA) return lut_rgb(f, ch, 1.0, false);
     if (pal == COL_TWILIGHT) {
