@@ -16,6 +16,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -2910,19 +2912,27 @@ static int set_ior_tables(Engine *e, int k, const short *xy, int ior, const doub
         a.xy = d_xy; a.tcc = d_t; a.tc = d_t + cells; a.tg = d_t + 2 * cells;
         err = (hipError_t)wv_launch_ior(a, e->stream);
     }
-    std::vector<double> h(3 * cells);
-    if (err == hipSuccess)
-        err = hipMemcpyAsync(h.data(), d_t, 3 * cells * sizeof(double), hipMemcpyDeviceToHost, e->stream);
+    // the tables come back straight into the caller's arrays when all three
+    // are given, else into uninitialised staging
+    const bool direct = tcc_out && tc_out && tg_out;
+    std::unique_ptr<double[]> stage(direct ? nullptr : new (std::nothrow) double[3 * cells]);
+    if (!direct && !stage) err = hipErrorOutOfMemory;
+    double *h[3] = {direct ? tcc_out : stage.get(), direct ? tc_out : stage.get() + cells,
+                    direct ? tg_out : stage.get() + 2 * cells};
+    for (int q = 0; q < 3 && err == hipSuccess; q++)
+        err = hipMemcpyAsync(h[q], d_t + q * cells, cells * sizeof(double), hipMemcpyDeviceToHost, e->stream);
     if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
     (void)hipFree(d_xy);
     (void)hipFree(d_t);
     if (err != hipSuccess) return fail(e, WV_EHIP, "media tables: %s", hipGetErrorString(err));
     // installed as wv_set_tables installs tables (media classes, masks, tiles)
-    const int rc = set_tables(e, k, xy, h.data(), h.data() + cells, h.data() + 2 * cells);
+    const int rc = set_tables(e, k, xy, h[0], h[1], h[2]);
     if (rc) return rc;
-    if (tcc_out) memcpy(tcc_out, h.data(), cells * sizeof(double));
-    if (tc_out) memcpy(tc_out, h.data() + cells, cells * sizeof(double));
-    if (tg_out) memcpy(tg_out, h.data() + 2 * cells, cells * sizeof(double));
+    if (!direct) {
+        double *o[3] = {tcc_out, tc_out, tg_out};
+        for (int q = 0; q < 3; q++)
+            if (o[q]) memcpy(o[q], h[q], cells * sizeof(double));
+    }
     return WV_OK;
 }
 
