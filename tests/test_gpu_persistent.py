@@ -121,3 +121,33 @@ def test_persistent_batch_matches_separate_launches(wh):
     for q in range(len(ws)):
         assert bits_equal(sa[q][0], sb[q][0]) and bits_equal(sa[q][1], sb[q][1]), q
     assert np.array_equal(ea, eb)
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_kernel_times_survive_prepare(wh, form):
+    """the per-launch durations of a timed run are the run's own: read after a
+    later wv_prepare (which records the timing events to set them up) they
+    equal those read at once on a twin context, to within run-to-run noise,
+    and a timed run's total covers its launches"""
+    from wavehip import workloads
+    w = workloads.random_case(1024, 1024, 5, bcond=2, neg_cells=False)
+    A = wh.Wave(w["params"], form=form, tb_depth=8, rows_per_tile=64)
+    B = wh.Wave(w["params"], form=form, tb_depth=8, rows_per_tile=64)
+    try:
+        for V in (A, B):
+            V.load(w)
+            V.prepare(64)
+            V.step_timed(64)  # warm
+        ms_a, nl_a = A.step_timed(64)
+        avg_a, n_a = A.kernel_ms()
+        _, nl_b = B.step_timed(64)
+        B.prepare(40)
+        avg_b, n_b = B.kernel_ms()
+        # (every launch of a 64-step run has the full depth: 64 = 8 x 8, or 64 x 1
+        # where the FP64 form steps one level a launch)
+        assert n_a == nl_a and n_b == nl_b and nl_a == nl_b
+        assert 0.5 * avg_a < avg_b < 2.0 * avg_a, (avg_a, avg_b)
+        assert n_a * avg_a <= ms_a * 1.05
+    finally:
+        A.close()
+        B.close()

@@ -119,9 +119,13 @@ struct Engine {
     bool wide = false;
     // per-kernel timing of the last wv_step_timed: one event before the first
     // stepping launch and one after every launch (kev[q], kev[q+1] bracket
-    // launch q; consecutive launches share an event)
+    // launch q; consecutive launches share an event).  These take timestamps
+    // only (no system-scope fence between launches: that writes the caches
+    // back and costs ~5 us a record at 4096^2); kev_end, recorded after the
+    // last, is an ordinary event, the one the host waits for.
     bool time_kernels = false;
     std::vector<hipEvent_t> kev;
+    hipEvent_t kev_end = nullptr;
     std::vector<int> kev_depth;  // fused depth of launch q
     std::vector<int> kev_epochs;  // launches of that depth it stands for (persistent: several)
     size_t kev_used = 0;         // events recorded (the last one follows the step counter's bump)
@@ -763,6 +767,7 @@ void engine_destroy(Engine *e)
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     drop_tiles(e);
     for (hipEvent_t ev : e->kev) (void)hipEventDestroy(ev);
+    if (e->kev_end) (void)hipEventDestroy(e->kev_end);
     for (auto &G : e->g) free_grid(G);
     void *ptrs[] = {e->d_inst, e->d_gstep, e->d_partial, e->d_eout, e->d_erows, e->d_row0,
                     e->tf_rgb, e->tf_strips, e->tf_body, e->tf_lens, e->tf_off, e->d_pad, e->d_pctr};
@@ -1505,9 +1510,10 @@ int launch_depth(Engine *e, int k, int cm, const WvTile *T, int ni, int nb, long
 // at least n timing events exist
 int reserve_kev(Engine *e, size_t n)
 {
+    if (!e->kev_end) HIPCHK(e, hipEventCreate(&e->kev_end));
     while (e->kev.size() < n) {
         hipEvent_t ev;
-        HIPCHK(e, hipEventCreate(&ev));
+        HIPCHK(e, hipEventCreateWithFlags(&ev, hipEventDisableSystemFence));
         e->kev.push_back(ev);
         e->kev_depth.push_back(0);
         e->kev_epochs.push_back(1);
@@ -1658,6 +1664,8 @@ int engine_step(Engine *e, long nsteps, bool force_k1)
 // uses, the first load of every stepping kernel it launches (one launch over a
 // padding tile, which returns at once), the energy kernel, and for batches the
 // captured graph.  The state does not change.
+int kernel_times(Engine *e);
+
 int engine_prepare(Engine *e, long nsteps)
 {
     if (nsteps < 0) return fail(e, WV_EINVAL, "negative step count");
@@ -1689,12 +1697,16 @@ int engine_prepare(Engine *e, long nsteps)
     }
     int lr = wv_launch_bump(e->d_gstep, 0, e->stream);
     if (lr) return fail(e, WV_EHIP, "kernel launch failed: %s", hipGetErrorString((hipError_t)lr));
-    // the events a timed run of this sequence records (one per launch and two more)
+    // the events a timed run of this sequence records (one per launch and two
+    // more); a timed run's durations not read yet are read before they are
+    // recorded over
+    if ((rc = kernel_times(e))) return rc;
     long nl = 0;
     for (long d = 0; d < nsteps; nl++) d += next_depth(kmax, nsteps - d);
     if ((rc = reserve_kev(e, (size_t)nl + 2))) return rc;
     // record each once: the runtime sets an event up on its first record
     for (size_t q = 0; q < (size_t)nl + 2; q++) HIPCHK(e, hipEventRecord(e->kev[q], e->stream));
+    HIPCHK(e, hipEventRecord(e->kev_end, e->stream));
     if (e->use_graph && !e->persistent) {
         hipGraphExec_t ex;
         if ((rc = step_graph(e, nsteps, &ex))) return rc;
@@ -2014,11 +2026,12 @@ int engine_step_timed(Engine *e, long nsteps, double *ms, long *launches)
         if (launches) *launches = 0;
         return WV_OK;
     }
-    // wait by polling the last event: a blocking wait can sleep and wake up
-    // tens of microseconds after the work is done, which a caller timing a
-    // short sequence by wall clock would count
+    // wait by polling the closing (fenced) event: a blocking wait can sleep
+    // and wake up tens of microseconds after the work is done, which a caller
+    // timing a short sequence by wall clock would count
+    HIPCHK(e, hipEventRecord(e->kev_end, e->stream));
     hipError_t q;
-    while ((q = hipEventQuery(e->kev[e->kev_used - 1])) == hipErrorNotReady) {
+    while ((q = hipEventQuery(e->kev_end)) == hipErrorNotReady) {
     }
     if (q != hipSuccess) return fail(e, WV_EHIP, "waiting for the timed steps: %s", hipGetErrorString(q));
     float t = 0;
