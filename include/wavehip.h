@@ -217,6 +217,15 @@ typedef struct wv_domain_params {
 int wv_build_domain_mask(wv_ctx *c, int domain, const double box[4], const wv_domain_params *p, int ncircles,
                          const double *circles, short *xy_in);
 
+/* xy_in_billiard with COMPARISON (sub_wave.c:7593-7602), the mask of the
+ * flat program's two halves (WV_FLAG_COMPARISON): domain B_DOMAIN with the
+ * circle list circles where y > 0, domain_b B_DOMAIN_B with circles_b where
+ * y <= 0 (both from the codes and with the parameters of
+ * wv_build_domain_mask), at the cell corners of ij_to_xy over box. */
+int wv_build_comparison_mask(wv_ctx *c, int domain, int domain_b, const double box[4], const wv_domain_params *p,
+                             int ncircles, const double *circles, int ncircles_b, const double *circles_b,
+                             short *xy_in);
+
 /* The macros init_ior_2d (sub_wave.c:12164-13004) reads, by reference name. */
 typedef struct wv_ior_params {
     double courant, courantb;   /* COURANT, COURANTB                          */

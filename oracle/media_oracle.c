@@ -734,3 +734,22 @@ double orc_rotating_wells(int ngridx, int ngridy, int large, double ior_angle, c
         }
     return 0.2 * dx * dx;
 }
+
+/* xy_in_billiard with COMPARISON, sub_wave.c:7593-7602: domain with circ
+ * where y > 0, domain_b with circ_b where y <= 0 */
+int orc_comparison_mask(int NX, int NY, const double box[4], int domain, int domain_b, const orc_domain_params *dp,
+                        int nc, const double *circ, int nb, const double *circ_b, short *out)
+{
+    if (o_domain_at(0.0, 0.0, domain, dp, box, 0, circ) < 0 || o_domain_at(0.0, 0.0, domain_b, dp, box, 0, circ_b) < 0)
+        return -1;
+    int i, j;
+#pragma omp parallel for private(i, j)
+    for (i = 0; i < NX; i++)
+        for (j = 0; j < NY; j++) {
+            const double x = box[0] + ((double)i) * (box[1] - box[0]) / ((double)NX);
+            const double y = box[2] + ((double)j) * (box[3] - box[2]) / ((double)NY);
+            out[(size_t)i * NY + j] = (short)(y > 0.0 ? o_domain_at(x, y, domain, dp, box, nc, circ)
+                                                      : o_domain_at(x, y, domain_b, dp, box, nb, circ_b));
+        }
+    return 0;
+}

@@ -401,6 +401,22 @@ def domain(nx, ny, box, dom, params, circles=None):
     return out
 
 
+def comparison_mask(nx, ny, box, dom, dom_b, params, circles=None, circles_b=None):
+    """orc_comparison_mask: xy_in_billiard with COMPARISON (dom and circles
+    where y > 0, dom_b and circles_b where y <= 0)"""
+    L = lib()
+    L.orc_comparison_mask.argtypes = [ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(DomainParams), ctypes.c_int, _D, ctypes.c_int, _D, _S]
+    cs = [np.zeros((4, 1)) if c is None else np.ascontiguousarray(c, np.float64) for c in (circles, circles_b)]
+    ns = [0 if c is None else cs[q].shape[1] for q, c in enumerate((circles, circles_b))]
+    out = np.zeros((nx, ny), np.int16)
+    bx = np.ascontiguousarray(box, np.float64)
+    if L.orc_comparison_mask(nx, ny, _p(bx, _D), dom, dom_b, ctypes.byref(params), ns[0], _p(cs[0], _D), ns[1],
+                             _p(cs[1], _D), _p(out, _S)):
+        raise ValueError("domain %d or %d not provided" % (dom, dom_b))
+    return out
+
+
 class IorParams(ctypes.Structure):
     """orc_ior_params: the macros init_ior_2d reads"""
     _fields_ = [(n, ctypes.c_double) for n in ("courant", "courantb", "gamma", "gammab", "lam", "mu",
@@ -542,6 +558,9 @@ class RefBuild:
         if hasattr(self.L, "ref_mask"):
             self.L.ref_mask.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D, _I, _S]
             self.L.ref_mask.restype = ctypes.c_int
+            if hasattr(self.L, "ref_mask_billiard"):
+                self.L.ref_mask_billiard.argtypes = [ctypes.c_int, _D, _D, _D, _I, ctypes.c_int, _D, _D, _D, _I, _S]
+                self.L.ref_mask_billiard.restype = ctypes.c_int
             return
         if hasattr(self.L, "ref_init_ior"):
             self.L.ref_init_ior.argtypes = [_S, _D, _D, _D, ctypes.c_uint]
@@ -630,6 +649,21 @@ class RefBuild:
         out = np.zeros((self.nx, self.ny), np.int16)
         cc = [np.ascontiguousarray(c[k]) for k in range(3)]
         rc = self.L.ref_mask(domain, n, _p(cc[0], _D), _p(cc[1], _D), _p(cc[2], _D), _p(a, _I), _p(out, _S))
+        assert rc == 0
+        return out
+
+    def mask_billiard(self, circles=None, circles_b=None):
+        """xy_in_billiard over the build's grid (with COMPARISON: B_DOMAIN and
+        circles where y > 0, B_DOMAIN_B and circles_b where y <= 0)"""
+        args = []
+        for c in (circles, circles_b):
+            c = np.zeros((4, 1)) if c is None else np.ascontiguousarray(c, np.float64)
+            n = 0 if c is None else c.shape[1]
+            args += [n] + [np.ascontiguousarray(c[k]) for k in range(3)] + [np.ascontiguousarray(c[3], np.int32)]
+        out = np.zeros((self.nx, self.ny), np.int16)
+        rc = self.L.ref_mask_billiard(args[0], _p(args[1], _D), _p(args[2], _D), _p(args[3], _D), _p(args[4], _I),
+                                      args[5], _p(args[6], _D), _p(args[7], _D), _p(args[8], _D), _p(args[9], _I),
+                                      _p(out, _S))
         assert rc == 0
         return out
 

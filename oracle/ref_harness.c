@@ -131,6 +131,30 @@ int ref_mask(int domain, int ncirc, const double *xc, const double *yc, const do
         }
     return 0;
 }
+/* xy_in_billiard itself (sub_wave.c:7593-7602: with COMPARISON, B_DOMAIN
+ * and circles where y > 0, B_DOMAIN_B and circles_b where y <= 0) at the cell
+ * corners, with the two circle lists of the caller */
+int ref_mask_billiard(int nc, const double *xc, const double *yc, const double *r, const int *act, int nb,
+                      const double *xb, const double *yb, const double *rb, const int *actb, short *out)
+{
+    if (nc < 0 || nc > NMAXCIRCLES || nb < 0 || nb > NMAXCIRCLES) return -1;
+    for (int k = 0; k < nc; k++) {
+        circles[k].xc = xc[k]; circles[k].yc = yc[k]; circles[k].radius = r[k]; circles[k].active = (short)act[k];
+    }
+    for (int k = 0; k < nb; k++) {
+        circles_b[k].xc = xb[k]; circles_b[k].yc = yb[k]; circles_b[k].radius = rb[k];
+        circles_b[k].active = (short)actb[k];
+    }
+    ncircles = nc;
+    ncircles_b = nb;
+    for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++) {
+            double xy[2];
+            ij_to_xy(i, j, xy);
+            out[(size_t)i * NY + j] = (short)xy_in_billiard(xy[0], xy[1]);
+        }
+    return 0;
+}
 #elif defined(COLOR_TWIN)
 /* wave_value (wave_common.c:1053) at (i, j) on flat arrays; total_flux is
  * flat already (index 2*j*NX + 2*i) */

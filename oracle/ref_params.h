@@ -203,7 +203,9 @@
 #ifndef B_DOMAIN
 #define B_DOMAIN 20
 #endif
+#ifndef B_DOMAIN_B
 #define B_DOMAIN_B 20
+#endif
 #ifndef MU_B
 #define MU_B 0.08
 #endif

@@ -234,6 +234,11 @@ typedef struct {
 int orc_domain(int NX, int NY, const double box[4], int domain, const orc_domain_params *dp, int ncirc,
                const double *circ, short *out);
 
+/* xy_in_billiard with COMPARISON (sub_wave.c:7593-7602): domain and circ
+ * where y > 0, domain_b and circ_b where y <= 0; -1 for an unknown code */
+int orc_comparison_mask(int NX, int NY, const double box[4], int domain, int domain_b, const orc_domain_params *dp,
+                        int nc, const double *circ, int nb, const double *circ_b, short *out);
+
 /* The macros of init_ior_2d (sub_wave.c:12164); layout of wv_ior_params. */
 typedef struct {
     double courant, courantb, gamma, gammab, lambda, mu, mandel_ior_scale, mandellimit;

@@ -259,6 +259,21 @@ DOMAIN_CODES = (999, 0, 1, 196, 199, 198, 197, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 1
                 68, 69, 73, 74, 76, 79, 791, 795, 796, 90, 91)
 
 
+def mask_circles_b(n=25):
+    """(4, n): the second circle list of a COMPARISON mask (circles_b), drawn
+    in the lower half, every third circle inactive"""
+    xc = uniform(case_seed("mask_b_xc"), n, -1.9, 1.9)
+    yc = uniform(case_seed("mask_b_yc"), n, -1.1, 0.2)
+    r = uniform(case_seed("mask_b_r"), n, 0.05, 0.35)
+    act = np.array([0.0 if k % 3 == 2 else 1.0 for k in range(n)])
+    return np.ascontiguousarray(np.vstack([xc, yc, r, act]))
+
+
+# xy_in_billiard with COMPARISON (oracle/build_ref.sh dom_cmp_a, dom_cmp_b:
+# the parameter block of dom_a): (B_DOMAIN, B_DOMAIN_B)
+COMPARISON_BUILDS = {"dom_cmp_a": (20, 20), "dom_cmp_b": (73, 8)}
+
+
 def domain_sets():
     """build tag -> (nx, ny, parameters for pyoracle.domain_params)"""
     return {

@@ -59,9 +59,9 @@ sys.path.insert(0, os.path.join(ROOT, "youtube-simulations_amd"))
 sys.path.insert(0, HERE)
 
 import pyoracle  # noqa: E402
-from gen_inputs import (COLOR_FRAMES, DOMAIN_CODES, EXT_STEPS, IOR_CODES, PULSE_INIT, PULSES, ROT_ANGLES,  # noqa: E402
+from gen_inputs import (COLOR_FRAMES, COMPARISON_BUILDS, DOMAIN_CODES, EXT_STEPS, IOR_CODES, PULSE_INIT, PULSES, ROT_ANGLES,  # noqa: E402
                         TABLE_STEPS, case_inputs, cmp_cases, color_cases, color_inputs, domain_sets, ext_cases,
-                        ext_inputs, ior_mask, mask_cases, mask_circles, small_cases, table_cases, table_inputs)
+                        ext_inputs, ior_mask, mask_cases, mask_circles, mask_circles_b, small_cases, table_cases, table_inputs)
 from wavehip import workloads  # noqa: E402
 
 NSTEPS_SMALL = 7
@@ -104,6 +104,9 @@ def domains():
         assert (R.nx, R.ny) == (nx, ny)
         for dom in DOMAIN_CODES:
             out["%s/%d" % (tag, dom)] = R.mask(dom, circ if dom in (20, 201, 202) else None)
+    # xy_in_billiard with COMPARISON: the reference's own function on two builds
+    for tag in COMPARISON_BUILDS:
+        out[tag + "/mask"] = pyoracle.RefBuild(tag).mask_billiard(circ, mask_circles_b())
     np.savez_compressed(os.path.join(HERE, "domain_cases.npz"), **out)
 
 

@@ -197,3 +197,33 @@ def test_rotating_wells_match_reference_fixture(wh):
                 assert np.allclose(tc, g["%s/%d/tc" % (tag, q)], rtol=1e-14, atol=0), (tag, q)
         finally:
             W.close()
+
+
+def test_comparison_mask_matches_reference_and_oracle(wh, oracle):
+    """wv_build_comparison_mask against the reference's xy_in_billiard with
+    COMPARISON (dom_cmp_a, dom_cmp_b) and the oracle on a larger grid with
+    other domain pairs (iterated and polygon domains in either half)"""
+    from gen_inputs import COMPARISON_BUILDS, mask_circles_b
+    g = np.load(os.path.join(GOLD, "domain_cases.npz"))
+    kw = dict(oracle.DOMAIN_DEFAULTS, **domain_sets()["dom_a"][2])
+    W = wh.Wave(dict(nx=97, ny=61))
+    try:
+        for tag, (d, db) in COMPARISON_BUILDS.items():
+            say("comparison fixture", tag)
+            got = W.build_comparison_mask(d, db, MASK_BOX, circles=mask_circles(), circles_b=mask_circles_b(), **kw)
+            assert np.array_equal(got, g[tag + "/mask"]), tag
+    finally:
+        W.close()
+    nx, ny = 640, 361
+    P = oracle.domain_params(**kw)
+    W = wh.Wave(dict(nx=nx, ny=ny))
+    try:
+        for d, db in ((24, 8), (32, 25), (20, 74), (3, 199)):
+            say("comparison", d, db)
+            got = W.build_comparison_mask(d, db, BOX_REF, circles=mask_circles(), circles_b=mask_circles_b(), **kw)
+            ref = oracle.comparison_mask(nx, ny, BOX_REF, d, db, P, mask_circles(), mask_circles_b())
+            assert np.array_equal(got, ref), (d, db)
+        with pytest.raises(wh.WaveError):
+            W.build_comparison_mask(3, 53, BOX_REF, **kw)
+    finally:
+        W.close()

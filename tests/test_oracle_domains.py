@@ -92,3 +92,17 @@ def test_domain_oracle_matches_reference_build_other_circles(oracle):
         R = pyoracle.RefBuild(tag)
         for dom in CIRCLE_CODES:
             assert np.array_equal(R.mask(dom, c), oracle.domain(nx, ny, MASK_BOX, dom, oracle.domain_params(**kw), c))
+
+
+def test_comparison_mask_matches_reference(oracle, gold):
+    """xy_in_billiard with COMPARISON (sub_wave.c:7593-7602) as the reference
+    builds dom_cmp_a (D_CIRCLES above, D_CIRCLES with the second list below)
+    and dom_cmp_b (D_TREE above, D_POLYGON below) compute it; the halves
+    swapped fail"""
+    from gen_inputs import COMPARISON_BUILDS, mask_circles_b
+    P = oracle.domain_params(**domain_sets()["dom_a"][2])
+    for tag, (d, db) in COMPARISON_BUILDS.items():
+        got = oracle.comparison_mask(97, 61, MASK_BOX, d, db, P, mask_circles(), mask_circles_b())
+        assert np.array_equal(got, gold[tag + "/mask"]), tag
+        swapped = oracle.comparison_mask(97, 61, MASK_BOX, db, d, P, mask_circles_b(), mask_circles())
+        assert not np.array_equal(swapped, gold[tag + "/mask"]), tag

@@ -2768,13 +2768,13 @@ static bool domain_known(int d)
     return false;
 }
 
-static int build_domain_mask(Engine *e, int domain, const double *box, const wv_domain_params *p, int ncirc,
-                             const double *circ, short *xy)
+// one domain's launch parameters: checks, the host constants (the
+// reference's expressions in its order) and the polygon-angle table
+static int domain_setup(Engine *e, int domain, const double *box, const wv_domain_params *p, int ncirc,
+                        const double *circ, WvDomain &a, std::vector<double> &trig)
 {
     if (!domain_known(domain))
         return fail(e, WV_EUNSUPPORTED, "domain %d is not among the closed-form domains provided", domain);
-    if (!box || !p || !xy) return fail(e, WV_EINVAL, "box, parameters or output is NULL");
-    if (!(box[1] > box[0]) || !(box[3] > box[2])) return fail(e, WV_EINVAL, "empty box");
     const bool circles = domain == 20 || domain == 201 || domain == 202;
     if (circles && (ncirc < 0 || (ncirc > 0 && !circ))) return fail(e, WV_EINVAL, "circle list");
     const bool poly = domain == 8 || domain == 32 || domain == 791;
@@ -2798,9 +2798,8 @@ static int build_domain_mask(Engine *e, int domain, const double *box, const wv_
         (p->ngridx < 0 || p->ngridy < 1 || (double)p->ngridx * (p->ngridy + 1) > 1e6))
         return fail(e, WV_EINVAL, "NGRIDX %d, NGRIDY %d: NGRIDY >= 1, at most 1e6 disks", p->ngridx, p->ngridy);
     const double L = p->lambda, M = p->mu;
-    WvDomain a{};
-    // host constants, the reference's expressions in its order
-    std::vector<double> trig;
+    a = WvDomain{};
+    trig.clear();
     if (domain == 8 || domain == 32) {  // sub_wave.c:5812-5817, 6034-6041
         const double omega = REF_DPI / ((double)p->npoly);
         for (int k = 0; k < p->npoly; k++) {
@@ -2834,29 +2833,52 @@ static int build_domain_mask(Engine *e, int domain, const double *box, const wv_
         a.c1 = x1;
         a.c2 = r2;
     }
-    const int nc = circles ? ncirc : 0;
     const Grid &G = e->g[0];
-    const int nx = G.p.nx, ny = G.p.ny;
-    const size_t cells = (size_t)nx * ny;
+    a.nx = G.p.nx; a.ny = G.p.ny; a.domain = domain; a.ncirc = circles ? ncirc : 0; a.i0 = 0; a.rows = G.p.nx;
+    a.npoly = p->npoly; a.mdepth = p->mdepth; a.mratio = p->mratio; a.mandellevel = p->mandellevel;
+    a.ngridx = p->ngridx; a.ngridy = p->ngridy;
+    a.xmin = box[0]; a.xmax = box[1]; a.ymin = box[2]; a.ymax = box[3];
+    a.lambda = L; a.mu = M; a.mu_b = p->mu_b; a.wall_width = p->wall_width; a.wall_width_b = p->wall_width_b;
+    a.julia_scale = p->julia_scale; a.julia_x = p->julia_x; a.julia_y = p->julia_y;
+    a.mandellimit = p->mandellimit;
+    return WV_OK;
+}
+
+// xy_in_billiard over the context's grid: one domain everywhere (nd = 1), or
+// with COMPARISON (nd = 2) domain[0] where y > 0 and domain[1] where y <= 0
+// (sub_wave.c:7593-7602), each with its own circle list
+static int build_domain_masks(Engine *e, int nd, const int *domain, const double *box, const wv_domain_params *p,
+                              const int *ncirc, const double *const *circ, short *xy)
+{
+    if (!box || !p || !xy) return fail(e, WV_EINVAL, "box, parameters or output is NULL");
+    if (!(box[1] > box[0]) || !(box[3] > box[2])) return fail(e, WV_EINVAL, "empty box");
+    WvDomain a[2];
+    std::vector<double> trig[2];
+    size_t off[2] = {0, 0}, nd_total = 0;
+    int rc;
+    for (int q = 0; q < nd; q++) {
+        if ((rc = domain_setup(e, domain[q], box, p, ncirc[q], circ[q], a[q], trig[q]))) return rc;
+        a[q].half = nd == 1 ? 0 : q + 1;
+        off[q] = nd_total;
+        nd_total += 4 * (size_t)a[q].ncirc + trig[q].size();
+    }
+    const Grid &G = e->g[0];
+    const size_t cells = (size_t)G.p.nx * G.p.ny;
     HIPCHK(e, hipSetDevice(e->device));
     short *d_out = nullptr;
     double *d_c = nullptr;
-    const size_t nd = 4 * (size_t)nc + trig.size();
     hipError_t err = hipMalloc(&d_out, cells * sizeof(short));
-    if (err == hipSuccess) err = hipMalloc(&d_c, sizeof(double) * std::max<size_t>(nd, 1));
-    if (err == hipSuccess && nc > 0) err = hipMemcpy(d_c, circ, sizeof(double) * 4 * (size_t)nc, hipMemcpyHostToDevice);
-    if (err == hipSuccess && !trig.empty())
-        err = hipMemcpy(d_c + 4 * (size_t)nc, trig.data(), sizeof(double) * trig.size(), hipMemcpyHostToDevice);
-    if (err == hipSuccess) {
-        a.nx = nx; a.ny = ny; a.domain = domain; a.ncirc = nc; a.i0 = 0; a.rows = nx;
-        a.npoly = p->npoly; a.mdepth = p->mdepth; a.mratio = p->mratio; a.mandellevel = p->mandellevel;
-        a.ngridx = p->ngridx; a.ngridy = p->ngridy;
-        a.xmin = box[0]; a.xmax = box[1]; a.ymin = box[2]; a.ymax = box[3];
-        a.lambda = L; a.mu = M; a.mu_b = p->mu_b; a.wall_width = p->wall_width; a.wall_width_b = p->wall_width_b;
-        a.julia_scale = p->julia_scale; a.julia_x = p->julia_x; a.julia_y = p->julia_y;
-        a.mandellimit = p->mandellimit;
-        a.circ = d_c; a.trig = d_c + 4 * (size_t)nc; a.out = d_out;
-        err = (hipError_t)wv_launch_domain(a, e->stream);
+    if (err == hipSuccess) err = hipMalloc(&d_c, sizeof(double) * std::max<size_t>(nd_total, 1));
+    for (int q = 0; q < nd && err == hipSuccess; q++) {
+        double *base = d_c + off[q];
+        const size_t nc4 = 4 * (size_t)a[q].ncirc;
+        if (nc4) err = hipMemcpy(base, circ[q], sizeof(double) * nc4, hipMemcpyHostToDevice);
+        if (err == hipSuccess && !trig[q].empty())
+            err = hipMemcpy(base + nc4, trig[q].data(), sizeof(double) * trig[q].size(), hipMemcpyHostToDevice);
+        a[q].circ = base;
+        a[q].trig = base + nc4;
+        a[q].out = d_out;
+        if (err == hipSuccess) err = (hipError_t)wv_launch_domain(a[q], e->stream);
     }
     if (err == hipSuccess) err = hipMemcpyAsync(xy, d_out, cells * sizeof(short), hipMemcpyDeviceToHost, e->stream);
     if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
@@ -2870,7 +2892,18 @@ extern "C" int wv_build_domain_mask(wv_ctx *c, int domain, const double box[4], 
                                     int ncircles, const double *circles, short *xy_in)
 {
     if (!c) return WV_EINVAL;
-    return build_domain_mask(c->e, domain, box, p, ncircles, circles, xy_in);
+    const double *cl[1] = {circles};
+    return build_domain_masks(c->e, 1, &domain, box, p, &ncircles, cl, xy_in);
+}
+
+extern "C" int wv_build_comparison_mask(wv_ctx *c, int domain, int domain_b, const double box[4],
+                                        const wv_domain_params *p, int ncircles, const double *circles,
+                                        int ncircles_b, const double *circles_b, short *xy_in)
+{
+    if (!c) return WV_EINVAL;
+    const int d[2] = {domain, domain_b}, n[2] = {ncircles, ncircles_b};
+    const double *cl[2] = {circles, circles_b};
+    return build_domain_masks(c->e, 2, d, box, p, n, cl, xy_in);
 }
 
 // ---- init_ior_2d's per-cell media models -------------------------------------

@@ -231,9 +231,11 @@ struct WvMask {
 // constants the host takes with the C library (cosines and sines of fixed
 // angles, square roots of parameter expressions).  trig: ntrig (cos, sin)
 // pairs of the domain's fixed angles (polygon sides), after the circle list
-// in the same device buffer.  Rows [i0, i0 + rows) into out[(i - i0)*NY + j].
+// in the same device buffer.  Rows [i0, i0 + rows) into out[(i - i0)*NY + j];
+// with half set, only that half of the cells is written.
 struct WvDomain {
     int nx, ny, domain, ncirc, i0, rows;
+    int half;   // 0: every cell; 1: cells with y > 0 only; 2: y <= 0 only
     int npoly, mdepth, mratio, mandellevel, ngridx, ngridy;
     double xmin, xmax, ymin, ymax;
     double lambda, mu, mu_b, wall_width, wall_width_b, julia_scale, julia_x, julia_y, mandellimit;

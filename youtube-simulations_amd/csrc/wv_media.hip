@@ -507,6 +507,8 @@ __global__ void __launch_bounds__(256) k_domain(WvDomain a)
     // ij_to_xy, sub_wave.c:469-476
     const double x = a.xmin + ((double)i) * (a.xmax - a.xmin) / ((double)a.nx);
     const double y = a.ymin + ((double)j) * (a.ymax - a.ymin) / ((double)a.ny);
+    // COMPARISON: one domain for y > 0, the other for y <= 0 (sub_wave.c:7597-7600)
+    if ((a.half == 1 && !(y > 0.0)) || (a.half == 2 && y > 0.0)) return;
     gm(a.out)[(size_t)(i - a.i0) * a.ny + j] = domain_value(a, x, y);
 }
 

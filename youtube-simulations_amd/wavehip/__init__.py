@@ -33,7 +33,7 @@ EXPORTS = [
     "wv_set_state", "wv_set_state3", "wv_get_state", "wv_get_state3", "wv_set_sources",
     "wv_set_neumann", "wv_set_drive", "wv_set_floor", "wv_set_laplacian", "wv_set_probes", "wv_get_probes",
     "wv_format_time_series", "wv_set_wells_tables", "wv_build_mask", "wv_build_mask_static", "wv_build_domain_mask",
-    "wv_set_ior_tables", "wv_rotating_wells", "wv_init_circular_wave",
+    "wv_build_comparison_mask", "wv_set_ior_tables", "wv_rotating_wells", "wv_init_circular_wave",
     "wv_add_circular_wave", "wv_batch_init_circular_wave", "wv_batch_add_circular_wave", "wv_tiff_bound", "wv_tiff_encode", "wv_tiff_write",
     "wv_step", "wv_prepare", "wv_step_timed", "wv_sync", "wv_steps_done", "wv_last_kernel_ms", "wv_energy",
     "wv_get_tiling", "wv_get_tile_counts", "wv_frame_params_default", "wv_frame_set_tables", "wv_frame_reset", "wv_frame",
@@ -188,6 +188,8 @@ def load():
                                                 ctypes.c_int, _D, _S]),
         "wv_build_domain_mask": (ctypes.c_int, [vp, ctypes.c_int, _D, ctypes.POINTER(WvDomainParams), ctypes.c_int,
                                                 _D, _S]),
+        "wv_build_comparison_mask": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, _D, ctypes.POINTER(WvDomainParams),
+                                                    ctypes.c_int, _D, ctypes.c_int, _D, _S]),
         "wv_set_ior_tables": (ctypes.c_int, [vp, _S, ctypes.c_int, _D, ctypes.POINTER(WvIorParams), _D, _D, _D]),
         "wv_rotating_wells": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _D, _D, _D,
                                              _D]),
@@ -489,6 +491,25 @@ class Wave:
         out = np.empty(self.shape, np.int16)
         self._chk(self._L.wv_build_domain_mask(self._h, int(domain), _d(bx), ctypes.byref(P), n, _d(c),
                                                out.ctypes.data_as(_S)), "wv_build_domain_mask")
+        return out
+
+    def build_comparison_mask(self, domain, domain_b, box, params=None, circles=None, circles_b=None, **kw):
+        """xy_in_billiard with COMPARISON on the GPU (wv_build_comparison_mask):
+        domain and circles where y > 0, domain_b and circles_b where y <= 0"""
+        P = params if params is not None else domain_params(**kw)
+        cs = []
+        for c in (circles, circles_b):
+            a = np.zeros((4, 1)) if c is None else np.ascontiguousarray(c, np.float64)
+            if a.ndim != 2 or a.shape[0] != 4:
+                raise ValueError("circles must be (4, n): xc, yc, radius, active")
+            cs.append((0 if c is None else a.shape[1], a))
+        bx = np.ascontiguousarray(box, np.float64).ravel()
+        if bx.size != 4:
+            raise ValueError("box must hold XMIN, XMAX, YMIN, YMAX")
+        out = np.empty(self.shape, np.int16)
+        self._chk(self._L.wv_build_comparison_mask(self._h, int(domain), int(domain_b), _d(bx), ctypes.byref(P),
+                                                   cs[0][0], _d(cs[0][1]), cs[1][0], _d(cs[1][1]),
+                                                   out.ctypes.data_as(_S)), "wv_build_comparison_mask")
         return out
 
     def set_ior_tables(self, xy_in, ior, box, params=None, copy=True, **kw):
