@@ -522,6 +522,17 @@ int wv_batch_init_circular_wave(wv_batch *b, int k, double x0, double y0, double
                                 double wavelength, const double box[4]);
 int wv_batch_add_circular_wave(wv_batch *b, int k, double factor, double x0, double y0, double amp,
                                double variance, double wavelength, const double box[4]);
+/* wv_set_wells_tables, wv_set_ior_tables and wv_build_domain_mask on grid k of
+ * a batch (the media installed for that grid only; the mask is returned for
+ * that grid's NX x NY) */
+int wv_batch_set_wells_tables(wv_batch *b, int k, const short *xy_in, int nwells, const double *xc,
+                              const double *yc, const double *height, double sigma, int use_tanh, double courant,
+                              double courantb, double gamma, const double box[4], double *tcc, double *tc,
+                              double *tgamma);
+int wv_batch_set_ior_tables(wv_batch *b, int k, const short *xy_in, int ior, const double box[4],
+                            const wv_ior_params *p, double *tcc, double *tc, double *tgamma);
+int wv_batch_build_domain_mask(wv_batch *b, int k, int domain, const double box[4], const wv_domain_params *p,
+                               int ncircles, const double *circles, short *xy_in);
 /* wv_autotune over all grids of the batch (one tiling for every launch) */
 int wv_batch_autotune(wv_batch *b, long steps_per_trial, double *best_gcells);
 

@@ -227,3 +227,53 @@ def test_comparison_mask_matches_reference_and_oracle(wh, oracle):
             W.build_comparison_mask(3, 53, BOX_REF, **kw)
     finally:
         W.close()
+
+
+def test_batch_builders_per_grid(wh, oracle):
+    """the builders on grids of a batch (three sizes): each grid's mask and
+    media as the oracle builds them, and 20 split-form steps of the batch
+    equal the oracle's split run of each grid on the same tables"""
+    box = MASK_BOX
+    sizes = [(130, 77), (64, 200), (97, 61)]
+    params = [dict(nx=nx, ny=ny, bcond=2, twospeeds=1) for nx, ny in sizes]
+    kw = dict(wh.DOMAIN_DEFAULTS, lam=0.8, mu=0.3, wall_width=0.1)
+    P = oracle.domain_params(**kw)
+    B = wh.Batch(params, form=1, tb_depth=4)
+    tables = []
+    try:
+        for k, ((nx, ny), dom) in enumerate(zip(sizes, (999, 74, 3))):
+            say("batch builders", k, dom)
+            xy = B.build_domain_mask(k, dom, box, **kw)
+            assert np.array_equal(xy, oracle.domain(nx, ny, box, dom, P))
+            p = oracle.make_params(nx, ny, bcond=2, twospeeds=1)
+            if k == 1:
+                got = B.set_ior_tables(k, xy, 17, box, **IOR_PARAMS)
+                ref = oracle.ior_tables(p, xy, 17, box, oracle.IorParams(**IOR_PARAMS))
+                assert all(bits_equal(a, b) for a, b in zip(got, ref))
+            elif k == 0:
+                got = B.set_ior_tables(k, xy, 182, box, **IOR_PARAMS)
+                ref = oracle.ior_tables(p, xy, 182, box, oracle.IorParams(**IOR_PARAMS))
+                assert all(bits_equal(a, b) for a, b in zip(got, ref))
+            else:
+                rng = np.random.default_rng(4)
+                xc, yc = rng.uniform(-2, 2, 30), rng.uniform(-1.1, 1.1, 30)
+                got = B.set_wells_tables(k, xy, xc, yc, None, 0.05, 1, 0.1, 0.05, 1e-4, box)
+                ref = oracle.wells_tables(p, xy, xc, yc, None, 0.05, 1, 0.1, 0.05, 1e-4, box)
+                assert all(np.allclose(a, b, rtol=1e-14, atol=0) for a, b in zip(got, ref))
+            x = box[0] + np.arange(nx) * (box[1] - box[0]) / nx
+            y = box[2] + np.arange(ny) * (box[3] - box[2]) / ny
+            X, Y = np.meshgrid(x, y, indexing="ij")
+            phi = np.exp(-((X + 0.7) ** 2 + (Y - 0.2) ** 2) / 0.05)
+            B.set_state(k, phi, phi)
+            tables.append((p, xy, got, phi))
+        with pytest.raises(wh.WaveError):
+            B.set_ior_tables(2, tables[2][1], 5, box)
+        B.step(20)
+        for k, (p, xy, t, phi) in enumerate(tables):
+            u = B.get_state(k)
+            hi, lo, v = oracle.split_from_fp64(phi, phi)
+            hi, lo, v = oracle.split_run(p, hi, lo, v, 20, xy, t[0], t[1], t[2])
+            ur, umr = oracle.split_to_fp64(hi, lo, v)
+            assert bits_equal(u[0], ur) and bits_equal(u[1], umr), k
+    finally:
+        B.close()

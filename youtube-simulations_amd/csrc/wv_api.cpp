@@ -2617,6 +2617,16 @@ extern "C" int wv_set_wells_tables(wv_ctx *c, const short *xy_in, int nwells, co
                             tcc, tc, tgamma);
 }
 
+extern "C" int wv_batch_set_wells_tables(wv_batch *b, int k, const short *xy_in, int nwells, const double *xc,
+                                         const double *yc, const double *height, double sigma, int use_tanh,
+                                         double courant, double courantb, double gamma, const double box[4],
+                                         double *tcc, double *tc, double *tgamma)
+{
+    if (!b) return WV_EINVAL;
+    return set_wells_tables(b->e, k, xy_in, nwells, xc, yc, height, sigma, use_tanh, courant, courantb, gamma, box,
+                            tcc, tc, tgamma);
+}
+
 // The billiard mask of init_wave_flat on the GPU (see wavehip.h)
 static int build_mask(Engine *e, int domain, const double *box, double lambda, double mu, int ncirc, const double *circ,
                       short *xy)
@@ -2770,8 +2780,8 @@ static bool domain_known(int d)
 
 // one domain's launch parameters: checks, the host constants (the
 // reference's expressions in its order) and the polygon-angle table
-static int domain_setup(Engine *e, int domain, const double *box, const wv_domain_params *p, int ncirc,
-                        const double *circ, WvDomain &a, std::vector<double> &trig)
+static int domain_setup(Engine *e, const Grid &G, int domain, const double *box, const wv_domain_params *p,
+                        int ncirc, const double *circ, WvDomain &a, std::vector<double> &trig)
 {
     if (!domain_known(domain))
         return fail(e, WV_EUNSUPPORTED, "domain %d is not among the closed-form domains provided", domain);
@@ -2833,7 +2843,6 @@ static int domain_setup(Engine *e, int domain, const double *box, const wv_domai
         a.c1 = x1;
         a.c2 = r2;
     }
-    const Grid &G = e->g[0];
     a.nx = G.p.nx; a.ny = G.p.ny; a.domain = domain; a.ncirc = circles ? ncirc : 0; a.i0 = 0; a.rows = G.p.nx;
     a.npoly = p->npoly; a.mdepth = p->mdepth; a.mratio = p->mratio; a.mandellevel = p->mandellevel;
     a.ngridx = p->ngridx; a.ngridy = p->ngridy;
@@ -2844,12 +2853,13 @@ static int domain_setup(Engine *e, int domain, const double *box, const wv_domai
     return WV_OK;
 }
 
-// xy_in_billiard over the context's grid: one domain everywhere (nd = 1), or
-// with COMPARISON (nd = 2) domain[0] where y > 0 and domain[1] where y <= 0
+// xy_in_billiard over grid k: one domain everywhere (nd = 1), or with
+// COMPARISON (nd = 2) domain[0] where y > 0 and domain[1] where y <= 0
 // (sub_wave.c:7593-7602), each with its own circle list
-static int build_domain_masks(Engine *e, int nd, const int *domain, const double *box, const wv_domain_params *p,
-                              const int *ncirc, const double *const *circ, short *xy)
+static int build_domain_masks(Engine *e, int k, int nd, const int *domain, const double *box,
+                              const wv_domain_params *p, const int *ncirc, const double *const *circ, short *xy)
 {
+    if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
     if (!box || !p || !xy) return fail(e, WV_EINVAL, "box, parameters or output is NULL");
     if (!(box[1] > box[0]) || !(box[3] > box[2])) return fail(e, WV_EINVAL, "empty box");
     WvDomain a[2];
@@ -2857,12 +2867,12 @@ static int build_domain_masks(Engine *e, int nd, const int *domain, const double
     size_t off[2] = {0, 0}, nd_total = 0;
     int rc;
     for (int q = 0; q < nd; q++) {
-        if ((rc = domain_setup(e, domain[q], box, p, ncirc[q], circ[q], a[q], trig[q]))) return rc;
+        if ((rc = domain_setup(e, e->g[k], domain[q], box, p, ncirc[q], circ[q], a[q], trig[q]))) return rc;
         a[q].half = nd == 1 ? 0 : q + 1;
         off[q] = nd_total;
         nd_total += 4 * (size_t)a[q].ncirc + trig[q].size();
     }
-    const Grid &G = e->g[0];
+    const Grid &G = e->g[k];
     const size_t cells = (size_t)G.p.nx * G.p.ny;
     HIPCHK(e, hipSetDevice(e->device));
     short *d_out = nullptr;
@@ -2893,7 +2903,16 @@ extern "C" int wv_build_domain_mask(wv_ctx *c, int domain, const double box[4], 
 {
     if (!c) return WV_EINVAL;
     const double *cl[1] = {circles};
-    return build_domain_masks(c->e, 1, &domain, box, p, &ncircles, cl, xy_in);
+    return build_domain_masks(c->e, 0, 1, &domain, box, p, &ncircles, cl, xy_in);
+}
+
+extern "C" int wv_batch_build_domain_mask(wv_batch *b, int k, int domain, const double box[4],
+                                          const wv_domain_params *p, int ncircles, const double *circles,
+                                          short *xy_in)
+{
+    if (!b) return WV_EINVAL;
+    const double *cl[1] = {circles};
+    return build_domain_masks(b->e, k, 1, &domain, box, p, &ncircles, cl, xy_in);
 }
 
 extern "C" int wv_build_comparison_mask(wv_ctx *c, int domain, int domain_b, const double box[4],
@@ -2903,7 +2922,7 @@ extern "C" int wv_build_comparison_mask(wv_ctx *c, int domain, int domain_b, con
     if (!c) return WV_EINVAL;
     const int d[2] = {domain, domain_b}, n[2] = {ncircles, ncircles_b};
     const double *cl[2] = {circles, circles_b};
-    return build_domain_masks(c->e, 2, d, box, p, n, cl, xy_in);
+    return build_domain_masks(c->e, 0, 2, d, box, p, n, cl, xy_in);
 }
 
 // ---- init_ior_2d's per-cell media models -------------------------------------
@@ -2984,6 +3003,13 @@ extern "C" int wv_set_ior_tables(wv_ctx *c, const short *xy_in, int ior, const d
 {
     if (!c) return WV_EINVAL;
     return set_ior_tables(c->e, 0, xy_in, ior, box, p, tcc, tc, tgamma);
+}
+
+extern "C" int wv_batch_set_ior_tables(wv_batch *b, int k, const short *xy_in, int ior, const double box[4],
+                                       const wv_ior_params *p, double *tcc, double *tc, double *tgamma)
+{
+    if (!b) return WV_EINVAL;
+    return set_ior_tables(b->e, k, xy_in, ior, box, p, tcc, tc, tgamma);
 }
 
 // The well centres of the rotating-well models, sub_wave.c:12390-12481:

@@ -34,7 +34,8 @@ EXPORTS = [
     "wv_set_neumann", "wv_set_drive", "wv_set_floor", "wv_set_laplacian", "wv_set_probes", "wv_get_probes",
     "wv_format_time_series", "wv_set_wells_tables", "wv_build_mask", "wv_build_mask_static", "wv_build_domain_mask",
     "wv_build_comparison_mask", "wv_set_ior_tables", "wv_rotating_wells", "wv_init_circular_wave",
-    "wv_add_circular_wave", "wv_batch_init_circular_wave", "wv_batch_add_circular_wave", "wv_tiff_bound", "wv_tiff_encode", "wv_tiff_write",
+    "wv_add_circular_wave", "wv_batch_init_circular_wave", "wv_batch_add_circular_wave",
+    "wv_batch_set_wells_tables", "wv_batch_set_ior_tables", "wv_batch_build_domain_mask", "wv_tiff_bound", "wv_tiff_encode", "wv_tiff_write",
     "wv_step", "wv_prepare", "wv_step_timed", "wv_sync", "wv_steps_done", "wv_last_kernel_ms", "wv_energy",
     "wv_get_tiling", "wv_get_tile_counts", "wv_frame_params_default", "wv_frame_set_tables", "wv_frame_reset", "wv_frame",
     "wv_frame_rgba8",
@@ -196,6 +197,13 @@ def load():
         "wv_init_circular_wave": (ctypes.c_int, [vp] + [ctypes.c_double] * 5 + [_D]),
         "wv_add_circular_wave": (ctypes.c_int, [vp] + [ctypes.c_double] * 6 + [_D]),
         "wv_batch_init_circular_wave": (ctypes.c_int, [vp, ctypes.c_int] + [ctypes.c_double] * 5 + [_D]),
+        "wv_batch_set_wells_tables": (ctypes.c_int, [vp, ctypes.c_int, _S, ctypes.c_int, _D, _D, _D, ctypes.c_double,
+                                                     ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                     _D, _D, _D, _D]),
+        "wv_batch_set_ior_tables": (ctypes.c_int, [vp, ctypes.c_int, _S, ctypes.c_int, _D,
+                                                   ctypes.POINTER(WvIorParams), _D, _D, _D]),
+        "wv_batch_build_domain_mask": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, _D,
+                                                      ctypes.POINTER(WvDomainParams), ctypes.c_int, _D, _S]),
         "wv_batch_add_circular_wave": (ctypes.c_int, [vp, ctypes.c_int] + [ctypes.c_double] * 6 + [_D]),
         "wv_tiff_bound": (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
         "wv_tiff_encode": (ctypes.c_int, [vp, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
@@ -777,6 +785,50 @@ class Batch:
         bx = np.ascontiguousarray(box, np.float64).ravel()
         self._chk(self._L.wv_batch_add_circular_wave(self._h, k, factor, x0, y0, amp, variance, wavelength,
                                                      _d(bx)), "wv_batch_add_circular_wave")
+
+    def _box(self, box):
+        bx = np.ascontiguousarray(box, np.float64).ravel()
+        if bx.size != 4:
+            raise ValueError("box must hold XMIN, XMAX, YMIN, YMAX")
+        return bx
+
+    def set_wells_tables(self, k, xy_in, xc, yc, height, sigma, use_tanh, courant, courantb, gamma, box, copy=True):
+        """Gaussian-well media of grid k built on the GPU (wv_batch_set_wells_tables)"""
+        xy = _i16(xy_in, self.shapes[k])
+        xc = np.ascontiguousarray(xc, np.float64).ravel()
+        yc = np.ascontiguousarray(yc, np.float64).ravel()
+        hh = None if height is None else np.ascontiguousarray(height, np.float64).ravel()
+        if yc.size != xc.size or (hh is not None and hh.size != xc.size):
+            raise ValueError("wells: xc, yc (and height) of one length")
+        t = [np.empty(self.shapes[k]) for _ in range(3)] if copy else None
+        outs = [_d(a) for a in t] if copy else [None, None, None]
+        self._chk(self._L.wv_batch_set_wells_tables(self._h, k, xy.ctypes.data_as(_S), len(xc), _d(xc), _d(yc),
+                                                    None if hh is None else _d(hh), sigma, int(use_tanh), courant,
+                                                    courantb, gamma, _d(self._box(box)), *outs),
+                  "wv_batch_set_wells_tables")
+        return tuple(t) if copy else None
+
+    def set_ior_tables(self, k, xy_in, ior, box, params=None, copy=True, **kw):
+        """init_ior_2d's per-cell model ior for grid k (wv_batch_set_ior_tables)"""
+        P = params if params is not None else ior_params(**kw)
+        xy = _i16(xy_in, self.shapes[k])
+        t = [np.empty(self.shapes[k]) for _ in range(3)] if copy else None
+        outs = [_d(a) for a in t] if copy else [None, None, None]
+        self._chk(self._L.wv_batch_set_ior_tables(self._h, k, xy.ctypes.data_as(_S), int(ior), _d(self._box(box)),
+                                                  ctypes.byref(P), *outs), "wv_batch_set_ior_tables")
+        return tuple(t) if copy else None
+
+    def build_domain_mask(self, k, domain, box, params=None, circles=None, **kw):
+        """xy_in_billiard_single_domain over grid k (wv_batch_build_domain_mask)"""
+        P = params if params is not None else domain_params(**kw)
+        c = np.zeros((4, 1)) if circles is None else np.ascontiguousarray(circles, np.float64)
+        if c.ndim != 2 or c.shape[0] != 4:
+            raise ValueError("circles must be (4, n): xc, yc, radius, active")
+        out = np.empty(self.shapes[k], np.int16)
+        self._chk(self._L.wv_batch_build_domain_mask(self._h, k, int(domain), _d(self._box(box)), ctypes.byref(P),
+                                                     0 if circles is None else c.shape[1], _d(c),
+                                                     out.ctypes.data_as(_S)), "wv_batch_build_domain_mask")
+        return out
 
     def step(self, n):
         self._chk(self._L.wv_batch_step(self._h, int(n)), "wv_batch_step")
