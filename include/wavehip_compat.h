@@ -56,6 +56,7 @@
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include "wavehip.h"
 
@@ -375,5 +376,170 @@ static void evolve_wave(double phi[NX * NY], double psi[NX * NY], double tmp[NX 
     if (rc) wavehip_die(wavehip_ctx(), "evolve_wave", rc);
 }
 #endif
+
+
+#ifdef WAVEHIP_INIT
+/* ---- start-up tables on the GPU (define WAVEHIP_INIT) ------------------------
+ * Call wavehip_init_wave_flat / wavehip_init_ior_2d where the program calls
+ * init_wave_flat (wave_common.c:294-319, wave_billiard.c:797) and init_ior_2d
+ * (sub_wave.c:12164, wave_billiard.c:803 and the REFRESH_IOR call at 943), or
+ * wavehip_init_wave_flat_mod for init_wave_flat_mod (wave_common.c:1894,
+ * wave_3d.c:1212, with WAVEHIP_FLAT); same arguments.  They read the
+ * program's #define block (B_DOMAIN, LAMBDA, MU, MU_B, APOLY, NPOLY,
+ * WALL_WIDTH, WALL_WIDTH_B, JULIA_SCALE, MDEPTH, MRATIO, MANDELLEVEL,
+ * MANDELLIMIT, NGRIDX, NGRIDY, XMIN..YMAX, XYIN_INITIALISED; VARIABLE_IOR,
+ * IOR, COURANT, COURANTB, GAMMA, GAMMAB, MANDEL_IOR_SCALE; COMPARISON and
+ * B_DOMAIN_B) and its globals (circles, ncircles, circles_b, ncircles_b,
+ * julia_x, julia_y).  The random wells (IOR 5, 8, 9) are drawn with the
+ * program's own gaussian() and init_circle_config_pattern(), in the
+ * reference's order, so the draws are the program's.  Masks equal the
+ * reference's; the tables too, up to the GPU's exp, tanh and log where the
+ * model uses them. */
+static const double wavehip_box[4] = {XMIN, XMAX, YMIN, YMAX};
+
+static void wavehip_mask(short *flat)
+{
+    wv_domain_params dp;
+    dp.lambda = LAMBDA; dp.mu = MU; dp.mu_b = MU_B; dp.apoly = APOLY; dp.wall_width = WALL_WIDTH;
+    dp.wall_width_b = WALL_WIDTH_B; dp.julia_scale = JULIA_SCALE; dp.julia_x = julia_x; dp.julia_y = julia_y;
+    dp.mandellimit = MANDELLIMIT; dp.npoly = NPOLY; dp.mdepth = MDEPTH; dp.mratio = MRATIO;
+    dp.mandellevel = MANDELLEVEL; dp.ngridx = NGRIDX; dp.ngridy = NGRIDY;
+    const int nl = COMPARISON ? 2 : 1;
+    const t_circle *cs[2] = {circles, COMPARISON ? circles_b : circles};
+    const int nc[2] = {ncircles, COMPARISON ? ncircles_b : 0};
+    double *cl[2] = {NULL, NULL};
+    for (int q = 0; q < nl; q++) {
+        const int n = nc[q] > 0 ? nc[q] : 0;
+        cl[q] = (double *)malloc(sizeof(double) * 4 * (size_t)(n > 0 ? n : 1));
+        if (!cl[q]) wavehip_die(wavehip_ctx(), "mask (no memory)", WV_ENOMEM);
+        for (int k = 0; k < n; k++) {
+            cl[q][k] = cs[q][k].xc;
+            cl[q][n + k] = cs[q][k].yc;
+            cl[q][2 * n + k] = cs[q][k].radius;
+            cl[q][3 * n + k] = cs[q][k].active ? 1.0 : 0.0;
+        }
+    }
+    int rc;
+#if COMPARISON
+    rc = wv_build_comparison_mask(wavehip_ctx(), B_DOMAIN, B_DOMAIN_B, wavehip_box, &dp, nc[0], cl[0], nc[1], cl[1],
+                                  flat);
+#else
+    rc = wv_build_domain_mask(wavehip_ctx(), B_DOMAIN, wavehip_box, &dp, nc[0], cl[0], flat);
+#endif
+    free(cl[0]);
+    free(cl[1]);
+    if (rc) wavehip_die(wavehip_ctx(), "xy_in_billiard", rc);
+}
+
+/* init_ior_2d's tables of the flat mask xy into flat tcc, tc, tgamma */
+static void wavehip_media(const short *xy, double ior_angle, double *tcc, double *tc, double *tg)
+{
+    int rc;
+#if VARIABLE_IOR && IOR >= 4 && IOR <= 9
+    /* the well models, sub_wave.c:12315-12555: the centres (and heights) as
+     * the reference places them, the sum on the GPU */
+    int nw = NGRIDX * NGRIDY, use_tanh = 0;
+    double sigma, *xc, *yc, *h = NULL;
+#if IOR == 8 || IOR == 9
+    static t_circle wc[NMAXCIRCLES];
+    nw = init_circle_config_pattern(wc, IOR == 8 ? C_POISSON_DISC : C_RAND_POISSON);
+#endif
+    xc = (double *)malloc(sizeof(double) * (size_t)nw);
+    yc = (double *)malloc(sizeof(double) * (size_t)nw);
+    if (!xc || !yc) wavehip_die(wavehip_ctx(), "init_ior_2d (no memory)", WV_ENOMEM);
+#if IOR == 6 || IOR == 7
+    rc = wv_rotating_wells(NGRIDX, NGRIDY, IOR == 7, ior_angle, wavehip_box, xc, yc, &sigma);
+    if (rc) wavehip_die(wavehip_ctx(), "init_ior_2d", rc);
+#elif IOR == 8 || IOR == 9
+    h = (double *)malloc(sizeof(double) * (size_t)nw);
+    for (int n = 0; n < nw; n++) {
+        h[n] = 0.5 + 0.5 * gaussian();
+        if (h[n] > 1.0) h[n] = 1.0;
+        if (h[n] < 0.0) h[n] = 0.0;
+        xc[n] = wc[n].xc;
+        yc[n] = wc[n].yc;
+    }
+    sigma = 0.2 * (XMAX - XMIN) * (YMAX - YMIN) / (double)nw;
+    use_tanh = 1;
+#else
+    {
+        const double dx = (XMAX - XMIN) / (double)NGRIDX, dy = (YMAX - YMIN) / (double)NGRIDY;
+        sigma = 0.2 * dx * dx;
+#if IOR == 5
+        h = (double *)malloc(sizeof(double) * (size_t)nw);
+        use_tanh = 1;
+#endif
+        for (int i = 0; i < NGRIDX; i++)
+            for (int j = 0; j < NGRIDY; j++) {
+                const int n = j * NGRIDX + i;
+#if IOR == 5
+                xc[n] = XMIN + dx * ((double)i + 0.5 + 0.1 * gaussian());
+                yc[n] = YMIN + dy * ((double)j + 0.5 + 0.1 * gaussian());
+                h[n] = 0.5 + 0.5 * gaussian();
+                if (h[n] > 1.0) h[n] = 1.0;
+                if (h[n] < 0.0) h[n] = 0.0;
+#else
+                xc[n] = XMIN + dx * ((double)i + 0.5);
+                yc[n] = YMIN + dy * ((double)j + 0.5);
+                if (j % 2 == 1) yc[n] += 0.5 * dx;
+#endif
+            }
+    }
+#endif
+    rc = wv_set_wells_tables(wavehip_ctx(), xy, nw, xc, yc, h, sigma, use_tanh, COURANT, COURANTB, GAMMA,
+                             wavehip_box, tcc, tc, tg);
+    free(xc);
+    free(yc);
+    free(h);
+#else
+    wv_ior_params ip;
+    ip.courant = COURANT; ip.courantb = COURANTB; ip.gamma = GAMMA; ip.gammab = GAMMAB; ip.lambda = LAMBDA;
+    ip.mu = MU; ip.mandel_ior_scale = MANDEL_IOR_SCALE; ip.mandellimit = MANDELLIMIT; ip.mandellevel = MANDELLEVEL;
+    ip.mdepth = MDEPTH;
+    (void)ior_angle;
+    rc = wv_set_ior_tables(wavehip_ctx(), xy, VARIABLE_IOR ? IOR : WV_IOR_CONSTANT, wavehip_box, &ip, tcc, tc, tg);
+#endif
+    if (rc) wavehip_die(wavehip_ctx(), "init_ior_2d", rc);
+}
+
+#ifndef WAVEHIP_FLAT
+static void wavehip_init_wave_flat(double *phi[NX], double *psi[NX], short int *xy_in[NX])
+{
+    if (!XYIN_INITIALISED) {
+        short *m = (short *)malloc(sizeof(short) * (size_t)NX * NY);
+        if (!m) wavehip_die(wavehip_ctx(), "init_wave_flat (no memory)", WV_ENOMEM);
+        wavehip_mask(m);
+        for (int i = 0; i < NX; i++) memcpy(xy_in[i], m + (size_t)i * NY, sizeof(short) * NY);
+        free(m);
+    }
+    for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++) phi[i][j] = psi[i][j] = 0.0;
+}
+
+static void wavehip_init_ior_2d(short int *xy_in[NX], double *tcc_table[NX], double *tc_table[NX],
+                                double *tgamma_table[NX], double ior_angle)
+{
+    const size_t nc = (size_t)NX * NY;
+    short *m = (short *)malloc(sizeof(short) * nc);
+    double *t = (double *)malloc(sizeof(double) * 3 * nc);
+    if (!m || !t) wavehip_die(wavehip_ctx(), "init_ior_2d (no memory)", WV_ENOMEM);
+    for (int i = 0; i < NX; i++) memcpy(m + (size_t)i * NY, xy_in[i], sizeof(short) * NY);
+    wavehip_media(m, ior_angle, t, t + nc, t + 2 * nc);
+    for (int i = 0; i < NX; i++) {
+        memcpy(tcc_table[i], t + (size_t)i * NY, sizeof(double) * NY);
+        memcpy(tc_table[i], t + nc + (size_t)i * NY, sizeof(double) * NY);
+        memcpy(tgamma_table[i], t + 2 * nc + (size_t)i * NY, sizeof(double) * NY);
+    }
+    free(m);
+    free(t);
+}
+#else
+static void wavehip_init_wave_flat_mod(double phi[NX * NY], double psi[NX * NY], short int xy_in[NX * NY])
+{
+    if (!XYIN_INITIALISED) wavehip_mask(xy_in);
+    for (size_t k = 0; k < (size_t)NX * NY; k++) phi[k] = psi[k] = 0.0;
+}
+#endif
+#endif /* WAVEHIP_INIT */
 
 #endif

@@ -168,3 +168,70 @@ def test_host_program_matches_oracle(oracle, tmp_path, name, mode, ncalls):
     for a, b, what in zip(got, exp, ("phi", "psi", "tmp")):
         assert np.array_equal(a.view(np.uint64), np.ascontiguousarray(b).view(np.uint64)), \
             "%s differs at %s" % (what, np.argwhere(a.view(np.uint64) != b.view(np.uint64))[:3])
+
+
+# ---- the start-up drop-ins (WAVEHIP_INIT, tests/compat/compat_init.c) --------
+
+INIT_VARIANTS = ["init_michelson", "init_wells", "init_flat_cmp"]
+# the macros compat_init.c fixes (its #define block)
+INIT_BOX = (-2.0, 2.0, -1.125, 1.125)
+INIT_DOMAIN = dict(lam=0.8, mu=0.3, mu_b=0.08, apoly=0.5, npoly=6, wall_width=0.1, wall_width_b=0.1,
+                   julia_scale=0.8, julia_x=0.37468, julia_y=0.21115, mandellimit=10.0, mdepth=4, mratio=3,
+                   mandellevel=100, ngridx=6, ngridy=5)
+INIT_IOR = dict(courant=0.1, courantb=0.05, gamma=1.0e-4, gammab=2.0e-4, lam=0.8, mu=0.3, mandel_ior_scale=-0.05,
+                mandellimit=10.0, mandellevel=100, mdepth=4)
+
+
+@pytest.mark.parametrize("name", INIT_VARIANTS)
+def test_init_program_built(name):
+    d = host_params(name)
+    assert d["nx"] >= 3 and d["ny"] >= 3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", INIT_VARIANTS)
+def test_init_program_matches_oracle(oracle, tmp_path, name):
+    """wavehip_init_wave_flat(_mod) and wavehip_init_ior_2d called as a
+    reference program calls init_wave_flat and init_ior_2d: the mask equals
+    the oracle's xy_in_billiard (with COMPARISON for the flat build), phi and
+    psi are zeroed, and the media equal the oracle's tables (bit for bit; the
+    wells within the GPU's exp, rtol 1e-14)"""
+    from gen_inputs import mask_circles, mask_circles_b
+    d = host_params(name)
+    nx, ny = d["nx"], d["ny"]
+    ca, cb = mask_circles(), mask_circles_b()
+    inp, outp = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as f:
+        for c in (ca, cb):
+            f.write(np.int32(c.shape[1]).tobytes())
+            f.write(np.ascontiguousarray(c, np.float64).tobytes())
+    say("compat init", name)
+    r = subprocess.run([os.path.join(BIN, name), str(inp), str(outp)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = open(outp, "rb").read()
+    n = nx * ny
+    xy = np.frombuffer(raw[:2 * n], np.int16).reshape(nx, ny)
+    rest = np.frombuffer(raw[2 * n:], np.float64)
+    P = oracle.domain_params(**INIT_DOMAIN)
+    if d["comparison"]:
+        want = oracle.comparison_mask(nx, ny, INIT_BOX, d["domain"], d["domain_b"], P, ca, cb)
+    else:
+        want = oracle.domain(nx, ny, INIT_BOX, d["domain"], P, ca if d["domain"] == 20 else None)
+    assert np.array_equal(xy, want)
+    assert np.all(rest[:2 * n] == 0.0)  # phi, psi
+    if rest.size == 2 * n:
+        return
+    tcc, tc, tg = (rest[(2 + q) * n:(3 + q) * n].reshape(nx, ny) for q in range(3))
+    p = oracle.make_params(nx, ny, twospeeds=d["twospeeds"])
+    if d["ior"] == 4:  # IOR_PERIODIC_WELLS, sub_wave.c:12317-12345
+        dx, dy = 4.0 / 6, 2.25 / 5
+        ii, jj = np.meshgrid(np.arange(6), np.arange(5), indexing="ij")
+        xc = (-2.0 + dx * (ii + 0.5)).T.ravel()
+        yc = (-1.125 + dy * (jj + 0.5) + np.where(jj % 2 == 1, 0.5 * dx, 0.0)).T.ravel()
+        ref = oracle.wells_tables(p, xy, xc, yc, None, 0.2 * dx * dx, 0, 0.1, 0.05, 1.0e-4, INIT_BOX)
+        for a, b in zip((tcc, tc, tg), ref):
+            assert np.allclose(a, b, rtol=1e-14, atol=0)
+    else:
+        ref = oracle.ior_tables(p, xy, d["ior"], INIT_BOX, oracle.IorParams(**INIT_IOR))
+        for a, b in zip((tcc, tc, tg), ref):
+            assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
