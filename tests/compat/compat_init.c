@@ -8,8 +8,10 @@
  * come from the input file.
  *
  *   compat_init IN OUT     IN: int32 n, 4n doubles (xc[], yc[], r[], active[]),
- *                          the same for circles_b; OUT: xy_in (int16), phi,
- *                          psi, and (row-pointer builds) tcc, tc, tgamma
+ *                          the same for circles_b (builds with XY_FROM_INPUT:
+ *                          then the mask, int16, and the int32 seed of the
+ *                          program's srand); OUT: xy_in (int16), phi, psi,
+ *                          and (row-pointer builds) tcc, tc, tgamma
  *   compat_init --params   prints the macros it was built with
  */
 #include <math.h>
@@ -60,8 +62,12 @@
 #define MRATIO 3
 #define MANDELLEVEL 100
 #define MANDELLIMIT 10.0
+#ifndef NGRIDX
 #define NGRIDX 6
+#endif
+#ifndef NGRIDY
 #define NGRIDY 5
+#endif
 #define COURANT 0.1
 #define COURANTB 0.05
 #define GAMMA 1.0e-4
@@ -89,6 +95,26 @@ int ncircles = NMAXCIRCLES, ncircles_b = NMAXCIRCLES;
 t_circle circles[NMAXCIRCLES], circles_b[NMAXCIRCLES];
 double julia_x = 0.37468, julia_y = 0.21115;
 double oscillating_bc(int time, int j) { (void)time; (void)j; return 0.0; }
+/* the program's normal deviates: Marsaglia's polar method on rand(), the
+ * second deviate of each accepted pair kept for the next call */
+double gaussian(void)
+{
+    static double v1, v2, s;
+    static int second = 0;
+    double x;
+    if (!second) {
+        do {
+            v1 = 2 * ((double)rand() / RAND_MAX) - 1;
+            v2 = 2 * ((double)rand() / RAND_MAX) - 1;
+            s = v1 * v1 + v2 * v2;
+        } while (s >= 1 || s == 0);
+        x = v1 * sqrt(-2 * log(s) / s);
+    } else {
+        x = v2 * sqrt(-2 * log(s) / s);
+    }
+    second = !second;
+    return x;
+}
 void xy_to_ij(double x, double y, int ij[2])
 {
     ij[0] = (int)((x - XMIN) / (XMAX - XMIN) * NX);
@@ -117,8 +143,13 @@ static void read_list(FILE *f, t_circle *c, int *n)
 int main(int argc, char **argv)
 {
     if (argc == 2 && strcmp(argv[1], "--params") == 0) {
-        printf("nx=%d ny=%d twospeeds=%d domain=%d domain_b=%d comparison=%d variable_ior=%d ior=%d\n", NX, NY,
-               TWOSPEEDS, B_DOMAIN, B_DOMAIN_B, COMPARISON, VARIABLE_IOR, IOR);
+#ifdef XY_FROM_INPUT
+        const int xyin = 1;
+#else
+        const int xyin = 0;
+#endif
+        printf("nx=%d ny=%d twospeeds=%d domain=%d domain_b=%d comparison=%d variable_ior=%d ior=%d xy_from_input=%d\n",
+               NX, NY, TWOSPEEDS, B_DOMAIN, B_DOMAIN_B, COMPARISON, VARIABLE_IOR, IOR, xyin);
         return 0;
     }
     if (argc != 3) {
@@ -129,6 +160,11 @@ int main(int argc, char **argv)
     if (!f) return 2;
     read_list(f, circles, &ncircles);
     read_list(f, circles_b, &ncircles_b);
+#ifdef XY_FROM_INPUT
+    static short xy_in_file[(size_t)NX * NY];
+    int seed = 0;
+    if (fread(xy_in_file, sizeof(short), NC, f) != NC || fread(&seed, sizeof(int), 1, f) != 1) return 2;
+#endif
     fclose(f);
     FILE *g = fopen(argv[2], "wb");
     if (!g) return 2;
@@ -142,6 +178,11 @@ int main(int argc, char **argv)
         for (int j = 0; j < NY; j++) phi[i][j] = psi[i][j] = 1.0;  /* init sets them to 0 */
     }
     wavehip_init_wave_flat(phi, psi, xy);
+#ifdef XY_FROM_INPUT
+    /* the program's own mask (XYIN_INITIALISED-like) and rand() sequence */
+    for (int i = 0; i < NX; i++) memcpy(xy[i], xy_in_file + (size_t)i * NY, sizeof(short) * NY);
+    srand((unsigned)seed);
+#endif
     wavehip_init_ior_2d(xy, tcc, tc, tg, 0.0);
     for (int i = 0; i < NX; i++) fwrite(xy[i], sizeof(short), NY, g);
     double **outs[5] = {phi, psi, tcc, tc, tg};

@@ -172,7 +172,7 @@ def test_host_program_matches_oracle(oracle, tmp_path, name, mode, ncalls):
 
 # ---- the start-up drop-ins (WAVEHIP_INIT, tests/compat/compat_init.c) --------
 
-INIT_VARIANTS = ["init_michelson", "init_wells", "init_flat_cmp"]
+INIT_VARIANTS = ["init_michelson", "init_wells", "init_flat_cmp", "init_rwells"]
 # the macros compat_init.c fixes (its #define block)
 INIT_BOX = (-2.0, 2.0, -1.125, 1.125)
 INIT_DOMAIN = dict(lam=0.8, mu=0.3, mu_b=0.08, apoly=0.5, npoly=6, wall_width=0.1, wall_width_b=0.1,
@@ -205,6 +205,10 @@ def test_init_program_matches_oracle(oracle, tmp_path, name):
         for c in (ca, cb):
             f.write(np.int32(c.shape[1]).tobytes())
             f.write(np.ascontiguousarray(c, np.float64).tobytes())
+        if d["xy_from_input"]:  # the mask and seed of the reference's IOR_RANDOM_WELLS fixture
+            from gen_inputs import case_inputs
+            f.write(np.ascontiguousarray(case_inputs("ior_wells_r", nx, ny)["xy_in"], np.int16).tobytes())
+            f.write(np.int32(7).tobytes())
     say("compat init", name)
     r = subprocess.run([os.path.join(BIN, name), str(inp), str(outp)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
@@ -212,6 +216,15 @@ def test_init_program_matches_oracle(oracle, tmp_path, name):
     n = nx * ny
     xy = np.frombuffer(raw[:2 * n], np.int16).reshape(nx, ny)
     rest = np.frombuffer(raw[2 * n:], np.float64)
+    if d["xy_from_input"]:
+        # init_ior_2d's own tables for this mask after srand(7)
+        # (tests/golden/media_cases.npz, reference build ior_wells_r)
+        g = np.load(os.path.join(HERE, "golden", "media_cases.npz"))
+        rest = np.frombuffer(raw[2 * n:], np.float64)
+        tcc, tc, tg = (rest[(2 + q) * n:(3 + q) * n].reshape(nx, ny) for q in range(3))
+        for a, key in ((tcc, "tcc"), (tc, "tc"), (tg, "tgamma")):
+            assert np.allclose(a, g["ior_wells_r/" + key], rtol=1e-14, atol=0), key
+        return
     P = oracle.domain_params(**INIT_DOMAIN)
     if d["comparison"]:
         want = oracle.comparison_mask(nx, ny, INIT_BOX, d["domain"], d["domain_b"], P, ca, cb)
