@@ -15,7 +15,12 @@ __global__ void __launch_bounds__(64)
 k_energy_rows(const WvInst *__restrict__ I, const int2 *__restrict__ rows, double *partial, int ia,
               int ib)
 {
-    const int2 rr = rows[blockIdx.x];
+    // workgroups go round-robin to the 8 XCDs: each XCD takes a contiguous
+    // band of the row list (a bijection), so the rows above and below a row
+    // are read through the same L2 (-15 % FP64, -24 % split at 4096^2)
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x, x = b & 7;
+    const int q = x * (nb >> 3) + min(x, nb & 7) + (b >> 3);
+    const int2 rr = rows[q];
     const WvInst &d = I[rr.x];
     const int i = rr.y, lane = threadIdx.x;
     const int nx = d.nx, ny = d.ny;
@@ -65,7 +70,7 @@ k_energy_rows(const WvInst *__restrict__ I, const int2 *__restrict__ rows, doubl
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) acc = acc + __shfl_down(acc, off, 64);
-    if (lane == 0) partial[blockIdx.x] = acc;
+    if (lane == 0) partial[q] = acc;
 }
 
 // one wave per instance over its row sums (same order as above)
