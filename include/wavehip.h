@@ -273,7 +273,7 @@ typedef struct wv_ior_params {
  * glibc's; the escape count itself is exact).  Returns WV_EUNSUPPORTED for
  * the Gaussian-well models 4-9 (wv_set_wells_tables, with wv_rotating_wells
  * for 6 and 7), for IOR_EXPLO_LENSING (3: the reference indexes its row
- * pointer array xy_in[i*NY+j], sub_wave.c:12292, past its NX entries) and for
+ * pointer array xy_in[i*NY+j], sub_wave.c:12287, past its NX entries) and for
  * codes init_ior_2d does not handle. */
 int wv_set_ior_tables(wv_ctx *c, const short *xy_in, int ior, const double box[4], const wv_ior_params *p,
                       double *tcc, double *tc, double *tgamma);
