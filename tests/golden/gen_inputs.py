@@ -251,9 +251,9 @@ PULSES = ((1.0, 0.7, -0.4), (-1.3, -1.6, 0.9), (0.35, 0.05, 0.02))
 
 
 # every closed-form domain of xy_in_billiard_single_domain (the codes of
-# global_pdes.c:14-236), on two reference builds (oracle/build_ref.sh dom_a,
-# dom_b) whose parameter blocks are given here; other macros keep the values
-# of oracle/ref_params.h
+# global_pdes.c:14-236), on three reference builds (oracle/build_ref.sh dom_a,
+# dom_b, dom_c) whose parameter blocks are given here; other macros keep the
+# values of oracle/ref_params.h
 DOMAIN_CODES = (999, 0, 1, 196, 199, 198, 197, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 15, 16, 17, 18, 19, 191, 195,
                 1951, 1952, 20, 201, 202, 24, 25, 26, 31, 32, 33, 34, 50, 51, 522, 523, 524, 525, 61, 62, 63, 64, 67,
                 68, 69, 73, 74, 76, 79, 791, 795, 796, 90, 91)
@@ -280,6 +280,9 @@ def domain_sets():
         "dom_a": (97, 61, dict(lam=0.6, mu=0.3, npoly=6, apoly=0.5)),
         "dom_b": (160, 90, dict(lam=1.2, mu=0.15, npoly=9, apoly=1.0, mdepth=3, julia_scale=1.0, mandellevel=200,
                                 wall_width=0.075, mu_b=0.3, wall_width_b=0.15, ngridx=8, ngridy=6)),
+        "dom_c": (121, 77, dict(lam=0.45, mu=0.05, npoly=12, apoly=0.25, mdepth=5, mratio=5, julia_scale=1.3,
+                                mandellevel=500, mandellimit=4.0, wall_width=0.2, mu_b=0.15, wall_width_b=0.05,
+                                ngridx=12, ngridy=8)),
     }
 
 

@@ -1,9 +1,10 @@
 """The closed-form billiard domains of the oracle (orc_domain) against the
 reference's own xy_in_billiard_single_domain (sub_wave.c:5712-7602).
 
-tests/golden/domain_cases.npz holds the masks two reference builds (dom_a:
-97x61, LAMBDA 0.6, MU 0.3, NPOLY 6, APOLY 0.5; dom_b: 160x90 with another
-value for every macro the domains read) computed for every code of
+tests/golden/domain_cases.npz holds the masks three reference builds (dom_a:
+97x61, LAMBDA 0.6, MU 0.3, NPOLY 6, APOLY 0.5; dom_b: 160x90 and dom_c:
+121x77, each with another value for every macro the domains read) computed
+for every code of
 gen_inputs.DOMAIN_CODES at the cell corners of ij_to_xy.  Every code but the
 two constant domains takes more than one value in one of the builds, and a
 planted mistake -- the cell centre instead of the corner, a parameter off by
@@ -50,17 +51,18 @@ def test_fixtures_are_not_trivial(gold):
 
 
 def test_fixtures_catch_planted_mistakes(oracle, gold):
-    nx, ny, _ = domain_sets()["dom_b"]
     b = MASK_BOX
-    hx, hy = 0.5 * (b[1] - b[0]) / nx, 0.5 * (b[3] - b[2]) / ny
-    centre = (b[0] + hx, b[1] + hx, b[2] + hy, b[3] + hy)
+
+    def centre(tag):
+        nx, ny, _ = domain_sets()[tag]
+        hx, hy = 0.5 * (b[1] - b[0]) / nx, 0.5 * (b[3] - b[2]) / ny
+        return (b[0] + hx, b[1] + hx, b[2] + hy, b[3] + hy)
+
     for dom in DOMAIN_CODES:
         if dom in CONSTANT:
             continue
-        hits = [not np.array_equal(oracle_mask(oracle, t, dom, centre if t == "dom_b" else
-                                               (b[0] + 0.5 * (b[1] - b[0]) / 97, b[1] + 0.5 * (b[1] - b[0]) / 97,
-                                                b[2] + 0.5 * (b[3] - b[2]) / 61, b[3] + 0.5 * (b[3] - b[2]) / 61)),
-                                   gold["%s/%d" % (t, dom)]) for t in domain_sets()]
+        hits = [not np.array_equal(oracle_mask(oracle, t, dom, centre(t)), gold["%s/%d" % (t, dom)])
+                for t in domain_sets()]
         assert any(hits), dom
     # LAMBDA or MU off by a part in a thousand moves the boundary of most domains
     moved = 0
