@@ -181,7 +181,10 @@ typedef struct wv_domain_params {
     int npoly;                        /* NPOLY (1..4096)                        */
     int mdepth, mratio;               /* MDEPTH (0..30), MRATIO (>= 1)          */
     int mandellevel;                  /* MANDELLEVEL                            */
-    int ngridx, ngridy;               /* NGRIDX, NGRIDY (D_DISK_GRID, D_DISK_HEX) */
+    int ngridx, ngridy;               /* NGRIDX, NGRIDY (disk grids, lattices)  */
+    double wall_width_asym;           /* WALL_WIDTH_ASYM (non-isotropic lattices) */
+    double wall_width_asym_b;         /* WALL_WIDTH_ASYM_B                      */
+    double oscil_ymid;                /* OSCIL_YMID (D_TREE_BEAM)               */
 } wv_domain_params;
 
 /* The billiard mask of init_wave_flat (wave_common.c:294-319) on the GPU for
@@ -205,11 +208,33 @@ typedef struct wv_domain_params {
  *   D_LENS_ROTATED 67, D_LENS_CONCAVE 68, D_LENS_CONVEX_CONCAVE 69, D_TREE 73,
  *   D_MICHELSON 74, D_GRADIENT_INDEX_LENS 76, D_TWOCIRCLES 79,
  *   D_POLYCIRCLES 791, D_TWO_ELLIPSES 795, D_TWO_RECTANGLES 796,
- *   D_CARDIOID 90, D_NEPHROID 91.
- * Values are the reference's (0, 1 and, for some domains, 2).  The circle
+ *   D_CARDIOID 90, D_NEPHROID 91, D_TOKARSKY 35, D_FRESNEL 43,
+ *   D_DOUBLE_FRESNEL 45, D_NOISEPANEL 44, D_NOISEPANEL_RECT 441, D_QRD 46,
+ *   D_QRD_ASYM 461, D_CIRCLE_SEGMENT 47, D_GROOVE 48, D_FABRY_PEROT 49,
+ *   D_WAVEGUIDE_W 52, D_WAVEGUIDES_W 521, D_WAVEGUIDE_BADSPLICE 526,
+ *   D_CHESSBOARD 55, D_TRIANGLE_TILES 56, D_HEX_TILES 57, D_FUNNELS 58,
+ *   D_ONE_FUNNEL 581, D_LENSES_RING 59, D_FRESNEL_ZONE_PLATE 65,
+ *   D_FRESNEL_ZONE_PLATE_INV 66, D_TREE_BEAM 731,
+ *   D_RITCHEY_CHRETIEN_SPHERICAL 75, D_RITCHEY_CHRETIEN_HYPERBOLIC 751,
+ *   D_MAGNETRON 78, D_MAGNETRON_CATHODE 781, D_POLYCIRCLES_ANGLED 792,
+ *   D_TWOCIRCLES_ELLIPSE 793, D_CIRCLES_ELLIPSE 794, D_TWO_POLYGONS 797,
+ *   D_TWO_POLY_PARABOLAS 798, D_CIRCLE_LATTICE 93, D_CIRCLE_LATTICE_NONISO 934,
+ *   D_CIRCLE_LATTICE_HALF 935, D_CIRCLE_LATTICE_HALF_V 936,
+ *   D_CIRCLE_LATTICE_STRIP 937, D_CIRCLE_LATTICE_HEX 94,
+ *   D_CIRCLE_LATTICE_HEX_NONISO 942, D_CIRCLE_LATTICE_HEX_STRIP 947,
+ *   D_DISC_WAVEGUIDE 98, D_DISC_WAVEGUIDE_SHIFTED 981,
+ *   D_DISC_WAVEGUIDE_ELLIPSE 982, D_WAVEGUIDE_ELLIPSE_OBLIQUE 983,
+ *   D_CIRCLE_PAIRS 99, D_CIRCLE_POLYGON 991, D_ANNULUS_HEATED 21,
+ *   D_MENGER_HEATED 22, D_MENGER_H_OPEN 23.
+ * Values are the reference's (0, 1 and, for some domains, 2; the heated
+ * Menger gaskets 22 and 23 give k + 2 in a hole of depth k).  The one-time
+ * constants the reference keeps in static variables (D_TRIANGLE_TILES,
+ * D_HEX_TILES, D_LENSES_RING, D_RITCHEY_CHRETIEN_HYPERBOLIC) are those of a
+ * program that uses this one domain.  The circle
  * list (4*ncircles doubles, as wv_build_mask) is read by the D_CIRCLES
- * family.  Cosines and sines of fixed angles (polygon sides, PI/6 of D_TREE)
- * are taken on the host with the C library, as the reference does; the
+ * family.  Cosines, sines and tangents of fixed angles (polygon sides, PI/6
+ * of D_TREE) are taken on the host with the C library, as the reference
+ * does (the zone plates' per-cell cosine enters by its sign only); the
  * per-cell arithmetic is IEEE binary64 without contraction, so masks equal
  * the reference's cell for cell.  Other codes return WV_EUNSUPPORTED (they
  * read lists that other reference functions fill: polygons, mazes, images,

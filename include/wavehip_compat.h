@@ -387,7 +387,8 @@ static void evolve_wave(double phi[NX * NY], double psi[NX * NY], double tmp[NX 
  * wave_3d.c:1212, with WAVEHIP_FLAT); same arguments.  They read the
  * program's #define block (B_DOMAIN, LAMBDA, MU, MU_B, APOLY, NPOLY,
  * WALL_WIDTH, WALL_WIDTH_B, JULIA_SCALE, MDEPTH, MRATIO, MANDELLEVEL,
- * MANDELLIMIT, NGRIDX, NGRIDY, XMIN..YMAX, XYIN_INITIALISED; VARIABLE_IOR,
+ * MANDELLIMIT, NGRIDX, NGRIDY, WALL_WIDTH_ASYM, WALL_WIDTH_ASYM_B, OSCIL_YMID,
+ * XMIN..YMAX, XYIN_INITIALISED; VARIABLE_IOR,
  * IOR, COURANT, COURANTB, GAMMA, GAMMAB, MANDEL_IOR_SCALE; COMPARISON and
  * B_DOMAIN_B) and its globals (circles, ncircles, circles_b, ncircles_b,
  * julia_x, julia_y).  The random wells (IOR 5, 8, 9) are drawn with the
@@ -404,6 +405,7 @@ static void wavehip_mask(short *flat)
     dp.wall_width_b = WALL_WIDTH_B; dp.julia_scale = JULIA_SCALE; dp.julia_x = julia_x; dp.julia_y = julia_y;
     dp.mandellimit = MANDELLIMIT; dp.npoly = NPOLY; dp.mdepth = MDEPTH; dp.mratio = MRATIO;
     dp.mandellevel = MANDELLEVEL; dp.ngridx = NGRIDX; dp.ngridy = NGRIDY;
+    dp.wall_width_asym = WALL_WIDTH_ASYM; dp.wall_width_asym_b = WALL_WIDTH_ASYM_B; dp.oscil_ymid = OSCIL_YMID;
     const int nl = COMPARISON ? 2 : 1;
     const t_circle *cs[2] = {circles, COMPARISON ? circles_b : circles};
     const int nc[2] = {ncircles, COMPARISON ? ncircles_b : 0};

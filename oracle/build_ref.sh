@@ -175,11 +175,12 @@ for p in "${pids[@]}"; do wait "$p"; done; pids=()
 build_one "dom_a" 5 -DNX=97 -DNY=61 -DLAMBDA=0.6 -DMU=0.3 -DNPOLY=6 -DAPOLY=0.5 &
 pids+=($!)
 build_one "dom_b" 5 -DNX=160 -DNY=90 -DLAMBDA=1.2 -DMU=0.15 -DNPOLY=9 -DAPOLY=1.0 -DMDEPTH=3 -DJULIA_SCALE=1.0 \
-    -DMANDELLEVEL=200 -DWALL_WIDTH=0.075 -DMU_B=0.3 -DWALL_WIDTH_B=0.15 -DNGRIDX=8 -DNGRIDY=6 &
+    -DMANDELLEVEL=200 -DWALL_WIDTH=0.075 -DMU_B=0.3 -DWALL_WIDTH_B=0.15 -DNGRIDX=8 -DNGRIDY=6 \
+    -DWALL_WIDTH_ASYM=1.25 -DWALL_WIDTH_ASYM_B=0.5 -DOSCIL_YMID=-0.3 &
 pids+=($!)
 build_one "dom_c" 5 -DNX=121 -DNY=77 -DLAMBDA=0.45 -DMU=0.05 -DNPOLY=12 -DAPOLY=0.25 -DMDEPTH=5 -DMRATIO=5 \
-    -DJULIA_SCALE=1.3 -DMANDELLEVEL=500 -DMANDELLIMIT=4.0 -DWALL_WIDTH=0.2 -DMU_B=0.15 -DWALL_WIDTH_B=0.05 \
-    -DNGRIDX=12 -DNGRIDY=8 &
+    -DJULIA_SCALE=1.3 -DMANDELLEVEL=500 -DMANDELLIMIT=4.0 -DWALL_WIDTH=0.04 -DMU_B=0.15 -DWALL_WIDTH_B=0.05 \
+    -DNGRIDX=12 -DNGRIDY=8 -DWALL_WIDTH_ASYM=0.6 -DWALL_WIDTH_ASYM_B=1.4 -DOSCIL_YMID=0.45 &
 pids+=($!)
 # xy_in_billiard with COMPARISON (two domains, two circle lists)
 build_one "dom_cmp_a" 5 -DNX=97 -DNY=61 -DLAMBDA=0.6 -DMU=0.3 -DNPOLY=6 -DAPOLY=0.5 -DCOMPARISON=1 -DB_DOMAIN=20 \

@@ -138,6 +138,435 @@ static int o_escapes(double u, double v, double cx, double cy, const orc_domain_
     return !(u * u + v * v < d->mandellimit);
 }
 
+/* in_polygon, sub_wave.c:362-376: inside the regular npoly-gon of
+ * circumradius r turned by apoly*PID */
+static int o_inpoly(double x, double y, double r, int npoly, double apoly)
+{
+    int condition = 1, k;
+    const double omega = O_DPI / ((double)npoly), cw = cos(omega * 0.5);
+    for (k = 0; k < npoly; k++) {
+        const double angle = apoly * O_PID + ((double)k + 0.5) * omega;
+        condition = condition * (x * cos(angle) + y * sin(angle) < r * cw);
+    }
+    return condition;
+}
+
+/* the closed-form cases of xy_in_billiard_single_domain beyond the families
+ * of o_domain_at; the reference's one-time constants (its static `first`
+ * blocks, 6631-6636, 6647-6652, 6685-6690, rc_hyp 5444-5472) are computed
+ * per call, as a program that uses one domain computes them once */
+static int o_domain_more(double x, double y, int dom, const orc_domain_params *d, const double box[4])
+{
+    const double LAMBDA = d->lambda, MU = d->mu, MU_B = d->mu_b, WALL_WIDTH = d->wall_width;
+    const double XMIN = box[0], XMAX = box[1], YMIN = box[2], YMAX = box[3];
+    const int NPOLY = d->npoly, NGRIDX = d->ngridx, NGRIDY = d->ngridy, MRATIO = d->mratio;
+    double x1, y1, x2, y2, r, a, b, h, hh, dx, dy, nx, ny, width, height, angle, omega, ca, sa, wa, wb;
+    int i, j, k, k1, condition;
+    switch (dom) {
+    case 35:                                                                         /* 6068 */
+        x1 = 4.0 + x / (XMAX - XMIN) * 8.4;
+        y1 = 2.0 + y / (XMAX - XMIN) * 8.4;
+        if ((x1 <= 0.0) || (x1 >= 8.0)) return 0;
+        if (x1 < 1.0) return (y1 > 2.0) && (y1 < x1 + 2.0);
+        if (x1 < 2.0) return (y1 > 1.0) && (y1 < 4.0);
+        if (x1 < 3.0) return (y1 > x1 - 2.0) && (y1 < 3.0);
+        if (x1 < 4.0) return (y1 > 1.0) && (y1 < 2.0);
+        if (x1 < 5.0) return (y1 > x1 - 4.0) && (y1 < 2.0);
+        if (x1 < 6.0) return (y1 > 1.0) && (y1 < 3.0);
+        if (x1 < 7.0) return (y1 > x1 - 6.0) && (y1 < 10.0 - x1);
+        return (y1 > 2.0) && (y1 < 3.0);
+    case 43:                                                                         /* 6288 */
+        if (o_abs(y) > 0.9 * o_abs(LAMBDA)) return 1;
+        if (o_abs(x) > MU) return 1;
+        x1 = sqrt(LAMBDA * LAMBDA - y * y) - o_abs(LAMBDA);
+        while (x1 <= 0.0) x1 += MU;
+        if (LAMBDA > 0.0) return (x < x1) ? 0 : 1;
+        return (x > -x1) ? 0 : 1;
+    case 45:                                                                         /* 6307 */
+        if (o_abs(y) > 0.9 * o_abs(LAMBDA)) return 1;
+        if (LAMBDA > 0.0) {
+            if (o_abs(x) > LAMBDA + MU) return 1;
+            x1 = sqrt(LAMBDA * LAMBDA - y * y) - LAMBDA;
+            while (x1 <= 0.0) x1 += MU;
+            x1 -= LAMBDA;
+            return (o_abs(x) > -x1) ? 0 : 1;
+        }
+        if (o_abs(x) < -LAMBDA - MU) return 1;
+        x1 = sqrt(LAMBDA * LAMBDA - y * y) + LAMBDA;
+        while (x1 <= 0.0) x1 += MU;
+        x1 -= LAMBDA;
+        return (o_abs(x) > x1) ? 1 : 0;
+    case 44: case 441:                                                               /* 6331, 6339 */
+        x1 = (dom == 44) ? o_abs(x) : -x;
+        if (dom == 441 && !(x1 > 0.02)) {                                            /* NPWIDTH 0.02 */
+            if (x > 0.02) return (o_abs(y) < YMAX - 0.1) && (x < XMAX - 0.1);
+            return 0;
+        }
+        while (x1 > 2.0 * LAMBDA) x1 -= 2.0 * LAMBDA;
+        if (x1 <= LAMBDA) y1 = 0.1 + MU * x1 / LAMBDA;
+        else y1 = 0.1 + 2.0 * MU - MU * x1 / LAMBDA;
+        if (dom == 44) return (y > YMIN + y1) && (y < YMAX - y1);
+        return (y > YMIN + y1) && (y < YMAX - y1) && (x > XMIN + 0.1);
+    case 46:                                                                         /* 6355 */
+        x1 = o_abs(x) / LAMBDA;
+        k = (int)(x1 + 0.5);
+        k1 = (k * k) % 13;
+        y1 = (MU / 13.0) * (14.0 - (double)k1);
+        return (y > YMIN + y1) && (y < YMAX - y1);
+    case 461:                                                                        /* 6363 */
+        if (y > 0.0) {
+            x1 = o_abs(x) / LAMBDA;
+            k = (int)(x1 + 0.5);
+            k1 = (k * k) % 13;
+            y1 = (MU / 13.0) * (14.0 - (double)k1);
+            return y < YMAX - y1;
+        }
+        x1 = o_abs(x + 1.0) / LAMBDA;
+        k = (int)(x1 + 0.5);
+        k1 = (k * k) % 17;
+        y1 = (MU / 17.0) * (18.0 - (double)k1);
+        return y > YMIN + y1;
+    case 47:                                                                         /* 6382 */
+        if (o_abs(y) > 0.9 * o_abs(LAMBDA)) return 1;
+        y1 = 0.9 * LAMBDA;
+        x1 = sqrt(LAMBDA * LAMBDA - y1 * y1) - o_abs(LAMBDA) + MU;
+        if ((LAMBDA > 0.0) && (x < x1)) return 1;
+        else if ((LAMBDA < 0.0) && (x > -x1)) return 1;
+        x1 = sqrt(LAMBDA * LAMBDA - y * y) - o_abs(LAMBDA) + MU;
+        if (LAMBDA > 0.0) return (x < x1) ? 0 : 1;
+        return (x > -x1) ? 0 : 1;
+    case 48:                                                                         /* 6403 */
+        b = 0.85 * LAMBDA;
+        a = 0.5 * LAMBDA;
+        x1 = x - XMIN - (double)((int)((x - XMIN) / LAMBDA)) * LAMBDA;
+        if (x1 < a) return y > YMIN + LAMBDA;
+        return y > YMIN + LAMBDA + b;
+    case 49: return o_abs(x - y * LAMBDA / YMAX) > 0.5 * MU;                         /* 6411 */
+    case 52:                                                                         /* 6437 */
+        x1 = o_abs(x);
+        width = LAMBDA - 2.0 * MU;
+        height = 0.5 * MU;
+        if (x1 > 2.0 * LAMBDA - MU) return 0;
+        if (o_abs(y) > MU + width + height) return 0;
+        if (y >= height) {
+            r = o_mod(x1, y - height);
+            if ((r > MU) && (r < MU + width)) return 1;
+            if (x1 > LAMBDA + MU) return 1;
+            return 0;
+        }
+        if (y <= -height) {
+            r = o_mod(x1 - LAMBDA, y + height);
+            return (r > MU) && (r < MU + width);
+        }
+        if (x1 > LAMBDA + MU) return 1;
+        return (x1 > MU) && (x1 < MU + width);
+    case 521:                                                                        /* 6461 */
+        x1 = o_abs(x);
+        width = LAMBDA - 2.0 * MU;                                                   /* upper fibre */
+        height = MU + width;
+        if (y > MU + width + height) return 0;
+        if (y >= height) {
+            r = o_mod(x1, y - height);
+            if ((r > MU) && (r < MU + width)) return 1;
+            return (y < height + MU) && (x1 > LAMBDA + MU) && (x1 < LAMBDA + MU + width);
+        }
+        r = o_mod(x1 - LAMBDA, y - height);
+        if ((r > MU) && (r < MU + width)) return 1;
+        height = -MU - width;                                                        /* lower fibre */
+        width = LAMBDA - 2.0 * MU_B;
+        if (y < -MU_B - width + height) return 0;
+        if (y >= height) {
+            r = o_mod(x1, y - height);
+            if ((r > MU_B) && (r < MU_B + width)) return 1;
+            return (y < height + MU_B) && (x1 > LAMBDA + MU_B) && (x1 < LAMBDA + MU_B + width);
+        }
+        r = o_mod(x1 - LAMBDA, y - height);
+        return (r > MU_B) && (r < MU_B + width);
+    case 526:                                                                        /* 6575 */
+        if (x < 0.0) y1 = o_abs(o_abs(y) - 0.5);
+        else if (y > 0.0) y1 = o_abs(y - 0.5 - MU);
+        else y1 = o_abs(y + 0.5 - MU_B);
+        if (y1 < 0.9 * LAMBDA) return 1;
+        if (y1 < LAMBDA) return 2;
+        return 0;
+    case 55:                                                                         /* 6622 */
+        i = (int)(o_abs(x) / LAMBDA + 0.5);
+        j = (int)(o_abs(y) / LAMBDA + 0.5);
+        return (i + j) % 2 == 0;
+    case 56:                                                                         /* 6629 */
+        h = LAMBDA / (2.0 * sqrt(3.0));
+        hh = h * 3.0;
+        i = (int)((y + h) / hh + 10.0);
+        y1 = sin(O_DPI / 3.0) * x - 0.5 * y;
+        j = (int)((y1 + h) / hh + 10.0);
+        y1 = sin(-O_DPI / 3.0) * x - 0.5 * y;
+        k = (int)((y1 + h) / hh + 10.0);
+        return (i + j + k) % 2 == 0;
+    case 57:                                                                         /* 6645 */
+        a = -1.0 / sqrt(3.0);
+        b = -2.0 * a;
+        x1 = (x + a * y) / LAMBDA + 30.0;
+        y1 = b * y / LAMBDA + 30.0;
+        x1 = x1 - (double)(3 * (int)(x1 / 3.0));
+        y1 = y1 - (double)(3 * (int)(y1 / 3.0));
+        if ((x1 > 2.0) && (y1 < 1.0)) return 1;
+        if ((x1 < 1.0) && (y1 > 2.0)) return 1;
+        if (x1 + y1 < 1.0) return 1;
+        return x1 + y1 > 5.0;
+    case 58:                                                                         /* 6665 */
+        y1 = y;
+        if (y > 0.5 * YMAX) y1 -= YMAX;
+        if (y < -0.5 * YMAX) y1 += YMAX;
+        y1 = o_abs(y1 - MU * x);
+        y1 = o_abs(y1 - 0.5 * YMAX) * 2.0 / YMAX;
+        return !(y1 > 0.25 * (1.0 + LAMBDA + x * x));
+    case 581: return !(o_abs(y) > MU + LAMBDA * x * x);                              /* 6675 */
+    case 59:                                                                         /* 6681 */
+        a = O_DPI / (double)NPOLY;
+        h = LAMBDA * tan(O_PI / (double)NPOLY);
+        b = (h < MU) ? sqrt(MU * MU - h * h) : 0.0;
+        for (i = 0; i < NPOLY; i++) {
+            ca = cos((double)i * a + d->apoly * O_PID);
+            sa = sin((double)i * a + d->apoly * O_PID);
+            x1 = x * ca + y * sa;
+            y1 = -x * sa + y * ca;
+            if ((o_mod(x1 - LAMBDA - b, y1) < MU) && (o_mod(x1 - LAMBDA + b, y1) < MU)) return 0;
+        }
+        return 1;
+    case 65:                                                                         /* 6736 */
+        if (o_abs(x + LAMBDA) >= MU) return 1;
+        return cos(O_DPI * (6.25 * y * y + 0.125)) < 0.0;
+    case 66:                                                                         /* 6743 */
+        if (o_abs(x + LAMBDA) >= MU) return 1;
+        return cos(O_DPI * 6.25 * y * y) > 0.0;
+    case 731: {                                                                      /* 6828 */
+        const double c6 = cos(O_PI / 6.0), lev[3] = {1.5, 0.25, -1.5}, rad[3] = {1.5, 2.0, 3.0};
+        x1 = o_abs(x);
+        for (k = 0; k < 3; k++) {
+            r = rad[k] * LAMBDA;
+            const double z1[2] = {r * c6, -0.5 * r}, z2[2] = {0.0, r}, z3[2] = {0.0, -0.5 * r};
+            if (o_tri(x1, y - lev[k] * LAMBDA, z1, z2, z3)) return 1;
+        }
+        if ((x < 0.0) && (y > d->oscil_ymid - 0.5 * WALL_WIDTH) && (y < d->oscil_ymid + 0.5 * WALL_WIDTH))
+            return 1;
+        if ((x < -(3.0 * LAMBDA) * c6 + 0.5 * WALL_WIDTH) && (y > d->oscil_ymid - 0.75 * WALL_WIDTH) &&
+            (y < d->oscil_ymid + 0.75 * WALL_WIDTH))
+            return 2;
+        return 0;
+    }
+    case 75:                                                                         /* 6892 */
+        a = 2.5;
+        b = 3.5;
+        x2 = 2.0 * (a + b / LAMBDA);                                                 /* r1 */
+        y2 = 6.0 / (LAMBDA - 1.0);                                                   /* r2 */
+        x1 = 0.5 * a - x2;
+        h = -0.5 * a - y2;
+        y1 = o_abs(y);
+        if (x < 0.0) {
+            if (y1 > 0.3) return 1;
+            if (x < -0.5 * a - WALL_WIDTH) return 1;
+            return o_mod(x - h, y1) > y2;
+        }
+        if (y1 < MU) return 1;
+        if (x > 0.5 * a + WALL_WIDTH) return 1;
+        return o_mod(x - x1, y1) < x2;
+    case 751: {                                                                      /* rc_hyp 5437 */
+        const double m = LAMBDA, dd = 2.5, bb = 3.0;
+        const double f = m * dd + bb, f1 = f / m, dprime = f1 - dd, f2 = m * dprime / (m + 1);
+        const double r1 = 2.0 * f / m, r2 = 2.0 * bb / (m - 1.0);
+        const double e1 = sqrt(1.0 + 2.0 * bb / (m * m * m * dd));
+        const double e2 = sqrt(1.0 + 2.0 * (m * (2.0 * m - 1.0) + bb / dd) / ((m - 1) * (m - 1) * (m - 1)));
+        const double a1 = f1 / e1, b1 = sqrt(a1 * r1), x01 = 0.5 * dd + a1;
+        const double a2 = f2 / e2, b2 = sqrt(a2 * r2), x02 = -0.5 * dd + a2;
+        y1 = o_abs(y);
+        if (x > 0.0) {
+            if (y1 < MU) return 1;
+            if (x > 0.5 * dd + WALL_WIDTH) return 1;
+            return (x - x01) * (x - x01) > a1 * a1 * (1.0 + y * y / (b1 * b1));
+        }
+        if (y1 > 0.3) return 1;
+        return (x > x02) || ((x - x02) * (x - x02) < a2 * a2 * (1.0 + y * y / (b2 * b2)));
+    }
+    case 78: case 781:                                                               /* 6929, 6947 */
+        r = o_mod(x, y);
+        if (dom == 781 && r < WALL_WIDTH) return 0;
+        if (r > LAMBDA) return 1;
+        if (r < 0.33 * LAMBDA) return 1;
+        if ((o_abs(y) < WALL_WIDTH) && (x > -0.66 * LAMBDA)) return 1;
+        if ((o_abs(x) < WALL_WIDTH) && (o_abs(y) < 0.66 * LAMBDA)) return 1;
+        if ((o_abs(x - y) < WALL_WIDTH * sqrt(2.0)) && (r < 0.66 * LAMBDA)) return 1;
+        if ((o_abs(x + y) < WALL_WIDTH * sqrt(2.0)) && (r < 0.66 * LAMBDA)) return 1;
+        for (k = 0; k < 8; k++) {
+            x1 = 0.66 * cos((double)k * 0.5 * O_PID);
+            y1 = 0.66 * sin((double)k * 0.5 * O_PID);
+            if (o_mod(x - x1, y - y1) < MU) return 1;
+        }
+        return 0;
+    case 792: {                                                                      /* 6992 */
+        if (o_mod(x, y) < LAMBDA) return 1;
+        const double cb = cos(-0.5 * O_PID), sb = sin(-0.5 * O_PID);
+        for (k = 0; k < NPOLY; k++) {
+            angle = d->apoly * O_PID + (double)k * O_DPI / (double)NPOLY;
+            ca = cos(angle);
+            sa = sin(angle);
+            x1 = x * ca + y * sa;
+            y1 = -x * sa + y * ca;
+            x2 = LAMBDA + 2.0 * MU * cb;
+            y2 = 2.0 * MU * sb;
+            if (o_mod(x1 - x2, y1 - y2) < MU) return 1;
+            const double c1 = -sb * LAMBDA + cb * WALL_WIDTH, c2 = -sb * LAMBDA - cb * WALL_WIDTH;
+            b = -sb * x1 + cb * y1;
+            if ((x1 > 0.5 * LAMBDA) && (x1 < LAMBDA + MU) && (b < c1) && (b > c2)) return 1;
+        }
+        return 0;
+    }
+    case 793:                                                                        /* 7015 */
+        a = 0.9 * XMAX - 3.0 * MU;
+        b = a / LAMBDA;
+        if (x * x / (a * a) + y * y / (b * b) < 1.0) return 1;
+        x1 = a + 2.0 * MU;
+        if ((x - x1) * (x - x1) + y * y < MU * MU) return 1;
+        if ((x + x1) * (x + x1) + y * y < MU * MU) return 1;
+        return (o_abs(x) < a + 2.0 * MU) && (o_abs(y) < WALL_WIDTH);
+    case 794:                                                                        /* 7026 */
+        a = 0.9 * XMAX - 3.0 * MU;
+        b = a / LAMBDA;
+        x = x / d->julia_scale;
+        y = y / d->julia_scale;
+        if (x * x / (a * a) + y * y / (b * b) < 1.0) return 1;
+        for (k = 0; k < NPOLY; k++) {
+            angle = d->apoly * O_PID + (double)k * O_DPI / (double)NPOLY;
+            x1 = a * cos(angle);
+            y1 = b * sin(angle);
+            nx = b * cos(angle);
+            ny = a * sin(angle);
+            r = 1.0 / o_mod(nx, ny);
+            nx *= r;
+            ny *= r;
+            x2 = x1 + 2.0 * MU * nx;
+            y2 = y1 + 2.0 * MU * ny;
+            if (o_mod(x - x2, y - y2) < MU) return 1;
+            r = o_mod(x - x1, y - y1);
+            h = (x - x1) * nx + (y - y1) * ny;
+            if ((h > -MU) && (h < 2.0 * MU) && (r * r - h * h < WALL_WIDTH * WALL_WIDTH)) return 1;
+        }
+        return 0;
+    case 797: return o_inpoly(o_abs(x) - LAMBDA, y, MU, NPOLY, d->apoly);            /* 7062 */
+    case 798:                                                                        /* 7068 */
+        condition = 1;
+        omega = O_DPI / ((double)NPOLY);
+        if (x > 0.0) x -= LAMBDA;
+        else x += LAMBDA;
+        for (k = 0; k < NPOLY; k++) {
+            angle = d->apoly * O_PID + ((double)k + 0.5) * omega;
+            x1 = x * cos(angle) + y * sin(angle);
+            y1 = -x * sin(angle) + y * cos(angle);
+            condition = condition * (x1 < MU - 0.25 * y1 * y1 / MU);
+        }
+        return condition;
+    case 93: case 934: case 935: case 936:                                           /* 7110, 7225, 7243, 7262 */
+        if (dom == 935 && y > 0.0) return 1;
+        if (dom == 936 && x < 0.0) return 1;
+        wa = (dom == 93) ? WALL_WIDTH : WALL_WIDTH * d->wall_width_asym;
+        dx = (XMAX - XMIN) / (double)NGRIDX;
+        dy = (YMAX - YMIN) / (double)NGRIDY;
+        for (i = 0; i < NGRIDX; i++) {
+            x1 = XMIN + ((double)i + 0.5) * dx;
+            if (o_abs(x - x1) < wa) return 1;
+            for (j = 0; j < NGRIDY; j++) {
+                y1 = YMIN + ((double)j + 0.5) * dy;
+                if (o_mod(x - x1, y - y1) < MU) return 1;
+                if (o_abs(y - y1) < WALL_WIDTH) return 1;
+            }
+        }
+        return 0;
+    case 937:                                                                        /* 7281 */
+        if (x > LAMBDA) return 1;
+        if (x < -LAMBDA) return 1;
+        dx = 2.0 * LAMBDA / (double)NGRIDX;
+        dy = (YMAX - YMIN) / (double)NGRIDY;
+        for (i = 0; i < NGRIDX + 1; i++) {
+            x1 = -LAMBDA + ((double)i) * dx;
+            if (o_abs(x - x1) < WALL_WIDTH * d->wall_width_asym) return 1;
+            for (j = 0; j < NGRIDY; j++) {
+                y1 = YMIN + ((double)j + 0.5) * dy;
+                if (o_mod(x - x1, y - y1) < MU) return 1;
+                if (o_abs(y - y1) < WALL_WIDTH) return 1;
+            }
+        }
+        return 0;
+    case 94: case 942: case 947:                                                     /* 7146, 7321, 7344 */
+        if (dom == 947 && o_abs(x) > LAMBDA) return 1;
+        dx = (dom == 947) ? 2.0 * LAMBDA / (double)NGRIDX : (XMAX - XMIN) / (double)NGRIDX;
+        dy = (YMAX - YMIN) / (double)NGRIDY;
+        r = o_mod(0.5 * dx, dy);
+        nx = -dy / r;
+        ny = dx / (2.0 * r);
+        wa = (dom == 94) ? WALL_WIDTH : WALL_WIDTH * d->wall_width_asym;
+        wb = (dom == 94) ? WALL_WIDTH : WALL_WIDTH * d->wall_width_asym_b;
+        for (j = 0; j < NGRIDY + 1; j++) {
+            y1 = YMIN + ((double)j) * dy;
+            if (o_abs(y - y1) < WALL_WIDTH) return 1;
+            for (i = -1; i < NGRIDX; i++) {
+                x1 = ((dom == 947) ? -LAMBDA : XMIN) + ((double)i + 0.5) * dx;
+                if (j % 2 == 0) x1 += 0.5 * dx;
+                if (o_mod(x - x1, y - y1) < MU) return 1;
+                if (o_abs(nx * (x - x1) + ny * (y - y1)) < wa) return 1;
+                if (o_abs(-nx * (x - x1) + ny * (y - y1)) < wb) return 1;
+            }
+        }
+        return 0;
+    case 98:                                                                         /* 7368 */
+        if (x > 0.0) return x * x + y * y < LAMBDA * LAMBDA;
+        y1 = y - 0.5 * WALL_WIDTH;
+        r = LAMBDA - 0.5 * WALL_WIDTH;
+        if (x * x + y1 * y1 < r * r) return 1;
+        if ((y > -LAMBDA) && (y < -LAMBDA + WALL_WIDTH)) return 1;
+        if ((y > -LAMBDA - 0.5 * WALL_WIDTH) && (y < -LAMBDA + 1.5 * WALL_WIDTH)) return 2;
+        return 0;
+    case 981: case 982: case 983:                                                    /* 7378, 7387, 7396 */
+        if (dom == 981 ? (x * x + y * y < LAMBDA * LAMBDA) : (x * x / (LAMBDA * LAMBDA) + y * y < 1.0)) return 1;
+        if (x > 0.0) return 0;
+        if (dom == 981) y1 = y + LAMBDA - MU - 0.5 * WALL_WIDTH;
+        else if (dom == 982) y1 = y + 1.0 - MU - 0.5 * WALL_WIDTH;
+        else y1 = y + 1.0 - MU - 0.5 * WALL_WIDTH - (x - XMIN) * tan(d->apoly * O_PID);
+        if (o_abs(y1) < 0.5 * WALL_WIDTH) return 1;
+        if (o_abs(y1) < 0.75 * WALL_WIDTH) return 2;
+        return 0;
+    case 99: case 991:                                                               /* 7405, 7418 */
+        dx = (XMAX - XMIN) / (double)NGRIDX;
+        for (k = 0; k < NGRIDX; k++) {
+            x1 = XMIN + dx * ((double)k + 0.5);
+            y1 = LAMBDA;
+            if (o_mod(x - x1, y - y1) < MU) return 1;
+            if (dom == 99 ? (o_mod(x - x1, y + y1) < MU_B)
+                          : o_inpoly(x - x1, y + y1, MU_B, NPOLY, 1.0 - 2.0 / (double)NPOLY))
+                return 1;
+            if ((o_abs(y) < LAMBDA) && (o_abs(x - x1) < 0.5 * WALL_WIDTH)) return 1;
+        }
+        return 0;
+    case 21: {                                                                       /* 7475 */
+        const double l2 = LAMBDA * LAMBDA, r2 = x * x + y * y, r2mu = (x - MU) * (x - MU) + y * y;
+        if ((r2mu > l2) && (r2 < 1.0)) return 1;
+        else if (r2mu <= l2) return 2;
+        return 0;
+    }
+    case 22: case 23:                                                                /* 7484, 7500 */
+        if (dom == 22 && ((o_abs(x) >= 1.0) || (o_abs(y) >= 1.0))) return 0;
+        x1 = 0.5 * (x + 1.0);
+        y1 = 0.5 * (y + 1.0);
+        for (k = 0; k < d->mdepth; k++) {
+            x1 = x1 * (double)MRATIO;
+            y1 = y1 * (double)MRATIO;
+            if ((dom == 22 || ((o_abs(x) < 1.0) && (o_abs(y) < 1.0))) && (((int)x1 % MRATIO) == MRATIO / 2) &&
+                (((int)y1 % MRATIO) == MRATIO / 2))
+                return k + 2;
+        }
+        return 1;
+    }
+    return -1;
+}
+
 static int o_domain_at(double x, double y, int dom, const orc_domain_params *d, const double box[4], int nc,
                        const double *circ)
 {
@@ -524,7 +953,7 @@ static int o_domain_at(double x, double y, int dom, const orc_domain_params *d, 
         return 108 * l4 * x * x > t3;
     }
     }
-    return -1;
+    return o_domain_more(x, y, dom, d, box);
 }
 
 int orc_domain(int NX, int NY, const double box[4], int domain, const orc_domain_params *dp, int ncirc,

@@ -367,14 +367,16 @@ class DomainParams(ctypes.Structure):
     """orc_domain_params: the macros the closed-form domains read"""
     _fields_ = [(n, ctypes.c_double) for n in ("lam", "mu", "mu_b", "apoly", "wall_width", "wall_width_b",
                                                 "julia_scale", "julia_x", "julia_y", "mandellimit")] + \
-               [(n, ctypes.c_int) for n in ("npoly", "mdepth", "mratio", "mandellevel", "ngridx", "ngridy")]
+               [(n, ctypes.c_int) for n in ("npoly", "mdepth", "mratio", "mandellevel", "ngridx", "ngridy")] + \
+               [(n, ctypes.c_double) for n in ("wall_width_asym", "wall_width_asym_b", "oscil_ymid")]
 
 
 # the values of the fixture builds' parameter block (oracle/ref_params.h) and
 # julia_x, julia_y of global_pdes.c:567
 DOMAIN_DEFAULTS = dict(lam=1.5, mu=0.1, mu_b=0.08, apoly=0.0, wall_width=0.1, wall_width_b=0.1, julia_scale=0.8,
                        julia_x=0.37468, julia_y=0.21115, mandellimit=10.0, npoly=6, mdepth=4, mratio=3,
-                       mandellevel=100, ngridx=10, ngridy=10)
+                       mandellevel=100, ngridx=10, ngridy=10, wall_width_asym=0.75, wall_width_asym_b=0.75,
+                       oscil_ymid=0.0)
 
 
 def domain_params(**kw):

@@ -61,7 +61,9 @@
 #define OSCILLATION_SCHEDULE 0
 #endif
 #define OSCIL_YMAX 1.2
+#ifndef OSCIL_YMID
 #define OSCIL_YMID 0.0
+#endif
 #define OSCIL_LEFT_YSHIFT 0.0
 #define INITIAL_SHIFT 20.0
 #define WAVE_PACKET_SHIFT 200.0
@@ -218,8 +220,12 @@
 #ifndef WALL_WIDTH_B
 #define WALL_WIDTH_B 0.1
 #endif
+#ifndef WALL_WIDTH_ASYM
 #define WALL_WIDTH_ASYM 0.75
+#endif
+#ifndef WALL_WIDTH_ASYM_B
 #define WALL_WIDTH_ASYM_B 0.75
+#endif
 #define PDISC_FACTOR 3.0
 #define P_PERCOL 0.25
 #define NPOISSON 300

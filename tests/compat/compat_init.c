@@ -57,6 +57,8 @@
 #define NPOLY 6
 #define WALL_WIDTH 0.1
 #define WALL_WIDTH_B 0.1
+#define WALL_WIDTH_ASYM 0.75
+#define WALL_WIDTH_ASYM_B 0.75
 #define JULIA_SCALE 0.8
 #define MDEPTH 4
 #define MRATIO 3

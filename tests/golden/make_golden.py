@@ -29,7 +29,8 @@ Outputs (committed):
                     gen_inputs.mask_cases() build (python make_golden.py masks
                     writes this file alone)
   domain_cases.npz  every closed-form domain of xy_in_billiard_single_domain
-                    (gen_inputs.DOMAIN_CODES) on the dom_a and dom_b builds
+                    (gen_inputs.DOMAIN_CODES) on the dom_a, dom_b and dom_c
+                    builds
                     (python make_golden.py domains writes this file alone)
   ior_cases.npz     init_ior_2d's per-cell media models (gen_inputs.IOR_CODES)
                     and its constant branch on gen_inputs.ior_mask(); the
@@ -59,7 +60,7 @@ sys.path.insert(0, os.path.join(ROOT, "youtube-simulations_amd"))
 sys.path.insert(0, HERE)
 
 import pyoracle  # noqa: E402
-from gen_inputs import (COLOR_FRAMES, COMPARISON_BUILDS, DOMAIN_CODES, EXT_STEPS, IOR_CODES, PULSE_INIT, PULSES, ROT_ANGLES,  # noqa: E402
+from gen_inputs import (COLOR_FRAMES, COMPARISON_BUILDS, DOMAIN_CODES, DOMAIN_FIRST_CODES, EXT_STEPS, IOR_CODES, PULSE_INIT, PULSES, ROT_ANGLES,  # noqa: E402
                         TABLE_STEPS, case_inputs, cmp_cases, color_cases, color_inputs, domain_sets, ext_cases,
                         ext_inputs, ior_mask, mask_cases, mask_circles, mask_circles_b, small_cases, table_cases, table_inputs)
 from wavehip import workloads  # noqa: E402
@@ -95,15 +96,32 @@ def masks():
     np.savez_compressed(os.path.join(HERE, "mask_cases.npz"), **out)
 
 
+def fresh_mask(tag, dom):
+    """one domain's mask from a build loaded in a new process, whose static
+    one-time constants are still unset"""
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        f = os.path.join(tmp, "m.npy")
+        code = ("import sys, numpy as np; sys.path.insert(0, %r); import pyoracle; "
+                "np.save(%r, pyoracle.RefBuild(%r).mask(%d, None))" % (os.path.join(ROOT, "oracle"), f, tag, dom))
+        subprocess.run([sys.executable, "-c", code], check=True)
+        return np.load(f)
+
+
 def domains():
-    """every closed-form domain on the dom_a / dom_b builds -> domain_cases.npz"""
+    """every closed-form domain on the dom_a / dom_b / dom_c builds ->
+    domain_cases.npz"""
     out = {}
     circ = mask_circles()
     for tag, (nx, ny, kw) in domain_sets().items():
         R = pyoracle.RefBuild(tag)
         assert (R.nx, R.ny) == (nx, ny)
         for dom in DOMAIN_CODES:
-            out["%s/%d" % (tag, dom)] = R.mask(dom, circ if dom in (20, 201, 202) else None)
+            if dom in DOMAIN_FIRST_CODES:
+                out["%s/%d" % (tag, dom)] = fresh_mask(tag, dom)
+            else:
+                out["%s/%d" % (tag, dom)] = R.mask(dom, circ if dom in (20, 201, 202) else None)
     # xy_in_billiard with COMPARISON: the reference's own function on two builds
     for tag in COMPARISON_BUILDS:
         out[tag + "/mask"] = pyoracle.RefBuild(tag).mask_billiard(circ, mask_circles_b())

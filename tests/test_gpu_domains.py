@@ -87,7 +87,10 @@ def test_domain_masks_match_oracle(wh, oracle, shape):
 def test_domain_mask_errors(wh):
     W = wh.Wave(dict(nx=40, ny=30))
     try:
-        for dom, kw in ((53, {}), (8, dict(npoly=0)), (15, dict(mratio=3, mdepth=25)), (25, dict(julia_scale=0.0))):
+        for dom, kw in ((53, {}), (8, dict(npoly=0)), (15, dict(mratio=3, mdepth=25)), (25, dict(julia_scale=0.0)),
+                        (93, dict(ngridx=0)), (947, dict(ngridy=0)), (991, dict(ngridx=0)), (43, dict(mu=0.0)),
+                        (46, dict(lam=1e-5)), (55, dict(lam=0.0)), (794, dict(julia_scale=0.0)),
+                        (22, dict(mratio=3, mdepth=25)), (59, dict(npoly=0))):
             with pytest.raises(wh.WaveError):
                 W.build_domain_mask(dom, MASK_BOX, **kw)
         with pytest.raises(wh.WaveError):

@@ -2772,7 +2772,11 @@ static bool domain_known(int d)
     static const int codes[] = {999, 0,   1,   196, 199, 198,  197,  2,   3,   4,   5,   6,   7,   8,   9,
                                 10,  11,  12,  13,  15,  16,   17,   18,  19,  191, 195, 1951, 1952, 20,
                                 201, 202, 24,  25,  26,  31,   32,   33,  34,  50,  51,  522, 523, 524, 525,
-                                61,  62,  63,  64,  67,  68,   69,   73,  74,  76,  79,  791, 795, 796, 90, 91};
+                                61,  62,  63,  64,  67,  68,   69,   73,  74,  76,  79,  791, 795, 796, 90, 91,
+                                35,  43,  45,  44,  441, 46,   461,  47,  48,  49,  52,  521, 526, 55,  56,
+                                57,  58,  581, 59,  65,  66,   731,  75,  751, 78,  781, 792, 793, 794, 797,
+                                798, 93,  934, 935, 936, 937,  94,   942, 947, 98,  981, 982, 983, 99,  991,
+                                21,  22,  23};
     for (int c : codes)
         if (c == d) return true;
     return false;
@@ -2787,9 +2791,27 @@ static int domain_setup(Engine *e, const Grid &G, int domain, const double *box,
         return fail(e, WV_EUNSUPPORTED, "domain %d is not among the closed-form domains provided", domain);
     const bool circles = domain == 20 || domain == 201 || domain == 202;
     if (circles && (ncirc < 0 || (ncirc > 0 && !circ))) return fail(e, WV_EINVAL, "circle list");
-    const bool poly = domain == 8 || domain == 32 || domain == 791;
+    const bool poly = domain == 8 || domain == 32 || domain == 791 || domain == 59 || domain == 792 ||
+                      domain == 794 || domain == 797 || domain == 798 || domain == 991;
     if (poly && (p->npoly < 1 || p->npoly > 4096)) return fail(e, WV_EINVAL, "NPOLY %d: 1..4096", p->npoly);
-    if (domain == 15 || domain == 17) {
+    const bool lattice = domain == 93 || domain == 934 || domain == 935 || domain == 936 || domain == 937 ||
+                         domain == 94 || domain == 942 || domain == 947;
+    if (lattice && (p->ngridx < 1 || p->ngridy < 1 || (double)(p->ngridx + 2) * (p->ngridy + 1) > 1e6))
+        return fail(e, WV_EINVAL, "NGRIDX %d, NGRIDY %d: both >= 1, at most 1e6 disks", p->ngridx, p->ngridy);
+    if ((domain == 99 || domain == 991) && (p->ngridx < 1 || p->ngridx > 1000000))
+        return fail(e, WV_EINVAL, "NGRIDX %d: 1..1e6", p->ngridx);
+    // the loops and integer conversions of the periodic domains: bounded by
+    // the box over LAMBDA (or LAMBDA over MU), and int-sized
+    const double ext = std::max(std::max(std::fabs(box[0]), std::fabs(box[1])),
+                                std::max(std::fabs(box[2]), std::fabs(box[3]))) + 2.0;
+    if ((domain == 43 || domain == 45) && (!(p->mu > 0) || !(std::fabs(p->lambda) / p->mu <= 1e6)))
+        return fail(e, WV_EINVAL, "D_FRESNEL: MU > 0 and |LAMBDA|/MU <= 1e6");
+    if ((domain == 44 || domain == 441 || domain == 48 || domain == 55 || domain == 56 || domain == 57) &&
+        (!(p->lambda > 0) || !(ext / p->lambda <= 1e6)))
+        return fail(e, WV_EINVAL, "domain %d: LAMBDA > 0 and the box at most 1e6 LAMBDA", domain);
+    if ((domain == 46 || domain == 461) && (!(p->lambda > 0) || !(ext / p->lambda < 46000.0)))
+        return fail(e, WV_EINVAL, "D_QRD: LAMBDA > 0 and (box/LAMBDA)^2 within an int");
+    if (domain == 15 || domain == 17 || domain == 22 || domain == 23) {
         // the digits (int)x1 % MRATIO of MRATIO^MDEPTH must fit an int
         if (p->mratio < 1 || p->mdepth < 0 || p->mdepth > 30 ||
             std::pow((double)p->mratio, (double)p->mdepth) >= 2147483647.0)
@@ -2798,7 +2820,8 @@ static int domain_setup(Engine *e, const Grid &G, int domain, const double *box,
     }
     if ((domain == 16 || domain == 24 || domain == 25 || domain == 26) && (p->mandellevel < 0))
         return fail(e, WV_EINVAL, "MANDELLEVEL %d", p->mandellevel);
-    if ((domain == 16 || domain == 25) && p->julia_scale == 0.0) return fail(e, WV_EINVAL, "JULIA_SCALE 0");
+    if ((domain == 16 || domain == 25 || domain == 794) && p->julia_scale == 0.0)
+        return fail(e, WV_EINVAL, "JULIA_SCALE 0");
     if (domain == 10) {
         // one term per line of the grating: bounded by the box over LAMBDA
         if (!(p->lambda > 0) || (std::fabs(box[2]) + std::fabs(box[3])) / p->lambda > 1e6)
@@ -2824,6 +2847,57 @@ static int domain_setup(Engine *e, const Grid &G, int domain, const double *box,
             trig.push_back(std::cos(angle));
             trig.push_back(std::sin(angle));
         }
+    } else if (domain == 792 || domain == 794) {  // sub_wave.c:6995-7000, 7035
+        for (int k = 0; k < p->npoly; k++) {
+            const double angle = p->apoly * REF_PID + (double)k * REF_DPI / (double)p->npoly;
+            trig.push_back(std::cos(angle));
+            trig.push_back(std::sin(angle));
+        }
+        a.c0 = std::cos(-0.5 * REF_PID);  // D_POLYCIRCLES_ANGLED's cb, sb (6995-6997)
+        a.c1 = std::sin(-0.5 * REF_PID);
+    } else if (domain == 797 || domain == 798 || domain == 991) {  // in_polygon 362-376, 7071-7078
+        const double apoly = domain == 991 ? 1.0 - 2.0 / (double)p->npoly : p->apoly;
+        const double omega = REF_DPI / ((double)p->npoly);
+        for (int k = 0; k < p->npoly; k++) {
+            const double angle = apoly * REF_PID + ((double)k + 0.5) * omega;
+            trig.push_back(std::cos(angle));
+            trig.push_back(std::sin(angle));
+        }
+        a.c0 = std::cos(omega * 0.5);
+    } else if (domain == 59) {  // sub_wave.c:6685-6694
+        const double salpha = REF_DPI / (double)p->npoly, h = L * std::tan(REF_PI / (double)p->npoly);
+        a.c0 = h < M ? std::sqrt(M * M - h * h) : 0.0;
+        for (int i = 0; i < p->npoly; i++) {
+            trig.push_back(std::cos((double)i * salpha + p->apoly * REF_PID));
+            trig.push_back(std::sin((double)i * salpha + p->apoly * REF_PID));
+        }
+    } else if (domain == 56) {  // sub_wave.c:6633-6640
+        a.c0 = L / (2.0 * std::sqrt(3.0));
+        a.c1 = a.c0 * 3.0;
+        a.c2 = std::sin(REF_DPI / 3.0);
+        trig.push_back(std::sin(-REF_DPI / 3.0));
+    } else if (domain == 57) {  // sub_wave.c:6649-6650
+        a.c0 = -1.0 / std::sqrt(3.0);
+        a.c1 = -2.0 * a.c0;
+    } else if (domain == 731) {  // sub_wave.c:6835
+        a.c0 = std::cos(REF_PI / 6.0);
+    } else if (domain == 78 || domain == 781) {  // sub_wave.c:6936-6941
+        a.c0 = std::sqrt(2.0);
+        for (int k = 0; k < 8; k++) {
+            trig.push_back(std::cos((double)k * 0.5 * REF_PID));
+            trig.push_back(std::sin((double)k * 0.5 * REF_PID));
+        }
+    } else if (domain == 983) {  // sub_wave.c:7400
+        a.c0 = std::tan(p->apoly * REF_PID);
+    } else if (domain == 751) {  // rc_hyp's one-time constants, sub_wave.c:5444-5472
+        const double m = L, d = 2.5, b = 3.0;
+        const double f = m * d + b, f1 = f / m, dprime = f1 - d, f2 = m * dprime / (m + 1);
+        const double r1 = 2.0 * f / m, r2 = 2.0 * b / (m - 1.0);
+        const double e1 = std::sqrt(1.0 + 2.0 * b / (m * m * m * d));
+        const double e2 = std::sqrt(1.0 + 2.0 * (m * (2.0 * m - 1.0) + b / d) / ((m - 1) * (m - 1) * (m - 1)));
+        const double a1 = f1 / e1, a2 = f2 / e2;
+        for (double v : {a1, std::sqrt(a1 * r1), 0.5 * d + a1, a2, std::sqrt(a2 * r2), -0.5 * d + a2})
+            trig.push_back(v);
     } else if (domain == 67) {  // sub_wave.c:6751-6752
         a.c0 = std::cos(p->apoly * REF_PID);
         a.c1 = std::sin(p->apoly * REF_PID);
@@ -2850,6 +2924,8 @@ static int domain_setup(Engine *e, const Grid &G, int domain, const double *box,
     a.lambda = L; a.mu = M; a.mu_b = p->mu_b; a.wall_width = p->wall_width; a.wall_width_b = p->wall_width_b;
     a.julia_scale = p->julia_scale; a.julia_x = p->julia_x; a.julia_y = p->julia_y;
     a.mandellimit = p->mandellimit;
+    a.wall_width_asym = p->wall_width_asym; a.wall_width_asym_b = p->wall_width_asym_b;
+    a.oscil_ymid = p->oscil_ymid;
     return WV_OK;
 }
 

@@ -126,6 +126,8 @@ __global__ void __launch_bounds__(256) k_mask(WvMask a)
 
 // ---- every closed-form domain of xy_in_billiard_single_domain ---------------
 
+constexpr double REF_DPI = 6.283185307;  // DPI, global_pdes.c:9
+
 // vabs and module2, sub_wave.c:270-285
 __device__ inline double vabs_(double x) { return x < 0.0 ? -x : x; }
 __device__ inline double module2_(double x, double y) { return sqrt(x * x + y * y); }
@@ -164,6 +166,394 @@ __device__ inline bool in_circles(const WvDomain &a, double x, double y)
         if ((x - x1) * (x - x1) + (y - y1) * (y - y1) < r * r) return true;
     }
     return false;
+}
+
+// The cases of xy_in_billiard_single_domain beyond the families of
+// domain_value, in the same manner.  The reference's one-time constants (its
+// static `first` blocks, sub_wave.c:6631-6636, 6647-6652, 6685-6690, and
+// rc_hyp's, 5444-5472) and every cosine, sine and tangent of a fixed angle
+// come from the host in c0..c2 and the table.  The Fresnel zone plates
+// compare a cosine of the cell's own argument with 0: any faithfully rounded
+// cosine has the true value's sign there, the GPU's and the C library's alike.
+__device__ short domain_value_more(const WvDomain &a, double x, double y)
+{
+    const double L = a.lambda, M = a.mu, MB = a.mu_b, WW = a.wall_width;
+    const auto T = gc(a.trig);
+    switch (a.domain) {
+    case 35: {                                                            // D_TOKARSKY 6068
+        const double x1 = 4.0 + x / (a.xmax - a.xmin) * 8.4, y1 = 2.0 + y / (a.xmax - a.xmin) * 8.4;
+        if (x1 <= 0.0 || x1 >= 8.0) return 0;
+        if (x1 < 1.0) return y1 > 2.0 && y1 < x1 + 2.0;
+        if (x1 < 2.0) return y1 > 1.0 && y1 < 4.0;
+        if (x1 < 3.0) return y1 > x1 - 2.0 && y1 < 3.0;
+        if (x1 < 4.0) return y1 > 1.0 && y1 < 2.0;
+        if (x1 < 5.0) return y1 > x1 - 4.0 && y1 < 2.0;
+        if (x1 < 6.0) return y1 > 1.0 && y1 < 3.0;
+        if (x1 < 7.0) return y1 > x1 - 6.0 && y1 < 10.0 - x1;
+        return y1 > 2.0 && y1 < 3.0;
+    }
+    case 43: {                                                            // D_FRESNEL 6288
+        if (vabs_(y) > 0.9 * vabs_(L)) return 1;
+        if (vabs_(x) > M) return 1;
+        double x1 = sqrt(L * L - y * y) - vabs_(L);
+        while (x1 <= 0.0) x1 += M;
+        if (L > 0.0) return x < x1 ? 0 : 1;
+        return x > -x1 ? 0 : 1;
+    }
+    case 45: {                                                            // D_DOUBLE_FRESNEL 6307
+        if (vabs_(y) > 0.9 * vabs_(L)) return 1;
+        if (L > 0.0) {
+            if (vabs_(x) > L + M) return 1;
+            double x1 = sqrt(L * L - y * y) - L;
+            while (x1 <= 0.0) x1 += M;
+            x1 -= L;
+            return vabs_(x) > -x1 ? 0 : 1;
+        }
+        if (vabs_(x) < -L - M) return 1;
+        double x1 = sqrt(L * L - y * y) + L;
+        while (x1 <= 0.0) x1 += M;
+        x1 -= L;
+        return vabs_(x) > x1 ? 1 : 0;
+    }
+    case 44: case 441: {                                                  // D_NOISEPANEL 6331, _RECT 6339
+        double x1 = a.domain == 44 ? vabs_(x) : -x;
+        if (a.domain == 441 && !(x1 > 0.02)) {                           // NPWIDTH, global_pdes.c:380
+            if (x > 0.02) return vabs_(y) < a.ymax - 0.1 && x < a.xmax - 0.1;
+            return 0;
+        }
+        while (x1 > 2.0 * L) x1 -= 2.0 * L;
+        const double y1 = x1 <= L ? 0.1 + M * x1 / L : 0.1 + 2.0 * M - M * x1 / L;
+        if (a.domain == 44) return y > a.ymin + y1 && y < a.ymax - y1;
+        return y > a.ymin + y1 && y < a.ymax - y1 && x > a.xmin + 0.1;
+    }
+    case 46: case 461: {                                                  // D_QRD 6355, D_QRD_ASYM 6363
+        const bool upper = a.domain == 46 || y > 0.0;
+        const int p = upper ? 13 : 17;
+        const double x1 = (upper ? vabs_(x) : vabs_(x + 1.0)) / L;
+        const int k = (int)(x1 + 0.5), k1 = (k * k) % p;
+        const double y1 = (M / (double)p) * ((double)(p + 1) - (double)k1);
+        if (a.domain == 46) return y > a.ymin + y1 && y < a.ymax - y1;
+        return upper ? y < a.ymax - y1 : y > a.ymin + y1;
+    }
+    case 47: {                                                            // D_CIRCLE_SEGMENT 6382
+        if (vabs_(y) > 0.9 * vabs_(L)) return 1;
+        const double y1 = 0.9 * L;
+        double x1 = sqrt(L * L - y1 * y1) - vabs_(L) + M;
+        if (L > 0.0 && x < x1) return 1;
+        else if (L < 0.0 && x > -x1) return 1;
+        x1 = sqrt(L * L - y * y) - vabs_(L) + M;
+        if (L > 0.0) return x < x1 ? 0 : 1;
+        return x > -x1 ? 0 : 1;
+    }
+    case 48: {                                                            // D_GROOVE 6403
+        const double s = 0.85 * L, h = 0.5 * L;
+        const double x1 = x - a.xmin - (double)((int)((x - a.xmin) / L)) * L;
+        if (x1 < h) return y > a.ymin + L;
+        return y > a.ymin + L + s;
+    }
+    case 49: return vabs_(x - y * L / a.ymax) > 0.5 * M;                  // D_FABRY_PEROT 6411
+    case 52: {                                                            // D_WAVEGUIDE_W 6437
+        const double x1 = vabs_(x), width = L - 2.0 * M, height = 0.5 * M;
+        if (x1 > 2.0 * L - M) return 0;
+        if (vabs_(y) > M + width + height) return 0;
+        if (y >= height) {
+            const double r = module2_(x1, y - height);
+            if (r > M && r < M + width) return 1;
+            return x1 > L + M;
+        }
+        if (y <= -height) {
+            const double r = module2_(x1 - L, y + height);
+            return r > M && r < M + width;
+        }
+        if (x1 > L + M) return 1;
+        return x1 > M && x1 < M + width;
+    }
+    case 521: {                                                           // D_WAVEGUIDES_W 6461
+        const double x1 = vabs_(x);
+        double width = L - 2.0 * M, height = M + width, r;                // upper fibre
+        if (y > M + width + height) return 0;
+        if (y >= height) {
+            r = module2_(x1, y - height);
+            if (r > M && r < M + width) return 1;
+            return y < height + M && x1 > L + M && x1 < L + M + width;
+        }
+        r = module2_(x1 - L, y - height);
+        if (r > M && r < M + width) return 1;
+        height = -M - width;                                              // lower fibre
+        width = L - 2.0 * MB;
+        if (y < -MB - width + height) return 0;
+        if (y >= height) {
+            r = module2_(x1, y - height);
+            if (r > MB && r < MB + width) return 1;
+            return y < height + MB && x1 > L + MB && x1 < L + MB + width;
+        }
+        r = module2_(x1 - L, y - height);
+        return r > MB && r < MB + width;
+    }
+    case 526: {                                                           // D_WAVEGUIDE_BADSPLICE 6575
+        const double y1 = x < 0.0 ? vabs_(vabs_(y) - 0.5) : (y > 0.0 ? vabs_(y - 0.5 - M) : vabs_(y + 0.5 - MB));
+        if (y1 < 0.9 * L) return 1;
+        return y1 < L ? 2 : 0;
+    }
+    case 55: {                                                            // D_CHESSBOARD 6622
+        const int i = (int)(vabs_(x) / L + 0.5), j = (int)(vabs_(y) / L + 0.5);
+        return (i + j) % 2 == 0;
+    }
+    case 56: {                                                            // D_TRIANGLE_TILES 6629
+        // c0 = h, c1 = hh, c2 = sin(DPI/3), T[0] = sin(-DPI/3)
+        const double h = a.c0, hh = a.c1;
+        const int i = (int)((y + h) / hh + 10.0);
+        double y1 = a.c2 * x - 0.5 * y;
+        const int j = (int)((y1 + h) / hh + 10.0);
+        y1 = T[0] * x - 0.5 * y;
+        const int k = (int)((y1 + h) / hh + 10.0);
+        return (i + j + k) % 2 == 0;
+    }
+    case 57: {                                                            // D_HEX_TILES 6645; c0 = ra, c1 = rb
+        double x1 = (x + a.c0 * y) / L + 30.0, y1 = a.c1 * y / L + 30.0;
+        x1 = x1 - (double)(3 * (int)(x1 / 3.0));
+        y1 = y1 - (double)(3 * (int)(y1 / 3.0));
+        if (x1 > 2.0 && y1 < 1.0) return 1;
+        if (x1 < 1.0 && y1 > 2.0) return 1;
+        if (x1 + y1 < 1.0) return 1;
+        return x1 + y1 > 5.0;
+    }
+    case 58: {                                                            // D_FUNNELS 6665
+        double y1 = y;
+        if (y > 0.5 * a.ymax) y1 -= a.ymax;
+        if (y < -0.5 * a.ymax) y1 += a.ymax;
+        y1 = vabs_(y1 - M * x);
+        y1 = vabs_(y1 - 0.5 * a.ymax) * 2.0 / a.ymax;
+        return !(y1 > 0.25 * (1.0 + L + x * x));
+    }
+    case 581: return !(vabs_(y) > M + L * x * x);                        // D_ONE_FUNNEL 6675
+    case 59: {                                                            // D_LENSES_RING 6681; c0 = ll
+        for (int i = 0; i < a.npoly; i++) {
+            const double ca = T[2 * i], sa = T[2 * i + 1];
+            const double x1 = x * ca + y * sa, y1 = -x * sa + y * ca;
+            if (module2_(x1 - L - a.c0, y1) < M && module2_(x1 - L + a.c0, y1) < M) return 0;
+        }
+        return 1;
+    }
+    case 65:                                                              // D_FRESNEL_ZONE_PLATE 6736
+        if (vabs_(x + L) >= M) return 1;
+        return cos(REF_DPI * (6.25 * y * y + 0.125)) < 0.0;
+    case 66:                                                              // D_FRESNEL_ZONE_PLATE_INV 6743
+        if (vabs_(x + L) >= M) return 1;
+        return cos(REF_DPI * 6.25 * y * y) > 0.0;
+    case 731: {                                                           // D_TREE_BEAM 6828; c0 = cos(PI/6)
+        const double x1 = vabs_(x), lev[3] = {1.5, 0.25, -1.5}, rad[3] = {1.5, 2.0, 3.0};
+        for (int k = 0; k < 3; k++) {
+            const double r = rad[k] * L;
+            if (in_triangle(x1, y - lev[k] * L, r * a.c0, -0.5 * r, 0.0, r, 0.0, -0.5 * r)) return 1;
+        }
+        const double ym = a.oscil_ymid;
+        if (x < 0.0 && y > ym - 0.5 * WW && y < ym + 0.5 * WW) return 1;
+        if (x < -((3.0 * L) * a.c0) + 0.5 * WW && y > ym - 0.75 * WW && y < ym + 0.75 * WW) return 2;
+        return 0;
+    }
+    case 75: {                                                            // D_RITCHEY_CHRETIEN_SPHERICAL 6892
+        const double d = 2.5, b = 3.5, r1 = 2.0 * (d + b / L), r2 = 6.0 / (L - 1.0);
+        const double x1 = 0.5 * d - r1, x2 = -0.5 * d - r2, y1 = vabs_(y);
+        if (x < 0.0) {
+            if (y1 > 0.3) return 1;
+            if (x < -0.5 * d - WW) return 1;
+            return module2_(x - x2, y1) > r2;
+        }
+        if (y1 < M) return 1;
+        if (x > 0.5 * d + WW) return 1;
+        return module2_(x - x1, y1) < r1;
+    }
+    case 751: {                                                           // D_RITCHEY_CHRETIEN_HYPERBOLIC, rc_hyp 5437
+        // T = a1, b1, x01, a2, b2, x02 (rc_hyp's one-time constants)
+        const double d = 2.5, y1 = vabs_(y);
+        if (x > 0.0) {
+            if (y1 < M) return 1;
+            if (x > 0.5 * d + WW) return 1;
+            return (x - T[2]) * (x - T[2]) > T[0] * T[0] * (1.0 + y * y / (T[1] * T[1]));
+        }
+        if (y1 > 0.3) return 1;
+        return x > T[5] || (x - T[5]) * (x - T[5]) < T[3] * T[3] * (1.0 + y * y / (T[4] * T[4]));
+    }
+    case 78: case 781: {                                                  // D_MAGNETRON 6929, _CATHODE 6947
+        // c0 = sqrt(2); T = cos, sin of k*0.5*PID
+        const double r = module2_(x, y);
+        if (a.domain == 781 && r < WW) return 0;
+        if (r > L) return 1;
+        if (r < 0.33 * L) return 1;
+        if (vabs_(y) < WW && x > -0.66 * L) return 1;
+        if (vabs_(x) < WW && vabs_(y) < 0.66 * L) return 1;
+        if (vabs_(x - y) < WW * a.c0 && r < 0.66 * L) return 1;
+        if (vabs_(x + y) < WW * a.c0 && r < 0.66 * L) return 1;
+        for (int k = 0; k < 8; k++)
+            if (module2_(x - 0.66 * T[2 * k], y - 0.66 * T[2 * k + 1]) < M) return 1;
+        return 0;
+    }
+    case 792: {                                                           // D_POLYCIRCLES_ANGLED 6992
+        // c0 = cb, c1 = sb; T = cos, sin of the NPOLY angles
+        if (module2_(x, y) < L) return 1;
+        const double cb = a.c0, sb = a.c1;
+        for (int k = 0; k < a.npoly; k++) {
+            const double ca = T[2 * k], sa = T[2 * k + 1];
+            const double x1 = x * ca + y * sa, y1 = -x * sa + y * ca;
+            const double x2 = L + 2.0 * M * cb, y2 = 2.0 * M * sb;
+            if (module2_(x1 - x2, y1 - y2) < M) return 1;
+            const double c1 = -sb * L + cb * WW, c2 = -sb * L - cb * WW, b = -sb * x1 + cb * y1;
+            if (x1 > 0.5 * L && x1 < L + M && b < c1 && b > c2) return 1;
+        }
+        return 0;
+    }
+    case 793: {                                                           // D_TWOCIRCLES_ELLIPSE 7015
+        const double aa = 0.9 * a.xmax - 3.0 * M, b = aa / L;
+        if (x * x / (aa * aa) + y * y / (b * b) < 1.0) return 1;
+        const double x1 = aa + 2.0 * M;
+        if ((x - x1) * (x - x1) + y * y < M * M) return 1;
+        if ((x + x1) * (x + x1) + y * y < M * M) return 1;
+        return vabs_(x) < aa + 2.0 * M && vabs_(y) < WW;
+    }
+    case 794: {                                                           // D_CIRCLES_ELLIPSE 7026; T = cos, sin
+        const double aa = 0.9 * a.xmax - 3.0 * M, b = aa / L;
+        x = x / a.julia_scale;
+        y = y / a.julia_scale;
+        if (x * x / (aa * aa) + y * y / (b * b) < 1.0) return 1;
+        for (int k = 0; k < a.npoly; k++) {
+            const double cp = T[2 * k], sp = T[2 * k + 1];
+            const double x1 = aa * cp, y1 = b * sp;
+            double nx = b * cp, ny = aa * sp;
+            double r = 1.0 / module2_(nx, ny);
+            nx *= r;
+            ny *= r;
+            const double x2 = x1 + 2.0 * M * nx, y2 = y1 + 2.0 * M * ny;
+            if (module2_(x - x2, y - y2) < M) return 1;
+            r = module2_(x - x1, y - y1);
+            const double h = (x - x1) * nx + (y - y1) * ny;
+            if (h > -M && h < 2.0 * M && r * r - h * h < WW * WW) return 1;
+        }
+        return 0;
+    }
+    case 797: case 798: {                                                 // D_TWO_POLYGONS 7062, D_TWO_POLY_PARABOLAS 7068
+        // c0 = cos(omega/2); T = cos, sin of the side normals (in_polygon, sub_wave.c:362)
+        int cond = 1;
+        if (a.domain == 797) {
+            x = vabs_(x) - L;
+            for (int k = 0; k < a.npoly; k++) cond = cond * (x * T[2 * k] + y * T[2 * k + 1] < M * a.c0);
+            return cond;
+        }
+        if (x > 0.0) x -= L;
+        else x += L;
+        for (int k = 0; k < a.npoly; k++) {
+            const double ca = T[2 * k], sa = T[2 * k + 1];
+            const double x1 = x * ca + y * sa, y1 = -x * sa + y * ca;
+            cond = cond * (x1 < M - 0.25 * y1 * y1 / M);
+        }
+        return cond;
+    }
+    case 93: case 934: case 935: case 936: {                              // D_CIRCLE_LATTICE 7110, _NONISO 7225, _HALF 7243, _HALF_V 7262
+        if (a.domain == 935 && y > 0.0) return 1;
+        if (a.domain == 936 && x < 0.0) return 1;
+        const double wa = a.domain == 93 ? WW : WW * a.wall_width_asym;
+        const double dx = (a.xmax - a.xmin) / (double)a.ngridx, dy = (a.ymax - a.ymin) / (double)a.ngridy;
+        for (int i = 0; i < a.ngridx; i++) {
+            const double x1 = a.xmin + ((double)i + 0.5) * dx;
+            if (vabs_(x - x1) < wa) return 1;
+            for (int j = 0; j < a.ngridy; j++) {
+                const double y1 = a.ymin + ((double)j + 0.5) * dy;
+                if (module2_(x - x1, y - y1) < M) return 1;
+                if (vabs_(y - y1) < WW) return 1;
+            }
+        }
+        return 0;
+    }
+    case 937: {                                                           // D_CIRCLE_LATTICE_STRIP 7281
+        if (x > L) return 1;
+        if (x < -L) return 1;
+        const double dx = 2.0 * L / (double)a.ngridx, dy = (a.ymax - a.ymin) / (double)a.ngridy;
+        for (int i = 0; i < a.ngridx + 1; i++) {
+            const double x1 = -L + ((double)i) * dx;
+            if (vabs_(x - x1) < WW * a.wall_width_asym) return 1;
+            for (int j = 0; j < a.ngridy; j++) {
+                const double y1 = a.ymin + ((double)j + 0.5) * dy;
+                if (module2_(x - x1, y - y1) < M) return 1;
+                if (vabs_(y - y1) < WW) return 1;
+            }
+        }
+        return 0;
+    }
+    case 94: case 942: case 947: {                                        // D_CIRCLE_LATTICE_HEX 7146, _HEX_NONISO 7321, _HEX_STRIP 7344
+        const bool strip = a.domain == 947;
+        if (strip && vabs_(x) > L) return 1;
+        const double dx = strip ? 2.0 * L / (double)a.ngridx : (a.xmax - a.xmin) / (double)a.ngridx;
+        const double dy = (a.ymax - a.ymin) / (double)a.ngridy, r = module2_(0.5 * dx, dy);
+        const double nx = -dy / r, ny = dx / (2.0 * r), x0 = strip ? -L : a.xmin;
+        const double wa = a.domain == 94 ? WW : WW * a.wall_width_asym;
+        const double wb = a.domain == 94 ? WW : WW * a.wall_width_asym_b;
+        for (int j = 0; j < a.ngridy + 1; j++) {
+            const double y1 = a.ymin + ((double)j) * dy;
+            if (vabs_(y - y1) < WW) return 1;
+            for (int i = -1; i < a.ngridx; i++) {
+                double x1 = x0 + ((double)i + 0.5) * dx;
+                if (j % 2 == 0) x1 += 0.5 * dx;
+                if (module2_(x - x1, y - y1) < M) return 1;
+                if (vabs_(nx * (x - x1) + ny * (y - y1)) < wa) return 1;
+                if (vabs_(-nx * (x - x1) + ny * (y - y1)) < wb) return 1;
+            }
+        }
+        return 0;
+    }
+    case 98: {                                                            // D_DISC_WAVEGUIDE 7368
+        if (x > 0.0) return x * x + y * y < L * L;
+        const double y1 = y - 0.5 * WW, r = L - 0.5 * WW;
+        if (x * x + y1 * y1 < r * r) return 1;
+        if (y > -L && y < -L + WW) return 1;
+        if (y > -L - 0.5 * WW && y < -L + 1.5 * WW) return 2;
+        return 0;
+    }
+    case 981: case 982: case 983: {                                       // D_DISC_WAVEGUIDE_SHIFTED 7378, _ELLIPSE 7387, D_WAVEGUIDE_ELLIPSE_OBLIQUE 7396
+        if (a.domain == 981 ? x * x + y * y < L * L : x * x / (L * L) + y * y < 1.0) return 1;
+        if (x > 0.0) return 0;
+        double y1;
+        if (a.domain == 981) y1 = y + L - M - 0.5 * WW;
+        else if (a.domain == 982) y1 = y + 1.0 - M - 0.5 * WW;
+        else y1 = y + 1.0 - M - 0.5 * WW - (x - a.xmin) * a.c0;          // c0 = tan(APOLY*PID)
+        if (vabs_(y1) < 0.5 * WW) return 1;
+        return vabs_(y1) < 0.75 * WW ? 2 : 0;
+    }
+    case 99: case 991: {                                                  // D_CIRCLE_PAIRS 7405, D_CIRCLE_POLYGON 7418
+        // 991: c0 = cos(omega/2); T = cos, sin of the side normals
+        const double dx = (a.xmax - a.xmin) / (double)a.ngridx;
+        for (int k = 0; k < a.ngridx; k++) {
+            const double x1 = a.xmin + dx * ((double)k + 0.5), y1 = L;
+            if (module2_(x - x1, y - y1) < M) return 1;
+            if (a.domain == 99) {
+                if (module2_(x - x1, y + y1) < MB) return 1;
+            } else {
+                int cond = 1;
+                for (int q = 0; q < a.npoly; q++) cond = cond * ((x - x1) * T[2 * q] + (y + y1) * T[2 * q + 1] < MB * a.c0);
+                if (cond) return 1;
+            }
+            if (vabs_(y) < L && vabs_(x - x1) < 0.5 * WW) return 1;
+        }
+        return 0;
+    }
+    case 21: {                                                            // D_ANNULUS_HEATED 7475
+        const double l2 = L * L, r2 = x * x + y * y, r2mu = (x - M) * (x - M) + y * y;
+        if (r2mu > l2 && r2 < 1.0) return 1;
+        return r2mu <= l2 ? 2 : 0;
+    }
+    case 22: case 23: {                                                   // D_MENGER_HEATED 7484, D_MENGER_H_OPEN 7500
+        const bool inside = vabs_(x) < 1.0 && vabs_(y) < 1.0;
+        if (a.domain == 22 && !inside) return 0;
+        const int m = a.mratio;
+        double x1 = 0.5 * (x + 1.0), y1 = 0.5 * (y + 1.0);
+        for (int k = 0; k < a.mdepth; k++) {
+            x1 = x1 * (double)m;
+            y1 = y1 * (double)m;
+            if (inside && ((int)x1 % m) == m / 2 && ((int)y1 % m) == m / 2) return (short)(k + 2);
+        }
+        return 1;
+    }
+    default: return 1;  // not reached: the launcher admits only the codes above
+    }
 }
 
 // xy_in_billiard_single_domain(x, y), sub_wave.c:5712-7602, for the
@@ -496,11 +886,14 @@ __device__ short domain_value(const WvDomain &a, double x, double y)
         const double l4 = L * L * L * L, t = x * x + y * y - 4.0 * L * L;
         return 108 * l4 * x * x > t * t * t;
     }
-    default: return 1;  // not reached: the launcher admits only the codes above
+    default: return domain_value_more(a, x, y);
     }
 }
 
-__global__ void __launch_bounds__(256) k_domain(WvDomain a)
+// 8 waves a SIMD (64 VGPRs): the escape loops of the fractal domains hold
+// their wave for up to MANDELLEVEL iterations, so occupancy hides their
+// latency; the cases of domain_value_more alone would ask for 67
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_domain(WvDomain a)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x, i = a.i0 + (int)blockIdx.y;
     if (j >= a.ny || i >= a.nx || i >= a.i0 + a.rows) return;
