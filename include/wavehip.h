@@ -251,6 +251,37 @@ int wv_build_comparison_mask(wv_ctx *c, int domain, int domain_b, const double b
                              int ncircles, const double *circles, int ncircles_b, const double *circles_b,
                              short *xy_in);
 
+/* The lists the remaining domains of xy_in_billiard_single_domain read,
+ * which the program's init functions fill (globals of global_pdes.c:533-545
+ * and its circle list): each a count and a flat array holding, for every
+ * field in turn, that many values; a count of 0 leaves the pointer unread. */
+typedef struct wv_domain_lists {
+    int ncircles;  const double *circles;   /* xc, yc, radius, active (circles, ncircles)        */
+    int nrects;    const double *rects;     /* x1, y1, x2, y2 (polyrect, npolyrect)              */
+    int nrectrot;  const double *rectrot;   /* x1, y1, x2, y2, width (polyrectrot, npolyrect_rot) */
+    int npolygons; const double *polygons;  /* xc, yc, radius, angle, nsides, active (polygons)  */
+    int npolyline; const double *polyline;  /* x, y (polyline, npolyline)                        */
+} wv_domain_lists;
+
+/* wv_build_domain_mask for the domains that read those lists, and for every
+ * code wv_build_domain_mask takes (the circle list from lists):
+ *   D_POLYGONS 40, D_VONKOCH 41, D_STAR 42, D_STAR_CHANNEL 421, D_MAZE 53,
+ *   D_MAZE_SMALL 531, D_MAZE_CLOSED 54, D_MAZE_CHANNELS 541,
+ *   D_TWO_PARABOLAS_ASYM_GUIDE 192, D_TWO_PARABOLAS_SEMITRANS 193,
+ *   D_TWO_PARABOLAS_LENS 194, D_CIRCLE_LATTICE_RANDOM 931,
+ *   D_CIRCLE_LATTICE_HONEY 95, D_CIRCLE_LATTICE_RHOMBUS 97,
+ *   D_CIRCLE_LATTICE_POISSON 96, D_CIRCLE_LATTICE_SQUARE_CROSS 932,
+ *   D_CIRCLE_LATTICE_MAZE 933, D_CIRCLE_LATTICE_HEX_MAZE 941
+ * (sub_wave.c:5903-7320).  D_POLYGONS reads npolygons polygons, the disc
+ * lattices 96, 932, 933 and 941 every circle of the list (radius MU, the
+ * active flag unread, as the reference reads them).  D_STAR needs NPOLY + 1
+ * vertices, D_STAR_CHANNEL NPOLY + 4, D_VONKOCH MDEPTH <= 15; polygons 1-4096
+ * sides; at most 1,048,576 entries a list.  Sines and cosines of the polygon
+ * sides are taken on the host, as for the regular polygons; masks equal the
+ * reference's cell for cell.  Others return WV_EUNSUPPORTED. */
+int wv_build_list_domain_mask(wv_ctx *c, int domain, const double box[4], const wv_domain_params *p,
+                              const wv_domain_lists *lists, short *xy_in);
+
 /* The macros init_ior_2d (sub_wave.c:12164-13004) reads, by reference name. */
 typedef struct wv_ior_params {
     double courant, courantb;   /* COURANT, COURANTB                          */
@@ -558,6 +589,8 @@ int wv_batch_set_ior_tables(wv_batch *b, int k, const short *xy_in, int ior, con
                             const wv_ior_params *p, double *tcc, double *tc, double *tgamma);
 int wv_batch_build_domain_mask(wv_batch *b, int k, int domain, const double box[4], const wv_domain_params *p,
                                int ncircles, const double *circles, short *xy_in);
+int wv_batch_build_list_domain_mask(wv_batch *b, int k, int domain, const double box[4], const wv_domain_params *p,
+                                    const wv_domain_lists *lists, short *xy_in);
 /* wv_autotune over all grids of the batch (one tiling for every launch) */
 int wv_batch_autotune(wv_batch *b, long steps_per_trial, double *best_gcells);
 

@@ -391,12 +391,89 @@ static void evolve_wave(double phi[NX * NY], double psi[NX * NY], double tmp[NX 
  * XMIN..YMAX, XYIN_INITIALISED; VARIABLE_IOR,
  * IOR, COURANT, COURANTB, GAMMA, GAMMAB, MANDEL_IOR_SCALE; COMPARISON and
  * B_DOMAIN_B) and its globals (circles, ncircles, circles_b, ncircles_b,
- * julia_x, julia_y).  The random wells (IOR 5, 8, 9) are drawn with the
+ * julia_x, julia_y; for a list-driven B_DOMAIN, the lists it reads:
+ * polyrect, npolyrect, polyrectrot, npolyrect_rot, polygons, polyline,
+ * npolyline).  The random wells (IOR 5, 8, 9) are drawn with the
  * program's own gaussian() and init_circle_config_pattern(), in the
  * reference's order, so the draws are the program's.  Masks equal the
  * reference's; the tables too, up to the GPU's exp, tanh and log where the
  * model uses them. */
 static const double wavehip_box[4] = {XMIN, XMAX, YMIN, YMAX};
+
+/* the lists of a list-driven B_DOMAIN (wv_build_list_domain_mask): only
+ * those the domain reads are taken from the program's globals (polyrect,
+ * polyrectrot, polygons with ncircles of them, polyline; global_pdes.c:
+ * 533-545), as its init functions left them */
+#define WAVEHIP_D_(d) (B_DOMAIN == (d))
+#define WAVEHIP_RECTS (WAVEHIP_D_(53) || WAVEHIP_D_(531) || WAVEHIP_D_(54) || WAVEHIP_D_(541) || WAVEHIP_D_(192) || \
+                       WAVEHIP_D_(193) || WAVEHIP_D_(194) || WAVEHIP_D_(931) || WAVEHIP_D_(932) || WAVEHIP_D_(933) || \
+                       WAVEHIP_D_(941))
+#define WAVEHIP_RECTROT (WAVEHIP_D_(95) || WAVEHIP_D_(96) || WAVEHIP_D_(97) || WAVEHIP_D_(932) || WAVEHIP_D_(933) || \
+                         WAVEHIP_D_(941))
+#define WAVEHIP_POLYGONS WAVEHIP_D_(40)
+#define WAVEHIP_POLYLINE (WAVEHIP_D_(41) || WAVEHIP_D_(42) || WAVEHIP_D_(421))
+#define WAVEHIP_LISTS (WAVEHIP_RECTS || WAVEHIP_RECTROT || WAVEHIP_POLYGONS || WAVEHIP_POLYLINE)
+
+#if WAVEHIP_LISTS && !COMPARISON
+static int wavehip_list_mask(const wv_domain_params *dp, int nc, const double *cl, short *flat)
+{
+    wv_domain_lists L;
+    memset(&L, 0, sizeof L);
+    L.ncircles = nc;
+    L.circles = cl;
+    double *r = NULL, *rr = NULL, *pg = NULL, *pl = NULL;
+#if WAVEHIP_RECTS
+    const int nr = npolyrect;
+    r = (double *)malloc(sizeof(double) * 4 * (size_t)(nr > 0 ? nr : 1));
+    if (!r) wavehip_die(wavehip_ctx(), "mask (no memory)", WV_ENOMEM);
+    for (int k = 0; k < nr; k++) {
+        r[k] = polyrect[k].x1; r[nr + k] = polyrect[k].y1; r[2 * nr + k] = polyrect[k].x2; r[3 * nr + k] = polyrect[k].y2;
+    }
+    L.nrects = nr;
+    L.rects = r;
+#endif
+#if WAVEHIP_RECTROT
+    const int nrr = npolyrect_rot;
+    rr = (double *)malloc(sizeof(double) * 5 * (size_t)(nrr > 0 ? nrr : 1));
+    if (!rr) wavehip_die(wavehip_ctx(), "mask (no memory)", WV_ENOMEM);
+    for (int k = 0; k < nrr; k++) {
+        rr[k] = polyrectrot[k].x1; rr[nrr + k] = polyrectrot[k].y1; rr[2 * nrr + k] = polyrectrot[k].x2;
+        rr[3 * nrr + k] = polyrectrot[k].y2; rr[4 * nrr + k] = polyrectrot[k].width;
+    }
+    L.nrectrot = nrr;
+    L.rectrot = rr;
+#endif
+#if WAVEHIP_POLYGONS
+    const int np = ncircles;
+    pg = (double *)malloc(sizeof(double) * 6 * (size_t)(np > 0 ? np : 1));
+    if (!pg) wavehip_die(wavehip_ctx(), "mask (no memory)", WV_ENOMEM);
+    for (int k = 0; k < np; k++) {
+        pg[k] = polygons[k].xc; pg[np + k] = polygons[k].yc; pg[2 * np + k] = polygons[k].radius;
+        pg[3 * np + k] = polygons[k].angle; pg[4 * np + k] = (double)polygons[k].nsides;
+        pg[5 * np + k] = polygons[k].active ? 1.0 : 0.0;
+    }
+    L.npolygons = np;
+    L.polygons = pg;
+#endif
+#if WAVEHIP_POLYLINE
+    const int npl = npolyline;
+    pl = (double *)malloc(sizeof(double) * 2 * (size_t)(npl > 0 ? npl : 1));
+    if (!pl) wavehip_die(wavehip_ctx(), "mask (no memory)", WV_ENOMEM);
+    for (int k = 0; k < npl; k++) {
+        pl[k] = polyline[k].x;
+        pl[npl + k] = polyline[k].y;
+    }
+    L.npolyline = npl;
+    L.polyline = pl;
+#endif
+    const int rc = wv_build_list_domain_mask(wavehip_ctx(), B_DOMAIN, wavehip_box, dp, &L, flat);
+    free(r);
+    free(rr);
+    free(pg);
+    free(pl);
+    return rc;
+}
+#endif
 
 static void wavehip_mask(short *flat)
 {
@@ -425,6 +502,8 @@ static void wavehip_mask(short *flat)
 #if COMPARISON
     rc = wv_build_comparison_mask(wavehip_ctx(), B_DOMAIN, B_DOMAIN_B, wavehip_box, &dp, nc[0], cl[0], nc[1], cl[1],
                                   flat);
+#elif WAVEHIP_LISTS
+    rc = wavehip_list_mask(&dp, nc[0], cl[0], flat);
 #else
     rc = wv_build_domain_mask(wavehip_ctx(), B_DOMAIN, wavehip_box, &dp, nc[0], cl[0], flat);
 #endif

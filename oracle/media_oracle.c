@@ -956,6 +956,191 @@ static int o_domain_at(double x, double y, int dom, const orc_domain_params *d, 
     return o_domain_more(x, y, dom, d, box);
 }
 
+/* ---- list-driven domains of xy_in_billiard_single_domain ------------------ */
+
+/* xy_in_polyrect, sub_wave.c:5328-5358: the closed rectangle, corners in
+ * either order */
+static int o_in_rect(double x, double y, const double *R, int n, int k)
+{
+    double x1 = R[k], y1 = R[n + k], x2 = R[2 * n + k], y2 = R[3 * n + k], t;
+    if (!(x1 < x2)) { t = x1; x1 = x2; x2 = t; }
+    if (!(y1 < y2)) { t = y1; y1 = y2; y2 = t; }
+    if (x < x1) return 0;
+    if (x > x2) return 0;
+    if (y < y1) return 0;
+    if (y > y2) return 0;
+    return 1;
+}
+
+/* the open rectangle the maze and guide domains test inline (6593, 5914) */
+static int o_in_rect_open(double x, double y, const double *R, int n, int k)
+{
+    return (x > R[k]) && (x < R[2 * n + k]) && (y > R[n + k]) && (y < R[3 * n + k]);
+}
+
+/* xy_in_rectrotated, sub_wave.c:5393-5416 */
+static int o_in_rectrot(double x, double y, const double *R, int n, int k)
+{
+    const double X1 = R[k], Y1 = R[n + k], X2 = R[2 * n + k], Y2 = R[3 * n + k], W = R[4 * n + k];
+    const double l = o_mod(X2 - X1, Y2 - Y1);
+    if (l == 0.0) return 0;
+    const double u1 = (X2 - X1) / l, u2 = (Y2 - Y1) / l, v1 = x - X1, v2 = y - Y1;
+    const double pscal = u1 * v1 + u2 * v2;
+    if (pscal < 0.0) return 0;
+    if (pscal > l) return 0;
+    const double h2 = v1 * v1 + v2 * v2 - pscal * pscal;
+    return 4.0 * h2 <= W * W;
+}
+
+/* in_tpolygon, sub_wave.c:380-400 */
+static int o_in_tpolygon(double x, double y, const double *P, int n, int k)
+{
+    const double xc = P[k], yc = P[n + k], radius = P[2 * n + k], ang = P[3 * n + k];
+    const int nsides = (int)P[4 * n + k];
+    int condition = 1, q;
+    const double x1 = (x - xc) / radius, y1 = (y - yc) / radius;
+    if (x1 * x1 + y1 * y1 >= 1.0) return 0;
+    const double omega = O_DPI / ((double)nsides), cw = cos(omega * 0.5);
+    for (q = 0; q < nsides; q++) {
+        const double angle = ang * O_PID + ((double)q + 0.5) * omega;
+        condition = condition * (x1 * cos(angle) + y1 * sin(angle) < cw);
+    }
+    return condition;
+}
+
+/* xy_in_triangle_tvertex, sub_wave.c:5313-5325, on polyline vertices */
+static int o_tri_v(double x, double y, const orc_domain_lists *L, int a, int b, int c)
+{
+    const int n = L->npolyline;
+    const double z1[2] = {L->polyline[a], L->polyline[n + a]}, z2[2] = {L->polyline[b], L->polyline[n + b]};
+    const double z3[2] = {L->polyline[c], L->polyline[n + c]};
+    return o_tri(x, y, z1, z2, z3);
+}
+
+/* the lattice discs of D_CIRCLE_LATTICE_HONEY (7169) and _RHOMBUS (7188) */
+static int o_lattice_rows(double x, double y, int honey, const orc_domain_params *d, const double box[4])
+{
+    const double dx = (box[1] - box[0]) / (double)d->ngridx, dy = (box[3] - box[2]) / (double)d->ngridy;
+    int i, j;
+    for (j = 0; j < d->ngridy + 1; j++) {
+        const double y1 = box[2] + ((double)j) * dy;
+        for (i = 0; i < d->ngridx + 1; i++) {
+            if (honey && !(i % 3 != 2 * ((j + 1) % 2))) continue;
+            double x1 = box[0] + ((double)i) * dx;
+            if (j % 2 == 1) x1 += 0.5 * dx;
+            if (o_mod(x - x1, y - y1) < d->mu) return 1;
+        }
+    }
+    return 0;
+}
+
+static int o_list_at(double x, double y, int dom, const orc_domain_params *d, const double box[4],
+                     const orc_domain_lists *L)
+{
+    const double LAMBDA = d->lambda, MU = d->mu, WALL_WIDTH = d->wall_width;
+    const int nr = L->nrects, nrr = L->nrectrot, nc = L->ncircles, npl = L->npolyline, NPOLY = d->npoly;
+    double x1;
+    int i, j, k, m, condition;
+    switch (dom) {
+    case 40:                                                                         /* 6254 */
+        for (i = 0; i < L->npolygons; i++)
+            if ((L->polygons[5 * L->npolygons + i] != 0.0) && o_in_tpolygon(x, y, L->polygons, L->npolygons, i))
+                return 0;
+        return 1;
+    case 41:                                                                         /* 6260 */
+        condition = o_tri_v(x, y, L, 0, npl / 3, 2 * npl / 3);
+        m = 1;
+        k = 1;
+        for (i = 0; i < d->mdepth; i++) {
+            m = m * 4;
+            for (j = 0; j < npl / m; j++) condition += o_tri_v(x, y, L, j * m + k, j * m + 2 * k, j * m + 3 * k);
+            k = k * 4;
+        }
+        return condition >= 1;
+    case 42:                                                                         /* 6274 */
+        condition = o_tri_v(x, y, L, NPOLY, NPOLY - 1, 0);
+        for (i = 0; i < NPOLY - 1; i++) condition += o_tri_v(x, y, L, NPOLY, i, i + 1);
+        return condition >= 1;
+    case 421:                                                                        /* 6281 */
+        condition = (x < 0.0) && (o_abs(y) < 0.5 * WALL_WIDTH);
+        for (i = 0; i < NPOLY + 3; i++) condition += o_tri_v(x, y, L, NPOLY + 3, i + 1, i);
+        return condition >= 1;
+    case 53: case 531: case 54: case 541:                                            /* 6590-6611 */
+        for (i = 0; i < nr; i++)
+            if (o_in_rect_open(x, y, L->rects, nr, i)) return 0;
+        return 1;
+    case 192: case 193: case 194:                                                    /* 5903, 5921, 5942 */
+        if (x > 0.0) {
+            if (o_abs(y) > 0.5 * LAMBDA) return 1;
+            if (dom == 194 && !(o_abs(y) > 0.1 * LAMBDA)) {
+                const double xa = 0.24 * LAMBDA, xb = xa + WALL_WIDTH;
+                double r2 = 1.5 * 0.25 * LAMBDA;
+                r2 = r2 * r2;
+                const double x3 = sqrt(r2 - 0.01 * LAMBDA * LAMBDA) - xb;
+                if (((x + x3) * (x + x3) + y * y < r2) && (x > xa)) return 0;
+                return 1;
+            }
+            x1 = -y * y / LAMBDA + 0.25 * LAMBDA;
+            if (dom == 192) return (x < x1) || (x > x1 + WALL_WIDTH);
+            if ((x < x1) || (x > x1 + WALL_WIDTH)) return 1;
+            if (dom == 193 && o_abs(y) < 0.1 * LAMBDA) return 0;
+            return 2;
+        }
+        for (i = 0; i < nr; i++)
+            if (o_in_rect_open(x, y, L->rects, nr, i)) return (dom == 192) ? 0 : 2;
+        if (o_abs(y) > 0.5 * MU) return 1;
+        x1 = y * y / MU - 0.25 * MU;
+        if (dom == 192) return (x > x1) || (x < x1 - WALL_WIDTH);
+        if ((x > x1) || (x < x1 - WALL_WIDTH)) return 1;
+        return 2;
+    case 931: {                                                                      /* 7128 */
+        const double dx = (box[1] - box[0]) / (double)d->ngridx, dy = (box[3] - box[2]) / (double)d->ngridy;
+        for (i = 0; i < d->ngridx; i++) {
+            x1 = box[0] + ((double)i + 0.5) * dx;
+            for (j = 0; j < d->ngridy; j++)
+                if (o_mod(x - x1, y - (box[2] + ((double)j + 0.5) * dy)) < MU) return 1;
+        }
+        for (i = 0; i < nr; i++)
+            if (o_in_rect(x, y, L->rects, nr, i)) return 1;
+        return 0;
+    }
+    case 95: case 97:                                                                /* 7169, 7188 */
+        if (o_lattice_rows(x, y, dom == 95, d, box)) return 1;
+        for (i = 0; i < nrr; i++)
+            if (o_in_rectrot(x, y, L->rectrot, nrr, i)) return 1;
+        return 0;
+    case 96: case 932: case 933: case 941:                                           /* 7207, 7215, 7301, 7311 */
+        for (i = 0; i < nc; i++)
+            if (o_mod(x - L->circles[i], y - L->circles[nc + i]) < MU) return 1;
+        if (dom != 96)
+            for (i = 0; i < nr; i++)
+                if (o_in_rect(x, y, L->rects, nr, i)) return 1;
+        for (i = 0; i < nrr; i++)
+            if (o_in_rectrot(x, y, L->rectrot, nrr, i)) return 1;
+        return 0;
+    }
+    return -1;
+}
+
+int orc_list_domain(int NX, int NY, const double box[4], int domain, const orc_domain_params *dp,
+                    const orc_domain_lists *L, short *out)
+{
+    /* the vertices each polyline domain reads */
+    if (domain == 41 && (L->npolyline < 1 || dp->mdepth < 0 || dp->mdepth > 15)) return -1;
+    if (domain == 42 && (dp->npoly < 1 || L->npolyline < dp->npoly + 1)) return -1;
+    if (domain == 421 && (dp->npoly < 0 || L->npolyline < dp->npoly + 4)) return -1;
+    if (o_list_at(0.0, 0.0, domain, dp, box, L) < 0) return -1;
+    int i, j;
+#pragma omp parallel for private(i, j)
+    for (i = 0; i < NX; i++)
+        for (j = 0; j < NY; j++) {
+            const double x = box[0] + ((double)i) * (box[1] - box[0]) / ((double)NX);
+            const double y = box[2] + ((double)j) * (box[3] - box[2]) / ((double)NY);
+            out[(size_t)i * NY + j] = (short)o_list_at(x, y, domain, dp, box, L);
+        }
+    return 0;
+}
+
 int orc_domain(int NX, int NY, const double box[4], int domain, const orc_domain_params *dp, int ncirc,
                const double *circ, short *out)
 {

@@ -388,6 +388,54 @@ def domain_params(**kw):
     return DomainParams(**d)
 
 
+class DomainLists(ctypes.Structure):
+    """orc_domain_lists: the lists of the list-driven domains, each a count
+    and a flat array of that many values per field"""
+    _fields_ = [("ncircles", ctypes.c_int), ("circles", _D), ("nrects", ctypes.c_int), ("rects", _D),
+                ("nrectrot", ctypes.c_int), ("rectrot", _D), ("npolygons", ctypes.c_int), ("polygons", _D),
+                ("npolyline", ctypes.c_int), ("polyline", _D)]
+
+
+# rows of each list: circles xc, yc, radius, active; rects x1, y1, x2, y2;
+# rectrot x1, y1, x2, y2, width; polygons xc, yc, radius, angle, nsides,
+# active; polyline x, y
+LIST_ROWS = dict(circles=4, rects=4, rectrot=5, polygons=6, polyline=2)
+
+
+def domain_lists(**lists):
+    """DomainLists from (rows, n) arrays; the arrays stay alive on the struct"""
+    bad = set(lists) - set(LIST_ROWS)
+    if bad:
+        raise KeyError(bad)
+    s = DomainLists()
+    keep = []
+    for name, rows in LIST_ROWS.items():
+        a = lists.get(name)
+        a = np.zeros((rows, 0)) if a is None else np.ascontiguousarray(a, np.float64)
+        if a.ndim != 2 or a.shape[0] != rows:
+            raise ValueError("%s: (%d, n) expected" % (name, rows))
+        keep.append(a)
+        setattr(s, "n" + name, a.shape[1])
+        setattr(s, name, _p(a, _D) if a.shape[1] else None)
+    s._keep = keep
+    return s
+
+
+def list_domain(nx, ny, box, dom, params, **lists):
+    """orc_list_domain: xy_in_billiard_single_domain over the grid for the
+    domains that read polyrect, polyrectrot, polygons, polyline or the global
+    circle list; lists as for domain_lists"""
+    L = lib()
+    L.orc_list_domain.argtypes = [ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, ctypes.POINTER(DomainParams),
+                                  ctypes.POINTER(DomainLists), _S]
+    dl = domain_lists(**lists)
+    out = np.zeros((nx, ny), np.int16)
+    bx = np.ascontiguousarray(box, np.float64)
+    if L.orc_list_domain(nx, ny, _p(bx, _D), dom, ctypes.byref(params), ctypes.byref(dl), _p(out, _S)):
+        raise ValueError("domain %d not provided, or its lists too short" % dom)
+    return out
+
+
 def domain(nx, ny, box, dom, params, circles=None):
     """orc_domain: xy_in_billiard_single_domain over the grid (int16 (nx, ny))
     for the closed-form domains; params from domain_params"""
@@ -563,6 +611,10 @@ class RefBuild:
             if hasattr(self.L, "ref_mask_billiard"):
                 self.L.ref_mask_billiard.argtypes = [ctypes.c_int, _D, _D, _D, _I, ctypes.c_int, _D, _D, _D, _I, _S]
                 self.L.ref_mask_billiard.restype = ctypes.c_int
+            if hasattr(self.L, "ref_set_lists"):
+                self.L.ref_set_lists.argtypes = [ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int,
+                                                 _D]
+                self.L.ref_set_lists.restype = ctypes.c_int
             return
         if hasattr(self.L, "ref_init_ior"):
             self.L.ref_init_ior.argtypes = [_S, _D, _D, _D, ctypes.c_uint]
@@ -653,6 +705,17 @@ class RefBuild:
         rc = self.L.ref_mask(domain, n, _p(cc[0], _D), _p(cc[1], _D), _p(cc[2], _D), _p(a, _I), _p(out, _S))
         assert rc == 0
         return out
+
+    def set_lists(self, rects=None, rectrot=None, polygons=None, polyline=None):
+        """fill the build's polyrect, polyrectrot, polygons and polyline
+        globals from (rows, n) arrays (rows as LIST_ROWS)"""
+        args, keep = [], []
+        for name, a in (("rects", rects), ("rectrot", rectrot), ("polygons", polygons), ("polyline", polyline)):
+            n = 0 if a is None else np.asarray(a).shape[1]
+            a = np.zeros((LIST_ROWS[name], 1)) if a is None else np.ascontiguousarray(a, np.float64)
+            keep.append(a)
+            args += [n, _p(a, _D)]
+        assert self.L.ref_set_lists(*args) == 0
 
     def mask_billiard(self, circles=None, circles_b=None):
         """xy_in_billiard over the build's grid (with COMPARISON: B_DOMAIN and

@@ -123,12 +123,47 @@ int ref_mask(int domain, int ncirc, const double *xc, const double *yc, const do
         circles[k].radius = r[k];
         circles[k].active = (short)active[k];
     }
+    ncircles = ncirc;  /* the disc lattices read the global count */
     for (int i = 0; i < NX; i++)
         for (int j = 0; j < NY; j++) {
             double xy[2];
             ij_to_xy(i, j, xy);
             out[(size_t)i * NY + j] = (short)xy_in_billiard_single_domain(xy[0], xy[1], domain, ncirc, circles);
         }
+    return 0;
+}
+/* the lists the list-driven domains read (global_pdes.c:533-545), as the
+ * program's init functions leave them, from flat arrays: rectangles x1, y1,
+ * x2, y2 (polyrect); rotated rectangles x1, y1, x2, y2, width (polyrectrot);
+ * polygons xc, yc, radius, angle, nsides, active (polygons); vertices x, y
+ * (polyline); each field a run of n values */
+int ref_set_lists(int nr, const double *rect, int nrr, const double *rrot, int np, const double *pg, int npl,
+                  const double *pl)
+{
+    if (nr < 0 || nr > NMAXPOLY || nrr < 0 || nrr > NMAXPOLY || np < 0 || np > NMAXCIRCLES || npl < 0 ||
+        npl > NMAXPOLY)
+        return -1;
+    for (int k = 0; k < nr; k++) {
+        polyrect[k].x1 = rect[k]; polyrect[k].y1 = rect[nr + k];
+        polyrect[k].x2 = rect[2 * nr + k]; polyrect[k].y2 = rect[3 * nr + k];
+    }
+    npolyrect = nr;
+    for (int k = 0; k < nrr; k++) {
+        polyrectrot[k].x1 = rrot[k]; polyrectrot[k].y1 = rrot[nrr + k];
+        polyrectrot[k].x2 = rrot[2 * nrr + k]; polyrectrot[k].y2 = rrot[3 * nrr + k];
+        polyrectrot[k].width = rrot[4 * nrr + k];
+    }
+    npolyrect_rot = nrr;
+    for (int k = 0; k < np; k++) {
+        polygons[k].xc = pg[k]; polygons[k].yc = pg[np + k]; polygons[k].radius = pg[2 * np + k];
+        polygons[k].angle = pg[3 * np + k]; polygons[k].nsides = (int)pg[4 * np + k];
+        polygons[k].active = (short)pg[5 * np + k];
+    }
+    for (int k = 0; k < npl; k++) {
+        polyline[k].x = pl[k];
+        polyline[k].y = pl[npl + k];
+    }
+    npolyline = npl;
     return 0;
 }
 /* xy_in_billiard itself (sub_wave.c:7593-7602: with COMPARISON, B_DOMAIN

@@ -235,6 +235,22 @@ typedef struct {
 int orc_domain(int NX, int NY, const double box[4], int domain, const orc_domain_params *dp, int ncirc,
                const double *circ, short *out);
 
+/* The lists of the list-driven domains, layout of wv_domain_lists: flat
+ * runs of n values per field. */
+typedef struct {
+    int ncircles;  const double *circles;   /* xc, yc, radius, active        */
+    int nrects;    const double *rects;     /* x1, y1, x2, y2 (polyrect)      */
+    int nrectrot;  const double *rectrot;   /* x1, y1, x2, y2, width          */
+    int npolygons; const double *polygons;  /* xc, yc, radius, angle, nsides, active */
+    int npolyline; const double *polyline;  /* x, y                           */
+} orc_domain_lists;
+
+/* xy_in_billiard_single_domain for the domains that read those lists
+ * (polyrect, polyrectrot, polygons, polyline, the global circle list);
+ * -1 for another code or a list too short for the code */
+int orc_list_domain(int NX, int NY, const double box[4], int domain, const orc_domain_params *dp,
+                    const orc_domain_lists *L, short *out);
+
 /* xy_in_billiard with COMPARISON (sub_wave.c:7593-7602): domain and circ
  * where y > 0, domain_b and circ_b where y <= 0; -1 for an unknown code */
 int orc_comparison_mask(int NX, int NY, const double box[4], int domain, int domain_b, const orc_domain_params *dp,

@@ -124,6 +124,23 @@ void xy_to_ij(double x, double y, int ij[2])
 }
 
 #define WAVEHIP_INIT
+#ifdef LIST_INPUT
+/* the rectangle lists a list-driven domain reads (global_pdes.c:534-545) */
+#define NMAXPOLY 50000
+typedef struct {
+    double x1, y1, x2, y2;
+    double posi1, posj1, posi2, posj2;
+} t_rectangle;
+typedef struct {
+    double x1, y1, x2, y2;
+    double width;
+    double posi1, posj1, posi2, posj2, posi3, posj3, posi4, posj4;
+} t_rect_rotated;
+int npolyrect = NMAXPOLY, npolyrect_rot = NMAXPOLY;
+t_rectangle polyrect[NMAXPOLY];
+t_rect_rotated polyrectrot[NMAXPOLY];
+#endif
+
 #include "wavehip_compat.h"
 
 #define NC ((size_t)NX * NY)
@@ -142,6 +159,17 @@ static void read_list(FILE *f, t_circle *c, int *n)
     free(a);
 }
 
+#ifdef LIST_INPUT
+/* a count, then rows runs of that many doubles */
+static double *read_runs(FILE *f, int rows, int *n)
+{
+    if (fread(n, sizeof(int), 1, f) != 1 || *n < 0 || *n > NMAXPOLY) exit(2);
+    double *a = malloc(sizeof(double) * rows * (size_t)(*n > 0 ? *n : 1));
+    if (!a || fread(a, sizeof(double), rows * (size_t)*n, f) != rows * (size_t)*n) exit(2);
+    return a;
+}
+#endif
+
 int main(int argc, char **argv)
 {
     if (argc == 2 && strcmp(argv[1], "--params") == 0) {
@@ -150,8 +178,13 @@ int main(int argc, char **argv)
 #else
         const int xyin = 0;
 #endif
-        printf("nx=%d ny=%d twospeeds=%d domain=%d domain_b=%d comparison=%d variable_ior=%d ior=%d xy_from_input=%d\n",
-               NX, NY, TWOSPEEDS, B_DOMAIN, B_DOMAIN_B, COMPARISON, VARIABLE_IOR, IOR, xyin);
+#ifdef LIST_INPUT
+        const int lists = 1;
+#else
+        const int lists = 0;
+#endif
+        printf("nx=%d ny=%d twospeeds=%d domain=%d domain_b=%d comparison=%d variable_ior=%d ior=%d xy_from_input=%d "
+               "lists=%d\n", NX, NY, TWOSPEEDS, B_DOMAIN, B_DOMAIN_B, COMPARISON, VARIABLE_IOR, IOR, xyin, lists);
         return 0;
     }
     if (argc != 3) {
@@ -162,6 +195,22 @@ int main(int argc, char **argv)
     if (!f) return 2;
     read_list(f, circles, &ncircles);
     read_list(f, circles_b, &ncircles_b);
+#ifdef LIST_INPUT
+    int n;
+    double *a = read_runs(f, 4, &n);
+    for (int k = 0; k < n; k++) {
+        polyrect[k].x1 = a[k]; polyrect[k].y1 = a[n + k]; polyrect[k].x2 = a[2 * n + k]; polyrect[k].y2 = a[3 * n + k];
+    }
+    npolyrect = n;
+    free(a);
+    a = read_runs(f, 5, &n);
+    for (int k = 0; k < n; k++) {
+        polyrectrot[k].x1 = a[k]; polyrectrot[k].y1 = a[n + k]; polyrectrot[k].x2 = a[2 * n + k];
+        polyrectrot[k].y2 = a[3 * n + k]; polyrectrot[k].width = a[4 * n + k];
+    }
+    npolyrect_rot = n;
+    free(a);
+#endif
 #ifdef XY_FROM_INPUT
     static short xy_in_file[(size_t)NX * NY];
     int seed = 0;

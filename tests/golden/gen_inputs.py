@@ -260,6 +260,45 @@ DOMAIN_CODES = (999, 0, 1, 196, 199, 198, 197, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 1
                 35, 43, 45, 44, 441, 46, 461, 47, 48, 49, 52, 521, 526, 55, 56, 57, 58, 581, 59, 65, 66, 731, 75,
                 751, 78, 781, 792, 793, 794, 797, 798, 93, 934, 935, 936, 937, 94, 942, 947, 98, 981, 982, 983, 99,
                 991, 21, 22, 23)
+# the domains of xy_in_billiard_single_domain that read lists other reference
+# functions fill (polyrect, polyrectrot, polygons, polyline, the global
+# circle list); their fixtures use list_data's lists on the dom_* builds
+LIST_DOMAIN_CODES = (40, 41, 42, 421, 53, 531, 54, 541, 192, 193, 194, 931, 95, 97, 96, 932, 933, 941)
+
+
+def list_data(npoly):
+    """the lists of the list-driven domains for a build with NPOLY npoly:
+    rects (4, 24), every fourth with its corners in reverse order; rectrot
+    (5, 16), one of length 0; polygons (6, 12), every fourth inactive;
+    polyline (2, 64): npoly star points counter-clockwise, the centre at
+    index npoly, then further points on a wavy circle"""
+    x1 = uniform(case_seed("rect_x"), 24, -1.9, 1.5)
+    y1 = uniform(case_seed("rect_y"), 24, -1.05, 0.8)
+    w = uniform(case_seed("rect_w"), 24, 0.05, 0.6)
+    h = uniform(case_seed("rect_h"), 24, 0.05, 0.4)
+    rect = np.vstack([x1, y1, x1 + w, y1 + h])
+    rect[:, 3::4] = rect[[2, 3, 0, 1]][:, 3::4]
+    ax = uniform(case_seed("rrot_ax"), 16, -1.9, 1.9)
+    ay = uniform(case_seed("rrot_ay"), 16, -1.0, 1.0)
+    th = uniform(case_seed("rrot_th"), 16, 0.0, 6.0)
+    ln = uniform(case_seed("rrot_l"), 16, 0.1, 0.9)
+    ln[5] = 0.0
+    rrot = np.vstack([ax, ay, ax + ln * np.cos(th), ay + ln * np.sin(th), uniform(case_seed("rrot_w"), 16, 0.02, 0.2)])
+    pg = np.vstack([uniform(case_seed("pg_x"), 12, -1.7, 1.7), uniform(case_seed("pg_y"), 12, -0.9, 0.9),
+                    uniform(case_seed("pg_r"), 12, 0.1, 0.5), uniform(case_seed("pg_a"), 12, 0.0, 4.0),
+                    np.array([3, 4, 5, 6, 7, 8, 3, 5, 9, 4, 6, 12], np.float64),
+                    np.array([0.0 if k % 4 == 3 else 1.0 for k in range(12)])])
+    t = 2.0 * np.pi * np.arange(npoly) / npoly
+    r = np.where(np.arange(npoly) % 2 == 0, 0.95, 0.45)
+    star = np.vstack([r * np.cos(t), r * np.sin(t)])
+    rest = 64 - npoly - 1
+    t2 = 2.0 * np.pi * np.arange(rest) / rest
+    wavy = np.vstack([(0.8 + 0.25 * np.sin(5 * t2)) * np.cos(t2), (0.8 + 0.25 * np.sin(5 * t2)) * np.sin(t2)])
+    pl = np.hstack([star, np.zeros((2, 1)), wavy])
+    return dict(rects=np.ascontiguousarray(rect), rectrot=np.ascontiguousarray(rrot), polygons=np.ascontiguousarray(pg),
+                polyline=np.ascontiguousarray(pl))
+
+
 # codes whose one-time constants the reference keeps in a static `first`
 # flag shared by the function (sub_wave.c:5717): their fixtures come from a
 # fresh process each, as a program that uses one domain would compute them

@@ -30,7 +30,8 @@ Outputs (committed):
                     writes this file alone)
   domain_cases.npz  every closed-form domain of xy_in_billiard_single_domain
                     (gen_inputs.DOMAIN_CODES) on the dom_a, dom_b and dom_c
-                    builds
+                    builds, and the list-driven ones (LIST_DOMAIN_CODES, keys
+                    <build>/list/<code>) with gen_inputs.list_data's lists
                     (python make_golden.py domains writes this file alone)
   ior_cases.npz     init_ior_2d's per-cell media models (gen_inputs.IOR_CODES)
                     and its constant branch on gen_inputs.ior_mask(); the
@@ -60,7 +61,7 @@ sys.path.insert(0, os.path.join(ROOT, "youtube-simulations_amd"))
 sys.path.insert(0, HERE)
 
 import pyoracle  # noqa: E402
-from gen_inputs import (COLOR_FRAMES, COMPARISON_BUILDS, DOMAIN_CODES, DOMAIN_FIRST_CODES, EXT_STEPS, IOR_CODES, PULSE_INIT, PULSES, ROT_ANGLES,  # noqa: E402
+from gen_inputs import (COLOR_FRAMES, COMPARISON_BUILDS, DOMAIN_CODES, DOMAIN_FIRST_CODES, EXT_STEPS, LIST_DOMAIN_CODES, list_data, IOR_CODES, PULSE_INIT, PULSES, ROT_ANGLES,  # noqa: E402
                         TABLE_STEPS, case_inputs, cmp_cases, color_cases, color_inputs, domain_sets, ext_cases,
                         ext_inputs, ior_mask, mask_cases, mask_circles, mask_circles_b, small_cases, table_cases, table_inputs)
 from wavehip import workloads  # noqa: E402
@@ -122,6 +123,14 @@ def domains():
                 out["%s/%d" % (tag, dom)] = fresh_mask(tag, dom)
             else:
                 out["%s/%d" % (tag, dom)] = R.mask(dom, circ if dom in (20, 201, 202) else None)
+    # the list-driven domains on the same builds, with list_data's lists
+    for tag, (nx, ny, kw) in domain_sets().items():
+        R = pyoracle.RefBuild(tag)
+        lists = list_data(kw.get("npoly", pyoracle.DOMAIN_DEFAULTS["npoly"]))
+        R.set_lists(**lists)
+        for dom in LIST_DOMAIN_CODES:
+            c = circ if dom in (96, 932, 933, 941) else (np.zeros((4, lists["polygons"].shape[1])) if dom == 40 else None)
+            out["%s/list/%d" % (tag, dom)] = R.mask(dom, c)
     # xy_in_billiard with COMPARISON: the reference's own function on two builds
     for tag in COMPARISON_BUILDS:
         out[tag + "/mask"] = pyoracle.RefBuild(tag).mask_billiard(circ, mask_circles_b())

@@ -172,7 +172,7 @@ def test_host_program_matches_oracle(oracle, tmp_path, name, mode, ncalls):
 
 # ---- the start-up drop-ins (WAVEHIP_INIT, tests/compat/compat_init.c) --------
 
-INIT_VARIANTS = ["init_michelson", "init_wells", "init_flat_cmp", "init_rwells"]
+INIT_VARIANTS = ["init_michelson", "init_wells", "init_flat_cmp", "init_rwells", "init_latmaze"]
 # the macros compat_init.c fixes (its #define block)
 INIT_BOX = (-2.0, 2.0, -1.125, 1.125)
 INIT_DOMAIN = dict(lam=0.8, mu=0.3, mu_b=0.08, apoly=0.5, npoly=6, wall_width=0.1, wall_width_b=0.1,
@@ -201,10 +201,18 @@ def test_init_program_matches_oracle(oracle, tmp_path, name):
     nx, ny = d["nx"], d["ny"]
     ca, cb = mask_circles(), mask_circles_b()
     inp, outp = tmp_path / "in.bin", tmp_path / "out.bin"
+    if d["lists"]:  # eight discs of radius MU, and list_data's rectangles
+        from gen_inputs import list_data
+        ca = np.ascontiguousarray(ca[:, :8])
+        lists = list_data(INIT_DOMAIN["npoly"])
     with open(inp, "wb") as f:
         for c in (ca, cb):
             f.write(np.int32(c.shape[1]).tobytes())
             f.write(np.ascontiguousarray(c, np.float64).tobytes())
+        if d["lists"]:
+            for key in ("rects", "rectrot"):
+                f.write(np.int32(lists[key].shape[1]).tobytes())
+                f.write(np.ascontiguousarray(lists[key], np.float64).tobytes())
         if d["xy_from_input"]:  # the mask and seed of the reference's IOR_RANDOM_WELLS fixture
             from gen_inputs import case_inputs
             f.write(np.ascontiguousarray(case_inputs("ior_wells_r", nx, ny)["xy_in"], np.int16).tobytes())
@@ -228,6 +236,10 @@ def test_init_program_matches_oracle(oracle, tmp_path, name):
     P = oracle.domain_params(**INIT_DOMAIN)
     if d["comparison"]:
         want = oracle.comparison_mask(nx, ny, INIT_BOX, d["domain"], d["domain_b"], P, ca, cb)
+    elif d["lists"]:
+        want = oracle.list_domain(nx, ny, INIT_BOX, d["domain"], P, circles=ca, rects=lists["rects"],
+                                  rectrot=lists["rectrot"])
+        assert 0 < np.count_nonzero(want) < want.size
     else:
         want = oracle.domain(nx, ny, INIT_BOX, d["domain"], P, ca if d["domain"] == 20 else None)
     assert np.array_equal(xy, want)

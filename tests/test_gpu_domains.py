@@ -280,3 +280,98 @@ def test_batch_builders_per_grid(wh, oracle):
             assert bits_equal(u[0], ur) and bits_equal(u[1], umr), k
     finally:
         B.close()
+
+
+# ---- the list-driven domains ---------------------------------------------------
+
+def test_list_domain_masks_match_reference_fixture(wh, oracle):
+    """every list-driven code on the three builds, with list_data's lists and
+    the circle list: the reference's masks; a closed-form code through the
+    same entry point equals wv_build_domain_mask's fixture"""
+    from gen_inputs import LIST_DOMAIN_CODES, list_data
+    g = np.load(os.path.join(GOLD, "domain_cases.npz"))
+    for tag, (nx, ny, kw0) in domain_sets().items():
+        kw = dict(oracle.DOMAIN_DEFAULTS, **kw0)
+        lists = dict(list_data(kw["npoly"]), circles=mask_circles())
+        W = wh.Wave(dict(nx=nx, ny=ny))
+        try:
+            for dom in LIST_DOMAIN_CODES:
+                say("list domain fixture", tag, dom)
+                got = W.build_list_domain_mask(dom, MASK_BOX, lists=lists, **kw)
+                assert np.array_equal(got, g["%s/list/%d" % (tag, dom)]), (tag, dom)
+            got = W.build_list_domain_mask(20, MASK_BOX, lists=lists, **kw)
+            assert np.array_equal(got, g["%s/20" % tag])
+        finally:
+            W.close()
+
+
+def long_lists(rng, npoly):
+    """lists longer than a wave: 300 rectangles (every seventh reversed), 150
+    rotated ones, 90 polygons of 3..16 sides, 70 circles, list_data's
+    polyline"""
+    from gen_inputs import list_data
+    nr, nrr, npg = 300, 150, 90
+    x1, y1 = rng.uniform(-1.9, 1.6, nr), rng.uniform(-1.0, 0.9, nr)
+    rects = np.vstack([x1, y1, x1 + rng.uniform(0.02, 0.3, nr), y1 + rng.uniform(0.02, 0.3, nr)])
+    rects[:, ::7] = rects[[2, 3, 0, 1]][:, ::7]
+    ax, ay, th = rng.uniform(-1.9, 1.9, nrr), rng.uniform(-1.0, 1.0, nrr), rng.uniform(0, 6.3, nrr)
+    ln = rng.uniform(0.05, 0.6, nrr)
+    rrot = np.vstack([ax, ay, ax + ln * np.cos(th), ay + ln * np.sin(th), rng.uniform(0.01, 0.1, nrr)])
+    pg = np.vstack([rng.uniform(-1.8, 1.8, npg), rng.uniform(-1.0, 1.0, npg), rng.uniform(0.03, 0.3, npg),
+                    rng.uniform(0, 4, npg), rng.integers(3, 17, npg).astype(float),
+                    (rng.uniform(0, 1, npg) > 0.2).astype(float)])
+    c = np.vstack([rng.uniform(-1.9, 1.9, 70), rng.uniform(-1.1, 1.1, 70), np.zeros(70), np.ones(70)])
+    return dict(rects=rects, rectrot=rrot, polygons=pg, circles=c, polyline=list_data(npoly)["polyline"])
+
+
+@pytest.mark.parametrize("shape", [(1280, 720), (333, 257)])
+def test_list_domain_masks_match_oracle(wh, oracle, shape):
+    """the checked-in configuration's macros and a second set, with lists
+    longer than a wave"""
+    from gen_inputs import LIST_DOMAIN_CODES
+    nx, ny = shape
+    lists = long_lists(np.random.default_rng(41), 9)
+    sets = [dict(wh.DOMAIN_DEFAULTS), dict(wh.DOMAIN_DEFAULTS, lam=0.9, mu=0.05, npoly=5, mdepth=3, ngridx=7, ngridy=5)]
+    W = wh.Wave(dict(nx=nx, ny=ny))
+    try:
+        for q, kw in enumerate(sets):
+            P = oracle.domain_params(**kw)
+            for dom in LIST_DOMAIN_CODES:
+                say("list domain", nx, ny, q, dom)
+                got = W.build_list_domain_mask(dom, BOX_REF, lists=lists, **kw)
+                assert np.array_equal(got, oracle.list_domain(nx, ny, BOX_REF, dom, P, **lists)), (q, dom)
+    finally:
+        W.close()
+
+
+def test_list_domain_mask_errors(wh):
+    from gen_inputs import list_data
+    L = list_data(6)
+    no_sides = dict(L, polygons=np.vstack([L["polygons"][:4], np.zeros((1, 12)), L["polygons"][5:]]))
+    W = wh.Wave(dict(nx=40, ny=30))
+    try:
+        for dom, lists, kw in ((42, dict(L, polyline=L["polyline"][:, :6]), dict(npoly=6)),
+                               (421, dict(L, polyline=L["polyline"][:, :9]), dict(npoly=6)),
+                               (41, L, dict(mdepth=16)), (40, no_sides, {}), (95, L, dict(ngridx=0)),
+                               (77, L, {})):
+            say("list domain error", dom)
+            with pytest.raises(wh.WaveError):
+                W.build_list_domain_mask(dom, MASK_BOX, lists=lists, **kw)
+        with pytest.raises(ValueError):
+            W.build_list_domain_mask(53, MASK_BOX, lists=dict(rects=np.zeros((3, 4))))
+    finally:
+        W.close()
+
+
+def test_batch_list_domain_mask(wh, oracle):
+    """the batch form on the second of two grids"""
+    lists = long_lists(np.random.default_rng(5), 9)
+    B = wh.Batch([dict(nx=50, ny=40), dict(nx=211, ny=97)])
+    try:
+        for dom in (53, 40, 941, 42):
+            say("batch list domain", dom)
+            got = B.build_list_domain_mask(1, dom, MASK_BOX, lists=lists)
+            assert np.array_equal(got, oracle.list_domain(211, 97, MASK_BOX, dom,
+                                                          oracle.domain_params(**wh.DOMAIN_DEFAULTS), **lists))
+    finally:
+        B.close()

@@ -133,3 +133,88 @@ def test_comparison_mask_matches_reference(oracle, gold):
         assert np.array_equal(got, gold[tag + "/mask"]), tag
         swapped = oracle.comparison_mask(97, 61, MASK_BOX, db, d, P, mask_circles_b(), mask_circles())
         assert not np.array_equal(swapped, gold[tag + "/mask"]), tag
+
+
+# ---- the list-driven domains ---------------------------------------------------
+
+LIST_CIRCLE_CODES = (96, 932, 933, 941)
+
+
+def list_mask(oracle, tag, dom, lists=None, **over):
+    from gen_inputs import list_data
+    nx, ny, kw = domain_sets()[tag]
+    kw = dict(kw, **over)
+    L = list_data(kw.get("npoly", oracle.DOMAIN_DEFAULTS["npoly"])) if lists is None else lists
+    return oracle.list_domain(nx, ny, MASK_BOX, dom, oracle.domain_params(**kw), circles=mask_circles(), **L)
+
+
+@pytest.mark.parametrize("tag", sorted(domain_sets()))
+@pytest.mark.parametrize("dom", __import__("gen_inputs").LIST_DOMAIN_CODES)
+def test_list_domain_oracle_matches_reference(oracle, gold, tag, dom):
+    """xy_in_billiard_single_domain of the three builds with list_data's
+    rectangles, rotated rectangles, polygons and polyline in the reference's
+    globals (ref_set_lists), and the circle list"""
+    got = list_mask(oracle, tag, dom)
+    assert got.dtype == np.int16 and np.array_equal(got, gold["%s/list/%d" % (tag, dom)])
+
+
+def test_list_domains_read_their_lists(oracle, gold):
+    """each code depends on the lists it reads: emptying them, or a planted
+    mistake in how one is read (rectangle corners not sorted, rotated
+    rectangle width doubled, polygon angle in radians, the polyline reversed)
+    fails the fixtures in at least one build"""
+    from gen_inputs import LIST_DOMAIN_CODES, list_data
+
+    def fails(dom, edit):
+        out = []
+        for t in domain_sets():
+            L = list_data(domain_sets()[t][2].get("npoly", oracle.DOMAIN_DEFAULTS["npoly"]))
+            try:
+                got = list_mask(oracle, t, dom, edit(L))
+            except ValueError:  # a polyline too short for the code is refused
+                out.append(True)
+                continue
+            out.append(not np.array_equal(got, gold["%s/list/%d" % (t, dom)]))
+        return any(out)
+
+    def empty(L):
+        return {k: v[:, :0] for k, v in L.items()}
+
+    for dom in LIST_DOMAIN_CODES:
+        if dom in LIST_CIRCLE_CODES:
+            continue  # the circle list is tested below
+        assert fails(dom, empty), dom
+    sorted_rects = lambda L: dict(L, rects=np.vstack([np.minimum(L["rects"][0], L["rects"][2]),  # noqa: E731
+                                                       np.minimum(L["rects"][1], L["rects"][3]),
+                                                       np.maximum(L["rects"][0], L["rects"][2]),
+                                                       np.maximum(L["rects"][1], L["rects"][3])]))
+    for dom in (53, 192, 193):  # the maze and guide tests leave reversed rectangles empty
+        assert fails(dom, sorted_rects), dom
+    wide = lambda L: dict(L, rectrot=np.vstack([L["rectrot"][:4], 2.0 * L["rectrot"][4:]]))  # noqa: E731
+    for dom in (95, 97, 96, 932):
+        assert fails(dom, wide), dom
+    rad = lambda L: dict(L, polygons=np.vstack([L["polygons"][:3], L["polygons"][3:4] * np.pi / 2,  # noqa: E731
+                                                 L["polygons"][4:]]))
+    assert fails(40, rad)
+    rev = lambda L: dict(L, polyline=np.ascontiguousarray(L["polyline"][:, ::-1]))  # noqa: E731
+    for dom in (41, 42, 421):
+        assert fails(dom, rev), dom
+    # the disc lattices read every circle of the list, radius MU, active or not
+    for dom in LIST_CIRCLE_CODES:
+        hits = []
+        for t, (nx, ny, kw) in domain_sets().items():
+            L = list_data(kw.get("npoly", oracle.DOMAIN_DEFAULTS["npoly"]))
+            c = mask_circles()
+            got = oracle.list_domain(nx, ny, MASK_BOX, dom, oracle.domain_params(**kw), circles=c[:, ::2], **L)
+            hits.append(not np.array_equal(got, gold["%s/list/%d" % (t, dom)]))
+        assert any(hits), dom
+
+
+def test_list_domain_oracle_refuses_short_lists(oracle):
+    from gen_inputs import list_data
+    L = list_data(6)
+    P = oracle.domain_params(npoly=6)
+    with pytest.raises(ValueError):
+        oracle.list_domain(20, 10, MASK_BOX, 421, P, **dict(L, polyline=L["polyline"][:, :9]))
+    with pytest.raises(ValueError):
+        oracle.list_domain(20, 10, MASK_BOX, 3, P, **L)  # a closed-form code goes through orc_domain

@@ -2782,15 +2782,64 @@ static bool domain_known(int d)
     return false;
 }
 
-// one domain's launch parameters: checks, the host constants (the
-// reference's expressions in its order) and the polygon-angle table
-static int domain_setup(Engine *e, const Grid &G, int domain, const double *box, const wv_domain_params *p,
-                        int ncirc, const double *circ, WvDomain &a, std::vector<double> &trig)
+// the domains that read the lists other reference functions fill
+static bool list_domain_known(int d)
 {
-    if (!domain_known(domain))
-        return fail(e, WV_EUNSUPPORTED, "domain %d is not among the closed-form domains provided", domain);
-    const bool circles = domain == 20 || domain == 201 || domain == 202;
-    if (circles && (ncirc < 0 || (ncirc > 0 && !circ))) return fail(e, WV_EINVAL, "circle list");
+    static const int codes[] = {40, 41, 42, 421, 53, 531, 54, 541, 192, 193, 194, 931, 95, 97, 96, 932, 933, 941};
+    for (int c : codes)
+        if (c == d) return true;
+    return false;
+}
+
+// the caller's lists (wv_domain_lists; the circle list alone for the
+// closed-form entry points)
+struct DomLists {
+    int nc = 0;
+    const double *c = nullptr;
+    int nr = 0;
+    const double *r = nullptr;
+    int nrr = 0;
+    const double *rr = nullptr;
+    int np = 0;
+    const double *pg = nullptr;
+    int npl = 0;
+    const double *pl = nullptr;
+};
+
+// one domain's launch parameters: checks, the host constants (the
+// reference's expressions in its order), the angle table and, for
+// D_POLYGONS, the polygons with their sides' table (polyg: xc, yc, radius,
+// active, nsides, cos(omega/2), offset in trig; runs of n values)
+static int domain_setup(Engine *e, const Grid &G, int domain, const double *box, const wv_domain_params *p,
+                        const DomLists &Ls, bool with_lists, WvDomain &a, std::vector<double> &trig,
+                        std::vector<double> &polyg)
+{
+    const bool listed = with_lists && list_domain_known(domain);
+    if (!domain_known(domain) && !listed)
+        return fail(e, WV_EUNSUPPORTED, "domain %d is not among the domains provided", domain);
+    const bool circles = domain == 20 || domain == 201 || domain == 202 ||
+                         (listed && (domain == 96 || domain == 932 || domain == 933 || domain == 941));
+    const int ncirc = Ls.nc;
+    if (circles && (ncirc < 0 || (ncirc > 0 && !Ls.c))) return fail(e, WV_EINVAL, "circle list");
+    if (listed) {
+        const auto bad = [](int n, const double *q) { return n < 0 || n > (1 << 20) || (n > 0 && !q); };
+        if (bad(Ls.nc, Ls.c) || bad(Ls.nr, Ls.r) || bad(Ls.nrr, Ls.rr) || bad(Ls.np, Ls.pg) || bad(Ls.npl, Ls.pl))
+            return fail(e, WV_EINVAL, "lists: counts 0..1048576 and an array for each nonzero count");
+        if (domain == 41 && (Ls.npl < 1 || p->mdepth < 0 || p->mdepth > 15))
+            return fail(e, WV_EINVAL, "D_VONKOCH: at least one vertex and MDEPTH 0..15");
+        if (domain == 42 && (p->npoly < 1 || Ls.npl < p->npoly + 1))
+            return fail(e, WV_EINVAL, "D_STAR: NPOLY >= 1 and NPOLY + 1 vertices");
+        if (domain == 421 && (p->npoly < 0 || Ls.npl < p->npoly + 4))
+            return fail(e, WV_EINVAL, "D_STAR_CHANNEL: NPOLY >= 0 and NPOLY + 4 vertices");
+        if ((domain == 931 || domain == 95 || domain == 97) &&
+            (p->ngridx < 1 || p->ngridy < 1 || (double)(p->ngridx + 1) * (p->ngridy + 1) > 1e6))
+            return fail(e, WV_EINVAL, "NGRIDX %d, NGRIDY %d: both >= 1, at most 1e6 disks", p->ngridx, p->ngridy);
+        if (domain == 40)
+            for (int k = 0; k < Ls.np; k++) {
+                const double ns = Ls.pg[4 * (size_t)Ls.np + k];
+                if (!(ns >= 1.0 && ns <= 4096.0)) return fail(e, WV_EINVAL, "polygon %d: 1..4096 sides", k);
+            }
+    }
     const bool poly = domain == 8 || domain == 32 || domain == 791 || domain == 59 || domain == 792 ||
                       domain == 794 || domain == 797 || domain == 798 || domain == 991;
     if (poly && (p->npoly < 1 || p->npoly > 4096)) return fail(e, WV_EINVAL, "NPOLY %d: 1..4096", p->npoly);
@@ -2833,7 +2882,27 @@ static int domain_setup(Engine *e, const Grid &G, int domain, const double *box,
     const double L = p->lambda, M = p->mu;
     a = WvDomain{};
     trig.clear();
-    if (domain == 8 || domain == 32) {  // sub_wave.c:5812-5817, 6034-6041
+    polyg.clear();
+    if (domain == 40 && listed) {  // in_tpolygon, sub_wave.c:380-400, per polygon
+        const size_t n = (size_t)Ls.np;
+        polyg.assign(7 * n, 0.0);
+        for (size_t k = 0; k < n; k++) {
+            const int ns = (int)Ls.pg[4 * n + k];
+            const double ang = Ls.pg[3 * n + k], omega = REF_DPI / ((double)ns);
+            polyg[k] = Ls.pg[k];
+            polyg[n + k] = Ls.pg[n + k];
+            polyg[2 * n + k] = Ls.pg[2 * n + k];
+            polyg[3 * n + k] = (short)Ls.pg[5 * n + k] != 0 ? 1.0 : 0.0;  // the short active flag
+            polyg[4 * n + k] = (double)ns;
+            polyg[5 * n + k] = std::cos(omega * 0.5);
+            polyg[6 * n + k] = (double)trig.size();
+            for (int q = 0; q < ns; q++) {
+                const double angle = ang * REF_PID + ((double)q + 0.5) * omega;
+                trig.push_back(std::cos(angle));
+                trig.push_back(std::sin(angle));
+            }
+        }
+    } else if (domain == 8 || domain == 32) {  // sub_wave.c:5812-5817, 6034-6041
         const double omega = REF_DPI / ((double)p->npoly);
         for (int k = 0; k < p->npoly; k++) {
             const double angle = p->apoly * REF_PID + ((double)k + 0.5) * omega;
@@ -2918,6 +2987,9 @@ static int domain_setup(Engine *e, const Grid &G, int domain, const double *box,
         a.c2 = r2;
     }
     a.nx = G.p.nx; a.ny = G.p.ny; a.domain = domain; a.ncirc = circles ? ncirc : 0; a.i0 = 0; a.rows = G.p.nx;
+    if (listed) {
+        a.nrect = Ls.nr; a.nrectrot = Ls.nrr; a.npolyg = domain == 40 ? Ls.np : 0; a.npline = Ls.npl;
+    }
     a.npoly = p->npoly; a.mdepth = p->mdepth; a.mratio = p->mratio; a.mandellevel = p->mandellevel;
     a.ngridx = p->ngridx; a.ngridy = p->ngridy;
     a.xmin = box[0]; a.xmax = box[1]; a.ymin = box[2]; a.ymax = box[3];
@@ -2933,53 +3005,86 @@ static int domain_setup(Engine *e, const Grid &G, int domain, const double *box,
 // COMPARISON (nd = 2) domain[0] where y > 0 and domain[1] where y <= 0
 // (sub_wave.c:7593-7602), each with its own circle list
 static int build_domain_masks(Engine *e, int k, int nd, const int *domain, const double *box,
-                              const wv_domain_params *p, const int *ncirc, const double *const *circ, short *xy)
+                              const wv_domain_params *p, const DomLists *Ls, bool with_lists, short *xy)
 {
     if (k < 0 || k >= e->n) return fail(e, WV_EINVAL, "grid index %d out of range", k);
     if (!box || !p || !xy) return fail(e, WV_EINVAL, "box, parameters or output is NULL");
     if (!(box[1] > box[0]) || !(box[3] > box[2])) return fail(e, WV_EINVAL, "empty box");
     WvDomain a[2];
-    std::vector<double> trig[2];
-    size_t off[2] = {0, 0}, nd_total = 0;
+    // one device block per domain: circles | rects | rectrot | polygons |
+    // polyline | angle table, each list only where the domain reads it
+    std::vector<double> blob[2];
     int rc;
     for (int q = 0; q < nd; q++) {
-        if ((rc = domain_setup(e, e->g[k], domain[q], box, p, ncirc[q], circ[q], a[q], trig[q]))) return rc;
+        std::vector<double> trig, polyg;
+        if ((rc = domain_setup(e, e->g[k], domain[q], box, p, Ls[q], with_lists, a[q], trig, polyg))) return rc;
         a[q].half = nd == 1 ? 0 : q + 1;
-        off[q] = nd_total;
-        nd_total += 4 * (size_t)a[q].ncirc + trig[q].size();
+        std::vector<double> &b = blob[q];
+        b.insert(b.end(), Ls[q].c, Ls[q].c + 4 * (size_t)a[q].ncirc);
+        if (a[q].nrect) b.insert(b.end(), Ls[q].r, Ls[q].r + 4 * (size_t)a[q].nrect);
+        if (a[q].nrectrot) b.insert(b.end(), Ls[q].rr, Ls[q].rr + 5 * (size_t)a[q].nrectrot);
+        b.insert(b.end(), polyg.begin(), polyg.end());
+        if (a[q].npline) b.insert(b.end(), Ls[q].pl, Ls[q].pl + 2 * (size_t)a[q].npline);
+        b.insert(b.end(), trig.begin(), trig.end());
     }
     const Grid &G = e->g[k];
     const size_t cells = (size_t)G.p.nx * G.p.ny;
     HIPCHK(e, hipSetDevice(e->device));
     short *d_out = nullptr;
-    double *d_c = nullptr;
+    double *d_c[2] = {nullptr, nullptr};
     hipError_t err = hipMalloc(&d_out, cells * sizeof(short));
-    if (err == hipSuccess) err = hipMalloc(&d_c, sizeof(double) * std::max<size_t>(nd_total, 1));
     for (int q = 0; q < nd && err == hipSuccess; q++) {
-        double *base = d_c + off[q];
-        const size_t nc4 = 4 * (size_t)a[q].ncirc;
-        if (nc4) err = hipMemcpy(base, circ[q], sizeof(double) * nc4, hipMemcpyHostToDevice);
-        if (err == hipSuccess && !trig[q].empty())
-            err = hipMemcpy(base + nc4, trig[q].data(), sizeof(double) * trig[q].size(), hipMemcpyHostToDevice);
+        err = hipMalloc(&d_c[q], sizeof(double) * std::max<size_t>(blob[q].size(), 1));
+        if (err == hipSuccess && !blob[q].empty())
+            err = hipMemcpy(d_c[q], blob[q].data(), sizeof(double) * blob[q].size(), hipMemcpyHostToDevice);
+        double *base = d_c[q];
         a[q].circ = base;
-        a[q].trig = base + nc4;
+        base += 4 * (size_t)a[q].ncirc;
+        a[q].rect = base;
+        base += 4 * (size_t)a[q].nrect;
+        a[q].rectrot = base;
+        base += 5 * (size_t)a[q].nrectrot;
+        a[q].polyg = base;
+        base += 7 * (size_t)a[q].npolyg;
+        a[q].pline = base;
+        base += 2 * (size_t)a[q].npline;
+        a[q].trig = base;
         a[q].out = d_out;
         if (err == hipSuccess) err = (hipError_t)wv_launch_domain(a[q], e->stream);
     }
     if (err == hipSuccess) err = hipMemcpyAsync(xy, d_out, cells * sizeof(short), hipMemcpyDeviceToHost, e->stream);
     if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
     (void)hipFree(d_out);
-    (void)hipFree(d_c);
+    for (double *d : d_c) (void)hipFree(d);
     if (err != hipSuccess) return fail(e, WV_EHIP, "domain mask: %s", hipGetErrorString(err));
     return WV_OK;
+}
+
+static DomLists circle_lists(int ncircles, const double *circles)
+{
+    DomLists L;
+    L.nc = ncircles;
+    L.c = circles;
+    return L;
+}
+
+static DomLists from_lists(const wv_domain_lists *l)
+{
+    DomLists L;
+    L.nc = l->ncircles; L.c = l->circles;
+    L.nr = l->nrects; L.r = l->rects;
+    L.nrr = l->nrectrot; L.rr = l->rectrot;
+    L.np = l->npolygons; L.pg = l->polygons;
+    L.npl = l->npolyline; L.pl = l->polyline;
+    return L;
 }
 
 extern "C" int wv_build_domain_mask(wv_ctx *c, int domain, const double box[4], const wv_domain_params *p,
                                     int ncircles, const double *circles, short *xy_in)
 {
     if (!c) return WV_EINVAL;
-    const double *cl[1] = {circles};
-    return build_domain_masks(c->e, 0, 1, &domain, box, p, &ncircles, cl, xy_in);
+    const DomLists L = circle_lists(ncircles, circles);
+    return build_domain_masks(c->e, 0, 1, &domain, box, p, &L, false, xy_in);
 }
 
 extern "C" int wv_batch_build_domain_mask(wv_batch *b, int k, int domain, const double box[4],
@@ -2987,8 +3092,27 @@ extern "C" int wv_batch_build_domain_mask(wv_batch *b, int k, int domain, const 
                                           short *xy_in)
 {
     if (!b) return WV_EINVAL;
-    const double *cl[1] = {circles};
-    return build_domain_masks(b->e, k, 1, &domain, box, p, &ncircles, cl, xy_in);
+    const DomLists L = circle_lists(ncircles, circles);
+    return build_domain_masks(b->e, k, 1, &domain, box, p, &L, false, xy_in);
+}
+
+extern "C" int wv_build_list_domain_mask(wv_ctx *c, int domain, const double box[4], const wv_domain_params *p,
+                                         const wv_domain_lists *lists, short *xy_in)
+{
+    if (!c) return WV_EINVAL;
+    if (!lists) return fail(c->e, WV_EINVAL, "lists is NULL");
+    const DomLists L = from_lists(lists);
+    return build_domain_masks(c->e, 0, 1, &domain, box, p, &L, true, xy_in);
+}
+
+extern "C" int wv_batch_build_list_domain_mask(wv_batch *b, int k, int domain, const double box[4],
+                                               const wv_domain_params *p, const wv_domain_lists *lists,
+                                               short *xy_in)
+{
+    if (!b) return WV_EINVAL;
+    if (!lists) return fail(b->e, WV_EINVAL, "lists is NULL");
+    const DomLists L = from_lists(lists);
+    return build_domain_masks(b->e, k, 1, &domain, box, p, &L, true, xy_in);
 }
 
 extern "C" int wv_build_comparison_mask(wv_ctx *c, int domain, int domain_b, const double box[4],
@@ -2996,9 +3120,9 @@ extern "C" int wv_build_comparison_mask(wv_ctx *c, int domain, int domain_b, con
                                         int ncircles_b, const double *circles_b, short *xy_in)
 {
     if (!c) return WV_EINVAL;
-    const int d[2] = {domain, domain_b}, n[2] = {ncircles, ncircles_b};
-    const double *cl[2] = {circles, circles_b};
-    return build_domain_masks(c->e, 0, 2, d, box, p, n, cl, xy_in);
+    const int d[2] = {domain, domain_b};
+    const DomLists L[2] = {circle_lists(ncircles, circles), circle_lists(ncircles_b, circles_b)};
+    return build_domain_masks(c->e, 0, 2, d, box, p, L, false, xy_in);
 }
 
 // ---- init_ior_2d's per-cell media models -------------------------------------

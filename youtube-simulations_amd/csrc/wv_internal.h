@@ -241,6 +241,12 @@ struct WvDomain {
     double lambda, mu, mu_b, wall_width, wall_width_b, julia_scale, julia_x, julia_y, mandellimit;
     double wall_width_asym, wall_width_asym_b, oscil_ymid;
     double c0, c1, c2;   // per-domain host constants (see k_domain)
+    // the list-driven domains: rectangles x1, y1, x2, y2; rotated ones x1, y1,
+    // x2, y2, width; polygons xc, yc, radius, active, nsides, cos(omega/2) and
+    // the offset of their sides' cosines and sines in trig; vertices x, y
+    // (each field a run of n values)
+    int nrect, nrectrot, npolyg, npline;
+    const double *rect, *rectrot, *polyg, *pline;
     const double *circ, *trig;
     short *out;
 };

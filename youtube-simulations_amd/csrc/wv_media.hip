@@ -168,6 +168,157 @@ __device__ inline bool in_circles(const WvDomain &a, double x, double y)
     return false;
 }
 
+// The domains that read lists other reference functions fill (polyrect,
+// polyrectrot, polygons, polyline, the global circle list): the lists are
+// wave-uniform reads, tested in list order with the reference's expressions.
+__device__ inline bool in_rect_open(const WvDomain &a, double x, double y, int k)   // inline tests, 5914, 6593
+{
+    const auto R = gc(a.rect);
+    const int n = a.nrect;
+    return x > R[k] && x < R[2 * n + k] && y > R[n + k] && y < R[3 * n + k];
+}
+
+__device__ inline bool in_rect_closed(const WvDomain &a, double x, double y, int k)  // xy_in_polyrect 5328
+{
+    const auto R = gc(a.rect);
+    const int n = a.nrect;
+    double x1 = R[k], y1 = R[n + k], x2 = R[2 * n + k], y2 = R[3 * n + k];
+    if (!(x1 < x2)) { const double t = x1; x1 = x2; x2 = t; }
+    if (!(y1 < y2)) { const double t = y1; y1 = y2; y2 = t; }
+    return !(x < x1) && !(x > x2) && !(y < y1) && !(y > y2);
+}
+
+__device__ inline bool in_rectrot(const WvDomain &a, double x, double y, int k)      // xy_in_rectrotated 5393
+{
+    const auto R = gc(a.rectrot);
+    const int n = a.nrectrot;
+    const double X1 = R[k], Y1 = R[n + k], X2 = R[2 * n + k], Y2 = R[3 * n + k], W = R[4 * n + k];
+    const double l = module2_(X2 - X1, Y2 - Y1);
+    if (l == 0.0) return false;
+    const double u1 = (X2 - X1) / l, u2 = (Y2 - Y1) / l, v1 = x - X1, v2 = y - Y1;
+    const double pscal = u1 * v1 + u2 * v2;
+    if (pscal < 0.0 || pscal > l) return false;
+    const double h2 = v1 * v1 + v2 * v2 - pscal * pscal;
+    return 4.0 * h2 <= W * W;
+}
+
+__device__ inline bool in_tri_v(const WvDomain &a, double x, double y, int p, int q, int r)  // 5313
+{
+    const auto P = gc(a.pline);
+    const int n = a.npline;
+    return in_triangle(x, y, P[p], P[n + p], P[q], P[n + q], P[r], P[n + r]);
+}
+
+__device__ short domain_value_list(const WvDomain &a, double x, double y)
+{
+    const double L = a.lambda, M = a.mu, WW = a.wall_width;
+    switch (a.domain) {
+    case 40: {                                                            // D_POLYGONS 6254, in_tpolygon 380
+        const auto G = gc(a.polyg);
+        const int n = a.npolyg;
+        for (int i = 0; i < n; i++) {
+            if (G[3 * n + i] == 0.0) continue;
+            const double x1 = (x - G[i]) / G[2 * n + i], y1 = (y - G[n + i]) / G[2 * n + i];
+            if (x1 * x1 + y1 * y1 >= 1.0) continue;
+            const int ns = (int)G[4 * n + i], off = (int)G[6 * n + i];
+            const double cw = G[5 * n + i];
+            int cond = 1;
+            for (int q = 0; q < ns; q++)
+                cond = cond * (x1 * gc(a.trig)[off + 2 * q] + y1 * gc(a.trig)[off + 2 * q + 1] < cw);
+            if (cond) return 0;
+        }
+        return 1;
+    }
+    case 41: {                                                            // D_VONKOCH 6260
+        const int n = a.npline;
+        int cond = in_tri_v(a, x, y, 0, n / 3, 2 * n / 3), m = 1, k = 1;
+        for (int i = 0; i < a.mdepth; i++) {
+            m = m * 4;
+            for (int j = 0; j < n / m; j++) cond += in_tri_v(a, x, y, j * m + k, j * m + 2 * k, j * m + 3 * k);
+            k = k * 4;
+        }
+        return cond >= 1;
+    }
+    case 42: {                                                            // D_STAR 6274
+        const int N = a.npoly;
+        int cond = in_tri_v(a, x, y, N, N - 1, 0);
+        for (int i = 0; i < N - 1; i++) cond += in_tri_v(a, x, y, N, i, i + 1);
+        return cond >= 1;
+    }
+    case 421: {                                                           // D_STAR_CHANNEL 6281
+        const int N = a.npoly;
+        int cond = x < 0.0 && vabs_(y) < 0.5 * WW;
+        for (int i = 0; i < N + 3; i++) cond += in_tri_v(a, x, y, N + 3, i + 1, i);
+        return cond >= 1;
+    }
+    case 53: case 531: case 54: case 541:                                 // D_MAZE 6590, _SMALL, _CLOSED, _CHANNELS
+        for (int i = 0; i < a.nrect; i++)
+            if (in_rect_open(a, x, y, i)) return 0;
+        return 1;
+    case 192: case 193: case 194: {                                       // D_TWO_PARABOLAS_ASYM_GUIDE 5903, _SEMITRANS 5921, _LENS 5942
+        if (x > 0.0) {
+            if (vabs_(y) > 0.5 * L) return 1;
+            if (a.domain == 194 && !(vabs_(y) > 0.1 * L)) {
+                const double xa = 0.24 * L, xb = xa + WW;
+                double r2 = 1.5 * 0.25 * L;
+                r2 = r2 * r2;
+                const double x3 = sqrt(r2 - 0.01 * L * L) - xb;
+                return ((x + x3) * (x + x3) + y * y < r2 && x > xa) ? 0 : 1;
+            }
+            const double x1 = -y * y / L + 0.25 * L;
+            if (a.domain == 192) return x < x1 || x > x1 + WW;
+            if (x < x1 || x > x1 + WW) return 1;
+            if (a.domain == 193 && vabs_(y) < 0.1 * L) return 0;
+            return 2;
+        }
+        for (int i = 0; i < a.nrect; i++)
+            if (in_rect_open(a, x, y, i)) return a.domain == 192 ? 0 : 2;
+        if (vabs_(y) > 0.5 * M) return 1;
+        const double x1 = y * y / M - 0.25 * M;
+        if (a.domain == 192) return x > x1 || x < x1 - WW;
+        return (x > x1 || x < x1 - WW) ? 1 : 2;
+    }
+    case 931: {                                                           // D_CIRCLE_LATTICE_RANDOM 7128
+        const double dx = (a.xmax - a.xmin) / (double)a.ngridx, dy = (a.ymax - a.ymin) / (double)a.ngridy;
+        for (int i = 0; i < a.ngridx; i++) {
+            const double x1 = a.xmin + ((double)i + 0.5) * dx;
+            for (int j = 0; j < a.ngridy; j++)
+                if (module2_(x - x1, y - (a.ymin + ((double)j + 0.5) * dy)) < M) return 1;
+        }
+        for (int i = 0; i < a.nrect; i++)
+            if (in_rect_closed(a, x, y, i)) return 1;
+        return 0;
+    }
+    case 95: case 97: {                                                   // D_CIRCLE_LATTICE_HONEY 7169, _RHOMBUS 7188
+        const double dx = (a.xmax - a.xmin) / (double)a.ngridx, dy = (a.ymax - a.ymin) / (double)a.ngridy;
+        for (int j = 0; j < a.ngridy + 1; j++) {
+            const double y1 = a.ymin + ((double)j) * dy;
+            for (int i = 0; i < a.ngridx + 1; i++) {
+                if (a.domain == 95 && !(i % 3 != 2 * ((j + 1) % 2))) continue;
+                double x1 = a.xmin + ((double)i) * dx;
+                if (j % 2 == 1) x1 += 0.5 * dx;
+                if (module2_(x - x1, y - y1) < M) return 1;
+            }
+        }
+        for (int i = 0; i < a.nrectrot; i++)
+            if (in_rectrot(a, x, y, i)) return 1;
+        return 0;
+    }
+    case 96: case 932: case 933: case 941: {                              // D_CIRCLE_LATTICE_POISSON 7207, _SQUARE_CROSS 7215, _MAZE 7301, _HEX_MAZE 7311
+        const int n = a.ncirc;
+        for (int i = 0; i < n; i++)
+            if (module2_(x - gc(a.circ)[i], y - gc(a.circ)[n + i]) < M) return 1;
+        if (a.domain != 96)
+            for (int i = 0; i < a.nrect; i++)
+                if (in_rect_closed(a, x, y, i)) return 1;
+        for (int i = 0; i < a.nrectrot; i++)
+            if (in_rectrot(a, x, y, i)) return 1;
+        return 0;
+    }
+    default: return 1;  // not reached: the launcher admits only the codes above
+    }
+}
+
 // The cases of xy_in_billiard_single_domain beyond the families of
 // domain_value, in the same manner.  The reference's one-time constants (its
 // static `first` blocks, sub_wave.c:6631-6636, 6647-6652, 6685-6690, and
@@ -552,7 +703,7 @@ __device__ short domain_value_more(const WvDomain &a, double x, double y)
         }
         return 1;
     }
-    default: return 1;  // not reached: the launcher admits only the codes above
+    default: return domain_value_list(a, x, y);
     }
 }
 

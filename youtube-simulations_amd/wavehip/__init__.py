@@ -33,7 +33,7 @@ EXPORTS = [
     "wv_set_state", "wv_set_state3", "wv_get_state", "wv_get_state3", "wv_set_sources",
     "wv_set_neumann", "wv_set_drive", "wv_set_floor", "wv_set_laplacian", "wv_set_probes", "wv_get_probes",
     "wv_format_time_series", "wv_set_wells_tables", "wv_build_mask", "wv_build_mask_static", "wv_build_domain_mask",
-    "wv_build_comparison_mask", "wv_set_ior_tables", "wv_rotating_wells", "wv_init_circular_wave",
+    "wv_build_list_domain_mask", "wv_batch_build_list_domain_mask", "wv_build_comparison_mask", "wv_set_ior_tables", "wv_rotating_wells", "wv_init_circular_wave",
     "wv_add_circular_wave", "wv_batch_init_circular_wave", "wv_batch_add_circular_wave",
     "wv_batch_set_wells_tables", "wv_batch_set_ior_tables", "wv_batch_build_domain_mask", "wv_tiff_bound", "wv_tiff_encode", "wv_tiff_write",
     "wv_step", "wv_prepare", "wv_step_timed", "wv_sync", "wv_steps_done", "wv_last_kernel_ms", "wv_energy",
@@ -92,6 +92,39 @@ class WvDomainParams(ctypes.Structure):
                                                 "julia_scale", "julia_x", "julia_y", "mandellimit")] + \
                [(n, ctypes.c_int) for n in ("npoly", "mdepth", "mratio", "mandellevel", "ngridx", "ngridy")] + \
                [(n, ctypes.c_double) for n in ("wall_width_asym", "wall_width_asym_b", "oscil_ymid")]
+
+
+class WvDomainLists(ctypes.Structure):
+    """struct wv_domain_lists of include/wavehip.h"""
+    _fields_ = [("ncircles", ctypes.c_int), ("circles", _D), ("nrects", ctypes.c_int), ("rects", _D),
+                ("nrectrot", ctypes.c_int), ("rectrot", _D), ("npolygons", ctypes.c_int), ("polygons", _D),
+                ("npolyline", ctypes.c_int), ("polyline", _D)]
+
+
+# rows of each list of wv_domain_lists
+LIST_ROWS = dict(circles=4, rects=4, rectrot=5, polygons=6, polyline=2)
+
+
+def domain_lists(**lists):
+    """WvDomainLists from (rows, n) arrays (rows as LIST_ROWS: circles xc, yc,
+    radius, active; rects x1, y1, x2, y2; rectrot x1, y1, x2, y2, width;
+    polygons xc, yc, radius, angle, nsides, active; polyline x, y); the
+    arrays stay alive on the struct"""
+    bad = set(lists) - set(LIST_ROWS)
+    if bad:
+        raise ValueError("unknown lists %s" % sorted(bad))
+    s = WvDomainLists()
+    keep = []
+    for name, rows in LIST_ROWS.items():
+        a = lists.get(name)
+        a = np.zeros((rows, 0)) if a is None else np.ascontiguousarray(a, np.float64)
+        if a.ndim != 2 or a.shape[0] != rows:
+            raise ValueError("%s must be (%d, n)" % (name, rows))
+        keep.append(a)
+        setattr(s, "n" + name, a.shape[1])
+        setattr(s, name, _d(a) if a.shape[1] else None)
+    s._keep = keep
+    return s
 
 
 class WvIorParams(ctypes.Structure):
@@ -191,6 +224,11 @@ def load():
                                                 ctypes.c_int, _D, _S]),
         "wv_build_domain_mask": (ctypes.c_int, [vp, ctypes.c_int, _D, ctypes.POINTER(WvDomainParams), ctypes.c_int,
                                                 _D, _S]),
+        "wv_build_list_domain_mask": (ctypes.c_int, [vp, ctypes.c_int, _D, ctypes.POINTER(WvDomainParams),
+                                                     ctypes.POINTER(WvDomainLists), _S]),
+        "wv_batch_build_list_domain_mask": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, _D,
+                                                           ctypes.POINTER(WvDomainParams),
+                                                           ctypes.POINTER(WvDomainLists), _S]),
         "wv_build_comparison_mask": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, _D, ctypes.POINTER(WvDomainParams),
                                                     ctypes.c_int, _D, ctypes.c_int, _D, _S]),
         "wv_set_ior_tables": (ctypes.c_int, [vp, _S, ctypes.c_int, _D, ctypes.POINTER(WvIorParams), _D, _D, _D]),
@@ -501,6 +539,22 @@ class Wave:
         out = np.empty(self.shape, np.int16)
         self._chk(self._L.wv_build_domain_mask(self._h, int(domain), _d(bx), ctypes.byref(P), n, _d(c),
                                                out.ctypes.data_as(_S)), "wv_build_domain_mask")
+        return out
+
+    def build_list_domain_mask(self, domain, box, params=None, lists=None, **kw):
+        """xy_in_billiard_single_domain over the grid on the GPU for the
+        domains that read polyrect, polyrectrot, polygons, polyline or the
+        circle list, and every closed-form code (wv_build_list_domain_mask);
+        lists: a dict of (rows, n) arrays as domain_lists takes them.
+        Returns int16 (nx, ny)."""
+        P = params if params is not None else domain_params(**kw)
+        dl = domain_lists(**(lists or {}))
+        bx = np.ascontiguousarray(box, np.float64).ravel()
+        if bx.size != 4:
+            raise ValueError("box must hold XMIN, XMAX, YMIN, YMAX")
+        out = np.empty(self.shape, np.int16)
+        self._chk(self._L.wv_build_list_domain_mask(self._h, int(domain), _d(bx), ctypes.byref(P), ctypes.byref(dl),
+                                                    out.ctypes.data_as(_S)), "wv_build_list_domain_mask")
         return out
 
     def build_comparison_mask(self, domain, domain_b, box, params=None, circles=None, circles_b=None, **kw):
@@ -830,6 +884,16 @@ class Batch:
         self._chk(self._L.wv_batch_build_domain_mask(self._h, k, int(domain), _d(self._box(box)), ctypes.byref(P),
                                                      0 if circles is None else c.shape[1], _d(c),
                                                      out.ctypes.data_as(_S)), "wv_batch_build_domain_mask")
+        return out
+
+    def build_list_domain_mask(self, k, domain, box, params=None, lists=None, **kw):
+        """wv_build_list_domain_mask over grid k (wv_batch_build_list_domain_mask)"""
+        P = params if params is not None else domain_params(**kw)
+        dl = domain_lists(**(lists or {}))
+        out = np.empty(self.shapes[k], np.int16)
+        self._chk(self._L.wv_batch_build_list_domain_mask(self._h, k, int(domain), _d(self._box(box)),
+                                                          ctypes.byref(P), ctypes.byref(dl), out.ctypes.data_as(_S)),
+                  "wv_batch_build_list_domain_mask")
         return out
 
     def step(self, n):
