@@ -185,6 +185,8 @@ typedef struct wv_domain_params {
     double wall_width_asym;           /* WALL_WIDTH_ASYM (non-isotropic lattices) */
     double wall_width_asym_b;         /* WALL_WIDTH_ASYM_B                      */
     double oscil_ymid;                /* OSCIL_YMID (D_TREE_BEAM)               */
+    double michelson_position;        /* michelson_position (global_pdes.c:570), the
+                                         moving mirror of D_MICHELSON_MOVING    */
 } wv_domain_params;
 
 /* The billiard mask of init_wave_flat (wave_common.c:294-319) on the GPU for
@@ -225,7 +227,8 @@ typedef struct wv_domain_params {
  *   D_DISC_WAVEGUIDE 98, D_DISC_WAVEGUIDE_SHIFTED 981,
  *   D_DISC_WAVEGUIDE_ELLIPSE 982, D_WAVEGUIDE_ELLIPSE_OBLIQUE 983,
  *   D_CIRCLE_PAIRS 99, D_CIRCLE_POLYGON 991, D_ANNULUS_HEATED 21,
- *   D_MENGER_HEATED 22, D_MENGER_H_OPEN 23.
+ *   D_MENGER_HEATED 22, D_MENGER_H_OPEN 23, D_MICHELSON_MOVING 741,
+ *   D_IMAGE 77 (1 everywhere: the program's mask comes from its image).
  * Values are the reference's (0, 1 and, for some domains, 2; the heated
  * Menger gaskets 22 and 23 give k + 2 in a hole of depth k).  The one-time
  * constants the reference keeps in static variables (D_TRIANGLE_TILES,

@@ -391,7 +391,7 @@ static void evolve_wave(double phi[NX * NY], double psi[NX * NY], double tmp[NX 
  * XMIN..YMAX, XYIN_INITIALISED; VARIABLE_IOR,
  * IOR, COURANT, COURANTB, GAMMA, GAMMAB, MANDEL_IOR_SCALE; COMPARISON and
  * B_DOMAIN_B) and its globals (circles, ncircles, circles_b, ncircles_b,
- * julia_x, julia_y; for a list-driven B_DOMAIN, the lists it reads:
+ * julia_x, julia_y, michelson_position; for a list-driven B_DOMAIN, the lists it reads:
  * polyrect, npolyrect, polyrectrot, npolyrect_rot, polygons, polyline,
  * npolyline).  The random wells (IOR 5, 8, 9) are drawn with the
  * program's own gaussian() and init_circle_config_pattern(), in the
@@ -483,6 +483,7 @@ static void wavehip_mask(short *flat)
     dp.mandellimit = MANDELLIMIT; dp.npoly = NPOLY; dp.mdepth = MDEPTH; dp.mratio = MRATIO;
     dp.mandellevel = MANDELLEVEL; dp.ngridx = NGRIDX; dp.ngridy = NGRIDY;
     dp.wall_width_asym = WALL_WIDTH_ASYM; dp.wall_width_asym_b = WALL_WIDTH_ASYM_B; dp.oscil_ymid = OSCIL_YMID;
+    dp.michelson_position = michelson_position;
     const int nl = COMPARISON ? 2 : 1;
     const t_circle *cs[2] = {circles, COMPARISON ? circles_b : circles};
     const int nc[2] = {ncircles, COMPARISON ? ncircles_b : 0};

@@ -545,6 +545,18 @@ static int o_domain_more(double x, double y, int dom, const orc_domain_params *d
             if ((o_abs(y) < LAMBDA) && (o_abs(x - x1) < 0.5 * WALL_WIDTH)) return 1;
         }
         return 0;
+    case 77: return 1;                                                               /* 6924 */
+    case 741:                                                                        /* 6879 */
+        if ((o_abs(x) > LAMBDA) && (o_abs(y) > LAMBDA)) return 2;
+        r = 0.5 * sqrt(2.0);
+        h = YMAX - MU - 0.5 * WALL_WIDTH;
+        hh = YMAX - MU - 0.5 * WALL_WIDTH + d->michelson_position;
+        x1 = (x + y) * r;
+        y1 = (-x + y) * r;
+        if ((o_abs(y1) < 0.5 * WALL_WIDTH) && (o_abs(x1) < LAMBDA * sqrt(2.0))) return 1;
+        if ((o_abs(x - hh) < 0.5 * WALL_WIDTH) && (o_abs(y) < LAMBDA)) return 2;
+        if ((o_abs(x) < LAMBDA) && (o_abs(y - h) < 0.5 * WALL_WIDTH)) return 2;
+        return 0;
     case 21: {                                                                       /* 7475 */
         const double l2 = LAMBDA * LAMBDA, r2 = x * x + y * y, r2mu = (x - MU) * (x - MU) + y * y;
         if ((r2mu > l2) && (r2 < 1.0)) return 1;

@@ -368,7 +368,8 @@ class DomainParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("lam", "mu", "mu_b", "apoly", "wall_width", "wall_width_b",
                                                 "julia_scale", "julia_x", "julia_y", "mandellimit")] + \
                [(n, ctypes.c_int) for n in ("npoly", "mdepth", "mratio", "mandellevel", "ngridx", "ngridy")] + \
-               [(n, ctypes.c_double) for n in ("wall_width_asym", "wall_width_asym_b", "oscil_ymid")]
+               [(n, ctypes.c_double) for n in ("wall_width_asym", "wall_width_asym_b", "oscil_ymid",
+                                                "michelson_position")]
 
 
 # the values of the fixture builds' parameter block (oracle/ref_params.h) and
@@ -376,7 +377,7 @@ class DomainParams(ctypes.Structure):
 DOMAIN_DEFAULTS = dict(lam=1.5, mu=0.1, mu_b=0.08, apoly=0.0, wall_width=0.1, wall_width_b=0.1, julia_scale=0.8,
                        julia_x=0.37468, julia_y=0.21115, mandellimit=10.0, npoly=6, mdepth=4, mratio=3,
                        mandellevel=100, ngridx=10, ngridy=10, wall_width_asym=0.75, wall_width_asym_b=0.75,
-                       oscil_ymid=0.0)
+                       oscil_ymid=0.0, michelson_position=0.0)
 
 
 def domain_params(**kw):
@@ -611,6 +612,9 @@ class RefBuild:
             if hasattr(self.L, "ref_mask_billiard"):
                 self.L.ref_mask_billiard.argtypes = [ctypes.c_int, _D, _D, _D, _I, ctypes.c_int, _D, _D, _D, _I, _S]
                 self.L.ref_mask_billiard.restype = ctypes.c_int
+            if hasattr(self.L, "ref_set_michelson"):
+                self.L.ref_set_michelson.argtypes = [ctypes.c_double]
+                self.L.ref_set_michelson.restype = None
             if hasattr(self.L, "ref_set_lists"):
                 self.L.ref_set_lists.argtypes = [ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int,
                                                  _D]
@@ -705,6 +709,10 @@ class RefBuild:
         rc = self.L.ref_mask(domain, n, _p(cc[0], _D), _p(cc[1], _D), _p(cc[2], _D), _p(a, _I), _p(out, _S))
         assert rc == 0
         return out
+
+    def set_michelson(self, position):
+        """the build's michelson_position global"""
+        self.L.ref_set_michelson(float(position))
 
     def set_lists(self, rects=None, rectrot=None, polygons=None, polyline=None):
         """fill the build's polyrect, polyrectrot, polygons and polyline

@@ -132,6 +132,8 @@ int ref_mask(int domain, int ncirc, const double *xc, const double *yc, const do
         }
     return 0;
 }
+/* the mirror position D_MICHELSON_MOVING reads (global_pdes.c:570) */
+void ref_set_michelson(double position) { michelson_position = position; }
 /* the lists the list-driven domains read (global_pdes.c:533-545), as the
  * program's init functions leave them, from flat arrays: rectangles x1, y1,
  * x2, y2 (polyrect); rotated rectangles x1, y1, x2, y2, width (polyrectrot);

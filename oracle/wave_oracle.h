@@ -226,7 +226,7 @@ void orc_hsluv_rgb(double h, double s, double l, double rgb[3]);
 typedef struct {
     double lambda, mu, mu_b, apoly, wall_width, wall_width_b, julia_scale, julia_x, julia_y, mandellimit;
     int npoly, mdepth, mratio, mandellevel, ngridx, ngridy;
-    double wall_width_asym, wall_width_asym_b, oscil_ymid;
+    double wall_width_asym, wall_width_asym_b, oscil_ymid, michelson_position;
 } orc_domain_params;
 
 /* xy_in_billiard_single_domain(x, y) at the cell corners of ij_to_xy over

@@ -96,6 +96,7 @@ int n_neumann_points = 0;
 int ncircles = NMAXCIRCLES, ncircles_b = NMAXCIRCLES;
 t_circle circles[NMAXCIRCLES], circles_b[NMAXCIRCLES];
 double julia_x = 0.37468, julia_y = 0.21115;
+double michelson_position = 0.0;
 double oscillating_bc(int time, int j) { (void)time; (void)j; return 0.0; }
 /* the program's normal deviates: Marsaglia's polar method on rand(), the
  * second deviate of each accepted pair kept for the next call */

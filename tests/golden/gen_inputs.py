@@ -259,7 +259,7 @@ DOMAIN_CODES = (999, 0, 1, 196, 199, 198, 197, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 1
                 68, 69, 73, 74, 76, 79, 791, 795, 796, 90, 91,
                 35, 43, 45, 44, 441, 46, 461, 47, 48, 49, 52, 521, 526, 55, 56, 57, 58, 581, 59, 65, 66, 731, 75,
                 751, 78, 781, 792, 793, 794, 797, 798, 93, 934, 935, 936, 937, 94, 942, 947, 98, 981, 982, 983, 99,
-                991, 21, 22, 23)
+                991, 21, 22, 23, 77, 741)
 # the domains of xy_in_billiard_single_domain that read lists other reference
 # functions fill (polyrect, polyrectrot, polygons, polyline, the global
 # circle list); their fixtures use list_data's lists on the dom_* builds
@@ -326,10 +326,12 @@ def domain_sets():
         "dom_a": (97, 61, dict(lam=0.6, mu=0.3, npoly=6, apoly=0.5)),
         "dom_b": (160, 90, dict(lam=1.2, mu=0.15, npoly=9, apoly=1.0, mdepth=3, julia_scale=1.0, mandellevel=200,
                                 wall_width=0.075, mu_b=0.3, wall_width_b=0.15, ngridx=8, ngridy=6,
-                                wall_width_asym=1.25, wall_width_asym_b=0.5, oscil_ymid=-0.3)),
+                                wall_width_asym=1.25, wall_width_asym_b=0.5, oscil_ymid=-0.3,
+                                michelson_position=0.13)),
         "dom_c": (121, 77, dict(lam=0.45, mu=0.05, npoly=12, apoly=0.25, mdepth=5, mratio=5, julia_scale=1.3,
                                 mandellevel=500, mandellimit=4.0, wall_width=0.04, mu_b=0.15, wall_width_b=0.05,
-                                ngridx=12, ngridy=8, wall_width_asym=0.6, wall_width_asym_b=1.4, oscil_ymid=0.45)),
+                                ngridx=12, ngridy=8, wall_width_asym=0.6, wall_width_asym_b=1.4, oscil_ymid=0.45,
+                                michelson_position=-0.071)),
     }
 
 

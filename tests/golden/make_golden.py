@@ -118,6 +118,7 @@ def domains():
     for tag, (nx, ny, kw) in domain_sets().items():
         R = pyoracle.RefBuild(tag)
         assert (R.nx, R.ny) == (nx, ny)
+        R.set_michelson(kw.get("michelson_position", 0.0))
         for dom in DOMAIN_CODES:
             if dom in DOMAIN_FIRST_CODES:
                 out["%s/%d" % (tag, dom)] = fresh_mask(tag, dom)

@@ -353,7 +353,7 @@ def test_list_domain_mask_errors(wh):
         for dom, lists, kw in ((42, dict(L, polyline=L["polyline"][:, :6]), dict(npoly=6)),
                                (421, dict(L, polyline=L["polyline"][:, :9]), dict(npoly=6)),
                                (41, L, dict(mdepth=16)), (40, no_sides, {}), (95, L, dict(ngridx=0)),
-                               (77, L, {})):
+                               (92, L, {})):
             say("list domain error", dom)
             with pytest.raises(wh.WaveError):
                 W.build_list_domain_mask(dom, MASK_BOX, lists=lists, **kw)

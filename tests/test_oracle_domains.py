@@ -19,7 +19,7 @@ from gen_inputs import DOMAIN_CODES, MASK_BOX, domain_sets, mask_circles
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 CIRCLE_CODES = (20, 201, 202)
-CONSTANT = (999, 76)  # D_NOTHING and D_GRADIENT_INDEX_LENS are 1 everywhere
+CONSTANT = (999, 76, 77)  # D_NOTHING, D_GRADIENT_INDEX_LENS and D_IMAGE are 1 everywhere
 # lattices placed from XMIN, YMIN with the box's own spacing: a box shifted
 # with the sample points moves them too, so the cell-centre mistake is
 # invisible there; NGRIDX off by one is not (test_new_macros_are_pinned)
@@ -85,8 +85,8 @@ def test_fixtures_catch_planted_mistakes(oracle, gold):
 def test_new_macros_are_pinned(oracle, gold):
     """the macros only the lattice, tree-beam and waveguide domains read:
     each planted mistake (WALL_WIDTH_ASYM or _B off by 10 %, the two
-    swapped, OSCIL_YMID shifted, NGRIDX off by one) fails the fixtures of
-    every code that reads it, in at least one build"""
+    swapped, OSCIL_YMID or the Michelson mirror shifted, NGRIDX off by one)
+    fails the fixtures of every code that reads it, in at least one build"""
     def fails(dom, **over):
         return any(not np.array_equal(oracle_mask(oracle, t, dom, **{k: v(domain_sets()[t][2]) for k, v in over.items()}),
                                       gold["%s/%d" % (t, dom)]) for t in domain_sets())
@@ -99,6 +99,7 @@ def test_new_macros_are_pinned(oracle, gold):
     for dom in (942, 947):
         assert fails(dom, wall_width_asym=g("wall_width_asym_b"), wall_width_asym_b=g("wall_width_asym")), dom
     assert fails(731, oscil_ymid=lambda kw: g("oscil_ymid")(kw) + 0.05)
+    assert fails(741, michelson_position=lambda kw: g("michelson_position")(kw) + 0.05)
     for dom in (93, 934, 935, 936, 937, 94, 942, 947, 99, 991):
         assert fails(dom, ngridx=lambda kw: g("ngridx")(kw) + 1), dom
 

@@ -2776,7 +2776,7 @@ static bool domain_known(int d)
                                 35,  43,  45,  44,  441, 46,   461,  47,  48,  49,  52,  521, 526, 55,  56,
                                 57,  58,  581, 59,  65,  66,   731,  75,  751, 78,  781, 792, 793, 794, 797,
                                 798, 93,  934, 935, 936, 937,  94,   942, 947, 98,  981, 982, 983, 99,  991,
-                                21,  22,  23};
+                                21,  22,  23,  77,  741};
     for (int c : codes)
         if (c == d) return true;
     return false;
@@ -2972,7 +2972,7 @@ static int domain_setup(Engine *e, const Grid &G, int domain, const double *box,
         a.c1 = std::sin(p->apoly * REF_PID);
     } else if (domain == 73) {  // sub_wave.c:6793
         a.c0 = std::cos(REF_PI / 6.0);
-    } else if (domain == 74) {  // sub_wave.c:6870
+    } else if (domain == 74 || domain == 741) {  // sub_wave.c:6870, 6882
         a.c0 = std::sqrt(2.0);
     } else if (domain == 33) {  // sub_wave.c:6047
         a.c0 = std::sqrt(L * L - (1.0 - M) * (1.0 - M));
@@ -2998,6 +2998,7 @@ static int domain_setup(Engine *e, const Grid &G, int domain, const double *box,
     a.mandellimit = p->mandellimit;
     a.wall_width_asym = p->wall_width_asym; a.wall_width_asym_b = p->wall_width_asym_b;
     a.oscil_ymid = p->oscil_ymid;
+    a.michelson_position = p->michelson_position;
     return WV_OK;
 }
 

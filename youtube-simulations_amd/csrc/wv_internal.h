@@ -239,7 +239,7 @@ struct WvDomain {
     int npoly, mdepth, mratio, mandellevel, ngridx, ngridy;
     double xmin, xmax, ymin, ymax;
     double lambda, mu, mu_b, wall_width, wall_width_b, julia_scale, julia_x, julia_y, mandellimit;
-    double wall_width_asym, wall_width_asym_b, oscil_ymid;
+    double wall_width_asym, wall_width_asym_b, oscil_ymid, michelson_position;
     double c0, c1, c2;   // per-domain host constants (see k_domain)
     // the list-driven domains: rectangles x1, y1, x2, y2; rotated ones x1, y1,
     // x2, y2, width; polygons xc, yc, radius, active, nsides, cos(omega/2) and

@@ -686,6 +686,16 @@ __device__ short domain_value_more(const WvDomain &a, double x, double y)
         }
         return 0;
     }
+    case 77: return 1;                                                    // D_IMAGE 6924
+    case 741: {                                                           // D_MICHELSON_MOVING 6879; c0 = sqrt(2)
+        if (vabs_(x) > L && vabs_(y) > L) return 2;
+        const double r = 0.5 * a.c0, h = a.ymax - M - 0.5 * WW, h1 = a.ymax - M - 0.5 * WW + a.michelson_position;
+        const double x1 = (x + y) * r, y1 = (-x + y) * r;
+        if (vabs_(y1) < 0.5 * WW && vabs_(x1) < L * a.c0) return 1;
+        if (vabs_(x - h1) < 0.5 * WW && vabs_(y) < L) return 2;
+        if (vabs_(x) < L && vabs_(y - h) < 0.5 * WW) return 2;
+        return 0;
+    }
     case 21: {                                                            // D_ANNULUS_HEATED 7475
         const double l2 = L * L, r2 = x * x + y * y, r2mu = (x - M) * (x - M) + y * y;
         if (r2mu > l2 && r2 < 1.0) return 1;

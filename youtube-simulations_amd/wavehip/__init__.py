@@ -91,7 +91,8 @@ class WvDomainParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("lam", "mu", "mu_b", "apoly", "wall_width", "wall_width_b",
                                                 "julia_scale", "julia_x", "julia_y", "mandellimit")] + \
                [(n, ctypes.c_int) for n in ("npoly", "mdepth", "mratio", "mandellevel", "ngridx", "ngridy")] + \
-               [(n, ctypes.c_double) for n in ("wall_width_asym", "wall_width_asym_b", "oscil_ymid")]
+               [(n, ctypes.c_double) for n in ("wall_width_asym", "wall_width_asym_b", "oscil_ymid",
+                                                "michelson_position")]
 
 
 class WvDomainLists(ctypes.Structure):
@@ -139,7 +140,7 @@ class WvIorParams(ctypes.Structure):
 DOMAIN_DEFAULTS = dict(lam=0.2, mu=0.008, mu_b=0.08, apoly=1.0, wall_width=0.075, wall_width_b=0.1,
                        julia_scale=0.8, julia_x=0.37468, julia_y=0.21115, mandellimit=10.0, npoly=9, mdepth=6,
                        mratio=3, mandellevel=1000, ngridx=10, ngridy=10, wall_width_asym=0.75,
-                       wall_width_asym_b=0.75, oscil_ymid=0.0)
+                       wall_width_asym_b=0.75, oscil_ymid=0.0, michelson_position=0.0)
 # init_ior_2d's macros of the checked-in wave_billiard.c (54, 92-100, 140-143)
 IOR_DEFAULTS = dict(courant=0.09, courantb=0.25, gamma=0.0, gammab=0.0, lam=0.2, mu=0.008, mandel_ior_scale=-0.05,
                     mandellimit=10.0, mandellevel=1000, mdepth=6)
