@@ -717,6 +717,30 @@ __device__ short domain_value_more(const WvDomain &a, double x, double y)
     }
 }
 
+// the escape-time domains (D_JULIA_INT, D_MANDELBROT, D_MANDELBROT_CIRCLE,
+// D_JULIA): up to MANDELLEVEL iterations a cell; they also run in a kernel of
+// their own, whose loop the other cases do not crowd
+__device__ short escape_domain_value(const WvDomain &a, double x, double y)
+{
+    const double L = a.lambda, M = a.mu;
+    switch (a.domain) {
+    case 16:                                                              // D_JULIA_INT 7443
+        return escape_norm2(x / a.julia_scale, y / a.julia_scale, a.julia_x, a.julia_y, a.mandellevel,
+                            a.mandellimit) < a.mandellimit;
+    case 24: case 26:                                                     // D_MANDELBROT 7512, _CIRCLE 7531
+        if (escape_norm2(0.0, 0.0, x, y, a.mandellevel, a.mandellimit) < a.mandellimit) return 0;
+        if (a.domain == 24) return (x - 0.5) * (x - 0.5) / 3.0 + y * y / 1.0 > 1.2 ? 2 : 1;
+        return (x - L) * (x - L) + (y - 0.5) * (y - 0.5) < M * M ? 2 : 1;
+    default:                                                              // D_JULIA 7547
+        if (escape_norm2(x / a.julia_scale, y / a.julia_scale, a.julia_x, a.julia_y, a.mandellevel,
+                         a.mandellimit) < a.mandellimit)
+            return 0;
+        return x * x / 3.0 + y * y / 1.0 > 1.2 ? 2 : 1;
+    }
+}
+
+static bool escape_domain(int d) { return d == 16 || d == 24 || d == 25 || d == 26; }
+
 // xy_in_billiard_single_domain(x, y), sub_wave.c:5712-7602, for the
 // closed-form domains; each case follows the reference's expressions in its
 // order of evaluation (line numbers of sub_wave.c).  Host constants c0..c2
@@ -816,9 +840,7 @@ __device__ short domain_value(const WvDomain &a, double x, double y)
         }
         return 1;
     }
-    case 16:                                                              // D_JULIA_INT 7443
-        return escape_norm2(x / a.julia_scale, y / a.julia_scale, a.julia_x, a.julia_y, a.mandellevel,
-                            a.mandellimit) < a.mandellimit;
+    case 16: return escape_domain_value(a, x, y);                        // D_JULIA_INT 7443
     case 18: return x > 0.25 * y * y / L - L;                             // D_PARABOLA 5872
     case 19: {                                                            // D_TWO_PARABOLAS 5876
         const double x1 = 0.25 * y * y / M - M - L, x2 = -x1;
@@ -858,16 +880,7 @@ __device__ short domain_value(const WvDomain &a, double x, double y)
     case 201:                                                             // D_CIRCLES_IN_RECT 6240
         if (in_circles(a, x, y)) return 0;
         return (vabs_(x) >= L || vabs_(y) >= a.ymax - a.xmax + L) ? 0 : 1;
-    case 24: case 26: {                                                   // D_MANDELBROT 7512, _CIRCLE 7531
-        if (escape_norm2(0.0, 0.0, x, y, a.mandellevel, a.mandellimit) < a.mandellimit) return 0;
-        if (a.domain == 24) return (x - 0.5) * (x - 0.5) / 3.0 + y * y / 1.0 > 1.2 ? 2 : 1;
-        return (x - L) * (x - L) + (y - 0.5) * (y - 0.5) < M * M ? 2 : 1;
-    }
-    case 25:                                                              // D_JULIA 7547
-        if (escape_norm2(x / a.julia_scale, y / a.julia_scale, a.julia_x, a.julia_y, a.mandellevel,
-                         a.mandellimit) < a.mandellimit)
-            return 0;
-        return x * x / 3.0 + y * y / 1.0 > 1.2 ? 2 : 1;
+    case 24: case 25: case 26: return escape_domain_value(a, x, y);      // D_MANDELBROT 7512, D_JULIA 7547, 7531
     case 31: {                                                            // D_FOUR_PARABOLAS 6026
         const double x1 = M + L - 0.25 * y * y / M, y1 = M + L - 0.25 * x * x / M;
         return vabs_(x) < x1 && vabs_(y) < y1;
@@ -1054,6 +1067,7 @@ __device__ short domain_value(const WvDomain &a, double x, double y)
 // 8 waves a SIMD (64 VGPRs): the escape loops of the fractal domains hold
 // their wave for up to MANDELLEVEL iterations, so occupancy hides their
 // latency; the cases of domain_value_more alone would ask for 67
+template <bool ESCAPE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_domain(WvDomain a)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x, i = a.i0 + (int)blockIdx.y;
@@ -1063,7 +1077,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
     const double y = a.ymin + ((double)j) * (a.ymax - a.ymin) / ((double)a.ny);
     // COMPARISON: one domain for y > 0, the other for y <= 0 (sub_wave.c:7597-7600)
     if ((a.half == 1 && !(y > 0.0)) || (a.half == 2 && y > 0.0)) return;
-    gm(a.out)[(size_t)(i - a.i0) * a.ny + j] = domain_value(a, x, y);
+    gm(a.out)[(size_t)(i - a.i0) * a.ny + j] = ESCAPE ? escape_domain_value(a, x, y) : domain_value(a, x, y);
 }
 
 // ---- init_ior_2d's per-cell media models -------------------------------------
@@ -1278,7 +1292,8 @@ int wv_launch_mask(const WvMask &a, hipStream_t s)
 int wv_launch_domain(const WvDomain &a, hipStream_t s)
 {
     const dim3 block(256), grid((a.ny + 255) / 256, a.rows);
-    hipLaunchKernelGGL(k_domain, grid, block, 0, s, a);
+    if (escape_domain(a.domain)) hipLaunchKernelGGL(k_domain<true>, grid, block, 0, s, a);
+    else hipLaunchKernelGGL(k_domain<false>, grid, block, 0, s, a);
     return (int)hipGetLastError();
 }
 
