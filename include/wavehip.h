@@ -274,11 +274,11 @@ typedef struct wv_domain_lists {
  *   D_TWO_PARABOLAS_LENS 194, D_CIRCLE_LATTICE_RANDOM 931,
  *   D_CIRCLE_LATTICE_HONEY 95, D_CIRCLE_LATTICE_RHOMBUS 97,
  *   D_CIRCLE_LATTICE_POISSON 96, D_CIRCLE_LATTICE_SQUARE_CROSS 932,
- *   D_CIRCLE_LATTICE_MAZE 933, D_CIRCLE_LATTICE_HEX_MAZE 941
- * (sub_wave.c:5903-7320).  D_POLYGONS reads npolygons polygons, the disc
+ *   D_CIRCLE_LATTICE_MAZE 933, D_CIRCLE_LATTICE_HEX_MAZE 941,
+ *   D_VONKOCH_HEATED 27 (sub_wave.c:5903-7590).  D_POLYGONS reads npolygons polygons, the disc
  * lattices 96, 932, 933 and 941 every circle of the list (radius MU, the
  * active flag unread, as the reference reads them).  D_STAR needs NPOLY + 1
- * vertices, D_STAR_CHANNEL NPOLY + 4, D_VONKOCH MDEPTH <= 15; polygons 1-4096
+ * vertices, D_STAR_CHANNEL NPOLY + 4, D_VONKOCH and D_VONKOCH_HEATED MDEPTH <= 15; polygons 1-4096
  * sides; at most 1,048,576 entries a list.  Sines and cosines of the polygon
  * sides are taken on the host, as for the regular polygons; masks equal the
  * reference's cell for cell.  Others return WV_EUNSUPPORTED. */

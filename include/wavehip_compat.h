@@ -411,7 +411,7 @@ static const double wavehip_box[4] = {XMIN, XMAX, YMIN, YMAX};
 #define WAVEHIP_RECTROT (WAVEHIP_D_(95) || WAVEHIP_D_(96) || WAVEHIP_D_(97) || WAVEHIP_D_(932) || WAVEHIP_D_(933) || \
                          WAVEHIP_D_(941))
 #define WAVEHIP_POLYGONS WAVEHIP_D_(40)
-#define WAVEHIP_POLYLINE (WAVEHIP_D_(41) || WAVEHIP_D_(42) || WAVEHIP_D_(421))
+#define WAVEHIP_POLYLINE (WAVEHIP_D_(41) || WAVEHIP_D_(42) || WAVEHIP_D_(421) || WAVEHIP_D_(27))
 #define WAVEHIP_LISTS (WAVEHIP_RECTS || WAVEHIP_RECTROT || WAVEHIP_POLYGONS || WAVEHIP_POLYLINE)
 
 #if WAVEHIP_LISTS && !COMPARISON

@@ -1059,7 +1059,8 @@ static int o_list_at(double x, double y, int dom, const orc_domain_params *d, co
             if ((L->polygons[5 * L->npolygons + i] != 0.0) && o_in_tpolygon(x, y, L->polygons, L->npolygons, i))
                 return 0;
         return 1;
-    case 41:                                                                         /* 6260 */
+    case 41: case 27:                                                                /* 6260, 7566 */
+        if (dom == 27 && x * x + y * y > LAMBDA * LAMBDA) return 2;
         condition = o_tri_v(x, y, L, 0, npl / 3, 2 * npl / 3);
         m = 1;
         k = 1;
@@ -1068,6 +1069,7 @@ static int o_list_at(double x, double y, int dom, const orc_domain_params *d, co
             for (j = 0; j < npl / m; j++) condition += o_tri_v(x, y, L, j * m + k, j * m + 2 * k, j * m + 3 * k);
             k = k * 4;
         }
+        if (dom == 27) return (condition > 0) ? 0 : 1;
         return condition >= 1;
     case 42:                                                                         /* 6274 */
         condition = o_tri_v(x, y, L, NPOLY, NPOLY - 1, 0);
@@ -1138,7 +1140,7 @@ int orc_list_domain(int NX, int NY, const double box[4], int domain, const orc_d
                     const orc_domain_lists *L, short *out)
 {
     /* the vertices each polyline domain reads */
-    if (domain == 41 && (L->npolyline < 1 || dp->mdepth < 0 || dp->mdepth > 15)) return -1;
+    if ((domain == 41 || domain == 27) && (L->npolyline < 1 || dp->mdepth < 0 || dp->mdepth > 15)) return -1;
     if (domain == 42 && (dp->npoly < 1 || L->npolyline < dp->npoly + 1)) return -1;
     if (domain == 421 && (dp->npoly < 0 || L->npolyline < dp->npoly + 4)) return -1;
     if (o_list_at(0.0, 0.0, domain, dp, box, L) < 0) return -1;

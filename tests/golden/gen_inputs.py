@@ -263,7 +263,7 @@ DOMAIN_CODES = (999, 0, 1, 196, 199, 198, 197, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 1
 # the domains of xy_in_billiard_single_domain that read lists other reference
 # functions fill (polyrect, polyrectrot, polygons, polyline, the global
 # circle list); their fixtures use list_data's lists on the dom_* builds
-LIST_DOMAIN_CODES = (40, 41, 42, 421, 53, 531, 54, 541, 192, 193, 194, 931, 95, 97, 96, 932, 933, 941)
+LIST_DOMAIN_CODES = (40, 41, 42, 421, 53, 531, 54, 541, 192, 193, 194, 931, 95, 97, 96, 932, 933, 941, 27)
 
 
 def list_data(npoly):

@@ -198,7 +198,7 @@ def test_list_domains_read_their_lists(oracle, gold):
                                                  L["polygons"][4:]]))
     assert fails(40, rad)
     rev = lambda L: dict(L, polyline=np.ascontiguousarray(L["polyline"][:, ::-1]))  # noqa: E731
-    for dom in (41, 42, 421):
+    for dom in (41, 42, 421, 27):
         assert fails(dom, rev), dom
     # the disc lattices read every circle of the list, radius MU, active or not
     for dom in LIST_CIRCLE_CODES:

@@ -2785,7 +2785,7 @@ static bool domain_known(int d)
 // the domains that read the lists other reference functions fill
 static bool list_domain_known(int d)
 {
-    static const int codes[] = {40, 41, 42, 421, 53, 531, 54, 541, 192, 193, 194, 931, 95, 97, 96, 932, 933, 941};
+    static const int codes[] = {40, 41, 42, 421, 53, 531, 54, 541, 192, 193, 194, 931, 95, 97, 96, 932, 933, 941, 27};
     for (int c : codes)
         if (c == d) return true;
     return false;
@@ -2825,7 +2825,7 @@ static int domain_setup(Engine *e, const Grid &G, int domain, const double *box,
         const auto bad = [](int n, const double *q) { return n < 0 || n > (1 << 20) || (n > 0 && !q); };
         if (bad(Ls.nc, Ls.c) || bad(Ls.nr, Ls.r) || bad(Ls.nrr, Ls.rr) || bad(Ls.np, Ls.pg) || bad(Ls.npl, Ls.pl))
             return fail(e, WV_EINVAL, "lists: counts 0..1048576 and an array for each nonzero count");
-        if (domain == 41 && (Ls.npl < 1 || p->mdepth < 0 || p->mdepth > 15))
+        if ((domain == 41 || domain == 27) && (Ls.npl < 1 || p->mdepth < 0 || p->mdepth > 15))
             return fail(e, WV_EINVAL, "D_VONKOCH: at least one vertex and MDEPTH 0..15");
         if (domain == 42 && (p->npoly < 1 || Ls.npl < p->npoly + 1))
             return fail(e, WV_EINVAL, "D_STAR: NPOLY >= 1 and NPOLY + 1 vertices");

@@ -229,7 +229,8 @@ __device__ short domain_value_list(const WvDomain &a, double x, double y)
         }
         return 1;
     }
-    case 41: {                                                            // D_VONKOCH 6260
+    case 41: case 27: {                                                   // D_VONKOCH 6260, D_VONKOCH_HEATED 7566
+        if (a.domain == 27 && x * x + y * y > L * L) return 2;
         const int n = a.npline;
         int cond = in_tri_v(a, x, y, 0, n / 3, 2 * n / 3), m = 1, k = 1;
         for (int i = 0; i < a.mdepth; i++) {
@@ -237,6 +238,7 @@ __device__ short domain_value_list(const WvDomain &a, double x, double y)
             for (int j = 0; j < n / m; j++) cond += in_tri_v(a, x, y, j * m + k, j * m + 2 * k, j * m + 3 * k);
             k = k * 4;
         }
+        if (a.domain == 27) return cond > 0 ? 0 : 1;
         return cond >= 1;
     }
     case 42: {                                                            // D_STAR 6274
